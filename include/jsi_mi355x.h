@@ -1,0 +1,154 @@
+/*
+ * jsi_mi355x.h - C ABI of the MI355X minimum-distance / surface-surface
+ * intersection library (libjsi_mi355x.so).
+ *
+ * Every entry point returns an int status (JSI_OK == 0, negative on error) and
+ * never exits the process.  Pointers are plain host pointers unless the name
+ * says "device".  No torch or HIP types appear in these signatures.
+ *
+ * Surfaces use the JSI_CAD_NEXT NurbsFace layout
+ * (JSI_CAD_NEXT_VERSION5/include/nurbs_data.hpp:1-10):
+ *   ivec[7] = {flags, degU, nCPu, nKu, degV, nCPv, nKv}
+ *   data    = U[nKu] | V[nKv] | CP xyz[3*nCPu*nCPv] (u fastest) | w[nCPu*nCPv] if flags&2
+ * (layout read by BuildEvalContext, JSI_CAD_NEXT_VERSION5/src/evaluation/evaluation.cu:239-257).
+ */
+#ifndef JSI_MI355X_H
+#define JSI_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define JSI_OK 0
+#define JSI_ERR_INVALID (-1)     /* bad argument, handle or surface */
+#define JSI_ERR_NOMEM (-2)       /* device arena or HBM cap exhausted */
+#define JSI_ERR_DEVICE (-3)      /* HIP runtime error */
+#define JSI_ERR_CAPACITY (-4)    /* an internal bounded buffer overflowed */
+#define JSI_ERR_UNSUPPORTED (-5) /* periodic surfaces, degree >= 25, nuv > 1025 ... */
+#define JSI_ERR_BUSY (-6)        /* serving queue full */
+
+/* ---- precision modes ---------------------------------------------------- */
+/* EXACT32: FP32 with the reference's operation order and no contraction;
+ *          bit-exact with VERSION5 (which computes in float throughout).
+ * FP64:    the same recipe in double ("plain FP64").
+ * FP32SCREEN: FP32 evaluation + screening with a rounding-error margin that
+ *          makes the candidate set a superset of the FP64 one, then FP64
+ *          re-ranking and FP64 refinement; results equal FP64 mode.          */
+#define JSI_PREC_EXACT32 0
+#define JSI_PREC_FP64 1
+#define JSI_PREC_FP32SCREEN 2
+
+#define JSI_MAX_ROUNDS 16
+#define JSI_MAX_NUV 1025 /* cell index must fit 20 bits */
+#define JSI_MAX_K 8192
+
+/* Region of one surface, the value part of EvalAndConstructTask
+ * (JSI_CAD_NEXT_VERSION5/include/jsi_cad.hpp:11-23). side is 1 or 2. */
+typedef struct jsi_region {
+  float ustart, vstart, ustop, vstop;
+  int nuv;
+  int side;
+} jsi_region;
+
+/* Query parameters.  jsi_params_default() fills the VERSION5 values
+ * (distance_analysis.cu:33-36, intersection.cu:36,368-370, jsi_cad.hpp:35-36). */
+typedef struct jsi_params {
+  int k;                 /* distance top-k (k_of_topk = 8) */
+  int dist_refine_nuv;   /* nuv of distance refinement units (next_nuv = 16) */
+  int dist_max_rounds;   /* distance screening rounds (max_layers = 4) */
+  int isect_max_rounds;  /* intersection screening rounds (6) */
+  int isect_refine_nuv;  /* starting intersection refinement nuv (static global_nuv = 16) */
+  int isect_halve_above; /* halve refinement nuv when pairs > this (512) ... */
+  int isect_min_nuv;     /* ... and nuv > this (4) */
+  int use_compute_k;     /* second-derivative AABB padding (CAD_INTERSECT_USE_K, default on) */
+  int precision;         /* JSI_PREC_* */
+  int cull;              /* conservative tile culling inside the screens (results unchanged) */
+  int trace;             /* fill jsi_round_info.hash (costs a device->host copy per round) */
+  int reserved[5];
+} jsi_params;
+
+typedef struct jsi_round_info {
+  int64_t npairs;  /* distance: size of the top-k list; intersection: overlap pairs */
+  int64_t slots;   /* aligned slots screened in this round */
+  int64_t units1;  /* unique side-1 refinement regions made from this round's pairs */
+  int64_t units2;  /* unique side-2 refinement regions */
+  double kth_d2;   /* distance: largest d2 of the round's top-k (k-th smallest) */
+  uint64_t hash;   /* FNV-1a 64 over the ordered pair list (trace only) */
+  int nuv;         /* sampling nuv of the regions screened in this round */
+  int pad_;
+} jsi_round_info;
+
+typedef struct jsi_result {
+  double distance;       /* distance: sqrt(k-th smallest centre d2) of the last round */
+  int64_t npairs;        /* pairs of the last round */
+  int rounds;            /* rounds screened */
+  int isect_refine_nuv;  /* intersection: refinement nuv after the query (halving state) */
+  double ms_total;       /* host wall clock for the whole query */
+  double ms_eval;        /* evaluation + AABB construction */
+  double ms_screen;      /* screening + top-k selection */
+  jsi_round_info round[JSI_MAX_ROUNDS];
+} jsi_result;
+
+/* One candidate pair: slot and cell ids as in DistPairRef / IntersectPairRef
+ * (distance_analysis.cu:190-195, intersection.cu:248-252). d2 = 0 for
+ * intersection pairs. */
+typedef struct jsi_pair {
+  double d2;
+  int32_t slot;
+  int32_t i;
+  int32_t j;
+  int32_t pad_;
+} jsi_pair;
+
+typedef struct jsi_ctx jsi_ctx;
+
+void jsi_params_default(jsi_params* p);
+const char* jsi_status_string(int status);
+const char* jsi_last_error(void);
+
+/* Context: one per device.  hbm_cap_bytes bounds the device arena the
+ * context may hold (0 = 3/4 of free HBM).  Replaces init_cad/deinit_cad
+ * (JSI_CAD_NEXT_VERSION5/src/common/common.cu:35-61). */
+int jsi_ctx_create(int device, size_t hbm_cap_bytes, jsi_ctx** out);
+int jsi_ctx_destroy(jsi_ctx* ctx);
+int jsi_ctx_sync(jsi_ctx* ctx);
+
+/* Copies a NurbsFace (host pointers) to device memory; the surface stays
+ * resident until released.  *handle >= 0 on success. */
+int jsi_surface_upload(jsi_ctx* ctx, const int ivec[7], const float* data, int ndata, int* handle);
+/* Registers a surface whose ivec/data already live in device memory (the
+ * EvalAndConstructTask::d_ivec/d_data convention of VERSION5). */
+int jsi_surface_register_device(jsi_ctx* ctx, const int* d_ivec, const float* d_data, int ndata, int* handle);
+int jsi_surface_release(jsi_ctx* ctx, int handle);
+
+/* Grid evaluation, the evalWithCuda contract
+ * (JSI_CAD_NEXT_VERSION5/src/evaluation/evaluation.cu:391-405 and :358-374):
+ * point (idx, idy) at u = ustep*idx + ustart, v = vstep*idy + vstart, written
+ * to out[(idx + idy*nuv)*18 + c], c = [S, Su, Sv, Suu, Suv, Svv] x xyz.
+ * out is host memory.  _f32 runs EXACT32, _f64 the FP64 recipe. */
+int jsi_eval_grid_f32(jsi_ctx* ctx, int handle, float ustep, float vstep, float ustart, float vstart, int nuv,
+                      float* out);
+int jsi_eval_grid_f64(jsi_ctx* ctx, int handle, double ustep, double vstep, double ustart, double vstart,
+                      int nuv, double* out);
+
+/* Minimum distance, the run_dist path: minimum_distance
+ * (JSI_CAD_NEXT_VERSION5/src/distance_analysis/distance_analysis.cu:356-506).
+ * pairs_out (optional, host) receives the last round's sorted top-k. */
+int jsi_min_distance(jsi_ctx* ctx, int h1, const jsi_region* r1, int h2, const jsi_region* r2,
+                     const jsi_params* params, jsi_result* result, jsi_pair* pairs_out, int pairs_cap);
+
+/* Surface-surface intersection screening, the run_intersect path:
+ * intersection (JSI_CAD_NEXT_VERSION5/src/intersection/intersection.cu:338-475).
+ * pairs_out (optional, host) receives the last round's ordered pairs. */
+int jsi_intersect(jsi_ctx* ctx, int h1, const jsi_region* r1, int h2, const jsi_region* r2,
+                  const jsi_params* params, jsi_result* result, jsi_pair* pairs_out, int pairs_cap);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* JSI_MI355X_H */

@@ -1,0 +1,117 @@
+"""ctypes mirrors of the C ABI structs in include/jsi_mi355x.h.
+
+The field order and types must match the header exactly; test_abi_layout
+checks the sizes against the values the library reports.
+"""
+import ctypes as C
+
+JSI_OK = 0
+JSI_ERR_INVALID = -1
+JSI_ERR_NOMEM = -2
+JSI_ERR_DEVICE = -3
+JSI_ERR_CAPACITY = -4
+JSI_ERR_UNSUPPORTED = -5
+JSI_ERR_BUSY = -6
+
+PREC_EXACT32 = 0
+PREC_FP64 = 1
+PREC_FP32SCREEN = 2
+PRECISIONS = {"exact32": PREC_EXACT32, "fp64": PREC_FP64, "fp32screen": PREC_FP32SCREEN}
+
+MAX_ROUNDS = 16
+MAX_NUV = 1025
+MAX_K = 8192
+
+
+class Region(C.Structure):
+    """jsi_region: the value part of EvalAndConstructTask (jsi_cad.hpp:11-23)."""
+    _fields_ = [
+        ("ustart", C.c_float),
+        ("vstart", C.c_float),
+        ("ustop", C.c_float),
+        ("vstop", C.c_float),
+        ("nuv", C.c_int),
+        ("side", C.c_int),
+    ]
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("k", C.c_int),
+        ("dist_refine_nuv", C.c_int),
+        ("dist_max_rounds", C.c_int),
+        ("isect_max_rounds", C.c_int),
+        ("isect_refine_nuv", C.c_int),
+        ("isect_halve_above", C.c_int),
+        ("isect_min_nuv", C.c_int),
+        ("use_compute_k", C.c_int),
+        ("precision", C.c_int),
+        ("cull", C.c_int),
+        ("trace", C.c_int),
+        ("reserved", C.c_int * 5),
+    ]
+
+
+class RoundInfo(C.Structure):
+    _fields_ = [
+        ("npairs", C.c_int64),
+        ("slots", C.c_int64),
+        ("units1", C.c_int64),
+        ("units2", C.c_int64),
+        ("kth_d2", C.c_double),
+        ("hash", C.c_uint64),
+        ("nuv", C.c_int),
+        ("pad_", C.c_int),
+    ]
+
+
+class Result(C.Structure):
+    _fields_ = [
+        ("distance", C.c_double),
+        ("npairs", C.c_int64),
+        ("rounds", C.c_int),
+        ("isect_refine_nuv", C.c_int),
+        ("ms_total", C.c_double),
+        ("ms_eval", C.c_double),
+        ("ms_screen", C.c_double),
+        ("round", RoundInfo * MAX_ROUNDS),
+    ]
+
+    def rounds_list(self):
+        out = []
+        for r in range(self.rounds):
+            ri = self.round[r]
+            out.append(dict(npairs=ri.npairs, slots=ri.slots, units1=ri.units1, units2=ri.units2,
+                            kth_d2=ri.kth_d2, hash=ri.hash, nuv=ri.nuv))
+        return out
+
+
+class Pair(C.Structure):
+    _fields_ = [("d2", C.c_double), ("slot", C.c_int32), ("i", C.c_int32), ("j", C.c_int32), ("pad_", C.c_int32)]
+
+
+def default_params(**overrides):
+    """VERSION5 defaults (distance_analysis.cu:33-36, intersection.cu:36,368-370)."""
+    p = Params()
+    p.k = 8
+    p.dist_refine_nuv = 16
+    p.dist_max_rounds = 4
+    p.isect_max_rounds = 6
+    p.isect_refine_nuv = 16
+    p.isect_halve_above = 512
+    p.isect_min_nuv = 4
+    p.use_compute_k = 1
+    p.precision = PREC_EXACT32
+    p.cull = 1
+    p.trace = 0
+    for key, val in overrides.items():
+        if key == "precision" and isinstance(val, str):
+            val = PRECISIONS[val]
+        setattr(p, key, int(val))
+    return p
+
+
+def pairs_to_numpy(buf, n):
+    import numpy as np
+    dt = np.dtype([("d2", "<f8"), ("slot", "<i4"), ("i", "<i4"), ("j", "<i4"), ("pad_", "<i4")])
+    return np.frombuffer(bytes(memoryview(buf).cast("B")[: n * C.sizeof(Pair)]), dtype=dt).copy()
