@@ -1,0 +1,146 @@
+"""Minimum-distance throughput on one MI355X.
+
+Workload: two rational bicubic 64x64 'hard wave' surfaces (weights U[0.8,1.2],
+z = 0.05 sin(16 pi x) sin(16 pi y) and the same wave shifted by pi/7 in x,
+lifted to a ~1e-3 gap), initial grid nuv=512 (261,121 cells per surface,
+6.8e10 cell pairs in round 1), top-k = 4096, refinement nuv = 16, 4 rounds.
+One step = one query.  Inputs are resident in HBM before the timed region.
+
+Prints ONE JSON line (see the keys below).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "queries/s + p99 latency at nuv=512, batched vs plain GPU, dist err ≤1e-9"
+PEAK = {"f32": 157.3, "f64": 78.6}  # TFLOP/s, MI355X published vector/matrix peaks
+
+
+def pct(xs, q):
+    return float(np.percentile(np.asarray(xs), q)) if xs else None
+
+
+def run_form(ctx, a, r1, b, r2, params, steps, warmup, profile_name=None):
+    for _ in range(warmup):
+        ctx.minimum_distance(a, r1, b, r2, params)
+    ctx.sync()
+    if profile_name:
+        ctx.profile(True)
+    lat = []
+    res = None
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        q0 = time.perf_counter()
+        res, _ = ctx.minimum_distance(a, r1, b, r2, params)
+        lat.append((time.perf_counter() - q0) * 1e3)
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    stats = ctx.kernel_stats(profile_name) if profile_name else None
+    if profile_name:
+        ctx.profile(False)
+    return res, wall, lat, stats
+
+
+def cpu_baseline(a, r1, b, r2, k, queries):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from jsi_cad_next_versions_amd._abi import default_params
+    from oracle import Oracle
+    o = Oracle()
+    p = default_params(k=k, precision="exact32")
+    t0 = time.perf_counter()
+    for _ in range(queries):
+        o.min_distance(a, r1, b, r2, p)
+    dt = time.perf_counter() - t0
+    return {"value": queries / dt, "unit": "queries/s", "cores": o.num_threads(), "kind": "port",
+            "sample": f"{queries} full query(ies) of the same workload (nuv={r1.nuv}, k={k}, EXACT32) "
+                      f"on the OpenMP oracle, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--nuv", type=int, default=512)
+    ap.add_argument("--k", type=int, default=4096)
+    ap.add_argument("--precision", default="exact32")
+    ap.add_argument("--plain-steps", type=int, default=5)
+    ap.add_argument("--cpu-queries", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (each GPU would serve its own "
+              "queries); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    from jsi_cad_next_versions_amd import surfaces as S
+    from jsi_cad_next_versions_amd._abi import default_params
+    from jsi_cad_next_versions_amd.runtime import Context
+
+    a, b = S.hard_wave_pair()
+    r1, r2 = S.full_region(a, args.nuv, 1), S.full_region(b, args.nuv, 2)
+    ctx = Context(0)
+    ctx.upload(a)
+    ctx.upload(b)
+
+    # chosen form
+    dt32 = "f64" if args.precision == "fp64" else "f32"
+    p = default_params(k=args.k, precision=args.precision, cull=1)
+    res, wall, lat, st = run_form(ctx, a, r1, b, r2, p, args.steps, args.warmup, f"dist_filter_{dt32}")
+    qps = args.steps / wall
+    # plain FP64 GPU path: FP64 everywhere, exhaustive screen (no culling)
+    pp = default_params(k=args.k, precision="fp64", cull=0)
+    pres, pwall, plat, _ = run_form(ctx, a, r1, b, r2, pp, args.plain_steps, 1)
+    pqps = args.plain_steps / pwall
+    dist_err = abs(res.distance - pres.distance)
+
+    launches, kms, pairs = st
+    flops = 8.0 * pairs  # 3 sub + 3 mul + 2 add per cell pair
+    achieved = flops / (kms * 1e-3) / 1e12 if kms > 0 else 0.0
+    cells = (args.nuv - 1) ** 2 * 2 + sum(
+        (res.round[r].units1 + res.round[r].units2) * 15 * 15 for r in range(res.rounds))
+    out = {
+        "metric": METRIC,
+        "value": qps,
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": dt32,
+        "data": "synthetic (seeded SplitMix64 surfaces)",
+        "config": {"workload": f"min-distance, rational bicubic 64x64 hard-wave pair, nuv={args.nuv}, "
+                               f"k={args.k}, refine nuv=16, 4 rounds",
+                   "precision": args.precision, "cull": 1},
+        "p50_ms": pct(lat, 50),
+        "p99_ms": pct(lat, 99),
+        "cells_per_s": cells * qps,
+        "distance": res.distance,
+        "plain_fp64": {"queries_per_s": pqps, "p50_ms": pct(plat, 50), "p99_ms": pct(plat, 99),
+                       "distance": pres.distance},
+        "speedup_vs_plain": qps / pqps if pqps > 0 else None,
+        "dist_err_vs_fp64": dist_err,
+        "roofline": {"bound": "mfma", "kernel": f"k_dist_filter<{'float' if dt32 == 'f32' else 'double'}>",
+                     "achieved": achieved, "peak": PEAK[dt32], "unit": "TFLOP/s",
+                     "frac": achieved / PEAK[dt32], "traffic": None,
+                     "launches": launches, "avg_launch_ms": kms / max(launches, 1),
+                     "pairs_per_launch": pairs / max(launches, 1)},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(a, r1, b, r2, args.k, args.cpu_queries)
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -116,6 +116,11 @@ const char* jsi_last_error(void);
 int jsi_ctx_create(int device, size_t hbm_cap_bytes, jsi_ctx** out);
 int jsi_ctx_destroy(jsi_ctx* ctx);
 int jsi_ctx_sync(jsi_ctx* ctx);
+/* Per-kernel timing (HIP events on the context stream) of the screen
+ * kernels: enable = 1 clears the counters and starts recording.  Names:
+ * "dist_filter_f32", "dist_filter_f64".  units = cell pairs evaluated. */
+int jsi_ctx_profile(jsi_ctx* ctx, int enable);
+int jsi_ctx_kernel_stats(jsi_ctx* ctx, const char* name, int64_t* launches, double* ms, double* units);
 
 /* Copies a NurbsFace (host pointers) to device memory; the surface stays
  * resident until released.  *handle >= 0 on success. */

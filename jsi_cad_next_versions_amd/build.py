@@ -1,0 +1,41 @@
+"""Builds libjsi_mi355x.so for gfx950 with hipcc (in-tree)."""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libjsi_mi355x.so")
+SOURCES = ["jsi_kernels.hip", "jsi_runtime.hip", "jsi_legacy.hip"]
+
+# -ffp-contract=off: no fused multiply-add, so float results match VERSION5
+# (built without FMA on x86-64) and double results match the FP64 oracle.
+# Packed FP32 VALU ops are disabled (a known-bad instruction family on this pool).
+HIPFLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math",
+            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wno-unused-command-line-argument"]
+
+
+def build(verbose=False, jobs=3):
+    objs = []
+    procs = []
+    for src in SOURCES:
+        path = os.path.join(CSRC, src)
+        if not os.path.exists(path):
+            continue
+        obj = os.path.join(CSRC, src.replace(".hip", ".o"))
+        cmd = ["hipcc", *HIPFLAGS, "-c", path, "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        procs.append((subprocess.Popen(cmd), cmd))
+        objs.append(obj)
+    bad = [cmd for p, cmd in procs if p.wait() != 0]
+    if bad:
+        raise RuntimeError("hipcc failed: " + " ".join(bad[0]))
+    cmd = ["hipcc", "--offload-arch=gfx950", "-shared", "-fPIC", "-o", LIB + ".tmp", *objs]
+    subprocess.run(cmd, check=True)
+    os.replace(LIB + ".tmp", LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

@@ -1,0 +1,150 @@
+// jsi_internal.hpp - types and kernel launchers shared by the HIP kernels
+// (jsi_kernels.hip) and the host runtime (jsi_runtime.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace jsi {
+
+template <class T>
+struct Vec4;
+template <>
+struct Vec4<float> {
+  using type = float4;
+};
+template <>
+struct Vec4<double> {
+  using type = double4;
+};
+template <class T>
+using vec4_t = typename Vec4<T>::type;
+
+// One distance candidate.  hi = bit pattern of d2 (non-negative, so the
+// unsigned order of the bits is the numeric order), lo = slot<<40 | i<<20 | j.
+// Lexicographic (hi, lo) order is the total order (d2, slot, i, j) of
+// dist_pair_ref_less (distance_analysis.cu:226-231).
+struct Cand {
+  uint64_t hi;
+  uint64_t lo;
+};
+
+// One intersection pair (IntersectPairRef, intersection.cu:248-252).
+struct IPair {
+  int32_t slot;
+  int32_t i;
+  int32_t j;
+};
+
+constexpr int kTile = 16;          // cells per tile edge in the screens
+constexpr int kTileCells = kTile * kTile;
+constexpr int kSortBlock = 2048;   // records per LDS bitonic block
+
+inline __host__ __device__ uint64_t pack_idx(uint32_t slot, uint32_t i, uint32_t j) {
+  return (uint64_t(slot) << 40) | (uint64_t(i) << 20) | uint64_t(j);
+}
+
+// Geometry of one evaluation batch: `units` regions of the same surface
+// sampled on the same nuv x nuv grid.
+template <class T>
+struct SurfDev {
+  const T* U;   // knots u
+  const T* V;   // knots v
+  const T* CP;  // xyz, u fastest
+  const T* W;   // weights or nullptr
+  int degu, degv, ncpu, ncpv, nku, nkv;
+  int rational;
+  int zero;  // periodic / degree >= 25: all-zero output (evaluation.cu:271-272)
+};
+
+enum EvalMode { kEvalFull18 = 0, kEvalPoint = 1, kEvalPointK = 2 };
+
+// ---- launchers (defined in jsi_kernels.hip) --------------------------------
+template <class T>
+hipError_t launch_promote(const float* src, T* dst, int n, hipStream_t s);
+
+template <class T>
+hipError_t launch_basis(const T* knots, int nk, int p, int der, const vec4_t<T>* regions, int nunits, int nuv,
+                        int dir, T* table, int* spans, hipStream_t s);
+
+// Evaluates nunits x nuv x nuv points.  mode kEvalFull18: out = AoS 18 per
+// point (evalWithCuda layout, unit-major).  kEvalPoint / kEvalPointK: out =
+// SoA xyz per unit [unit][3][nuv*nuv]; kEvalPointK also max-reduces
+// |Suu|,|Suv|,|Svv| into kmax[unit*3 + {0,1,2}] (ordered bit patterns).
+template <class T>
+hipError_t launch_points(const SurfDev<T>& sd, const T* tabU, const int* spanU, const T* tabV, const int* spanV,
+                         int nunits, int nuv, int mode, T* out, unsigned long long* kmax, hipStream_t s);
+
+// Cell AABBs of every unit from its SoA points.  isect = 0: distance centres
+// (construct_aabbs_distance_host); isect = 1: centre + padded radius
+// (construct_aabbs_host) with the compute_K padding from kmax.
+template <class T>
+hipError_t launch_aabb(const T* pts, int nunits, int nuv, int isect, int use_k, const unsigned long long* kmax,
+                       const vec4_t<T>* regions, T init_u, T init_v, vec4_t<T>* C, vec4_t<T>* R, hipStream_t s);
+
+// Per-tile bounding boxes.  isect = 0: box of centres; isect = 1: box of the
+// padded cells, widened outward so the tile test is conservative.
+template <class T>
+hipError_t launch_tilebox(const vec4_t<T>* C, const vec4_t<T>* R, int nunits, int w, int isect, vec4_t<T>* lo,
+                          vec4_t<T>* hi, hipStream_t s);
+
+// Distance screen.
+template <class T>
+hipError_t launch_dist_seed(int nslots, const int* su1, const int* su2, int w1, int w2, const vec4_t<T>* lo1,
+                            const vec4_t<T>* hi1, const vec4_t<T>* lo2, const vec4_t<T>* hi2, int k,
+                            unsigned long long* best, hipStream_t s);
+template <class T>
+hipError_t launch_dist_list(int nslots, const int* su1, const int* su2, int w1, int w2, const vec4_t<T>* lo1,
+                            const vec4_t<T>* hi1, const vec4_t<T>* lo2, const vec4_t<T>* hi2, const T* thresh,
+                            int cull, uint64_t* list, unsigned* list_count, hipStream_t s);
+hipError_t launch_seed_list(const unsigned long long* best, uint64_t* list, unsigned* list_count, int w1, int w2,
+                            hipStream_t s);
+template <class T>
+hipError_t launch_dist_filter(const uint64_t* list, const unsigned* list_count, unsigned list_cap, const int* su1,
+                              const int* su2, int w1, int w2, const vec4_t<T>* C1, const vec4_t<T>* C2,
+                              const T* thresh, Cand* out, unsigned long long* out_count, unsigned long long cap,
+                              int grid, hipStream_t s, unsigned long long* pairs_done = nullptr);
+template <class T>
+hipError_t launch_set_thresh(T* thresh, const Cand* rec, int idx, int use_inf, hipStream_t s);
+
+// Sorting of candidate records (bitonic, n2 = power of two >= kSortBlock).
+hipError_t launch_fill_sentinel(Cand* a, unsigned long long from, unsigned long long n2, hipStream_t s);
+hipError_t sort_cands(Cand* a, unsigned long long n2, hipStream_t s);
+
+// Distance refinement: first-appearance dedup of the sorted top-k endpoints,
+// child regions and the next round's slot -> unit maps (single workgroup).
+template <class T>
+hipError_t launch_dist_dedup(const Cand* rec, int n, int cells1, int cells2, const int* su1, const int* su2,
+                             const vec4_t<T>* reg1, const vec4_t<T>* reg2, int parent_nuv1, int parent_nuv2,
+                             int* firstpos1, int* firstpos2, int* nsu1, int* nsu2, vec4_t<T>* nreg1,
+                             vec4_t<T>* nreg2, int* nunits, hipStream_t s);
+
+// Intersection screen.
+template <class T>
+hipError_t launch_isect_list(int nslots, const int* su1, const int* su2, int w1, int w2, const vec4_t<T>* lo1,
+                             const vec4_t<T>* hi1, const vec4_t<T>* lo2, const vec4_t<T>* hi2, int cull,
+                             uint64_t* list, unsigned long long* list_count, unsigned long long list_cap,
+                             hipStream_t s);
+template <class T>
+hipError_t launch_isect_pass(int write, const uint64_t* list, const unsigned long long* list_count, const int* su1,
+                             const int* su2, int w1, int w2, const vec4_t<T>* C1, const vec4_t<T>* R1,
+                             const vec4_t<T>* C2, const vec4_t<T>* R2, unsigned* rowcount,
+                             const unsigned long long* rowoff, unsigned* rowcursor, IPair* out, int grid,
+                             hipStream_t s);
+hipError_t launch_row_sort(const unsigned long long* rowoff, unsigned long long nrows, IPair* out, hipStream_t s);
+hipError_t scan_u32_to_u64(const unsigned* in, unsigned long long* out, unsigned long long n, void* tmp,
+                           size_t tmp_bytes, hipStream_t s);
+size_t scan_tmp_bytes(unsigned long long n);
+hipError_t launch_isect_dedup_flags(const IPair* pairs, unsigned long long n, int cells2, int* firstpos2,
+                                    unsigned* flag1, unsigned* flag2, hipStream_t s);
+template <class T>
+hipError_t launch_isect_units(const IPair* pairs, unsigned long long n, int cells2, const int* su1, const int* su2,
+                              const vec4_t<T>* reg1, const vec4_t<T>* reg2, int parent_nuv, int parent_w,
+                              const unsigned* flag1, const unsigned* flag2, const unsigned long long* idx1,
+                              const unsigned long long* idx2, int* firstpos2, int* nsu1, int* nsu2,
+                              vec4_t<T>* nreg1, vec4_t<T>* nreg2, hipStream_t s);
+
+hipError_t launch_fill_i32(int* p, int value, unsigned long long n, hipStream_t s);
+hipError_t launch_fill_u32(unsigned* p, unsigned value, unsigned long long n, hipStream_t s);
+
+}  // namespace jsi

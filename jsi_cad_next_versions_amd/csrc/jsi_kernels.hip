@@ -1,0 +1,1346 @@
+// jsi_kernels.hip - hand-written gfx950 kernels of the minimum-distance /
+// surface-surface-intersection hot path.
+//
+// Every arithmetic expression that produces a result the reference also
+// produces keeps the reference's operation order; the library is compiled
+// with -ffp-contract=off, so float kernels are bit-exact with VERSION5 and
+// double kernels with the FP64 oracle.  File:line citations refer to
+// JSI_CAD_NEXT_VERSION5/.
+#include <algorithm>
+#include <float.h>
+#include <limits.h>
+
+#include "jsi_internal.hpp"
+
+namespace jsi {
+
+namespace {
+
+constexpr int kMaxDeg = 25;  // MAXDEG, evaluation.cu:13
+
+// std::min / std::max: the first argument wins unless the second is strictly
+// smaller / larger; the nesting order fixes the sign of zero.
+template <class T>
+__device__ __forceinline__ T smin(T a, T b) {
+  return (b < a) ? b : a;
+}
+template <class T>
+__device__ __forceinline__ T smax(T a, T b) {
+  return (a < b) ? b : a;
+}
+// CadMax (common.cu:132-135)
+template <class T>
+__device__ __forceinline__ T cmax(T a, T b) {
+  return a > b ? a : b;
+}
+
+__device__ __forceinline__ unsigned long long ord_bits(float x) { return (unsigned long long)__float_as_uint(x); }
+__device__ __forceinline__ unsigned long long ord_bits(double x) {
+  return (unsigned long long)__double_as_longlong(x);
+}
+template <class T>
+__device__ __forceinline__ T from_bits(unsigned long long b);
+template <>
+__device__ __forceinline__ float from_bits<float>(unsigned long long b) {
+  return __uint_as_float((unsigned)b);
+}
+template <>
+__device__ __forceinline__ double from_bits<double>(unsigned long long b) {
+  return __longlong_as_double((long long)b);
+}
+__device__ __forceinline__ unsigned ord32_up(float x) { return __float_as_uint(x); }
+__device__ __forceinline__ unsigned ord32_up(double x) { return __float_as_uint(__double2float_ru(x)); }
+
+template <class T>
+__device__ __forceinline__ T infinity_v();
+template <>
+__device__ __forceinline__ float infinity_v<float>() {
+  return __uint_as_float(0x7f800000u);
+}
+template <>
+__device__ __forceinline__ double infinity_v<double>() {
+  return __longlong_as_double(0x7ff0000000000000ll);
+}
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+inline unsigned blocks_for(unsigned long long n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// ---------------------------------------------------------------------------
+// Knot span and basis functions (FindSpan / DersBasisFuns, evaluation.cu:152-219;
+// The NURBS Book A2.1 and A2.3).
+// ---------------------------------------------------------------------------
+template <class T>
+__device__ int find_span(int nk, int p, T u, const T* U) {
+  if (u <= U[p]) return p;
+  const int n = nk - p - 1;
+  if (u >= U[n]) return n - 1;
+  int lo = p, hi = n, mid = (lo + hi) / 2;
+  // A non-decreasing knot vector ends the search within log2(n)+1 steps; the
+  // cap only keeps a malformed vector from spinning forever.
+  for (int it = 0; it < 64 && (u < U[mid] || u >= U[mid + 1]); ++it) {
+    if (u < U[mid])
+      hi = mid;
+    else
+      lo = mid;
+    mid = (lo + hi) / 2;
+  }
+  return mid;
+}
+
+template <class T, int MAXP>
+__device__ void basis_ders(int span, int p, T u, const T* U, int der, T (&ders)[3][MAXP + 1]) {
+  T ndu[MAXP + 1][MAXP + 1];
+  T left[MAXP + 1], right[MAXP + 1];
+  T a[2][MAXP + 1];
+  ndu[0][0] = T(1);
+  for (int j = 1; j <= p; ++j) {
+    left[j] = u - U[span + 1 - j];
+    right[j] = U[span + j] - u;
+    T saved = T(0);
+    for (int r = 0; r < j; ++r) {
+      ndu[j][r] = right[r + 1] + left[j - r];
+      const T temp = ndu[r][j - 1] / ndu[j][r];
+      ndu[r][j] = saved + right[r + 1] * temp;
+      saved = left[j - r] * temp;
+    }
+    ndu[j][j] = saved;
+  }
+  for (int j = 0; j <= p; ++j) ders[0][j] = ndu[j][p];
+  for (int r = 0; r <= p; ++r) {
+    int s1 = 0, s2 = 1;
+    a[0][0] = T(1);
+    for (int k = 1; k <= der; ++k) {
+      T d = T(0);
+      const int rk = r - k, pk = p - k;
+      if (r >= k) {
+        a[s2][0] = a[s1][0] / ndu[pk + 1][rk];
+        d = a[s2][0] * ndu[rk][pk];
+      }
+      const int j1 = (rk >= -1) ? 1 : -rk;
+      const int j2 = (r - 1 <= pk) ? k - 1 : p - r;
+      for (int j = j1; j <= j2; ++j) {
+        a[s2][j] = (a[s1][j] - a[s1][j - 1]) / ndu[pk + 1][rk + j];
+        d += a[s2][j] * ndu[rk + j][pk];
+      }
+      if (r <= pk) {
+        a[s2][k] = -a[s1][k - 1] / ndu[pk + 1][r];
+        d += a[s2][k] * ndu[r][pk];
+      }
+      ders[k][r] = d;
+      const int tmp = s1;
+      s1 = s2;
+      s2 = tmp;
+    }
+  }
+  int r = p;
+  for (int k = 1; k <= der; ++k) {
+    for (int j = 0; j <= p; ++j) ders[k][j] *= T(r);
+    r *= p - k;
+  }
+}
+
+// One thread per (unit, sample index, direction).  The basis of a grid point
+// depends only on its column (u) or row (v), so it is computed once per
+// column/row and shared by the nuv points that use it.  `regions` holds
+// (ustart, vstart, ustop, vstop); the sample parameter follows
+// run_evaluation_and_construction_task + the grid driver
+// (distance_analysis.cu:92-93, evaluation.cu:366): step = (stop-start)/nuv,
+// t = step*idx + start.  steps_given = 1: regions hold (ustart, vstart,
+// ustep, vstep) directly (evalWithCuda).
+template <class T, int MAXP>
+__global__ void k_basis(const T* knots, int nk, int p, int der, const vec4_t<T>* regions, int nunits, int nuv,
+                        int dir, int steps_given, T* table, int* spans) {
+  const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= (long long)nunits * nuv) return;
+  const int unit = (int)(gid / nuv), idx = (int)(gid % nuv);
+  const vec4_t<T> r = regions[unit];
+  const T start = dir == 0 ? r.x : r.y;
+  T step;
+  if (steps_given)
+    step = dir == 0 ? r.z : r.w;
+  else
+    step = ((dir == 0 ? r.z : r.w) - start) / T(nuv);
+  const T t = step * T(idx) + start;
+  const int span = find_span(nk, p, t, knots);
+  T d[3][MAXP + 1];
+  basis_ders<T, MAXP>(span, p, t, knots, der, d);
+  T* o = table + (size_t)gid * 3 * (p + 1);
+  for (int k = 0; k < 3; ++k)
+    for (int j = 0; j <= p; ++j) o[k * (p + 1) + j] = (k <= der) ? d[k][j] : T(0);
+  spans[gid] = span;
+}
+
+// Position of derivative (k = u order, l = v order) in the output
+// [S, Su, Sv, Suu, Suv, Svv] (EvaluateOneUV loop order + row swap,
+// evaluation.cu:281-310).
+__device__ __forceinline__ int out_row(int k, int l) {
+  const int m = (l == 0) ? k : (l == 1 ? 3 + k : 5);
+  return m == 2 ? 3 : (m == 3 ? 2 : m);
+}
+
+// Grid points: EvaluateOneUV (evaluation.cu:259-356) on precomputed basis
+// rows.  The u-direction partial sums R[k][s] = sum_j Nu[k][j]*(w)*P are
+// identical for every l, so they are formed once per (k, s).
+template <class T, int MAXD, bool RAT, int MODE>
+__global__ __launch_bounds__(256) void k_points(SurfDev<T> sd, const T* tabU, const int* spanU, const T* tabV,
+                                                const int* spanV, int nunits, int nuv, T* out,
+                                                unsigned long long* kmax) {
+  const int npts = nuv * nuv;
+  const int bpu = (npts + 255) / 256;  // blocks per unit: a block never spans two units
+  const int unit = blockIdx.x / bpu;
+  const int pt = (blockIdx.x % bpu) * 256 + threadIdx.x;
+  const bool active = unit < nunits && pt < npts;
+  const int idx = pt % nuv, idy = pt / nuv;
+  const int p = sd.degu, q = sd.degv;
+  const int du = p < 2 ? p : 2, dv = q < 2 ? q : 2;
+  T res[18];
+#pragma unroll
+  for (int e = 0; e < 18; ++e) res[e] = T(0);
+  if (active && !sd.zero) {
+    const int su = spanU[unit * nuv + idx], sv = spanV[unit * nuv + idy];
+    const T* Nu = tabU + (size_t)(unit * nuv + idx) * 3 * (p + 1);
+    const T* Nv = tabV + (size_t)(unit * nuv + idy) * 3 * (q + 1);
+    constexpr int NK = (MODE == kEvalPoint) ? 1 : 3;
+    const int kmax_u = (MODE == kEvalPoint) ? 0 : du;
+    constexpr int NC = RAT ? 4 : 3;
+    T R[NK][MAXD + 1][NC];
+    for (int s = 0; s <= q; ++s) {
+      const int base = sd.ncpu * (sv - q + s) + su - p;
+      for (int k = 0; k <= kmax_u; ++k) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) R[k][s][c] = T(0);
+        for (int j = 0; j <= p; ++j) {
+          const int id = base + j;
+          const T n = Nu[k * (p + 1) + j];
+          if (RAT) {
+            const T nw = n * sd.W[id];
+            R[k][s][0] += nw * sd.CP[3 * id + 0];
+            R[k][s][1] += nw * sd.CP[3 * id + 1];
+            R[k][s][2] += nw * sd.CP[3 * id + 2];
+            R[k][s][NC - 1] += nw;
+          } else {
+            R[k][s][0] += n * sd.CP[3 * id + 0];
+            R[k][s][1] += n * sd.CP[3 * id + 1];
+            R[k][s][2] += n * sd.CP[3 * id + 2];
+          }
+        }
+      }
+    }
+    T h[6][4];
+#pragma unroll
+    for (int m = 0; m < 6; ++m)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) h[m][c] = T(0);
+    const int lmax = (MODE == kEvalPoint) ? 0 : dv;
+    for (int l = 0; l <= lmax; ++l) {
+      for (int k = 0; k <= 2 - l; ++k) {
+        if (k > kmax_u) continue;
+        T acc[NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) acc[c] = T(0);
+        for (int s = 0; s <= q; ++s) {
+          const T nv = Nv[l * (q + 1) + s];
+#pragma unroll
+          for (int c = 0; c < NC; ++c) acc[c] += nv * R[k][s][c];
+        }
+        const int m = out_row(k, l);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) h[m][c] = acc[c];
+      }
+    }
+    if (RAT) {
+      // EG_EvaluateQuotientRule2 for dim 3, der 2, stride 4 (evaluation.cu:77-128).
+      const T F = T(1) / h[0][3];
+      if (MODE == kEvalPoint) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) h[0][c] *= F;
+      } else {
+#pragma unroll
+        for (int m = 0; m < 6; ++m)
+#pragma unroll
+          for (int c = 0; c < 4; ++c) h[m][c] *= F;
+        const T ws = -h[1][3], wt = -h[2][3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          h[1][c] += ws * h[0][c];
+          h[2][c] += wt * h[0][c];
+        }
+        const T wss = -h[3][3], wst = -h[4][3], wtt = -h[5][3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const T S = h[0][c], Su = h[1][c], Sv = h[2][c];
+          h[3][c] += wss * S + T(2) * ws * Su;
+          h[4][c] += wst * S + wt * Su + ws * Sv;
+          h[5][c] += wtt * S + T(2) * wt * Sv;
+        }
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < 6; ++m)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) res[3 * m + c] = h[m][c];
+  }
+  if (MODE == kEvalFull18) {
+    if (active) {
+      T* o = out + ((size_t)unit * npts + pt) * 18;
+#pragma unroll
+      for (int e = 0; e < 18; ++e) o[e] = res[e];
+    }
+    return;
+  }
+  if (active) {
+    T* o = out + (size_t)unit * 3 * npts;
+    o[pt] = res[0];
+    o[npts + pt] = res[1];
+    o[2 * npts + pt] = res[2];
+  }
+  if (MODE == kEvalPointK) {
+    // compute_K maxima (common.cu:155-168); max is exact and order-free.
+    unsigned long long mk[3];
+#pragma unroll
+    for (int g = 0; g < 3; ++g) {
+      const T v = cmax(fabs(res[9 + 3 * g]), cmax(fabs(res[10 + 3 * g]), fabs(res[11 + 3 * g])));
+      mk[g] = active ? ord_bits(v) : 0ull;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_xor(mk[g], off);
+        mk[g] = o > mk[g] ? o : mk[g];
+      }
+    }
+    __shared__ unsigned long long sk[4][3];
+    const int wv = threadIdx.x >> 6;
+    if (lane_id() == 0)
+      for (int g = 0; g < 3; ++g) sk[wv][g] = mk[g];
+    __syncthreads();
+    if (threadIdx.x < 3 && unit < nunits) {
+      unsigned long long m = 0;
+      for (int w2 = 0; w2 < 4; ++w2) m = sk[w2][threadIdx.x] > m ? sk[w2][threadIdx.x] : m;
+      atomicMax(&kmax[unit * 3 + threadIdx.x], m);
+    }
+  }
+}
+
+// Cell AABBs (construct_aabbs_distance_host, distance_analysis.cu:45-86;
+// construct_aabbs_host + compute_K padding, intersection.cu:106-148,187-206,
+// common.cu:170-171).  Cell id = iv*(nuv-1) + iu, corners (iv,iu), (iv,iu+1),
+// (iv+1,iu), (iv+1,iu+1).
+template <class T, bool ISECT>
+__global__ void k_aabb(const T* pts, int nunits, int nuv, int use_k, const unsigned long long* kmax,
+                       const vec4_t<T>* regions, T init_u, T init_v, vec4_t<T>* C, vec4_t<T>* R) {
+  const int w = nuv - 1, cells = w * w, npts = nuv * nuv;
+  const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= (long long)nunits * cells) return;
+  const int unit = (int)(gid / cells), cell = (int)(gid % cells);
+  const int iv = cell / w, iu = cell % w;
+  const T* base = pts + (size_t)unit * 3 * npts;
+  const int p0 = iv * nuv + iu, p1 = p0 + 1, p2 = p0 + nuv, p3 = p2 + 1;
+  T lo[3], hi[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const T* b = base + c * npts;
+    const T a0 = b[p0], a1 = b[p1], a2 = b[p2], a3 = b[p3];
+    lo[c] = smin(a0, smin(a1, smin(a2, a3)));
+    hi[c] = smax(a0, smax(a1, smax(a2, a3)));
+  }
+  vec4_t<T> ctr, rad;
+  if (!ISECT) {
+    const T k = T(0);
+    ctr.x = (lo[0] - k + hi[0] + k) / T(2);
+    ctr.y = (lo[1] - k + hi[1] + k) / T(2);
+    ctr.z = (lo[2] - k + hi[2] + k) / T(2);
+    ctr.w = T(0);
+    C[gid] = ctr;
+    return;
+  }
+  T pad = T(0);
+  if (use_k) {
+    const T muu = from_bits<T>(kmax[unit * 3 + 0]);
+    const T muv = from_bits<T>(kmax[unit * 3 + 1]);
+    const T mvv = from_bits<T>(kmax[unit * 3 + 2]);
+    const int side = nuv > 1 ? nuv - 1 : 1;
+    const T s2 = T(side * side);
+    const T raw = (muu / s2 + T(2) * muv / s2 + mvv / s2) / T(8);
+    const vec4_t<T> r = regions[unit];
+    const T us = fabs(r.z - r.x) / init_u;
+    const T vs = fabs(r.w - r.y) / init_v;
+    const T ps = smax(us, vs);
+    pad = raw * ps * ps;
+    if (!isfinite(pad) || pad < T(0)) pad = T(0);
+  }
+  ctr.x = (lo[0] + hi[0]) / T(2);
+  ctr.y = (lo[1] + hi[1]) / T(2);
+  ctr.z = (lo[2] + hi[2]) / T(2);
+  ctr.w = T(0);
+  rad.x = (hi[0] - lo[0]) / T(2) + pad;
+  rad.y = (hi[1] - lo[1]) / T(2) + pad;
+  rad.z = (hi[2] - lo[2]) / T(2) + pad;
+  rad.w = T(0);
+  C[gid] = ctr;
+  R[gid] = rad;
+}
+
+// ---------------------------------------------------------------------------
+// Tiles: 16x16 blocks of cells of one unit's (nuv-1)^2 cell grid.
+// ---------------------------------------------------------------------------
+struct TileGeo {
+  int row0, col0, ht, wd;
+};
+__device__ __forceinline__ TileGeo tile_geo(int t, int w) {
+  const int tw = (w + kTile - 1) / kTile;
+  TileGeo g;
+  g.row0 = (t / tw) * kTile;
+  g.col0 = (t % tw) * kTile;
+  g.ht = min(kTile, w - g.row0);
+  g.wd = min(kTile, w - g.col0);
+  return g;
+}
+__device__ __forceinline__ int tiles_per_unit(int w) {
+  const int tw = (w + kTile - 1) / kTile;
+  return tw * tw;
+}
+
+// One wavefront per tile: min/max over the tile's cells.
+template <class T, bool ISECT>
+__global__ void k_tilebox(const vec4_t<T>* C, const vec4_t<T>* R, int nunits, int w, vec4_t<T>* lo,
+                          vec4_t<T>* hi) {
+  const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = lane_id();
+  const int tpu = tiles_per_unit(w);
+  if (wave >= (long long)nunits * tpu) return;
+  const int unit = (int)(wave / tpu), t = (int)(wave % tpu);
+  const TileGeo g = tile_geo(t, w);
+  const T inf = infinity_v<T>();
+  T mn[3] = {inf, inf, inf}, mx[3] = {-inf, -inf, -inf};
+  T amax = T(0);
+  const size_t ubase = (size_t)unit * w * w;
+  for (int e = lane; e < g.ht * g.wd; e += 64) {
+    const int cell = (g.row0 + e / g.wd) * w + g.col0 + e % g.wd;
+    const vec4_t<T> c = C[ubase + cell];
+    T v[3] = {c.x, c.y, c.z};
+    if (ISECT) {
+      const vec4_t<T> r = R[ubase + cell];
+      const T rr[3] = {r.x, r.y, r.z};
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        mn[a] = fmin(mn[a], v[a] - rr[a]);
+        mx[a] = fmax(mx[a], v[a] + rr[a]);
+        amax = fmax(amax, fabs(v[a]) + fabs(rr[a]));
+      }
+    } else {
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        mn[a] = fmin(mn[a], v[a]);
+        mx[a] = fmax(mx[a], v[a]);
+      }
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      mn[a] = fmin(mn[a], __shfl_xor(mn[a], off));
+      mx[a] = fmax(mx[a], __shfl_xor(mx[a], off));
+    }
+    amax = fmax(amax, __shfl_xor(amax, off));
+  }
+  if (lane == 0) {
+    vec4_t<T> l, h;
+    T delta = T(0);
+    if (ISECT) {
+      // Outward margin covering the rounding of c-r, c+r and of the cell test
+      // |c1-c2| < r1+r2 (a few units in the last place); 2^-20 relative is far
+      // above that for float and double.
+      delta = amax * T(9.5367431640625e-07) + T(1e-30);
+    }
+    l.x = mn[0] - delta;
+    l.y = mn[1] - delta;
+    l.z = mn[2] - delta;
+    l.w = T(0);
+    h.x = mx[0] + delta;
+    h.y = mx[1] + delta;
+    h.z = mx[2] + delta;
+    h.w = T(0);
+    lo[wave] = l;
+    hi[wave] = h;
+  }
+}
+
+struct TPIndex {
+  int slot, t1, t2;
+};
+__device__ __forceinline__ TPIndex tp_decode(long long gid, int tpu1, int tpu2) {
+  TPIndex r;
+  const long long per = (long long)tpu1 * tpu2;
+  r.slot = (int)(gid / per);
+  const int rem = (int)(gid % per);
+  r.t1 = rem / tpu2;
+  r.t2 = rem % tpu2;
+  return r;
+}
+__device__ __forceinline__ uint64_t tp_encode(int slot, int t1, int t2) {
+  return ((uint64_t)(uint32_t)slot << 32) | ((uint64_t)t1 << 16) | (uint64_t)t2;
+}
+
+// Seed tile pair of the distance screen: the one with the smallest upper
+// bound on centre d2 among tile pairs holding at least k cell pairs.  Its
+// exact k-th d2 is an upper bound of the global k-th d2.
+template <class T>
+__global__ void k_dist_seed(int nslots, const int* su1, const int* su2, int w1, int w2, const vec4_t<T>* lo1,
+                            const vec4_t<T>* hi1, const vec4_t<T>* lo2, const vec4_t<T>* hi2, int k,
+                            unsigned long long* best) {
+  const int tpu1 = tiles_per_unit(w1), tpu2 = tiles_per_unit(w2);
+  const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long key = ~0ull;
+  if (gid < (long long)nslots * tpu1 * tpu2) {
+    const TPIndex x = tp_decode(gid, tpu1, tpu2);
+    const TileGeo g1 = tile_geo(x.t1, w1), g2 = tile_geo(x.t2, w2);
+    if ((long long)g1.ht * g1.wd * g2.ht * g2.wd >= k) {
+      const int a = su1[x.slot] * tpu1 + x.t1, b = su2[x.slot] * tpu2 + x.t2;
+      const vec4_t<T> l1 = lo1[a], h1 = hi1[a], l2 = lo2[b], h2 = hi2[b];
+      const T ux = smax(h1.x - l2.x, h2.x - l1.x);
+      const T uy = smax(h1.y - l2.y, h2.y - l1.y);
+      const T uz = smax(h1.z - l2.z, h2.z - l1.z);
+      const T ub2 = ux * ux + uy * uy + uz * uz;
+      key = ((unsigned long long)ord32_up(ub2) << 32) | (unsigned long long)gid;
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long o = __shfl_xor(key, off);
+    key = o < key ? o : key;
+  }
+  if (lane_id() == 0 && key != ~0ull) atomicMin(best, key);
+}
+
+__global__ void k_seed_list(const unsigned long long* best, uint64_t* list, unsigned* list_count, int w1, int w2) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const unsigned long long b = *best;
+  if (b == ~0ull) {
+    *list_count = 0;
+    return;
+  }
+  const TPIndex x = tp_decode((long long)(b & 0xffffffffull), tiles_per_unit(w1), tiles_per_unit(w2));
+  list[0] = tp_encode(x.slot, x.t1, x.t2);
+  *list_count = 1;
+}
+
+// Active tile pairs of the distance screen.  Lower bound on the centre d2 of
+// any cell pair of a tile pair: per axis the gap between the two centre boxes.
+// Correctly rounded subtraction, squaring and addition are monotone, so the
+// bound computed in T never exceeds any computed cell-pair d2; a tile pair is
+// skipped only when the bound is strictly above the threshold.
+template <class T>
+__global__ void k_dist_list(int nslots, const int* su1, const int* su2, int w1, int w2, const vec4_t<T>* lo1,
+                            const vec4_t<T>* hi1, const vec4_t<T>* lo2, const vec4_t<T>* hi2, const T* thresh,
+                            int cull, uint64_t* list, unsigned* list_count) {
+  const int tpu1 = tiles_per_unit(w1), tpu2 = tiles_per_unit(w2);
+  const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  bool keep = false;
+  TPIndex x{0, 0, 0};
+  if (gid < (long long)nslots * tpu1 * tpu2) {
+    x = tp_decode(gid, tpu1, tpu2);
+    keep = true;
+    if (cull) {
+      const int a = su1[x.slot] * tpu1 + x.t1, b = su2[x.slot] * tpu2 + x.t2;
+      const vec4_t<T> l1 = lo1[a], h1 = hi1[a], l2 = lo2[b], h2 = hi2[b];
+      const T gx = smax(T(0), smax(l1.x - h2.x, l2.x - h1.x));
+      const T gy = smax(T(0), smax(l1.y - h2.y, l2.y - h1.y));
+      const T gz = smax(T(0), smax(l1.z - h2.z, l2.z - h1.z));
+      const T lb2 = gx * gx + gy * gy + gz * gz;
+      keep = !(lb2 > *thresh);
+    }
+  }
+  const unsigned long long m = __ballot(keep);
+  if (m) {
+    unsigned base = 0;
+    const int leader = __ffsll((long long)m) - 1;
+    if (lane_id() == leader) base = atomicAdd(list_count, (unsigned)__popcll(m));
+    base = __shfl(base, leader);
+    if (keep) list[base + __popcll(m & ((1ull << lane_id()) - 1))] = tp_encode(x.slot, x.t1, x.t2);
+  }
+}
+
+// Distance filter: every cell pair (i, j) of every listed tile pair with
+// centre d2 <= threshold is appended to `out` (order irrelevant: records are
+// sorted by the full key afterwards).  One tile pair per workgroup iteration;
+// side-2 cells are staged in LDS and read as broadcasts, each lane owns one
+// side-1 cell.  d2 is center_distance_sq (distance_analysis.cu:202-207).
+template <class T>
+__global__ __launch_bounds__(256) void k_dist_filter(const uint64_t* list, const unsigned* list_count,
+                                                     const int* su1, const int* su2, int w1, int w2,
+                                                     const vec4_t<T>* C1, const vec4_t<T>* C2, const T* thresh,
+                                                     Cand* out, unsigned long long* out_count,
+                                                     unsigned long long cap, unsigned long long* pairs_done) {
+  __shared__ vec4_t<T> sB[kTileCells];
+  unsigned long long done = 0;
+  __shared__ int sJ[kTileCells];
+  const unsigned n = *list_count;
+  const T tv = *thresh;
+  const int lane = lane_id();
+  const int tpu1 = tiles_per_unit(w1);
+  (void)tpu1;
+  for (unsigned it = blockIdx.x; it < n; it += gridDim.x) {
+    const uint64_t e = list[it];
+    const int slot = (int)(e >> 32), t1 = (int)((e >> 16) & 0xffff), t2 = (int)(e & 0xffff);
+    const int u1 = su1[slot], u2 = su2[slot];
+    const TileGeo g1 = tile_geo(t1, w1), g2 = tile_geo(t2, w2);
+    const int nA = g1.ht * g1.wd, nB = g2.ht * g2.wd;
+    done += (unsigned long long)nA * nB;
+    __syncthreads();
+    if ((int)threadIdx.x < nB) {
+      const int j = (g2.row0 + threadIdx.x / g2.wd) * w2 + g2.col0 + threadIdx.x % g2.wd;
+      sB[threadIdx.x] = C2[(size_t)u2 * w2 * w2 + j];
+      sJ[threadIdx.x] = j;
+    }
+    const bool va = (int)threadIdx.x < nA;
+    int i = 0;
+    vec4_t<T> a;
+    a.x = a.y = a.z = a.w = T(0);
+    if (va) {
+      i = (g1.row0 + threadIdx.x / g1.wd) * w1 + g1.col0 + threadIdx.x % g1.wd;
+      a = C1[(size_t)u1 * w1 * w1 + i];
+    }
+    __syncthreads();
+    for (int jj = 0; jj < nB; ++jj) {
+      const vec4_t<T> b = sB[jj];
+      const T dx = a.x - b.x;
+      const T dy = a.y - b.y;
+      const T dz = a.z - b.z;
+      const T d2 = dx * dx + dy * dy + dz * dz;
+      const bool hit = va && (d2 <= tv);
+      const unsigned long long m = __ballot(hit);
+      if (m) {
+        unsigned long long base = 0;
+        const int leader = __ffsll((long long)m) - 1;
+        if (lane == leader) base = atomicAdd(out_count, (unsigned long long)__popcll(m));
+        base = __shfl(base, leader);
+        if (hit) {
+          const unsigned long long pos = base + __popcll(m & ((1ull << lane) - 1));
+          if (pos < cap) {
+            Cand c;
+            c.hi = ord_bits(d2);
+            c.lo = pack_idx((uint32_t)slot, (uint32_t)i, (uint32_t)sJ[jj]);
+            out[pos] = c;
+          }
+        }
+      }
+    }
+  }
+  if (pairs_done && threadIdx.x == 0 && done) atomicAdd(pairs_done, done);
+}
+
+template <class T>
+__global__ void k_set_thresh(T* thresh, const Cand* rec, int idx, int use_inf) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *thresh = use_inf ? infinity_v<T>() : from_bits<T>(rec[idx].hi);
+}
+
+// ---------------------------------------------------------------------------
+// Bitonic sort of candidate records (power-of-two length, >= kSortBlock).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool cand_less(const Cand& a, const Cand& b) {
+  return a.hi < b.hi || (a.hi == b.hi && a.lo < b.lo);
+}
+
+__global__ void k_fill_sentinel(Cand* a, unsigned long long from, unsigned long long n2) {
+  const unsigned long long i = from + (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n2) {
+    Cand c;
+    c.hi = ~0ull;
+    c.lo = ~0ull;
+    a[i] = c;
+  }
+}
+
+__global__ __launch_bounds__(1024) void k_bsort_local(Cand* a, unsigned long long merge_k) {
+  __shared__ Cand s[kSortBlock];
+  const unsigned long long base = (unsigned long long)blockIdx.x * kSortBlock;
+  for (int t = threadIdx.x; t < kSortBlock; t += blockDim.x) s[t] = a[base + t];
+  __syncthreads();
+  auto step = [&](unsigned long long k, int j) {
+    for (int t = threadIdx.x; t < kSortBlock / 2; t += blockDim.x) {
+      const int i = 2 * t - (t & (j - 1));
+      const int l = i + j;
+      const bool asc = ((base + i) & k) == 0;
+      const Cand x = s[i], y = s[l];
+      const bool swap = asc ? cand_less(y, x) : cand_less(x, y);
+      if (swap) {
+        s[i] = y;
+        s[l] = x;
+      }
+    }
+    __syncthreads();
+  };
+  if (merge_k == 0) {
+    for (int k = 2; k <= kSortBlock; k <<= 1)
+      for (int j = k >> 1; j > 0; j >>= 1) step((unsigned long long)k, j);
+  } else {
+    for (int j = kSortBlock >> 1; j > 0; j >>= 1) step(merge_k, j);
+  }
+  for (int t = threadIdx.x; t < kSortBlock; t += blockDim.x) a[base + t] = s[t];
+}
+
+__global__ void k_bsort_global(Cand* a, unsigned long long n2, unsigned long long k, unsigned long long j) {
+  const unsigned long long t = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n2 / 2) return;
+  const unsigned long long i = 2 * t - (t & (j - 1));
+  const unsigned long long l = i + j;
+  const bool asc = (i & k) == 0;
+  const Cand x = a[i], y = a[l];
+  const bool swap = asc ? cand_less(y, x) : cand_less(x, y);
+  if (swap) {
+    a[i] = y;
+    a[l] = x;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Distance refinement bookkeeping (minimum_distance, distance_analysis.cu:400-491,
+// JSI_CAD_PRESERVE_PAIR_RELATION=1): jobs are the sorted top-k pairs; each side
+// keeps one unit per distinct (slot, cell) in order of first appearance; slot k
+// of the next round is (unit of job k's side-1 endpoint, unit of its side-2
+// endpoint).  Child region: make_dist_refine_task (distance_analysis.cu:340-354).
+// ---------------------------------------------------------------------------
+template <class T>
+__device__ __forceinline__ vec4_t<T> child_region(const vec4_t<T>& pr, int pnuv, int bbox) {
+  const int w = pnuv - 1;
+  const T ustep = (pr.z - pr.x) / T(pnuv);
+  const T vstep = (pr.w - pr.y) / T(pnuv);
+  const int x = bbox % w, y = bbox / w;
+  vec4_t<T> c;
+  c.x = pr.x + ustep * T(x);
+  c.z = c.x + ustep;
+  c.y = pr.y + vstep * T(y);
+  c.w = c.y + vstep;
+  return c;
+}
+
+__device__ __forceinline__ int atomic_load_i32(const int* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+constexpr int kDedupMax = 8192;
+
+template <class T>
+__global__ __launch_bounds__(1024) void k_dist_dedup(const Cand* rec, int n, int cells1, int cells2, const int* su1,
+                                                     const int* su2, const vec4_t<T>* reg1, const vec4_t<T>* reg2,
+                                                     int pnuv1, int pnuv2, int* firstpos1, int* firstpos2,
+                                                     int* nsu1, int* nsu2, vec4_t<T>* nreg1, vec4_t<T>* nreg2,
+                                                     int* nunits) {
+  __shared__ int sUnit[kDedupMax];
+  __shared__ int sScan[1024];
+  const int per = (n + 1023) / 1024;  // <= 8 entries per thread, contiguous
+  for (int side = 0; side < 2; ++side) {
+    int* fp = side == 0 ? firstpos1 : firstpos2;
+    const int cells = side == 0 ? cells1 : cells2;
+    const int* su = side == 0 ? su1 : su2;
+    const vec4_t<T>* reg = side == 0 ? reg1 : reg2;
+    const int pnuv = side == 0 ? pnuv1 : pnuv2;
+    int* nsu = side == 0 ? nsu1 : nsu2;
+    vec4_t<T>* nreg = side == 0 ? nreg1 : nreg2;
+    auto key_of = [&](int e, int* slot, int* cell) {
+      const uint64_t lo = rec[e].lo;
+      *slot = (int)(lo >> 40);
+      *cell = side == 0 ? (int)((lo >> 20) & 0xfffff) : (int)(lo & 0xfffff);
+      return (long long)(*slot) * cells + *cell;
+    };
+    for (int e = threadIdx.x; e < n; e += 1024) {
+      int s, c;
+      atomicMin(&fp[key_of(e, &s, &c)], e);
+    }
+    __threadfence();
+    __syncthreads();
+    // first-appearance flags, prefix-counted over the contiguous chunk of each thread
+    int local = 0;
+    const int e0 = threadIdx.x * per;
+    for (int e = e0; e < min(n, e0 + per); ++e) {
+      int s, c;
+      local += atomic_load_i32(&fp[key_of(e, &s, &c)]) == e ? 1 : 0;
+    }
+    sScan[threadIdx.x] = local;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+      const int v = threadIdx.x >= off ? sScan[threadIdx.x - off] : 0;
+      __syncthreads();
+      sScan[threadIdx.x] += v;
+      __syncthreads();
+    }
+    int u = sScan[threadIdx.x] - local;  // exclusive prefix: first unit id of this chunk
+    for (int e = e0; e < min(n, e0 + per); ++e) {
+      int s, c;
+      if (atomic_load_i32(&fp[key_of(e, &s, &c)]) == e) {
+        sUnit[e] = u;
+        nreg[u] = child_region<T>(reg[su[s]], pnuv, c);
+        ++u;
+      }
+    }
+    if (threadIdx.x == 1023) nunits[side] = sScan[1023];
+    __syncthreads();
+    for (int e = threadIdx.x; e < n; e += 1024) {
+      int s, c;
+      nsu[e] = sUnit[atomic_load_i32(&fp[key_of(e, &s, &c)])];
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < n; e += 1024) {
+      int s, c;
+      fp[key_of(e, &s, &c)] = INT_MAX;
+    }
+    __threadfence();
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Intersection screen (AabbOverlap / intersect_collect_pairs,
+// intersection.cu:46-63,259-309).
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ void k_isect_list(int nslots, const int* su1, const int* su2, int w1, int w2, const vec4_t<T>* lo1,
+                             const vec4_t<T>* hi1, const vec4_t<T>* lo2, const vec4_t<T>* hi2, int cull,
+                             uint64_t* list, unsigned long long* list_count, unsigned long long list_cap) {
+  const int tpu1 = tiles_per_unit(w1), tpu2 = tiles_per_unit(w2);
+  const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  bool keep = false;
+  TPIndex x{0, 0, 0};
+  if (gid < (long long)nslots * tpu1 * tpu2) {
+    x = tp_decode(gid, tpu1, tpu2);
+    keep = true;
+    if (cull) {
+      const int a = su1[x.slot] * tpu1 + x.t1, b = su2[x.slot] * tpu2 + x.t2;
+      const vec4_t<T> l1 = lo1[a], h1 = hi1[a], l2 = lo2[b], h2 = hi2[b];
+      keep = l1.x <= h2.x && l2.x <= h1.x && l1.y <= h2.y && l2.y <= h1.y && l1.z <= h2.z && l2.z <= h1.z;
+    }
+  }
+  const unsigned long long m = __ballot(keep);
+  if (m) {
+    unsigned long long base = 0;
+    const int leader = __ffsll((long long)m) - 1;
+    if (lane_id() == leader) base = atomicAdd(list_count, (unsigned long long)__popcll(m));
+    base = __shfl(base, leader);
+    const unsigned long long pos = base + __popcll(m & ((1ull << lane_id()) - 1));
+    if (keep && pos < list_cap) list[pos] = tp_encode(x.slot, x.t1, x.t2);
+  }
+}
+
+// WRITE = false: per-row hit counts (row = slot*cells1 + i).  WRITE = true:
+// each hit goes to rowoff[row] + (arrival order within the row); rows are
+// sorted by j afterwards, which restores the (slot, i, j) order.
+template <class T, bool WRITE>
+__global__ __launch_bounds__(256) void k_isect_pass(const uint64_t* list, const unsigned long long* list_count,
+                                                    const int* su1, const int* su2, int w1, int w2,
+                                                    const vec4_t<T>* C1, const vec4_t<T>* R1,
+                                                    const vec4_t<T>* C2, const vec4_t<T>* R2, unsigned* rowcount,
+                                                    const unsigned long long* rowoff, unsigned* rowcursor,
+                                                    IPair* out) {
+  __shared__ vec4_t<T> sC[kTileCells];
+  __shared__ vec4_t<T> sR[kTileCells];
+  __shared__ int sJ[kTileCells];
+  const unsigned long long n = *list_count;
+  const int cells1 = w1 * w1;
+  for (unsigned long long it = blockIdx.x; it < n; it += gridDim.x) {
+    const uint64_t e = list[it];
+    const int slot = (int)(e >> 32), t1 = (int)((e >> 16) & 0xffff), t2 = (int)(e & 0xffff);
+    const int u1 = su1[slot], u2 = su2[slot];
+    const TileGeo g1 = tile_geo(t1, w1), g2 = tile_geo(t2, w2);
+    const int nA = g1.ht * g1.wd, nB = g2.ht * g2.wd;
+    __syncthreads();
+    if ((int)threadIdx.x < nB) {
+      const int j = (g2.row0 + threadIdx.x / g2.wd) * w2 + g2.col0 + threadIdx.x % g2.wd;
+      sC[threadIdx.x] = C2[(size_t)u2 * w2 * w2 + j];
+      sR[threadIdx.x] = R2[(size_t)u2 * w2 * w2 + j];
+      sJ[threadIdx.x] = j;
+    }
+    const bool va = (int)threadIdx.x < nA;
+    int i = 0;
+    vec4_t<T> c1, r1;
+    c1.x = c1.y = c1.z = c1.w = T(0);
+    r1 = c1;
+    if (va) {
+      i = (g1.row0 + threadIdx.x / g1.wd) * w1 + g1.col0 + threadIdx.x % g1.wd;
+      c1 = C1[(size_t)u1 * cells1 + i];
+      r1 = R1[(size_t)u1 * cells1 + i];
+    }
+    __syncthreads();
+    if (!va) continue;
+    const size_t row = (size_t)slot * cells1 + i;
+    unsigned cnt = 0;
+    for (int jj = 0; jj < nB; ++jj) {
+      const vec4_t<T> c2 = sC[jj], r2 = sR[jj];
+      const bool hit = fabs(c1.x - c2.x) < (r1.x + r2.x) && fabs(c1.y - c2.y) < (r1.y + r2.y) &&
+                       fabs(c1.z - c2.z) < (r1.z + r2.z);
+      if (hit) {
+        if (WRITE) {
+          const unsigned k = atomicAdd(&rowcursor[row], 1u);
+          IPair pr;
+          pr.slot = slot;
+          pr.i = i;
+          pr.j = sJ[jj];
+          out[rowoff[row] + k] = pr;
+        } else {
+          ++cnt;
+        }
+      }
+    }
+    if (!WRITE && cnt) atomicAdd(&rowcount[row], cnt);
+  }
+}
+
+// Insertion sort of each row's pairs by j (rows are short).
+__global__ void k_row_sort(const unsigned long long* rowoff, unsigned long long nrows, IPair* out) {
+  const unsigned long long r = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrows) return;
+  const unsigned long long b = rowoff[r], e = rowoff[r + 1];
+  for (unsigned long long x = b + 1; x < e; ++x) {
+    const IPair v = out[x];
+    unsigned long long y = x;
+    while (y > b && out[y - 1].j > v.j) {
+      out[y] = out[y - 1];
+      --y;
+    }
+    out[y] = v;
+  }
+}
+
+// Exclusive scan (u32 or u64 input -> u64 output with the total at out[n]).
+constexpr int kScanBlock = 4096;  // 1024 threads x 4
+
+template <class IN>
+__global__ __launch_bounds__(1024) void k_scan_reduce(const IN* in, unsigned long long n,
+                                                      unsigned long long* sums) {
+  __shared__ unsigned long long s[1024];
+  const unsigned long long base = (unsigned long long)blockIdx.x * kScanBlock;
+  unsigned long long v = 0;
+  for (int e = 0; e < 4; ++e) {
+    const unsigned long long i = base + threadIdx.x * 4 + e;
+    if (i < n) v += (unsigned long long)in[i];
+  }
+  s[threadIdx.x] = v;
+  __syncthreads();
+  for (int off = 512; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) s[threadIdx.x] += s[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) sums[blockIdx.x] = s[0];
+}
+
+template <class IN>
+__global__ __launch_bounds__(1024) void k_scan_apply(const IN* in, unsigned long long n,
+                                                     const unsigned long long* block_off, unsigned long long* out) {
+  __shared__ unsigned long long s[1024];
+  const unsigned long long base = (unsigned long long)blockIdx.x * kScanBlock;
+  unsigned long long v[4];
+  unsigned long long t = 0;
+  for (int e = 0; e < 4; ++e) {
+    const unsigned long long i = base + threadIdx.x * 4 + e;
+    v[e] = i < n ? (unsigned long long)in[i] : 0ull;
+    t += v[e];
+  }
+  s[threadIdx.x] = t;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const unsigned long long x = threadIdx.x >= (unsigned)off ? s[threadIdx.x - off] : 0ull;
+    __syncthreads();
+    s[threadIdx.x] += x;
+    __syncthreads();
+  }
+  unsigned long long run = block_off[blockIdx.x] + s[threadIdx.x] - t;
+  for (int e = 0; e < 4; ++e) {
+    const unsigned long long i = base + threadIdx.x * 4 + e;
+    if (i < n) out[i] = run;
+    run += v[e];
+    if (i == n - 1) out[n] = run;
+  }
+}
+
+__global__ void k_isect_flags1(const IPair* pairs, unsigned long long n, unsigned* flag1) {
+  const unsigned long long p = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const IPair a = pairs[p];
+  bool f = true;
+  if (p > 0) {
+    const IPair b = pairs[p - 1];
+    f = a.slot != b.slot || a.i != b.i;
+  }
+  flag1[p] = f ? 1u : 0u;
+}
+
+__global__ void k_isect_first2(const IPair* pairs, unsigned long long n, int cells2, int* firstpos2) {
+  const unsigned long long p = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const IPair a = pairs[p];
+  atomicMin(&firstpos2[(size_t)a.slot * cells2 + a.j], (int)p);
+}
+
+__global__ void k_isect_flags2(const IPair* pairs, unsigned long long n, int cells2, const int* firstpos2,
+                               unsigned* flag2) {
+  const unsigned long long p = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const IPair a = pairs[p];
+  flag2[p] = firstpos2[(size_t)a.slot * cells2 + a.j] == (int)p ? 1u : 0u;
+}
+
+template <class T>
+__global__ void k_isect_units(const IPair* pairs, unsigned long long n, int cells2, const int* su1, const int* su2,
+                              const vec4_t<T>* reg1, const vec4_t<T>* reg2, int pnuv, const unsigned* flag1,
+                              const unsigned* flag2, const unsigned long long* idx1,
+                              const unsigned long long* idx2, const int* firstpos2, int* nsu1, int* nsu2,
+                              vec4_t<T>* nreg1, vec4_t<T>* nreg2) {
+  const unsigned long long p = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const IPair a = pairs[p];
+  nsu1[p] = (int)(idx1[p] + flag1[p] - 1);
+  if (flag1[p]) nreg1[idx1[p]] = child_region<T>(reg1[su1[a.slot]], pnuv, a.i);
+  const int first = firstpos2[(size_t)a.slot * cells2 + a.j];
+  nsu2[p] = (int)idx2[first];
+  if (flag2[p]) nreg2[idx2[p]] = child_region<T>(reg2[su2[a.slot]], pnuv, a.j);
+}
+
+__global__ void k_isect_reset2(const IPair* pairs, unsigned long long n, int cells2, int* firstpos2) {
+  const unsigned long long p = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const IPair a = pairs[p];
+  firstpos2[(size_t)a.slot * cells2 + a.j] = INT_MAX;
+}
+
+template <class T>
+__global__ void k_promote(const float* src, T* dst, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = (T)src[i];
+}
+
+__global__ void k_fill_i32(int* p, int v, unsigned long long n) {
+  const unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+__global__ void k_fill_u32(unsigned* p, unsigned v, unsigned long long n) {
+  const unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+template <class T>
+hipError_t launch_promote(const float* src, T* dst, int n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_promote<T>, dim3(blocks_for(n, 256)), dim3(256), 0, s, src, dst, n);
+  return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_basis(const T* knots, int nk, int p, int der, const vec4_t<T>* regions, int nunits, int nuv,
+                        int dir, T* table, int* spans, hipStream_t s) {
+  const long long n = (long long)nunits * nuv;
+  if (n <= 0) return hipSuccess;
+  const int steps_given = dir >> 1;
+  dir &= 1;
+  const dim3 g(blocks_for(n, 128)), b(128);
+  if (p <= 3)
+    hipLaunchKernelGGL((k_basis<T, 3>), g, b, 0, s, knots, nk, p, der, regions, nunits, nuv, dir, steps_given,
+                       table, spans);
+  else if (p <= 7)
+    hipLaunchKernelGGL((k_basis<T, 7>), g, b, 0, s, knots, nk, p, der, regions, nunits, nuv, dir, steps_given,
+                       table, spans);
+  else
+    hipLaunchKernelGGL((k_basis<T, kMaxDeg - 1>), g, b, 0, s, knots, nk, p, der, regions, nunits, nuv, dir,
+                       steps_given, table, spans);
+  return hipGetLastError();
+}
+
+template <class T, int MAXD, bool RAT>
+static void points_mode(const SurfDev<T>& sd, const T* tU, const int* sU, const T* tV, const int* sV, int nunits,
+                        int nuv, int mode, T* out, unsigned long long* kmax, hipStream_t s) {
+  const int bpu = (nuv * nuv + 255) / 256;
+  const dim3 g((unsigned)((long long)bpu * nunits)), b(256);
+  if (mode == kEvalFull18)
+    hipLaunchKernelGGL((k_points<T, MAXD, RAT, kEvalFull18>), g, b, 0, s, sd, tU, sU, tV, sV, nunits, nuv, out,
+                       kmax);
+  else if (mode == kEvalPoint)
+    hipLaunchKernelGGL((k_points<T, MAXD, RAT, kEvalPoint>), g, b, 0, s, sd, tU, sU, tV, sV, nunits, nuv, out,
+                       kmax);
+  else
+    hipLaunchKernelGGL((k_points<T, MAXD, RAT, kEvalPointK>), g, b, 0, s, sd, tU, sU, tV, sV, nunits, nuv, out,
+                       kmax);
+}
+
+template <class T, int MAXD>
+static void points_rat(const SurfDev<T>& sd, const T* tU, const int* sU, const T* tV, const int* sV, int nunits,
+                       int nuv, int mode, T* out, unsigned long long* kmax, hipStream_t s) {
+  if (sd.rational)
+    points_mode<T, MAXD, true>(sd, tU, sU, tV, sV, nunits, nuv, mode, out, kmax, s);
+  else
+    points_mode<T, MAXD, false>(sd, tU, sU, tV, sV, nunits, nuv, mode, out, kmax, s);
+}
+
+template <class T>
+hipError_t launch_points(const SurfDev<T>& sd, const T* tabU, const int* spanU, const T* tabV, const int* spanV,
+                         int nunits, int nuv, int mode, T* out, unsigned long long* kmax, hipStream_t s) {
+  if (nunits <= 0 || nuv <= 0) return hipSuccess;
+  const int q = sd.degv;
+  if (q <= 3)
+    points_rat<T, 3>(sd, tabU, spanU, tabV, spanV, nunits, nuv, mode, out, kmax, s);
+  else if (q <= 7)
+    points_rat<T, 7>(sd, tabU, spanU, tabV, spanV, nunits, nuv, mode, out, kmax, s);
+  else
+    points_rat<T, kMaxDeg - 1>(sd, tabU, spanU, tabV, spanV, nunits, nuv, mode, out, kmax, s);
+  return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_aabb(const T* pts, int nunits, int nuv, int isect, int use_k, const unsigned long long* kmax,
+                       const vec4_t<T>* regions, T init_u, T init_v, vec4_t<T>* C, vec4_t<T>* R, hipStream_t s) {
+  const long long n = (long long)nunits * (nuv - 1) * (nuv - 1);
+  if (n <= 0) return hipSuccess;
+  const dim3 g(blocks_for(n, 256)), b(256);
+  if (isect)
+    hipLaunchKernelGGL((k_aabb<T, true>), g, b, 0, s, pts, nunits, nuv, use_k, kmax, regions, init_u, init_v, C, R);
+  else
+    hipLaunchKernelGGL((k_aabb<T, false>), g, b, 0, s, pts, nunits, nuv, use_k, kmax, regions, init_u, init_v, C,
+                       R);
+  return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_tilebox(const vec4_t<T>* C, const vec4_t<T>* R, int nunits, int w, int isect, vec4_t<T>* lo,
+                          vec4_t<T>* hi, hipStream_t s) {
+  const int tw = (w + kTile - 1) / kTile;
+  const long long waves = (long long)nunits * tw * tw;
+  if (waves <= 0) return hipSuccess;
+  const dim3 g(blocks_for(waves * 64, 256)), b(256);
+  if (isect)
+    hipLaunchKernelGGL((k_tilebox<T, true>), g, b, 0, s, C, R, nunits, w, lo, hi);
+  else
+    hipLaunchKernelGGL((k_tilebox<T, false>), g, b, 0, s, C, R, nunits, w, lo, hi);
+  return hipGetLastError();
+}
+
+static long long n_tile_pairs(int nslots, int w1, int w2) {
+  const long long t1 = (w1 + kTile - 1) / kTile, t2 = (w2 + kTile - 1) / kTile;
+  return (long long)nslots * t1 * t1 * t2 * t2;
+}
+
+template <class T>
+hipError_t launch_dist_seed(int nslots, const int* su1, const int* su2, int w1, int w2, const vec4_t<T>* lo1,
+                            const vec4_t<T>* hi1, const vec4_t<T>* lo2, const vec4_t<T>* hi2, int k,
+                            unsigned long long* best, hipStream_t s) {
+  const long long n = n_tile_pairs(nslots, w1, w2);
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_dist_seed<T>, dim3(blocks_for(n, 256)), dim3(256), 0, s, nslots, su1, su2, w1, w2, lo1, hi1,
+                     lo2, hi2, k, best);
+  return hipGetLastError();
+}
+
+hipError_t launch_seed_list(const unsigned long long* best, uint64_t* list, unsigned* list_count, int w1, int w2,
+                            hipStream_t s) {
+  hipLaunchKernelGGL(k_seed_list, dim3(1), dim3(64), 0, s, best, list, list_count, w1, w2);
+  return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_dist_list(int nslots, const int* su1, const int* su2, int w1, int w2, const vec4_t<T>* lo1,
+                            const vec4_t<T>* hi1, const vec4_t<T>* lo2, const vec4_t<T>* hi2, const T* thresh,
+                            int cull, uint64_t* list, unsigned* list_count, hipStream_t s) {
+  const long long n = n_tile_pairs(nslots, w1, w2);
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_dist_list<T>, dim3(blocks_for(n, 256)), dim3(256), 0, s, nslots, su1, su2, w1, w2, lo1, hi1,
+                     lo2, hi2, thresh, cull, list, list_count);
+  return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_dist_filter(const uint64_t* list, const unsigned* list_count, unsigned list_cap, const int* su1,
+                              const int* su2, int w1, int w2, const vec4_t<T>* C1, const vec4_t<T>* C2,
+                              const T* thresh, Cand* out, unsigned long long* out_count, unsigned long long cap,
+                              int grid, hipStream_t s, unsigned long long* pairs_done) {
+  if (list_cap == 0) return hipSuccess;
+  const unsigned g = (unsigned)std::min<long long>(grid, (long long)list_cap);
+  hipLaunchKernelGGL(k_dist_filter<T>, dim3(g), dim3(256), 0, s, list, list_count, su1, su2, w1, w2, C1, C2, thresh,
+                     out, out_count, cap, pairs_done);
+  return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_set_thresh(T* thresh, const Cand* rec, int idx, int use_inf, hipStream_t s) {
+  hipLaunchKernelGGL(k_set_thresh<T>, dim3(1), dim3(64), 0, s, thresh, rec, idx, use_inf);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_sentinel(Cand* a, unsigned long long from, unsigned long long n2, hipStream_t s) {
+  if (n2 <= from) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_sentinel, dim3(blocks_for(n2 - from, 256)), dim3(256), 0, s, a, from, n2);
+  return hipGetLastError();
+}
+
+hipError_t sort_cands(Cand* a, unsigned long long n2, hipStream_t s) {
+  if (n2 < (unsigned long long)kSortBlock || (n2 & (n2 - 1)) != 0) return hipErrorInvalidValue;
+  const unsigned nb = (unsigned)(n2 / kSortBlock);
+  hipLaunchKernelGGL(k_bsort_local, dim3(nb), dim3(1024), 0, s, a, 0ull);
+  for (unsigned long long k = 2ull * kSortBlock; k <= n2; k <<= 1) {
+    for (unsigned long long j = k >> 1; j >= (unsigned long long)kSortBlock; j >>= 1)
+      hipLaunchKernelGGL(k_bsort_global, dim3(blocks_for(n2 / 2, 256)), dim3(256), 0, s, a, n2, k, j);
+    hipLaunchKernelGGL(k_bsort_local, dim3(nb), dim3(1024), 0, s, a, k);
+  }
+  return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_dist_dedup(const Cand* rec, int n, int cells1, int cells2, const int* su1, const int* su2,
+                             const vec4_t<T>* reg1, const vec4_t<T>* reg2, int parent_nuv1, int parent_nuv2,
+                             int* firstpos1, int* firstpos2, int* nsu1, int* nsu2, vec4_t<T>* nreg1,
+                             vec4_t<T>* nreg2, int* nunits, hipStream_t s) {
+  if (n <= 0 || n > kDedupMax) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_dist_dedup<T>, dim3(1), dim3(1024), 0, s, rec, n, cells1, cells2, su1, su2, reg1, reg2,
+                     parent_nuv1, parent_nuv2, firstpos1, firstpos2, nsu1, nsu2, nreg1, nreg2, nunits);
+  return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_isect_list(int nslots, const int* su1, const int* su2, int w1, int w2, const vec4_t<T>* lo1,
+                             const vec4_t<T>* hi1, const vec4_t<T>* lo2, const vec4_t<T>* hi2, int cull,
+                             uint64_t* list, unsigned long long* list_count, unsigned long long list_cap,
+                             hipStream_t s) {
+  const long long n = n_tile_pairs(nslots, w1, w2);
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_isect_list<T>, dim3(blocks_for(n, 256)), dim3(256), 0, s, nslots, su1, su2, w1, w2, lo1,
+                     hi1, lo2, hi2, cull, list, list_count, list_cap);
+  return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_isect_pass(int write, const uint64_t* list, const unsigned long long* list_count, const int* su1,
+                             const int* su2, int w1, int w2, const vec4_t<T>* C1, const vec4_t<T>* R1,
+                             const vec4_t<T>* C2, const vec4_t<T>* R2, unsigned* rowcount,
+                             const unsigned long long* rowoff, unsigned* rowcursor, IPair* out, int grid,
+                             hipStream_t s) {
+  if (write)
+    hipLaunchKernelGGL((k_isect_pass<T, true>), dim3(grid), dim3(256), 0, s, list, list_count, su1, su2, w1, w2, C1,
+                       R1, C2, R2, rowcount, rowoff, rowcursor, out);
+  else
+    hipLaunchKernelGGL((k_isect_pass<T, false>), dim3(grid), dim3(256), 0, s, list, list_count, su1, su2, w1, w2,
+                       C1, R1, C2, R2, rowcount, rowoff, rowcursor, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_row_sort(const unsigned long long* rowoff, unsigned long long nrows, IPair* out, hipStream_t s) {
+  if (nrows == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_row_sort, dim3(blocks_for(nrows, 256)), dim3(256), 0, s, rowoff, nrows, out);
+  return hipGetLastError();
+}
+
+size_t scan_tmp_bytes(unsigned long long n) {
+  size_t total = 0;
+  unsigned long long m = n;
+  while (true) {
+    const unsigned long long nb = (m + kScanBlock - 1) / kScanBlock;
+    total += (nb + 1) * 2 * sizeof(unsigned long long);
+    if (nb <= 1) break;
+    m = nb;
+  }
+  return total + 256;
+}
+
+template <class IN>
+static hipError_t scan_rec(const IN* in, unsigned long long* out, unsigned long long n, unsigned long long* tmp,
+                           hipStream_t s) {
+  const unsigned long long nb = (n + kScanBlock - 1) / kScanBlock;
+  unsigned long long* sums = tmp;
+  unsigned long long* offs = tmp + nb + 1;
+  hipLaunchKernelGGL(k_scan_reduce<IN>, dim3((unsigned)nb), dim3(1024), 0, s, in, n, sums);
+  if (nb == 1) {
+    hipError_t e = hipMemsetAsync(offs, 0, sizeof(unsigned long long), s);
+    if (e != hipSuccess) return e;
+  } else {
+    hipError_t e = scan_rec<unsigned long long>(sums, offs, nb, tmp + 2 * (nb + 1), s);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(k_scan_apply<IN>, dim3((unsigned)nb), dim3(1024), 0, s, in, n, offs, out);
+  return hipGetLastError();
+}
+
+hipError_t scan_u32_to_u64(const unsigned* in, unsigned long long* out, unsigned long long n, void* tmp,
+                           size_t tmp_bytes, hipStream_t s) {
+  if (n == 0) return hipMemsetAsync(out, 0, sizeof(unsigned long long), s);
+  if (tmp_bytes < scan_tmp_bytes(n)) return hipErrorInvalidValue;
+  return scan_rec<unsigned>(in, out, n, static_cast<unsigned long long*>(tmp), s);
+}
+
+hipError_t launch_isect_dedup_flags(const IPair* pairs, unsigned long long n, int cells2, int* firstpos2,
+                                    unsigned* flag1, unsigned* flag2, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  const dim3 g(blocks_for(n, 256)), b(256);
+  hipLaunchKernelGGL(k_isect_flags1, g, b, 0, s, pairs, n, flag1);
+  hipLaunchKernelGGL(k_isect_first2, g, b, 0, s, pairs, n, cells2, firstpos2);
+  hipLaunchKernelGGL(k_isect_flags2, g, b, 0, s, pairs, n, cells2, firstpos2, flag2);
+  return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_isect_units(const IPair* pairs, unsigned long long n, int cells2, const int* su1, const int* su2,
+                              const vec4_t<T>* reg1, const vec4_t<T>* reg2, int parent_nuv, int parent_w,
+                              const unsigned* flag1, const unsigned* flag2, const unsigned long long* idx1,
+                              const unsigned long long* idx2, int* firstpos2, int* nsu1, int* nsu2,
+                              vec4_t<T>* nreg1, vec4_t<T>* nreg2, hipStream_t s) {
+  (void)parent_w;
+  if (n == 0) return hipSuccess;
+  const dim3 g(blocks_for(n, 256)), b(256);
+  hipLaunchKernelGGL(k_isect_units<T>, g, b, 0, s, pairs, n, cells2, su1, su2, reg1, reg2, parent_nuv, flag1, flag2,
+                     idx1, idx2, firstpos2, nsu1, nsu2, nreg1, nreg2);
+  hipLaunchKernelGGL(k_isect_reset2, g, b, 0, s, pairs, n, cells2, firstpos2);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_i32(int* p, int value, unsigned long long n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(n, 256)), dim3(256), 0, s, p, value, n);
+  return hipGetLastError();
+}
+hipError_t launch_fill_u32(unsigned* p, unsigned value, unsigned long long n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(n, 256)), dim3(256), 0, s, p, value, n);
+  return hipGetLastError();
+}
+
+// ---- explicit instantiations ----------------------------------------------
+#define JSI_INST(T)                                                                                                 \
+  template hipError_t launch_promote<T>(const float*, T*, int, hipStream_t);                                       \
+  template hipError_t launch_basis<T>(const T*, int, int, int, const vec4_t<T>*, int, int, int, T*, int*,          \
+                                      hipStream_t);                                                                \
+  template hipError_t launch_points<T>(const SurfDev<T>&, const T*, const int*, const T*, const int*, int, int,    \
+                                       int, T*, unsigned long long*, hipStream_t);                                 \
+  template hipError_t launch_aabb<T>(const T*, int, int, int, int, const unsigned long long*, const vec4_t<T>*, T, \
+                                     T, vec4_t<T>*, vec4_t<T>*, hipStream_t);                                      \
+  template hipError_t launch_tilebox<T>(const vec4_t<T>*, const vec4_t<T>*, int, int, int, vec4_t<T>*,             \
+                                        vec4_t<T>*, hipStream_t);                                                  \
+  template hipError_t launch_dist_seed<T>(int, const int*, const int*, int, int, const vec4_t<T>*,                 \
+                                          const vec4_t<T>*, const vec4_t<T>*, const vec4_t<T>*, int,               \
+                                          unsigned long long*, hipStream_t);                                       \
+  template hipError_t launch_dist_list<T>(int, const int*, const int*, int, int, const vec4_t<T>*,                 \
+                                          const vec4_t<T>*, const vec4_t<T>*, const vec4_t<T>*, const T*, int,     \
+                                          uint64_t*, unsigned*, hipStream_t);                                      \
+  template hipError_t launch_dist_filter<T>(const uint64_t*, const unsigned*, unsigned, const int*, const int*,    \
+                                            int, int, const vec4_t<T>*, const vec4_t<T>*, const T*, Cand*,         \
+                                            unsigned long long*, unsigned long long, int, hipStream_t,             \
+                                            unsigned long long*);            \
+  template hipError_t launch_set_thresh<T>(T*, const Cand*, int, int, hipStream_t);                                \
+  template hipError_t launch_dist_dedup<T>(const Cand*, int, int, int, const int*, const int*, const vec4_t<T>*,   \
+                                           const vec4_t<T>*, int, int, int*, int*, int*, int*, vec4_t<T>*,         \
+                                           vec4_t<T>*, int*, hipStream_t);                                         \
+  template hipError_t launch_isect_list<T>(int, const int*, const int*, int, int, const vec4_t<T>*,                \
+                                           const vec4_t<T>*, const vec4_t<T>*, const vec4_t<T>*, int, uint64_t*,   \
+                                           unsigned long long*, unsigned long long, hipStream_t);                  \
+  template hipError_t launch_isect_pass<T>(int, const uint64_t*, const unsigned long long*, const int*, const int*, \
+                                           int, int, const vec4_t<T>*, const vec4_t<T>*, const vec4_t<T>*,         \
+                                           const vec4_t<T>*, unsigned*, const unsigned long long*, unsigned*,      \
+                                           IPair*, int, hipStream_t);                                              \
+  template hipError_t launch_isect_units<T>(const IPair*, unsigned long long, int, const int*, const int*,         \
+                                            const vec4_t<T>*, const vec4_t<T>*, int, int, const unsigned*,         \
+                                            const unsigned*, const unsigned long long*, const unsigned long long*, \
+                                            int*, int*, int*, vec4_t<T>*, vec4_t<T>*, hipStream_t);
+
+JSI_INST(float)
+JSI_INST(double)
+#undef JSI_INST
+
+}  // namespace jsi

@@ -1,0 +1,933 @@
+// jsi_runtime.hip - host runtime of libjsi_mi355x.so: contexts, device
+// buffers, resident surfaces, the distance / intersection round loops and the
+// C ABI of include/jsi_mi355x.h.  All compute runs in jsi_kernels.hip; the
+// host only sequences kernels and reads back counts and results.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/jsi_mi355x.h"
+#include "jsi_internal.hpp"
+#include "jsi_runtime.hpp"
+
+using namespace jsi;
+
+namespace {
+
+thread_local char g_err[512] = "";
+
+void set_err(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+double now_ms() {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+#define HCK(expr)                                                                          \
+  do {                                                                                     \
+    hipError_t e_ = (expr);                                                                \
+    if (e_ != hipSuccess) {                                                                \
+      set_err("%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
+      return JSI_ERR_DEVICE;                                                               \
+    }                                                                                      \
+  } while (0)
+
+#define RCK(expr)             \
+  do {                        \
+    int r_ = (expr);          \
+    if (r_ != JSI_OK) return r_; \
+  } while (0)
+
+unsigned long long next_pow2(unsigned long long x) {
+  unsigned long long p = 1;
+  while (p < x) p <<= 1;
+  return p;
+}
+
+constexpr unsigned long long kCandCap = 1ull << 22;  // candidate records kept per screening pass
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Context
+// ---------------------------------------------------------------------------
+int jsi_ctx::buf_raw(const char* name, size_t bytes, void** out, int fill_int) {
+  DevBuf& b = bufs[name];
+  if (b.bytes < bytes) {
+    if (b.p) {
+      HCK(hipStreamSynchronize(stream));
+      HCK(hipFree(b.p));
+      used -= b.bytes;
+      b.p = nullptr;
+      b.bytes = 0;
+    }
+    size_t want = std::max(bytes + bytes / 2, (size_t)4096);
+    if (used + want > cap) want = std::max(bytes, (size_t)256);
+    if (used + want > cap) {
+      set_err("device arena cap exceeded: buffer %s needs %zu bytes, %zu of %zu in use", name, want, used, cap);
+      return JSI_ERR_NOMEM;
+    }
+    hipError_t e = hipMalloc(&b.p, want);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      set_err("hipMalloc(%zu) for %s failed: %s", want, name, hipGetErrorString(e));
+      b.p = nullptr;
+      return JSI_ERR_NOMEM;
+    }
+    b.bytes = want;
+    used += want;
+    if (fill_int != 0) {
+      hipError_t f = launch_fill_i32(static_cast<int*>(b.p), fill_int, want / sizeof(int), stream);
+      if (f != hipSuccess) {
+        set_err("fill %s: %s", name, hipGetErrorString(f));
+        return JSI_ERR_DEVICE;
+      }
+    }
+  }
+  *out = b.p;
+  return JSI_OK;
+}
+
+int jsi_ctx::d2h(void* dst, const void* src, size_t bytes) {
+  if (bytes == 0) return JSI_OK;
+  HCK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream));
+  HCK(hipStreamSynchronize(stream));
+  return JSI_OK;
+}
+
+int jsi_ctx::h2d(void* dst, const void* src, size_t bytes) {
+  if (bytes == 0) return JSI_OK;
+  HCK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream));
+  HCK(hipStreamSynchronize(stream));
+  return JSI_OK;
+}
+
+template <class T>
+int jsi_ctx::surf_dev(int h, SurfDev<T>* out) {
+  if (h < 0 || h >= (int)surf.size() || !surf[h].live) {
+    set_err("invalid surface handle %d", h);
+    return JSI_ERR_INVALID;
+  }
+  SurfaceRec& s = surf[h];
+  const T* base;
+  if (std::is_same<T, float>::value) {
+    base = reinterpret_cast<const T*>(s.d_data);
+  } else {
+    if (!s.d_data64) {
+      HCK(hipMalloc(&s.d_data64, sizeof(double) * (size_t)s.ndata));
+      used += sizeof(double) * (size_t)s.ndata;
+      HCK(launch_promote<double>(s.d_data, s.d_data64, s.ndata, stream));
+    }
+    base = reinterpret_cast<const T*>(s.d_data64);
+  }
+  const int* iv = s.ivec;
+  SurfDev<T> d;
+  d.degu = iv[1];
+  d.ncpu = iv[2];
+  d.nku = iv[3];
+  d.degv = iv[4];
+  d.ncpv = iv[5];
+  d.nkv = iv[6];
+  d.rational = (iv[0] & 2) != 0;
+  d.zero = (((iv[0] & 1) == 0 && (iv[0] & 12) != 0) || d.degu >= 25 || d.degv >= 25) ? 1 : 0;
+  d.U = base;
+  d.V = base + d.nku;
+  d.CP = d.V + d.nkv;
+  d.W = d.rational ? d.CP + 3 * d.ncpu * d.ncpv : nullptr;
+  *out = d;
+  return JSI_OK;
+}
+
+// Evaluates `nunits` regions (device array `reg`) of surface h on an nuv x nuv
+// grid.  mode: kEvalFull18 / kEvalPoint / kEvalPointK (see jsi_internal.hpp).
+template <class T>
+int jsi_ctx::eval_units(int h, const vec4_t<T>* reg, int nunits, int nuv, int mode, int steps_given, T* out,
+                         unsigned long long* kmax, const char* tag) {
+  SurfDev<T> sd;
+  RCK(surf_dev<T>(h, &sd));
+  if (sd.zero) {
+    // periodic / unsupported degree: all-zero output, as EvaluateOneUV (evaluation.cu:271-272)
+  }
+  const int p = std::max(sd.degu, 1), q = std::max(sd.degv, 1);
+  std::string nu = std::string(tag) + ".tabU", nv = std::string(tag) + ".tabV";
+  std::string su = std::string(tag) + ".spanU", sv = std::string(tag) + ".spanV";
+  T *tU, *tV;
+  int *sU, *sV;
+  const size_t nn = (size_t)nunits * nuv;
+  RCK(buf(nu.c_str(), nn * 3 * (p + 1), &tU));
+  RCK(buf(nv.c_str(), nn * 3 * (q + 1), &tV));
+  RCK(buf(su.c_str(), nn, &sU));
+  RCK(buf(sv.c_str(), nn, &sV));
+  const int der_u = std::min(2, sd.degu), der_v = std::min(2, sd.degv);
+  if (!sd.zero) {
+    HCK(launch_basis<T>(sd.U, sd.nku, sd.degu, der_u, reg, nunits, nuv, 0 | (steps_given << 1), tU, sU, stream));
+    HCK(launch_basis<T>(sd.V, sd.nkv, sd.degv, der_v, reg, nunits, nuv, 1 | (steps_given << 1), tV, sV, stream));
+  }
+  if (mode == kEvalPointK) HCK(launch_fill_u32(reinterpret_cast<unsigned*>(kmax), 0u, (size_t)nunits * 6, stream));
+  HCK(launch_points<T>(sd, tU, sU, tV, sV, nunits, nuv, mode, out, kmax, stream));
+  return JSI_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Distance screen: exact top-k under (d2, slot, i, j)
+// ---------------------------------------------------------------------------
+template <class T>
+int jsi_ctx::dist_topk(int nslots, const int* su1, const int* su2, int w1, int nunits1, const vec4_t<T>* C1, int w2,
+                        int nunits2, const vec4_t<T>* C2, int k, int cull, Cand** rec_out, long long* n_out,
+                        long long* kk_out) {
+  using V = vec4_t<T>;
+  const int tpu1 = ((w1 + kTile - 1) / kTile) * ((w1 + kTile - 1) / kTile);
+  const int tpu2 = ((w2 + kTile - 1) / kTile) * ((w2 + kTile - 1) / kTile);
+  V *lo1, *hi1, *lo2, *hi2;
+  RCK(buf("tb.lo1", (size_t)nunits1 * tpu1, &lo1));
+  RCK(buf("tb.hi1", (size_t)nunits1 * tpu1, &hi1));
+  RCK(buf("tb.lo2", (size_t)nunits2 * tpu2, &lo2));
+  RCK(buf("tb.hi2", (size_t)nunits2 * tpu2, &hi2));
+  HCK(launch_tilebox<T>(C1, nullptr, nunits1, w1, 0, lo1, hi1, stream));
+  HCK(launch_tilebox<T>(C2, nullptr, nunits2, w2, 0, lo2, hi2, stream));
+  const unsigned long long ntp = (unsigned long long)nslots * tpu1 * tpu2;
+  if (ntp > 0xffffffffull) {
+    set_err("too many tile pairs (%llu)", ntp);
+    return JSI_ERR_CAPACITY;
+  }
+  uint64_t* list;
+  unsigned* list_count;
+  unsigned long long* best;
+  unsigned long long* out_count;
+  T* thresh;
+  Cand* cand;
+  RCK(buf("d.list", ntp, &list));
+  RCK(buf("d.small", 64, &list_count));
+  best = reinterpret_cast<unsigned long long*>(list_count + 2);
+  out_count = reinterpret_cast<unsigned long long*>(list_count + 4);
+  thresh = reinterpret_cast<T*>(list_count + 6);
+  RCK(buf("d.cand", kCandCap, &cand));
+  // seed: exact k-th d2 of the best single tile pair
+  HCK(launch_fill_u32(reinterpret_cast<unsigned*>(best), 0xffffffffu, 2, stream));
+  HCK(launch_dist_seed<T>(nslots, su1, su2, w1, w2, lo1, hi1, lo2, hi2, k, best, stream));
+  HCK(launch_seed_list(best, list, list_count, w1, w2, stream));
+  HCK(launch_set_thresh<T>(thresh, cand, 0, 1, stream));
+  HCK(launch_fill_u32(reinterpret_cast<unsigned*>(out_count), 0u, 2, stream));
+  HCK(launch_dist_filter<T>(list, list_count, 1, su1, su2, w1, w2, C1, C2, thresh, cand, out_count, kCandCap, 1,
+                            stream));
+  unsigned long long ns = 0;
+  RCK(d2h(&ns, out_count, sizeof(ns)));
+  if (ns > 0) {
+    if (ns < (unsigned long long)k || ns > kCandCap) {
+      set_err("seed tile pair produced %llu candidates (k=%d)", ns, k);
+      return JSI_ERR_CAPACITY;
+    }
+    const unsigned long long n2 = std::max(next_pow2(ns), (unsigned long long)kSortBlock);
+    HCK(launch_fill_sentinel(cand, ns, n2, stream));
+    HCK(sort_cands(cand, n2, stream));
+    HCK(launch_set_thresh<T>(thresh, cand, k - 1, 0, stream));
+  }
+  unsigned long long n = 0;
+  for (int attempt = 0;; ++attempt) {
+    HCK(launch_fill_u32(list_count, 0u, 1, stream));
+    HCK(launch_dist_list<T>(nslots, su1, su2, w1, w2, lo1, hi1, lo2, hi2, thresh, cull, list, list_count, stream));
+    HCK(launch_fill_u32(reinterpret_cast<unsigned*>(out_count), 0u, 2, stream));
+    unsigned long long* pairs_done = prof ? reinterpret_cast<unsigned long long*>(list_count + 8) : nullptr;
+    if (prof) {
+      HCK(launch_fill_u32(reinterpret_cast<unsigned*>(pairs_done), 0u, 2, stream));
+      HCK(hipEventRecord(ev[0], stream));
+    }
+    HCK(launch_dist_filter<T>(list, list_count, (unsigned)ntp, su1, su2, w1, w2, C1, C2, thresh, cand, out_count,
+                              kCandCap, grid, stream, pairs_done));
+    if (prof) HCK(hipEventRecord(ev[1], stream));
+    RCK(d2h(&n, out_count, sizeof(n)));
+    if (prof) {
+      float ms = 0.f;
+      unsigned long long pd = 0;
+      HCK(hipEventElapsedTime(&ms, ev[0], ev[1]));
+      RCK(d2h(&pd, pairs_done, sizeof(pd)));
+      KernelStat& st = stats[std::is_same<T, float>::value ? "dist_filter_f32" : "dist_filter_f64"];
+      st.launches += 1;
+      st.ms += ms;
+      st.units += (double)pd;
+    }
+    if (n <= kCandCap) break;
+    if (attempt > 8) {
+      set_err("distance screen could not bound its candidate set (%llu > %llu)", n, kCandCap);
+      return JSI_ERR_CAPACITY;
+    }
+    // too many candidates: the k-th smallest of the first kCandCap (all real
+    // pairs) is a tighter upper bound of the k-th d2
+    HCK(sort_cands(cand, kCandCap, stream));
+    HCK(launch_set_thresh<T>(thresh, cand, k - 1, 0, stream));
+  }
+  const unsigned long long n2 = std::max(next_pow2(std::max(n, 1ull)), (unsigned long long)kSortBlock);
+  HCK(launch_fill_sentinel(cand, n, n2, stream));
+  HCK(sort_cands(cand, n2, stream));
+  *rec_out = cand;
+  *n_out = (long long)n;
+  *kk_out = (long long)std::min<unsigned long long>(n, (unsigned long long)k);
+  return JSI_OK;
+}
+
+static uint64_t fnv_bytes(uint64_t h, const void* p, size_t n) {
+  const unsigned char* b = static_cast<const unsigned char*>(p);
+  for (size_t i = 0; i < n; ++i) {
+    h ^= b[i];
+    h *= 1099511628211ull;
+  }
+  return h;
+}
+static constexpr uint64_t kFnvSeed = 1469598103934665603ull;
+
+template <class T>
+static double d2_from_bits(uint64_t b) {
+  if (std::is_same<T, float>::value) {
+    uint32_t u = (uint32_t)b;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return (double)f;
+  }
+  double d;
+  std::memcpy(&d, &b, 8);
+  return d;
+}
+
+template <class T>
+int jsi_ctx::min_distance(int h1, const jsi_region& r1, int h2, const jsi_region& r2, const jsi_params& P,
+                           jsi_result* res, jsi_pair* out, int cap_pairs) {
+  using V = vec4_t<T>;
+  const double t0 = now_ms();
+  // round-1 regions
+  V g1, g2;
+  g1.x = T(r1.ustart);
+  g1.y = T(r1.vstart);
+  g1.z = T(r1.ustop);
+  g1.w = T(r1.vstop);
+  g2.x = T(r2.ustart);
+  g2.y = T(r2.vstart);
+  g2.z = T(r2.ustop);
+  g2.w = T(r2.vstop);
+  int cur = 0;  // ping-pong index of the current round's unit buffers
+  auto nm = [](const char* base, int side, int pp) {
+    return std::string(base) + (side == 1 ? "1" : "2") + (pp ? "b" : "a");
+  };
+  V *reg1, *reg2;
+  RCK(buf(nm("reg", 1, cur).c_str(), 1, &reg1));
+  RCK(buf(nm("reg", 2, cur).c_str(), 1, &reg2));
+  RCK(h2d(reg1, &g1, sizeof(V)));
+  RCK(h2d(reg2, &g2, sizeof(V)));
+  int nuv1 = r1.nuv, nuv2 = r2.nuv;
+  int nunits1 = 1, nunits2 = 1;
+  T* pts;
+  V *C1, *C2;
+  double te = now_ms();
+  {
+    RCK(buf("pts", (size_t)3 * std::max(nuv1 * nuv1, nuv2 * nuv2), &pts));
+    RCK(buf(nm("C", 1, cur).c_str(), (size_t)(nuv1 - 1) * (nuv1 - 1), &C1));
+    RCK(buf(nm("C", 2, cur).c_str(), (size_t)(nuv2 - 1) * (nuv2 - 1), &C2));
+    RCK(eval_units<T>(h1, reg1, 1, nuv1, kEvalPoint, 0, pts, nullptr, "e1"));
+    HCK(launch_aabb<T>(pts, 1, nuv1, 0, 0, nullptr, reg1, T(1), T(1), C1, nullptr, stream));
+    RCK(eval_units<T>(h2, reg2, 1, nuv2, kEvalPoint, 0, pts, nullptr, "e2"));
+    HCK(launch_aabb<T>(pts, 1, nuv2, 0, 0, nullptr, reg2, T(1), T(1), C2, nullptr, stream));
+  }
+  int* su1;
+  int* su2;
+  RCK(buf(nm("su", 1, cur).c_str(), 1, &su1));
+  RCK(buf(nm("su", 2, cur).c_str(), 1, &su2));
+  HCK(launch_fill_i32(su1, 0, 1, stream));
+  HCK(launch_fill_i32(su2, 0, 1, stream));
+  HCK(hipStreamSynchronize(stream));
+  res->ms_eval += now_ms() - te;
+  int nslots = 1;
+  const int nref = P.dist_refine_nuv;
+  std::vector<Cand> host_rec;
+  Cand* rec = nullptr;
+  long long kk = 0;
+  for (int round = 1;; ++round) {
+    const int w1 = nuv1 - 1, w2 = nuv2 - 1;
+    const double ts = now_ms();
+    long long n = 0;
+    RCK(dist_topk<T>(nslots, su1, su2, w1, nunits1, C1, w2, nunits2, C2, P.k, P.cull, &rec, &n, &kk));
+    jsi_round_info& ri = res->round[round - 1];
+    ri.npairs = kk;
+    ri.slots = nslots;
+    ri.nuv = nuv1;
+    Cand last{0, 0};
+    if (kk > 0) RCK(d2h(&last, rec + (kk - 1), sizeof(Cand)));
+    res->ms_screen += now_ms() - ts;
+    ri.kth_d2 = kk > 0 ? d2_from_bits<T>(last.hi) : 0.0;
+    if (P.trace && kk > 0) {
+      host_rec.resize(kk);
+      RCK(d2h(host_rec.data(), rec, sizeof(Cand) * kk));
+      uint64_t hsh = kFnvSeed;
+      for (const Cand& c : host_rec) {
+        const double dd = d2_from_bits<T>(c.hi);
+        const int32_t s = (int32_t)(c.lo >> 40), i = (int32_t)((c.lo >> 20) & 0xfffff), j = (int32_t)(c.lo & 0xfffff);
+        hsh = fnv_bytes(hsh, &dd, 8);
+        hsh = fnv_bytes(hsh, &s, 4);
+        hsh = fnv_bytes(hsh, &i, 4);
+        hsh = fnv_bytes(hsh, &j, 4);
+      }
+      ri.hash = hsh;
+    }
+    res->rounds = round;
+    res->npairs = kk;
+    if (kk > 0) {
+      const T kd = T(d2_from_bits<T>(last.hi));
+      res->distance = (double)std::sqrt(kd);
+    } else {
+      res->distance = 0.0;
+    }
+    if (kk == 0 || round == P.dist_max_rounds || round >= JSI_MAX_ROUNDS) break;
+    // refinement units for the next round
+    te = now_ms();
+    const int nxt = cur ^ 1;
+    int *fp1, *fp2, *nsu1, *nsu2, *nunits_dev;
+    V *nreg1, *nreg2;
+    RCK(buf_raw("fp1", sizeof(int) * (size_t)nslots * w1 * w1, reinterpret_cast<void**>(&fp1), INT_MAX));
+    RCK(buf_raw("fp2", sizeof(int) * (size_t)nslots * w2 * w2, reinterpret_cast<void**>(&fp2), INT_MAX));
+    RCK(buf(nm("su", 1, nxt).c_str(), (size_t)kk, &nsu1));
+    RCK(buf(nm("su", 2, nxt).c_str(), (size_t)kk, &nsu2));
+    RCK(buf(nm("reg", 1, nxt).c_str(), (size_t)kk, &nreg1));
+    RCK(buf(nm("reg", 2, nxt).c_str(), (size_t)kk, &nreg2));
+    RCK(buf("nunits", 4, &nunits_dev));
+    HCK(launch_dist_dedup<T>(rec, (int)kk, w1 * w1, w2 * w2, su1, su2, reg1, reg2, nuv1, nuv2, fp1, fp2, nsu1, nsu2,
+                             nreg1, nreg2, nunits_dev, stream));
+    int nu[2];
+    RCK(d2h(nu, nunits_dev, sizeof(nu)));
+    ri.units1 = nu[0];
+    ri.units2 = nu[1];
+    V *nC1, *nC2;
+    const int wr = nref - 1;
+    RCK(buf("pts", (size_t)3 * std::max(nu[0], nu[1]) * nref * nref, &pts));
+    RCK(buf(nm("C", 1, nxt).c_str(), (size_t)nu[0] * wr * wr, &nC1));
+    RCK(buf(nm("C", 2, nxt).c_str(), (size_t)nu[1] * wr * wr, &nC2));
+    RCK(eval_units<T>(h1, nreg1, nu[0], nref, kEvalPoint, 0, pts, nullptr, "r1"));
+    HCK(launch_aabb<T>(pts, nu[0], nref, 0, 0, nullptr, nreg1, T(1), T(1), nC1, nullptr, stream));
+    RCK(eval_units<T>(h2, nreg2, nu[1], nref, kEvalPoint, 0, pts, nullptr, "r2"));
+    HCK(launch_aabb<T>(pts, nu[1], nref, 0, 0, nullptr, nreg2, T(1), T(1), nC2, nullptr, stream));
+    HCK(hipStreamSynchronize(stream));
+    res->ms_eval += now_ms() - te;
+    cur = nxt;
+    su1 = nsu1;
+    su2 = nsu2;
+    reg1 = nreg1;
+    reg2 = nreg2;
+    C1 = nC1;
+    C2 = nC2;
+    nunits1 = nu[0];
+    nunits2 = nu[1];
+    nslots = (int)kk;
+    nuv1 = nuv2 = nref;
+  }
+  if (out && cap_pairs > 0 && kk > 0) {
+    const long long m = std::min<long long>(kk, cap_pairs);
+    host_rec.resize(m);
+    RCK(d2h(host_rec.data(), rec, sizeof(Cand) * m));
+    for (long long e = 0; e < m; ++e) {
+      const Cand& c = host_rec[e];
+      out[e].d2 = d2_from_bits<T>(c.hi);
+      out[e].slot = (int32_t)(c.lo >> 40);
+      out[e].i = (int32_t)((c.lo >> 20) & 0xfffff);
+      out[e].j = (int32_t)(c.lo & 0xfffff);
+      out[e].pad_ = 0;
+    }
+  }
+  res->ms_total = now_ms() - t0;
+  return JSI_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Intersection
+// ---------------------------------------------------------------------------
+template <class T>
+int jsi_ctx::isect_units(int h, const vec4_t<T>* reg, int nunits, int nuv, int use_k, T init_u, T init_v,
+                          vec4_t<T>* C, vec4_t<T>* R, const char* tag) {
+  T* pts;
+  unsigned long long* kmax;
+  RCK(buf("ipts", (size_t)3 * nunits * nuv * nuv, &pts));
+  RCK(buf("kmax", (size_t)3 * nunits, &kmax));
+  RCK(eval_units<T>(h, reg, nunits, nuv, use_k ? kEvalPointK : kEvalPoint, 0, pts, kmax, tag));
+  HCK(launch_aabb<T>(pts, nunits, nuv, 1, use_k, kmax, reg, init_u, init_v, C, R, stream));
+  return JSI_OK;
+}
+
+template <class T>
+int jsi_ctx::intersect(int h1, const jsi_region& r1, int h2, const jsi_region& r2, const jsi_params& P,
+                        jsi_result* res, jsi_pair* out, int cap_pairs) {
+  using V = vec4_t<T>;
+  const double t0 = now_ms();
+  auto smax_h = [](T a, T b) { return (a < b) ? b : a; };
+  const T tiny = T(1.0e-12f);
+  T iu[3] = {T(1), T(1), T(1)}, iv[3] = {T(1), T(1), T(1)};
+  iu[1] = smax_h(std::fabs(T(r1.ustop) - T(r1.ustart)), tiny);
+  iv[1] = smax_h(std::fabs(T(r1.vstop) - T(r1.vstart)), tiny);
+  iu[2] = smax_h(std::fabs(T(r2.ustop) - T(r2.ustart)), tiny);
+  iv[2] = smax_h(std::fabs(T(r2.vstop) - T(r2.vstart)), tiny);
+  const int use_k = P.use_compute_k != 0;
+  auto nm = [](const char* base, int side, int pp) {
+    return std::string("i") + base + (side == 1 ? "1" : "2") + (pp ? "b" : "a");
+  };
+  int cur = 0;
+  V g1, g2;
+  g1.x = T(r1.ustart);
+  g1.y = T(r1.vstart);
+  g1.z = T(r1.ustop);
+  g1.w = T(r1.vstop);
+  g2.x = T(r2.ustart);
+  g2.y = T(r2.vstart);
+  g2.z = T(r2.ustop);
+  g2.w = T(r2.vstop);
+  V *reg1, *reg2, *C1, *R1, *C2, *R2;
+  int *su1, *su2;
+  RCK(buf(nm("reg", 1, cur).c_str(), 1, &reg1));
+  RCK(buf(nm("reg", 2, cur).c_str(), 1, &reg2));
+  RCK(h2d(reg1, &g1, sizeof(V)));
+  RCK(h2d(reg2, &g2, sizeof(V)));
+  int nuv1 = r1.nuv, nuv2 = r2.nuv;
+  double te = now_ms();
+  RCK(buf(nm("C", 1, cur).c_str(), (size_t)(nuv1 - 1) * (nuv1 - 1), &C1));
+  RCK(buf(nm("R", 1, cur).c_str(), (size_t)(nuv1 - 1) * (nuv1 - 1), &R1));
+  RCK(buf(nm("C", 2, cur).c_str(), (size_t)(nuv2 - 1) * (nuv2 - 1), &C2));
+  RCK(buf(nm("R", 2, cur).c_str(), (size_t)(nuv2 - 1) * (nuv2 - 1), &R2));
+  RCK(isect_units<T>(h1, reg1, 1, nuv1, use_k, iu[1], iv[1], C1, R1, "i1"));
+  RCK(isect_units<T>(h2, reg2, 1, nuv2, use_k, iu[2], iv[2], C2, R2, "i2"));
+  RCK(buf(nm("su", 1, cur).c_str(), 1, &su1));
+  RCK(buf(nm("su", 2, cur).c_str(), 1, &su2));
+  HCK(launch_fill_i32(su1, 0, 1, stream));
+  HCK(launch_fill_i32(su2, 0, 1, stream));
+  HCK(hipStreamSynchronize(stream));
+  res->ms_eval += now_ms() - te;
+  int nunits1 = 1, nunits2 = 1;
+  long long nslots = 1;
+  int g_nuv = P.isect_refine_nuv;
+  IPair* pairs = nullptr;
+  unsigned long long npairs = 0;
+  std::vector<IPair> hp;
+  for (int round = 1;; ++round) {
+    const double ts = now_ms();
+    const int w1 = nuv1 - 1, w2 = nuv2 - 1;
+    const int tpu1 = ((w1 + kTile - 1) / kTile) * ((w1 + kTile - 1) / kTile);
+    const int tpu2 = ((w2 + kTile - 1) / kTile) * ((w2 + kTile - 1) / kTile);
+    V *lo1, *hi1, *lo2, *hi2;
+    RCK(buf("tb.lo1", (size_t)nunits1 * tpu1, &lo1));
+    RCK(buf("tb.hi1", (size_t)nunits1 * tpu1, &hi1));
+    RCK(buf("tb.lo2", (size_t)nunits2 * tpu2, &lo2));
+    RCK(buf("tb.hi2", (size_t)nunits2 * tpu2, &hi2));
+    HCK(launch_tilebox<T>(C1, R1, nunits1, w1, 1, lo1, hi1, stream));
+    HCK(launch_tilebox<T>(C2, R2, nunits2, w2, 1, lo2, hi2, stream));
+    const unsigned long long ntp = (unsigned long long)nslots * tpu1 * tpu2;
+    if (nslots > INT_MAX || ntp > (1ull << 40)) {
+      set_err("intersection round %d: %lld slots exceed the supported range", round, nslots);
+      return JSI_ERR_CAPACITY;
+    }
+    uint64_t* list;
+    unsigned long long* small;
+    RCK(buf("i.list", ntp, &list));
+    RCK(buf("i.small", 8, &small));
+    HCK(launch_fill_u32(reinterpret_cast<unsigned*>(small), 0u, 2, stream));
+    HCK(launch_isect_list<T>((int)nslots, su1, su2, w1, w2, lo1, hi1, lo2, hi2, P.cull, list, small, ntp, stream));
+    const unsigned long long nrows = (unsigned long long)nslots * w1 * w1;
+    unsigned *rowcount, *rowcursor;
+    unsigned long long* rowoff;
+    void* scantmp;
+    RCK(buf("i.rowcount", nrows, &rowcount));
+    RCK(buf("i.rowcursor", nrows, &rowcursor));
+    RCK(buf("i.rowoff", nrows + 1, &rowoff));
+    const size_t stb = scan_tmp_bytes(nrows);
+    RCK(buf_raw("i.scantmp", stb, &scantmp, 0));
+    HCK(launch_fill_u32(rowcount, 0u, nrows, stream));
+    HCK(launch_isect_pass<T>(0, list, small, su1, su2, w1, w2, C1, R1, C2, R2, rowcount, rowoff, rowcursor, nullptr,
+                             grid, stream));
+    HCK(scan_u32_to_u64(rowcount, rowoff, nrows, scantmp, stb, stream));
+    RCK(d2h(&npairs, rowoff + nrows, sizeof(npairs)));
+    RCK(buf(nm("pairs", 1, cur).c_str(), std::max(npairs, 1ull), &pairs));
+    if (npairs > 0) {
+      HCK(launch_fill_u32(rowcursor, 0u, nrows, stream));
+      HCK(launch_isect_pass<T>(1, list, small, su1, su2, w1, w2, C1, R1, C2, R2, rowcount, rowoff, rowcursor, pairs,
+                               grid, stream));
+      HCK(launch_row_sort(rowoff, nrows, pairs, stream));
+    }
+    HCK(hipStreamSynchronize(stream));
+    res->ms_screen += now_ms() - ts;
+    jsi_round_info& ri = res->round[round - 1];
+    ri.npairs = (int64_t)npairs;
+    ri.slots = nslots;
+    ri.nuv = nuv1;
+    if (P.trace) {
+      hp.resize(npairs);
+      RCK(d2h(hp.data(), pairs, sizeof(IPair) * npairs));
+      uint64_t hsh = kFnvSeed;
+      for (const IPair& p : hp) {
+        hsh = fnv_bytes(hsh, &p.slot, 4);
+        hsh = fnv_bytes(hsh, &p.i, 4);
+        hsh = fnv_bytes(hsh, &p.j, 4);
+      }
+      ri.hash = hsh;
+    }
+    res->rounds = round;
+    res->npairs = (int64_t)npairs;
+    if (round == P.isect_max_rounds || round >= JSI_MAX_ROUNDS) break;
+    if (npairs == 0) break;
+    if ((long long)npairs > P.isect_halve_above && g_nuv > P.isect_min_nuv) g_nuv /= 2;
+    if (npairs > (unsigned long long)INT_MAX) {
+      set_err("intersection round %d: %llu pairs exceed the supported range", round, npairs);
+      return JSI_ERR_CAPACITY;
+    }
+    te = now_ms();
+    const int nxt = cur ^ 1;
+    unsigned *flag1, *flag2;
+    unsigned long long *idx1, *idx2;
+    int* fp2;
+    RCK(buf("i.flag1", npairs, &flag1));
+    RCK(buf("i.flag2", npairs, &flag2));
+    RCK(buf("i.idx1", npairs + 1, &idx1));
+    RCK(buf("i.idx2", npairs + 1, &idx2));
+    RCK(buf_raw("i.fp2", sizeof(int) * (size_t)nslots * w2 * w2, reinterpret_cast<void**>(&fp2), INT_MAX));
+    HCK(launch_isect_dedup_flags(pairs, npairs, w2 * w2, fp2, flag1, flag2, stream));
+    const size_t stb2 = scan_tmp_bytes(npairs);
+    RCK(buf_raw("i.scantmp2", stb2, &scantmp, 0));
+    HCK(scan_u32_to_u64(flag1, idx1, npairs, scantmp, stb2, stream));
+    HCK(scan_u32_to_u64(flag2, idx2, npairs, scantmp, stb2, stream));
+    unsigned long long nu1 = 0, nu2 = 0;
+    RCK(d2h(&nu1, idx1 + npairs, sizeof(nu1)));
+    RCK(d2h(&nu2, idx2 + npairs, sizeof(nu2)));
+    ri.units1 = (int64_t)nu1;
+    ri.units2 = (int64_t)nu2;
+    int *nsu1, *nsu2;
+    V *nreg1, *nreg2;
+    RCK(buf(nm("su", 1, nxt).c_str(), npairs, &nsu1));
+    RCK(buf(nm("su", 2, nxt).c_str(), npairs, &nsu2));
+    RCK(buf(nm("reg", 1, nxt).c_str(), nu1, &nreg1));
+    RCK(buf(nm("reg", 2, nxt).c_str(), nu2, &nreg2));
+    HCK(launch_isect_units<T>(pairs, npairs, w2 * w2, su1, su2, reg1, reg2, nuv1, w1, flag1, flag2, idx1, idx2, fp2,
+                              nsu1, nsu2, nreg1, nreg2, stream));
+    const int wr = g_nuv - 1;
+    V *nC1, *nR1, *nC2, *nR2;
+    RCK(buf(nm("C", 1, nxt).c_str(), (size_t)nu1 * wr * wr, &nC1));
+    RCK(buf(nm("R", 1, nxt).c_str(), (size_t)nu1 * wr * wr, &nR1));
+    RCK(buf(nm("C", 2, nxt).c_str(), (size_t)nu2 * wr * wr, &nC2));
+    RCK(buf(nm("R", 2, nxt).c_str(), (size_t)nu2 * wr * wr, &nR2));
+    RCK(isect_units<T>(h1, nreg1, (int)nu1, g_nuv, use_k, iu[1], iv[1], nC1, nR1, "i1"));
+    RCK(isect_units<T>(h2, nreg2, (int)nu2, g_nuv, use_k, iu[2], iv[2], nC2, nR2, "i2"));
+    HCK(hipStreamSynchronize(stream));
+    res->ms_eval += now_ms() - te;
+    cur = nxt;
+    su1 = nsu1;
+    su2 = nsu2;
+    reg1 = nreg1;
+    reg2 = nreg2;
+    C1 = nC1;
+    R1 = nR1;
+    C2 = nC2;
+    R2 = nR2;
+    nunits1 = (int)nu1;
+    nunits2 = (int)nu2;
+    nslots = (long long)npairs;
+    nuv1 = nuv2 = g_nuv;
+  }
+  res->isect_refine_nuv = g_nuv;
+  if (out && cap_pairs > 0 && npairs > 0) {
+    const unsigned long long m = std::min<unsigned long long>(npairs, (unsigned long long)cap_pairs);
+    hp.resize(m);
+    RCK(d2h(hp.data(), pairs, sizeof(IPair) * m));
+    for (unsigned long long e = 0; e < m; ++e) out[e] = jsi_pair{0.0, hp[e].slot, hp[e].i, hp[e].j, 0};
+  }
+  res->ms_total = now_ms() - t0;
+  return JSI_OK;
+}
+
+template <class T>
+int jsi_ctx::eval_grid(int h, T ustep, T vstep, T ustart, T vstart, int nuv, T* host_out) {
+  using V = vec4_t<T>;
+  V g;
+  g.x = ustart;
+  g.y = vstart;
+  g.z = ustep;
+  g.w = vstep;
+  V* reg;
+  T* out;
+  RCK(buf("eg.reg", 1, &reg));
+  RCK(h2d(reg, &g, sizeof(V)));
+  RCK(buf("eg.out", (size_t)nuv * nuv * 18, &out));
+  RCK(eval_units<T>(h, reg, 1, nuv, kEvalFull18, 1, out, nullptr, "eg"));
+  RCK(d2h(host_out, out, sizeof(T) * (size_t)nuv * nuv * 18));
+  return JSI_OK;
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+namespace {
+
+int validate_surface(const int* iv, int ndata) {
+  if (!iv) return JSI_ERR_INVALID;
+  if (iv[1] < 1 || iv[4] < 1 || iv[2] <= iv[1] || iv[5] <= iv[4] || iv[3] != iv[2] + iv[1] + 1 ||
+      iv[6] != iv[5] + iv[4] + 1) {
+    set_err("inconsistent ivec {%d,%d,%d,%d,%d,%d,%d}", iv[0], iv[1], iv[2], iv[3], iv[4], iv[5], iv[6]);
+    return JSI_ERR_INVALID;
+  }
+  const long long need = (long long)iv[3] + iv[6] + 3ll * iv[2] * iv[5] + ((iv[0] & 2) ? (long long)iv[2] * iv[5] : 0);
+  if (ndata < need) {
+    set_err("ndata %d < %lld floats required by ivec", ndata, need);
+    return JSI_ERR_INVALID;
+  }
+  return JSI_OK;
+}
+
+// Knot vectors must be non-decreasing (FindSpan's binary search assumes it).
+int validate_knots(const int* iv, const float* host_data) {
+  for (int d = 0; d < 2; ++d) {
+    const float* k = host_data + (d == 0 ? 0 : iv[3]);
+    const int n = d == 0 ? iv[3] : iv[6];
+    for (int i = 1; i < n; ++i)
+      if (!(k[i - 1] <= k[i])) {
+        set_err("knot vector %c is not non-decreasing at %d", d == 0 ? 'u' : 'v', i);
+        return JSI_ERR_INVALID;
+      }
+  }
+  return JSI_OK;
+}
+
+int validate_query(jsi_ctx* ctx, int h1, const jsi_region* r1, int h2, const jsi_region* r2, const jsi_params* P) {
+  if (!ctx || !r1 || !r2 || !P) {
+    set_err("null argument");
+    return JSI_ERR_INVALID;
+  }
+  for (const jsi_region* r : {r1, r2})
+    if (r->nuv < 2 || r->nuv > JSI_MAX_NUV) {
+      set_err("nuv %d outside [2, %d]", r->nuv, JSI_MAX_NUV);
+      return JSI_ERR_UNSUPPORTED;
+    }
+  if (P->precision < JSI_PREC_EXACT32 || P->precision > JSI_PREC_FP32SCREEN) {
+    set_err("unknown precision %d", P->precision);
+    return JSI_ERR_INVALID;
+  }
+  for (int h : {h1, h2})
+    if (h < 0 || h >= (int)ctx->surf.size() || !ctx->surf[h].live) {
+      set_err("invalid surface handle %d", h);
+      return JSI_ERR_INVALID;
+    }
+  return JSI_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void jsi_params_default(jsi_params* p) {
+  if (!p) return;
+  std::memset(p, 0, sizeof(*p));
+  p->k = 8;
+  p->dist_refine_nuv = 16;
+  p->dist_max_rounds = 4;
+  p->isect_max_rounds = 6;
+  p->isect_refine_nuv = 16;
+  p->isect_halve_above = 512;
+  p->isect_min_nuv = 4;
+  p->use_compute_k = 1;
+  p->precision = JSI_PREC_EXACT32;
+  p->cull = 1;
+  p->trace = 0;
+}
+
+const char* jsi_status_string(int s) {
+  switch (s) {
+    case JSI_OK: return "ok";
+    case JSI_ERR_INVALID: return "invalid argument";
+    case JSI_ERR_NOMEM: return "out of device memory";
+    case JSI_ERR_DEVICE: return "device error";
+    case JSI_ERR_CAPACITY: return "capacity exceeded";
+    case JSI_ERR_UNSUPPORTED: return "unsupported";
+    case JSI_ERR_BUSY: return "busy";
+    default: return "unknown status";
+  }
+}
+
+const char* jsi_last_error(void) { return g_err; }
+
+int jsi_ctx_create(int device, size_t hbm_cap_bytes, jsi_ctx** out) {
+  if (!out) return JSI_ERR_INVALID;
+  *out = nullptr;
+  int ndev = 0;
+  HCK(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) {
+    set_err("device %d not present (%d devices)", device, ndev);
+    return JSI_ERR_INVALID;
+  }
+  HCK(hipSetDevice(device));
+  jsi_ctx* c = new jsi_ctx();
+  c->device = device;
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete c;
+    set_err("hipStreamCreate: %s", hipGetErrorString(e));
+    return JSI_ERR_DEVICE;
+  }
+  size_t free_b = 0, total_b = 0;
+  (void)hipMemGetInfo(&free_b, &total_b);
+  c->cap = hbm_cap_bytes ? hbm_cap_bytes : (free_b / 4) * 3;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->ncu = prop.multiProcessorCount;
+  if (c->ncu <= 0) c->ncu = 256;
+  c->grid = c->ncu * 8;
+  *out = c;
+  return JSI_OK;
+}
+
+int jsi_ctx_destroy(jsi_ctx* c) {
+  if (!c) return JSI_ERR_INVALID;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  for (auto& kv : c->bufs)
+    if (kv.second.p) (void)hipFree(kv.second.p);
+  for (auto& s : c->surf) {
+    if (s.owned && s.d_data) (void)hipFree(s.d_data);
+    if (s.d_data64) (void)hipFree(s.d_data64);
+  }
+  for (hipEvent_t e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+  return JSI_OK;
+}
+
+int jsi_ctx_profile(jsi_ctx* c, int enable) {
+  if (!c) return JSI_ERR_INVALID;
+  HCK(hipSetDevice(c->device));
+  if (enable && !c->ev[0]) {
+    HCK(hipEventCreate(&c->ev[0]));
+    HCK(hipEventCreate(&c->ev[1]));
+  }
+  c->prof = enable ? 1 : 0;
+  c->stats.clear();
+  return JSI_OK;
+}
+
+int jsi_ctx_kernel_stats(jsi_ctx* c, const char* name, int64_t* launches, double* ms, double* units) {
+  if (!c || !name) return JSI_ERR_INVALID;
+  auto it = c->stats.find(name);
+  const KernelStat st = it == c->stats.end() ? KernelStat() : it->second;
+  if (launches) *launches = st.launches;
+  if (ms) *ms = st.ms;
+  if (units) *units = st.units;
+  return JSI_OK;
+}
+
+int jsi_ctx_sync(jsi_ctx* c) {
+  if (!c) return JSI_ERR_INVALID;
+  HCK(hipStreamSynchronize(c->stream));
+  return JSI_OK;
+}
+
+int jsi_surface_upload(jsi_ctx* c, const int ivec[7], const float* data, int ndata, int* handle) {
+  if (!c || !ivec || !data || !handle) return JSI_ERR_INVALID;
+  RCK(validate_surface(ivec, ndata));
+  RCK(validate_knots(ivec, data));
+  HCK(hipSetDevice(c->device));
+  SurfaceRec s;
+  s.live = true;
+  std::memcpy(s.ivec, ivec, sizeof(s.ivec));
+  s.ndata = ndata;
+  s.owned = true;
+  const size_t bytes = sizeof(float) * (size_t)ndata;
+  if (c->used + bytes > c->cap) {
+    set_err("HBM cap reached uploading a surface");
+    return JSI_ERR_NOMEM;
+  }
+  HCK(hipMalloc(&s.d_data, bytes));
+  c->used += bytes;
+  HCK(hipMemcpy(s.d_data, data, bytes, hipMemcpyHostToDevice));
+  c->surf.push_back(s);
+  *handle = (int)c->surf.size() - 1;
+  return JSI_OK;
+}
+
+int jsi_surface_register_device(jsi_ctx* c, const int* d_ivec, const float* d_data, int ndata, int* handle) {
+  if (!c || !d_ivec || !d_data || !handle) return JSI_ERR_INVALID;
+  HCK(hipSetDevice(c->device));
+  SurfaceRec s;
+  HCK(hipMemcpy(s.ivec, d_ivec, sizeof(s.ivec), hipMemcpyDeviceToHost));
+  RCK(validate_surface(s.ivec, ndata));
+  {
+    std::vector<float> knots((size_t)s.ivec[3] + s.ivec[6]);
+    HCK(hipMemcpy(knots.data(), d_data, sizeof(float) * knots.size(), hipMemcpyDeviceToHost));
+    RCK(validate_knots(s.ivec, knots.data()));
+  }
+  s.live = true;
+  s.ndata = ndata;
+  s.owned = false;
+  s.d_data = const_cast<float*>(d_data);
+  c->surf.push_back(s);
+  *handle = (int)c->surf.size() - 1;
+  return JSI_OK;
+}
+
+int jsi_surface_release(jsi_ctx* c, int h) {
+  if (!c || h < 0 || h >= (int)c->surf.size() || !c->surf[h].live) return JSI_ERR_INVALID;
+  SurfaceRec& s = c->surf[h];
+  HCK(hipStreamSynchronize(c->stream));
+  if (s.owned && s.d_data) {
+    HCK(hipFree(s.d_data));
+    c->used -= sizeof(float) * (size_t)s.ndata;
+  }
+  if (s.d_data64) {
+    HCK(hipFree(s.d_data64));
+    c->used -= sizeof(double) * (size_t)s.ndata;
+  }
+  s = SurfaceRec();
+  return JSI_OK;
+}
+
+int jsi_eval_grid_f32(jsi_ctx* c, int h, float ustep, float vstep, float ustart, float vstart, int nuv, float* out) {
+  if (!c || !out || nuv < 1 || nuv > 16384) return JSI_ERR_INVALID;
+  HCK(hipSetDevice(c->device));
+  return c->eval_grid<float>(h, ustep, vstep, ustart, vstart, nuv, out);
+}
+
+int jsi_eval_grid_f64(jsi_ctx* c, int h, double ustep, double vstep, double ustart, double vstart, int nuv,
+                      double* out) {
+  if (!c || !out || nuv < 1 || nuv > 16384) return JSI_ERR_INVALID;
+  HCK(hipSetDevice(c->device));
+  return c->eval_grid<double>(h, ustep, vstep, ustart, vstart, nuv, out);
+}
+
+int jsi_min_distance(jsi_ctx* c, int h1, const jsi_region* r1, int h2, const jsi_region* r2, const jsi_params* P,
+                     jsi_result* res, jsi_pair* pairs_out, int pairs_cap) {
+  RCK(validate_query(c, h1, r1, h2, r2, P));
+  if (!res || P->k < 1 || P->k > JSI_MAX_K || P->dist_max_rounds < 1 || P->dist_refine_nuv < 2 ||
+      P->dist_refine_nuv > JSI_MAX_NUV) {
+    set_err("invalid distance parameters");
+    return JSI_ERR_INVALID;
+  }
+  std::memset(res, 0, sizeof(*res));
+  HCK(hipSetDevice(c->device));
+  if (P->precision == JSI_PREC_EXACT32) return c->min_distance<float>(h1, *r1, h2, *r2, *P, res, pairs_out, pairs_cap);
+  return c->min_distance<double>(h1, *r1, h2, *r2, *P, res, pairs_out, pairs_cap);
+}
+
+int jsi_intersect(jsi_ctx* c, int h1, const jsi_region* r1, int h2, const jsi_region* r2, const jsi_params* P,
+                  jsi_result* res, jsi_pair* pairs_out, int pairs_cap) {
+  RCK(validate_query(c, h1, r1, h2, r2, P));
+  if (!res || P->isect_max_rounds < 1 || P->isect_refine_nuv < 2 || P->isect_refine_nuv > JSI_MAX_NUV ||
+      P->isect_min_nuv < 1) {
+    set_err("invalid intersection parameters");
+    return JSI_ERR_INVALID;
+  }
+  std::memset(res, 0, sizeof(*res));
+  HCK(hipSetDevice(c->device));
+  if (P->precision == JSI_PREC_EXACT32) return c->intersect<float>(h1, *r1, h2, *r2, *P, res, pairs_out, pairs_cap);
+  return c->intersect<double>(h1, *r1, h2, *r2, *P, res, pairs_out, pairs_cap);
+}
+
+}  // extern "C"

@@ -1,0 +1,81 @@
+// jsi_runtime.hpp - the context object behind the C ABI handle jsi_ctx.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "../../include/jsi_mi355x.h"
+#include "jsi_internal.hpp"
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+struct SurfaceRec {
+  bool live = false;
+  int ivec[7] = {0, 0, 0, 0, 0, 0, 0};
+  int ndata = 0;
+  float* d_data = nullptr;      // NurbsFace data in HBM (owned unless registered from device pointers)
+  bool owned = false;
+  double* d_data64 = nullptr;   // promoted copy for the FP64 recipe, made on first FP64 use
+};
+
+// One context per device: a stream, a grow-only set of named device buffers
+// (reused across queries, so steady-state queries allocate nothing) and the
+// resident surfaces.  `cap` bounds the bytes the context may hold.
+struct KernelStat {
+  long long launches = 0;
+  double ms = 0.0;     // summed launch durations (HIP events on the context stream)
+  double units = 0.0;  // summed algorithmic units (cell pairs for the screens)
+};
+
+struct jsi_ctx {
+  int device = 0;
+  int prof = 0;  // time the screen kernels with HIP events
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  std::map<std::string, KernelStat> stats;
+  hipStream_t stream = nullptr;
+  size_t cap = 0;
+  size_t used = 0;
+  int ncu = 0;
+  int grid = 0;  // persistent-kernel grid size
+  std::vector<SurfaceRec> surf;
+  std::map<std::string, DevBuf> bufs;
+
+  int buf_raw(const char* name, size_t bytes, void** out, int fill_int);
+  template <class X>
+  int buf(const char* name, size_t count, X** out) {
+    void* p = nullptr;
+    int r = buf_raw(name, count * sizeof(X), &p, 0);
+    *out = static_cast<X*>(p);
+    return r;
+  }
+  int d2h(void* dst, const void* src, size_t bytes);
+  int h2d(void* dst, const void* src, size_t bytes);
+
+  template <class T>
+  int surf_dev(int h, jsi::SurfDev<T>* out);
+  template <class T>
+  int eval_units(int h, const jsi::vec4_t<T>* reg, int nunits, int nuv, int mode, int steps_given, T* out,
+                 unsigned long long* kmax, const char* tag);
+  template <class T>
+  int dist_topk(int nslots, const int* su1, const int* su2, int w1, int nunits1, const jsi::vec4_t<T>* C1, int w2,
+                int nunits2, const jsi::vec4_t<T>* C2, int k, int cull, jsi::Cand** rec_out, long long* n_out,
+                long long* kk_out);
+  template <class T>
+  int min_distance(int h1, const jsi_region& r1, int h2, const jsi_region& r2, const jsi_params& P,
+                   jsi_result* res, jsi_pair* out, int cap_pairs);
+  template <class T>
+  int isect_units(int h, const jsi::vec4_t<T>* reg, int nunits, int nuv, int use_k, T init_u, T init_v,
+                  jsi::vec4_t<T>* C, jsi::vec4_t<T>* R, const char* tag);
+  template <class T>
+  int intersect(int h1, const jsi_region& r1, int h2, const jsi_region& r2, const jsi_params& P, jsi_result* res,
+                jsi_pair* out, int cap_pairs);
+  template <class T>
+  int eval_grid(int h, T ustep, T vstep, T ustart, T vstart, int nuv, T* host_out);
+};

@@ -1,0 +1,164 @@
+"""Python host side of libjsi_mi355x.so: a thin ctypes layer over the C ABI
+declared in include/jsi_mi355x.h.
+
+It mirrors the reference's query API (JSI_CAD_NEXT_VERSION5/include/jsi_cad.hpp):
+``Context.minimum_distance(s1, r1, s2, r2)`` is minimum_distance,
+``Context.intersection(...)`` is intersection, ``Context.eval_grid(...)`` is
+evalWithCuda.  There is no CPU fallback: if the library is missing or no GPU
+is present, loading raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from ._abi import (JSI_OK, MAX_ROUNDS, Pair, Params, Region, Result, default_params,  # noqa: F401
+                   pairs_to_numpy)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libjsi_mi355x.so")
+
+_f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+
+# Every extern "C" symbol of include/jsi_mi355x.h (test_library_exports checks them).
+EXPORTS = [
+    "jsi_params_default", "jsi_status_string", "jsi_last_error", "jsi_ctx_create", "jsi_ctx_destroy",
+    "jsi_ctx_sync", "jsi_ctx_profile", "jsi_ctx_kernel_stats", "jsi_surface_upload", "jsi_surface_register_device", "jsi_surface_release",
+    "jsi_eval_grid_f32", "jsi_eval_grid_f64", "jsi_min_distance", "jsi_intersect",
+]
+
+_lib = None
+
+
+class JsiError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"jsi status {status}: {message}")
+        self.status = status
+
+
+def load_library(path: str = LIB_PATH):
+    """Loads libjsi_mi355x.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with python -m jsi_cad_next_versions_amd.build "
+                           "(or __graft_entry__.build())")
+    L = C.CDLL(path)
+    L.jsi_params_default.argtypes = [C.POINTER(Params)]
+    L.jsi_params_default.restype = None
+    L.jsi_status_string.argtypes = [C.c_int]
+    L.jsi_status_string.restype = C.c_char_p
+    L.jsi_last_error.restype = C.c_char_p
+    L.jsi_ctx_create.argtypes = [C.c_int, C.c_size_t, C.POINTER(C.c_void_p)]
+    L.jsi_ctx_destroy.argtypes = [C.c_void_p]
+    L.jsi_ctx_sync.argtypes = [C.c_void_p]
+    L.jsi_ctx_profile.argtypes = [C.c_void_p, C.c_int]
+    L.jsi_ctx_kernel_stats.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_int64), C.POINTER(C.c_double),
+                                       C.POINTER(C.c_double)]
+    L.jsi_surface_upload.argtypes = [C.c_void_p, _i32p, _f32p, C.c_int, C.POINTER(C.c_int)]
+    L.jsi_surface_register_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_int)]
+    L.jsi_surface_release.argtypes = [C.c_void_p, C.c_int]
+    L.jsi_eval_grid_f32.argtypes = [C.c_void_p, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int, _f32p]
+    L.jsi_eval_grid_f64.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int,
+                                    _f64p]
+    q = [C.c_void_p, C.c_int, C.POINTER(Region), C.c_int, C.POINTER(Region), C.POINTER(Params), C.POINTER(Result),
+         C.c_void_p, C.c_int]
+    L.jsi_min_distance.argtypes = q
+    L.jsi_intersect.argtypes = q
+    _lib = L
+    return L
+
+
+def _check(L, rc):
+    if rc != JSI_OK:
+        raise JsiError(rc, L.jsi_last_error().decode(errors="replace"))
+
+
+class Context:
+    """One device context with resident surfaces (jsi_ctx)."""
+
+    def __init__(self, device: int = 0, hbm_cap_bytes: int = 0):
+        self.L = load_library()
+        h = C.c_void_p()
+        _check(self.L, self.L.jsi_ctx_create(device, hbm_cap_bytes, C.byref(h)))
+        self.h = h
+        self._handles = {}
+
+    def close(self):
+        if self.h:
+            self.L.jsi_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sync(self):
+        _check(self.L, self.L.jsi_ctx_sync(self.h))
+
+    def profile(self, enable=True):
+        _check(self.L, self.L.jsi_ctx_profile(self.h, int(enable)))
+
+    def kernel_stats(self, name):
+        """(launches, summed ms, summed algorithmic units) of a profiled kernel."""
+        n, ms, u = C.c_int64(), C.c_double(), C.c_double()
+        _check(self.L, self.L.jsi_ctx_kernel_stats(self.h, name.encode(), C.byref(n), C.byref(ms), C.byref(u)))
+        return n.value, ms.value, u.value
+
+    def upload(self, s) -> int:
+        """Uploads a Surface once; later calls with the same object reuse it."""
+        key = id(s)
+        if key in self._handles and self._handles[key][0] is s:
+            return self._handles[key][1]
+        h = C.c_int()
+        _check(self.L, self.L.jsi_surface_upload(self.h, s.ivec, s.data, s.ndata, C.byref(h)))
+        self._handles[key] = (s, h.value)
+        return h.value
+
+    def release(self, s):
+        key = id(s)
+        if key in self._handles:
+            _check(self.L, self.L.jsi_surface_release(self.h, self._handles.pop(key)[1]))
+
+    def eval_grid(self, s, ustep, vstep, ustart, vstart, nuv, dtype=np.float32):
+        """evalWithCuda: 18 values per grid point, point (idx, idy) at offset idx + idy*nuv."""
+        h = self.upload(s)
+        out = np.empty(nuv * nuv * 18, dtype)
+        if dtype == np.float32:
+            _check(self.L, self.L.jsi_eval_grid_f32(self.h, h, ustep, vstep, ustart, vstart, nuv, out))
+        else:
+            _check(self.L, self.L.jsi_eval_grid_f64(self.h, h, ustep, vstep, ustart, vstart, nuv, out))
+        return out
+
+    def _query(self, fn, s1, r1, s2, r2, params, cap):
+        h1, h2 = self.upload(s1), self.upload(s2)
+        res = Result()
+        buf = (Pair * max(cap, 1))()
+        _check(self.L, fn(self.h, h1, C.byref(r1), h2, C.byref(r2), C.byref(params), C.byref(res),
+                          C.cast(buf, C.c_void_p), cap))
+        n = int(min(cap, res.npairs))
+        return res, pairs_to_numpy(buf, n)
+
+    def minimum_distance(self, s1, r1, s2, r2, params=None, cap=None):
+        """Returns (Result, last-round top-k pairs)."""
+        params = params or default_params()
+        cap = params.k if cap is None else cap
+        return self._query(self.L.jsi_min_distance, s1, r1, s2, r2, params, cap)
+
+    def intersection(self, s1, r1, s2, r2, params=None, cap=0):
+        """Returns (Result, last-round overlap pairs (up to cap))."""
+        params = params or default_params()
+        return self._query(self.L.jsi_intersect, s1, r1, s2, r2, params, cap)

@@ -1,0 +1,102 @@
+"""Writes the golden fixtures of tests/golden/ by running the reference's own
+VERSION5 library, compiled from its sources by oracle/ref.mk into
+oracle/_ref/libjsi_v5_ref.so (this container only: /root/reference does not
+exist on the GPU box).  Inputs are the reference's sample surfaces
+(examples/data/inter_case*.cu, linked into that library) and the seeded
+synthetic surfaces of jsi_cad_next_versions_amd.surfaces.
+
+Outputs (data only): eval_f32.npz (grid evaluations), golden.json (distances,
+compute_K values, intersection round counts, all floats as exact bit patterns).
+"""
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+from oracle import RefV5  # noqa: E402
+from jsi_cad_next_versions_amd import surfaces as S  # noqa: E402
+
+
+def f32bits(x):
+    return int(np.float32(x).view(np.uint32))
+
+
+def surfaces(ref):
+    wa, wb = S.wave_pair_small()
+    ha, hb = S.hard_wave_pair()
+    sa, sb = S.saddle_wave_pair()
+    out = {"wave16_A": wa, "wave16_B": wb, "hardwave_A": ha, "hardwave_B": hb, "saddle": sa, "waveplane": sb}
+    for c in (1, 2, 3, 4):
+        for o in (1, 2):
+            out[f"inter_case{c}_obj{o}"] = ref.surface(f"inter_case{c}_obj{o}")
+    return out
+
+
+def main():
+    ref = RefV5()
+    surf = surfaces(ref)
+    arrays = {}
+    gold = {"eval": [], "compute_k": [], "dist_centers": [], "min_distance": [], "intersection": []}
+    # grid evaluations (evalWithCuda) at small nuv
+    for name, nuv in [("inter_case3_obj1", 17), ("inter_case4_obj2", 17), ("wave16_A", 19), ("hardwave_A", 23),
+                      ("saddle", 21), ("waveplane", 21), ("inter_case1_obj1", 7), ("inter_case2_obj2", 6)]:
+        s = surf[name]
+        us = np.float32(np.float32(s.uend() - s.ubegin()) / np.float32(nuv))
+        vs = np.float32(np.float32(s.vend() - s.vbegin()) / np.float32(nuv))
+        out = ref.eval_grid(s, us, vs, np.float32(s.ubegin()), np.float32(s.vbegin()), nuv)
+        key = f"{name}_n{nuv}"
+        arrays[key] = out
+        gold["eval"].append({"surface": name, "nuv": nuv, "ustep": f32bits(us), "vstep": f32bits(vs),
+                             "ustart": f32bits(s.ubegin()), "vstart": f32bits(s.vbegin()), "key": key})
+        gold["compute_k"].append({"key": key, "nuv": nuv, "k": f32bits(ref.compute_k(out, nuv))})
+    # distance centres (run_evaluation_and_construction_task)
+    for name, nuv in [("hardwave_B", 33), ("wave16_B", 24)]:
+        s = surf[name]
+        r = S.full_region(s, nuv, 2)
+        c = ref.dist_centers(s, r)
+        key = f"centers_{name}_n{nuv}"
+        arrays[key] = c
+        gold["dist_centers"].append({"surface": name, "nuv": nuv, "key": key})
+    # end-to-end minimum_distance (k = 8, 4 rounds, refinement nuv 16)
+    for a, b, nuv in [("wave16_A", "wave16_B", 64), ("hardwave_A", "hardwave_B", 64),
+                      ("hardwave_A", "hardwave_B", 128), ("saddle", "waveplane", 48),
+                      ("inter_case3_obj1", "inter_case3_obj2", 40), ("inter_case4_obj1", "inter_case4_obj2", 33)]:
+        s1, s2 = surf[a], surf[b]
+        d, _ = ref.min_distance(s1, S.full_region(s1, nuv, 1), s2, S.full_region(s2, nuv, 2))
+        gold["min_distance"].append({"a": a, "b": b, "nuv": nuv, "distance": f32bits(d)})
+    # intersection round counts; the refinement nuv is a static carried across calls
+    g = 16
+    for a, b, nuv, use_k in [("inter_case3_obj1", "inter_case3_obj2", 16, True),
+                             ("inter_case4_obj1", "inter_case4_obj2", 16, True),
+                             ("saddle", "waveplane", 32, True),
+                             ("saddle", "waveplane", 40, False),
+                             ("inter_case3_obj1", "inter_case3_obj2", 24, False)]:
+        s1, s2 = surf[a], surf[b]
+        counts, _ = ref.intersect(s1, S.full_region(s1, nuv, 1), s2, S.full_region(s2, nuv, 2), use_k=use_k)
+        gold["intersection"].append({"a": a, "b": b, "nuv": nuv, "use_k": use_k, "refine_nuv_in": g,
+                                     "counts": counts})
+        # halving rule (intersection.cu:370) applied to the printed counts gives the next call's start
+        for c in counts[:-1]:
+            if c == 0:
+                break
+            if c > 512 and g > 4:
+                g //= 2
+    np.savez_compressed(os.path.join(HERE, "eval_f32.npz"), **arrays)
+    # the sample surfaces (data only) so the fixtures are self-contained on machines without the reference
+    small = {k: v for k, v in surf.items() if k.startswith(("inter_case3", "inter_case4"))}
+    np.savez_compressed(os.path.join(HERE, "sample_surfaces.npz"),
+                        **{f"{k}_ivec": v.ivec for k, v in small.items()},
+                        **{f"{k}_data": v.data for k, v in small.items()})
+    with open(os.path.join(HERE, "golden.json"), "w") as f:
+        json.dump(gold, f, indent=1)
+    print("wrote", len(arrays), "arrays;", {k: len(v) for k, v in gold.items()})
+
+
+if __name__ == "__main__":
+    main()
