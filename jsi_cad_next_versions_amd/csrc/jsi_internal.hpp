@@ -107,6 +107,15 @@ hipError_t launch_dist_filter(const uint64_t* list, const unsigned* list_count, 
 template <class T>
 hipError_t launch_set_thresh(T* thresh, const Cand* rec, int idx, int use_inf, hipStream_t s);
 
+// FP32SCREEN helpers: round FP64 centres to float (and max-reduce
+// |coordinate|), widen a float threshold by the rounding bound, and replace
+// candidate keys by their FP64 d2.
+hipError_t launch_narrow(const double4* src, float4* dst, long long n, unsigned long long* absmax_bits,
+                         hipStream_t s);
+hipError_t launch_inflate(float* thresh, const unsigned long long* absmax_bits, hipStream_t s);
+hipError_t launch_rerank(Cand* cand, unsigned long long n, const int* su1, const int* su2, int cells1, int cells2,
+                         const double4* C1, const double4* C2, hipStream_t s);
+
 // Sorting of candidate records (bitonic, n2 = power of two >= kSortBlock).
 hipError_t launch_fill_sentinel(Cand* a, unsigned long long from, unsigned long long n2, hipStream_t s);
 hipError_t sort_cands(Cand* a, unsigned long long n2, hipStream_t s);

@@ -630,6 +630,70 @@ __global__ __launch_bounds__(256) void k_dist_filter(const uint64_t* list, const
   if (pairs_done && threadIdx.x == 0 && done) atomicAdd(pairs_done, done);
 }
 
+// ---------------------------------------------------------------------------
+// FP32 screening with an FP64 result (FP32SCREEN).  Centres are evaluated in
+// FP64 and rounded to float for the screen; the screen's threshold is widened
+// by a bound on every rounding step so that its candidate set contains every
+// pair the FP64 ranking can select, and the candidates are re-ranked with
+// their FP64 d2.
+// ---------------------------------------------------------------------------
+__global__ void k_narrow(const double4* __restrict__ src, float4* __restrict__ dst, long long n,
+                         unsigned long long* absmax_bits) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  double m = 0.0;
+  if (i < n) {
+    const double4 c = src[i];
+    float4 f;
+    f.x = __double2float_rn(c.x);
+    f.y = __double2float_rn(c.y);
+    f.z = __double2float_rn(c.z);
+    f.w = 0.f;
+    dst[i] = f;
+    m = fmax(fabs(c.x), fmax(fabs(c.y), fabs(c.z)));
+  }
+  unsigned long long b = (unsigned long long)__double_as_longlong(m);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long o = __shfl_xor(b, off);
+    b = o > b ? o : b;
+  }
+  if (lane_id() == 0 && b) atomicMax(absmax_bits, b);
+}
+
+// T <- (sqrt(T)(1+16u) + 4 sqrt(3) E (1+16u))^2 (1+16u) + 1e-36, u = 2^-24,
+// E = 2^-24 max|centre coordinate|: one widening covers (a) the float
+// rounding of both centres (<= E per coordinate, <= 2 sqrt(3) E in distance),
+// (b) the float evaluation of d2 (<= 5u relative) and (c) the double
+// evaluation of d2 (<= 5 * 2^-53 relative), applied once to bound the FP64
+// k-th d2 from k float values and once more to reach every pair below it.
+__global__ void k_inflate(float* thresh, const unsigned long long* absmax_bits) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const float t = *thresh;
+  if (!(t < __uint_as_float(0x7f800000u))) return;  // +inf stays +inf
+  const double u = 5.9604644775390625e-08;         // 2^-24
+  const double E = __longlong_as_double((long long)*absmax_bits) * u;
+  const double g = 1.0 + 16.0 * u;
+  const double r = sqrt((double)t) * g + 4.0 * 1.7320508075688772 * E * g;
+  *thresh = __double2float_ru(r * r * g + 1e-36);
+}
+
+// Candidate records get their FP64 d2 (same recipe as the FP64 screen).
+__global__ void k_rerank(Cand* cand, unsigned long long n, const int* su1, const int* su2, int cells1, int cells2,
+                         const double4* C1, const double4* C2) {
+  const unsigned long long e = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  Cand c = cand[e];
+  const int slot = (int)(c.lo >> 40), i = (int)((c.lo >> 20) & 0xfffff), j = (int)(c.lo & 0xfffff);
+  const double4 a = C1[(size_t)su1[slot] * cells1 + i];
+  const double4 b = C2[(size_t)su2[slot] * cells2 + j];
+  const double dx = a.x - b.x;
+  const double dy = a.y - b.y;
+  const double dz = a.z - b.z;
+  const double d2 = dx * dx + dy * dy + dz * dz;
+  c.hi = (unsigned long long)__double_as_longlong(d2);
+  cand[e] = c;
+}
+
 template <class T>
 __global__ void k_set_thresh(T* thresh, const Cand* rec, int idx, int use_inf) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *thresh = use_inf ? infinity_v<T>() : from_bits<T>(rec[idx].hi);
@@ -1164,6 +1228,26 @@ hipError_t launch_dist_filter(const uint64_t* list, const unsigned* list_count, 
 template <class T>
 hipError_t launch_set_thresh(T* thresh, const Cand* rec, int idx, int use_inf, hipStream_t s) {
   hipLaunchKernelGGL(k_set_thresh<T>, dim3(1), dim3(64), 0, s, thresh, rec, idx, use_inf);
+  return hipGetLastError();
+}
+
+hipError_t launch_narrow(const double4* src, float4* dst, long long n, unsigned long long* absmax_bits,
+                         hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_narrow, dim3(blocks_for(n, 256)), dim3(256), 0, s, src, dst, n, absmax_bits);
+  return hipGetLastError();
+}
+
+hipError_t launch_inflate(float* thresh, const unsigned long long* absmax_bits, hipStream_t s) {
+  hipLaunchKernelGGL(k_inflate, dim3(1), dim3(64), 0, s, thresh, absmax_bits);
+  return hipGetLastError();
+}
+
+hipError_t launch_rerank(Cand* cand, unsigned long long n, const int* su1, const int* su2, int cells1, int cells2,
+                         const double4* C1, const double4* C2, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_rerank, dim3(blocks_for(n, 256)), dim3(256), 0, s, cand, n, su1, su2, cells1, cells2, C1,
+                     C2);
   return hipGetLastError();
 }
 

@@ -188,8 +188,13 @@ int jsi_ctx::eval_units(int h, const vec4_t<T>* reg, int nunits, int nuv, int mo
 template <class T>
 int jsi_ctx::dist_topk(int nslots, const int* su1, const int* su2, int w1, int nunits1, const vec4_t<T>* C1, int w2,
                         int nunits2, const vec4_t<T>* C2, int k, int cull, Cand** rec_out, long long* n_out,
-                        long long* kk_out) {
+                        long long* kk_out, Rerank rr) {
   using V = vec4_t<T>;
+  const bool rerank = rr.C1 != nullptr;
+  auto widen = [&]() -> int {
+    if (rerank) HCK(launch_inflate(reinterpret_cast<float*>(thresh_ptr_), rr.absmax, stream));
+    return JSI_OK;
+  };
   const int tpu1 = ((w1 + kTile - 1) / kTile) * ((w1 + kTile - 1) / kTile);
   const int tpu2 = ((w2 + kTile - 1) / kTile) * ((w2 + kTile - 1) / kTile);
   V *lo1, *hi1, *lo2, *hi2;
@@ -216,6 +221,7 @@ int jsi_ctx::dist_topk(int nslots, const int* su1, const int* su2, int w1, int n
   out_count = reinterpret_cast<unsigned long long*>(list_count + 4);
   thresh = reinterpret_cast<T*>(list_count + 6);
   RCK(buf("d.cand", kCandCap, &cand));
+  thresh_ptr_ = thresh;
   // seed: exact k-th d2 of the best single tile pair
   HCK(launch_fill_u32(reinterpret_cast<unsigned*>(best), 0xffffffffu, 2, stream));
   HCK(launch_dist_seed<T>(nslots, su1, su2, w1, w2, lo1, hi1, lo2, hi2, k, best, stream));
@@ -235,6 +241,7 @@ int jsi_ctx::dist_topk(int nslots, const int* su1, const int* su2, int w1, int n
     HCK(launch_fill_sentinel(cand, ns, n2, stream));
     HCK(sort_cands(cand, n2, stream));
     HCK(launch_set_thresh<T>(thresh, cand, k - 1, 0, stream));
+    RCK(widen());
   }
   unsigned long long n = 0;
   for (int attempt = 0;; ++attempt) {
@@ -269,7 +276,9 @@ int jsi_ctx::dist_topk(int nslots, const int* su1, const int* su2, int w1, int n
     // pairs) is a tighter upper bound of the k-th d2
     HCK(sort_cands(cand, kCandCap, stream));
     HCK(launch_set_thresh<T>(thresh, cand, k - 1, 0, stream));
+    RCK(widen());
   }
+  if (rerank) HCK(launch_rerank(cand, n, su1, su2, w1 * w1, w2 * w2, rr.C1, rr.C2, stream));
   const unsigned long long n2 = std::max(next_pow2(std::max(n, 1ull)), (unsigned long long)kSortBlock);
   HCK(launch_fill_sentinel(cand, n, n2, stream));
   HCK(sort_cands(cand, n2, stream));
@@ -357,7 +366,28 @@ int jsi_ctx::min_distance(int h1, const jsi_region& r1, int h2, const jsi_region
     const int w1 = nuv1 - 1, w2 = nuv2 - 1;
     const double ts = now_ms();
     long long n = 0;
-    RCK(dist_topk<T>(nslots, su1, su2, w1, nunits1, C1, w2, nunits2, C2, P.k, P.cull, &rec, &n, &kk));
+    if constexpr (std::is_same<T, double>::value) {
+      if (P.precision == JSI_PREC_FP32SCREEN) {
+        float4 *f1, *f2;
+        unsigned long long* amax;
+        const long long n1 = (long long)nunits1 * w1 * w1, n2 = (long long)nunits2 * w2 * w2;
+        RCK(buf("s32.C1", (size_t)n1, &f1));
+        RCK(buf("s32.C2", (size_t)n2, &f2));
+        RCK(buf("s32.amax", 2, &amax));
+        HCK(launch_fill_u32(reinterpret_cast<unsigned*>(amax), 0u, 2, stream));
+        HCK(launch_narrow(C1, f1, n1, amax, stream));
+        HCK(launch_narrow(C2, f2, n2, amax, stream));
+        Rerank rr;
+        rr.C1 = C1;
+        rr.C2 = C2;
+        rr.absmax = amax;
+        RCK(dist_topk<float>(nslots, su1, su2, w1, nunits1, f1, w2, nunits2, f2, P.k, P.cull, &rec, &n, &kk, rr));
+      } else {
+        RCK(dist_topk<T>(nslots, su1, su2, w1, nunits1, C1, w2, nunits2, C2, P.k, P.cull, &rec, &n, &kk));
+      }
+    } else {
+      RCK(dist_topk<T>(nslots, su1, su2, w1, nunits1, C1, w2, nunits2, C2, P.k, P.cull, &rec, &n, &kk));
+    }
     jsi_round_info& ri = res->round[round - 1];
     ri.npairs = kk;
     ri.slots = nslots;

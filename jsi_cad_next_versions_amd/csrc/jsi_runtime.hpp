@@ -28,6 +28,13 @@ struct SurfaceRec {
 // One context per device: a stream, a grow-only set of named device buffers
 // (reused across queries, so steady-state queries allocate nothing) and the
 // resident surfaces.  `cap` bounds the bytes the context may hold.
+// FP32SCREEN: float centres are screened, FP64 centres re-rank the candidates.
+struct RerankSpec {
+  const double4* C1 = nullptr;
+  const double4* C2 = nullptr;
+  const unsigned long long* absmax = nullptr;
+};
+
 struct KernelStat {
   long long launches = 0;
   double ms = 0.0;     // summed launch durations (HIP events on the context stream)
@@ -46,6 +53,7 @@ struct jsi_ctx {
   int grid = 0;  // persistent-kernel grid size
   std::vector<SurfaceRec> surf;
   std::map<std::string, DevBuf> bufs;
+  void* thresh_ptr_ = nullptr;  // device threshold of the screen in flight
 
   int buf_raw(const char* name, size_t bytes, void** out, int fill_int);
   template <class X>
@@ -63,10 +71,11 @@ struct jsi_ctx {
   template <class T>
   int eval_units(int h, const jsi::vec4_t<T>* reg, int nunits, int nuv, int mode, int steps_given, T* out,
                  unsigned long long* kmax, const char* tag);
+  using Rerank = RerankSpec;
   template <class T>
   int dist_topk(int nslots, const int* su1, const int* su2, int w1, int nunits1, const jsi::vec4_t<T>* C1, int w2,
                 int nunits2, const jsi::vec4_t<T>* C2, int k, int cull, jsi::Cand** rec_out, long long* n_out,
-                long long* kk_out);
+                long long* kk_out, Rerank rr = Rerank());
   template <class T>
   int min_distance(int h1, const jsi_region& r1, int h2, const jsi_region& r2, const jsi_params& P,
                    jsi_result* res, jsi_pair* out, int cap_pairs);

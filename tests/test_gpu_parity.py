@@ -72,9 +72,11 @@ def test_min_distance_exact32_matches_reference(ctx, orc, e):
     np.testing.assert_array_equal(pairs, opairs)
 
 
-@pytest.mark.parametrize("prec", ["exact32", "fp64"])
+@pytest.mark.parametrize("prec", ["exact32", "fp64", "fp32screen"])
 @pytest.mark.parametrize("k", [1, 64, 4096])
 def test_min_distance_hardwave_matches_oracle(ctx, orc, prec, k):
+    """FP32SCREEN must reproduce the FP64 oracle exactly (its FP32 candidate
+    set is a superset, re-ranked in FP64)."""
     a, b = S.hard_wave_pair()
     r1, r2 = regions(a, b, 96)
     p = default_params(k=k, precision=prec, trace=1)
@@ -85,11 +87,12 @@ def test_min_distance_hardwave_matches_oracle(ctx, orc, prec, k):
     np.testing.assert_array_equal(pairs, opairs)
 
 
+@pytest.mark.parametrize("prec", ["exact32", "fp32screen"])
 @pytest.mark.parametrize("cull", [0, 1])
-def test_culling_leaves_results_unchanged(ctx, orc, cull):
+def test_culling_leaves_results_unchanged(ctx, orc, cull, prec):
     a, b = S.wave_pair_small()
     r1, r2 = regions(a, b, 80, 72)
-    p = default_params(k=300, cull=cull, trace=1)
+    p = default_params(k=300, cull=cull, precision=prec, trace=1)
     res, pairs = ctx.minimum_distance(a, r1, b, r2, p)
     ores, opairs = orc.min_distance(a, r1, b, r2, p)
     same_rounds(res, ores)
