@@ -96,14 +96,14 @@ hipError_t launch_dist_seed(int nslots, const int* su1, const int* su2, int w1, 
 template <class T>
 hipError_t launch_dist_list(int nslots, const int* su1, const int* su2, int w1, int w2, const vec4_t<T>* lo1,
                             const vec4_t<T>* hi1, const vec4_t<T>* lo2, const vec4_t<T>* hi2, const T* thresh,
-                            int cull, uint64_t* list, unsigned* list_count, hipStream_t s);
+                            int cull, uint64_t* list, unsigned* list_count, hipStream_t s, int fma = 0);
 hipError_t launch_seed_list(const unsigned long long* best, uint64_t* list, unsigned* list_count, int w1, int w2,
                             hipStream_t s);
 template <class T>
 hipError_t launch_dist_filter(const uint64_t* list, const unsigned* list_count, unsigned list_cap, const int* su1,
                               const int* su2, int w1, int w2, const vec4_t<T>* C1, const vec4_t<T>* C2,
                               const T* thresh, Cand* out, unsigned long long* out_count, unsigned long long cap,
-                              int grid, hipStream_t s, unsigned long long* pairs_done = nullptr);
+                              int grid, hipStream_t s, unsigned long long* pairs_done = nullptr, int fma = 0);
 template <class T>
 hipError_t launch_set_thresh(T* thresh, const Cand* rec, int idx, int use_inf, hipStream_t s);
 
@@ -115,6 +115,26 @@ hipError_t launch_narrow(const double4* src, float4* dst, long long n, unsigned 
 hipError_t launch_inflate(float* thresh, const unsigned long long* absmax_bits, hipStream_t s);
 hipError_t launch_rerank(Cand* cand, unsigned long long n, const int* su1, const int* su2, int cells1, int cells2,
                          const double4* C1, const double4* C2, hipStream_t s);
+
+// Operator ABI: flat AoS arrays -> padded w x w float4 grids; endpoint masks.
+hipError_t launch_pack_cells(const float* c, const float* r, long long n, int w, float4* C, float4* R,
+                             hipStream_t s);
+hipError_t launch_pair_mask(int mode, const uint64_t* list, const unsigned long long* list_count,
+                            const unsigned* list_count32, int w1, int w2, const float4* C1, const float4* R1,
+                            const float4* C2, const float4* R2, const float* thresh, long long n1, long long n2,
+                            int* m1, int* m2, int grid, hipStream_t s);
+
+// Histogram radix selection of the k smallest candidate keys (see jsi_kernels.hip).
+struct SelState {
+  unsigned long long lo, hi, width, need, below, final_hi;
+  int done;
+  int pad_;
+};
+hipError_t select_topk(const Cand* cand, unsigned long long n, unsigned long long k, unsigned long long maxkey,
+                       SelState* st, unsigned* hist, Cand* out, unsigned long long* out_count, unsigned small_cap,
+                       hipStream_t s);
+template <class T>
+hipError_t launch_thresh_from_state(T* thresh, const SelState* st, hipStream_t s);
 
 // Sorting of candidate records (bitonic, n2 = power of two >= kSortBlock).
 hipError_t launch_fill_sentinel(Cand* a, unsigned long long from, unsigned long long n2, hipStream_t s);

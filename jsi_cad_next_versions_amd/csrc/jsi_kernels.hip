@@ -530,7 +530,7 @@ __global__ void k_seed_list(const unsigned long long* best, uint64_t* list, unsi
 // Correctly rounded subtraction, squaring and addition are monotone, so the
 // bound computed in T never exceeds any computed cell-pair d2; a tile pair is
 // skipped only when the bound is strictly above the threshold.
-template <class T>
+template <class T, bool FMA>
 __global__ void k_dist_list(int nslots, const int* su1, const int* su2, int w1, int w2, const vec4_t<T>* lo1,
                             const vec4_t<T>* hi1, const vec4_t<T>* lo2, const vec4_t<T>* hi2, const T* thresh,
                             int cull, uint64_t* list, unsigned* list_count) {
@@ -547,7 +547,7 @@ __global__ void k_dist_list(int nslots, const int* su1, const int* su2, int w1, 
       const T gx = smax(T(0), smax(l1.x - h2.x, l2.x - h1.x));
       const T gy = smax(T(0), smax(l1.y - h2.y, l2.y - h1.y));
       const T gz = smax(T(0), smax(l1.z - h2.z, l2.z - h1.z));
-      const T lb2 = gx * gx + gy * gy + gz * gz;
+      const T lb2 = FMA ? fma(gx, gx, fma(gy, gy, gz * gz)) : gx * gx + gy * gy + gz * gz;
       keep = !(lb2 > *thresh);
     }
   }
@@ -561,67 +561,104 @@ __global__ void k_dist_list(int nslots, const int* su1, const int* su2, int w1, 
   }
 }
 
+// d2 of two centres.  FMA = false: center_distance_sq's order
+// (distance_analysis.cu:202-207), no contraction (EXACT32 / FP64).  FMA = true:
+// fma(dx, dx, fma(dy, dy, dz*dz)) for the FP32SCREEN screen, whose threshold
+// margin covers its rounding (three roundings instead of five).
+template <class T, bool FMA>
+__device__ __forceinline__ T pair_d2(T ax, T ay, T az, T bx, T by, T bz) {
+  const T dx = ax - bx;
+  const T dy = ay - by;
+  const T dz = az - bz;
+  if (FMA) return fma(dx, dx, fma(dy, dy, dz * dz));
+  return dx * dx + dy * dy + dz * dz;
+}
+
+template <class T>
+__device__ __forceinline__ T quiet_nan();
+template <>
+__device__ __forceinline__ float quiet_nan<float>() {
+  return __uint_as_float(0x7fc00000u);
+}
+template <>
+__device__ __forceinline__ double quiet_nan<double>() {
+  return __longlong_as_double(0x7ff8000000000000ll);
+}
+
 // Distance filter: every cell pair (i, j) of every listed tile pair with
 // centre d2 <= threshold is appended to `out` (order irrelevant: records are
 // sorted by the full key afterwards).  One tile pair per workgroup iteration;
-// side-2 cells are staged in LDS and read as broadcasts, each lane owns one
-// side-1 cell.  d2 is center_distance_sq (distance_analysis.cu:202-207).
-template <class T>
+// side-2 cells are staged in LDS (padded to a multiple of 8 with NaN, which
+// never passes the test) and read as broadcasts, each lane owns one side-1
+// cell.  The j loop is unrolled by 8 and tests the minimum of 8 d2 values
+// once; only a passing group re-tests its members and appends them.
+template <class T, bool FMA>
 __global__ __launch_bounds__(256) void k_dist_filter(const uint64_t* list, const unsigned* list_count,
                                                      const int* su1, const int* su2, int w1, int w2,
                                                      const vec4_t<T>* C1, const vec4_t<T>* C2, const T* thresh,
                                                      Cand* out, unsigned long long* out_count,
                                                      unsigned long long cap, unsigned long long* pairs_done) {
+  constexpr int U = 8;
   __shared__ vec4_t<T> sB[kTileCells];
-  unsigned long long done = 0;
   __shared__ int sJ[kTileCells];
+  unsigned long long done = 0;
   const unsigned n = *list_count;
   const T tv = *thresh;
   const int lane = lane_id();
-  const int tpu1 = tiles_per_unit(w1);
-  (void)tpu1;
   for (unsigned it = blockIdx.x; it < n; it += gridDim.x) {
     const uint64_t e = list[it];
     const int slot = (int)(e >> 32), t1 = (int)((e >> 16) & 0xffff), t2 = (int)(e & 0xffff);
     const int u1 = su1[slot], u2 = su2[slot];
     const TileGeo g1 = tile_geo(t1, w1), g2 = tile_geo(t2, w2);
     const int nA = g1.ht * g1.wd, nB = g2.ht * g2.wd;
+    const int nBp = (nB + U - 1) / U * U;
     done += (unsigned long long)nA * nB;
     __syncthreads();
     if ((int)threadIdx.x < nB) {
       const int j = (g2.row0 + threadIdx.x / g2.wd) * w2 + g2.col0 + threadIdx.x % g2.wd;
       sB[threadIdx.x] = C2[(size_t)u2 * w2 * w2 + j];
       sJ[threadIdx.x] = j;
+    } else if ((int)threadIdx.x < nBp) {
+      vec4_t<T> b;
+      b.x = b.y = b.z = b.w = quiet_nan<T>();
+      sB[threadIdx.x] = b;
+      sJ[threadIdx.x] = 0;
     }
     const bool va = (int)threadIdx.x < nA;
     int i = 0;
     vec4_t<T> a;
-    a.x = a.y = a.z = a.w = T(0);
+    a.x = a.y = a.z = a.w = quiet_nan<T>();  // inactive lanes never pass
     if (va) {
       i = (g1.row0 + threadIdx.x / g1.wd) * w1 + g1.col0 + threadIdx.x % g1.wd;
       a = C1[(size_t)u1 * w1 * w1 + i];
     }
     __syncthreads();
-    for (int jj = 0; jj < nB; ++jj) {
-      const vec4_t<T> b = sB[jj];
-      const T dx = a.x - b.x;
-      const T dy = a.y - b.y;
-      const T dz = a.z - b.z;
-      const T d2 = dx * dx + dy * dy + dz * dz;
-      const bool hit = va && (d2 <= tv);
-      const unsigned long long m = __ballot(hit);
-      if (m) {
-        unsigned long long base = 0;
-        const int leader = __ffsll((long long)m) - 1;
-        if (lane == leader) base = atomicAdd(out_count, (unsigned long long)__popcll(m));
-        base = __shfl(base, leader);
-        if (hit) {
-          const unsigned long long pos = base + __popcll(m & ((1ull << lane) - 1));
-          if (pos < cap) {
-            Cand c;
-            c.hi = ord_bits(d2);
-            c.lo = pack_idx((uint32_t)slot, (uint32_t)i, (uint32_t)sJ[jj]);
-            out[pos] = c;
+    for (int j0 = 0; j0 < nBp; j0 += U) {
+      T d[U];
+#pragma unroll
+      for (int q = 0; q < U; ++q) {
+        const vec4_t<T> b = sB[j0 + q];
+        d[q] = pair_d2<T, FMA>(a.x, a.y, a.z, b.x, b.y, b.z);
+      }
+      T m = fmin(fmin(fmin(d[0], d[1]), fmin(d[2], d[3])), fmin(fmin(d[4], d[5]), fmin(d[6], d[7])));
+      if (__ballot(m <= tv) == 0ull) continue;
+#pragma unroll
+      for (int q = 0; q < U; ++q) {
+        const bool hit = d[q] <= tv;
+        const unsigned long long mk = __ballot(hit);
+        if (mk) {
+          unsigned long long base = 0;
+          const int leader = __ffsll((long long)mk) - 1;
+          if (lane == leader) base = atomicAdd(out_count, (unsigned long long)__popcll(mk));
+          base = __shfl(base, leader);
+          if (hit) {
+            const unsigned long long pos = base + __popcll(mk & ((1ull << lane) - 1));
+            if (pos < cap) {
+              Cand c;
+              c.hi = ord_bits(d[q]);
+              c.lo = pack_idx((uint32_t)slot, (uint32_t)i, (uint32_t)sJ[j0 + q]);
+              out[pos] = c;
+            }
           }
         }
       }
@@ -694,9 +731,213 @@ __global__ void k_rerank(Cand* cand, unsigned long long n, const int* su1, const
   cand[e] = c;
 }
 
+// ---------------------------------------------------------------------------
+// Operator ABI (mask semantics of jsi_ascend_distance_topk_mask /
+// jsi_ascend_intersection_mask, hotspots_910b/src/jsi_ascend_hotspots.cpp:33-113):
+// flat AoS centre arrays are laid out on a w x w cell grid padded with +inf
+// centres (radius 0), which never pass a finite threshold or an overlap test.
+// ---------------------------------------------------------------------------
+__global__ void k_pack_cells(const float* __restrict__ c, const float* __restrict__ r, long long n, int w,
+                             float4* __restrict__ C, float4* __restrict__ R) {
+  const long long x = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= (long long)w * w) return;
+  const float inf = __uint_as_float(0x7f800000u);
+  float4 a, b;
+  if (x < n) {
+    a.x = c[3 * x];
+    a.y = c[3 * x + 1];
+    a.z = c[3 * x + 2];
+    if (r) {
+      b.x = r[3 * x];
+      b.y = r[3 * x + 1];
+      b.z = r[3 * x + 2];
+    } else {
+      b.x = b.y = b.z = 0.f;
+    }
+  } else {
+    a.x = a.y = a.z = inf;
+    b.x = b.y = b.z = 0.f;
+  }
+  a.w = b.w = 0.f;
+  C[x] = a;
+  if (R) R[x] = b;
+}
+
+// MODE 0: distance, mark both endpoints of every pair with d2 <= *thresh.
+// MODE 1: overlap, mark both endpoints of every overlapping pair.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_pair_mask(const uint64_t* list, const unsigned long long* list_count,
+                                                   const unsigned* list_count32, int w1, int w2, const float4* C1, const float4* R1,
+                                                   const float4* C2, const float4* R2, const float* thresh,
+                                                   long long n1, long long n2, int* m1, int* m2) {
+  __shared__ float4 sC[kTileCells];
+  __shared__ float4 sR[kTileCells];
+  __shared__ int sJ[kTileCells];
+  const unsigned long long n = list_count ? *list_count : (unsigned long long)*list_count32;
+  const float tv = MODE == 0 ? *thresh : 0.f;
+  for (unsigned long long it = blockIdx.x; it < n; it += gridDim.x) {
+    const uint64_t e = list[it];
+    const int t1 = (int)((e >> 16) & 0xffff), t2 = (int)(e & 0xffff);
+    const TileGeo g1 = tile_geo(t1, w1), g2 = tile_geo(t2, w2);
+    const int nA = g1.ht * g1.wd, nB = g2.ht * g2.wd;
+    __syncthreads();
+    if ((int)threadIdx.x < nB) {
+      const int j = (g2.row0 + threadIdx.x / g2.wd) * w2 + g2.col0 + threadIdx.x % g2.wd;
+      sC[threadIdx.x] = C2[j];
+      if (MODE == 1) sR[threadIdx.x] = R2[j];
+      sJ[threadIdx.x] = j;
+    }
+    const bool va = (int)threadIdx.x < nA;
+    int i = 0;
+    float4 a = make_float4(0.f, 0.f, 0.f, 0.f), ra = a;
+    if (va) {
+      i = (g1.row0 + threadIdx.x / g1.wd) * w1 + g1.col0 + threadIdx.x % g1.wd;
+      a = C1[i];
+      if (MODE == 1) ra = R1[i];
+    }
+    __syncthreads();
+    if (!va || i >= n1) continue;
+    bool any = false;
+    for (int jj = 0; jj < nB; ++jj) {
+      const float4 b = sC[jj];
+      bool hit;
+      if (MODE == 0) {
+        const float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+        hit = dx * dx + dy * dy + dz * dz <= tv;
+      } else {
+        const float4 rb = sR[jj];
+        hit = fabsf(a.x - b.x) < (ra.x + rb.x) && fabsf(a.y - b.y) < (ra.y + rb.y) &&
+              fabsf(a.z - b.z) < (ra.z + rb.z);
+      }
+      if (hit && sJ[jj] < n2) {
+        any = true;
+        m2[sJ[jj]] = 1;
+      }
+    }
+    if (any) m1[i] = 1;
+  }
+}
+
 template <class T>
 __global__ void k_set_thresh(T* thresh, const Cand* rec, int idx, int use_inf) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *thresh = use_inf ? infinity_v<T>() : from_bits<T>(rec[idx].hi);
+}
+
+// ---------------------------------------------------------------------------
+// Top-k selection without a full sort: histogram radix select on the d2 key
+// narrows [lo, hi] to the bin holding rank k until at most kSmallCap records
+// have key <= hi (or the bin is a single key); those records are compacted and
+// sorted.  hi is then also an upper bound of the k-th key with at least k
+// records at or below it, which is what a screening threshold needs.
+// ---------------------------------------------------------------------------
+constexpr int kSelBins = 4096;
+
+__global__ void k_sel_init(SelState* st, unsigned long long n, unsigned long long k, unsigned long long maxkey,
+                           unsigned small_cap) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  SelState s;
+  s.lo = 0;
+  s.hi = maxkey;
+  s.width = maxkey / kSelBins + 1;
+  s.need = k < n ? k : n;
+  s.below = 0;
+  s.final_hi = maxkey;
+  s.done = (n <= small_cap || s.need == 0) ? 1 : 0;
+  *st = s;
+}
+
+__global__ __launch_bounds__(256) void k_sel_hist(const Cand* cand, unsigned long long n, const SelState* st,
+                                                  unsigned* hist) {
+  __shared__ unsigned h[kSelBins];
+  if (st->done) return;
+  const unsigned long long lo = st->lo, hi = st->hi, width = st->width;
+  for (int b = threadIdx.x; b < kSelBins; b += blockDim.x) h[b] = 0;
+  __syncthreads();
+  for (unsigned long long e = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (unsigned long long)gridDim.x * blockDim.x) {
+    const unsigned long long key = cand[e].hi;
+    if (key >= lo && key <= hi) atomicAdd(&h[(key - lo) / width], 1u);
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < kSelBins; b += blockDim.x)
+    if (h[b]) atomicAdd(&hist[b], h[b]);
+}
+
+__global__ __launch_bounds__(1024) void k_sel_step(SelState* st, unsigned* hist, unsigned small_cap) {
+  __shared__ unsigned long long cum[1024];
+  __shared__ int found;
+  __shared__ unsigned long long found_before;
+  SelState s = *st;
+  if (s.done) return;
+  constexpr int per = kSelBins / 1024;
+  unsigned long long loc = 0;
+  for (int q = 0; q < per; ++q) loc += hist[threadIdx.x * per + q];
+  cum[threadIdx.x] = loc;
+  if (threadIdx.x == 0) found = kSelBins;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const unsigned long long v = threadIdx.x >= (unsigned)off ? cum[threadIdx.x - off] : 0ull;
+    __syncthreads();
+    cum[threadIdx.x] += v;
+    __syncthreads();
+  }
+  // first bin whose inclusive count reaches `need`
+  unsigned long long run = cum[threadIdx.x] - loc;
+  for (int q = 0; q < per; ++q) {
+    const unsigned c = hist[threadIdx.x * per + q];
+    if (run < s.need && run + c >= s.need) {
+      found = threadIdx.x * per + q;
+      found_before = run;
+    }
+    run += c;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int b = found < kSelBins ? found : kSelBins - 1;
+    const unsigned long long before = found < kSelBins ? found_before : 0ull;
+    const unsigned long long nlo = s.lo + (unsigned long long)b * s.width;
+    const unsigned long long span_hi = nlo + s.width - 1;
+    const unsigned long long nhi = span_hi < s.hi ? span_hi : s.hi;
+    s.below += before;
+    s.need -= before;
+    s.lo = nlo;
+    s.hi = nhi;
+    s.final_hi = nhi;
+    if (s.below + hist[b] <= small_cap || nlo == nhi) {
+      s.done = 1;
+    } else {
+      s.width = (nhi - nlo) / kSelBins + 1;
+    }
+    *st = s;
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < kSelBins; b += 1024) hist[b] = 0;
+}
+
+__global__ void k_sel_compact(const Cand* cand, unsigned long long n, const SelState* st, Cand* out,
+                              unsigned long long* out_count, unsigned long long cap) {
+  const unsigned long long fh = st->final_hi;
+  const unsigned long long e = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+  bool keep = false;
+  Cand c;
+  if (e < n) {
+    c = cand[e];
+    keep = c.hi <= fh;
+  }
+  const unsigned long long m = __ballot(keep);
+  if (m) {
+    unsigned long long base = 0;
+    const int leader = __ffsll((long long)m) - 1;
+    if (lane_id() == leader) base = atomicAdd(out_count, (unsigned long long)__popcll(m));
+    base = __shfl(base, leader);
+    const unsigned long long pos = base + __popcll(m & ((1ull << lane_id()) - 1));
+    if (keep && pos < cap) out[pos] = c;
+  }
+}
+
+template <class T>
+__global__ void k_thresh_from_state(T* thresh, const SelState* st) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *thresh = from_bits<T>(st->final_hi);
 }
 
 // ---------------------------------------------------------------------------
@@ -1205,11 +1446,15 @@ hipError_t launch_seed_list(const unsigned long long* best, uint64_t* list, unsi
 template <class T>
 hipError_t launch_dist_list(int nslots, const int* su1, const int* su2, int w1, int w2, const vec4_t<T>* lo1,
                             const vec4_t<T>* hi1, const vec4_t<T>* lo2, const vec4_t<T>* hi2, const T* thresh,
-                            int cull, uint64_t* list, unsigned* list_count, hipStream_t s) {
+                            int cull, uint64_t* list, unsigned* list_count, hipStream_t s, int fma) {
   const long long n = n_tile_pairs(nslots, w1, w2);
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_dist_list<T>, dim3(blocks_for(n, 256)), dim3(256), 0, s, nslots, su1, su2, w1, w2, lo1, hi1,
-                     lo2, hi2, thresh, cull, list, list_count);
+  if (fma)
+    hipLaunchKernelGGL((k_dist_list<T, true>), dim3(blocks_for(n, 256)), dim3(256), 0, s, nslots, su1, su2, w1, w2,
+                       lo1, hi1, lo2, hi2, thresh, cull, list, list_count);
+  else
+    hipLaunchKernelGGL((k_dist_list<T, false>), dim3(blocks_for(n, 256)), dim3(256), 0, s, nslots, su1, su2, w1, w2,
+                       lo1, hi1, lo2, hi2, thresh, cull, list, list_count);
   return hipGetLastError();
 }
 
@@ -1217,11 +1462,15 @@ template <class T>
 hipError_t launch_dist_filter(const uint64_t* list, const unsigned* list_count, unsigned list_cap, const int* su1,
                               const int* su2, int w1, int w2, const vec4_t<T>* C1, const vec4_t<T>* C2,
                               const T* thresh, Cand* out, unsigned long long* out_count, unsigned long long cap,
-                              int grid, hipStream_t s, unsigned long long* pairs_done) {
+                              int grid, hipStream_t s, unsigned long long* pairs_done, int fma) {
   if (list_cap == 0) return hipSuccess;
   const unsigned g = (unsigned)std::min<long long>(grid, (long long)list_cap);
-  hipLaunchKernelGGL(k_dist_filter<T>, dim3(g), dim3(256), 0, s, list, list_count, su1, su2, w1, w2, C1, C2, thresh,
-                     out, out_count, cap, pairs_done);
+  if (fma)
+    hipLaunchKernelGGL((k_dist_filter<T, true>), dim3(g), dim3(256), 0, s, list, list_count, su1, su2, w1, w2, C1, C2,
+                       thresh, out, out_count, cap, pairs_done);
+  else
+    hipLaunchKernelGGL((k_dist_filter<T, false>), dim3(g), dim3(256), 0, s, list, list_count, su1, su2, w1, w2, C1,
+                       C2, thresh, out, out_count, cap, pairs_done);
   return hipGetLastError();
 }
 
@@ -1250,6 +1499,53 @@ hipError_t launch_rerank(Cand* cand, unsigned long long n, const int* su1, const
                      C2);
   return hipGetLastError();
 }
+
+hipError_t launch_pack_cells(const float* c, const float* r, long long n, int w, float4* C, float4* R,
+                             hipStream_t s) {
+  hipLaunchKernelGGL(k_pack_cells, dim3(blocks_for((long long)w * w, 256)), dim3(256), 0, s, c, r, n, w, C, R);
+  return hipGetLastError();
+}
+
+hipError_t launch_pair_mask(int mode, const uint64_t* list, const unsigned long long* list_count,
+                            const unsigned* list_count32, int w1, int w2, const float4* C1, const float4* R1,
+                            const float4* C2, const float4* R2, const float* thresh, long long n1, long long n2,
+                            int* m1, int* m2, int grid, hipStream_t s) {
+  if (mode == 0)
+    hipLaunchKernelGGL(k_pair_mask<0>, dim3(grid), dim3(256), 0, s, list, list_count, list_count32, w1, w2, C1, R1,
+                       C2, R2, thresh, n1, n2, m1, m2);
+  else
+    hipLaunchKernelGGL(k_pair_mask<1>, dim3(grid), dim3(256), 0, s, list, list_count, list_count32, w1, w2, C1, R1,
+                       C2, R2, thresh, n1, n2, m1, m2);
+  return hipGetLastError();
+}
+
+hipError_t select_topk(const Cand* cand, unsigned long long n, unsigned long long k, unsigned long long maxkey,
+                       SelState* st, unsigned* hist, Cand* out, unsigned long long* out_count, unsigned small_cap,
+                       hipStream_t s) {
+  hipLaunchKernelGGL(k_sel_init, dim3(1), dim3(64), 0, s, st, n, k, maxkey, small_cap);
+  hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(kSelBins, 256)), dim3(256), 0, s, hist, 0u,
+                     (unsigned long long)kSelBins);
+  const unsigned g = (unsigned)std::max(1ull, std::min<unsigned long long>(1024, (n + 255) / 256));
+  for (int it = 0; it < 6; ++it) {
+    hipLaunchKernelGGL(k_sel_hist, dim3(g), dim3(256), 0, s, cand, n, st, hist);
+    hipLaunchKernelGGL(k_sel_step, dim3(1), dim3(1024), 0, s, st, hist, small_cap);
+  }
+  if (out) {
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, reinterpret_cast<unsigned*>(out_count), 0u, 2ull);
+    if (n > 0)
+      hipLaunchKernelGGL(k_sel_compact, dim3(blocks_for(n, 256)), dim3(256), 0, s, cand, n, st, out, out_count,
+                         (unsigned long long)small_cap);
+  }
+  return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_thresh_from_state(T* thresh, const SelState* st, hipStream_t s) {
+  hipLaunchKernelGGL(k_thresh_from_state<T>, dim3(1), dim3(64), 0, s, thresh, st);
+  return hipGetLastError();
+}
+template hipError_t launch_thresh_from_state<float>(float*, const SelState*, hipStream_t);
+template hipError_t launch_thresh_from_state<double>(double*, const SelState*, hipStream_t);
 
 hipError_t launch_fill_sentinel(Cand* a, unsigned long long from, unsigned long long n2, hipStream_t s) {
   if (n2 <= from) return hipSuccess;
@@ -1402,11 +1698,11 @@ hipError_t launch_fill_u32(unsigned* p, unsigned value, unsigned long long n, hi
                                           unsigned long long*, hipStream_t);                                       \
   template hipError_t launch_dist_list<T>(int, const int*, const int*, int, int, const vec4_t<T>*,                 \
                                           const vec4_t<T>*, const vec4_t<T>*, const vec4_t<T>*, const T*, int,     \
-                                          uint64_t*, unsigned*, hipStream_t);                                      \
+                                          uint64_t*, unsigned*, hipStream_t, int);                                      \
   template hipError_t launch_dist_filter<T>(const uint64_t*, const unsigned*, unsigned, const int*, const int*,    \
                                             int, int, const vec4_t<T>*, const vec4_t<T>*, const T*, Cand*,         \
                                             unsigned long long*, unsigned long long, int, hipStream_t,             \
-                                            unsigned long long*);            \
+                                            unsigned long long*, int);            \
   template hipError_t launch_set_thresh<T>(T*, const Cand*, int, int, hipStream_t);                                \
   template hipError_t launch_dist_dedup<T>(const Cand*, int, int, int, const int*, const int*, const vec4_t<T>*,   \
                                            const vec4_t<T>*, int, int, int*, int*, int*, int*, vec4_t<T>*,         \

@@ -59,6 +59,7 @@ unsigned long long next_pow2(unsigned long long x) {
 }
 
 constexpr unsigned long long kCandCap = 1ull << 22;  // candidate records kept per screening pass
+constexpr unsigned kSmallCap = 16384;               // records sorted after selection
 
 }  // namespace
 
@@ -191,10 +192,9 @@ int jsi_ctx::dist_topk(int nslots, const int* su1, const int* su2, int w1, int n
                         long long* kk_out, Rerank rr) {
   using V = vec4_t<T>;
   const bool rerank = rr.C1 != nullptr;
-  auto widen = [&]() -> int {
-    if (rerank) HCK(launch_inflate(reinterpret_cast<float*>(thresh_ptr_), rr.absmax, stream));
-    return JSI_OK;
-  };
+  const int fma = rerank ? 1 : 0;  // the FP32SCREEN screen may fuse; its margin covers the rounding
+  const unsigned long long maxkey_t =
+      std::is_same<T, float>::value ? 0x7f800000ull : 0x7ff0000000000000ull;  // +inf bits
   const int tpu1 = ((w1 + kTile - 1) / kTile) * ((w1 + kTile - 1) / kTile);
   const int tpu2 = ((w2 + kTile - 1) / kTile) * ((w2 + kTile - 1) / kTile);
   V *lo1, *hi1, *lo2, *hi2;
@@ -211,42 +211,46 @@ int jsi_ctx::dist_topk(int nslots, const int* su1, const int* su2, int w1, int n
   }
   uint64_t* list;
   unsigned* list_count;
-  unsigned long long* best;
-  unsigned long long* out_count;
+  unsigned long long *best, *out_count, *small_count;
   T* thresh;
-  Cand* cand;
+  Cand *cand, *small;
+  SelState* st;
+  unsigned* hist;
   RCK(buf("d.list", ntp, &list));
   RCK(buf("d.small", 64, &list_count));
   best = reinterpret_cast<unsigned long long*>(list_count + 2);
   out_count = reinterpret_cast<unsigned long long*>(list_count + 4);
   thresh = reinterpret_cast<T*>(list_count + 6);
+  small_count = reinterpret_cast<unsigned long long*>(list_count + 10);
   RCK(buf("d.cand", kCandCap, &cand));
+  RCK(buf("d.topk", kSmallCap, &small));
+  RCK(buf("d.selstate", 1, &st));
+  RCK(buf("d.hist", 4096, &hist));
   thresh_ptr_ = thresh;
-  // seed: exact k-th d2 of the best single tile pair
+  auto widen = [&]() -> int {
+    if (rerank) HCK(launch_inflate(reinterpret_cast<float*>(thresh), rr.absmax, stream));
+    return JSI_OK;
+  };
+  // seed: an upper bound of the k-th d2 from the best single tile pair
   HCK(launch_fill_u32(reinterpret_cast<unsigned*>(best), 0xffffffffu, 2, stream));
   HCK(launch_dist_seed<T>(nslots, su1, su2, w1, w2, lo1, hi1, lo2, hi2, k, best, stream));
   HCK(launch_seed_list(best, list, list_count, w1, w2, stream));
   HCK(launch_set_thresh<T>(thresh, cand, 0, 1, stream));
   HCK(launch_fill_u32(reinterpret_cast<unsigned*>(out_count), 0u, 2, stream));
   HCK(launch_dist_filter<T>(list, list_count, 1, su1, su2, w1, w2, C1, C2, thresh, cand, out_count, kCandCap, 1,
-                            stream));
+                            stream, nullptr, fma));
   unsigned long long ns = 0;
   RCK(d2h(&ns, out_count, sizeof(ns)));
-  if (ns > 0) {
-    if (ns < (unsigned long long)k || ns > kCandCap) {
-      set_err("seed tile pair produced %llu candidates (k=%d)", ns, k);
-      return JSI_ERR_CAPACITY;
-    }
-    const unsigned long long n2 = std::max(next_pow2(ns), (unsigned long long)kSortBlock);
-    HCK(launch_fill_sentinel(cand, ns, n2, stream));
-    HCK(sort_cands(cand, n2, stream));
-    HCK(launch_set_thresh<T>(thresh, cand, k - 1, 0, stream));
+  if (ns >= (unsigned long long)k && ns <= kCandCap) {
+    HCK(select_topk(cand, ns, k, maxkey_t, st, hist, nullptr, nullptr, kSmallCap, stream));
+    HCK(launch_thresh_from_state<T>(thresh, st, stream));
     RCK(widen());
   }
   unsigned long long n = 0;
   for (int attempt = 0;; ++attempt) {
     HCK(launch_fill_u32(list_count, 0u, 1, stream));
-    HCK(launch_dist_list<T>(nslots, su1, su2, w1, w2, lo1, hi1, lo2, hi2, thresh, cull, list, list_count, stream));
+    HCK(launch_dist_list<T>(nslots, su1, su2, w1, w2, lo1, hi1, lo2, hi2, thresh, cull, list, list_count, stream,
+                            fma));
     HCK(launch_fill_u32(reinterpret_cast<unsigned*>(out_count), 0u, 2, stream));
     unsigned long long* pairs_done = prof ? reinterpret_cast<unsigned long long*>(list_count + 8) : nullptr;
     if (prof) {
@@ -254,7 +258,7 @@ int jsi_ctx::dist_topk(int nslots, const int* su1, const int* su2, int w1, int n
       HCK(hipEventRecord(ev[0], stream));
     }
     HCK(launch_dist_filter<T>(list, list_count, (unsigned)ntp, su1, su2, w1, w2, C1, C2, thresh, cand, out_count,
-                              kCandCap, grid, stream, pairs_done));
+                              kCandCap, grid, stream, pairs_done, fma));
     if (prof) HCK(hipEventRecord(ev[1], stream));
     RCK(d2h(&n, out_count, sizeof(n)));
     if (prof) {
@@ -262,27 +266,46 @@ int jsi_ctx::dist_topk(int nslots, const int* su1, const int* su2, int w1, int n
       unsigned long long pd = 0;
       HCK(hipEventElapsedTime(&ms, ev[0], ev[1]));
       RCK(d2h(&pd, pairs_done, sizeof(pd)));
-      KernelStat& st = stats[std::is_same<T, float>::value ? "dist_filter_f32" : "dist_filter_f64"];
-      st.launches += 1;
-      st.ms += ms;
-      st.units += (double)pd;
+      KernelStat& kst = stats[std::is_same<T, float>::value ? "dist_filter_f32" : "dist_filter_f64"];
+      kst.launches += 1;
+      kst.ms += ms;
+      kst.units += (double)pd;
     }
     if (n <= kCandCap) break;
     if (attempt > 8) {
       set_err("distance screen could not bound its candidate set (%llu > %llu)", n, kCandCap);
       return JSI_ERR_CAPACITY;
     }
-    // too many candidates: the k-th smallest of the first kCandCap (all real
-    // pairs) is a tighter upper bound of the k-th d2
-    HCK(sort_cands(cand, kCandCap, stream));
-    HCK(launch_set_thresh<T>(thresh, cand, k - 1, 0, stream));
+    // the first kCandCap candidates are real pairs: their k-th d2 bounds the global k-th
+    HCK(select_topk(cand, kCandCap, k, maxkey_t, st, hist, nullptr, nullptr, kSmallCap, stream));
+    HCK(launch_thresh_from_state<T>(thresh, st, stream));
     RCK(widen());
   }
-  if (rerank) HCK(launch_rerank(cand, n, su1, su2, w1 * w1, w2 * w2, rr.C1, rr.C2, stream));
-  const unsigned long long n2 = std::max(next_pow2(std::max(n, 1ull)), (unsigned long long)kSortBlock);
-  HCK(launch_fill_sentinel(cand, n, n2, stream));
-  HCK(sort_cands(cand, n2, stream));
-  *rec_out = cand;
+  unsigned long long maxkey = maxkey_t;
+  if (rerank) {
+    HCK(launch_rerank(cand, n, su1, su2, w1 * w1, w2 * w2, rr.C1, rr.C2, stream));
+    maxkey = 0x7ff0000000000000ull;
+  }
+  HCK(select_topk(cand, n, k, maxkey, st, hist, small, small_count, kSmallCap, stream));
+  unsigned long long m = 0;
+  RCK(d2h(&m, small_count, sizeof(m)));
+  Cand* rec = small;
+  if (m > kSmallCap) {
+    // more than kSmallCap candidates tie with the k-th key: sort them all
+    const unsigned long long n2 = std::max(next_pow2(std::max(n, 1ull)), (unsigned long long)kSortBlock);
+    if (n2 > kCandCap) {
+      set_err("distance screen: %llu candidates tie at the k-th key", n);
+      return JSI_ERR_CAPACITY;
+    }
+    HCK(launch_fill_sentinel(cand, n, n2, stream));
+    HCK(sort_cands(cand, n2, stream));
+    rec = cand;
+  } else {
+    const unsigned long long n2 = std::max(next_pow2(std::max(m, 1ull)), (unsigned long long)kSortBlock);
+    HCK(launch_fill_sentinel(small, m, n2, stream));
+    HCK(sort_cands(small, n2, stream));
+  }
+  *rec_out = rec;
   *n_out = (long long)n;
   *kk_out = (long long)std::min<unsigned long long>(n, (unsigned long long)k);
   return JSI_OK;
@@ -367,7 +390,11 @@ int jsi_ctx::min_distance(int h1, const jsi_region& r1, int h2, const jsi_region
     const double ts = now_ms();
     long long n = 0;
     if constexpr (std::is_same<T, double>::value) {
-      if (P.precision == JSI_PREC_FP32SCREEN) {
+      // FP32SCREEN screens round 1 (6.8e10 pairs at nuv=512) in float.  Refined
+      // cells shrink below float resolution of the coordinates, so later rounds
+      // (and a round whose float candidate set cannot be bounded) run in FP64.
+      int s32 = (P.precision == JSI_PREC_FP32SCREEN && round == 1) ? 1 : 0;
+      if (s32) {
         float4 *f1, *f2;
         unsigned long long* amax;
         const long long n1 = (long long)nunits1 * w1 * w1, n2 = (long long)nunits2 * w2 * w2;
@@ -381,8 +408,14 @@ int jsi_ctx::min_distance(int h1, const jsi_region& r1, int h2, const jsi_region
         rr.C1 = C1;
         rr.C2 = C2;
         rr.absmax = amax;
-        RCK(dist_topk<float>(nslots, su1, su2, w1, nunits1, f1, w2, nunits2, f2, P.k, P.cull, &rec, &n, &kk, rr));
-      } else {
+        const int rc = dist_topk<float>(nslots, su1, su2, w1, nunits1, f1, w2, nunits2, f2, P.k, P.cull, &rec, &n,
+                                        &kk, rr);
+        if (rc == JSI_ERR_CAPACITY)
+          s32 = 0;
+        else if (rc != JSI_OK)
+          return rc;
+      }
+      if (!s32) {
         RCK(dist_topk<T>(nslots, su1, su2, w1, nunits1, C1, w2, nunits2, C2, P.k, P.cull, &rec, &n, &kk));
       }
     } else {
@@ -961,3 +994,7 @@ int jsi_intersect(jsi_ctx* c, int h1, const jsi_region* r1, int h2, const jsi_re
 }
 
 }  // extern "C"
+
+// used by the operator ABI (jsi_opabi.hip)
+template int jsi_ctx::dist_topk<float>(int, const int*, const int*, int, int, const float4*, int, int, const float4*,
+                                       int, int, Cand**, long long*, long long*, Rerank);

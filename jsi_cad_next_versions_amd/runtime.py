@@ -30,6 +30,8 @@ EXPORTS = [
     "jsi_ctx_sync", "jsi_ctx_profile", "jsi_ctx_kernel_stats", "jsi_surface_upload", "jsi_surface_register_device", "jsi_surface_release",
     "jsi_eval_grid_f32", "jsi_eval_grid_f64", "jsi_min_distance", "jsi_intersect",
 ]
+# include/jsi_ascend_hotspots.h
+OP_EXPORTS = ["jsi_ascend_distance_topk_mask", "jsi_ascend_intersection_mask"]
 
 _lib = None
 
@@ -70,6 +72,9 @@ def load_library(path: str = LIB_PATH):
          C.c_void_p, C.c_int]
     L.jsi_min_distance.argtypes = q
     L.jsi_intersect.argtypes = q
+    L.jsi_ascend_distance_topk_mask.argtypes = [_f32p, C.c_size_t, _f32p, C.c_size_t, C.c_int, _i32p, _i32p,
+                                                C.POINTER(C.c_float)]
+    L.jsi_ascend_intersection_mask.argtypes = [_f32p, _f32p, C.c_int, _f32p, _f32p, C.c_int, _i32p, _i32p]
     _lib = L
     return L
 
@@ -162,3 +167,32 @@ class Context:
         """Returns (Result, last-round overlap pairs (up to cap))."""
         params = params or default_params()
         return self._query(self.L.jsi_intersect, s1, r1, s2, r2, params, cap)
+
+
+# ---------------------------------------------------------------------------
+# Hotspot operator ABI (include/jsi_ascend_hotspots.h)
+# ---------------------------------------------------------------------------
+def distance_topk_mask(c1, c2, topk):
+    """k-th smallest centre distance and the endpoint masks
+    (jsi_ascend_distance_topk_mask).  c1, c2: float32 [n, 3]."""
+    L = load_library()
+    c1 = np.ascontiguousarray(c1, np.float32).reshape(-1)
+    c2 = np.ascontiguousarray(c2, np.float32).reshape(-1)
+    m1 = np.zeros(c1.size // 3, np.int32)
+    m2 = np.zeros(c2.size // 3, np.int32)
+    kth = C.c_float()
+    rc = L.jsi_ascend_distance_topk_mask(c1, c1.size // 3, c2, c2.size // 3, int(topk), m1, m2, C.byref(kth))
+    if rc != 0:
+        raise JsiError(rc, "jsi_ascend_distance_topk_mask failed")
+    return kth.value, m1, m2
+
+
+def intersection_mask(c1, r1, c2, r2):
+    L = load_library()
+    a = [np.ascontiguousarray(x, np.float32).reshape(-1) for x in (c1, r1, c2, r2)]
+    m1 = np.zeros(a[0].size // 3, np.int32)
+    m2 = np.zeros(a[2].size // 3, np.int32)
+    rc = L.jsi_ascend_intersection_mask(a[0], a[1], a[0].size // 3, a[2], a[3], a[2].size // 3, m1, m2)
+    if rc != 0:
+        raise JsiError(rc, "jsi_ascend_intersection_mask failed")
+    return m1, m2

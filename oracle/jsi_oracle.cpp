@@ -842,6 +842,47 @@ int orc_topk_f32(int n1, const float* c1, int n2, const float* c2, int k, jsi_pa
   return int(v.size());
 }
 
+// ---------------------------------------------------------------------------
+// Operator ABI of the reference's hotspot library (V0 mask semantics):
+// jsi_ascend_distance_topk_mask / jsi_ascend_intersection_mask,
+// JSI_CAD_NEXT_VERSION5/src/ascend/hotspots_910b/src/jsi_ascend_hotspots.cpp:33-113.
+// ---------------------------------------------------------------------------
+int orc_distance_topk_mask(const float* c1, size_t n1, const float* c2, size_t n2, int topk, int* m1, int* m2,
+                           float* kth) {
+  if (!c1 || !c2 || !m1 || !m2 || !kth || n1 == 0 || n2 == 0 || topk <= 0) return -1;
+  const size_t total = n1 * n2;
+  std::vector<float> dist(total);
+#pragma omp parallel for schedule(static)
+  for (long long x = 0; x < (long long)total; ++x) {
+    const size_t i = size_t(x) / n2, j = size_t(x) % n2;
+    const float dx = std::fabs(c1[i * 3] - c2[j * 3]);
+    const float dy = std::fabs(c1[i * 3 + 1] - c2[j * 3 + 1]);
+    const float dz = std::fabs(c1[i * 3 + 2] - c2[j * 3 + 2]);
+    dist[x] = std::sqrt(dx * dx + dy * dy + dz * dz);
+  }
+  const size_t k = std::min<size_t>(size_t(topk), total) - 1;
+  std::vector<float> tmp = dist;
+  std::nth_element(tmp.begin(), tmp.begin() + (long long)k, tmp.end());
+  *kth = tmp[k];
+  for (size_t i = 0; i < n1; ++i) m1[i] = 0;
+  for (size_t j = 0; j < n2; ++j) m2[j] = 0;
+  for (size_t i = 0; i < n1; ++i)
+    for (size_t j = 0; j < n2; ++j)
+      if (dist[i * n2 + j] <= *kth) m1[i] = m2[j] = 1;
+  return 0;
+}
+
+int orc_intersection_mask(const float* c1, const float* r1, int n1, const float* c2, const float* r2, int n2, int* m1,
+                          int* m2) {
+  if (!c1 || !r1 || !c2 || !r2 || !m1 || !m2 || n1 <= 0 || n2 <= 0) return -1;
+  for (int i = 0; i < n1; ++i) m1[i] = 0;
+  for (int j = 0; j < n2; ++j) m2[j] = 0;
+  for (int i = 0; i < n1; ++i)
+    for (int j = 0; j < n2; ++j)
+      if (overlap(c1 + 3 * i, r1 + 3 * i, c2 + 3 * j, r2 + 3 * j)) m1[i] = m2[j] = 1;
+  return 0;
+}
+
 int orc_num_threads(void) {
 #ifdef _OPENMP
   return omp_get_max_threads();

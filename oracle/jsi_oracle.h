@@ -51,6 +51,12 @@ int orc_intersect(const jsi_params* params, const int* ivec1, const float* data1
  * given as float AoS arrays; for brute-force checks. */
 int orc_topk_f32(int n1, const float* c1, int n2, const float* c2, int k, jsi_pair* out);
 
+/* operator ABI of the reference's hotspot library (jsi_ascend_hotspots.cpp:33-113) */
+int orc_distance_topk_mask(const float* c1, size_t n1, const float* c2, size_t n2, int topk, int* m1, int* m2,
+                           float* kth);
+int orc_intersection_mask(const float* c1, const float* r1, int n1, const float* c2, const float* r2, int n2, int* m1,
+                          int* m2);
+
 int orc_num_threads(void);
 
 #ifdef __cplusplus

@@ -59,6 +59,9 @@ class Oracle:
         L.orc_intersect.argtypes = q
         L.orc_topk_f32.argtypes = [C.c_int, _f32p, C.c_int, _f32p, C.c_int, C.c_void_p]
         L.orc_num_threads.restype = C.c_int
+        L.orc_distance_topk_mask.argtypes = [_f32p, C.c_size_t, _f32p, C.c_size_t, C.c_int, _i32p, _i32p,
+                                             C.POINTER(C.c_float)]
+        L.orc_intersection_mask.argtypes = [_f32p, _f32p, C.c_int, _f32p, _f32p, C.c_int, _i32p, _i32p]
 
     def num_threads(self):
         return self.lib.orc_num_threads()
@@ -116,6 +119,20 @@ class Oracle:
     def intersect(self, s1, r1, s2, r2, params=None, cap=0):
         params = params or default_params()
         return self._query(self.lib.orc_intersect, s1, r1, s2, r2, params, cap)
+
+    def distance_topk_mask(self, c1, c2, topk):
+        c1 = np.ascontiguousarray(c1, np.float32).reshape(-1)
+        c2 = np.ascontiguousarray(c2, np.float32).reshape(-1)
+        m1, m2 = np.zeros(c1.size // 3, np.int32), np.zeros(c2.size // 3, np.int32)
+        kth = C.c_float()
+        assert self.lib.orc_distance_topk_mask(c1, c1.size // 3, c2, c2.size // 3, topk, m1, m2, C.byref(kth)) == 0
+        return kth.value, m1, m2
+
+    def intersection_mask(self, c1, r1, c2, r2):
+        a = [np.ascontiguousarray(x, np.float32).reshape(-1) for x in (c1, r1, c2, r2)]
+        m1, m2 = np.zeros(a[0].size // 3, np.int32), np.zeros(a[2].size // 3, np.int32)
+        assert self.lib.orc_intersection_mask(a[0], a[1], a[0].size // 3, a[2], a[3], a[2].size // 3, m1, m2) == 0
+        return m1, m2
 
     def topk(self, c1, c2, k):
         buf = (Pair * k)()
@@ -280,3 +297,33 @@ class RefV5:
             if line.startswith("Round ") and "overlap pairs =" in line:
                 counts.append(int(line.split("=")[1]))
         return counts, cap.text
+
+
+ASCEND_REF_SO = os.path.join(HERE, "_ref", "libjsi_ascend_ref.so")
+
+
+class RefHotspots:
+    """The reference's hotspot operator library (CPU implementation), built
+    from src/ascend/hotspots_910b/src/jsi_ascend_hotspots.cpp by oracle/ref.mk."""
+
+    def __init__(self, path=ASCEND_REF_SO):
+        L = self.lib = C.CDLL(path)
+        L.jsi_ascend_distance_topk_mask.argtypes = [_f32p, C.c_size_t, _f32p, C.c_size_t, C.c_int, _i32p, _i32p,
+                                                    C.POINTER(C.c_float)]
+        L.jsi_ascend_intersection_mask.argtypes = [_f32p, _f32p, C.c_int, _f32p, _f32p, C.c_int, _i32p, _i32p]
+
+    def distance_topk_mask(self, c1, c2, topk):
+        c1 = np.ascontiguousarray(c1, np.float32).reshape(-1)
+        c2 = np.ascontiguousarray(c2, np.float32).reshape(-1)
+        m1, m2 = np.zeros(c1.size // 3, np.int32), np.zeros(c2.size // 3, np.int32)
+        kth = C.c_float()
+        assert self.lib.jsi_ascend_distance_topk_mask(c1, c1.size // 3, c2, c2.size // 3, topk, m1, m2,
+                                                      C.byref(kth)) == 0
+        return kth.value, m1, m2
+
+    def intersection_mask(self, c1, r1, c2, r2):
+        a = [np.ascontiguousarray(x, np.float32).reshape(-1) for x in (c1, r1, c2, r2)]
+        m1, m2 = np.zeros(a[0].size // 3, np.int32), np.zeros(a[2].size // 3, np.int32)
+        assert self.lib.jsi_ascend_intersection_mask(a[0], a[1], a[0].size // 3, a[2], a[3], a[2].size // 3,
+                                                     m1, m2) == 0
+        return m1, m2

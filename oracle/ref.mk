@@ -15,7 +15,7 @@ DATA     := inter_case1-1 inter_case1-2 inter_case2-1 inter_case2-2 inter_case3-
             inter_case4-1 inter_case4-2
 OBJS     := $(addprefix $(OUT)/,$(notdir $(SRCS:.cu=.o))) $(addprefix $(OUT)/,$(addsuffix .o,$(DATA)))
 
-all: $(OUT)/libjsi_v5_ref.so
+all: $(OUT)/libjsi_v5_ref.so $(OUT)/libjsi_ascend_ref.so
 
 $(OUT):
 	mkdir -p $(OUT)
@@ -35,3 +35,8 @@ $(OUT)/libjsi_v5_ref.so: $(OBJS)
 	$(CXX) -shared -fopenmp -o $@ $(OBJS)
 
 .PHONY: all
+
+# The reference's hotspot operator library (CPU implementation of the
+# jsi_ascend_* ABI), src/ascend/hotspots_910b/src/jsi_ascend_hotspots.cpp.
+$(OUT)/libjsi_ascend_ref.so: $(REF)/src/ascend/hotspots_910b/src/jsi_ascend_hotspots.cpp | $(OUT)
+	$(CXX) -O3 -std=c++17 -fPIC -shared -I $(REF)/src/ascend/hotspots_910b/include $< -o $@

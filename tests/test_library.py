@@ -36,7 +36,7 @@ def test_library_exports_all_symbols():
     for name in header_functions():
         assert hasattr(lib, name), name
     out = subprocess.run(["nm", "-D", "--defined-only", runtime.LIB_PATH], capture_output=True, text=True).stdout
-    for sym in LEGACY:
+    for sym in LEGACY + runtime.OP_EXPORTS:
         assert re.search(rf"\bT {sym}$", out, re.M), sym
 
 
