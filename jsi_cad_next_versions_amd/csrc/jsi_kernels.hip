@@ -482,47 +482,65 @@ __device__ __forceinline__ uint64_t tp_encode(int slot, int t1, int t2) {
   return ((uint64_t)(uint32_t)slot << 32) | ((uint64_t)t1 << 16) | (uint64_t)t2;
 }
 
-// Seed tile pair of the distance screen: the one with the smallest upper
-// bound on centre d2 among tile pairs holding at least k cell pairs.  Its
-// exact k-th d2 is an upper bound of the global k-th d2.
+// Seed tile pairs of the distance screen.  For every (slot, side-1 tile) the
+// side-2 tile with the smallest upper bound on centre d2 is kept (packed
+// key: upper-bound bits << 32 | side-2 tile), then the kSeedMax best of those
+// form the seed list.  The exact k-th d2 over the seed's cell pairs bounds
+// the global k-th d2 from above; any choice is valid, a good one is tight.
+constexpr int kSeedMax = 64;
+constexpr int kSeedSortMax = 8192;
+
 template <class T>
 __global__ void k_dist_seed(int nslots, const int* su1, const int* su2, int w1, int w2, const vec4_t<T>* lo1,
                             const vec4_t<T>* hi1, const vec4_t<T>* lo2, const vec4_t<T>* hi2, int k,
                             unsigned long long* best) {
   const int tpu1 = tiles_per_unit(w1), tpu2 = tiles_per_unit(w2);
   const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  unsigned long long key = ~0ull;
-  if (gid < (long long)nslots * tpu1 * tpu2) {
-    const TPIndex x = tp_decode(gid, tpu1, tpu2);
-    const TileGeo g1 = tile_geo(x.t1, w1), g2 = tile_geo(x.t2, w2);
-    if ((long long)g1.ht * g1.wd * g2.ht * g2.wd >= k) {
-      const int a = su1[x.slot] * tpu1 + x.t1, b = su2[x.slot] * tpu2 + x.t2;
-      const vec4_t<T> l1 = lo1[a], h1 = hi1[a], l2 = lo2[b], h2 = hi2[b];
-      const T ux = smax(h1.x - l2.x, h2.x - l1.x);
-      const T uy = smax(h1.y - l2.y, h2.y - l1.y);
-      const T uz = smax(h1.z - l2.z, h2.z - l1.z);
-      const T ub2 = ux * ux + uy * uy + uz * uz;
-      key = ((unsigned long long)ord32_up(ub2) << 32) | (unsigned long long)gid;
-    }
-  }
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    const unsigned long long o = __shfl_xor(key, off);
-    key = o < key ? o : key;
-  }
-  if (lane_id() == 0 && key != ~0ull) atomicMin(best, key);
+  if (gid >= (long long)nslots * tpu1 * tpu2) return;
+  (void)k;
+  const TPIndex x = tp_decode(gid, tpu1, tpu2);
+  const int a = su1[x.slot] * tpu1 + x.t1, b = su2[x.slot] * tpu2 + x.t2;
+  const vec4_t<T> l1 = lo1[a], h1 = hi1[a], l2 = lo2[b], h2 = hi2[b];
+  const T ux = smax(h1.x - l2.x, h2.x - l1.x);
+  const T uy = smax(h1.y - l2.y, h2.y - l1.y);
+  const T uz = smax(h1.z - l2.z, h2.z - l1.z);
+  const T ub2 = ux * ux + uy * uy + uz * uz;
+  const unsigned long long key = ((unsigned long long)ord32_up(ub2) << 32) | (unsigned long long)x.t2;
+  atomicMin(&best[(size_t)x.slot * tpu1 + x.t1], key);
 }
 
-__global__ void k_seed_list(const unsigned long long* best, uint64_t* list, unsigned* list_count, int w1, int w2) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  const unsigned long long b = *best;
-  if (b == ~0ull) {
-    *list_count = 0;
-    return;
+// Single workgroup: sort (bound, position) of the per-tile bests and emit the
+// first kSeedMax as tile pairs.  npos = nslots * tpu1 <= kSeedSortMax.
+__global__ __launch_bounds__(1024) void k_seed_list(const unsigned long long* best, int npos, int tpu1, int nseed,
+                                                    uint64_t* list, unsigned* list_count) {
+  __shared__ unsigned long long s[kSeedSortMax];
+  int n2 = 1;
+  while (n2 < npos) n2 <<= 1;
+  for (int e = threadIdx.x; e < n2; e += blockDim.x)
+    s[e] = e < npos ? ((best[e] & 0xffffffff00000000ull) | (unsigned long long)e) : ~0ull;
+  __syncthreads();
+  for (int k = 2; k <= n2; k <<= 1)
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int t = threadIdx.x; t < n2 / 2; t += blockDim.x) {
+        const int i = 2 * t - (t & (j - 1)), l = i + j;
+        const bool asc = (i & k) == 0;
+        const unsigned long long x = s[i], y = s[l];
+        if (asc ? (y < x) : (x < y)) {
+          s[i] = y;
+          s[l] = x;
+        }
+      }
+      __syncthreads();
+    }
+  const int m = nseed < npos ? nseed : npos;
+  if ((int)threadIdx.x < m) {
+    const unsigned long long v = s[threadIdx.x];
+    const int pos = (int)(v & 0xffffffffull);
+    const unsigned long long b = (v >> 32) == 0xffffffffull ? ~0ull : best[pos];
+    if (b != ~0ull) list[threadIdx.x] = tp_encode(pos / tpu1, pos % tpu1, (int)(b & 0xffffffffull));
+    else list[threadIdx.x] = tp_encode(0, 0, 0);
   }
-  const TPIndex x = tp_decode((long long)(b & 0xffffffffull), tiles_per_unit(w1), tiles_per_unit(w2));
-  list[0] = tp_encode(x.slot, x.t1, x.t2);
-  *list_count = 1;
+  if (threadIdx.x == 0) *list_count = (unsigned)m;
 }
 
 // Active tile pairs of the distance screen.  Lower bound on the centre d2 of
@@ -599,12 +617,28 @@ __global__ __launch_bounds__(256) void k_dist_filter(const uint64_t* list, const
                                                      Cand* out, unsigned long long* out_count,
                                                      unsigned long long cap, unsigned long long* pairs_done) {
   constexpr int U = 8;
+  constexpr int kStage = 128;  // candidates staged per wavefront before one global reservation
   __shared__ vec4_t<T> sB[kTileCells];
   __shared__ int sJ[kTileCells];
+  __shared__ Cand stage[4][kStage];
   unsigned long long done = 0;
   const unsigned n = *list_count;
   const T tv = *thresh;
   const int lane = lane_id();
+  const int wv = threadIdx.x >> 6;
+  int wcount = 0;  // staged candidates of this wavefront (wave-uniform)
+  auto flush = [&]() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    unsigned long long base = 0;
+    if (lane == 0) base = atomicAdd(out_count, (unsigned long long)wcount);
+    base = __shfl(base, 0);
+    for (int e = lane; e < wcount; e += 64)
+      if (base + e < cap) out[base + e] = stage[wv][e];
+    __builtin_amdgcn_wave_barrier();
+    wcount = 0;
+  };
   for (unsigned it = blockIdx.x; it < n; it += gridDim.x) {
     const uint64_t e = list[it];
     const int slot = (int)(e >> 32), t1 = (int)((e >> 16) & 0xffff), t2 = (int)(e & 0xffff);
@@ -633,6 +667,7 @@ __global__ __launch_bounds__(256) void k_dist_filter(const uint64_t* list, const
       a = C1[(size_t)u1 * w1 * w1 + i];
     }
     __syncthreads();
+    const uint64_t key_si = pack_idx((uint32_t)slot, (uint32_t)i, 0u);
     for (int j0 = 0; j0 < nBp; j0 += U) {
       T d[U];
 #pragma unroll
@@ -647,23 +682,20 @@ __global__ __launch_bounds__(256) void k_dist_filter(const uint64_t* list, const
         const bool hit = d[q] <= tv;
         const unsigned long long mk = __ballot(hit);
         if (mk) {
-          unsigned long long base = 0;
-          const int leader = __ffsll((long long)mk) - 1;
-          if (lane == leader) base = atomicAdd(out_count, (unsigned long long)__popcll(mk));
-          base = __shfl(base, leader);
+          const int cnt = __popcll(mk);
+          if (wcount + cnt > kStage) flush();
           if (hit) {
-            const unsigned long long pos = base + __popcll(mk & ((1ull << lane) - 1));
-            if (pos < cap) {
-              Cand c;
-              c.hi = ord_bits(d[q]);
-              c.lo = pack_idx((uint32_t)slot, (uint32_t)i, (uint32_t)sJ[j0 + q]);
-              out[pos] = c;
-            }
+            Cand c;
+            c.hi = ord_bits(d[q]);
+            c.lo = key_si | (uint64_t)(uint32_t)sJ[j0 + q];
+            stage[wv][wcount + __popcll(mk & ((1ull << lane) - 1))] = c;
           }
+          wcount += cnt;
         }
       }
     }
   }
+  if (wcount) flush();
   if (pairs_done && threadIdx.x == 0 && done) atomicAdd(pairs_done, done);
 }
 
@@ -1437,9 +1469,10 @@ hipError_t launch_dist_seed(int nslots, const int* su1, const int* su2, int w1, 
   return hipGetLastError();
 }
 
-hipError_t launch_seed_list(const unsigned long long* best, uint64_t* list, unsigned* list_count, int w1, int w2,
-                            hipStream_t s) {
-  hipLaunchKernelGGL(k_seed_list, dim3(1), dim3(64), 0, s, best, list, list_count, w1, w2);
+hipError_t launch_seed_list(const unsigned long long* best, int npos, int tpu1, int nseed, uint64_t* list,
+                            unsigned* list_count, hipStream_t s) {
+  if (npos > kSeedSortMax || nseed > 1024) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_seed_list, dim3(1), dim3(1024), 0, s, best, npos, tpu1, nseed, list, list_count);
   return hipGetLastError();
 }
 

@@ -61,6 +61,7 @@ unsigned long long next_pow2(unsigned long long x) {
 constexpr unsigned long long kCandCap = 1ull << 22;  // candidate records kept per screening pass
 constexpr unsigned kSmallCap = 16384;               // records sorted after selection
 
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -231,14 +232,26 @@ int jsi_ctx::dist_topk(int nslots, const int* su1, const int* su2, int w1, int n
     if (rerank) HCK(launch_inflate(reinterpret_cast<float*>(thresh), rr.absmax, stream));
     return JSI_OK;
   };
-  // seed: an upper bound of the k-th d2 from the best single tile pair
-  HCK(launch_fill_u32(reinterpret_cast<unsigned*>(best), 0xffffffffu, 2, stream));
-  HCK(launch_dist_seed<T>(nslots, su1, su2, w1, w2, lo1, hi1, lo2, hi2, k, best, stream));
-  HCK(launch_seed_list(best, list, list_count, w1, w2, stream));
+  // seed: the exact k-th d2 over the cell pairs of up to kSeed promising tile
+  // pairs (best side-2 tile per side-1 tile, ranked by upper bound) bounds the
+  // global k-th d2 from above
+  const long long npos = (long long)nslots * tpu1;
+  const int pairs_tp = std::min(w1 * w1, kTileCells) * std::min(w2 * w2, kTileCells);
+  const int nseed = (int)std::max(1ull, std::min<unsigned long long>(64ull, kCandCap / (unsigned long long)pairs_tp));
+  if (npos <= 8192) {
+    unsigned long long* bestv;
+    RCK(buf("d.best", (size_t)npos, &bestv));
+    HCK(launch_fill_u32(reinterpret_cast<unsigned*>(bestv), 0xffffffffu, (size_t)npos * 2, stream));
+    HCK(launch_dist_seed<T>(nslots, su1, su2, w1, w2, lo1, hi1, lo2, hi2, k, bestv, stream));
+    HCK(launch_seed_list(bestv, (int)npos, tpu1, nseed, list, list_count, stream));
+  } else {
+    HCK(launch_fill_u32(list_count, 0u, 1, stream));
+  }
+  (void)best;
   HCK(launch_set_thresh<T>(thresh, cand, 0, 1, stream));
   HCK(launch_fill_u32(reinterpret_cast<unsigned*>(out_count), 0u, 2, stream));
-  HCK(launch_dist_filter<T>(list, list_count, 1, su1, su2, w1, w2, C1, C2, thresh, cand, out_count, kCandCap, 1,
-                            stream, nullptr, fma));
+  HCK(launch_dist_filter<T>(list, list_count, (unsigned)nseed, su1, su2, w1, w2, C1, C2, thresh, cand, out_count,
+                            kCandCap, nseed, stream, nullptr, fma));
   unsigned long long ns = 0;
   RCK(d2h(&ns, out_count, sizeof(ns)));
   if (ns >= (unsigned long long)k && ns <= kCandCap) {
