@@ -152,6 +152,46 @@ int jsi_min_distance(jsi_ctx* ctx, int h1, const jsi_region* r1, int h2, const j
 int jsi_intersect(jsi_ctx* ctx, int h1, const jsi_region* r1, int h2, const jsi_region* r2,
                   const jsi_params* params, jsi_result* result, jsi_pair* pairs_out, int pairs_cap);
 
+/* ---- batches and serving ------------------------------------------------ */
+#define JSI_QUERY_DISTANCE 0
+#define JSI_QUERY_INTERSECT 1
+
+/* One query of a batch or a server: surfaces are handles of the context the
+ * batch / server was created on. */
+typedef struct jsi_query {
+  int kind; /* JSI_QUERY_* */
+  int h1, h2;
+  jsi_region r1, r2;
+  jsi_params params;
+} jsi_query;
+
+/* Runs n independent queries with up to `concurrency` of them in flight, each
+ * on its own stream (their kernels overlap on the device).  results[i] is
+ * identical to running queries[i] alone; status[i] (optional) receives each
+ * query's status.  Returns the first non-OK status, or JSI_OK.  No
+ * counterpart in the reference (VERSION5 runs one query at a time). */
+int jsi_run_batch(jsi_ctx* ctx, int n, const jsi_query* queries, jsi_result* results, int* status,
+                  int concurrency);
+
+/* Serving loop: `slots` worker streams take admitted queries as they free up
+ * (continuous batching); submissions are refused with JSI_ERR_BUSY when the
+ * estimated device memory of the queries in flight would exceed hbm_cap_bytes
+ * (0 = no limit) or max_pending requests wait.  Surfaces can be cached by a
+ * caller key with LRU eviction beyond cache_cap_bytes. */
+typedef struct jsi_server jsi_server;
+int jsi_server_create(jsi_ctx* ctx, int slots, size_t hbm_cap_bytes, int max_pending, size_t cache_cap_bytes,
+                      jsi_server** out);
+int jsi_server_destroy(jsi_server* server);
+/* cached upload: same key -> same resident surface (LRU touch) */
+int jsi_server_surface(jsi_server* server, uint64_t key, const int ivec[7], const float* data, int ndata,
+                       int* handle);
+int jsi_server_submit(jsi_server* server, const jsi_query* query, int64_t* ticket);
+/* 1 = done (result filled), 0 = still pending (wait = 0), < 0 = the query's error status */
+int jsi_server_poll(jsi_server* server, int64_t ticket, jsi_result* result, int wait);
+/* counters: admitted, refused (busy), completed, cache hits, cache evictions */
+int jsi_server_counters(jsi_server* server, int64_t* admitted, int64_t* refused, int64_t* completed,
+                        int64_t* cache_hits, int64_t* cache_evictions);
+
 #ifdef __cplusplus
 }
 #endif

@@ -6,7 +6,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libjsi_mi355x.so")
-SOURCES = ["jsi_kernels.hip", "jsi_runtime.hip", "jsi_legacy.hip", "jsi_opabi.hip"]
+SOURCES = ["jsi_kernels.hip", "jsi_runtime.hip", "jsi_legacy.hip", "jsi_opabi.hip", "jsi_serve.hip"]
 
 # -ffp-contract=off: no fused multiply-add, so float results match VERSION5
 # (built without FMA on x86-64) and double results match the FP64 oracle.
@@ -31,7 +31,7 @@ def build(verbose=False, jobs=3):
     bad = [cmd for p, cmd in procs if p.wait() != 0]
     if bad:
         raise RuntimeError("hipcc failed: " + " ".join(bad[0]))
-    cmd = ["hipcc", "--offload-arch=gfx950", "-shared", "-fPIC", "-o", LIB + ".tmp", *objs]
+    cmd = ["hipcc", "--offload-arch=gfx950", "-shared", "-fPIC", "-Wl,--no-undefined", "-o", LIB + ".tmp", *objs]
     subprocess.run(cmd, check=True)
     os.replace(LIB + ".tmp", LIB)
     return LIB

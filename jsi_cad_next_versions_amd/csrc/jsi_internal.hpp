@@ -97,9 +97,15 @@ template <class T>
 hipError_t launch_dist_list(int nslots, const int* su1, const int* su2, int w1, int w2, const vec4_t<T>* lo1,
                             const vec4_t<T>* hi1, const vec4_t<T>* lo2, const vec4_t<T>* hi2, const T* thresh,
                             int cull, uint64_t* list, unsigned* list_count, hipStream_t s, int fma = 0);
-// best[] holds nslots * tiles_per_unit(w1) packed keys (prefill with ~0).
-hipError_t launch_seed_list(const unsigned long long* best, int npos, int tpu1, int nseed, uint64_t* list,
-                            unsigned* list_count, hipStream_t s);
+// Seed threshold: picks up to nseed tile pairs from best[] (npos =
+// nslots * tiles per side-1 unit; ordered = take the first slots), histograms
+// the d2 of their cell pairs and writes the upper edge of the bin holding
+// rank k to *thresh (+inf if they hold fewer than k pairs).
+template <class T>
+hipError_t launch_seed_thresh(const unsigned long long* best, int npos, int tpu1, int nseed, int ordered,
+                              uint64_t* list, unsigned* list_count, unsigned long long* range, unsigned* ghist,
+                              const int* su1, const int* su2, int w1, int w2, const vec4_t<T>* C1,
+                              const vec4_t<T>* C2, int k, int fma, T* thresh, hipStream_t s);
 template <class T>
 hipError_t launch_dist_filter(const uint64_t* list, const unsigned* list_count, unsigned list_cap, const int* su1,
                               const int* su2, int w1, int w2, const vec4_t<T>* C1, const vec4_t<T>* C2,

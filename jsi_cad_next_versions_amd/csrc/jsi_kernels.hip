@@ -487,60 +487,186 @@ __device__ __forceinline__ uint64_t tp_encode(int slot, int t1, int t2) {
 // key: upper-bound bits << 32 | side-2 tile), then the kSeedMax best of those
 // form the seed list.  The exact k-th d2 over the seed's cell pairs bounds
 // the global k-th d2 from above; any choice is valid, a good one is tight.
+template <class T, bool FMA>
+__device__ __forceinline__ T pair_d2(T ax, T ay, T az, T bx, T by, T bz);
+
 constexpr int kSeedMax = 64;
 constexpr int kSeedSortMax = 8192;
+constexpr int kSeedBins = 4096;
 
+// One workgroup per (slot, side-1 tile): the side-2 tile with the smallest
+// upper bound, packed as (upper-bound bits << 32 | side-2 tile).
 template <class T>
-__global__ void k_dist_seed(int nslots, const int* su1, const int* su2, int w1, int w2, const vec4_t<T>* lo1,
-                            const vec4_t<T>* hi1, const vec4_t<T>* lo2, const vec4_t<T>* hi2, int k,
-                            unsigned long long* best) {
+__global__ __launch_bounds__(256) void k_dist_seed(int nslots, const int* su1, const int* su2, int w1, int w2,
+                                                   const vec4_t<T>* lo1, const vec4_t<T>* hi1,
+                                                   const vec4_t<T>* lo2, const vec4_t<T>* hi2, int k,
+                                                   unsigned long long* best) {
+  __shared__ unsigned long long red[4];
   const int tpu1 = tiles_per_unit(w1), tpu2 = tiles_per_unit(w2);
-  const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= (long long)nslots * tpu1 * tpu2) return;
+  const int pos = blockIdx.x;  // slot * tpu1 + t1
+  if (pos >= nslots * tpu1) return;
   (void)k;
-  const TPIndex x = tp_decode(gid, tpu1, tpu2);
-  const int a = su1[x.slot] * tpu1 + x.t1, b = su2[x.slot] * tpu2 + x.t2;
-  const vec4_t<T> l1 = lo1[a], h1 = hi1[a], l2 = lo2[b], h2 = hi2[b];
-  const T ux = smax(h1.x - l2.x, h2.x - l1.x);
-  const T uy = smax(h1.y - l2.y, h2.y - l1.y);
-  const T uz = smax(h1.z - l2.z, h2.z - l1.z);
-  const T ub2 = ux * ux + uy * uy + uz * uz;
-  const unsigned long long key = ((unsigned long long)ord32_up(ub2) << 32) | (unsigned long long)x.t2;
-  atomicMin(&best[(size_t)x.slot * tpu1 + x.t1], key);
+  const int slot = pos / tpu1, t1 = pos % tpu1;
+  const int a = su1[slot] * tpu1 + t1;
+  const vec4_t<T> l1 = lo1[a], h1 = hi1[a];
+  unsigned long long key = ~0ull;
+  for (int t2 = threadIdx.x; t2 < tpu2; t2 += blockDim.x) {
+    const int b = su2[slot] * tpu2 + t2;
+    const vec4_t<T> l2 = lo2[b], h2 = hi2[b];
+    const T ux = smax(h1.x - l2.x, h2.x - l1.x);
+    const T uy = smax(h1.y - l2.y, h2.y - l1.y);
+    const T uz = smax(h1.z - l2.z, h2.z - l1.z);
+    const T ub2 = ux * ux + uy * uy + uz * uz;
+    const unsigned long long kk = ((unsigned long long)ord32_up(ub2) << 32) | (unsigned long long)t2;
+    key = kk < key ? kk : key;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long o = __shfl_xor(key, off);
+    key = o < key ? o : key;
+  }
+  if (lane_id() == 0) red[threadIdx.x >> 6] = key;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long m = red[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = red[w] < m ? red[w] : m;
+    best[pos] = m;
+  }
 }
 
-// Single workgroup: sort (bound, position) of the per-tile bests and emit the
-// first kSeedMax as tile pairs.  npos = nslots * tpu1 <= kSeedSortMax.
+// Single workgroup: the seed tile pairs.  ordered = 1 (refinement rounds, one
+// tile pair per slot, slots in rank order of their parent pairs): the first
+// nseed slots.  Otherwise the nseed positions with the smallest upper bound.
+// range[0..1] receives the key range [ub * 2^-24, ub] of the seed histogram,
+// ub = the largest upper bound among the seeds.
+template <class T>
 __global__ __launch_bounds__(1024) void k_seed_list(const unsigned long long* best, int npos, int tpu1, int nseed,
-                                                    uint64_t* list, unsigned* list_count) {
+                                                    int ordered, uint64_t* list, unsigned* list_count,
+                                                    unsigned long long* range) {
   __shared__ unsigned long long s[kSeedSortMax];
-  int n2 = 1;
-  while (n2 < npos) n2 <<= 1;
-  for (int e = threadIdx.x; e < n2; e += blockDim.x)
-    s[e] = e < npos ? ((best[e] & 0xffffffff00000000ull) | (unsigned long long)e) : ~0ull;
-  __syncthreads();
-  for (int k = 2; k <= n2; k <<= 1)
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int t = threadIdx.x; t < n2 / 2; t += blockDim.x) {
-        const int i = 2 * t - (t & (j - 1)), l = i + j;
-        const bool asc = (i & k) == 0;
-        const unsigned long long x = s[i], y = s[l];
-        if (asc ? (y < x) : (x < y)) {
-          s[i] = y;
-          s[l] = x;
-        }
-      }
-      __syncthreads();
-    }
+  __shared__ unsigned umax;
   const int m = nseed < npos ? nseed : npos;
-  if ((int)threadIdx.x < m) {
-    const unsigned long long v = s[threadIdx.x];
-    const int pos = (int)(v & 0xffffffffull);
-    const unsigned long long b = (v >> 32) == 0xffffffffull ? ~0ull : best[pos];
-    if (b != ~0ull) list[threadIdx.x] = tp_encode(pos / tpu1, pos % tpu1, (int)(b & 0xffffffffull));
-    else list[threadIdx.x] = tp_encode(0, 0, 0);
+  if (threadIdx.x == 0) umax = 0;
+  if (!ordered) {
+    int n2 = 1;
+    while (n2 < npos) n2 <<= 1;
+    for (int e = threadIdx.x; e < n2; e += blockDim.x)
+      s[e] = e < npos ? ((best[e] & 0xffffffff00000000ull) | (unsigned long long)e) : ~0ull;
+    __syncthreads();
+    for (int k = 2; k <= n2; k <<= 1)
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        for (int t = threadIdx.x; t < n2 / 2; t += blockDim.x) {
+          const int i = 2 * t - (t & (j - 1)), l = i + j;
+          const bool asc = (i & k) == 0;
+          const unsigned long long x = s[i], y = s[l];
+          if (asc ? (y < x) : (x < y)) {
+            s[i] = y;
+            s[l] = x;
+          }
+        }
+        __syncthreads();
+      }
+  } else {
+    for (int e = threadIdx.x; e < m; e += blockDim.x)
+      s[e] = (best[e] & 0xffffffff00000000ull) | (unsigned long long)e;
+    __syncthreads();
   }
-  if (threadIdx.x == 0) *list_count = (unsigned)m;
+  if ((int)threadIdx.x < m) {
+    const int pos = (int)(s[threadIdx.x] & 0xffffffffull);
+    const unsigned long long b = best[pos];
+    list[threadIdx.x] = tp_encode(pos / tpu1, pos % tpu1, (int)(b & 0xffffffffull));
+    atomicMax(&umax, (unsigned)(b >> 32));
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    *list_count = (unsigned)m;
+    const T ub = (T)__uint_as_float(umax);
+    range[0] = ord_bits(ub * T(5.9604644775390625e-08));
+    range[1] = ord_bits(ub);
+  }
+}
+
+// Histogram of d2 over the seed tile pairs (no records are written): bins
+// are linear in the key bits over [range0, range1]; keys below range0 land in
+// bin 0, keys above range1 in the last bin.
+template <class T, bool FMA>
+__global__ __launch_bounds__(256) void k_seed_hist(const uint64_t* list, const unsigned* list_count, const int* su1,
+                                                   const int* su2, int w1, int w2, const vec4_t<T>* C1,
+                                                   const vec4_t<T>* C2, const unsigned long long* range,
+                                                   unsigned* ghist) {
+  __shared__ vec4_t<T> sB[kTileCells];
+  __shared__ unsigned h[kSeedBins];
+  const unsigned n = *list_count;
+  const unsigned long long lo = range[0], hi = range[1];
+  const unsigned long long width = (hi > lo ? hi - lo : 0ull) / kSeedBins + 1;
+  for (int b = threadIdx.x; b < kSeedBins; b += blockDim.x) h[b] = 0;
+  for (unsigned it = blockIdx.x; it < n; it += gridDim.x) {
+    const uint64_t e = list[it];
+    const int slot = (int)(e >> 32), t1 = (int)((e >> 16) & 0xffff), t2 = (int)(e & 0xffff);
+    const int u1 = su1[slot], u2 = su2[slot];
+    const TileGeo g1 = tile_geo(t1, w1), g2 = tile_geo(t2, w2);
+    const int nA = g1.ht * g1.wd, nB = g2.ht * g2.wd;
+    __syncthreads();
+    if ((int)threadIdx.x < nB) {
+      const int j = (g2.row0 + threadIdx.x / g2.wd) * w2 + g2.col0 + threadIdx.x % g2.wd;
+      sB[threadIdx.x] = C2[(size_t)u2 * w2 * w2 + j];
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < nA) {
+      const int i = (g1.row0 + threadIdx.x / g1.wd) * w1 + g1.col0 + threadIdx.x % g1.wd;
+      const vec4_t<T> a = C1[(size_t)u1 * w1 * w1 + i];
+      for (int jj = 0; jj < nB; ++jj) {
+        const vec4_t<T> b = sB[jj];
+        const unsigned long long key = ord_bits(pair_d2<T, FMA>(a.x, a.y, a.z, b.x, b.y, b.z));
+        unsigned long long bin = key <= lo ? 0ull : (key - lo) / width;
+        if (bin >= (unsigned long long)kSeedBins) bin = kSeedBins - 1;
+        atomicAdd(&h[bin], 1u);
+      }
+    }
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < kSeedBins; b += blockDim.x)
+    if (h[b]) atomicAdd(&ghist[b], h[b]);
+}
+
+// threshold = upper edge of the bin holding rank k (at least k seed pairs lie
+// at or below it); +inf when the seeds hold fewer than k pairs.
+template <class T>
+__global__ __launch_bounds__(1024) void k_seed_thresh(const unsigned* ghist, const unsigned long long* range, int k,
+                                                      T* thresh) {
+  __shared__ unsigned long long cum[1024];
+  __shared__ int found;
+  constexpr int per = kSeedBins / 1024;
+  unsigned long long loc = 0;
+  for (int q = 0; q < per; ++q) loc += ghist[threadIdx.x * per + q];
+  cum[threadIdx.x] = loc;
+  if (threadIdx.x == 0) found = -1;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const unsigned long long v = threadIdx.x >= (unsigned)off ? cum[threadIdx.x - off] : 0ull;
+    __syncthreads();
+    cum[threadIdx.x] += v;
+    __syncthreads();
+  }
+  unsigned long long run = cum[threadIdx.x] - loc;
+  for (int q = 0; q < per; ++q) {
+    const unsigned c = ghist[threadIdx.x * per + q];
+    if (run < (unsigned long long)k && run + c >= (unsigned long long)k) found = threadIdx.x * per + q;
+    run += c;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (found < 0) {
+      *thresh = infinity_v<T>();
+    } else {
+      const unsigned long long lo = range[0], hi = range[1];
+      const unsigned long long width = (hi > lo ? hi - lo : 0ull) / kSeedBins + 1;
+      const unsigned long long maxbits = sizeof(T) == 4 ? 0x7f800000ull : 0x7ff0000000000000ull;
+      const unsigned long long edge =
+          found == kSeedBins - 1 ? maxbits : lo + (unsigned long long)(found + 1) * width - 1;
+      *thresh = edge >= maxbits ? infinity_v<T>() : from_bits<T>(edge);
+    }
+  }
 }
 
 // Active tile pairs of the distance screen.  Lower bound on the centre d2 of
@@ -1453,6 +1579,11 @@ hipError_t launch_tilebox(const vec4_t<T>* C, const vec4_t<T>* R, int nunits, in
   return hipGetLastError();
 }
 
+static int tiles_per_unit_h(int w) {
+  const int tw = (w + kTile - 1) / kTile;
+  return tw * tw;
+}
+
 static long long n_tile_pairs(int nslots, int w1, int w2) {
   const long long t1 = (w1 + kTile - 1) / kTile, t2 = (w2 + kTile - 1) / kTile;
   return (long long)nslots * t1 * t1 * t2 * t2;
@@ -1462,17 +1593,31 @@ template <class T>
 hipError_t launch_dist_seed(int nslots, const int* su1, const int* su2, int w1, int w2, const vec4_t<T>* lo1,
                             const vec4_t<T>* hi1, const vec4_t<T>* lo2, const vec4_t<T>* hi2, int k,
                             unsigned long long* best, hipStream_t s) {
-  const long long n = n_tile_pairs(nslots, w1, w2);
-  if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_dist_seed<T>, dim3(blocks_for(n, 256)), dim3(256), 0, s, nslots, su1, su2, w1, w2, lo1, hi1,
-                     lo2, hi2, k, best);
+  const long long npos = (long long)nslots * tiles_per_unit_h(w1);
+  if (npos <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_dist_seed<T>, dim3((unsigned)npos), dim3(256), 0, s, nslots, su1, su2, w1, w2, lo1, hi1, lo2,
+                     hi2, k, best);
   return hipGetLastError();
 }
 
-hipError_t launch_seed_list(const unsigned long long* best, int npos, int tpu1, int nseed, uint64_t* list,
-                            unsigned* list_count, hipStream_t s) {
+template <class T>
+hipError_t launch_seed_thresh(const unsigned long long* best, int npos, int tpu1, int nseed, int ordered,
+                              uint64_t* list, unsigned* list_count, unsigned long long* range, unsigned* ghist,
+                              const int* su1, const int* su2, int w1, int w2, const vec4_t<T>* C1,
+                              const vec4_t<T>* C2, int k, int fma, T* thresh, hipStream_t s) {
   if (npos > kSeedSortMax || nseed > 1024) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_seed_list, dim3(1), dim3(1024), 0, s, best, npos, tpu1, nseed, list, list_count);
+  hipLaunchKernelGGL(k_seed_list<T>, dim3(1), dim3(1024), 0, s, best, npos, tpu1, nseed, ordered, list, list_count,
+                     range);
+  hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(kSeedBins, 256)), dim3(256), 0, s, ghist, 0u,
+                     (unsigned long long)kSeedBins);
+  const int m = nseed < npos ? nseed : npos;
+  if (fma)
+    hipLaunchKernelGGL((k_seed_hist<T, true>), dim3(m), dim3(256), 0, s, list, list_count, su1, su2, w1, w2, C1, C2,
+                       range, ghist);
+  else
+    hipLaunchKernelGGL((k_seed_hist<T, false>), dim3(m), dim3(256), 0, s, list, list_count, su1, su2, w1, w2, C1, C2,
+                       range, ghist);
+  hipLaunchKernelGGL(k_seed_thresh<T>, dim3(1), dim3(1024), 0, s, ghist, range, k, thresh);
   return hipGetLastError();
 }
 
@@ -1559,7 +1704,8 @@ hipError_t select_topk(const Cand* cand, unsigned long long n, unsigned long lon
   hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(kSelBins, 256)), dim3(256), 0, s, hist, 0u,
                      (unsigned long long)kSelBins);
   const unsigned g = (unsigned)std::max(1ull, std::min<unsigned long long>(1024, (n + 255) / 256));
-  for (int it = 0; it < 6; ++it) {
+  const int iters = maxkey <= 0xffffffffull ? 3 : 6;  // 4096^3 > 2^31 float keys, 4096^6 > 2^63
+  for (int it = 0; it < iters; ++it) {
     hipLaunchKernelGGL(k_sel_hist, dim3(g), dim3(256), 0, s, cand, n, st, hist);
     hipLaunchKernelGGL(k_sel_step, dim3(1), dim3(1024), 0, s, st, hist, small_cap);
   }
@@ -1737,6 +1883,9 @@ hipError_t launch_fill_u32(unsigned* p, unsigned value, unsigned long long n, hi
                                             unsigned long long*, unsigned long long, int, hipStream_t,             \
                                             unsigned long long*, int);            \
   template hipError_t launch_set_thresh<T>(T*, const Cand*, int, int, hipStream_t);                                \
+  template hipError_t launch_seed_thresh<T>(const unsigned long long*, int, int, int, int, uint64_t*, unsigned*,   \
+                                            unsigned long long*, unsigned*, const int*, const int*, int, int,      \
+                                            const vec4_t<T>*, const vec4_t<T>*, int, int, T*, hipStream_t);        \
   template hipError_t launch_dist_dedup<T>(const Cand*, int, int, int, const int*, const int*, const vec4_t<T>*,   \
                                            const vec4_t<T>*, int, int, int*, int*, int*, int*, vec4_t<T>*,         \
                                            vec4_t<T>*, int*, hipStream_t);                                         \

@@ -120,19 +120,23 @@ int jsi_ctx::h2d(void* dst, const void* src, size_t bytes) {
 
 template <class T>
 int jsi_ctx::surf_dev(int h, SurfDev<T>* out) {
-  if (h < 0 || h >= (int)surf.size() || !surf[h].live) {
+  std::lock_guard<std::mutex> lock(surfaces->mu);
+  auto& tab = surfaces->recs;
+  if (h < 0 || h >= (int)tab.size() || !tab[h].live) {
     set_err("invalid surface handle %d", h);
     return JSI_ERR_INVALID;
   }
-  SurfaceRec& s = surf[h];
+  SurfaceRec& s = tab[h];
   const T* base;
   if (std::is_same<T, float>::value) {
     base = reinterpret_cast<const T*>(s.d_data);
   } else {
     if (!s.d_data64) {
+      // promoted once, synchronously, so every stream sees the finished copy
       HCK(hipMalloc(&s.d_data64, sizeof(double) * (size_t)s.ndata));
       used += sizeof(double) * (size_t)s.ndata;
       HCK(launch_promote<double>(s.d_data, s.d_data64, s.ndata, stream));
+      HCK(hipStreamSynchronize(stream));
     }
     base = reinterpret_cast<const T*>(s.d_data64);
   }
@@ -239,26 +243,20 @@ int jsi_ctx::dist_topk(int nslots, const int* su1, const int* su2, int w1, int n
   const int pairs_tp = std::min(w1 * w1, kTileCells) * std::min(w2 * w2, kTileCells);
   const int nseed = (int)std::max(1ull, std::min<unsigned long long>(64ull, kCandCap / (unsigned long long)pairs_tp));
   if (npos <= 8192) {
-    unsigned long long* bestv;
+    unsigned long long *bestv, *range;
+    unsigned* ghist;
     RCK(buf("d.best", (size_t)npos, &bestv));
-    HCK(launch_fill_u32(reinterpret_cast<unsigned*>(bestv), 0xffffffffu, (size_t)npos * 2, stream));
+    RCK(buf("d.range", 2, &range));
+    RCK(buf("d.ghist", 4096, &ghist));
+    const int ordered = (nslots > 1 && tpu1 == 1 && tpu2 == 1) ? 1 : 0;
     HCK(launch_dist_seed<T>(nslots, su1, su2, w1, w2, lo1, hi1, lo2, hi2, k, bestv, stream));
-    HCK(launch_seed_list(bestv, (int)npos, tpu1, nseed, list, list_count, stream));
+    HCK(launch_seed_thresh<T>(bestv, (int)npos, tpu1, nseed, ordered, list, list_count, range, ghist, su1, su2, w1,
+                              w2, C1, C2, k, fma, thresh, stream));
+    RCK(widen());
   } else {
-    HCK(launch_fill_u32(list_count, 0u, 1, stream));
+    HCK(launch_set_thresh<T>(thresh, cand, 0, 1, stream));
   }
   (void)best;
-  HCK(launch_set_thresh<T>(thresh, cand, 0, 1, stream));
-  HCK(launch_fill_u32(reinterpret_cast<unsigned*>(out_count), 0u, 2, stream));
-  HCK(launch_dist_filter<T>(list, list_count, (unsigned)nseed, su1, su2, w1, w2, C1, C2, thresh, cand, out_count,
-                            kCandCap, nseed, stream, nullptr, fma));
-  unsigned long long ns = 0;
-  RCK(d2h(&ns, out_count, sizeof(ns)));
-  if (ns >= (unsigned long long)k && ns <= kCandCap) {
-    HCK(select_topk(cand, ns, k, maxkey_t, st, hist, nullptr, nullptr, kSmallCap, stream));
-    HCK(launch_thresh_from_state<T>(thresh, st, stream));
-    RCK(widen());
-  }
   unsigned long long n = 0;
   for (int attempt = 0;; ++attempt) {
     HCK(launch_fill_u32(list_count, 0u, 1, stream));
@@ -790,9 +788,12 @@ int validate_query(jsi_ctx* ctx, int h1, const jsi_region* r1, int h2, const jsi
     return JSI_ERR_INVALID;
   }
   for (int h : {h1, h2})
-    if (h < 0 || h >= (int)ctx->surf.size() || !ctx->surf[h].live) {
-      set_err("invalid surface handle %d", h);
-      return JSI_ERR_INVALID;
+    {
+      std::lock_guard<std::mutex> lock(ctx->surfaces->mu);
+      if (h < 0 || h >= (int)ctx->surfaces->recs.size() || !ctx->surfaces->recs[h].live) {
+        set_err("invalid surface handle %d", h);
+        return JSI_ERR_INVALID;
+      }
     }
   return JSI_OK;
 }
@@ -861,15 +862,40 @@ int jsi_ctx_create(int device, size_t hbm_cap_bytes, jsi_ctx** out) {
   return JSI_OK;
 }
 
+// A worker context: own stream and buffers, the parent's surfaces and cap.
+int jsi_ctx_create_worker(jsi_ctx* parent, jsi_ctx** out) {
+  if (!parent || !out) return JSI_ERR_INVALID;
+  HCK(hipSetDevice(parent->device));
+  jsi_ctx* c = new jsi_ctx();
+  c->device = parent->device;
+  c->worker = true;
+  c->surfaces = parent->surfaces;
+  c->cap = parent->cap;
+  c->ncu = parent->ncu;
+  c->grid = parent->grid;
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete c;
+    set_err("hipStreamCreate: %s", hipGetErrorString(e));
+    return JSI_ERR_DEVICE;
+  }
+  *out = c;
+  return JSI_OK;
+}
+
 int jsi_ctx_destroy(jsi_ctx* c) {
   if (!c) return JSI_ERR_INVALID;
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
   for (auto& kv : c->bufs)
     if (kv.second.p) (void)hipFree(kv.second.p);
-  for (auto& s : c->surf) {
-    if (s.owned && s.d_data) (void)hipFree(s.d_data);
-    if (s.d_data64) (void)hipFree(s.d_data64);
+  if (!c->worker) {
+    std::lock_guard<std::mutex> lock(c->surfaces->mu);
+    for (auto& s : c->surfaces->recs) {
+      if (s.owned && s.d_data) (void)hipFree(s.d_data);
+      if (s.d_data64) (void)hipFree(s.d_data64);
+    }
+    c->surfaces->recs.clear();
   }
   for (hipEvent_t e : c->ev)
     if (e) (void)hipEventDestroy(e);
@@ -924,8 +950,9 @@ int jsi_surface_upload(jsi_ctx* c, const int ivec[7], const float* data, int nda
   HCK(hipMalloc(&s.d_data, bytes));
   c->used += bytes;
   HCK(hipMemcpy(s.d_data, data, bytes, hipMemcpyHostToDevice));
-  c->surf.push_back(s);
-  *handle = (int)c->surf.size() - 1;
+  std::lock_guard<std::mutex> lock(c->surfaces->mu);
+  c->surfaces->recs.push_back(s);
+  *handle = (int)c->surfaces->recs.size() - 1;
   return JSI_OK;
 }
 
@@ -944,15 +971,19 @@ int jsi_surface_register_device(jsi_ctx* c, const int* d_ivec, const float* d_da
   s.ndata = ndata;
   s.owned = false;
   s.d_data = const_cast<float*>(d_data);
-  c->surf.push_back(s);
-  *handle = (int)c->surf.size() - 1;
+  std::lock_guard<std::mutex> lock(c->surfaces->mu);
+  c->surfaces->recs.push_back(s);
+  *handle = (int)c->surfaces->recs.size() - 1;
   return JSI_OK;
 }
 
 int jsi_surface_release(jsi_ctx* c, int h) {
-  if (!c || h < 0 || h >= (int)c->surf.size() || !c->surf[h].live) return JSI_ERR_INVALID;
-  SurfaceRec& s = c->surf[h];
+  if (!c) return JSI_ERR_INVALID;
   HCK(hipStreamSynchronize(c->stream));
+  std::lock_guard<std::mutex> lock(c->surfaces->mu);
+  auto& tab = c->surfaces->recs;
+  if (h < 0 || h >= (int)tab.size() || !tab[h].live) return JSI_ERR_INVALID;
+  SurfaceRec& s = tab[h];
   if (s.owned && s.d_data) {
     HCK(hipFree(s.d_data));
     c->used -= sizeof(float) * (size_t)s.ndata;

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 
 #include <map>
+#include <memory>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -28,6 +30,13 @@ struct SurfaceRec {
 // One context per device: a stream, a grow-only set of named device buffers
 // (reused across queries, so steady-state queries allocate nothing) and the
 // resident surfaces.  `cap` bounds the bytes the context may hold.
+// Resident surfaces, shared by a context and the worker contexts of its
+// batch runner / server (each worker has its own stream and buffers).
+struct SurfaceTable {
+  std::mutex mu;
+  std::vector<SurfaceRec> recs;
+};
+
 // FP32SCREEN: float centres are screened, FP64 centres re-rank the candidates.
 struct RerankSpec {
   const double4* C1 = nullptr;
@@ -51,7 +60,8 @@ struct jsi_ctx {
   size_t used = 0;
   int ncu = 0;
   int grid = 0;  // persistent-kernel grid size
-  std::vector<SurfaceRec> surf;
+  std::shared_ptr<SurfaceTable> surfaces = std::make_shared<SurfaceTable>();
+  bool worker = false;  // shares its parent's surfaces; destroy frees only its own buffers
   std::map<std::string, DevBuf> bufs;
   void* thresh_ptr_ = nullptr;  // device threshold of the screen in flight
 
@@ -88,3 +98,6 @@ struct jsi_ctx {
   template <class T>
   int eval_grid(int h, T ustep, T vstep, T ustart, T vstart, int nuv, T* host_out);
 };
+
+// Worker context sharing the parent's surfaces (batch runner / server).
+extern "C" int jsi_ctx_create_worker(jsi_ctx* parent, jsi_ctx** out);
