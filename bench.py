@@ -45,6 +45,25 @@ def run_form(ctx, a, r1, b, r2, params, steps, warmup, profile=False):
     return res, wall, lat
 
 
+def run_served(ctx, q, n, slots, warmup):
+    """n copies of the query through the serving loop with `slots` worker
+    streams; latency = submit -> result."""
+    from jsi_cad_next_versions_amd.runtime import Server
+    with Server(ctx, slots=slots) as srv:
+        for t in [srv.submit(q) for _ in range(warmup * slots)]:
+            srv.poll(t)
+        t0 = time.perf_counter()
+        subs = []
+        for _ in range(n):
+            subs.append((srv.submit(q), time.perf_counter()))
+        lat, res = [], None
+        for t, ts in subs:
+            res = srv.poll(t)
+            lat.append((time.perf_counter() - ts) * 1e3)
+        wall = time.perf_counter() - t0
+    return res, wall, lat
+
+
 def cpu_baseline(a, r1, b, r2, k, queries):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from jsi_cad_next_versions_amd._abi import default_params
@@ -71,6 +90,7 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=5)
     ap.add_argument("--cpu-queries", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--concurrency", type=int, default=4, help="worker streams of the served (batched) form")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each GPU would serve its own "
@@ -89,9 +109,11 @@ def main():
 
     # chosen form: FP32 screening with FP64 re-ranking, tile culling
     p = default_params(k=args.k, precision=args.precision, cull=1)
-    res, wall, lat = run_form(ctx, a, r1, b, r2, p, args.steps, args.warmup, profile=True)
+    ares, awall, alat = run_form(ctx, a, r1, b, r2, p, args.steps, args.warmup, profile=True)
     stats = {name: ctx.kernel_stats(name) for name in ("dist_filter_f32", "dist_filter_f64")}
     ctx.profile(False)
+    # served / batched: the same queries through the serving loop, several in flight
+    res, wall, lat = run_served(ctx, ctx.query(0, a, r1, b, r2, p), args.steps, args.concurrency, 1)
     qps = args.steps / wall
     # drop-in EXACT32 mode (bit-exact with the reference, FP32 end to end)
     pe = default_params(k=args.k, precision="exact32", cull=1)
@@ -125,9 +147,11 @@ def main():
         "data": "synthetic (seeded SplitMix64 surfaces)",
         "config": {"workload": f"min-distance, rational bicubic 64x64 hard-wave pair, nuv={args.nuv}, "
                                f"k={args.k}, refine nuv=16, 4 rounds",
-                   "precision": args.precision, "cull": 1},
+                   "precision": args.precision, "cull": 1, "served_streams": args.concurrency},
         "p50_ms": pct(lat, 50),
         "p99_ms": pct(lat, 99),
+        "concurrency": args.concurrency,
+        "alone": {"queries_per_s": args.steps / awall, "p50_ms": pct(alat, 50), "p99_ms": pct(alat, 99)},
         "cells_per_s": cells * qps,
         "distance": res.distance,
         "dist_err_vs_fp64": dist_err,

@@ -115,3 +115,13 @@ def pairs_to_numpy(buf, n):
     import numpy as np
     dt = np.dtype([("d2", "<f8"), ("slot", "<i4"), ("i", "<i4"), ("j", "<i4"), ("pad_", "<i4")])
     return np.frombuffer(bytes(memoryview(buf).cast("B")[: n * C.sizeof(Pair)]), dtype=dt).copy()
+
+
+QUERY_DISTANCE = 0
+QUERY_INTERSECT = 1
+
+
+class Query(C.Structure):
+    """jsi_query: one query of a batch or a server."""
+    _fields_ = [("kind", C.c_int), ("h1", C.c_int), ("h2", C.c_int), ("r1", Region), ("r2", Region),
+                ("params", Params)]

@@ -537,8 +537,7 @@ __global__ __launch_bounds__(256) void k_dist_seed(int nslots, const int* su1, c
 // Single workgroup: the seed tile pairs.  ordered = 1 (refinement rounds, one
 // tile pair per slot, slots in rank order of their parent pairs): the first
 // nseed slots.  Otherwise the nseed positions with the smallest upper bound.
-// range[0..1] receives the key range [ub * 2^-24, ub] of the seed histogram,
-// ub = the largest upper bound among the seeds.
+// range[0..1] is reset for the min/max pass of the seed histogram.
 template <class T>
 __global__ __launch_bounds__(1024) void k_seed_list(const unsigned long long* best, int npos, int tpu1, int nseed,
                                                     int ordered, uint64_t* list, unsigned* list_count,
@@ -580,26 +579,31 @@ __global__ __launch_bounds__(1024) void k_seed_list(const unsigned long long* be
   __syncthreads();
   if (threadIdx.x == 0) {
     *list_count = (unsigned)m;
-    const T ub = (T)__uint_as_float(umax);
-    range[0] = ord_bits(ub * T(5.9604644775390625e-08));
-    range[1] = ord_bits(ub);
+    range[0] = ~0ull;  // min / max d2 bits over the seed pairs (k_seed_hist<..., true>)
+    range[1] = 0ull;
   }
 }
 
-// Histogram of d2 over the seed tile pairs (no records are written): bins
-// are linear in the key bits over [range0, range1]; keys below range0 land in
-// bin 0, keys above range1 in the last bin.
-template <class T, bool FMA>
+// Two passes over the d2 of the seed tile pairs (no records are written).
+// MINMAX: the smallest and largest key.  Otherwise: a histogram with bins
+// linear in the key bits over [min, max], so its resolution follows the
+// spread of the values (refined cells have d2 values within 1e-5 of each other).
+template <class T, bool FMA, bool MINMAX>
 __global__ __launch_bounds__(256) void k_seed_hist(const uint64_t* list, const unsigned* list_count, const int* su1,
                                                    const int* su2, int w1, int w2, const vec4_t<T>* C1,
-                                                   const vec4_t<T>* C2, const unsigned long long* range,
+                                                   const vec4_t<T>* C2, unsigned long long* range,
                                                    unsigned* ghist) {
   __shared__ vec4_t<T> sB[kTileCells];
-  __shared__ unsigned h[kSeedBins];
+  __shared__ unsigned h[MINMAX ? 1 : kSeedBins];
   const unsigned n = *list_count;
-  const unsigned long long lo = range[0], hi = range[1];
-  const unsigned long long width = (hi > lo ? hi - lo : 0ull) / kSeedBins + 1;
-  for (int b = threadIdx.x; b < kSeedBins; b += blockDim.x) h[b] = 0;
+  unsigned long long lo = 0, width = 1;
+  if (!MINMAX) {
+    lo = range[0];
+    const unsigned long long hi = range[1];
+    width = (hi > lo ? hi - lo : 0ull) / kSeedBins + 1;
+    for (int b = threadIdx.x; b < kSeedBins; b += blockDim.x) h[b] = 0;
+  }
+  unsigned long long kmin = ~0ull, kmax = 0ull;
   for (unsigned it = blockIdx.x; it < n; it += gridDim.x) {
     const uint64_t e = list[it];
     const int slot = (int)(e >> 32), t1 = (int)((e >> 16) & 0xffff), t2 = (int)(e & 0xffff);
@@ -618,11 +622,29 @@ __global__ __launch_bounds__(256) void k_seed_hist(const uint64_t* list, const u
       for (int jj = 0; jj < nB; ++jj) {
         const vec4_t<T> b = sB[jj];
         const unsigned long long key = ord_bits(pair_d2<T, FMA>(a.x, a.y, a.z, b.x, b.y, b.z));
-        unsigned long long bin = key <= lo ? 0ull : (key - lo) / width;
-        if (bin >= (unsigned long long)kSeedBins) bin = kSeedBins - 1;
-        atomicAdd(&h[bin], 1u);
+        if (MINMAX) {
+          kmin = key < kmin ? key : kmin;
+          kmax = key > kmax ? key : kmax;
+        } else {
+          unsigned long long bin = key <= lo ? 0ull : (key - lo) / width;
+          if (bin >= (unsigned long long)kSeedBins) bin = kSeedBins - 1;
+          atomicAdd(&h[bin], 1u);
+        }
       }
     }
+  }
+  if (MINMAX) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const unsigned long long a = __shfl_xor(kmin, off), b = __shfl_xor(kmax, off);
+      kmin = a < kmin ? a : kmin;
+      kmax = b > kmax ? b : kmax;
+    }
+    if (lane_id() == 0) {
+      if (kmin != ~0ull) atomicMin(&range[0], kmin);
+      if (kmax) atomicMax(&range[1], kmax);
+    }
+    return;
   }
   __syncthreads();
   for (int b = threadIdx.x; b < kSeedBins; b += blockDim.x)
@@ -662,8 +684,8 @@ __global__ __launch_bounds__(1024) void k_seed_thresh(const unsigned* ghist, con
       const unsigned long long lo = range[0], hi = range[1];
       const unsigned long long width = (hi > lo ? hi - lo : 0ull) / kSeedBins + 1;
       const unsigned long long maxbits = sizeof(T) == 4 ? 0x7f800000ull : 0x7ff0000000000000ull;
-      const unsigned long long edge =
-          found == kSeedBins - 1 ? maxbits : lo + (unsigned long long)(found + 1) * width - 1;
+      unsigned long long edge = lo + (unsigned long long)(found + 1) * width - 1;
+      if (edge > hi) edge = hi;  // hi is the largest seed key, so rank k is at or below it
       *thresh = edge >= maxbits ? infinity_v<T>() : from_bits<T>(edge);
     }
   }
@@ -1611,12 +1633,17 @@ hipError_t launch_seed_thresh(const unsigned long long* best, int npos, int tpu1
   hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(kSeedBins, 256)), dim3(256), 0, s, ghist, 0u,
                      (unsigned long long)kSeedBins);
   const int m = nseed < npos ? nseed : npos;
-  if (fma)
-    hipLaunchKernelGGL((k_seed_hist<T, true>), dim3(m), dim3(256), 0, s, list, list_count, su1, su2, w1, w2, C1, C2,
-                       range, ghist);
-  else
-    hipLaunchKernelGGL((k_seed_hist<T, false>), dim3(m), dim3(256), 0, s, list, list_count, su1, su2, w1, w2, C1, C2,
-                       range, ghist);
+  if (fma) {
+    hipLaunchKernelGGL((k_seed_hist<T, true, true>), dim3(m), dim3(256), 0, s, list, list_count, su1, su2, w1, w2, C1,
+                       C2, range, ghist);
+    hipLaunchKernelGGL((k_seed_hist<T, true, false>), dim3(m), dim3(256), 0, s, list, list_count, su1, su2, w1, w2,
+                       C1, C2, range, ghist);
+  } else {
+    hipLaunchKernelGGL((k_seed_hist<T, false, true>), dim3(m), dim3(256), 0, s, list, list_count, su1, su2, w1, w2,
+                       C1, C2, range, ghist);
+    hipLaunchKernelGGL((k_seed_hist<T, false, false>), dim3(m), dim3(256), 0, s, list, list_count, su1, su2, w1, w2,
+                       C1, C2, range, ghist);
+  }
   hipLaunchKernelGGL(k_seed_thresh<T>, dim3(1), dim3(1024), 0, s, ghist, range, k, thresh);
   return hipGetLastError();
 }

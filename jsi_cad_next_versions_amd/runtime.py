@@ -14,8 +14,8 @@ import os
 
 import numpy as np
 
-from ._abi import (JSI_OK, MAX_ROUNDS, Pair, Params, Region, Result, default_params,  # noqa: F401
-                   pairs_to_numpy)
+from ._abi import (JSI_ERR_BUSY, JSI_OK, MAX_ROUNDS, QUERY_DISTANCE, QUERY_INTERSECT, Pair, Params,  # noqa: F401
+                   Query, Region, Result, default_params, pairs_to_numpy)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libjsi_mi355x.so")
@@ -28,7 +28,9 @@ _i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
 EXPORTS = [
     "jsi_params_default", "jsi_status_string", "jsi_last_error", "jsi_ctx_create", "jsi_ctx_destroy",
     "jsi_ctx_sync", "jsi_ctx_profile", "jsi_ctx_kernel_stats", "jsi_surface_upload", "jsi_surface_register_device", "jsi_surface_release",
-    "jsi_eval_grid_f32", "jsi_eval_grid_f64", "jsi_min_distance", "jsi_intersect",
+    "jsi_eval_grid_f32", "jsi_eval_grid_f64", "jsi_min_distance", "jsi_intersect", "jsi_run_batch",
+    "jsi_server_create", "jsi_server_destroy", "jsi_server_surface", "jsi_server_submit", "jsi_server_poll",
+    "jsi_server_counters",
 ]
 # include/jsi_ascend_hotspots.h
 OP_EXPORTS = ["jsi_ascend_distance_topk_mask", "jsi_ascend_intersection_mask"]
@@ -72,6 +74,14 @@ def load_library(path: str = LIB_PATH):
          C.c_void_p, C.c_int]
     L.jsi_min_distance.argtypes = q
     L.jsi_intersect.argtypes = q
+    L.jsi_run_batch.argtypes = [C.c_void_p, C.c_int, C.POINTER(Query), C.POINTER(Result), C.POINTER(C.c_int),
+                                C.c_int]
+    L.jsi_server_create.argtypes = [C.c_void_p, C.c_int, C.c_size_t, C.c_int, C.c_size_t, C.POINTER(C.c_void_p)]
+    L.jsi_server_destroy.argtypes = [C.c_void_p]
+    L.jsi_server_surface.argtypes = [C.c_void_p, C.c_uint64, _i32p, _f32p, C.c_int, C.POINTER(C.c_int)]
+    L.jsi_server_submit.argtypes = [C.c_void_p, C.POINTER(Query), C.POINTER(C.c_int64)]
+    L.jsi_server_poll.argtypes = [C.c_void_p, C.c_int64, C.POINTER(Result), C.c_int]
+    L.jsi_server_counters.argtypes = [C.c_void_p] + [C.POINTER(C.c_int64)] * 5
     L.jsi_ascend_distance_topk_mask.argtypes = [_f32p, C.c_size_t, _f32p, C.c_size_t, C.c_int, _i32p, _i32p,
                                                 C.POINTER(C.c_float)]
     L.jsi_ascend_intersection_mask.argtypes = [_f32p, _f32p, C.c_int, _f32p, _f32p, C.c_int, _i32p, _i32p]
@@ -167,6 +177,71 @@ class Context:
         """Returns (Result, last-round overlap pairs (up to cap))."""
         params = params or default_params()
         return self._query(self.L.jsi_intersect, s1, r1, s2, r2, params, cap)
+
+    def query(self, kind, s1, r1, s2, r2, params=None):
+        """A jsi_query for run_batch / Server.submit (uploads the surfaces)."""
+        return Query(kind, self.upload(s1), self.upload(s2), r1, r2, params or default_params())
+
+    def run_batch(self, queries, concurrency=8):
+        """Runs the queries with up to `concurrency` in flight; returns (results, statuses)."""
+        n = len(queries)
+        qa = (Query * max(n, 1))(*queries)
+        ra = (Result * max(n, 1))()
+        sa = (C.c_int * max(n, 1))()
+        self.L.jsi_run_batch(self.h, n, qa, ra, sa, concurrency)
+        return [ra[i] for i in range(n)], [sa[i] for i in range(n)]
+
+
+class Server:
+    """Serving loop (jsi_server): `slots` worker streams, admission by device
+    memory, an LRU cache of resident surfaces keyed by the caller."""
+
+    def __init__(self, ctx: Context, slots=8, hbm_cap_bytes=0, max_pending=0, cache_cap_bytes=0):
+        self.ctx, self.L = ctx, ctx.L
+        h = C.c_void_p()
+        _check(self.L, self.L.jsi_server_create(ctx.h, slots, hbm_cap_bytes, max_pending, cache_cap_bytes,
+                                                C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.L.jsi_server_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def surface(self, key, s):
+        h = C.c_int()
+        _check(self.L, self.L.jsi_server_surface(self.h, key, s.ivec, s.data, s.ndata, C.byref(h)))
+        return h.value
+
+    def submit(self, query):
+        """Returns a ticket, or None when admission refuses (JSI_ERR_BUSY)."""
+        t = C.c_int64()
+        rc = self.L.jsi_server_submit(self.h, C.byref(query), C.byref(t))
+        if rc == JSI_ERR_BUSY:
+            return None
+        _check(self.L, rc)
+        return t.value
+
+    def poll(self, ticket, wait=True):
+        """Result when done, None while pending."""
+        res = Result()
+        rc = self.L.jsi_server_poll(self.h, ticket, C.byref(res), int(wait))
+        if rc == 0:
+            return None
+        if rc < 0:
+            raise JsiError(rc, self.L.jsi_last_error().decode(errors="replace"))
+        return res
+
+    def counters(self):
+        v = [C.c_int64() for _ in range(5)]
+        _check(self.L, self.L.jsi_server_counters(self.h, *[C.byref(x) for x in v]))
+        return dict(zip(["admitted", "refused", "completed", "cache_hits", "cache_evictions"], [x.value for x in v]))
 
 
 # ---------------------------------------------------------------------------
