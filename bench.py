@@ -82,7 +82,7 @@ def cpu_baseline(a, r1, b, r2, k, queries):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--nuv", type=int, default=512)
     ap.add_argument("--k", type=int, default=4096)
@@ -90,7 +90,7 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=5)
     ap.add_argument("--cpu-queries", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--concurrency", type=int, default=4, help="worker streams of the served (batched) form")
+    ap.add_argument("--concurrency", type=int, default=8, help="worker streams of the served (batched) form")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each GPU would serve its own "

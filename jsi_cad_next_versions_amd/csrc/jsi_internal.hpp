@@ -75,6 +75,12 @@ template <class T>
 hipError_t launch_points(const SurfDev<T>& sd, const T* tabU, const int* spanU, const T* tabV, const int* spanV,
                          int nunits, int nuv, int mode, T* out, unsigned long long* kmax, hipStream_t s);
 
+// Fused evaluation + distance centres of nunits regions (one workgroup per
+// 16x16-cell tile); C as launch_aabb(isect = 0) writes it.
+template <class T>
+hipError_t launch_eval_centres(const SurfDev<T>& sd, const vec4_t<T>* reg, int nunits, int nuv, vec4_t<T>* C,
+                               hipStream_t s);
+
 // Cell AABBs of every unit from its SoA points.  isect = 0: distance centres
 // (construct_aabbs_distance_host); isect = 1: centre + padded radius
 // (construct_aabbs_host) with the compute_K padding from kmax.

@@ -321,6 +321,111 @@ __global__ __launch_bounds__(256) void k_points(SurfDev<T> sd, const T* tabU, co
   }
 }
 
+// Fused evaluation + distance AABB centres.  One workgroup per 16x16-cell
+// tile of a unit: it computes the basis values (A2.3, order 0 only: the
+// values row ders[0] does not depend on the derivative order) of the tile's
+// 17 columns and 17 rows into LDS, evaluates the 17x17 corner points
+// (EvaluateOneUV's S row, evaluation.cu:281-300 / :313-348 + the 1/W scaling
+// of the quotient rule), and reduces each cell's four corners to its centre
+// (construct_aabbs_distance_host, distance_analysis.cu:45-86).  Every value
+// is computed with the same operations as the unfused kernels.
+template <class T, int MAXD, bool RAT>
+__global__ __launch_bounds__(256) void k_eval_centres(SurfDev<T> sd, const vec4_t<T>* regions, int nunits, int nuv,
+                                                      vec4_t<T>* C) {
+  constexpr int E = kTile + 1;  // corner points per tile edge
+  __shared__ T sNu[E][MAXD + 1], sNv[E][MAXD + 1];
+  __shared__ int sSu[E], sSv[E];
+  __shared__ T sP[E * E][3];
+  const int w = nuv - 1;
+  const int tw = (w + kTile - 1) / kTile, tpu = tw * tw;
+  const int unit = blockIdx.x / tpu, t = blockIdx.x % tpu;
+  if (unit >= nunits) return;
+  const int row0 = (t / tw) * kTile, col0 = (t % tw) * kTile;
+  const int ht = min(kTile, w - row0), wd = min(kTile, w - col0);  // cells
+  const int p = sd.degu, q = sd.degv;
+  const vec4_t<T> r = regions[unit];
+  if (!sd.zero && threadIdx.x < 2 * E) {
+    const int dir = threadIdx.x >= E;
+    const int e = dir ? threadIdx.x - E : threadIdx.x;
+    if (e <= (dir ? ht : wd)) {
+      const T start = dir ? r.y : r.x;
+      const T step = ((dir ? r.w : r.z) - start) / T(nuv);
+      const int idx = (dir ? row0 : col0) + e;
+      const T tt = step * T(idx) + start;
+      const int deg = dir ? q : p;
+      const T* knots = dir ? sd.V : sd.U;
+      const int span = find_span(dir ? sd.nkv : sd.nku, deg, tt, knots);
+      T d[3][MAXD + 1];
+      basis_ders<T, MAXD>(span, deg, tt, knots, 0, d);
+      for (int j = 0; j <= deg; ++j) (dir ? sNv : sNu)[e][j] = d[0][j];
+      (dir ? sSv : sSu)[e] = span;
+    }
+  }
+  __syncthreads();
+  for (int x = threadIdx.x; x < E * E; x += blockDim.x) {
+    const int ey = x / E, ex = x % E;
+    T S[3] = {T(0), T(0), T(0)};
+    if (ey <= ht && ex <= wd && !sd.zero) {
+      const int su = sSu[ex], sv = sSv[ey];
+      T acc[4] = {T(0), T(0), T(0), T(0)};
+      for (int s2 = 0; s2 <= q; ++s2) {
+        const int base = sd.ncpu * (sv - q + s2) + su - p;
+        T row[4] = {T(0), T(0), T(0), T(0)};
+        for (int j = 0; j <= p; ++j) {
+          const int id = base + j;
+          const T n = sNu[ex][j];
+          if (RAT) {
+            const T nw = n * sd.W[id];
+            row[0] += nw * sd.CP[3 * id + 0];
+            row[1] += nw * sd.CP[3 * id + 1];
+            row[2] += nw * sd.CP[3 * id + 2];
+            row[3] += nw;
+          } else {
+            row[0] += n * sd.CP[3 * id + 0];
+            row[1] += n * sd.CP[3 * id + 1];
+            row[2] += n * sd.CP[3 * id + 2];
+          }
+        }
+        const T nv = sNv[ey][s2];
+        acc[0] += nv * row[0];
+        acc[1] += nv * row[1];
+        acc[2] += nv * row[2];
+        if (RAT) acc[3] += nv * row[3];
+      }
+      if (RAT) {
+        const T F = T(1) / acc[3];
+        S[0] = acc[0] * F;
+        S[1] = acc[1] * F;
+        S[2] = acc[2] * F;
+      } else {
+        S[0] = acc[0];
+        S[1] = acc[1];
+        S[2] = acc[2];
+      }
+    }
+    sP[x][0] = S[0];
+    sP[x][1] = S[1];
+    sP[x][2] = S[2];
+  }
+  __syncthreads();
+  const int ly = threadIdx.x / kTile, lx = threadIdx.x % kTile;
+  if (ly < ht && lx < wd) {
+    const int p0 = ly * E + lx, p1 = p0 + 1, p2 = p0 + E, p3 = p2 + 1;
+    vec4_t<T> ctr;
+    T* c = &ctr.x;
+    const T k = T(0);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const T a0 = sP[p0][a], a1 = sP[p1][a], a2 = sP[p2][a], a3 = sP[p3][a];
+      const T lo = smin(a0, smin(a1, smin(a2, a3)));
+      const T hi = smax(a0, smax(a1, smax(a2, a3)));
+      c[a] = (lo - k + hi + k) / T(2);
+    }
+    ctr.w = T(0);
+    C[(size_t)unit * w * w + (row0 + ly) * w + col0 + lx] = ctr;
+  }
+}
+
 // Cell AABBs (construct_aabbs_distance_host, distance_analysis.cu:45-86;
 // construct_aabbs_host + compute_K padding, intersection.cu:106-148,187-206,
 // common.cu:170-171).  Cell id = iv*(nuv-1) + iu, corners (iv,iu), (iv,iu+1),
@@ -769,6 +874,7 @@ __global__ __launch_bounds__(256) void k_dist_filter(const uint64_t* list, const
   __shared__ vec4_t<T> sB[kTileCells];
   __shared__ int sJ[kTileCells];
   __shared__ Cand stage[4][kStage];
+  __shared__ vec4_t<T> sLo[kTileCells / 32], sHi[kTileCells / 32];
   unsigned long long done = 0;
   const unsigned n = *list_count;
   const T tv = *thresh;
@@ -815,8 +921,44 @@ __global__ __launch_bounds__(256) void k_dist_filter(const uint64_t* list, const
       a = C1[(size_t)u1 * w1 * w1 + i];
     }
     __syncthreads();
+    // bounding box of every 32-cell chunk of the side-2 tile
+    constexpr int kChunk = 32;
+    const int nchunk = (nBp + kChunk - 1) / kChunk;
+    if ((int)threadIdx.x < nchunk) {
+      const T inf = infinity_v<T>();
+      vec4_t<T> lo, hi;
+      lo.x = lo.y = lo.z = inf;
+      hi.x = hi.y = hi.z = -inf;
+      lo.w = hi.w = T(0);
+      const int c0 = threadIdx.x * kChunk, c1 = min(nB, c0 + kChunk);
+      for (int c = c0; c < c1; ++c) {
+        const vec4_t<T> b = sB[c];
+        lo.x = fmin(lo.x, b.x);
+        lo.y = fmin(lo.y, b.y);
+        lo.z = fmin(lo.z, b.z);
+        hi.x = fmax(hi.x, b.x);
+        hi.y = fmax(hi.y, b.y);
+        hi.z = fmax(hi.z, b.z);
+      }
+      sLo[threadIdx.x] = lo;
+      sHi[threadIdx.x] = hi;
+    }
+    __syncthreads();
     const uint64_t key_si = pack_idx((uint32_t)slot, (uint32_t)i, 0u);
     for (int j0 = 0; j0 < nBp; j0 += U) {
+      if ((j0 & (kChunk - 1)) == 0) {
+        // point-to-box gap: a lower bound of every d2 of the chunk (monotone
+        // rounding), so a chunk no lane can reach is skipped
+        const vec4_t<T> lo = sLo[j0 / kChunk], hi = sHi[j0 / kChunk];
+        const T gx = smax(T(0), smax(lo.x - a.x, a.x - hi.x));
+        const T gy = smax(T(0), smax(lo.y - a.y, a.y - hi.y));
+        const T gz = smax(T(0), smax(lo.z - a.z, a.z - hi.z));
+        const T lb2 = FMA ? fma(gx, gx, fma(gy, gy, gz * gz)) : gx * gx + gy * gy + gz * gz;
+        if (__ballot(lb2 <= tv) == 0ull) {
+          j0 += kChunk - U;
+          continue;
+        }
+      }
       T d[U];
 #pragma unroll
       for (int q = 0; q < U; ++q) {
@@ -874,7 +1016,13 @@ __global__ void k_narrow(const double4* __restrict__ src, float4* __restrict__ d
     const unsigned long long o = __shfl_xor(b, off);
     b = o > b ? o : b;
   }
-  if (lane_id() == 0 && b) atomicMax(absmax_bits, b);
+  __shared__ unsigned long long red[4];
+  if (lane_id() == 0) red[threadIdx.x >> 6] = b;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) b = red[w] > b ? red[w] : b;
+    if (b) atomicMax(absmax_bits, b);
+  }
 }
 
 // T <- (sqrt(T)(1+16u) + 4 sqrt(3) E (1+16u))^2 (1+16u) + 1e-36, u = 2^-24,
@@ -1163,6 +1311,28 @@ __global__ __launch_bounds__(1024) void k_bsort_local(Cand* a, unsigned long lon
     for (int j = kSortBlock >> 1; j > 0; j >>= 1) step(merge_k, j);
   }
   for (int t = threadIdx.x; t < kSortBlock; t += blockDim.x) a[base + t] = s[t];
+}
+
+// Whole sort of n2 <= kBlockSortMax records in one workgroup's LDS (128 KiB).
+constexpr int kBlockSortMax = 8192;
+__global__ __launch_bounds__(1024) void k_bsort_single(Cand* a, int n2) {
+  extern __shared__ Cand sh[];
+  for (int t = threadIdx.x; t < n2; t += blockDim.x) sh[t] = a[t];
+  __syncthreads();
+  for (int k = 2; k <= n2; k <<= 1)
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int t = threadIdx.x; t < n2 / 2; t += blockDim.x) {
+        const int i = 2 * t - (t & (j - 1)), l = i + j;
+        const bool asc = (i & k) == 0;
+        const Cand x = sh[i], y = sh[l];
+        if (asc ? cand_less(y, x) : cand_less(x, y)) {
+          sh[i] = y;
+          sh[l] = x;
+        }
+      }
+      __syncthreads();
+    }
+  for (int t = threadIdx.x; t < n2; t += blockDim.x) a[t] = sh[t];
 }
 
 __global__ void k_bsort_global(Cand* a, unsigned long long n2, unsigned long long k, unsigned long long j) {
@@ -1573,6 +1743,31 @@ hipError_t launch_points(const SurfDev<T>& sd, const T* tabU, const int* spanU, 
   return hipGetLastError();
 }
 
+template <class T, int MAXD>
+static void eval_centres_rat(const SurfDev<T>& sd, const vec4_t<T>* reg, int nunits, int nuv, vec4_t<T>* C,
+                             hipStream_t s) {
+  const int w = nuv - 1, tw = (w + kTile - 1) / kTile;
+  const dim3 g((unsigned)((long long)nunits * tw * tw)), b(256);
+  if (sd.rational)
+    hipLaunchKernelGGL((k_eval_centres<T, MAXD, true>), g, b, 0, s, sd, reg, nunits, nuv, C);
+  else
+    hipLaunchKernelGGL((k_eval_centres<T, MAXD, false>), g, b, 0, s, sd, reg, nunits, nuv, C);
+}
+
+template <class T>
+hipError_t launch_eval_centres(const SurfDev<T>& sd, const vec4_t<T>* reg, int nunits, int nuv, vec4_t<T>* C,
+                               hipStream_t s) {
+  if (nunits <= 0 || nuv < 2) return hipSuccess;
+  const int d = std::max(sd.degu, sd.degv);
+  if (d <= 3)
+    eval_centres_rat<T, 3>(sd, reg, nunits, nuv, C, s);
+  else if (d <= 7)
+    eval_centres_rat<T, 7>(sd, reg, nunits, nuv, C, s);
+  else
+    eval_centres_rat<T, kMaxDeg - 1>(sd, reg, nunits, nuv, C, s);
+  return hipGetLastError();
+}
+
 template <class T>
 hipError_t launch_aabb(const T* pts, int nunits, int nuv, int isect, int use_k, const unsigned long long* kmax,
                        const vec4_t<T>* regions, T init_u, T init_v, vec4_t<T>* C, vec4_t<T>* R, hipStream_t s) {
@@ -1760,7 +1955,15 @@ hipError_t launch_fill_sentinel(Cand* a, unsigned long long from, unsigned long 
 }
 
 hipError_t sort_cands(Cand* a, unsigned long long n2, hipStream_t s) {
-  if (n2 < (unsigned long long)kSortBlock || (n2 & (n2 - 1)) != 0) return hipErrorInvalidValue;
+  if ((n2 & (n2 - 1)) != 0) return hipErrorInvalidValue;
+  static const bool big_lds =
+      hipFuncSetAttribute((const void*)k_bsort_single, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          kBlockSortMax * (int)sizeof(Cand)) == hipSuccess;
+  if (n2 <= (unsigned long long)(big_lds ? kBlockSortMax : 4096)) {
+    hipLaunchKernelGGL(k_bsort_single, dim3(1), dim3(1024), n2 * sizeof(Cand), s, a, (int)n2);
+    return hipGetLastError();
+  }
+  if (n2 < (unsigned long long)kSortBlock) return hipErrorInvalidValue;
   const unsigned nb = (unsigned)(n2 / kSortBlock);
   hipLaunchKernelGGL(k_bsort_local, dim3(nb), dim3(1024), 0, s, a, 0ull);
   for (unsigned long long k = 2ull * kSortBlock; k <= n2; k <<= 1) {
@@ -1910,6 +2113,8 @@ hipError_t launch_fill_u32(unsigned* p, unsigned value, unsigned long long n, hi
                                             unsigned long long*, unsigned long long, int, hipStream_t,             \
                                             unsigned long long*, int);            \
   template hipError_t launch_set_thresh<T>(T*, const Cand*, int, int, hipStream_t);                                \
+  template hipError_t launch_eval_centres<T>(const SurfDev<T>&, const vec4_t<T>*, int, int, vec4_t<T>*,            \
+                                             hipStream_t);                                                         \
   template hipError_t launch_seed_thresh<T>(const unsigned long long*, int, int, int, int, uint64_t*, unsigned*,   \
                                             unsigned long long*, unsigned*, const int*, const int*, int, int,      \
                                             const vec4_t<T>*, const vec4_t<T>*, int, int, T*, hipStream_t);        \

@@ -188,6 +188,15 @@ int jsi_ctx::eval_units(int h, const vec4_t<T>* reg, int nunits, int nuv, int mo
   return JSI_OK;
 }
 
+// Distance cell centres of `nunits` regions, evaluation fused with the AABB.
+template <class T>
+int jsi_ctx::eval_centres(int h, const vec4_t<T>* reg, int nunits, int nuv, vec4_t<T>* C) {
+  SurfDev<T> sd;
+  RCK(surf_dev<T>(h, &sd));
+  HCK(launch_eval_centres<T>(sd, reg, nunits, nuv, C, stream));
+  return JSI_OK;
+}
+
 // ---------------------------------------------------------------------------
 // Distance screen: exact top-k under (d2, slot, i, j)
 // ---------------------------------------------------------------------------
@@ -241,7 +250,10 @@ int jsi_ctx::dist_topk(int nslots, const int* su1, const int* su2, int w1, int n
   // global k-th d2 from above
   const long long npos = (long long)nslots * tpu1;
   const int pairs_tp = std::min(w1 * w1, kTileCells) * std::min(w2 * w2, kTileCells);
-  const int nseed = (int)std::max(1ull, std::min<unsigned long long>(64ull, kCandCap / (unsigned long long)pairs_tp));
+  // about 32 k seed pairs (at least 4 tile pairs, at most 64): enough for a
+  // threshold near the true k-th without scanning millions of seed pairs
+  const unsigned long long want = (32ull * (unsigned long long)k + pairs_tp - 1) / (unsigned long long)pairs_tp;
+  const int nseed = (int)std::max(4ull, std::min(64ull, want));
   if (npos <= 8192) {
     unsigned long long *bestv, *range;
     unsigned* ghist;
@@ -297,11 +309,12 @@ int jsi_ctx::dist_topk(int nslots, const int* su1, const int* su2, int w1, int n
     HCK(launch_rerank(cand, n, su1, su2, w1 * w1, w2 * w2, rr.C1, rr.C2, stream));
     maxkey = 0x7ff0000000000000ull;
   }
-  HCK(select_topk(cand, n, k, maxkey, st, hist, small, small_count, kSmallCap, stream));
+  const unsigned small_cap = (unsigned)std::max(2048, std::min((int)kSmallCap, 2 * k));
+  HCK(select_topk(cand, n, k, maxkey, st, hist, small, small_count, small_cap, stream));
   unsigned long long m = 0;
   RCK(d2h(&m, small_count, sizeof(m)));
   Cand* rec = small;
-  if (m > kSmallCap) {
+  if (m > small_cap) {
     // more than kSmallCap candidates tie with the k-th key: sort them all
     const unsigned long long n2 = std::max(next_pow2(std::max(n, 1ull)), (unsigned long long)kSortBlock);
     if (n2 > kCandCap) {
@@ -312,7 +325,7 @@ int jsi_ctx::dist_topk(int nslots, const int* su1, const int* su2, int w1, int n
     HCK(sort_cands(cand, n2, stream));
     rec = cand;
   } else {
-    const unsigned long long n2 = std::max(next_pow2(std::max(m, 1ull)), (unsigned long long)kSortBlock);
+    const unsigned long long n2 = std::max(next_pow2(std::max(m, 1ull)), 64ull);
     HCK(launch_fill_sentinel(small, m, n2, stream));
     HCK(sort_cands(small, n2, stream));
   }
@@ -378,10 +391,8 @@ int jsi_ctx::min_distance(int h1, const jsi_region& r1, int h2, const jsi_region
     RCK(buf("pts", (size_t)3 * std::max(nuv1 * nuv1, nuv2 * nuv2), &pts));
     RCK(buf(nm("C", 1, cur).c_str(), (size_t)(nuv1 - 1) * (nuv1 - 1), &C1));
     RCK(buf(nm("C", 2, cur).c_str(), (size_t)(nuv2 - 1) * (nuv2 - 1), &C2));
-    RCK(eval_units<T>(h1, reg1, 1, nuv1, kEvalPoint, 0, pts, nullptr, "e1"));
-    HCK(launch_aabb<T>(pts, 1, nuv1, 0, 0, nullptr, reg1, T(1), T(1), C1, nullptr, stream));
-    RCK(eval_units<T>(h2, reg2, 1, nuv2, kEvalPoint, 0, pts, nullptr, "e2"));
-    HCK(launch_aabb<T>(pts, 1, nuv2, 0, 0, nullptr, reg2, T(1), T(1), C2, nullptr, stream));
+    RCK(eval_centres<T>(h1, reg1, 1, nuv1, C1));
+    RCK(eval_centres<T>(h2, reg2, 1, nuv2, C2));
   }
   int* su1;
   int* su2;
@@ -486,10 +497,8 @@ int jsi_ctx::min_distance(int h1, const jsi_region& r1, int h2, const jsi_region
     RCK(buf("pts", (size_t)3 * std::max(nu[0], nu[1]) * nref * nref, &pts));
     RCK(buf(nm("C", 1, nxt).c_str(), (size_t)nu[0] * wr * wr, &nC1));
     RCK(buf(nm("C", 2, nxt).c_str(), (size_t)nu[1] * wr * wr, &nC2));
-    RCK(eval_units<T>(h1, nreg1, nu[0], nref, kEvalPoint, 0, pts, nullptr, "r1"));
-    HCK(launch_aabb<T>(pts, nu[0], nref, 0, 0, nullptr, nreg1, T(1), T(1), nC1, nullptr, stream));
-    RCK(eval_units<T>(h2, nreg2, nu[1], nref, kEvalPoint, 0, pts, nullptr, "r2"));
-    HCK(launch_aabb<T>(pts, nu[1], nref, 0, 0, nullptr, nreg2, T(1), T(1), nC2, nullptr, stream));
+    RCK(eval_centres<T>(h1, nreg1, nu[0], nref, nC1));
+    RCK(eval_centres<T>(h2, nreg2, nu[1], nref, nC2));
     HCK(hipStreamSynchronize(stream));
     res->ms_eval += now_ms() - te;
     cur = nxt;

@@ -83,6 +83,8 @@ struct jsi_ctx {
                  unsigned long long* kmax, const char* tag);
   using Rerank = RerankSpec;
   template <class T>
+  int eval_centres(int h, const jsi::vec4_t<T>* reg, int nunits, int nuv, jsi::vec4_t<T>* C);
+  template <class T>
   int dist_topk(int nslots, const int* su1, const int* su2, int w1, int nunits1, const jsi::vec4_t<T>* C1, int w2,
                 int nunits2, const jsi::vec4_t<T>* C2, int k, int cull, jsi::Cand** rec_out, long long* n_out,
                 long long* kk_out, Rerank rr = Rerank());
