@@ -46,20 +46,28 @@ def run_form(ctx, a, r1, b, r2, params, steps, warmup, profile=False):
 
 
 def run_served(ctx, q, n, slots, warmup):
-    """n copies of the query through the serving loop with `slots` worker
-    streams; latency = submit -> result."""
+    """n copies of the query through the serving loop, closed loop: `slots`
+    requests stay in flight (a new one is admitted as each completes);
+    latency = submit -> result."""
+    from collections import deque
+
     from jsi_cad_next_versions_amd.runtime import Server
     with Server(ctx, slots=slots) as srv:
         for t in [srv.submit(q) for _ in range(warmup * slots)]:
             srv.poll(t)
         t0 = time.perf_counter()
-        subs = []
-        for _ in range(n):
-            subs.append((srv.submit(q), time.perf_counter()))
-        lat, res = [], None
-        for t, ts in subs:
+        inflight = deque()
+        lat, res, sent = [], None, 0
+        while sent < n and len(inflight) < slots:
+            inflight.append((srv.submit(q), time.perf_counter()))
+            sent += 1
+        while inflight:
+            t, ts = inflight.popleft()
             res = srv.poll(t)
             lat.append((time.perf_counter() - ts) * 1e3)
+            if sent < n:
+                inflight.append((srv.submit(q), time.perf_counter()))
+                sent += 1
         wall = time.perf_counter() - t0
     return res, wall, lat
 
@@ -87,7 +95,7 @@ def main():
     ap.add_argument("--nuv", type=int, default=512)
     ap.add_argument("--k", type=int, default=4096)
     ap.add_argument("--precision", default="fp32screen")
-    ap.add_argument("--plain-steps", type=int, default=5)
+    ap.add_argument("--plain-steps", type=int, default=20)
     ap.add_argument("--cpu-queries", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--concurrency", type=int, default=8, help="worker streams of the served (batched) form")
@@ -118,10 +126,13 @@ def main():
     # drop-in EXACT32 mode (bit-exact with the reference, FP32 end to end)
     pe = default_params(k=args.k, precision="exact32", cull=1)
     eres, ewall, elat = run_form(ctx, a, r1, b, r2, pe, max(1, args.steps // 2), 1)
-    # plain FP64 GPU path: FP64 everywhere, exhaustive screen (no culling)
-    pp = default_params(k=args.k, precision="fp64", cull=0)
+    # plain FP64 GPU path: FP64 everywhere, one query at a time (the drop-in FP64 form)
+    pp = default_params(k=args.k, precision="fp64", cull=1)
     pres, pwall, plat = run_form(ctx, a, r1, b, r2, pp, args.plain_steps, 1)
     pqps = args.plain_steps / pwall
+    # the same FP64 path with every cell pair evaluated (no tile or chunk culling)
+    xp = default_params(k=args.k, precision="fp64", cull=0)
+    xres, xwall, xlat = run_form(ctx, a, r1, b, r2, xp, 2, 1)
     dist_err = abs(res.distance - pres.distance)
 
     # dominant screen kernel of the chosen form (most summed time), timed with HIP events
@@ -158,6 +169,7 @@ def main():
         "plain_fp64": {"queries_per_s": pqps, "p50_ms": pct(plat, 50), "p99_ms": pct(plat, 99),
                        "distance": pres.distance},
         "speedup_vs_plain": qps / pqps if pqps > 0 else None,
+        "exhaustive_fp64": {"queries_per_s": 2 / xwall, "p50_ms": pct(xlat, 50), "distance": xres.distance},
         "exact32": {"queries_per_s": max(1, args.steps // 2) / ewall, "p50_ms": pct(elat, 50),
                     "p99_ms": pct(elat, 99), "distance": eres.distance},
         "roofline": {"bound": "mfma", "kernel": f"k_dist_filter ({dom})",

@@ -116,7 +116,8 @@ template <class T>
 hipError_t launch_dist_filter(const uint64_t* list, const unsigned* list_count, unsigned list_cap, const int* su1,
                               const int* su2, int w1, int w2, const vec4_t<T>* C1, const vec4_t<T>* C2,
                               const T* thresh, Cand* out, unsigned long long* out_count, unsigned long long cap,
-                              int grid, hipStream_t s, unsigned long long* pairs_done = nullptr, int fma = 0);
+                              int grid, hipStream_t s, unsigned long long* pairs_done = nullptr, int fma = 0,
+                              int cull = 1);
 template <class T>
 hipError_t launch_set_thresh(T* thresh, const Cand* rec, int idx, int use_inf, hipStream_t s);
 
@@ -152,6 +153,9 @@ hipError_t launch_thresh_from_state(T* thresh, const SelState* st, hipStream_t s
 // Sorting of candidate records (bitonic, n2 = power of two >= kSortBlock).
 hipError_t launch_fill_sentinel(Cand* a, unsigned long long from, unsigned long long n2, hipStream_t s);
 hipError_t sort_cands(Cand* a, unsigned long long n2, hipStream_t s);
+// Sorted first `keep` of m (device count, at most m_cap) distinct records.
+hipError_t rank_sort(const Cand* in, const unsigned long long* m_dev, unsigned m_cap, unsigned keep, Cand* out,
+                     hipStream_t s);
 
 // Distance refinement: first-appearance dedup of the sorted top-k endpoints,
 // child regions and the next round's slot -> unit maps (single workgroup).

@@ -693,6 +693,8 @@ __global__ __launch_bounds__(1024) void k_seed_list(const unsigned long long* be
 // MINMAX: the smallest and largest key.  Otherwise: a histogram with bins
 // linear in the key bits over [min, max], so its resolution follows the
 // spread of the values (refined cells have d2 values within 1e-5 of each other).
+constexpr int kSeedSplit = 16;  // workgroups per seed tile pair
+
 template <class T, bool FMA, bool MINMAX>
 __global__ __launch_bounds__(256) void k_seed_hist(const uint64_t* list, const unsigned* list_count, const int* su1,
                                                    const int* su2, int w1, int w2, const vec4_t<T>* C1,
@@ -709,22 +711,25 @@ __global__ __launch_bounds__(256) void k_seed_hist(const uint64_t* list, const u
     for (int b = threadIdx.x; b < kSeedBins; b += blockDim.x) h[b] = 0;
   }
   unsigned long long kmin = ~0ull, kmax = 0ull;
-  for (unsigned it = blockIdx.x; it < n; it += gridDim.x) {
+  const unsigned it = blockIdx.x / kSeedSplit;
+  const int part = blockIdx.x % kSeedSplit;
+  if (it < n) {
     const uint64_t e = list[it];
     const int slot = (int)(e >> 32), t1 = (int)((e >> 16) & 0xffff), t2 = (int)(e & 0xffff);
     const int u1 = su1[slot], u2 = su2[slot];
     const TileGeo g1 = tile_geo(t1, w1), g2 = tile_geo(t2, w2);
     const int nA = g1.ht * g1.wd, nB = g2.ht * g2.wd;
-    __syncthreads();
-    if ((int)threadIdx.x < nB) {
-      const int j = (g2.row0 + threadIdx.x / g2.wd) * w2 + g2.col0 + threadIdx.x % g2.wd;
+    const int j0 = part * nB / kSeedSplit, j1 = (part + 1) * nB / kSeedSplit;
+    if ((int)threadIdx.x < j1 - j0) {
+      const int jj = j0 + threadIdx.x;
+      const int j = (g2.row0 + jj / g2.wd) * w2 + g2.col0 + jj % g2.wd;
       sB[threadIdx.x] = C2[(size_t)u2 * w2 * w2 + j];
     }
     __syncthreads();
     if ((int)threadIdx.x < nA) {
       const int i = (g1.row0 + threadIdx.x / g1.wd) * w1 + g1.col0 + threadIdx.x % g1.wd;
       const vec4_t<T> a = C1[(size_t)u1 * w1 * w1 + i];
-      for (int jj = 0; jj < nB; ++jj) {
+      for (int jj = 0; jj < j1 - j0; ++jj) {
         const vec4_t<T> b = sB[jj];
         const unsigned long long key = ord_bits(pair_d2<T, FMA>(a.x, a.y, a.z, b.x, b.y, b.z));
         if (MINMAX) {
@@ -868,7 +873,8 @@ __global__ __launch_bounds__(256) void k_dist_filter(const uint64_t* list, const
                                                      const int* su1, const int* su2, int w1, int w2,
                                                      const vec4_t<T>* C1, const vec4_t<T>* C2, const T* thresh,
                                                      Cand* out, unsigned long long* out_count,
-                                                     unsigned long long cap, unsigned long long* pairs_done) {
+                                                     unsigned long long cap, unsigned long long* pairs_done,
+                                                     int cull) {
   constexpr int U = 8;
   constexpr int kStage = 128;  // candidates staged per wavefront before one global reservation
   __shared__ vec4_t<T> sB[kTileCells];
@@ -900,7 +906,6 @@ __global__ __launch_bounds__(256) void k_dist_filter(const uint64_t* list, const
     const TileGeo g1 = tile_geo(t1, w1), g2 = tile_geo(t2, w2);
     const int nA = g1.ht * g1.wd, nB = g2.ht * g2.wd;
     const int nBp = (nB + U - 1) / U * U;
-    done += (unsigned long long)nA * nB;
     __syncthreads();
     if ((int)threadIdx.x < nB) {
       const int j = (g2.row0 + threadIdx.x / g2.wd) * w2 + g2.col0 + threadIdx.x % g2.wd;
@@ -945,19 +950,24 @@ __global__ __launch_bounds__(256) void k_dist_filter(const uint64_t* list, const
     }
     __syncthreads();
     const uint64_t key_si = pack_idx((uint32_t)slot, (uint32_t)i, 0u);
+    const int nvalid = __popcll(__ballot(va));
     for (int j0 = 0; j0 < nBp; j0 += U) {
       if ((j0 & (kChunk - 1)) == 0) {
-        // point-to-box gap: a lower bound of every d2 of the chunk (monotone
-        // rounding), so a chunk no lane can reach is skipped
-        const vec4_t<T> lo = sLo[j0 / kChunk], hi = sHi[j0 / kChunk];
-        const T gx = smax(T(0), smax(lo.x - a.x, a.x - hi.x));
-        const T gy = smax(T(0), smax(lo.y - a.y, a.y - hi.y));
-        const T gz = smax(T(0), smax(lo.z - a.z, a.z - hi.z));
-        const T lb2 = FMA ? fma(gx, gx, fma(gy, gy, gz * gz)) : gx * gx + gy * gy + gz * gz;
-        if (__ballot(lb2 <= tv) == 0ull) {
-          j0 += kChunk - U;
-          continue;
+        if (cull) {
+          // point-to-box gap: a lower bound of every d2 of the chunk (monotone
+          // rounding), so a chunk no lane can reach is skipped
+          const vec4_t<T> lo = sLo[j0 / kChunk], hi = sHi[j0 / kChunk];
+          const T gx = smax(T(0), smax(lo.x - a.x, a.x - hi.x));
+          const T gy = smax(T(0), smax(lo.y - a.y, a.y - hi.y));
+          const T gz = smax(T(0), smax(lo.z - a.z, a.z - hi.z));
+          const T lb2 = FMA ? fma(gx, gx, fma(gy, gy, gz * gz)) : gx * gx + gy * gy + gz * gz;
+          if (__ballot(lb2 <= tv) == 0ull) {
+            j0 += kChunk - U;
+            continue;
+          }
         }
+        // algorithmic work: this wavefront evaluates its lanes' pairs with the chunk
+        if (lane == 0) done += (unsigned long long)nvalid * (unsigned long long)max(0, min(kChunk, nB - j0));
       }
       T d[U];
 #pragma unroll
@@ -986,7 +996,7 @@ __global__ __launch_bounds__(256) void k_dist_filter(const uint64_t* list, const
     }
   }
   if (wcount) flush();
-  if (pairs_done && threadIdx.x == 0 && done) atomicAdd(pairs_done, done);
+  if (pairs_done && lane == 0 && done) atomicAdd(pairs_done, done);
 }
 
 // ---------------------------------------------------------------------------
@@ -1333,6 +1343,30 @@ __global__ __launch_bounds__(1024) void k_bsort_single(Cand* a, int n2) {
       __syncthreads();
     }
   for (int t = threadIdx.x; t < n2; t += blockDim.x) a[t] = sh[t];
+}
+
+// Rank sort of m distinct records (the keys (d2, slot, i, j) are unique):
+// each record's position is the number of records smaller than it.  Only
+// positions below `keep` are written.  O(m^2) compares, fully parallel: for
+// the few thousand records left after selection it beats a bitonic network.
+__global__ __launch_bounds__(256) void k_rank_sort(const Cand* in, const unsigned long long* m_dev, unsigned m_cap,
+                                                   unsigned keep, Cand* out) {
+  __shared__ Cand chunk[1024];
+  const unsigned m = (unsigned)min((unsigned long long)m_cap, *m_dev);
+  const unsigned x = blockIdx.x * blockDim.x + threadIdx.x;
+  Cand mine;
+  mine.hi = mine.lo = ~0ull;
+  if (x < m) mine = in[x];
+  unsigned rank = 0;
+  for (unsigned c0 = 0; c0 < m; c0 += 1024) {
+    const unsigned cn = min(1024u, m - c0);
+    __syncthreads();
+    for (unsigned t = threadIdx.x; t < cn; t += blockDim.x) chunk[t] = in[c0 + t];
+    __syncthreads();
+    if (x < m)
+      for (unsigned t = 0; t < cn; ++t) rank += cand_less(chunk[t], mine) ? 1u : 0u;
+  }
+  if (x < m && rank < keep) out[rank] = mine;
 }
 
 __global__ void k_bsort_global(Cand* a, unsigned long long n2, unsigned long long k, unsigned long long j) {
@@ -1829,14 +1863,14 @@ hipError_t launch_seed_thresh(const unsigned long long* best, int npos, int tpu1
                      (unsigned long long)kSeedBins);
   const int m = nseed < npos ? nseed : npos;
   if (fma) {
-    hipLaunchKernelGGL((k_seed_hist<T, true, true>), dim3(m), dim3(256), 0, s, list, list_count, su1, su2, w1, w2, C1,
+    hipLaunchKernelGGL((k_seed_hist<T, true, true>), dim3(m * kSeedSplit), dim3(256), 0, s, list, list_count, su1, su2, w1, w2, C1,
                        C2, range, ghist);
-    hipLaunchKernelGGL((k_seed_hist<T, true, false>), dim3(m), dim3(256), 0, s, list, list_count, su1, su2, w1, w2,
+    hipLaunchKernelGGL((k_seed_hist<T, true, false>), dim3(m * kSeedSplit), dim3(256), 0, s, list, list_count, su1, su2, w1, w2,
                        C1, C2, range, ghist);
   } else {
-    hipLaunchKernelGGL((k_seed_hist<T, false, true>), dim3(m), dim3(256), 0, s, list, list_count, su1, su2, w1, w2,
+    hipLaunchKernelGGL((k_seed_hist<T, false, true>), dim3(m * kSeedSplit), dim3(256), 0, s, list, list_count, su1, su2, w1, w2,
                        C1, C2, range, ghist);
-    hipLaunchKernelGGL((k_seed_hist<T, false, false>), dim3(m), dim3(256), 0, s, list, list_count, su1, su2, w1, w2,
+    hipLaunchKernelGGL((k_seed_hist<T, false, false>), dim3(m * kSeedSplit), dim3(256), 0, s, list, list_count, su1, su2, w1, w2,
                        C1, C2, range, ghist);
   }
   hipLaunchKernelGGL(k_seed_thresh<T>, dim3(1), dim3(1024), 0, s, ghist, range, k, thresh);
@@ -1862,15 +1896,15 @@ template <class T>
 hipError_t launch_dist_filter(const uint64_t* list, const unsigned* list_count, unsigned list_cap, const int* su1,
                               const int* su2, int w1, int w2, const vec4_t<T>* C1, const vec4_t<T>* C2,
                               const T* thresh, Cand* out, unsigned long long* out_count, unsigned long long cap,
-                              int grid, hipStream_t s, unsigned long long* pairs_done, int fma) {
+                              int grid, hipStream_t s, unsigned long long* pairs_done, int fma, int cull) {
   if (list_cap == 0) return hipSuccess;
   const unsigned g = (unsigned)std::min<long long>(grid, (long long)list_cap);
   if (fma)
     hipLaunchKernelGGL((k_dist_filter<T, true>), dim3(g), dim3(256), 0, s, list, list_count, su1, su2, w1, w2, C1, C2,
-                       thresh, out, out_count, cap, pairs_done);
+                       thresh, out, out_count, cap, pairs_done, cull);
   else
     hipLaunchKernelGGL((k_dist_filter<T, false>), dim3(g), dim3(256), 0, s, list, list_count, su1, su2, w1, w2, C1,
-                       C2, thresh, out, out_count, cap, pairs_done);
+                       C2, thresh, out, out_count, cap, pairs_done, cull);
   return hipGetLastError();
 }
 
@@ -1951,6 +1985,13 @@ template hipError_t launch_thresh_from_state<double>(double*, const SelState*, h
 hipError_t launch_fill_sentinel(Cand* a, unsigned long long from, unsigned long long n2, hipStream_t s) {
   if (n2 <= from) return hipSuccess;
   hipLaunchKernelGGL(k_fill_sentinel, dim3(blocks_for(n2 - from, 256)), dim3(256), 0, s, a, from, n2);
+  return hipGetLastError();
+}
+
+hipError_t rank_sort(const Cand* in, const unsigned long long* m_dev, unsigned m_cap, unsigned keep, Cand* out,
+                     hipStream_t s) {
+  if (m_cap == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_rank_sort, dim3(blocks_for(m_cap, 256)), dim3(256), 0, s, in, m_dev, m_cap, keep, out);
   return hipGetLastError();
 }
 
@@ -2111,7 +2152,7 @@ hipError_t launch_fill_u32(unsigned* p, unsigned value, unsigned long long n, hi
   template hipError_t launch_dist_filter<T>(const uint64_t*, const unsigned*, unsigned, const int*, const int*,    \
                                             int, int, const vec4_t<T>*, const vec4_t<T>*, const T*, Cand*,         \
                                             unsigned long long*, unsigned long long, int, hipStream_t,             \
-                                            unsigned long long*, int);            \
+                                            unsigned long long*, int, int);            \
   template hipError_t launch_set_thresh<T>(T*, const Cand*, int, int, hipStream_t);                                \
   template hipError_t launch_eval_centres<T>(const SurfDev<T>&, const vec4_t<T>*, int, int, vec4_t<T>*,            \
                                              hipStream_t);                                                         \

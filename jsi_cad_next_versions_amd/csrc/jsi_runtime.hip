@@ -281,7 +281,7 @@ int jsi_ctx::dist_topk(int nslots, const int* su1, const int* su2, int w1, int n
       HCK(hipEventRecord(ev[0], stream));
     }
     HCK(launch_dist_filter<T>(list, list_count, (unsigned)ntp, su1, su2, w1, w2, C1, C2, thresh, cand, out_count,
-                              kCandCap, grid, stream, pairs_done, fma));
+                              kCandCap, grid, stream, pairs_done, fma, cull));
     if (prof) HCK(hipEventRecord(ev[1], stream));
     RCK(d2h(&n, out_count, sizeof(n)));
     if (prof) {
@@ -325,9 +325,10 @@ int jsi_ctx::dist_topk(int nslots, const int* su1, const int* su2, int w1, int n
     HCK(sort_cands(cand, n2, stream));
     rec = cand;
   } else {
-    const unsigned long long n2 = std::max(next_pow2(std::max(m, 1ull)), 64ull);
-    HCK(launch_fill_sentinel(small, m, n2, stream));
-    HCK(sort_cands(small, n2, stream));
+    Cand* sorted;
+    RCK(buf("d.sorted", (size_t)k, &sorted));
+    HCK(rank_sort(small, small_count, small_cap, (unsigned)k, sorted, stream));
+    rec = sorted;
   }
   *rec_out = rec;
   *n_out = (long long)n;
