@@ -39,6 +39,7 @@ struct IPair {
 constexpr int kTile = 16;          // cells per tile edge in the screens
 constexpr int kTileCells = kTile * kTile;
 constexpr int kSortBlock = 2048;   // records per LDS bitonic block
+constexpr int kAmaxSlots = 64;     // slots of a max |coordinate| reduction (read as their max)
 
 inline __host__ __device__ uint64_t pack_idx(uint32_t slot, uint32_t i, uint32_t j) {
   return (uint64_t(slot) << 40) | (uint64_t(i) << 20) | uint64_t(j);
@@ -76,10 +77,12 @@ hipError_t launch_points(const SurfDev<T>& sd, const T* tabU, const int* spanU, 
                          int nunits, int nuv, int mode, T* out, unsigned long long* kmax, hipStream_t s);
 
 // Fused evaluation + distance centres of nunits regions (one workgroup per
-// 16x16-cell tile); C as launch_aabb(isect = 0) writes it.
+// 16x16-cell tile); C as launch_aabb(isect = 0) writes it.  T = double only:
+// absmax max-reduces the largest |coordinate| (double bits); C32 (requires
+// absmax) also writes the centres rounded to float.
 template <class T>
 hipError_t launch_eval_centres(const SurfDev<T>& sd, const vec4_t<T>* reg, int nunits, int nuv, vec4_t<T>* C,
-                               hipStream_t s);
+                               hipStream_t s, float4* C32 = nullptr, unsigned long long* absmax = nullptr);
 
 // Cell AABBs of every unit from its SoA points.  isect = 0: distance centres
 // (construct_aabbs_distance_host); isect = 1: centre + padded radius
@@ -109,15 +112,18 @@ hipError_t launch_dist_list(int nslots, const int* su1, const int* su2, int w1, 
 // rank k to *thresh (+inf if they hold fewer than k pairs).
 template <class T>
 hipError_t launch_seed_thresh(const unsigned long long* best, int npos, int tpu1, int nseed, int ordered,
-                              uint64_t* list, unsigned* list_count, unsigned long long* range, unsigned* ghist,
-                              const int* su1, const int* su2, int w1, int w2, const vec4_t<T>* C1,
-                              const vec4_t<T>* C2, int k, int fma, T* thresh, hipStream_t s);
+                              uint64_t* list, unsigned* list_count, unsigned long long* range,
+                              unsigned long long* work, const int* su1, const int* su2, int w1, int w2,
+                              const vec4_t<T>* C1, const vec4_t<T>* C2, int k, int fma, T* thresh, hipStream_t s);
+size_t seed_work_bytes();  // size of `work` for launch_seed_thresh
 template <class T>
 hipError_t launch_dist_filter(const uint64_t* list, const unsigned* list_count, unsigned list_cap, const int* su1,
                               const int* su2, int w1, int w2, const vec4_t<T>* C1, const vec4_t<T>* C2,
                               const T* thresh, Cand* out, unsigned long long* out_count, unsigned long long cap,
                               int grid, hipStream_t s, unsigned long long* pairs_done = nullptr, int fma = 0,
-                              int cull = 1);
+                              int cull = 1, const unsigned long long* absmax = nullptr);
+// absmax (double only): the expanded-form test (three FMAs per pair with a
+// rounding margin from the largest |centre coordinate|, exact keys).
 template <class T>
 hipError_t launch_set_thresh(T* thresh, const Cand* rec, int idx, int use_inf, hipStream_t s);
 
@@ -158,12 +164,13 @@ hipError_t rank_sort(const Cand* in, const unsigned long long* m_dev, unsigned m
                      hipStream_t s);
 
 // Distance refinement: first-appearance dedup of the sorted top-k endpoints,
-// child regions and the next round's slot -> unit maps (single workgroup).
+// child regions and the next round's slot -> unit maps (three passes).
 template <class T>
 hipError_t launch_dist_dedup(const Cand* rec, int n, int cells1, int cells2, const int* su1, const int* su2,
                              const vec4_t<T>* reg1, const vec4_t<T>* reg2, int parent_nuv1, int parent_nuv2,
                              int* firstpos1, int* firstpos2, int* nsu1, int* nsu2, vec4_t<T>* nreg1,
-                             vec4_t<T>* nreg2, int* nunits, hipStream_t s);
+                             vec4_t<T>* nreg2, int* nunits, unsigned char* work, hipStream_t s);
+size_t dedup_work_bytes();  // size of `work` for launch_dist_dedup
 
 // Intersection screen.
 template <class T>

@@ -7,6 +7,7 @@
 // double kernels with the FP64 oracle.  File:line citations refer to
 // JSI_CAD_NEXT_VERSION5/.
 #include <algorithm>
+#include <type_traits>
 #include <float.h>
 #include <limits.h>
 
@@ -137,6 +138,32 @@ __device__ void basis_ders(int span, int p, T u, const T* U, int der, T (&ders)[
   for (int k = 1; k <= der; ++k) {
     for (int j = 0; j <= p; ++j) ders[k][j] *= T(r);
     r *= p - k;
+  }
+}
+
+// Values row only (ders[0] of basis_ders): the same recurrence, kept in
+// registers.  N[r] holds ndu[r][j-1] while stage j runs, and each stage
+// performs exactly the operations basis_ders performs on ndu, so the results
+// are bit-identical.  Loops are unrolled up to MAXP and cut at p.
+template <class T, int MAXP>
+__device__ __forceinline__ void basis_vals(int span, int p, T u, const T* U, T (&N)[MAXP + 1]) {
+  T left[MAXP + 1], right[MAXP + 1];
+  constexpr int UR = MAXP <= 7 ? MAXP + 1 : 1;  // unroll low degrees only
+  N[0] = T(1);
+#pragma unroll UR
+  for (int j = 1; j <= MAXP; ++j) {
+    if (j > p) break;
+    left[j] = u - U[span + 1 - j];
+    right[j] = U[span + j] - u;
+    T saved = T(0);
+#pragma unroll UR
+    for (int r = 0; r < j; ++r) {
+      const T den = right[r + 1] + left[j - r];
+      const T temp = N[r] / den;
+      N[r] = saved + right[r + 1] * temp;
+      saved = left[j - r] * temp;
+    }
+    N[j] = saved;
   }
 }
 
@@ -321,108 +348,222 @@ __global__ __launch_bounds__(256) void k_points(SurfDev<T> sd, const T* tabU, co
   }
 }
 
-// Fused evaluation + distance AABB centres.  One workgroup per 16x16-cell
-// tile of a unit: it computes the basis values (A2.3, order 0 only: the
-// values row ders[0] does not depend on the derivative order) of the tile's
-// 17 columns and 17 rows into LDS, evaluates the 17x17 corner points
-// (EvaluateOneUV's S row, evaluation.cu:281-300 / :313-348 + the 1/W scaling
-// of the quotient rule), and reduces each cell's four corners to its centre
-// (construct_aabbs_distance_host, distance_analysis.cu:45-86).  Every value
-// is computed with the same operations as the unfused kernels.
-template <class T, int MAXD, bool RAT>
-__global__ __launch_bounds__(256) void k_eval_centres(SurfDev<T> sd, const vec4_t<T>* regions, int nunits, int nuv,
-                                                      vec4_t<T>* C) {
+// Fused evaluation + distance AABB centres.  One wavefront per 16x16-cell
+// tile of a unit (a refinement unit is one tile).  The tile's 17 corner
+// columns and 17 corner rows get their spans and basis values (A2.1, A2.3
+// order 0: the values row ders[0] does not depend on the derivative order);
+// EvaluateOneUV's S row (evaluation.cu:281-300 / :313-348) is then formed in
+// its two contractions: the u-sum `row` of control row r for corner column e
+// depends on (e, r) only, so it is computed once per (e, r) into LDS and
+// shared by every corner of that column, and each corner accumulates its q+1
+// rows in the reference order, followed by the 1/W scaling of the quotient
+// rule; each cell's four corners reduce to its centre
+// (construct_aabbs_distance_host, distance_analysis.cu:45-86).  The operations
+// on every value are those of the reference, so results are bit-identical.
+// Control rows are processed in chunks of kRows (a tile normally needs one).
+// KL: the knot vectors are staged in LDS (chosen on the host from their
+// length), so the span search reads LDS instead of chaining HBM loads.
+// A single wavefront keeps ~8-12 KB of LDS, so many tiles are resident per
+// CU and their memory latencies overlap.  Double only: with absmax the
+// largest |centre coordinate| is max-reduced into absmax[block % kAmaxSlots]
+// (rounding margins of the screens); with C32 the centres are also written
+// rounded to float (FP32 screening).
+static_assert(kAmaxSlots == 64, "the filter reduces the slots across one wavefront");
+constexpr int kEcKnots = 128;  // knots per direction staged in LDS when KL
+template <int MAXD>
+struct EcRows {  // control rows per chunk of u-contractions
+  static constexpr int value = MAXD <= 3 ? 8 : 16;
+};
+
+template <class T, int MAXD, bool RAT, bool KL>
+__global__ __launch_bounds__(64) void k_eval_centres(SurfDev<T> sd, const vec4_t<T>* regions, int nunits, int nuv,
+                                                     vec4_t<T>* C, float4* C32, unsigned long long* absmax) {
   constexpr int E = kTile + 1;  // corner points per tile edge
-  __shared__ T sNu[E][MAXD + 1], sNv[E][MAXD + 1];
+  constexpr int kRows = EcRows<MAXD>::value;
+  constexpr int kCorners = (E * E + 63) / 64;  // corners per lane
+  constexpr int kK = KL ? 2 * kEcKnots : 1;
+  constexpr int kPts = 3 * E * E;
+  // LDS strides are padded so that the lanes of a wavefront, which read
+  // different corner columns, fall on different banks
+  constexpr int kNb = MAXD + 2;           // basis row stride
+  constexpr int kRs = 4 * kRows + 1;      // u-contraction column stride
+  __shared__ T sNu[E][kNb], sNv[E][kNb];
   __shared__ int sSu[E], sSv[E];
-  __shared__ T sP[E * E][3];
+  __shared__ T sR[E][kRs];                 // [column][4 * row + coordinate]
+  __shared__ T sA[kK > kPts ? kK : kPts];  // knots (phases 0-1), then corner points x[], y[], z[]
+  T* const sKu = sA;
+  T* const sKv = sA + kEcKnots;
+  T* const sP = sA;
+  const int lane = threadIdx.x;
   const int w = nuv - 1;
   const int tw = (w + kTile - 1) / kTile, tpu = tw * tw;
   const int unit = blockIdx.x / tpu, t = blockIdx.x % tpu;
-  if (unit >= nunits) return;
+  if (unit >= nunits) return;  // whole wavefront
   const int row0 = (t / tw) * kTile, col0 = (t % tw) * kTile;
   const int ht = min(kTile, w - row0), wd = min(kTile, w - col0);  // cells
   const int p = sd.degu, q = sd.degv;
   const vec4_t<T> r = regions[unit];
-  if (!sd.zero && threadIdx.x < 2 * E) {
-    const int dir = threadIdx.x >= E;
-    const int e = dir ? threadIdx.x - E : threadIdx.x;
-    if (e <= (dir ? ht : wd)) {
+  // phase 0: knot vectors in LDS
+  if (KL && !sd.zero) {
+    for (int x = lane; x < sd.nku; x += 64) sKu[x] = sd.U[x];
+    for (int x = lane; x < sd.nkv; x += 64) sKv[x] = sd.V[x];
+  }
+  __syncthreads();
+  // phase 1: spans and basis values of the corner columns and rows
+  if (!sd.zero)
+    for (int task = lane; task < 2 * E; task += 64) {
+      const int dir = task >= E;
+      const int e = dir ? task - E : task;
+      if (e > (dir ? ht : wd)) continue;
       const T start = dir ? r.y : r.x;
       const T step = ((dir ? r.w : r.z) - start) / T(nuv);
       const int idx = (dir ? row0 : col0) + e;
       const T tt = step * T(idx) + start;
       const int deg = dir ? q : p;
-      const T* knots = dir ? sd.V : sd.U;
+      const T* knots = KL ? (dir ? sKv : sKu) : (dir ? sd.V : sd.U);
       const int span = find_span(dir ? sd.nkv : sd.nku, deg, tt, knots);
-      T d[3][MAXD + 1];
-      basis_ders<T, MAXD>(span, deg, tt, knots, 0, d);
-      for (int j = 0; j <= deg; ++j) (dir ? sNv : sNu)[e][j] = d[0][j];
+      T nb[MAXD + 1];
+      basis_vals<T, MAXD>(span, deg, tt, knots, nb);
+      constexpr int UB = MAXD <= 7 ? MAXD + 1 : 1;
+#pragma unroll UB
+      for (int j = 0; j <= MAXD; ++j)
+        if (j <= deg) (dir ? sNv : sNu)[e][j] = nb[j];
       (dir ? sSv : sSu)[e] = span;
     }
-  }
   __syncthreads();
-  for (int x = threadIdx.x; x < E * E; x += blockDim.x) {
+  // control rows the tile reads: [vlo - q, vhi] (a wavefront reduction over
+  // the corner rows' spans)
+  int vlo = (!sd.zero && lane <= ht) ? sSv[lane] : INT_MAX;
+  int vhi = (!sd.zero && lane <= ht) ? sSv[lane] : INT_MIN;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    vlo = min(vlo, __shfl_xor(vlo, off));
+    vhi = max(vhi, __shfl_xor(vhi, off));
+  }
+  const int v0 = vlo - q, nrows = sd.zero ? 0 : vhi - vlo + q + 1;
+  T acc[kCorners][4];
+#pragma unroll
+  for (int c = 0; c < kCorners; ++c) acc[c][0] = acc[c][1] = acc[c][2] = acc[c][3] = T(0);
+  for (int r0 = 0; r0 < nrows; r0 += kRows) {
+    const int nr = min(kRows, nrows - r0);
+    // phase 2: u-contractions of rows [r0, r0 + nr) for every corner column
+    for (int task = lane; task < (wd + 1) * nr; task += 64) {
+      const int ex = task / nr, rr = task % nr;
+      const int su = sSu[ex];
+      const int base = sd.ncpu * (v0 + r0 + rr) + su - p;
+      T row[4] = {T(0), T(0), T(0), T(0)};
+      constexpr int UR = MAXD <= 7 ? MAXD + 1 : 1;
+#pragma unroll UR
+      for (int j = 0; j <= MAXD; ++j) {
+        if (j > p) break;
+        const int id = base + j;
+        const T n = sNu[ex][j];
+        if (RAT) {
+          const T nw = n * sd.W[id];
+          row[0] += nw * sd.CP[3 * id + 0];
+          row[1] += nw * sd.CP[3 * id + 1];
+          row[2] += nw * sd.CP[3 * id + 2];
+          row[3] += nw;
+        } else {
+          row[0] += n * sd.CP[3 * id + 0];
+          row[1] += n * sd.CP[3 * id + 1];
+          row[2] += n * sd.CP[3 * id + 2];
+        }
+      }
+      sR[ex][4 * rr + 0] = row[0];
+      sR[ex][4 * rr + 1] = row[1];
+      sR[ex][4 * rr + 2] = row[2];
+      sR[ex][4 * rr + 3] = row[3];
+    }
+    __syncthreads();
+    // phase 3: each corner adds the rows of this chunk, in s2 order
+#pragma unroll
+    for (int c = 0; c < kCorners; ++c) {
+      const int x = lane + 64 * c;
+      const int ey = x / E, ex = x % E;
+      if (x >= E * E || ey > ht || ex > wd) continue;
+      const int sv = sSv[ey];
+      constexpr int UR = MAXD <= 7 ? MAXD + 1 : 1;
+#pragma unroll UR
+      for (int s2 = 0; s2 <= MAXD; ++s2) {
+        if (s2 > q) break;
+        const int rr = sv - q + s2 - v0 - r0;
+        if (rr < 0 || rr >= nr) continue;
+        const T nvs = sNv[ey][s2];
+        acc[c][0] += nvs * sR[ex][4 * rr + 0];
+        acc[c][1] += nvs * sR[ex][4 * rr + 1];
+        acc[c][2] += nvs * sR[ex][4 * rr + 2];
+        if (RAT) acc[c][3] += nvs * sR[ex][4 * rr + 3];
+      }
+    }
+    __syncthreads();  // sR is rewritten by the next chunk
+  }
+  // phase 4: corner points into sP (reuses the knot storage)
+#pragma unroll
+  for (int c = 0; c < kCorners; ++c) {
+    const int x = lane + 64 * c;
+    if (x >= E * E) continue;
     const int ey = x / E, ex = x % E;
     T S[3] = {T(0), T(0), T(0)};
     if (ey <= ht && ex <= wd && !sd.zero) {
-      const int su = sSu[ex], sv = sSv[ey];
-      T acc[4] = {T(0), T(0), T(0), T(0)};
-      for (int s2 = 0; s2 <= q; ++s2) {
-        const int base = sd.ncpu * (sv - q + s2) + su - p;
-        T row[4] = {T(0), T(0), T(0), T(0)};
-        for (int j = 0; j <= p; ++j) {
-          const int id = base + j;
-          const T n = sNu[ex][j];
-          if (RAT) {
-            const T nw = n * sd.W[id];
-            row[0] += nw * sd.CP[3 * id + 0];
-            row[1] += nw * sd.CP[3 * id + 1];
-            row[2] += nw * sd.CP[3 * id + 2];
-            row[3] += nw;
-          } else {
-            row[0] += n * sd.CP[3 * id + 0];
-            row[1] += n * sd.CP[3 * id + 1];
-            row[2] += n * sd.CP[3 * id + 2];
-          }
-        }
-        const T nv = sNv[ey][s2];
-        acc[0] += nv * row[0];
-        acc[1] += nv * row[1];
-        acc[2] += nv * row[2];
-        if (RAT) acc[3] += nv * row[3];
-      }
       if (RAT) {
-        const T F = T(1) / acc[3];
-        S[0] = acc[0] * F;
-        S[1] = acc[1] * F;
-        S[2] = acc[2] * F;
+        const T F = T(1) / acc[c][3];
+        S[0] = acc[c][0] * F;
+        S[1] = acc[c][1] * F;
+        S[2] = acc[c][2] * F;
       } else {
-        S[0] = acc[0];
-        S[1] = acc[1];
-        S[2] = acc[2];
+        S[0] = acc[c][0];
+        S[1] = acc[c][1];
+        S[2] = acc[c][2];
       }
     }
-    sP[x][0] = S[0];
-    sP[x][1] = S[1];
-    sP[x][2] = S[2];
+    sP[x] = S[0];
+    sP[E * E + x] = S[1];
+    sP[2 * E * E + x] = S[2];
   }
   __syncthreads();
-  const int ly = threadIdx.x / kTile, lx = threadIdx.x % kTile;
-  if (ly < ht && lx < wd) {
+  // phase 5: cell centres
+  T am = T(0);
+  for (int cidx = lane; cidx < kTileCells; cidx += 64) {
+    const int ly = cidx / kTile, lx = cidx % kTile;
+    if (ly >= ht || lx >= wd) continue;
     const int p0 = ly * E + lx, p1 = p0 + 1, p2 = p0 + E, p3 = p2 + 1;
     vec4_t<T> ctr;
     T* c = &ctr.x;
     const T k = T(0);
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-      const T a0 = sP[p0][a], a1 = sP[p1][a], a2 = sP[p2][a], a3 = sP[p3][a];
+      const T* pa = sP + a * E * E;
+      const T a0 = pa[p0], a1 = pa[p1], a2 = pa[p2], a3 = pa[p3];
       const T lo = smin(a0, smin(a1, smin(a2, a3)));
       const T hi = smax(a0, smax(a1, smax(a2, a3)));
       c[a] = (lo - k + hi + k) / T(2);
     }
     ctr.w = T(0);
-    C[(size_t)unit * w * w + (row0 + ly) * w + col0 + lx] = ctr;
+    const size_t o = (size_t)unit * w * w + (row0 + ly) * w + col0 + lx;
+    C[o] = ctr;
+    if constexpr (std::is_same<T, double>::value) {
+      if (C32) {
+        float4 f;
+        f.x = __double2float_rn(ctr.x);
+        f.y = __double2float_rn(ctr.y);
+        f.z = __double2float_rn(ctr.z);
+        f.w = 0.f;
+        C32[o] = f;
+      }
+      am = fmax(am, fmax(fabs(ctr.x), fmax(fabs(ctr.y), fabs(ctr.z))));
+    }
+  }
+  if constexpr (std::is_same<T, double>::value) {
+    if (absmax) {  // uniform: a kernel argument
+      unsigned long long b = (unsigned long long)__double_as_longlong(am);
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_xor(b, off);
+        b = o > b ? o : b;
+      }
+      if (lane == 0 && b) atomicMax(&absmax[blockIdx.x % kAmaxSlots], b);
+    }
   }
 }
 
@@ -589,15 +730,15 @@ __device__ __forceinline__ uint64_t tp_encode(int slot, int t1, int t2) {
 
 // Seed tile pairs of the distance screen.  For every (slot, side-1 tile) the
 // side-2 tile with the smallest upper bound on centre d2 is kept (packed
-// key: upper-bound bits << 32 | side-2 tile), then the kSeedMax best of those
-// form the seed list.  The exact k-th d2 over the seed's cell pairs bounds
-// the global k-th d2 from above; any choice is valid, a good one is tight.
+// key: upper-bound bits << 32 | side-2 tile), then the nseed best of those
+// (at most kSeedMaxTiles) form the seed list.  The exact k-th d2 over the
+// seed's cell pairs bounds the global k-th d2 from above; any choice is
+// valid, a good one is tight.
 template <class T, bool FMA>
 __device__ __forceinline__ T pair_d2(T ax, T ay, T az, T bx, T by, T bz);
 
-constexpr int kSeedMax = 64;
+constexpr int kSeedMaxTiles = 256;  // seed tile pairs at most
 constexpr int kSeedSortMax = 8192;
-constexpr int kSeedBins = 4096;
 
 // One workgroup per (slot, side-1 tile): the side-2 tile with the smallest
 // upper bound, packed as (upper-bound bits << 32 | side-2 tile).
@@ -689,37 +830,42 @@ __global__ __launch_bounds__(1024) void k_seed_list(const unsigned long long* be
   }
 }
 
-// Two passes over the d2 of the seed tile pairs (no records are written).
-// MINMAX: the smallest and largest key.  Otherwise: a histogram with bins
-// linear in the key bits over [min, max], so its resolution follows the
-// spread of the values (refined cells have d2 values within 1e-5 of each other).
-constexpr int kSeedSplit = 16;  // workgroups per seed tile pair
+constexpr int kSeedSplitMax = 16;    // workgroups per seed tile pair at most
+constexpr int kSeedHistBins = 1024;  // bins of the seed histogram
 
+// Two passes over the d2 of the seed tile pairs (no candidate records are
+// written).  Each workgroup handles one seed tile pair against a 1/split
+// slice of its side-2 cells.  MINMAX: per-workgroup smallest and largest key
+// -> mm[2*block].  Otherwise: a histogram in LDS, bins linear in the key bits
+// over [min, max] (its resolution follows the spread of the values: refined
+// cells have d2 within 1e-5 of each other), whose non-zero bins are added to
+// the global histogram gh.
 template <class T, bool FMA, bool MINMAX>
 __global__ __launch_bounds__(256) void k_seed_hist(const uint64_t* list, const unsigned* list_count, const int* su1,
                                                    const int* su2, int w1, int w2, const vec4_t<T>* C1,
-                                                   const vec4_t<T>* C2, unsigned long long* range,
-                                                   unsigned* ghist) {
+                                                   const vec4_t<T>* C2, int split, const unsigned long long* range,
+                                                   unsigned long long* mm, unsigned* gh) {
   __shared__ vec4_t<T> sB[kTileCells];
-  __shared__ unsigned h[MINMAX ? 1 : kSeedBins];
+  __shared__ unsigned h[MINMAX ? 1 : kSeedHistBins];
+  __shared__ unsigned long long rmin[4], rmax[4];
   const unsigned n = *list_count;
   unsigned long long lo = 0, width = 1;
   if (!MINMAX) {
     lo = range[0];
     const unsigned long long hi = range[1];
-    width = (hi > lo ? hi - lo : 0ull) / kSeedBins + 1;
-    for (int b = threadIdx.x; b < kSeedBins; b += blockDim.x) h[b] = 0;
+    width = (hi > lo ? hi - lo : 0ull) / kSeedHistBins + 1;
+    for (int b = threadIdx.x; b < kSeedHistBins; b += blockDim.x) h[b] = 0;
   }
   unsigned long long kmin = ~0ull, kmax = 0ull;
-  const unsigned it = blockIdx.x / kSeedSplit;
-  const int part = blockIdx.x % kSeedSplit;
+  const unsigned it = blockIdx.x / split;
+  const int part = blockIdx.x % split;
   if (it < n) {
     const uint64_t e = list[it];
     const int slot = (int)(e >> 32), t1 = (int)((e >> 16) & 0xffff), t2 = (int)(e & 0xffff);
     const int u1 = su1[slot], u2 = su2[slot];
     const TileGeo g1 = tile_geo(t1, w1), g2 = tile_geo(t2, w2);
     const int nA = g1.ht * g1.wd, nB = g2.ht * g2.wd;
-    const int j0 = part * nB / kSeedSplit, j1 = (part + 1) * nB / kSeedSplit;
+    const int j0 = part * nB / split, j1 = (part + 1) * nB / split;
     if ((int)threadIdx.x < j1 - j0) {
       const int jj = j0 + threadIdx.x;
       const int j = (g2.row0 + jj / g2.wd) * w2 + g2.col0 + jj % g2.wd;
@@ -737,7 +883,7 @@ __global__ __launch_bounds__(256) void k_seed_hist(const uint64_t* list, const u
           kmax = key > kmax ? key : kmax;
         } else {
           unsigned long long bin = key <= lo ? 0ull : (key - lo) / width;
-          if (bin >= (unsigned long long)kSeedBins) bin = kSeedBins - 1;
+          if (bin >= (unsigned long long)kSeedHistBins) bin = kSeedHistBins - 1;
           atomicAdd(&h[bin], 1u);
         }
       }
@@ -751,52 +897,83 @@ __global__ __launch_bounds__(256) void k_seed_hist(const uint64_t* list, const u
       kmax = b > kmax ? b : kmax;
     }
     if (lane_id() == 0) {
-      if (kmin != ~0ull) atomicMin(&range[0], kmin);
-      if (kmax) atomicMax(&range[1], kmax);
+      rmin[threadIdx.x >> 6] = kmin;
+      rmax[threadIdx.x >> 6] = kmax;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int w = 1; w < 4; ++w) {
+        kmin = rmin[w] < kmin ? rmin[w] : kmin;
+        kmax = rmax[w] > kmax ? rmax[w] : kmax;
+      }
+      mm[2 * blockIdx.x] = kmin;
+      mm[2 * blockIdx.x + 1] = kmax;
     }
     return;
   }
   __syncthreads();
-  for (int b = threadIdx.x; b < kSeedBins; b += blockDim.x)
-    if (h[b]) atomicAdd(&ghist[b], h[b]);
+  for (int b = threadIdx.x; b < kSeedHistBins; b += blockDim.x)
+    if (h[b]) atomicAdd(&gh[b], h[b]);
 }
 
-// threshold = upper edge of the bin holding rank k (at least k seed pairs lie
-// at or below it); +inf when the seeds hold fewer than k pairs.
-template <class T>
-__global__ __launch_bounds__(1024) void k_seed_thresh(const unsigned* ghist, const unsigned long long* range, int k,
-                                                      T* thresh) {
-  __shared__ unsigned long long cum[1024];
-  __shared__ int found;
-  constexpr int per = kSeedBins / 1024;
-  unsigned long long loc = 0;
-  for (int q = 0; q < per; ++q) loc += ghist[threadIdx.x * per + q];
-  cum[threadIdx.x] = loc;
-  if (threadIdx.x == 0) found = -1;
+// Single workgroup: reduce the per-workgroup min/max into range[0..1] and
+// clear the global histogram for the second pass.
+__global__ __launch_bounds__(1024) void k_seed_range(const unsigned long long* mm, int nblocks,
+                                                     unsigned long long* range, unsigned* gh) {
+  __shared__ unsigned long long smin[1024], smax[1024];
+  unsigned long long a = ~0ull, b = 0ull;
+  for (int x = threadIdx.x; x < nblocks; x += blockDim.x) {
+    a = mm[2 * x] < a ? mm[2 * x] : a;
+    b = mm[2 * x + 1] > b ? mm[2 * x + 1] : b;
+  }
+  smin[threadIdx.x] = a;
+  smax[threadIdx.x] = b;
+  for (int x = threadIdx.x; x < kSeedHistBins; x += blockDim.x) gh[x] = 0u;
   __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    const unsigned long long v = threadIdx.x >= (unsigned)off ? cum[threadIdx.x - off] : 0ull;
-    __syncthreads();
-    cum[threadIdx.x] += v;
+  for (int off = 512; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) {
+      smin[threadIdx.x] = smin[threadIdx.x + off] < smin[threadIdx.x] ? smin[threadIdx.x + off] : smin[threadIdx.x];
+      smax[threadIdx.x] = smax[threadIdx.x + off] > smax[threadIdx.x] ? smax[threadIdx.x + off] : smax[threadIdx.x];
+    }
     __syncthreads();
   }
-  unsigned long long run = cum[threadIdx.x] - loc;
-  for (int q = 0; q < per; ++q) {
-    const unsigned c = ghist[threadIdx.x * per + q];
-    if (run < (unsigned long long)k && run + c >= (unsigned long long)k) found = threadIdx.x * per + q;
-    run += c;
-  }
-  __syncthreads();
   if (threadIdx.x == 0) {
+    range[0] = smin[0];
+    range[1] = smax[0];
+  }
+}
+
+// Single workgroup: threshold = upper edge of the histogram bin holding rank
+// k (at least k seed pairs lie at or below it); +inf when the seeds hold
+// fewer than k pairs.
+template <class T>
+__global__ __launch_bounds__(1024) void k_seed_thresh(const unsigned* gh, const unsigned long long* range, int k,
+                                                      T* thresh) {
+  __shared__ unsigned long long cum[kSeedHistBins];
+  __shared__ int found;
+  const int b = threadIdx.x;  // blockDim.x == kSeedHistBins
+  const unsigned long long c = gh[b];
+  cum[b] = c;
+  if (b == 0) found = -1;
+  __syncthreads();
+  for (int off = 1; off < kSeedHistBins; off <<= 1) {
+    const unsigned long long v = b >= off ? cum[b - off] : 0ull;
+    __syncthreads();
+    cum[b] += v;
+    __syncthreads();
+  }
+  const unsigned long long before = cum[b] - c;
+  if (before < (unsigned long long)k && cum[b] >= (unsigned long long)k) found = b;
+  __syncthreads();
+  if (b == 0) {
     if (found < 0) {
       *thresh = infinity_v<T>();
     } else {
       const unsigned long long lo = range[0], hi = range[1];
-      const unsigned long long width = (hi > lo ? hi - lo : 0ull) / kSeedBins + 1;
-      const unsigned long long maxbits = sizeof(T) == 4 ? 0x7f800000ull : 0x7ff0000000000000ull;
+      const unsigned long long width = (hi > lo ? hi - lo : 0ull) / kSeedHistBins + 1;
       unsigned long long edge = lo + (unsigned long long)(found + 1) * width - 1;
-      if (edge > hi) edge = hi;  // hi is the largest seed key, so rank k is at or below it
-      *thresh = edge >= maxbits ? infinity_v<T>() : from_bits<T>(edge);
+      if (edge > hi) edge = hi;  // hi is the largest seed key, so rank k lies at or below it
+      *thresh = from_bits<T>(edge);
     }
   }
 }
@@ -866,15 +1043,27 @@ __device__ __forceinline__ double quiet_nan<double>() {
 // sorted by the full key afterwards).  One tile pair per workgroup iteration;
 // side-2 cells are staged in LDS (padded to a multiple of 8 with NaN, which
 // never passes the test) and read as broadcasts, each lane owns one side-1
-// cell.  The j loop is unrolled by 8 and tests the minimum of 8 d2 values
-// once; only a passing group re-tests its members and appends them.
-template <class T, bool FMA>
+// cell.  The j loop is unrolled by 8 and tests the minimum of 8 values once;
+// only a passing group re-tests its members and appends them.
+//
+// EXP (double): the test runs on the expanded form.  Side-2 cells are staged
+// as (-2b, |b|^2), so s = a.(-2b) + |b|^2 = d2 - |a|^2 costs three FMAs, and
+// a pair passes when s <= T - |a|^2 + delta, delta = 32 u (T + 4 m^2), u =
+// 2^-53, m = largest |centre coordinate| of the round (*absmax).  delta
+// bounds the rounding of s, of |a|^2, |b|^2, of T - |a|^2 and of the
+// reference formula dx*dx + dy*dy + dz*dz (each at most a few u (T + 3 m^2)),
+// so every pair whose reference d2 is <= T passes.  A passing pair's key is
+// the reference d2 recomputed from a and b = -(1/2)(-2b) (exact), so records
+// are identical to the direct test's; pairs that pass only by the margin carry
+// their true key and cannot displace a smaller one.
+template <class T, bool FMA, bool EXP>
 __global__ __launch_bounds__(256) void k_dist_filter(const uint64_t* list, const unsigned* list_count,
                                                      const int* su1, const int* su2, int w1, int w2,
                                                      const vec4_t<T>* C1, const vec4_t<T>* C2, const T* thresh,
                                                      Cand* out, unsigned long long* out_count,
                                                      unsigned long long cap, unsigned long long* pairs_done,
-                                                     int cull) {
+                                                     int cull, const unsigned long long* absmax) {
+  static_assert(!EXP || std::is_same<T, double>::value, "expanded test: double only");
   constexpr int U = 8;
   constexpr int kStage = 128;  // candidates staged per wavefront before one global reservation
   __shared__ vec4_t<T> sB[kTileCells];
@@ -884,6 +1073,17 @@ __global__ __launch_bounds__(256) void k_dist_filter(const uint64_t* list, const
   unsigned long long done = 0;
   const unsigned n = *list_count;
   const T tv = *thresh;
+  T delta = T(0);
+  if (EXP) {
+    unsigned long long mb = absmax[lane_id()];  // kAmaxSlots == wavefront size
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const unsigned long long o = __shfl_xor(mb, off);
+      mb = o > mb ? o : mb;
+    }
+    const T m = __longlong_as_double((long long)mb);
+    delta = T(32) * T(1.1102230246251565e-16) * (tv + T(4) * m * m);
+  }
   const int lane = lane_id();
   const int wv = threadIdx.x >> 6;
   int wcount = 0;  // staged candidates of this wavefront (wave-uniform)
@@ -948,6 +1148,20 @@ __global__ __launch_bounds__(256) void k_dist_filter(const uint64_t* list, const
       sLo[threadIdx.x] = lo;
       sHi[threadIdx.x] = hi;
     }
+    T thr = tv;
+    if (EXP) {
+      __syncthreads();  // chunk boxes read the plain centres
+      if ((int)threadIdx.x < nB) {
+        const vec4_t<T> b = sB[threadIdx.x];
+        vec4_t<T> x;
+        x.x = T(-2) * b.x;
+        x.y = T(-2) * b.y;
+        x.z = T(-2) * b.z;
+        x.w = b.x * b.x + b.y * b.y + b.z * b.z;
+        sB[threadIdx.x] = x;
+      }
+      thr = (tv - (a.x * a.x + a.y * a.y + a.z * a.z)) + delta;  // NaN for inactive lanes
+    }
     __syncthreads();
     const uint64_t key_si = pack_idx((uint32_t)slot, (uint32_t)i, 0u);
     const int nvalid = __popcll(__ballot(va));
@@ -973,20 +1187,28 @@ __global__ __launch_bounds__(256) void k_dist_filter(const uint64_t* list, const
 #pragma unroll
       for (int q = 0; q < U; ++q) {
         const vec4_t<T> b = sB[j0 + q];
-        d[q] = pair_d2<T, FMA>(a.x, a.y, a.z, b.x, b.y, b.z);
+        if (EXP)
+          d[q] = fma(a.x, b.x, fma(a.y, b.y, fma(a.z, b.z, b.w)));
+        else
+          d[q] = pair_d2<T, FMA>(a.x, a.y, a.z, b.x, b.y, b.z);
       }
       T m = fmin(fmin(fmin(d[0], d[1]), fmin(d[2], d[3])), fmin(fmin(d[4], d[5]), fmin(d[6], d[7])));
-      if (__ballot(m <= tv) == 0ull) continue;
+      if (__ballot(m <= thr) == 0ull) continue;
 #pragma unroll
       for (int q = 0; q < U; ++q) {
-        const bool hit = d[q] <= tv;
+        const bool hit = d[q] <= thr;
         const unsigned long long mk = __ballot(hit);
         if (mk) {
           const int cnt = __popcll(mk);
           if (wcount + cnt > kStage) flush();
           if (hit) {
+            T dq = d[q];
+            if (EXP) {
+              const vec4_t<T> b = sB[j0 + q];
+              dq = pair_d2<T, false>(a.x, a.y, a.z, T(-0.5) * b.x, T(-0.5) * b.y, T(-0.5) * b.z);
+            }
             Cand c;
-            c.hi = ord_bits(d[q]);
+            c.hi = ord_bits(dq);
             c.lo = key_si | (uint64_t)(uint32_t)sJ[j0 + q];
             stage[wv][wcount + __popcll(mk & ((1ull << lane) - 1))] = c;
           }
@@ -997,6 +1219,138 @@ __global__ __launch_bounds__(256) void k_dist_filter(const uint64_t* list, const
   }
   if (wcount) flush();
   if (pairs_done && lane == 0 && done) atomicAdd(pairs_done, done);
+}
+
+// Distance filter for slots whose units are single tiles (w1, w2 <= 16: the
+// refinement rounds).  Same records as k_dist_filter (including the EXP
+// test), different work split: each lane keeps R = 4 side-1 cells in
+// registers and each wavefront takes a quarter of the side-2 cells, so one
+// broadcast LDS read of a side-2 cell serves four pairs per lane (the tile
+// kernel is bound by the LDS return bandwidth of those broadcasts).  Within a
+// group of U side-2 cells each of the R cells tests the minimum of its U
+// values first.  No chunk culling: refined units are near each other
+// everywhere, so it would prune nothing.
+template <class T, bool FMA, bool EXP>
+__global__ __launch_bounds__(256) void k_dist_filter_units(const uint64_t* list, const unsigned* list_count,
+                                                           const int* su1, const int* su2, int w1, int w2,
+                                                           const vec4_t<T>* C1, const vec4_t<T>* C2,
+                                                           const T* thresh, Cand* out, unsigned long long* out_count,
+                                                           unsigned long long cap, unsigned long long* pairs_done,
+                                                           const unsigned long long* absmax) {
+  static_assert(!EXP || std::is_same<T, double>::value, "expanded test: double only");
+  constexpr int R = 4;  // side-1 cells per lane: 4 x 64 covers a 16x16 unit
+  constexpr int U = 4;  // side-2 cells per unrolled group
+  constexpr int kStage = 128;
+  __shared__ vec4_t<T> sB[kTileCells];
+  __shared__ Cand stage[4][kStage];
+  unsigned long long done = 0;
+  const unsigned n = *list_count;
+  const T tv = *thresh;
+  const int lane = lane_id();
+  const int wv = threadIdx.x >> 6;
+  T delta = T(0);
+  if (EXP) {
+    unsigned long long mb = absmax[lane];  // kAmaxSlots == wavefront size
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const unsigned long long o = __shfl_xor(mb, off);
+      mb = o > mb ? o : mb;
+    }
+    const T m = __longlong_as_double((long long)mb);
+    delta = T(32) * T(1.1102230246251565e-16) * (tv + T(4) * m * m);
+  }
+  const int nA = w1 * w1, nB = w2 * w2;
+  const int nBp = (nB + 4 * U - 1) / (4 * U) * (4 * U);
+  const int per = nBp / 4;  // side-2 cells per wavefront, a multiple of U
+  int wcount = 0;
+  auto flush = [&]() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    unsigned long long base = 0;
+    if (lane == 0) base = atomicAdd(out_count, (unsigned long long)wcount);
+    base = __shfl(base, 0);
+    for (int e = lane; e < wcount; e += 64)
+      if (base + e < cap) out[base + e] = stage[wv][e];
+    __builtin_amdgcn_wave_barrier();
+    wcount = 0;
+  };
+  for (unsigned it = blockIdx.x; it < n; it += gridDim.x) {
+    const int slot = (int)(list[it] >> 32);
+    const int u1 = su1[slot], u2 = su2[slot];
+    __syncthreads();  // sB of the previous slot is no longer read
+    if ((int)threadIdx.x < nB) {
+      vec4_t<T> b = C2[(size_t)u2 * nB + threadIdx.x];
+      if (EXP) {
+        vec4_t<T> x;
+        x.x = T(-2) * b.x;
+        x.y = T(-2) * b.y;
+        x.z = T(-2) * b.z;
+        x.w = b.x * b.x + b.y * b.y + b.z * b.z;
+        b = x;
+      }
+      sB[threadIdx.x] = b;
+    } else if ((int)threadIdx.x < nBp) {
+      vec4_t<T> b;
+      b.x = b.y = b.z = b.w = quiet_nan<T>();  // never passes
+      sB[threadIdx.x] = b;
+    }
+    vec4_t<T> a[R];
+    T thr[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int i = lane + 64 * r;
+      a[r].x = a[r].y = a[r].z = a[r].w = quiet_nan<T>();  // absent cells never pass
+      if (i < nA) a[r] = C1[(size_t)u1 * nA + i];
+      thr[r] = EXP ? (tv - (a[r].x * a[r].x + a[r].y * a[r].y + a[r].z * a[r].z)) + delta : tv;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) done += (unsigned long long)nA * (unsigned long long)nB;
+    const uint64_t key_s = pack_idx((uint32_t)slot, 0u, 0u);
+    for (int j0 = wv * per; j0 < (wv + 1) * per; j0 += U) {
+      T d[U][R];
+#pragma unroll
+      for (int q = 0; q < U; ++q) {
+        const vec4_t<T> b = sB[j0 + q];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          if (EXP)
+            d[q][r] = fma(a[r].x, b.x, fma(a[r].y, b.y, fma(a[r].z, b.z, b.w)));
+          else
+            d[q][r] = pair_d2<T, FMA>(a[r].x, a[r].y, a[r].z, b.x, b.y, b.z);
+        }
+      }
+      bool any = false;
+#pragma unroll
+      for (int r = 0; r < R; ++r) any |= fmin(fmin(d[0][r], d[1][r]), fmin(d[2][r], d[3][r])) <= thr[r];
+      if (__ballot(any) == 0ull) continue;
+#pragma unroll
+      for (int q = 0; q < U; ++q)
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const bool hit = d[q][r] <= thr[r];
+          const unsigned long long mk = __ballot(hit);
+          if (mk) {
+            const int cnt = __popcll(mk);
+            if (wcount + cnt > kStage) flush();
+            if (hit) {
+              T dq = d[q][r];
+              if (EXP) {
+                const vec4_t<T> b = sB[j0 + q];
+                dq = pair_d2<T, false>(a[r].x, a[r].y, a[r].z, T(-0.5) * b.x, T(-0.5) * b.y, T(-0.5) * b.z);
+              }
+              Cand c;
+              c.hi = ord_bits(dq);
+              c.lo = key_s | ((uint64_t)(uint32_t)(lane + 64 * r) << 20) | (uint64_t)(uint32_t)(j0 + q);
+              stage[wv][wcount + __popcll(mk & ((1ull << lane) - 1))] = c;
+            }
+            wcount += cnt;
+          }
+        }
+    }
+  }
+  if (wcount) flush();
+  if (pairs_done && threadIdx.x == 0 && done) atomicAdd(pairs_done, done);
 }
 
 // ---------------------------------------------------------------------------
@@ -1036,7 +1390,8 @@ __global__ void k_narrow(const double4* __restrict__ src, float4* __restrict__ d
 }
 
 // T <- (sqrt(T)(1+16u) + 4 sqrt(3) E (1+16u))^2 (1+16u) + 1e-36, u = 2^-24,
-// E = 2^-24 max|centre coordinate|: one widening covers (a) the float
+// E = 2^-24 max|centre coordinate| (the largest of the kAmaxSlots partial
+// maxima): one widening covers (a) the float
 // rounding of both centres (<= E per coordinate, <= 2 sqrt(3) E in distance),
 // (b) the float evaluation of d2 (<= 5u relative) and (c) the double
 // evaluation of d2 (<= 5 * 2^-53 relative), applied once to bound the FP64
@@ -1046,7 +1401,9 @@ __global__ void k_inflate(float* thresh, const unsigned long long* absmax_bits) 
   const float t = *thresh;
   if (!(t < __uint_as_float(0x7f800000u))) return;  // +inf stays +inf
   const double u = 5.9604644775390625e-08;         // 2^-24
-  const double E = __longlong_as_double((long long)*absmax_bits) * u;
+  unsigned long long mb = 0;
+  for (int x = 0; x < kAmaxSlots; ++x) mb = absmax_bits[x] > mb ? absmax_bits[x] : mb;
+  const double E = __longlong_as_double((long long)mb) * u;
   const double g = 1.0 + 16.0 * u;
   const double r = sqrt((double)t) * g + 4.0 * 1.7320508075688772 * E * g;
   *thresh = __double2float_ru(r * r * g + 1e-36);
@@ -1347,26 +1704,44 @@ __global__ __launch_bounds__(1024) void k_bsort_single(Cand* a, int n2) {
 
 // Rank sort of m distinct records (the keys (d2, slot, i, j) are unique):
 // each record's position is the number of records smaller than it.  Only
-// positions below `keep` are written.  O(m^2) compares, fully parallel: for
-// the few thousand records left after selection it beats a bitonic network.
-__global__ __launch_bounds__(256) void k_rank_sort(const Cand* in, const unsigned long long* m_dev, unsigned m_cap,
-                                                   unsigned keep, Cand* out) {
-  __shared__ Cand chunk[1024];
+// positions below `keep` are written.  A workgroup ranks 64 records (one per
+// lane); each of its 16 wavefronts counts against 1/16 of the set, staged 64
+// records at a time in LDS (broadcast reads), and the partial ranks are
+// summed in LDS.  O(m^2) compares spread over m/64 * 16 wavefronts.
+constexpr int kRankWaves = 16;
+__global__ __launch_bounds__(1024) void k_rank_sort(const Cand* in, const unsigned long long* m_dev, unsigned m_cap,
+                                                    unsigned keep, Cand* out) {
+  __shared__ Cand stage[kRankWaves][64];
+  __shared__ unsigned part[kRankWaves][64];
   const unsigned m = (unsigned)min((unsigned long long)m_cap, *m_dev);
-  const unsigned x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (blockIdx.x * 64u >= m) return;  // whole workgroup
+  const unsigned lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+  const unsigned x = blockIdx.x * 64u + lane;
   Cand mine;
   mine.hi = mine.lo = ~0ull;
   if (x < m) mine = in[x];
+  const unsigned slice = (m + kRankWaves - 1) / kRankWaves;
+  const unsigned s0 = w * slice, s1 = min(m, s0 + slice);
   unsigned rank = 0;
-  for (unsigned c0 = 0; c0 < m; c0 += 1024) {
-    const unsigned cn = min(1024u, m - c0);
+  for (unsigned c0 = 0; c0 < slice; c0 += 64) {  // same trip count in every wavefront
+    const unsigned t = s0 + c0 + lane;
+    Cand r;
+    r.hi = r.lo = ~0ull;  // never below a real record (real keys have d2 bits < ~0)
+    if (t < s1) r = in[t];
+    stage[w][lane] = r;
     __syncthreads();
-    for (unsigned t = threadIdx.x; t < cn; t += blockDim.x) chunk[t] = in[c0 + t];
+    const unsigned cn = min(64u, slice - c0);
+    for (unsigned q = 0; q < cn; ++q) rank += cand_less(stage[w][q], mine) ? 1u : 0u;
     __syncthreads();
-    if (x < m)
-      for (unsigned t = 0; t < cn; ++t) rank += cand_less(chunk[t], mine) ? 1u : 0u;
   }
-  if (x < m && rank < keep) out[rank] = mine;
+  part[w][lane] = rank;
+  __syncthreads();
+  if (w == 0 && x < m) {
+    unsigned r = 0;
+#pragma unroll
+    for (int v = 0; v < kRankWaves; ++v) r += part[v][lane];
+    if (r < keep) out[r] = mine;
+  }
 }
 
 __global__ void k_bsort_global(Cand* a, unsigned long long n2, unsigned long long k, unsigned long long j) {
@@ -1404,79 +1779,98 @@ __device__ __forceinline__ vec4_t<T> child_region(const vec4_t<T>& pr, int pnuv,
   return c;
 }
 
-__device__ __forceinline__ int atomic_load_i32(const int* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+constexpr int kDedupMax = 8192;
+constexpr int kDedupPer = kDedupMax / 1024;  // records per thread of k_dedup_units
+
+__device__ __forceinline__ void dedup_decode(const Cand* rec, int e, int* s, int* i, int* j) {
+  const uint64_t lo = rec[e].lo;
+  *s = (int)(lo >> 40);
+  *i = (int)((lo >> 20) & 0xfffff);
+  *j = (int)(lo & 0xfffff);
 }
 
-constexpr int kDedupMax = 8192;
+// Pass 1: firstpos[slot * cells + cell] = smallest record index with that
+// (slot, cell), per side (tables hold INT_MAX between uses).
+__global__ __launch_bounds__(256) void k_dedup_mark(const Cand* rec, int n, int cells1, int cells2, int* fp1,
+                                                    int* fp2) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  int s, i, j;
+  dedup_decode(rec, e, &s, &i, &j);
+  atomicMin(&fp1[(long long)s * cells1 + i], e);
+  atomicMin(&fp2[(long long)s * cells2 + j], e);
+}
 
-template <class T>
-__global__ __launch_bounds__(1024) void k_dist_dedup(const Cand* rec, int n, int cells1, int cells2, const int* su1,
-                                                     const int* su2, const vec4_t<T>* reg1, const vec4_t<T>* reg2,
-                                                     int pnuv1, int pnuv2, int* firstpos1, int* firstpos2,
-                                                     int* nsu1, int* nsu2, vec4_t<T>* nreg1, vec4_t<T>* nreg2,
-                                                     int* nunits) {
+// Pass 2 (single workgroup, contiguous chunks of <= 8 records per thread, all
+// loads issued up front): a record whose firstpos is itself opens a new unit;
+// unit ids follow record order (first appearance); every record's next-round
+// slot map entry is the unit of its first appearance.
+__global__ __launch_bounds__(1024) void k_dedup_units(const Cand* rec, int n, int cells1, int cells2,
+                                                      const int* fp1, const int* fp2, int* nsu1, int* nsu2,
+                                                      unsigned char* isfirst, int* nunits) {
   __shared__ int sUnit[kDedupMax];
   __shared__ int sScan[1024];
-  const int per = (n + 1023) / 1024;  // <= 8 entries per thread, contiguous
+  const int per = (n + 1023) / 1024;
+  const int e0 = threadIdx.x * per;
+  int f[2][kDedupPer];
+#pragma unroll
+  for (int q = 0; q < kDedupPer; ++q) {
+    f[0][q] = f[1][q] = -1;
+    const int e = e0 + q;
+    if (q < per && e < n) {
+      int s, i, j;
+      dedup_decode(rec, e, &s, &i, &j);
+      f[0][q] = fp1[(long long)s * cells1 + i];
+      f[1][q] = fp2[(long long)s * cells2 + j];
+    }
+  }
+#pragma unroll
   for (int side = 0; side < 2; ++side) {
-    int* fp = side == 0 ? firstpos1 : firstpos2;
-    const int cells = side == 0 ? cells1 : cells2;
-    const int* su = side == 0 ? su1 : su2;
-    const vec4_t<T>* reg = side == 0 ? reg1 : reg2;
-    const int pnuv = side == 0 ? pnuv1 : pnuv2;
-    int* nsu = side == 0 ? nsu1 : nsu2;
-    vec4_t<T>* nreg = side == 0 ? nreg1 : nreg2;
-    auto key_of = [&](int e, int* slot, int* cell) {
-      const uint64_t lo = rec[e].lo;
-      *slot = (int)(lo >> 40);
-      *cell = side == 0 ? (int)((lo >> 20) & 0xfffff) : (int)(lo & 0xfffff);
-      return (long long)(*slot) * cells + *cell;
-    };
-    for (int e = threadIdx.x; e < n; e += 1024) {
-      int s, c;
-      atomicMin(&fp[key_of(e, &s, &c)], e);
-    }
-    __threadfence();
-    __syncthreads();
-    // first-appearance flags, prefix-counted over the contiguous chunk of each thread
     int local = 0;
-    const int e0 = threadIdx.x * per;
-    for (int e = e0; e < min(n, e0 + per); ++e) {
-      int s, c;
-      local += atomic_load_i32(&fp[key_of(e, &s, &c)]) == e ? 1 : 0;
-    }
+#pragma unroll
+    for (int q = 0; q < kDedupPer; ++q) local += f[side][q] == e0 + q ? 1 : 0;
     sScan[threadIdx.x] = local;
     __syncthreads();
     for (int off = 1; off < 1024; off <<= 1) {
-      const int v = threadIdx.x >= off ? sScan[threadIdx.x - off] : 0;
+      const int v = threadIdx.x >= (unsigned)off ? sScan[threadIdx.x - off] : 0;
       __syncthreads();
       sScan[threadIdx.x] += v;
       __syncthreads();
     }
-    int u = sScan[threadIdx.x] - local;  // exclusive prefix: first unit id of this chunk
-    for (int e = e0; e < min(n, e0 + per); ++e) {
-      int s, c;
-      if (atomic_load_i32(&fp[key_of(e, &s, &c)]) == e) {
-        sUnit[e] = u;
-        nreg[u] = child_region<T>(reg[su[s]], pnuv, c);
-        ++u;
-      }
+    int u = sScan[threadIdx.x] - local;
+#pragma unroll
+    for (int q = 0; q < kDedupPer; ++q) {
+      const int e = e0 + q;
+      if (f[side][q] < 0) continue;
+      const bool first = f[side][q] == e;
+      if (first) sUnit[e] = u++;
+      isfirst[side * kDedupMax + e] = first ? 1 : 0;
     }
     if (threadIdx.x == 1023) nunits[side] = sScan[1023];
     __syncthreads();
-    for (int e = threadIdx.x; e < n; e += 1024) {
-      int s, c;
-      nsu[e] = sUnit[atomic_load_i32(&fp[key_of(e, &s, &c)])];
-    }
-    __syncthreads();
-    for (int e = threadIdx.x; e < n; e += 1024) {
-      int s, c;
-      fp[key_of(e, &s, &c)] = INT_MAX;
-    }
-    __threadfence();
-    __syncthreads();
+    int* nsu = side == 0 ? nsu1 : nsu2;
+#pragma unroll
+    for (int q = 0; q < kDedupPer; ++q)
+      if (f[side][q] >= 0) nsu[e0 + q] = sUnit[f[side][q]];
+    __syncthreads();  // sUnit is reused by the next side
   }
+}
+
+// Pass 3: child regions of the new units, and firstpos back to INT_MAX.
+template <class T>
+__global__ __launch_bounds__(256) void k_dedup_apply(const Cand* rec, int n, int cells1, int cells2, const int* su1,
+                                                     const int* su2, const vec4_t<T>* reg1, const vec4_t<T>* reg2,
+                                                     int pnuv1, int pnuv2, const unsigned char* isfirst,
+                                                     const int* nsu1, const int* nsu2, vec4_t<T>* nreg1,
+                                                     vec4_t<T>* nreg2, int* fp1, int* fp2) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  int s, i, j;
+  dedup_decode(rec, e, &s, &i, &j);
+  if (isfirst[e]) nreg1[nsu1[e]] = child_region<T>(reg1[su1[s]], pnuv1, i);
+  if (isfirst[kDedupMax + e]) nreg2[nsu2[e]] = child_region<T>(reg2[su2[s]], pnuv2, j);
+  fp1[(long long)s * cells1 + i] = INT_MAX;
+  fp2[(long long)s * cells2 + j] = INT_MAX;
 }
 
 // ---------------------------------------------------------------------------
@@ -1779,26 +2173,36 @@ hipError_t launch_points(const SurfDev<T>& sd, const T* tabU, const int* spanU, 
 
 template <class T, int MAXD>
 static void eval_centres_rat(const SurfDev<T>& sd, const vec4_t<T>* reg, int nunits, int nuv, vec4_t<T>* C,
-                             hipStream_t s) {
+                             float4* C32, unsigned long long* absmax, hipStream_t s) {
   const int w = nuv - 1, tw = (w + kTile - 1) / kTile;
-  const dim3 g((unsigned)((long long)nunits * tw * tw)), b(256);
-  if (sd.rational)
-    hipLaunchKernelGGL((k_eval_centres<T, MAXD, true>), g, b, 0, s, sd, reg, nunits, nuv, C);
-  else
-    hipLaunchKernelGGL((k_eval_centres<T, MAXD, false>), g, b, 0, s, sd, reg, nunits, nuv, C);
+  const dim3 g((unsigned)((long long)nunits * tw * tw)), b(64);
+  const bool kl = sd.nku <= kEcKnots && sd.nkv <= kEcKnots;
+  if (sd.rational) {
+    if (kl)
+      hipLaunchKernelGGL((k_eval_centres<T, MAXD, true, true>), g, b, 0, s, sd, reg, nunits, nuv, C, C32, absmax);
+    else
+      hipLaunchKernelGGL((k_eval_centres<T, MAXD, true, false>), g, b, 0, s, sd, reg, nunits, nuv, C, C32, absmax);
+  } else {
+    if (kl)
+      hipLaunchKernelGGL((k_eval_centres<T, MAXD, false, true>), g, b, 0, s, sd, reg, nunits, nuv, C, C32, absmax);
+    else
+      hipLaunchKernelGGL((k_eval_centres<T, MAXD, false, false>), g, b, 0, s, sd, reg, nunits, nuv, C, C32, absmax);
+  }
 }
 
 template <class T>
 hipError_t launch_eval_centres(const SurfDev<T>& sd, const vec4_t<T>* reg, int nunits, int nuv, vec4_t<T>* C,
-                               hipStream_t s) {
+                               hipStream_t s, float4* C32, unsigned long long* absmax) {
   if (nunits <= 0 || nuv < 2) return hipSuccess;
+  if ((C32 || absmax) && !std::is_same<T, double>::value) return hipErrorInvalidValue;
+  if (C32 && !absmax) return hipErrorInvalidValue;
   const int d = std::max(sd.degu, sd.degv);
   if (d <= 3)
-    eval_centres_rat<T, 3>(sd, reg, nunits, nuv, C, s);
+    eval_centres_rat<T, 3>(sd, reg, nunits, nuv, C, C32, absmax, s);
   else if (d <= 7)
-    eval_centres_rat<T, 7>(sd, reg, nunits, nuv, C, s);
+    eval_centres_rat<T, 7>(sd, reg, nunits, nuv, C, C32, absmax, s);
   else
-    eval_centres_rat<T, kMaxDeg - 1>(sd, reg, nunits, nuv, C, s);
+    eval_centres_rat<T, kMaxDeg - 1>(sd, reg, nunits, nuv, C, C32, absmax, s);
   return hipGetLastError();
 }
 
@@ -1853,28 +2257,37 @@ hipError_t launch_dist_seed(int nslots, const int* su1, const int* su2, int w1, 
 
 template <class T>
 hipError_t launch_seed_thresh(const unsigned long long* best, int npos, int tpu1, int nseed, int ordered,
-                              uint64_t* list, unsigned* list_count, unsigned long long* range, unsigned* ghist,
-                              const int* su1, const int* su2, int w1, int w2, const vec4_t<T>* C1,
-                              const vec4_t<T>* C2, int k, int fma, T* thresh, hipStream_t s) {
-  if (npos > kSeedSortMax || nseed > 1024) return hipErrorInvalidValue;
+                              uint64_t* list, unsigned* list_count, unsigned long long* range,
+                              unsigned long long* work, const int* su1, const int* su2, int w1, int w2,
+                              const vec4_t<T>* C1, const vec4_t<T>* C2, int k, int fma, T* thresh, hipStream_t s) {
+  if (npos > kSeedSortMax || nseed > kSeedMaxTiles) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_seed_list<T>, dim3(1), dim3(1024), 0, s, best, npos, tpu1, nseed, ordered, list, list_count,
                      range);
-  hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(kSeedBins, 256)), dim3(256), 0, s, ghist, 0u,
-                     (unsigned long long)kSeedBins);
   const int m = nseed < npos ? nseed : npos;
+  // about 1024 workgroups: fill the chip without splitting tiles finely
+  int split = m > 0 ? 1024 / m : 1;
+  split = split < 1 ? 1 : (split > kSeedSplitMax ? kSeedSplitMax : split);
+  const int nb = m * split;
+  unsigned* gh = reinterpret_cast<unsigned*>(work + 2 * (size_t)kSeedMaxTiles * kSeedSplitMax);
   if (fma) {
-    hipLaunchKernelGGL((k_seed_hist<T, true, true>), dim3(m * kSeedSplit), dim3(256), 0, s, list, list_count, su1, su2, w1, w2, C1,
-                       C2, range, ghist);
-    hipLaunchKernelGGL((k_seed_hist<T, true, false>), dim3(m * kSeedSplit), dim3(256), 0, s, list, list_count, su1, su2, w1, w2,
-                       C1, C2, range, ghist);
+    hipLaunchKernelGGL((k_seed_hist<T, true, true>), dim3(nb), dim3(256), 0, s, list, list_count, su1, su2, w1, w2,
+                       C1, C2, split, range, work, gh);
+    hipLaunchKernelGGL(k_seed_range, dim3(1), dim3(1024), 0, s, work, nb, range, gh);
+    hipLaunchKernelGGL((k_seed_hist<T, true, false>), dim3(nb), dim3(256), 0, s, list, list_count, su1, su2, w1, w2,
+                       C1, C2, split, range, work, gh);
   } else {
-    hipLaunchKernelGGL((k_seed_hist<T, false, true>), dim3(m * kSeedSplit), dim3(256), 0, s, list, list_count, su1, su2, w1, w2,
-                       C1, C2, range, ghist);
-    hipLaunchKernelGGL((k_seed_hist<T, false, false>), dim3(m * kSeedSplit), dim3(256), 0, s, list, list_count, su1, su2, w1, w2,
-                       C1, C2, range, ghist);
+    hipLaunchKernelGGL((k_seed_hist<T, false, true>), dim3(nb), dim3(256), 0, s, list, list_count, su1, su2, w1, w2,
+                       C1, C2, split, range, work, gh);
+    hipLaunchKernelGGL(k_seed_range, dim3(1), dim3(1024), 0, s, work, nb, range, gh);
+    hipLaunchKernelGGL((k_seed_hist<T, false, false>), dim3(nb), dim3(256), 0, s, list, list_count, su1, su2, w1, w2,
+                       C1, C2, split, range, work, gh);
   }
-  hipLaunchKernelGGL(k_seed_thresh<T>, dim3(1), dim3(1024), 0, s, ghist, range, k, thresh);
+  hipLaunchKernelGGL(k_seed_thresh<T>, dim3(1), dim3(kSeedHistBins), 0, s, gh, range, k, thresh);
   return hipGetLastError();
+}
+
+size_t seed_work_bytes() {
+  return 2 * (size_t)kSeedMaxTiles * kSeedSplitMax * sizeof(unsigned long long) + kSeedHistBins * sizeof(unsigned);
 }
 
 template <class T>
@@ -1896,15 +2309,39 @@ template <class T>
 hipError_t launch_dist_filter(const uint64_t* list, const unsigned* list_count, unsigned list_cap, const int* su1,
                               const int* su2, int w1, int w2, const vec4_t<T>* C1, const vec4_t<T>* C2,
                               const T* thresh, Cand* out, unsigned long long* out_count, unsigned long long cap,
-                              int grid, hipStream_t s, unsigned long long* pairs_done, int fma, int cull) {
+                              int grid, hipStream_t s, unsigned long long* pairs_done, int fma, int cull,
+                              const unsigned long long* absmax) {
   if (list_cap == 0) return hipSuccess;
   const unsigned g = (unsigned)std::min<long long>(grid, (long long)list_cap);
-  if (fma)
-    hipLaunchKernelGGL((k_dist_filter<T, true>), dim3(g), dim3(256), 0, s, list, list_count, su1, su2, w1, w2, C1, C2,
-                       thresh, out, out_count, cap, pairs_done, cull);
-  else
-    hipLaunchKernelGGL((k_dist_filter<T, false>), dim3(g), dim3(256), 0, s, list, list_count, su1, su2, w1, w2, C1,
-                       C2, thresh, out, out_count, cap, pairs_done, cull);
+  if (w1 <= kTile && w2 <= kTile) {  // every unit is one tile: one slot per list entry
+    if (absmax) {
+      if constexpr (std::is_same<T, double>::value)
+        hipLaunchKernelGGL((k_dist_filter_units<T, false, true>), dim3(g), dim3(256), 0, s, list, list_count, su1,
+                           su2, w1, w2, C1, C2, thresh, out, out_count, cap, pairs_done, absmax);
+      else
+        return hipErrorInvalidValue;
+    } else if (fma) {
+      hipLaunchKernelGGL((k_dist_filter_units<T, true, false>), dim3(g), dim3(256), 0, s, list, list_count, su1, su2,
+                         w1, w2, C1, C2, thresh, out, out_count, cap, pairs_done, absmax);
+    } else {
+      hipLaunchKernelGGL((k_dist_filter_units<T, false, false>), dim3(g), dim3(256), 0, s, list, list_count, su1, su2,
+                         w1, w2, C1, C2, thresh, out, out_count, cap, pairs_done, absmax);
+    }
+    return hipGetLastError();
+  }
+  if (absmax) {
+    if constexpr (std::is_same<T, double>::value)
+      hipLaunchKernelGGL((k_dist_filter<T, false, true>), dim3(g), dim3(256), 0, s, list, list_count, su1, su2, w1,
+                         w2, C1, C2, thresh, out, out_count, cap, pairs_done, cull, absmax);
+    else
+      return hipErrorInvalidValue;
+  } else if (fma) {
+    hipLaunchKernelGGL((k_dist_filter<T, true, false>), dim3(g), dim3(256), 0, s, list, list_count, su1, su2, w1, w2,
+                       C1, C2, thresh, out, out_count, cap, pairs_done, cull, absmax);
+  } else {
+    hipLaunchKernelGGL((k_dist_filter<T, false, false>), dim3(g), dim3(256), 0, s, list, list_count, su1, su2, w1,
+                       w2, C1, C2, thresh, out, out_count, cap, pairs_done, cull, absmax);
+  }
   return hipGetLastError();
 }
 
@@ -1991,7 +2428,7 @@ hipError_t launch_fill_sentinel(Cand* a, unsigned long long from, unsigned long 
 hipError_t rank_sort(const Cand* in, const unsigned long long* m_dev, unsigned m_cap, unsigned keep, Cand* out,
                      hipStream_t s) {
   if (m_cap == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_rank_sort, dim3(blocks_for(m_cap, 256)), dim3(256), 0, s, in, m_dev, m_cap, keep, out);
+  hipLaunchKernelGGL(k_rank_sort, dim3(blocks_for(m_cap, 64)), dim3(64 * kRankWaves), 0, s, in, m_dev, m_cap, keep, out);
   return hipGetLastError();
 }
 
@@ -2019,12 +2456,18 @@ template <class T>
 hipError_t launch_dist_dedup(const Cand* rec, int n, int cells1, int cells2, const int* su1, const int* su2,
                              const vec4_t<T>* reg1, const vec4_t<T>* reg2, int parent_nuv1, int parent_nuv2,
                              int* firstpos1, int* firstpos2, int* nsu1, int* nsu2, vec4_t<T>* nreg1,
-                             vec4_t<T>* nreg2, int* nunits, hipStream_t s) {
+                             vec4_t<T>* nreg2, int* nunits, unsigned char* work, hipStream_t s) {
   if (n <= 0 || n > kDedupMax) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_dist_dedup<T>, dim3(1), dim3(1024), 0, s, rec, n, cells1, cells2, su1, su2, reg1, reg2,
-                     parent_nuv1, parent_nuv2, firstpos1, firstpos2, nsu1, nsu2, nreg1, nreg2, nunits);
+  hipLaunchKernelGGL(k_dedup_mark, dim3(blocks_for(n, 256)), dim3(256), 0, s, rec, n, cells1, cells2, firstpos1,
+                     firstpos2);
+  hipLaunchKernelGGL(k_dedup_units, dim3(1), dim3(1024), 0, s, rec, n, cells1, cells2, firstpos1, firstpos2, nsu1,
+                     nsu2, work, nunits);
+  hipLaunchKernelGGL(k_dedup_apply<T>, dim3(blocks_for(n, 256)), dim3(256), 0, s, rec, n, cells1, cells2, su1, su2,
+                     reg1, reg2, parent_nuv1, parent_nuv2, work, nsu1, nsu2, nreg1, nreg2, firstpos1, firstpos2);
   return hipGetLastError();
 }
+
+size_t dedup_work_bytes() { return 2 * (size_t)kDedupMax; }
 
 template <class T>
 hipError_t launch_isect_list(int nslots, const int* su1, const int* su2, int w1, int w2, const vec4_t<T>* lo1,
@@ -2152,16 +2595,16 @@ hipError_t launch_fill_u32(unsigned* p, unsigned value, unsigned long long n, hi
   template hipError_t launch_dist_filter<T>(const uint64_t*, const unsigned*, unsigned, const int*, const int*,    \
                                             int, int, const vec4_t<T>*, const vec4_t<T>*, const T*, Cand*,         \
                                             unsigned long long*, unsigned long long, int, hipStream_t,             \
-                                            unsigned long long*, int, int);            \
+                                            unsigned long long*, int, int, const unsigned long long*);             \
   template hipError_t launch_set_thresh<T>(T*, const Cand*, int, int, hipStream_t);                                \
   template hipError_t launch_eval_centres<T>(const SurfDev<T>&, const vec4_t<T>*, int, int, vec4_t<T>*,            \
-                                             hipStream_t);                                                         \
+                                             hipStream_t, float4*, unsigned long long*);                           \
   template hipError_t launch_seed_thresh<T>(const unsigned long long*, int, int, int, int, uint64_t*, unsigned*,   \
-                                            unsigned long long*, unsigned*, const int*, const int*, int, int,      \
-                                            const vec4_t<T>*, const vec4_t<T>*, int, int, T*, hipStream_t);        \
+                                            unsigned long long*, unsigned long long*, const int*, const int*, int, \
+                                            int, const vec4_t<T>*, const vec4_t<T>*, int, int, T*, hipStream_t);   \
   template hipError_t launch_dist_dedup<T>(const Cand*, int, int, int, const int*, const int*, const vec4_t<T>*,   \
                                            const vec4_t<T>*, int, int, int*, int*, int*, int*, vec4_t<T>*,         \
-                                           vec4_t<T>*, int*, hipStream_t);                                         \
+                                           vec4_t<T>*, int*, unsigned char*, hipStream_t);                         \
   template hipError_t launch_isect_list<T>(int, const int*, const int*, int, int, const vec4_t<T>*,                \
                                            const vec4_t<T>*, const vec4_t<T>*, const vec4_t<T>*, int, uint64_t*,   \
                                            unsigned long long*, unsigned long long, hipStream_t);                  \

@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -190,10 +191,11 @@ int jsi_ctx::eval_units(int h, const vec4_t<T>* reg, int nunits, int nuv, int mo
 
 // Distance cell centres of `nunits` regions, evaluation fused with the AABB.
 template <class T>
-int jsi_ctx::eval_centres(int h, const vec4_t<T>* reg, int nunits, int nuv, vec4_t<T>* C) {
+int jsi_ctx::eval_centres(int h, const vec4_t<T>* reg, int nunits, int nuv, vec4_t<T>* C, float4* C32,
+                          unsigned long long* absmax) {
   SurfDev<T> sd;
   RCK(surf_dev<T>(h, &sd));
-  HCK(launch_eval_centres<T>(sd, reg, nunits, nuv, C, stream));
+  HCK(launch_eval_centres<T>(sd, reg, nunits, nuv, C, stream, C32, absmax));
   return JSI_OK;
 }
 
@@ -203,7 +205,7 @@ int jsi_ctx::eval_centres(int h, const vec4_t<T>* reg, int nunits, int nuv, vec4
 template <class T>
 int jsi_ctx::dist_topk(int nslots, const int* su1, const int* su2, int w1, int nunits1, const vec4_t<T>* C1, int w2,
                         int nunits2, const vec4_t<T>* C2, int k, int cull, Cand** rec_out, long long* n_out,
-                        long long* kk_out, Rerank rr) {
+                        long long* kk_out, Rerank rr, const unsigned long long* exp_absmax) {
   using V = vec4_t<T>;
   const bool rerank = rr.C1 != nullptr;
   const int fma = rerank ? 1 : 0;  // the FP32SCREEN screen may fuse; its margin covers the rounding
@@ -250,19 +252,19 @@ int jsi_ctx::dist_topk(int nslots, const int* su1, const int* su2, int w1, int n
   // global k-th d2 from above
   const long long npos = (long long)nslots * tpu1;
   const int pairs_tp = std::min(w1 * w1, kTileCells) * std::min(w2 * w2, kTileCells);
-  // about 32 k seed pairs (at least 4 tile pairs, at most 64): enough for a
-  // threshold near the true k-th without scanning millions of seed pairs
+  // round 1 (one slot of many tiles): up to 256 tile pairs, so the seeds
+  // cover most regions that hold top-k pairs; refinement rounds (slots in
+  // parent-rank order): about 32 k seed pairs from the first slots
   const unsigned long long want = (32ull * (unsigned long long)k + pairs_tp - 1) / (unsigned long long)pairs_tp;
-  const int nseed = (int)std::max(4ull, std::min(64ull, want));
+  const int nseed = nslots == 1 ? std::min(256, tpu1) : (int)std::max(4ull, std::min(64ull, want));
   if (npos <= 8192) {
-    unsigned long long *bestv, *range;
-    unsigned* ghist;
+    unsigned long long *bestv, *range, *work;
     RCK(buf("d.best", (size_t)npos, &bestv));
     RCK(buf("d.range", 2, &range));
-    RCK(buf("d.ghist", 4096, &ghist));
+    RCK(buf_raw("d.seedwork", seed_work_bytes(), reinterpret_cast<void**>(&work), 0));
     const int ordered = (nslots > 1 && tpu1 == 1 && tpu2 == 1) ? 1 : 0;
     HCK(launch_dist_seed<T>(nslots, su1, su2, w1, w2, lo1, hi1, lo2, hi2, k, bestv, stream));
-    HCK(launch_seed_thresh<T>(bestv, (int)npos, tpu1, nseed, ordered, list, list_count, range, ghist, su1, su2, w1,
+    HCK(launch_seed_thresh<T>(bestv, (int)npos, tpu1, nseed, ordered, list, list_count, range, work, su1, su2, w1,
                               w2, C1, C2, k, fma, thresh, stream));
     RCK(widen());
   } else {
@@ -281,7 +283,7 @@ int jsi_ctx::dist_topk(int nslots, const int* su1, const int* su2, int w1, int n
       HCK(hipEventRecord(ev[0], stream));
     }
     HCK(launch_dist_filter<T>(list, list_count, (unsigned)ntp, su1, su2, w1, w2, C1, C2, thresh, cand, out_count,
-                              kCandCap, grid, stream, pairs_done, fma, cull));
+                              kCandCap, grid, stream, pairs_done, fma, cull, exp_absmax));
     if (prof) HCK(hipEventRecord(ev[1], stream));
     RCK(d2h(&n, out_count, sizeof(n)));
     if (prof) {
@@ -293,6 +295,15 @@ int jsi_ctx::dist_topk(int nslots, const int* su1, const int* su2, int w1, int n
       kst.launches += 1;
       kst.ms += ms;
       kst.units += (double)pd;
+    }
+    if (debug_) {
+      unsigned lc = 0;
+      T tv = T(0);
+      RCK(d2h(&lc, list_count, sizeof(lc)));
+      RCK(d2h(&tv, thresh, sizeof(tv)));
+      fprintf(stderr, "[jsi] topk %s slots=%d w=%d/%d tilepairs=%llu listed=%u thresh=%.9g candidates=%llu%s\n",
+              std::is_same<T, float>::value ? (rerank ? "f32s" : "f32") : "f64", nslots, w1, w2, ntp, lc,
+              (double)tv, n, n > kCandCap ? " (overflow)" : "");
     }
     if (n <= kCandCap) break;
     if (attempt > 8) {
@@ -388,12 +399,27 @@ int jsi_ctx::min_distance(int h1, const jsi_region& r1, int h2, const jsi_region
   T* pts;
   V *C1, *C2;
   double te = now_ms();
+  // double: the evaluation max-reduces the round's largest |centre
+  // coordinate| (rounding margins of the screens) into amax[0]; for FP32
+  // screening of round 1 it also writes the centres rounded to float
+  float4 *f32c1 = nullptr, *f32c2 = nullptr;
+  unsigned long long* amax = nullptr;
+  const bool dbl = std::is_same<T, double>::value;
+  const bool s32_first = dbl && P.precision == JSI_PREC_FP32SCREEN;
   {
     RCK(buf("pts", (size_t)3 * std::max(nuv1 * nuv1, nuv2 * nuv2), &pts));
     RCK(buf(nm("C", 1, cur).c_str(), (size_t)(nuv1 - 1) * (nuv1 - 1), &C1));
     RCK(buf(nm("C", 2, cur).c_str(), (size_t)(nuv2 - 1) * (nuv2 - 1), &C2));
-    RCK(eval_centres<T>(h1, reg1, 1, nuv1, C1));
-    RCK(eval_centres<T>(h2, reg2, 1, nuv2, C2));
+    if (dbl) {
+      RCK(buf("d.amax", kAmaxSlots, &amax));
+      HCK(launch_fill_u32(reinterpret_cast<unsigned*>(amax), 0u, 2 * kAmaxSlots, stream));
+    }
+    if (s32_first) {
+      RCK(buf("s32.C1", (size_t)(nuv1 - 1) * (nuv1 - 1), &f32c1));
+      RCK(buf("s32.C2", (size_t)(nuv2 - 1) * (nuv2 - 1), &f32c2));
+    }
+    RCK(eval_centres<T>(h1, reg1, 1, nuv1, C1, f32c1, amax));
+    RCK(eval_centres<T>(h2, reg2, 1, nuv2, C2, f32c2, amax));
   }
   int* su1;
   int* su2;
@@ -418,28 +444,20 @@ int jsi_ctx::min_distance(int h1, const jsi_region& r1, int h2, const jsi_region
       // (and a round whose float candidate set cannot be bounded) run in FP64.
       int s32 = (P.precision == JSI_PREC_FP32SCREEN && round == 1) ? 1 : 0;
       if (s32) {
-        float4 *f1, *f2;
-        unsigned long long* amax;
-        const long long n1 = (long long)nunits1 * w1 * w1, n2 = (long long)nunits2 * w2 * w2;
-        RCK(buf("s32.C1", (size_t)n1, &f1));
-        RCK(buf("s32.C2", (size_t)n2, &f2));
-        RCK(buf("s32.amax", 2, &amax));
-        HCK(launch_fill_u32(reinterpret_cast<unsigned*>(amax), 0u, 2, stream));
-        HCK(launch_narrow(C1, f1, n1, amax, stream));
-        HCK(launch_narrow(C2, f2, n2, amax, stream));
         Rerank rr;
         rr.C1 = C1;
         rr.C2 = C2;
         rr.absmax = amax;
-        const int rc = dist_topk<float>(nslots, su1, su2, w1, nunits1, f1, w2, nunits2, f2, P.k, P.cull, &rec, &n,
-                                        &kk, rr);
+        const int rc = dist_topk<float>(nslots, su1, su2, w1, nunits1, f32c1, w2, nunits2, f32c2, P.k, P.cull, &rec,
+                                        &n, &kk, rr);
         if (rc == JSI_ERR_CAPACITY)
           s32 = 0;
         else if (rc != JSI_OK)
           return rc;
       }
       if (!s32) {
-        RCK(dist_topk<T>(nslots, su1, su2, w1, nunits1, C1, w2, nunits2, C2, P.k, P.cull, &rec, &n, &kk));
+        RCK(dist_topk<T>(nslots, su1, su2, w1, nunits1, C1, w2, nunits2, C2, P.k, P.cull, &rec, &n, &kk, Rerank(),
+                         amax));
       }
     } else {
       RCK(dist_topk<T>(nslots, su1, su2, w1, nunits1, C1, w2, nunits2, C2, P.k, P.cull, &rec, &n, &kk));
@@ -487,8 +505,10 @@ int jsi_ctx::min_distance(int h1, const jsi_region& r1, int h2, const jsi_region
     RCK(buf(nm("reg", 1, nxt).c_str(), (size_t)kk, &nreg1));
     RCK(buf(nm("reg", 2, nxt).c_str(), (size_t)kk, &nreg2));
     RCK(buf("nunits", 4, &nunits_dev));
+    unsigned char* dwork;
+    RCK(buf_raw("d.dedupwork", dedup_work_bytes(), reinterpret_cast<void**>(&dwork), 0));
     HCK(launch_dist_dedup<T>(rec, (int)kk, w1 * w1, w2 * w2, su1, su2, reg1, reg2, nuv1, nuv2, fp1, fp2, nsu1, nsu2,
-                             nreg1, nreg2, nunits_dev, stream));
+                             nreg1, nreg2, nunits_dev, dwork, stream));
     int nu[2];
     RCK(d2h(nu, nunits_dev, sizeof(nu)));
     ri.units1 = nu[0];
@@ -498,8 +518,9 @@ int jsi_ctx::min_distance(int h1, const jsi_region& r1, int h2, const jsi_region
     RCK(buf("pts", (size_t)3 * std::max(nu[0], nu[1]) * nref * nref, &pts));
     RCK(buf(nm("C", 1, nxt).c_str(), (size_t)nu[0] * wr * wr, &nC1));
     RCK(buf(nm("C", 2, nxt).c_str(), (size_t)nu[1] * wr * wr, &nC2));
-    RCK(eval_centres<T>(h1, nreg1, nu[0], nref, nC1));
-    RCK(eval_centres<T>(h2, nreg2, nu[1], nref, nC2));
+    if (dbl) HCK(launch_fill_u32(reinterpret_cast<unsigned*>(amax), 0u, 2 * kAmaxSlots, stream));
+    RCK(eval_centres<T>(h1, nreg1, nu[0], nref, nC1, nullptr, amax));
+    RCK(eval_centres<T>(h2, nreg2, nu[1], nref, nC2, nullptr, amax));
     HCK(hipStreamSynchronize(stream));
     res->ms_eval += now_ms() - te;
     cur = nxt;
@@ -855,6 +876,7 @@ int jsi_ctx_create(int device, size_t hbm_cap_bytes, jsi_ctx** out) {
   HCK(hipSetDevice(device));
   jsi_ctx* c = new jsi_ctx();
   c->device = device;
+  c->debug_ = std::getenv("JSI_DEBUG") != nullptr;
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     delete c;
@@ -878,6 +900,7 @@ int jsi_ctx_create_worker(jsi_ctx* parent, jsi_ctx** out) {
   HCK(hipSetDevice(parent->device));
   jsi_ctx* c = new jsi_ctx();
   c->device = parent->device;
+  c->debug_ = parent->debug_;
   c->worker = true;
   c->surfaces = parent->surfaces;
   c->cap = parent->cap;
@@ -1051,4 +1074,4 @@ int jsi_intersect(jsi_ctx* c, int h1, const jsi_region* r1, int h2, const jsi_re
 
 // used by the operator ABI (jsi_opabi.hip)
 template int jsi_ctx::dist_topk<float>(int, const int*, const int*, int, int, const float4*, int, int, const float4*,
-                                       int, int, Cand**, long long*, long long*, Rerank);
+                                       int, int, Cand**, long long*, long long*, Rerank, const unsigned long long*);

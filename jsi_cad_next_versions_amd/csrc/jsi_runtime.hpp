@@ -64,6 +64,7 @@ struct jsi_ctx {
   bool worker = false;  // shares its parent's surfaces; destroy frees only its own buffers
   std::map<std::string, DevBuf> bufs;
   void* thresh_ptr_ = nullptr;  // device threshold of the screen in flight
+  bool debug_ = false;          // JSI_DEBUG: per-screen counts on stderr
 
   int buf_raw(const char* name, size_t bytes, void** out, int fill_int);
   template <class X>
@@ -83,11 +84,12 @@ struct jsi_ctx {
                  unsigned long long* kmax, const char* tag);
   using Rerank = RerankSpec;
   template <class T>
-  int eval_centres(int h, const jsi::vec4_t<T>* reg, int nunits, int nuv, jsi::vec4_t<T>* C);
+  int eval_centres(int h, const jsi::vec4_t<T>* reg, int nunits, int nuv, jsi::vec4_t<T>* C, float4* C32 = nullptr,
+                   unsigned long long* absmax = nullptr);
   template <class T>
   int dist_topk(int nslots, const int* su1, const int* su2, int w1, int nunits1, const jsi::vec4_t<T>* C1, int w2,
                 int nunits2, const jsi::vec4_t<T>* C2, int k, int cull, jsi::Cand** rec_out, long long* n_out,
-                long long* kk_out, Rerank rr = Rerank());
+                long long* kk_out, Rerank rr = Rerank(), const unsigned long long* exp_absmax = nullptr);
   template <class T>
   int min_distance(int h1, const jsi_region& r1, int h2, const jsi_region& r2, const jsi_params& P,
                    jsi_result* res, jsi_pair* out, int cap_pairs);

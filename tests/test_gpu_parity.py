@@ -99,6 +99,34 @@ def test_culling_leaves_results_unchanged(ctx, orc, cull, prec):
     np.testing.assert_array_equal(pairs, opairs)
 
 
+def dense_pair(nu, nv, deg=3):
+    """Hard-wave shaped surfaces on a nu x nv control net: long knot vectors
+    (read from HBM by the evaluation when over 128 knots) and tiles that cover
+    many knot spans (control rows processed in several chunks)."""
+    ku, kv = S.clamped_uniform_knots(nu, deg), S.clamped_uniform_knots(nv, deg)
+    X, Y = np.meshgrid(np.arange(nu) / (nu - 1), np.arange(nv) / (nv - 1))
+    za = 0.05 * np.sin(16 * np.pi * X) * np.sin(16 * np.pi * Y)
+    zb = 0.05 * np.sin(16 * np.pi * X + np.pi / 7) * np.sin(16 * np.pi * Y) + 0.06
+    wa = S.Rng(11).uniform(nu * nv, 0.8, 1.2).reshape(nv, nu)
+    wb = S.Rng(12).uniform(nu * nv, 0.8, 1.2).reshape(nv, nu)
+    a = S.make_surface(deg, deg, ku, kv, np.stack([X, Y, za], -1), wa, f"dense{nu}x{nv}_A")
+    b = S.make_surface(deg, deg, ku, kv, np.stack([X, Y, zb], -1), wb, f"dense{nu}x{nv}_B")
+    return a, b
+
+
+@pytest.mark.parametrize("prec", ["exact32", "fp64"])
+@pytest.mark.parametrize("net", [(200, 200), (150, 20), (20, 150)])
+def test_min_distance_dense_nets_match_oracle(ctx, orc, prec, net):
+    a, b = dense_pair(*net)
+    r1, r2 = regions(a, b, 64, 48)
+    p = default_params(k=64, precision=prec, trace=1)
+    res, pairs = ctx.minimum_distance(a, r1, b, r2, p)
+    ores, opairs = orc.min_distance(a, r1, b, r2, p)
+    same_rounds(res, ores)
+    assert res.distance == ores.distance
+    np.testing.assert_array_equal(pairs, opairs)
+
+
 def test_min_distance_more_k_than_pairs(ctx, orc):
     """k larger than the number of cell pairs: every pair is kept, sorted."""
     a, b = S.wave_pair_small()
