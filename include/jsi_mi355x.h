@@ -68,7 +68,10 @@ typedef struct jsi_params {
   int precision;         /* JSI_PREC_* */
   int cull;              /* conservative tile culling inside the screens (results unchanged) */
   int trace;             /* fill jsi_round_info.hash (costs a device->host copy per round) */
-  int reserved[5];
+  int cand_cap;          /* distance screen candidate capacity, 0 = default (4 Mi records), at least
+                            2k is used; a smaller value exercises the overflow path (threshold
+                            tightening and a repeated screen), results unchanged */
+  int reserved[4];
 } jsi_params;
 
 typedef struct jsi_round_info {

@@ -48,7 +48,8 @@ class Params(C.Structure):
         ("precision", C.c_int),
         ("cull", C.c_int),
         ("trace", C.c_int),
-        ("reserved", C.c_int * 5),
+        ("cand_cap", C.c_int),
+        ("reserved", C.c_int * 4),
     ]
 
 
@@ -104,6 +105,7 @@ def default_params(**overrides):
     p.precision = PREC_EXACT32
     p.cull = 1
     p.trace = 0
+    p.cand_cap = 0
     for key, val in overrides.items():
         if key == "precision" and isinstance(val, str):
             val = PRECISIONS[val]

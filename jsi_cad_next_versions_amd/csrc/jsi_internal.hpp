@@ -81,8 +81,10 @@ hipError_t launch_points(const SurfDev<T>& sd, const T* tabU, const int* spanU, 
 // absmax max-reduces the largest |coordinate| (double bits); C32 (requires
 // absmax) also writes the centres rounded to float.
 template <class T>
+// nunits_dev (optional): the true unit count on the device, <= nunits.
 hipError_t launch_eval_centres(const SurfDev<T>& sd, const vec4_t<T>* reg, int nunits, int nuv, vec4_t<T>* C,
-                               hipStream_t s, float4* C32 = nullptr, unsigned long long* absmax = nullptr);
+                               hipStream_t s, float4* C32 = nullptr, unsigned long long* absmax = nullptr,
+                               const int* nunits_dev = nullptr);
 
 // Cell AABBs of every unit from its SoA points.  isect = 0: distance centres
 // (construct_aabbs_distance_host); isect = 1: centre + padded radius
@@ -133,8 +135,8 @@ hipError_t launch_set_thresh(T* thresh, const Cand* rec, int idx, int use_inf, h
 hipError_t launch_narrow(const double4* src, float4* dst, long long n, unsigned long long* absmax_bits,
                          hipStream_t s);
 hipError_t launch_inflate(float* thresh, const unsigned long long* absmax_bits, hipStream_t s);
-hipError_t launch_rerank(Cand* cand, unsigned long long n, const int* su1, const int* su2, int cells1, int cells2,
-                         const double4* C1, const double4* C2, hipStream_t s);
+hipError_t launch_rerank(Cand* cand, const unsigned long long* n_dev, unsigned long long cap, const int* su1,
+                         const int* su2, int cells1, int cells2, const double4* C1, const double4* C2, hipStream_t s);
 
 // Operator ABI: flat AoS arrays -> padded w x w float4 grids; endpoint masks.
 hipError_t launch_pack_cells(const float* c, const float* r, long long n, int w, float4* C, float4* R,
@@ -150,9 +152,19 @@ struct SelState {
   int done;
   int pad_;
 };
+// Runs `iters` narrowing steps (resume: continue a previous selection on the
+// same records instead of starting one), then compacts the records with key
+// <= the current bound into out when out is given.  Converged when the
+// compacted count is <= small_cap or the state's done flag is set.
+// n_dev (optional): the record count on the device; n is then its upper bound.
 hipError_t select_topk(const Cand* cand, unsigned long long n, unsigned long long k, unsigned long long maxkey,
                        SelState* st, unsigned* hist, Cand* out, unsigned long long* out_count, unsigned small_cap,
-                       hipStream_t s);
+                       hipStream_t s, int iters = 2, bool resume = false, const unsigned long long* n_dev = nullptr);
+hipError_t launch_thresh_from_state_ru(float* thresh, const SelState* st, hipStream_t s);
+hipError_t launch_set_thresh_ru(float* thresh, const Cand* rec, int idx, hipStream_t s);
+hipError_t launch_copy_cand(Cand* dst, const Cand* src, hipStream_t s);
+template <class T>
+hipError_t launch_store_vec4(vec4_t<T>* dst, vec4_t<T> v, hipStream_t s);
 template <class T>
 hipError_t launch_thresh_from_state(T* thresh, const SelState* st, hipStream_t s);
 

@@ -376,8 +376,9 @@ struct EcRows {  // control rows per chunk of u-contractions
 };
 
 template <class T, int MAXD, bool RAT, bool KL>
-__global__ __launch_bounds__(64) void k_eval_centres(SurfDev<T> sd, const vec4_t<T>* regions, int nunits, int nuv,
-                                                     vec4_t<T>* C, float4* C32, unsigned long long* absmax) {
+__global__ __launch_bounds__(64) void k_eval_centres(SurfDev<T> sd, const vec4_t<T>* regions, int nunits,
+                                                     const int* nunits_dev, int nuv, vec4_t<T>* C, float4* C32,
+                                                     unsigned long long* absmax) {
   constexpr int E = kTile + 1;  // corner points per tile edge
   constexpr int kRows = EcRows<MAXD>::value;
   constexpr int kCorners = (E * E + 63) / 64;  // corners per lane
@@ -387,18 +388,22 @@ __global__ __launch_bounds__(64) void k_eval_centres(SurfDev<T> sd, const vec4_t
   // different corner columns, fall on different banks
   constexpr int kNb = MAXD + 2;           // basis row stride
   constexpr int kRs = 4 * kRows + 1;      // u-contraction column stride
+  // one buffer, three lives: knots (phases 0-1), u-contractions (phases
+  // 2-3), corner points x[], y[], z[] (phases 4-5); barriers separate them
+  constexpr int kRsz = E * kRs;
+  constexpr int kA = kK > kPts ? (kK > kRsz ? kK : kRsz) : (kPts > kRsz ? kPts : kRsz);
   __shared__ T sNu[E][kNb], sNv[E][kNb];
   __shared__ int sSu[E], sSv[E];
-  __shared__ T sR[E][kRs];                 // [column][4 * row + coordinate]
-  __shared__ T sA[kK > kPts ? kK : kPts];  // knots (phases 0-1), then corner points x[], y[], z[]
+  __shared__ T sA[kA];
   T* const sKu = sA;
   T* const sKv = sA + kEcKnots;
+  T(*const sR)[kRs] = reinterpret_cast<T(*)[kRs]>(sA);  // [column][4 * row + coordinate]
   T* const sP = sA;
   const int lane = threadIdx.x;
   const int w = nuv - 1;
   const int tw = (w + kTile - 1) / kTile, tpu = tw * tw;
   const int unit = blockIdx.x / tpu, t = blockIdx.x % tpu;
-  if (unit >= nunits) return;  // whole wavefront
+  if (unit >= nunits || (nunits_dev && unit >= *nunits_dev)) return;  // whole wavefront
   const int row0 = (t / tw) * kTile, col0 = (t % tw) * kTile;
   const int ht = min(kTile, w - row0), wd = min(kTile, w - col0);  // cells
   const int p = sd.degu, q = sd.degv;
@@ -496,7 +501,7 @@ __global__ __launch_bounds__(64) void k_eval_centres(SurfDev<T> sd, const vec4_t
         if (RAT) acc[c][3] += nvs * sR[ex][4 * rr + 3];
       }
     }
-    __syncthreads();  // sR is rewritten by the next chunk
+    __syncthreads();  // sR is rewritten by the next chunk (or becomes sP)
   }
   // phase 4: corner points into sP (reuses the knot storage)
 #pragma unroll
@@ -1409,11 +1414,21 @@ __global__ void k_inflate(float* thresh, const unsigned long long* absmax_bits) 
   *thresh = __double2float_ru(r * r * g + 1e-36);
 }
 
-// Candidate records get their FP64 d2 (same recipe as the FP64 screen).
-__global__ void k_rerank(Cand* cand, unsigned long long n, const int* su1, const int* su2, int cells1, int cells2,
-                         const double4* C1, const double4* C2) {
-  const unsigned long long e = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n) return;
+__device__ __forceinline__ void rerank_one(Cand* cand, unsigned long long e, const int* su1, const int* su2,
+                                           int cells1, int cells2, const double4* C1, const double4* C2);
+
+// Candidate records get their FP64 d2 (same recipe as the FP64 screen).  The
+// record count is min(*n_dev, cap).
+__global__ void k_rerank(Cand* cand, const unsigned long long* n_dev, unsigned long long cap, const int* su1,
+                         const int* su2, int cells1, int cells2, const double4* C1, const double4* C2) {
+  const unsigned long long n = min(*n_dev, cap);
+  for (unsigned long long e = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (unsigned long long)gridDim.x * blockDim.x)
+    rerank_one(cand, e, su1, su2, cells1, cells2, C1, C2);
+}
+
+__device__ __forceinline__ void rerank_one(Cand* cand, unsigned long long e, const int* su1, const int* su2,
+                                           int cells1, int cells2, const double4* C1, const double4* C2) {
   Cand c = cand[e];
   const int slot = (int)(c.lo >> 40), i = (int)((c.lo >> 20) & 0xfffff), j = (int)(c.lo & 0xfffff);
   const double4 a = C1[(size_t)su1[slot] * cells1 + i];
@@ -1519,21 +1534,32 @@ __global__ void k_set_thresh(T* thresh, const Cand* rec, int idx, int use_inf) {
 }
 
 // ---------------------------------------------------------------------------
-// Top-k selection without a full sort: histogram radix select on the d2 key
-// narrows [lo, hi] to the bin holding rank k until at most kSmallCap records
-// have key <= hi (or the bin is a single key); those records are compacted and
-// sorted.  hi is then also an upper bound of the k-th key with at least k
+// Top-k selection without a full sort: histogram radix select on the d2 key.
+// The key range starts at the smallest and largest key present (one reduction
+// pass); each iteration histograms [lo, hi] into kSelBins linear bins and
+// narrows it to the bin holding rank k, until at most small_cap records have
+// key <= hi (or the bin is a single key).  Those records are compacted and
+// sorted.  hi is always an upper bound of the k-th key with at least k
 // records at or below it, which is what a screening threshold needs.
 // ---------------------------------------------------------------------------
 constexpr int kSelBins = 4096;
 
-__global__ void k_sel_init(SelState* st, unsigned long long n, unsigned long long k, unsigned long long maxkey,
-                           unsigned small_cap) {
+__device__ __forceinline__ unsigned long long sel_width(unsigned long long lo, unsigned long long hi) {
+  return (hi > lo ? hi - lo : 0ull) / kSelBins + 1;
+}
+
+__device__ __forceinline__ unsigned long long sel_n(unsigned long long n, const unsigned long long* n_dev) {
+  return n_dev ? min(*n_dev, n) : n;
+}
+
+__global__ void k_sel_init(SelState* st, unsigned long long n, const unsigned long long* n_dev, unsigned long long k,
+                           unsigned long long maxkey, unsigned small_cap) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  n = sel_n(n, n_dev);
   SelState s;
-  s.lo = 0;
-  s.hi = maxkey;
-  s.width = maxkey / kSelBins + 1;
+  s.lo = ~0ull;  // min / max key, reduced by k_sel_minmax
+  s.hi = 0;
+  s.width = 0;
   s.need = k < n ? k : n;
   s.below = 0;
   s.final_hi = maxkey;
@@ -1541,11 +1567,46 @@ __global__ void k_sel_init(SelState* st, unsigned long long n, unsigned long lon
   *st = s;
 }
 
-__global__ __launch_bounds__(256) void k_sel_hist(const Cand* cand, unsigned long long n, const SelState* st,
+__global__ __launch_bounds__(256) void k_sel_minmax(const Cand* cand, unsigned long long n,
+                                                    const unsigned long long* n_dev, SelState* st) {
+  __shared__ unsigned long long rmin[4], rmax[4];
+  if (st->done) return;
+  n = sel_n(n, n_dev);
+  unsigned long long a = ~0ull, b = 0ull;
+  for (unsigned long long e = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (unsigned long long)gridDim.x * blockDim.x) {
+    const unsigned long long key = cand[e].hi;
+    a = key < a ? key : a;
+    b = key > b ? key : b;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long x = __shfl_xor(a, off), y = __shfl_xor(b, off);
+    a = x < a ? x : a;
+    b = y > b ? y : b;
+  }
+  if (lane_id() == 0) {
+    rmin[threadIdx.x >> 6] = a;
+    rmax[threadIdx.x >> 6] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < 4; ++w) {
+      a = rmin[w] < a ? rmin[w] : a;
+      b = rmax[w] > b ? rmax[w] : b;
+    }
+    atomicMin(&st->lo, a);
+    atomicMax(&st->hi, b);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_sel_hist(const Cand* cand, unsigned long long n,
+                                                  const unsigned long long* n_dev, const SelState* st,
                                                   unsigned* hist) {
   __shared__ unsigned h[kSelBins];
   if (st->done) return;
-  const unsigned long long lo = st->lo, hi = st->hi, width = st->width;
+  n = sel_n(n, n_dev);
+  const unsigned long long lo = st->lo, hi = st->hi, width = sel_width(lo, hi);
   for (int b = threadIdx.x; b < kSelBins; b += blockDim.x) h[b] = 0;
   __syncthreads();
   for (unsigned long long e = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; e < n;
@@ -1564,6 +1625,7 @@ __global__ __launch_bounds__(1024) void k_sel_step(SelState* st, unsigned* hist,
   __shared__ unsigned long long found_before;
   SelState s = *st;
   if (s.done) return;
+  const unsigned long long width = sel_width(s.lo, s.hi);
   constexpr int per = kSelBins / 1024;
   unsigned long long loc = 0;
   for (int q = 0; q < per; ++q) loc += hist[threadIdx.x * per + q];
@@ -1590,49 +1652,68 @@ __global__ __launch_bounds__(1024) void k_sel_step(SelState* st, unsigned* hist,
   if (threadIdx.x == 0) {
     const int b = found < kSelBins ? found : kSelBins - 1;
     const unsigned long long before = found < kSelBins ? found_before : 0ull;
-    const unsigned long long nlo = s.lo + (unsigned long long)b * s.width;
-    const unsigned long long span_hi = nlo + s.width - 1;
+    const unsigned long long nlo = s.lo + (unsigned long long)b * width;
+    const unsigned long long span_hi = nlo + width - 1;
     const unsigned long long nhi = span_hi < s.hi ? span_hi : s.hi;
     s.below += before;
     s.need -= before;
     s.lo = nlo;
     s.hi = nhi;
     s.final_hi = nhi;
-    if (s.below + hist[b] <= small_cap || nlo == nhi) {
-      s.done = 1;
-    } else {
-      s.width = (nhi - nlo) / kSelBins + 1;
-    }
+    if (s.below + hist[b] <= small_cap || nlo == nhi) s.done = 1;
     *st = s;
   }
   __syncthreads();
   for (int b = threadIdx.x; b < kSelBins; b += 1024) hist[b] = 0;
 }
 
-__global__ void k_sel_compact(const Cand* cand, unsigned long long n, const SelState* st, Cand* out,
-                              unsigned long long* out_count, unsigned long long cap) {
+__global__ __launch_bounds__(256) void k_sel_compact(const Cand* cand, unsigned long long n,
+                                                     const unsigned long long* n_dev, const SelState* st, Cand* out,
+                                                     unsigned long long* out_count, unsigned long long cap) {
+  n = sel_n(n, n_dev);
   const unsigned long long fh = st->final_hi;
-  const unsigned long long e = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
-  bool keep = false;
-  Cand c;
-  if (e < n) {
-    c = cand[e];
-    keep = c.hi <= fh;
-  }
-  const unsigned long long m = __ballot(keep);
-  if (m) {
-    unsigned long long base = 0;
-    const int leader = __ffsll((long long)m) - 1;
-    if (lane_id() == leader) base = atomicAdd(out_count, (unsigned long long)__popcll(m));
-    base = __shfl(base, leader);
-    const unsigned long long pos = base + __popcll(m & ((1ull << lane_id()) - 1));
-    if (keep && pos < cap) out[pos] = c;
+  for (unsigned long long e0 = (unsigned long long)blockIdx.x * blockDim.x; e0 < n;
+       e0 += (unsigned long long)gridDim.x * blockDim.x) {
+    const unsigned long long e = e0 + threadIdx.x;
+    bool keep = false;
+    Cand c;
+    if (e < n) {
+      c = cand[e];
+      keep = c.hi <= fh;
+    }
+    const unsigned long long m = __ballot(keep);
+    if (m) {
+      unsigned long long base = 0;
+      const int leader = __ffsll((long long)m) - 1;
+      if (lane_id() == leader) base = atomicAdd(out_count, (unsigned long long)__popcll(m));
+      base = __shfl(base, leader);
+      const unsigned long long pos = base + __popcll(m & ((1ull << lane_id()) - 1));
+      if (keep && pos < cap) out[pos] = c;
+    }
   }
 }
 
 template <class T>
 __global__ void k_thresh_from_state(T* thresh, const SelState* st) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *thresh = from_bits<T>(st->final_hi);
+}
+
+// Float screening threshold from a bound on FP64 keys: rounded up, so every
+// pair whose FP64 d2 is <= the bound stays below it before the widening.
+__global__ void k_thresh_from_state_ru(float* thresh, const SelState* st) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *thresh = __double2float_ru(from_bits<double>(st->final_hi));
+}
+__global__ void k_set_thresh_ru(float* thresh, const Cand* rec, int idx) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *thresh = __double2float_ru(from_bits<double>(rec[idx].hi));
+}
+
+// Small device-side stores (no host round trip): one record, one region.
+__global__ void k_copy_cand(Cand* dst, const Cand* src) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *dst = *src;
+}
+template <class T>
+__global__ void k_store_vec4(vec4_t<T>* dst, vec4_t<T> v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *dst = v;
 }
 
 // ---------------------------------------------------------------------------
@@ -2172,37 +2253,41 @@ hipError_t launch_points(const SurfDev<T>& sd, const T* tabU, const int* spanU, 
 }
 
 template <class T, int MAXD>
-static void eval_centres_rat(const SurfDev<T>& sd, const vec4_t<T>* reg, int nunits, int nuv, vec4_t<T>* C,
-                             float4* C32, unsigned long long* absmax, hipStream_t s) {
+static void eval_centres_rat(const SurfDev<T>& sd, const vec4_t<T>* reg, int nunits, const int* nunits_dev, int nuv,
+                             vec4_t<T>* C, float4* C32, unsigned long long* absmax, hipStream_t s) {
   const int w = nuv - 1, tw = (w + kTile - 1) / kTile;
   const dim3 g((unsigned)((long long)nunits * tw * tw)), b(64);
   const bool kl = sd.nku <= kEcKnots && sd.nkv <= kEcKnots;
   if (sd.rational) {
     if (kl)
-      hipLaunchKernelGGL((k_eval_centres<T, MAXD, true, true>), g, b, 0, s, sd, reg, nunits, nuv, C, C32, absmax);
+      hipLaunchKernelGGL((k_eval_centres<T, MAXD, true, true>), g, b, 0, s, sd, reg, nunits, nunits_dev, nuv, C, C32,
+                         absmax);
     else
-      hipLaunchKernelGGL((k_eval_centres<T, MAXD, true, false>), g, b, 0, s, sd, reg, nunits, nuv, C, C32, absmax);
+      hipLaunchKernelGGL((k_eval_centres<T, MAXD, true, false>), g, b, 0, s, sd, reg, nunits, nunits_dev, nuv, C, C32,
+                         absmax);
   } else {
     if (kl)
-      hipLaunchKernelGGL((k_eval_centres<T, MAXD, false, true>), g, b, 0, s, sd, reg, nunits, nuv, C, C32, absmax);
+      hipLaunchKernelGGL((k_eval_centres<T, MAXD, false, true>), g, b, 0, s, sd, reg, nunits, nunits_dev, nuv, C,
+                         C32, absmax);
     else
-      hipLaunchKernelGGL((k_eval_centres<T, MAXD, false, false>), g, b, 0, s, sd, reg, nunits, nuv, C, C32, absmax);
+      hipLaunchKernelGGL((k_eval_centres<T, MAXD, false, false>), g, b, 0, s, sd, reg, nunits, nunits_dev, nuv, C,
+                         C32, absmax);
   }
 }
 
 template <class T>
 hipError_t launch_eval_centres(const SurfDev<T>& sd, const vec4_t<T>* reg, int nunits, int nuv, vec4_t<T>* C,
-                               hipStream_t s, float4* C32, unsigned long long* absmax) {
+                               hipStream_t s, float4* C32, unsigned long long* absmax, const int* nunits_dev) {
   if (nunits <= 0 || nuv < 2) return hipSuccess;
   if ((C32 || absmax) && !std::is_same<T, double>::value) return hipErrorInvalidValue;
   if (C32 && !absmax) return hipErrorInvalidValue;
   const int d = std::max(sd.degu, sd.degv);
   if (d <= 3)
-    eval_centres_rat<T, 3>(sd, reg, nunits, nuv, C, C32, absmax, s);
+    eval_centres_rat<T, 3>(sd, reg, nunits, nunits_dev, nuv, C, C32, absmax, s);
   else if (d <= 7)
-    eval_centres_rat<T, 7>(sd, reg, nunits, nuv, C, C32, absmax, s);
+    eval_centres_rat<T, 7>(sd, reg, nunits, nunits_dev, nuv, C, C32, absmax, s);
   else
-    eval_centres_rat<T, kMaxDeg - 1>(sd, reg, nunits, nuv, C, C32, absmax, s);
+    eval_centres_rat<T, kMaxDeg - 1>(sd, reg, nunits, nunits_dev, nuv, C, C32, absmax, s);
   return hipGetLastError();
 }
 
@@ -2314,17 +2399,18 @@ hipError_t launch_dist_filter(const uint64_t* list, const unsigned* list_count, 
   if (list_cap == 0) return hipSuccess;
   const unsigned g = (unsigned)std::min<long long>(grid, (long long)list_cap);
   if (w1 <= kTile && w2 <= kTile) {  // every unit is one tile: one slot per list entry
+    const unsigned gu = list_cap;     // one workgroup per entry: setup latency overlaps across slots
     if (absmax) {
       if constexpr (std::is_same<T, double>::value)
-        hipLaunchKernelGGL((k_dist_filter_units<T, false, true>), dim3(g), dim3(256), 0, s, list, list_count, su1,
+        hipLaunchKernelGGL((k_dist_filter_units<T, false, true>), dim3(gu), dim3(256), 0, s, list, list_count, su1,
                            su2, w1, w2, C1, C2, thresh, out, out_count, cap, pairs_done, absmax);
       else
         return hipErrorInvalidValue;
     } else if (fma) {
-      hipLaunchKernelGGL((k_dist_filter_units<T, true, false>), dim3(g), dim3(256), 0, s, list, list_count, su1, su2,
+      hipLaunchKernelGGL((k_dist_filter_units<T, true, false>), dim3(gu), dim3(256), 0, s, list, list_count, su1, su2,
                          w1, w2, C1, C2, thresh, out, out_count, cap, pairs_done, absmax);
     } else {
-      hipLaunchKernelGGL((k_dist_filter_units<T, false, false>), dim3(g), dim3(256), 0, s, list, list_count, su1, su2,
+      hipLaunchKernelGGL((k_dist_filter_units<T, false, false>), dim3(gu), dim3(256), 0, s, list, list_count, su1, su2,
                          w1, w2, C1, C2, thresh, out, out_count, cap, pairs_done, absmax);
     }
     return hipGetLastError();
@@ -2363,11 +2449,11 @@ hipError_t launch_inflate(float* thresh, const unsigned long long* absmax_bits, 
   return hipGetLastError();
 }
 
-hipError_t launch_rerank(Cand* cand, unsigned long long n, const int* su1, const int* su2, int cells1, int cells2,
-                         const double4* C1, const double4* C2, hipStream_t s) {
-  if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_rerank, dim3(blocks_for(n, 256)), dim3(256), 0, s, cand, n, su1, su2, cells1, cells2, C1,
-                     C2);
+hipError_t launch_rerank(Cand* cand, const unsigned long long* n_dev, unsigned long long cap, const int* su1,
+                         const int* su2, int cells1, int cells2, const double4* C1, const double4* C2, hipStream_t s) {
+  if (cap == 0) return hipSuccess;
+  const unsigned g = (unsigned)std::min<unsigned long long>(1024, (cap + 255) / 256);
+  hipLaunchKernelGGL(k_rerank, dim3(g), dim3(256), 0, s, cand, n_dev, cap, su1, su2, cells1, cells2, C1, C2);
   return hipGetLastError();
 }
 
@@ -2392,24 +2478,49 @@ hipError_t launch_pair_mask(int mode, const uint64_t* list, const unsigned long 
 
 hipError_t select_topk(const Cand* cand, unsigned long long n, unsigned long long k, unsigned long long maxkey,
                        SelState* st, unsigned* hist, Cand* out, unsigned long long* out_count, unsigned small_cap,
-                       hipStream_t s) {
-  hipLaunchKernelGGL(k_sel_init, dim3(1), dim3(64), 0, s, st, n, k, maxkey, small_cap);
-  hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(kSelBins, 256)), dim3(256), 0, s, hist, 0u,
-                     (unsigned long long)kSelBins);
+                       hipStream_t s, int iters, bool resume, const unsigned long long* n_dev) {
   const unsigned g = (unsigned)std::max(1ull, std::min<unsigned long long>(1024, (n + 255) / 256));
-  const int iters = maxkey <= 0xffffffffull ? 3 : 6;  // 4096^3 > 2^31 float keys, 4096^6 > 2^63
+  if (!resume) {
+    hipLaunchKernelGGL(k_sel_init, dim3(1), dim3(64), 0, s, st, n, n_dev, k, maxkey, small_cap);
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(kSelBins, 256)), dim3(256), 0, s, hist, 0u,
+                       (unsigned long long)kSelBins);
+    hipLaunchKernelGGL(k_sel_minmax, dim3(g), dim3(256), 0, s, cand, n, n_dev, st);
+  }
   for (int it = 0; it < iters; ++it) {
-    hipLaunchKernelGGL(k_sel_hist, dim3(g), dim3(256), 0, s, cand, n, st, hist);
+    hipLaunchKernelGGL(k_sel_hist, dim3(g), dim3(256), 0, s, cand, n, n_dev, st, hist);
     hipLaunchKernelGGL(k_sel_step, dim3(1), dim3(1024), 0, s, st, hist, small_cap);
   }
   if (out) {
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, reinterpret_cast<unsigned*>(out_count), 0u, 2ull);
     if (n > 0)
-      hipLaunchKernelGGL(k_sel_compact, dim3(blocks_for(n, 256)), dim3(256), 0, s, cand, n, st, out, out_count,
+      hipLaunchKernelGGL(k_sel_compact, dim3(g), dim3(256), 0, s, cand, n, n_dev, st, out, out_count,
                          (unsigned long long)small_cap);
   }
   return hipGetLastError();
 }
+
+hipError_t launch_thresh_from_state_ru(float* thresh, const SelState* st, hipStream_t s) {
+  hipLaunchKernelGGL(k_thresh_from_state_ru, dim3(1), dim3(64), 0, s, thresh, st);
+  return hipGetLastError();
+}
+
+hipError_t launch_set_thresh_ru(float* thresh, const Cand* rec, int idx, hipStream_t s) {
+  hipLaunchKernelGGL(k_set_thresh_ru, dim3(1), dim3(64), 0, s, thresh, rec, idx);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy_cand(Cand* dst, const Cand* src, hipStream_t s) {
+  hipLaunchKernelGGL(k_copy_cand, dim3(1), dim3(64), 0, s, dst, src);
+  return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_store_vec4(vec4_t<T>* dst, vec4_t<T> v, hipStream_t s) {
+  hipLaunchKernelGGL(k_store_vec4<T>, dim3(1), dim3(64), 0, s, dst, v);
+  return hipGetLastError();
+}
+template hipError_t launch_store_vec4<float>(float4*, float4, hipStream_t);
+template hipError_t launch_store_vec4<double>(double4*, double4, hipStream_t);
 
 template <class T>
 hipError_t launch_thresh_from_state(T* thresh, const SelState* st, hipStream_t s) {
@@ -2598,7 +2709,7 @@ hipError_t launch_fill_u32(unsigned* p, unsigned value, unsigned long long n, hi
                                             unsigned long long*, int, int, const unsigned long long*);             \
   template hipError_t launch_set_thresh<T>(T*, const Cand*, int, int, hipStream_t);                                \
   template hipError_t launch_eval_centres<T>(const SurfDev<T>&, const vec4_t<T>*, int, int, vec4_t<T>*,            \
-                                             hipStream_t, float4*, unsigned long long*);                           \
+                                             hipStream_t, float4*, unsigned long long*, const int*);               \
   template hipError_t launch_seed_thresh<T>(const unsigned long long*, int, int, int, int, uint64_t*, unsigned*,   \
                                             unsigned long long*, unsigned long long*, const int*, const int*, int, \
                                             int, const vec4_t<T>*, const vec4_t<T>*, int, int, T*, hipStream_t);   \

@@ -192,10 +192,10 @@ int jsi_ctx::eval_units(int h, const vec4_t<T>* reg, int nunits, int nuv, int mo
 // Distance cell centres of `nunits` regions, evaluation fused with the AABB.
 template <class T>
 int jsi_ctx::eval_centres(int h, const vec4_t<T>* reg, int nunits, int nuv, vec4_t<T>* C, float4* C32,
-                          unsigned long long* absmax) {
+                          unsigned long long* absmax, const int* nunits_dev) {
   SurfDev<T> sd;
   RCK(surf_dev<T>(h, &sd));
-  HCK(launch_eval_centres<T>(sd, reg, nunits, nuv, C, stream, C32, absmax));
+  HCK(launch_eval_centres<T>(sd, reg, nunits, nuv, C, stream, C32, absmax, nunits_dev));
   return JSI_OK;
 }
 
@@ -205,7 +205,8 @@ int jsi_ctx::eval_centres(int h, const vec4_t<T>* reg, int nunits, int nuv, vec4
 template <class T>
 int jsi_ctx::dist_topk(int nslots, const int* su1, const int* su2, int w1, int nunits1, const vec4_t<T>* C1, int w2,
                         int nunits2, const vec4_t<T>* C2, int k, int cull, Cand** rec_out, long long* n_out,
-                        long long* kk_out, Rerank rr, const unsigned long long* exp_absmax) {
+                        long long* kk_out, Rerank rr, const unsigned long long* exp_absmax,
+                        unsigned long long cand_cap) {
   using V = vec4_t<T>;
   const bool rerank = rr.C1 != nullptr;
   const int fma = rerank ? 1 : 0;  // the FP32SCREEN screen may fuse; its margin covers the rounding
@@ -271,7 +272,15 @@ int jsi_ctx::dist_topk(int nslots, const int* su1, const int* su2, int w1, int n
     HCK(launch_set_thresh<T>(thresh, cand, 0, 1, stream));
   }
   (void)best;
-  unsigned long long n = 0;
+  // screen, re-rank and select with the candidate count kept on the device;
+  // one copy brings back both the candidate count and the selected count
+  const unsigned long long cap =
+      cand_cap ? std::min<unsigned long long>(std::max<unsigned long long>(cand_cap, 2ull * (unsigned long long)k),
+                                              kCandCap)
+               : kCandCap;
+  const unsigned small_cap = (unsigned)std::max(2048, std::min((int)kSmallCap, 2 * k));
+  const unsigned long long maxkey = rerank ? 0x7ff0000000000000ull : maxkey_t;
+  unsigned long long n = 0, m = 0;
   for (int attempt = 0;; ++attempt) {
     HCK(launch_fill_u32(list_count, 0u, 1, stream));
     HCK(launch_dist_list<T>(nslots, su1, su2, w1, w2, lo1, hi1, lo2, hi2, thresh, cull, list, list_count, stream,
@@ -282,19 +291,22 @@ int jsi_ctx::dist_topk(int nslots, const int* su1, const int* su2, int w1, int n
       HCK(launch_fill_u32(reinterpret_cast<unsigned*>(pairs_done), 0u, 2, stream));
       HCK(hipEventRecord(ev[0], stream));
     }
-    HCK(launch_dist_filter<T>(list, list_count, (unsigned)ntp, su1, su2, w1, w2, C1, C2, thresh, cand, out_count,
-                              kCandCap, grid, stream, pairs_done, fma, cull, exp_absmax));
+    HCK(launch_dist_filter<T>(list, list_count, (unsigned)ntp, su1, su2, w1, w2, C1, C2, thresh, cand, out_count, cap,
+                              grid, stream, pairs_done, fma, cull, exp_absmax));
     if (prof) HCK(hipEventRecord(ev[1], stream));
-    RCK(d2h(&n, out_count, sizeof(n)));
+    if (rerank) HCK(launch_rerank(cand, out_count, cap, su1, su2, w1 * w1, w2 * w2, rr.C1, rr.C2, stream));
+    HCK(select_topk(cand, cap, k, maxkey, st, hist, small, small_count, small_cap, stream, 2, false, out_count));
+    unsigned long long cnt[4];  // out_count, thresh, pairs_done, small_count (adjacent in d.small)
+    RCK(d2h(cnt, out_count, sizeof(cnt)));
+    n = cnt[0];
+    m = cnt[3];
     if (prof) {
       float ms = 0.f;
-      unsigned long long pd = 0;
       HCK(hipEventElapsedTime(&ms, ev[0], ev[1]));
-      RCK(d2h(&pd, pairs_done, sizeof(pd)));
       KernelStat& kst = stats[std::is_same<T, float>::value ? "dist_filter_f32" : "dist_filter_f64"];
       kst.launches += 1;
       kst.ms += ms;
-      kst.units += (double)pd;
+      kst.units += (double)cnt[2];
     }
     if (debug_) {
       unsigned lc = 0;
@@ -303,27 +315,42 @@ int jsi_ctx::dist_topk(int nslots, const int* su1, const int* su2, int w1, int n
       RCK(d2h(&tv, thresh, sizeof(tv)));
       fprintf(stderr, "[jsi] topk %s slots=%d w=%d/%d tilepairs=%llu listed=%u thresh=%.9g candidates=%llu%s\n",
               std::is_same<T, float>::value ? (rerank ? "f32s" : "f32") : "f64", nslots, w1, w2, ntp, lc,
-              (double)tv, n, n > kCandCap ? " (overflow)" : "");
+              (double)tv, n, n > cap ? " (overflow)" : "");
     }
-    if (n <= kCandCap) break;
+    if (n <= cap) break;
     if (attempt > 8) {
-      set_err("distance screen could not bound its candidate set (%llu > %llu)", n, kCandCap);
+      set_err("distance screen could not bound its candidate set (%llu > %llu)", n, cap);
       return JSI_ERR_CAPACITY;
     }
-    // the first kCandCap candidates are real pairs: their k-th d2 bounds the global k-th
-    HCK(select_topk(cand, kCandCap, k, maxkey_t, st, hist, nullptr, nullptr, kSmallCap, stream));
-    HCK(launch_thresh_from_state<T>(thresh, st, stream));
+    // the first `cap` candidates are real pairs: the k-th smallest of their
+    // keys bounds the global k-th key.  When the selection compacted all the
+    // records it needs (m <= small_cap), their k-th is read after a rank
+    // sort; otherwise the selection's bound on it is used.
+    if (m <= small_cap) {
+      Cand* sorted;
+      RCK(buf("d.sorted", (size_t)k, &sorted));
+      HCK(rank_sort(small, small_count, small_cap, (unsigned)k, sorted, stream));
+      if (rerank)
+        HCK(launch_set_thresh_ru(reinterpret_cast<float*>(thresh), sorted, k - 1, stream));
+      else
+        HCK(launch_set_thresh<T>(thresh, sorted, k - 1, 0, stream));
+    } else if (rerank) {
+      HCK(launch_thresh_from_state_ru(reinterpret_cast<float*>(thresh), st, stream));
+    } else {
+      HCK(launch_thresh_from_state<T>(thresh, st, stream));
+    }
     RCK(widen());
   }
-  unsigned long long maxkey = maxkey_t;
-  if (rerank) {
-    HCK(launch_rerank(cand, n, su1, su2, w1 * w1, w2 * w2, rr.C1, rr.C2, stream));
-    maxkey = 0x7ff0000000000000ull;
+  // two narrowing steps from the keys' own range almost always leave at most
+  // small_cap records; otherwise continue until they do or the bin holding
+  // rank k is a single key (then all its ties are sorted below)
+  for (int extra = 0; m > small_cap && extra < 32; ++extra) {
+    SelState hs;
+    RCK(d2h(&hs, st, sizeof(hs)));
+    if (hs.done) break;
+    HCK(select_topk(cand, n, k, maxkey, st, hist, small, small_count, small_cap, stream, 2, true));
+    RCK(d2h(&m, small_count, sizeof(m)));
   }
-  const unsigned small_cap = (unsigned)std::max(2048, std::min((int)kSmallCap, 2 * k));
-  HCK(select_topk(cand, n, k, maxkey, st, hist, small, small_count, small_cap, stream));
-  unsigned long long m = 0;
-  RCK(d2h(&m, small_count, sizeof(m)));
   Cand* rec = small;
   if (m > small_cap) {
     // more than kSmallCap candidates tie with the k-th key: sort them all
@@ -392,12 +419,19 @@ int jsi_ctx::min_distance(int h1, const jsi_region& r1, int h2, const jsi_region
   V *reg1, *reg2;
   RCK(buf(nm("reg", 1, cur).c_str(), 1, &reg1));
   RCK(buf(nm("reg", 2, cur).c_str(), 1, &reg2));
-  RCK(h2d(reg1, &g1, sizeof(V)));
-  RCK(h2d(reg2, &g2, sizeof(V)));
+  HCK(launch_store_vec4<T>(reg1, g1, stream));
+  HCK(launch_store_vec4<T>(reg2, g2, stream));
   int nuv1 = r1.nuv, nuv2 = r2.nuv;
-  int nunits1 = 1, nunits2 = 1;
-  T* pts;
+  int nunits1 = 1, nunits2 = 1;  // upper bounds (exact counts of later rounds stay on the device)
+  const int* nunits_cur = nullptr;
   V *C1, *C2;
+  // per-round results kept on the device and read once at the end: the k-th
+  // record and the refinement unit counts (phase timings need a
+  // synchronisation each and are taken only when profiling)
+  Cand* lastrec;
+  int* nunits_all;
+  RCK(buf("d.lastrec", JSI_MAX_ROUNDS, &lastrec));
+  RCK(buf("d.nunits", 2 * JSI_MAX_ROUNDS, &nunits_all));
   double te = now_ms();
   // double: the evaluation max-reduces the round's largest |centre
   // coordinate| (rounding margins of the screens) into amax[0]; for FP32
@@ -407,7 +441,6 @@ int jsi_ctx::min_distance(int h1, const jsi_region& r1, int h2, const jsi_region
   const bool dbl = std::is_same<T, double>::value;
   const bool s32_first = dbl && P.precision == JSI_PREC_FP32SCREEN;
   {
-    RCK(buf("pts", (size_t)3 * std::max(nuv1 * nuv1, nuv2 * nuv2), &pts));
     RCK(buf(nm("C", 1, cur).c_str(), (size_t)(nuv1 - 1) * (nuv1 - 1), &C1));
     RCK(buf(nm("C", 2, cur).c_str(), (size_t)(nuv2 - 1) * (nuv2 - 1), &C2));
     if (dbl) {
@@ -427,8 +460,10 @@ int jsi_ctx::min_distance(int h1, const jsi_region& r1, int h2, const jsi_region
   RCK(buf(nm("su", 2, cur).c_str(), 1, &su2));
   HCK(launch_fill_i32(su1, 0, 1, stream));
   HCK(launch_fill_i32(su2, 0, 1, stream));
-  HCK(hipStreamSynchronize(stream));
-  res->ms_eval += now_ms() - te;
+  if (prof) {
+    HCK(hipStreamSynchronize(stream));
+    res->ms_eval += now_ms() - te;
+  }
   int nslots = 1;
   const int nref = P.dist_refine_nuv;
   std::vector<Cand> host_rec;
@@ -449,7 +484,7 @@ int jsi_ctx::min_distance(int h1, const jsi_region& r1, int h2, const jsi_region
         rr.C2 = C2;
         rr.absmax = amax;
         const int rc = dist_topk<float>(nslots, su1, su2, w1, nunits1, f32c1, w2, nunits2, f32c2, P.k, P.cull, &rec,
-                                        &n, &kk, rr);
+                                        &n, &kk, rr, nullptr, (unsigned long long)P.cand_cap);
         if (rc == JSI_ERR_CAPACITY)
           s32 = 0;
         else if (rc != JSI_OK)
@@ -457,19 +492,19 @@ int jsi_ctx::min_distance(int h1, const jsi_region& r1, int h2, const jsi_region
       }
       if (!s32) {
         RCK(dist_topk<T>(nslots, su1, su2, w1, nunits1, C1, w2, nunits2, C2, P.k, P.cull, &rec, &n, &kk, Rerank(),
-                         amax));
+                         amax, (unsigned long long)P.cand_cap));
       }
     } else {
-      RCK(dist_topk<T>(nslots, su1, su2, w1, nunits1, C1, w2, nunits2, C2, P.k, P.cull, &rec, &n, &kk));
+      RCK(dist_topk<T>(nslots, su1, su2, w1, nunits1, C1, w2, nunits2, C2, P.k, P.cull, &rec, &n, &kk, Rerank(),
+                       nullptr, (unsigned long long)P.cand_cap));
     }
+    (void)nunits_cur;
     jsi_round_info& ri = res->round[round - 1];
     ri.npairs = kk;
     ri.slots = nslots;
     ri.nuv = nuv1;
-    Cand last{0, 0};
-    if (kk > 0) RCK(d2h(&last, rec + (kk - 1), sizeof(Cand)));
-    res->ms_screen += now_ms() - ts;
-    ri.kth_d2 = kk > 0 ? d2_from_bits<T>(last.hi) : 0.0;
+    if (kk > 0) HCK(launch_copy_cand(lastrec + (round - 1), rec + (kk - 1), stream));
+    if (prof) res->ms_screen += now_ms() - ts;
     if (P.trace && kk > 0) {
       host_rec.resize(kk);
       RCK(d2h(host_rec.data(), rec, sizeof(Cand) * kk));
@@ -486,17 +521,12 @@ int jsi_ctx::min_distance(int h1, const jsi_region& r1, int h2, const jsi_region
     }
     res->rounds = round;
     res->npairs = kk;
-    if (kk > 0) {
-      const T kd = T(d2_from_bits<T>(last.hi));
-      res->distance = (double)std::sqrt(kd);
-    } else {
-      res->distance = 0.0;
-    }
     if (kk == 0 || round == P.dist_max_rounds || round >= JSI_MAX_ROUNDS) break;
     // refinement units for the next round
     te = now_ms();
     const int nxt = cur ^ 1;
-    int *fp1, *fp2, *nsu1, *nsu2, *nunits_dev;
+    int *fp1, *fp2, *nsu1, *nsu2;
+    int* const nunits_dev = nunits_all + 2 * (round - 1);
     V *nreg1, *nreg2;
     RCK(buf_raw("fp1", sizeof(int) * (size_t)nslots * w1 * w1, reinterpret_cast<void**>(&fp1), INT_MAX));
     RCK(buf_raw("fp2", sizeof(int) * (size_t)nslots * w2 * w2, reinterpret_cast<void**>(&fp2), INT_MAX));
@@ -504,25 +534,24 @@ int jsi_ctx::min_distance(int h1, const jsi_region& r1, int h2, const jsi_region
     RCK(buf(nm("su", 2, nxt).c_str(), (size_t)kk, &nsu2));
     RCK(buf(nm("reg", 1, nxt).c_str(), (size_t)kk, &nreg1));
     RCK(buf(nm("reg", 2, nxt).c_str(), (size_t)kk, &nreg2));
-    RCK(buf("nunits", 4, &nunits_dev));
     unsigned char* dwork;
     RCK(buf_raw("d.dedupwork", dedup_work_bytes(), reinterpret_cast<void**>(&dwork), 0));
     HCK(launch_dist_dedup<T>(rec, (int)kk, w1 * w1, w2 * w2, su1, su2, reg1, reg2, nuv1, nuv2, fp1, fp2, nsu1, nsu2,
                              nreg1, nreg2, nunits_dev, dwork, stream));
-    int nu[2];
-    RCK(d2h(nu, nunits_dev, sizeof(nu)));
-    ri.units1 = nu[0];
-    ri.units2 = nu[1];
+    // at most kk units per side; the exact counts stay on the device and the
+    // evaluation skips the tiles beyond them
+    const int nu_max = (int)kk;
     V *nC1, *nC2;
     const int wr = nref - 1;
-    RCK(buf("pts", (size_t)3 * std::max(nu[0], nu[1]) * nref * nref, &pts));
-    RCK(buf(nm("C", 1, nxt).c_str(), (size_t)nu[0] * wr * wr, &nC1));
-    RCK(buf(nm("C", 2, nxt).c_str(), (size_t)nu[1] * wr * wr, &nC2));
+    RCK(buf(nm("C", 1, nxt).c_str(), (size_t)nu_max * wr * wr, &nC1));
+    RCK(buf(nm("C", 2, nxt).c_str(), (size_t)nu_max * wr * wr, &nC2));
     if (dbl) HCK(launch_fill_u32(reinterpret_cast<unsigned*>(amax), 0u, 2 * kAmaxSlots, stream));
-    RCK(eval_centres<T>(h1, nreg1, nu[0], nref, nC1, nullptr, amax));
-    RCK(eval_centres<T>(h2, nreg2, nu[1], nref, nC2, nullptr, amax));
-    HCK(hipStreamSynchronize(stream));
-    res->ms_eval += now_ms() - te;
+    RCK(eval_centres<T>(h1, nreg1, nu_max, nref, nC1, nullptr, amax, nunits_dev));
+    RCK(eval_centres<T>(h2, nreg2, nu_max, nref, nC2, nullptr, amax, nunits_dev + 1));
+    if (prof) {
+      HCK(hipStreamSynchronize(stream));
+      res->ms_eval += now_ms() - te;
+    }
     cur = nxt;
     su1 = nsu1;
     su2 = nsu2;
@@ -530,10 +559,31 @@ int jsi_ctx::min_distance(int h1, const jsi_region& r1, int h2, const jsi_region
     reg2 = nreg2;
     C1 = nC1;
     C2 = nC2;
-    nunits1 = nu[0];
-    nunits2 = nu[1];
+    nunits1 = nu_max;
+    nunits2 = nu_max;
     nslots = (int)kk;
     nuv1 = nuv2 = nref;
+  }
+  {
+    const int nr = res->rounds;
+    std::vector<Cand> lr(nr);
+    std::vector<int> nu(2 * nr);
+    RCK(d2h(lr.data(), lastrec, sizeof(Cand) * nr));
+    RCK(d2h(nu.data(), nunits_all, sizeof(int) * 2 * nr));
+    for (int r = 0; r < nr; ++r) {
+      jsi_round_info& ri = res->round[r];
+      ri.kth_d2 = ri.npairs > 0 ? d2_from_bits<T>(lr[r].hi) : 0.0;
+      if (r + 1 < nr) {
+        ri.units1 = nu[2 * r];
+        ri.units2 = nu[2 * r + 1];
+      }
+    }
+    if (kk > 0) {
+      const T kd = T(d2_from_bits<T>(lr[nr - 1].hi));
+      res->distance = (double)std::sqrt(kd);
+    } else {
+      res->distance = 0.0;
+    }
   }
   if (out && cap_pairs > 0 && kk > 0) {
     const long long m = std::min<long long>(kk, cap_pairs);
@@ -611,8 +661,10 @@ int jsi_ctx::intersect(int h1, const jsi_region& r1, int h2, const jsi_region& r
   RCK(buf(nm("su", 2, cur).c_str(), 1, &su2));
   HCK(launch_fill_i32(su1, 0, 1, stream));
   HCK(launch_fill_i32(su2, 0, 1, stream));
-  HCK(hipStreamSynchronize(stream));
-  res->ms_eval += now_ms() - te;
+  if (prof) {
+    HCK(hipStreamSynchronize(stream));
+    res->ms_eval += now_ms() - te;
+  }
   int nunits1 = 1, nunits2 = 1;
   long long nslots = 1;
   int g_nuv = P.isect_refine_nuv;
@@ -1074,4 +1126,5 @@ int jsi_intersect(jsi_ctx* c, int h1, const jsi_region* r1, int h2, const jsi_re
 
 // used by the operator ABI (jsi_opabi.hip)
 template int jsi_ctx::dist_topk<float>(int, const int*, const int*, int, int, const float4*, int, int, const float4*,
-                                       int, int, Cand**, long long*, long long*, Rerank, const unsigned long long*);
+                                       int, int, Cand**, long long*, long long*, Rerank, const unsigned long long*,
+                                       unsigned long long);

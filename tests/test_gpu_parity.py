@@ -99,6 +99,21 @@ def test_culling_leaves_results_unchanged(ctx, orc, cull, prec):
     np.testing.assert_array_equal(pairs, opairs)
 
 
+@pytest.mark.parametrize("prec", ["exact32", "fp64", "fp32screen"])
+def test_candidate_overflow_path_matches_oracle(ctx, orc, prec):
+    """A candidate capacity below the screen's candidate count forces the
+    overflow path (k-th bound of the captured candidates, repeated screen)."""
+    a, b = S.hard_wave_pair()
+    r1, r2 = regions(a, b, 128)
+    k = 512
+    p = default_params(k=k, precision=prec, trace=1, cand_cap=2 * k)
+    res, pairs = ctx.minimum_distance(a, r1, b, r2, p)
+    ores, opairs = orc.min_distance(a, r1, b, r2, default_params(k=k, precision=prec, trace=1))
+    same_rounds(res, ores)
+    assert res.distance == ores.distance
+    np.testing.assert_array_equal(pairs, opairs)
+
+
 def dense_pair(nu, nv, deg=3):
     """Hard-wave shaped surfaces on a nu x nv control net: long knot vectors
     (read from HBM by the evaluation when over 128 knots) and tiles that cover
