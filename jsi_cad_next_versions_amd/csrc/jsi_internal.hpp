@@ -76,15 +76,26 @@ template <class T>
 hipError_t launch_points(const SurfDev<T>& sd, const T* tabU, const int* spanU, const T* tabV, const int* spanV,
                          int nunits, int nuv, int mode, T* out, unsigned long long* kmax, hipStream_t s);
 
-// Fused evaluation + distance centres of nunits regions (one workgroup per
-// 16x16-cell tile); C as launch_aabb(isect = 0) writes it.  T = double only:
-// absmax max-reduces the largest |coordinate| (double bits); C32 (requires
-// absmax) also writes the centres rounded to float.
-template <class T>
+// Outputs of the fused evaluation + distance centres.  C as launch_aabb(isect
+// = 0) writes it; optional: lo/hi, the per-tile box of the centres
+// (launch_tilebox isect = 0); T = double only: absmax max-reduces the largest
+// |coordinate| (double bits, kAmaxSlots partial maxima), C32 (requires
+// absmax) the centres rounded to float and lo32/hi32 their tile boxes.
 // nunits_dev (optional): the true unit count on the device, <= nunits.
-hipError_t launch_eval_centres(const SurfDev<T>& sd, const vec4_t<T>* reg, int nunits, int nuv, vec4_t<T>* C,
-                               hipStream_t s, float4* C32 = nullptr, unsigned long long* absmax = nullptr,
-                               const int* nunits_dev = nullptr);
+template <class T>
+struct EcOut {
+  vec4_t<T>* C = nullptr;
+  vec4_t<T>* lo = nullptr;
+  vec4_t<T>* hi = nullptr;
+  float4* C32 = nullptr;
+  float4* lo32 = nullptr;
+  float4* hi32 = nullptr;
+  unsigned long long* absmax = nullptr;
+  const int* nunits_dev = nullptr;
+};
+template <class T>
+hipError_t launch_eval_centres(const SurfDev<T>& sd, const vec4_t<T>* reg, int nunits, int nuv, const EcOut<T>& o,
+                               hipStream_t s);
 
 // Cell AABBs of every unit from its SoA points.  isect = 0: distance centres
 // (construct_aabbs_distance_host); isect = 1: centre + padded radius
@@ -116,8 +127,9 @@ template <class T>
 hipError_t launch_seed_thresh(const unsigned long long* best, int npos, int tpu1, int nseed, int ordered,
                               uint64_t* list, unsigned* list_count, unsigned long long* range,
                               unsigned long long* work, const int* su1, const int* su2, int w1, int w2,
-                              const vec4_t<T>* C1, const vec4_t<T>* C2, int k, int fma, T* thresh, hipStream_t s);
-size_t seed_work_bytes();  // size of `work` for launch_seed_thresh
+                              const vec4_t<T>* C1, const vec4_t<T>* C2, int k, int fma, T* thresh, hipStream_t s,
+                              unsigned* clear = nullptr, int nclear = 0);
+size_t seed_work_bytes();  // size of `work` for launch_seed_thresh; clear: words zeroed at the end
 template <class T>
 hipError_t launch_dist_filter(const uint64_t* list, const unsigned* list_count, unsigned list_cap, const int* su1,
                               const int* su2, int w1, int w2, const vec4_t<T>* C1, const vec4_t<T>* C2,
@@ -166,14 +178,18 @@ hipError_t launch_copy_cand(Cand* dst, const Cand* src, hipStream_t s);
 template <class T>
 hipError_t launch_store_vec4(vec4_t<T>* dst, vec4_t<T> v, hipStream_t s);
 template <class T>
+hipError_t launch_query_init(vec4_t<T>* reg1, vec4_t<T> g1, vec4_t<T>* reg2, vec4_t<T> g2, int* su1, int* su2,
+                             unsigned long long* amax, hipStream_t s);
+template <class T>
 hipError_t launch_thresh_from_state(T* thresh, const SelState* st, hipStream_t s);
 
 // Sorting of candidate records (bitonic, n2 = power of two >= kSortBlock).
 hipError_t launch_fill_sentinel(Cand* a, unsigned long long from, unsigned long long n2, hipStream_t s);
 hipError_t sort_cands(Cand* a, unsigned long long n2, hipStream_t s);
 // Sorted first `keep` of m (device count, at most m_cap) distinct records.
+// last_out (optional) receives the last kept record (rank min(keep, m) - 1).
 hipError_t rank_sort(const Cand* in, const unsigned long long* m_dev, unsigned m_cap, unsigned keep, Cand* out,
-                     hipStream_t s);
+                     hipStream_t s, Cand* last_out = nullptr);
 
 // Distance refinement: first-appearance dedup of the sorted top-k endpoints,
 // child regions and the next round's slot -> unit maps (three passes).
@@ -181,7 +197,8 @@ template <class T>
 hipError_t launch_dist_dedup(const Cand* rec, int n, int cells1, int cells2, const int* su1, const int* su2,
                              const vec4_t<T>* reg1, const vec4_t<T>* reg2, int parent_nuv1, int parent_nuv2,
                              int* firstpos1, int* firstpos2, int* nsu1, int* nsu2, vec4_t<T>* nreg1,
-                             vec4_t<T>* nreg2, int* nunits, unsigned char* work, hipStream_t s);
+                             vec4_t<T>* nreg2, int* nunits, unsigned char* work, hipStream_t s,
+                             unsigned long long* zero = nullptr, int nzero = 0);
 size_t dedup_work_bytes();  // size of `work` for launch_dist_dedup
 
 // Intersection screen.

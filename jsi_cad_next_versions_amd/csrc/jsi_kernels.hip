@@ -376,9 +376,12 @@ struct EcRows {  // control rows per chunk of u-contractions
 };
 
 template <class T, int MAXD, bool RAT, bool KL>
-__global__ __launch_bounds__(64) void k_eval_centres(SurfDev<T> sd, const vec4_t<T>* regions, int nunits,
-                                                     const int* nunits_dev, int nuv, vec4_t<T>* C, float4* C32,
-                                                     unsigned long long* absmax) {
+__global__ __launch_bounds__(64) void k_eval_centres(SurfDev<T> sd, const vec4_t<T>* regions, int nunits, int nuv,
+                                                     EcOut<T> out) {
+  vec4_t<T>* const C = out.C;
+  float4* const C32 = out.C32;
+  unsigned long long* const absmax = out.absmax;
+  const int* const nunits_dev = out.nunits_dev;
   constexpr int E = kTile + 1;  // corner points per tile edge
   constexpr int kRows = EcRows<MAXD>::value;
   constexpr int kCorners = (E * E + 63) / 64;  // corners per lane
@@ -527,8 +530,12 @@ __global__ __launch_bounds__(64) void k_eval_centres(SurfDev<T> sd, const vec4_t
     sP[2 * E * E + x] = S[2];
   }
   __syncthreads();
-  // phase 5: cell centres
+  // phase 5: cell centres (and the tile's box of them)
   T am = T(0);
+  const T inf = infinity_v<T>();
+  T bmn[3] = {inf, inf, inf}, bmx[3] = {-inf, -inf, -inf};
+  float fmn[3] = {__uint_as_float(0x7f800000u), __uint_as_float(0x7f800000u), __uint_as_float(0x7f800000u)};
+  float fmx[3] = {-fmn[0], -fmn[0], -fmn[0]};
   for (int cidx = lane; cidx < kTileCells; cidx += 64) {
     const int ly = cidx / kTile, lx = cidx % kTile;
     if (ly >= ht || lx >= wd) continue;
@@ -547,6 +554,11 @@ __global__ __launch_bounds__(64) void k_eval_centres(SurfDev<T> sd, const vec4_t
     ctr.w = T(0);
     const size_t o = (size_t)unit * w * w + (row0 + ly) * w + col0 + lx;
     C[o] = ctr;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      bmn[a] = fmin(bmn[a], c[a]);
+      bmx[a] = fmax(bmx[a], c[a]);
+    }
     if constexpr (std::is_same<T, double>::value) {
       if (C32) {
         float4 f;
@@ -555,8 +567,49 @@ __global__ __launch_bounds__(64) void k_eval_centres(SurfDev<T> sd, const vec4_t
         f.z = __double2float_rn(ctr.z);
         f.w = 0.f;
         C32[o] = f;
+        fmn[0] = fminf(fmn[0], f.x);
+        fmn[1] = fminf(fmn[1], f.y);
+        fmn[2] = fminf(fmn[2], f.z);
+        fmx[0] = fmaxf(fmx[0], f.x);
+        fmx[1] = fmaxf(fmx[1], f.y);
+        fmx[2] = fmaxf(fmx[2], f.z);
       }
       am = fmax(am, fmax(fabs(ctr.x), fmax(fabs(ctr.y), fabs(ctr.z))));
+    }
+  }
+  if (out.lo) {  // uniform: a kernel argument
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        bmn[a] = fmin(bmn[a], __shfl_xor(bmn[a], off));
+        bmx[a] = fmax(bmx[a], __shfl_xor(bmx[a], off));
+      }
+    if (lane == 0) {
+      vec4_t<T> l, h;
+      l.x = bmn[0];
+      l.y = bmn[1];
+      l.z = bmn[2];
+      l.w = T(0);
+      h.x = bmx[0];
+      h.y = bmx[1];
+      h.z = bmx[2];
+      h.w = T(0);
+      out.lo[blockIdx.x] = l;  // block = unit * tiles per unit + tile, the tile box index
+      out.hi[blockIdx.x] = h;
+    }
+  }
+  if (out.lo32) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        fmn[a] = fminf(fmn[a], __shfl_xor(fmn[a], off));
+        fmx[a] = fmaxf(fmx[a], __shfl_xor(fmx[a], off));
+      }
+    if (lane == 0) {
+      out.lo32[blockIdx.x] = make_float4(fmn[0], fmn[1], fmn[2], 0.f);
+      out.hi32[blockIdx.x] = make_float4(fmx[0], fmx[1], fmx[2], 0.f);
     }
   }
   if constexpr (std::is_same<T, double>::value) {
@@ -953,7 +1006,8 @@ __global__ __launch_bounds__(1024) void k_seed_range(const unsigned long long* m
 // fewer than k pairs.
 template <class T>
 __global__ __launch_bounds__(1024) void k_seed_thresh(const unsigned* gh, const unsigned long long* range, int k,
-                                                      T* thresh) {
+                                                      T* thresh, unsigned* clear, int nclear) {
+  if ((int)threadIdx.x < nclear) clear[threadIdx.x] = 0u;  // the screen's counters, free again after the seed
   __shared__ unsigned long long cum[kSeedHistBins];
   __shared__ int found;
   const int b = threadIdx.x;  // blockDim.x == kSeedHistBins
@@ -1552,9 +1606,13 @@ __device__ __forceinline__ unsigned long long sel_n(unsigned long long n, const 
   return n_dev ? min(*n_dev, n) : n;
 }
 
-__global__ void k_sel_init(SelState* st, unsigned long long n, const unsigned long long* n_dev, unsigned long long k,
-                           unsigned long long maxkey, unsigned small_cap) {
+__global__ __launch_bounds__(256) void k_sel_init(SelState* st, unsigned long long n,
+                                                  const unsigned long long* n_dev, unsigned long long k,
+                                                  unsigned long long maxkey, unsigned small_cap, unsigned* hist,
+                                                  unsigned long long* out_count) {
+  for (int b = threadIdx.x; b < kSelBins; b += blockDim.x) hist[b] = 0;
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (out_count) *out_count = 0;
   n = sel_n(n, n_dev);
   SelState s;
   s.lo = ~0ull;  // min / max key, reduced by k_sel_minmax
@@ -1572,6 +1630,7 @@ __global__ __launch_bounds__(256) void k_sel_minmax(const Cand* cand, unsigned l
   __shared__ unsigned long long rmin[4], rmax[4];
   if (st->done) return;
   n = sel_n(n, n_dev);
+  if ((unsigned long long)blockIdx.x * blockDim.x >= n) return;  // whole block: no records
   unsigned long long a = ~0ull, b = 0ull;
   for (unsigned long long e = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; e < n;
        e += (unsigned long long)gridDim.x * blockDim.x) {
@@ -1606,6 +1665,7 @@ __global__ __launch_bounds__(256) void k_sel_hist(const Cand* cand, unsigned lon
   __shared__ unsigned h[kSelBins];
   if (st->done) return;
   n = sel_n(n, n_dev);
+  if ((unsigned long long)blockIdx.x * blockDim.x >= n) return;  // whole block: no records
   const unsigned long long lo = st->lo, hi = st->hi, width = sel_width(lo, hi);
   for (int b = threadIdx.x; b < kSelBins; b += blockDim.x) h[b] = 0;
   __syncthreads();
@@ -1707,6 +1767,20 @@ __global__ void k_set_thresh_ru(float* thresh, const Cand* rec, int idx) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *thresh = __double2float_ru(from_bits<double>(rec[idx].hi));
 }
 
+// Start of a distance query: both sides' round-1 regions, slot 0 -> unit 0,
+// and the max |coordinate| slots cleared (one launch instead of five).
+template <class T>
+__global__ void k_query_init(vec4_t<T>* reg1, vec4_t<T> g1, vec4_t<T>* reg2, vec4_t<T> g2, int* su1, int* su2,
+                             unsigned long long* amax) {
+  if (amax && threadIdx.x < kAmaxSlots) amax[threadIdx.x] = 0ull;
+  if (threadIdx.x == 0) {
+    *reg1 = g1;
+    *reg2 = g2;
+    *su1 = 0;
+    *su2 = 0;
+  }
+}
+
 // Small device-side stores (no host round trip): one record, one region.
 __global__ void k_copy_cand(Cand* dst, const Cand* src) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *dst = *src;
@@ -1791,7 +1865,7 @@ __global__ __launch_bounds__(1024) void k_bsort_single(Cand* a, int n2) {
 // summed in LDS.  O(m^2) compares spread over m/64 * 16 wavefronts.
 constexpr int kRankWaves = 16;
 __global__ __launch_bounds__(1024) void k_rank_sort(const Cand* in, const unsigned long long* m_dev, unsigned m_cap,
-                                                    unsigned keep, Cand* out) {
+                                                    unsigned keep, Cand* out, Cand* last_out) {
   __shared__ Cand stage[kRankWaves][64];
   __shared__ unsigned part[kRankWaves][64];
   const unsigned m = (unsigned)min((unsigned long long)m_cap, *m_dev);
@@ -1822,6 +1896,7 @@ __global__ __launch_bounds__(1024) void k_rank_sort(const Cand* in, const unsign
 #pragma unroll
     for (int v = 0; v < kRankWaves; ++v) r += part[v][lane];
     if (r < keep) out[r] = mine;
+    if (last_out && r == min(keep, m) - 1) *last_out = mine;  // the last kept record
   }
 }
 
@@ -1888,7 +1963,9 @@ __global__ __launch_bounds__(256) void k_dedup_mark(const Cand* rec, int n, int 
 // slot map entry is the unit of its first appearance.
 __global__ __launch_bounds__(1024) void k_dedup_units(const Cand* rec, int n, int cells1, int cells2,
                                                       const int* fp1, const int* fp2, int* nsu1, int* nsu2,
-                                                      unsigned char* isfirst, int* nunits) {
+                                                      unsigned char* isfirst, int* nunits, unsigned long long* zero,
+                                                      int nzero) {
+  if ((int)threadIdx.x < nzero) zero[threadIdx.x] = 0ull;  // next round's max |coordinate| slots
   __shared__ int sUnit[kDedupMax];
   __shared__ int sScan[1024];
   const int per = (n + 1023) / 1024;
@@ -2253,41 +2330,38 @@ hipError_t launch_points(const SurfDev<T>& sd, const T* tabU, const int* spanU, 
 }
 
 template <class T, int MAXD>
-static void eval_centres_rat(const SurfDev<T>& sd, const vec4_t<T>* reg, int nunits, const int* nunits_dev, int nuv,
-                             vec4_t<T>* C, float4* C32, unsigned long long* absmax, hipStream_t s) {
+static void eval_centres_rat(const SurfDev<T>& sd, const vec4_t<T>* reg, int nunits, int nuv, const EcOut<T>& o,
+                             hipStream_t s) {
   const int w = nuv - 1, tw = (w + kTile - 1) / kTile;
   const dim3 g((unsigned)((long long)nunits * tw * tw)), b(64);
   const bool kl = sd.nku <= kEcKnots && sd.nkv <= kEcKnots;
   if (sd.rational) {
     if (kl)
-      hipLaunchKernelGGL((k_eval_centres<T, MAXD, true, true>), g, b, 0, s, sd, reg, nunits, nunits_dev, nuv, C, C32,
-                         absmax);
+      hipLaunchKernelGGL((k_eval_centres<T, MAXD, true, true>), g, b, 0, s, sd, reg, nunits, nuv, o);
     else
-      hipLaunchKernelGGL((k_eval_centres<T, MAXD, true, false>), g, b, 0, s, sd, reg, nunits, nunits_dev, nuv, C, C32,
-                         absmax);
+      hipLaunchKernelGGL((k_eval_centres<T, MAXD, true, false>), g, b, 0, s, sd, reg, nunits, nuv, o);
   } else {
     if (kl)
-      hipLaunchKernelGGL((k_eval_centres<T, MAXD, false, true>), g, b, 0, s, sd, reg, nunits, nunits_dev, nuv, C,
-                         C32, absmax);
+      hipLaunchKernelGGL((k_eval_centres<T, MAXD, false, true>), g, b, 0, s, sd, reg, nunits, nuv, o);
     else
-      hipLaunchKernelGGL((k_eval_centres<T, MAXD, false, false>), g, b, 0, s, sd, reg, nunits, nunits_dev, nuv, C,
-                         C32, absmax);
+      hipLaunchKernelGGL((k_eval_centres<T, MAXD, false, false>), g, b, 0, s, sd, reg, nunits, nuv, o);
   }
 }
 
 template <class T>
-hipError_t launch_eval_centres(const SurfDev<T>& sd, const vec4_t<T>* reg, int nunits, int nuv, vec4_t<T>* C,
-                               hipStream_t s, float4* C32, unsigned long long* absmax, const int* nunits_dev) {
+hipError_t launch_eval_centres(const SurfDev<T>& sd, const vec4_t<T>* reg, int nunits, int nuv, const EcOut<T>& o,
+                               hipStream_t s) {
   if (nunits <= 0 || nuv < 2) return hipSuccess;
-  if ((C32 || absmax) && !std::is_same<T, double>::value) return hipErrorInvalidValue;
-  if (C32 && !absmax) return hipErrorInvalidValue;
+  if ((o.C32 || o.absmax || o.lo32) && !std::is_same<T, double>::value) return hipErrorInvalidValue;
+  if ((o.C32 && !o.absmax) || (o.lo32 && !o.C32) || (!o.lo != !o.hi) || (!o.lo32 != !o.hi32))
+    return hipErrorInvalidValue;
   const int d = std::max(sd.degu, sd.degv);
   if (d <= 3)
-    eval_centres_rat<T, 3>(sd, reg, nunits, nunits_dev, nuv, C, C32, absmax, s);
+    eval_centres_rat<T, 3>(sd, reg, nunits, nuv, o, s);
   else if (d <= 7)
-    eval_centres_rat<T, 7>(sd, reg, nunits, nunits_dev, nuv, C, C32, absmax, s);
+    eval_centres_rat<T, 7>(sd, reg, nunits, nuv, o, s);
   else
-    eval_centres_rat<T, kMaxDeg - 1>(sd, reg, nunits, nunits_dev, nuv, C, C32, absmax, s);
+    eval_centres_rat<T, kMaxDeg - 1>(sd, reg, nunits, nuv, o, s);
   return hipGetLastError();
 }
 
@@ -2344,8 +2418,9 @@ template <class T>
 hipError_t launch_seed_thresh(const unsigned long long* best, int npos, int tpu1, int nseed, int ordered,
                               uint64_t* list, unsigned* list_count, unsigned long long* range,
                               unsigned long long* work, const int* su1, const int* su2, int w1, int w2,
-                              const vec4_t<T>* C1, const vec4_t<T>* C2, int k, int fma, T* thresh, hipStream_t s) {
-  if (npos > kSeedSortMax || nseed > kSeedMaxTiles) return hipErrorInvalidValue;
+                              const vec4_t<T>* C1, const vec4_t<T>* C2, int k, int fma, T* thresh, hipStream_t s,
+                              unsigned* clear, int nclear) {
+  if (npos > kSeedSortMax || nseed > kSeedMaxTiles || nclear > kSeedHistBins) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_seed_list<T>, dim3(1), dim3(1024), 0, s, best, npos, tpu1, nseed, ordered, list, list_count,
                      range);
   const int m = nseed < npos ? nseed : npos;
@@ -2367,7 +2442,7 @@ hipError_t launch_seed_thresh(const unsigned long long* best, int npos, int tpu1
     hipLaunchKernelGGL((k_seed_hist<T, false, false>), dim3(nb), dim3(256), 0, s, list, list_count, su1, su2, w1, w2,
                        C1, C2, split, range, work, gh);
   }
-  hipLaunchKernelGGL(k_seed_thresh<T>, dim3(1), dim3(kSeedHistBins), 0, s, gh, range, k, thresh);
+  hipLaunchKernelGGL(k_seed_thresh<T>, dim3(1), dim3(kSeedHistBins), 0, s, gh, range, k, thresh, clear, nclear);
   return hipGetLastError();
 }
 
@@ -2481,17 +2556,17 @@ hipError_t select_topk(const Cand* cand, unsigned long long n, unsigned long lon
                        hipStream_t s, int iters, bool resume, const unsigned long long* n_dev) {
   const unsigned g = (unsigned)std::max(1ull, std::min<unsigned long long>(1024, (n + 255) / 256));
   if (!resume) {
-    hipLaunchKernelGGL(k_sel_init, dim3(1), dim3(64), 0, s, st, n, n_dev, k, maxkey, small_cap);
-    hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(kSelBins, 256)), dim3(256), 0, s, hist, 0u,
-                       (unsigned long long)kSelBins);
+    hipLaunchKernelGGL(k_sel_init, dim3(1), dim3(256), 0, s, st, n, n_dev, k, maxkey, small_cap, hist,
+                       out ? out_count : nullptr);
     hipLaunchKernelGGL(k_sel_minmax, dim3(g), dim3(256), 0, s, cand, n, n_dev, st);
+  } else if (out) {
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, reinterpret_cast<unsigned*>(out_count), 0u, 2ull);
   }
   for (int it = 0; it < iters; ++it) {
     hipLaunchKernelGGL(k_sel_hist, dim3(g), dim3(256), 0, s, cand, n, n_dev, st, hist);
     hipLaunchKernelGGL(k_sel_step, dim3(1), dim3(1024), 0, s, st, hist, small_cap);
   }
   if (out) {
-    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, reinterpret_cast<unsigned*>(out_count), 0u, 2ull);
     if (n > 0)
       hipLaunchKernelGGL(k_sel_compact, dim3(g), dim3(256), 0, s, cand, n, n_dev, st, out, out_count,
                          (unsigned long long)small_cap);
@@ -2523,6 +2598,17 @@ template hipError_t launch_store_vec4<float>(float4*, float4, hipStream_t);
 template hipError_t launch_store_vec4<double>(double4*, double4, hipStream_t);
 
 template <class T>
+hipError_t launch_query_init(vec4_t<T>* reg1, vec4_t<T> g1, vec4_t<T>* reg2, vec4_t<T> g2, int* su1, int* su2,
+                             unsigned long long* amax, hipStream_t s) {
+  hipLaunchKernelGGL(k_query_init<T>, dim3(1), dim3(64), 0, s, reg1, g1, reg2, g2, su1, su2, amax);
+  return hipGetLastError();
+}
+template hipError_t launch_query_init<float>(float4*, float4, float4*, float4, int*, int*, unsigned long long*,
+                                             hipStream_t);
+template hipError_t launch_query_init<double>(double4*, double4, double4*, double4, int*, int*, unsigned long long*,
+                                              hipStream_t);
+
+template <class T>
 hipError_t launch_thresh_from_state(T* thresh, const SelState* st, hipStream_t s) {
   hipLaunchKernelGGL(k_thresh_from_state<T>, dim3(1), dim3(64), 0, s, thresh, st);
   return hipGetLastError();
@@ -2537,9 +2623,10 @@ hipError_t launch_fill_sentinel(Cand* a, unsigned long long from, unsigned long 
 }
 
 hipError_t rank_sort(const Cand* in, const unsigned long long* m_dev, unsigned m_cap, unsigned keep, Cand* out,
-                     hipStream_t s) {
+                     hipStream_t s, Cand* last_out) {
   if (m_cap == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_rank_sort, dim3(blocks_for(m_cap, 64)), dim3(64 * kRankWaves), 0, s, in, m_dev, m_cap, keep, out);
+  hipLaunchKernelGGL(k_rank_sort, dim3(blocks_for(m_cap, 64)), dim3(64 * kRankWaves), 0, s, in, m_dev, m_cap, keep, out,
+                     last_out);
   return hipGetLastError();
 }
 
@@ -2567,12 +2654,13 @@ template <class T>
 hipError_t launch_dist_dedup(const Cand* rec, int n, int cells1, int cells2, const int* su1, const int* su2,
                              const vec4_t<T>* reg1, const vec4_t<T>* reg2, int parent_nuv1, int parent_nuv2,
                              int* firstpos1, int* firstpos2, int* nsu1, int* nsu2, vec4_t<T>* nreg1,
-                             vec4_t<T>* nreg2, int* nunits, unsigned char* work, hipStream_t s) {
-  if (n <= 0 || n > kDedupMax) return hipErrorInvalidValue;
+                             vec4_t<T>* nreg2, int* nunits, unsigned char* work, hipStream_t s,
+                             unsigned long long* zero, int nzero) {
+  if (n <= 0 || n > kDedupMax || nzero > 1024) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_dedup_mark, dim3(blocks_for(n, 256)), dim3(256), 0, s, rec, n, cells1, cells2, firstpos1,
                      firstpos2);
   hipLaunchKernelGGL(k_dedup_units, dim3(1), dim3(1024), 0, s, rec, n, cells1, cells2, firstpos1, firstpos2, nsu1,
-                     nsu2, work, nunits);
+                     nsu2, work, nunits, zero, nzero);
   hipLaunchKernelGGL(k_dedup_apply<T>, dim3(blocks_for(n, 256)), dim3(256), 0, s, rec, n, cells1, cells2, su1, su2,
                      reg1, reg2, parent_nuv1, parent_nuv2, work, nsu1, nsu2, nreg1, nreg2, firstpos1, firstpos2);
   return hipGetLastError();
@@ -2708,14 +2796,16 @@ hipError_t launch_fill_u32(unsigned* p, unsigned value, unsigned long long n, hi
                                             unsigned long long*, unsigned long long, int, hipStream_t,             \
                                             unsigned long long*, int, int, const unsigned long long*);             \
   template hipError_t launch_set_thresh<T>(T*, const Cand*, int, int, hipStream_t);                                \
-  template hipError_t launch_eval_centres<T>(const SurfDev<T>&, const vec4_t<T>*, int, int, vec4_t<T>*,            \
-                                             hipStream_t, float4*, unsigned long long*, const int*);               \
+  template hipError_t launch_eval_centres<T>(const SurfDev<T>&, const vec4_t<T>*, int, int, const EcOut<T>&,       \
+                                             hipStream_t);                                                         \
   template hipError_t launch_seed_thresh<T>(const unsigned long long*, int, int, int, int, uint64_t*, unsigned*,   \
                                             unsigned long long*, unsigned long long*, const int*, const int*, int, \
-                                            int, const vec4_t<T>*, const vec4_t<T>*, int, int, T*, hipStream_t);   \
+                                            int, const vec4_t<T>*, const vec4_t<T>*, int, int, T*, hipStream_t,    \
+                                            unsigned*, int);                                                       \
   template hipError_t launch_dist_dedup<T>(const Cand*, int, int, int, const int*, const int*, const vec4_t<T>*,   \
                                            const vec4_t<T>*, int, int, int*, int*, int*, int*, vec4_t<T>*,         \
-                                           vec4_t<T>*, int*, unsigned char*, hipStream_t);                         \
+                                           vec4_t<T>*, int*, unsigned char*, hipStream_t, unsigned long long*,     \
+                                           int);                                                                   \
   template hipError_t launch_isect_list<T>(int, const int*, const int*, int, int, const vec4_t<T>*,                \
                                            const vec4_t<T>*, const vec4_t<T>*, const vec4_t<T>*, int, uint64_t*,   \
                                            unsigned long long*, unsigned long long, hipStream_t);                  \

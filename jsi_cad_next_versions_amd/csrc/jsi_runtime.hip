@@ -191,11 +191,10 @@ int jsi_ctx::eval_units(int h, const vec4_t<T>* reg, int nunits, int nuv, int mo
 
 // Distance cell centres of `nunits` regions, evaluation fused with the AABB.
 template <class T>
-int jsi_ctx::eval_centres(int h, const vec4_t<T>* reg, int nunits, int nuv, vec4_t<T>* C, float4* C32,
-                          unsigned long long* absmax, const int* nunits_dev) {
+int jsi_ctx::eval_centres(int h, const vec4_t<T>* reg, int nunits, int nuv, const EcOut<T>& o) {
   SurfDev<T> sd;
   RCK(surf_dev<T>(h, &sd));
-  HCK(launch_eval_centres<T>(sd, reg, nunits, nuv, C, stream, C32, absmax, nunits_dev));
+  HCK(launch_eval_centres<T>(sd, reg, nunits, nuv, o, stream));
   return JSI_OK;
 }
 
@@ -206,7 +205,7 @@ template <class T>
 int jsi_ctx::dist_topk(int nslots, const int* su1, const int* su2, int w1, int nunits1, const vec4_t<T>* C1, int w2,
                         int nunits2, const vec4_t<T>* C2, int k, int cull, Cand** rec_out, long long* n_out,
                         long long* kk_out, Rerank rr, const unsigned long long* exp_absmax,
-                        unsigned long long cand_cap) {
+                        unsigned long long cand_cap, TileBoxes<vec4_t<T>> tbx, Cand* last_out) {
   using V = vec4_t<T>;
   const bool rerank = rr.C1 != nullptr;
   const int fma = rerank ? 1 : 0;  // the FP32SCREEN screen may fuse; its margin covers the rounding
@@ -214,13 +213,20 @@ int jsi_ctx::dist_topk(int nslots, const int* su1, const int* su2, int w1, int n
       std::is_same<T, float>::value ? 0x7f800000ull : 0x7ff0000000000000ull;  // +inf bits
   const int tpu1 = ((w1 + kTile - 1) / kTile) * ((w1 + kTile - 1) / kTile);
   const int tpu2 = ((w2 + kTile - 1) / kTile) * ((w2 + kTile - 1) / kTile);
-  V *lo1, *hi1, *lo2, *hi2;
-  RCK(buf("tb.lo1", (size_t)nunits1 * tpu1, &lo1));
-  RCK(buf("tb.hi1", (size_t)nunits1 * tpu1, &hi1));
-  RCK(buf("tb.lo2", (size_t)nunits2 * tpu2, &lo2));
-  RCK(buf("tb.hi2", (size_t)nunits2 * tpu2, &hi2));
-  HCK(launch_tilebox<T>(C1, nullptr, nunits1, w1, 0, lo1, hi1, stream));
-  HCK(launch_tilebox<T>(C2, nullptr, nunits2, w2, 0, lo2, hi2, stream));
+  const V *lo1 = tbx.lo1, *hi1 = tbx.hi1, *lo2 = tbx.lo2, *hi2 = tbx.hi2;
+  if (!lo1) {
+    V *l1, *h1, *l2, *h2;
+    RCK(buf("tb.lo1", (size_t)nunits1 * tpu1, &l1));
+    RCK(buf("tb.hi1", (size_t)nunits1 * tpu1, &h1));
+    RCK(buf("tb.lo2", (size_t)nunits2 * tpu2, &l2));
+    RCK(buf("tb.hi2", (size_t)nunits2 * tpu2, &h2));
+    HCK(launch_tilebox<T>(C1, nullptr, nunits1, w1, 0, l1, h1, stream));
+    HCK(launch_tilebox<T>(C2, nullptr, nunits2, w2, 0, l2, h2, stream));
+    lo1 = l1;
+    hi1 = h1;
+    lo2 = l2;
+    hi2 = h2;
+  }
   const unsigned long long ntp = (unsigned long long)nslots * tpu1 * tpu2;
   if (ntp > 0xffffffffull) {
     set_err("too many tile pairs (%llu)", ntp);
@@ -252,6 +258,7 @@ int jsi_ctx::dist_topk(int nslots, const int* su1, const int* su2, int w1, int n
   // pairs (best side-2 tile per side-1 tile, ranked by upper bound) bounds the
   // global k-th d2 from above
   const long long npos = (long long)nslots * tpu1;
+  bool counters_clear = false;
   const int pairs_tp = std::min(w1 * w1, kTileCells) * std::min(w2 * w2, kTileCells);
   // round 1 (one slot of many tiles): up to 256 tile pairs, so the seeds
   // cover most regions that hold top-k pairs; refinement rounds (slots in
@@ -265,9 +272,12 @@ int jsi_ctx::dist_topk(int nslots, const int* su1, const int* su2, int w1, int n
     RCK(buf_raw("d.seedwork", seed_work_bytes(), reinterpret_cast<void**>(&work), 0));
     const int ordered = (nslots > 1 && tpu1 == 1 && tpu2 == 1) ? 1 : 0;
     HCK(launch_dist_seed<T>(nslots, su1, su2, w1, w2, lo1, hi1, lo2, hi2, k, bestv, stream));
+    // the seed's last kernel also clears the screen's counters (words 0-5 of
+    // d.small: list count, unused, candidate count)
     HCK(launch_seed_thresh<T>(bestv, (int)npos, tpu1, nseed, ordered, list, list_count, range, work, su1, su2, w1,
-                              w2, C1, C2, k, fma, thresh, stream));
+                              w2, C1, C2, k, fma, thresh, stream, list_count, 6));
     RCK(widen());
+    counters_clear = true;
   } else {
     HCK(launch_set_thresh<T>(thresh, cand, 0, 1, stream));
   }
@@ -282,10 +292,11 @@ int jsi_ctx::dist_topk(int nslots, const int* su1, const int* su2, int w1, int n
   const unsigned long long maxkey = rerank ? 0x7ff0000000000000ull : maxkey_t;
   unsigned long long n = 0, m = 0;
   for (int attempt = 0;; ++attempt) {
-    HCK(launch_fill_u32(list_count, 0u, 1, stream));
+    if (!counters_clear) HCK(launch_fill_u32(list_count, 0u, 1, stream));
     HCK(launch_dist_list<T>(nslots, su1, su2, w1, w2, lo1, hi1, lo2, hi2, thresh, cull, list, list_count, stream,
                             fma));
-    HCK(launch_fill_u32(reinterpret_cast<unsigned*>(out_count), 0u, 2, stream));
+    if (!counters_clear) HCK(launch_fill_u32(reinterpret_cast<unsigned*>(out_count), 0u, 2, stream));
+    counters_clear = false;  // a repeated screen clears them itself
     unsigned long long* pairs_done = prof ? reinterpret_cast<unsigned long long*>(list_count + 8) : nullptr;
     if (prof) {
       HCK(launch_fill_u32(reinterpret_cast<unsigned*>(pairs_done), 0u, 2, stream));
@@ -362,10 +373,12 @@ int jsi_ctx::dist_topk(int nslots, const int* su1, const int* su2, int w1, int n
     HCK(launch_fill_sentinel(cand, n, n2, stream));
     HCK(sort_cands(cand, n2, stream));
     rec = cand;
+    if (last_out && n > 0)
+      HCK(launch_copy_cand(last_out, cand + (std::min<unsigned long long>(n, (unsigned long long)k) - 1), stream));
   } else {
     Cand* sorted;
     RCK(buf("d.sorted", (size_t)k, &sorted));
-    HCK(rank_sort(small, small_count, small_cap, (unsigned)k, sorted, stream));
+    HCK(rank_sort(small, small_count, small_cap, (unsigned)k, sorted, stream, last_out));
     rec = sorted;
   }
   *rec_out = rec;
@@ -419,8 +432,6 @@ int jsi_ctx::min_distance(int h1, const jsi_region& r1, int h2, const jsi_region
   V *reg1, *reg2;
   RCK(buf(nm("reg", 1, cur).c_str(), 1, &reg1));
   RCK(buf(nm("reg", 2, cur).c_str(), 1, &reg2));
-  HCK(launch_store_vec4<T>(reg1, g1, stream));
-  HCK(launch_store_vec4<T>(reg2, g2, stream));
   int nuv1 = r1.nuv, nuv2 = r2.nuv;
   int nunits1 = 1, nunits2 = 1;  // upper bounds (exact counts of later rounds stay on the device)
   const int* nunits_cur = nullptr;
@@ -438,28 +449,53 @@ int jsi_ctx::min_distance(int h1, const jsi_region& r1, int h2, const jsi_region
   // screening of round 1 it also writes the centres rounded to float
   float4 *f32c1 = nullptr, *f32c2 = nullptr;
   unsigned long long* amax = nullptr;
+  int* su1 = nullptr;
+  int* su2 = nullptr;
+  TileBoxes<V> tb;          // tile boxes of the current round's centres (written by the evaluation)
+  TileBoxes<float4> tb32;   // and of their float copies (FP32 screening of round 1)
+  auto tiles_of = [](int w) {
+    const int tw = (w + kTile - 1) / kTile;
+    return tw * tw;
+  };
   const bool dbl = std::is_same<T, double>::value;
   const bool s32_first = dbl && P.precision == JSI_PREC_FP32SCREEN;
   {
     RCK(buf(nm("C", 1, cur).c_str(), (size_t)(nuv1 - 1) * (nuv1 - 1), &C1));
     RCK(buf(nm("C", 2, cur).c_str(), (size_t)(nuv2 - 1) * (nuv2 - 1), &C2));
-    if (dbl) {
-      RCK(buf("d.amax", kAmaxSlots, &amax));
-      HCK(launch_fill_u32(reinterpret_cast<unsigned*>(amax), 0u, 2 * kAmaxSlots, stream));
-    }
+    if (dbl) RCK(buf("d.amax", kAmaxSlots, &amax));
+    RCK(buf(nm("su", 1, cur).c_str(), 1, &su1));
+    RCK(buf(nm("su", 2, cur).c_str(), 1, &su2));
+    // regions, slot 0 -> unit 0 and the cleared max |coordinate| slots in one launch
+    HCK(launch_query_init<T>(reg1, g1, reg2, g2, su1, su2, amax, stream));
+    EcOut<T> o1, o2;
+    o1.C = C1;
+    o2.C = C2;
+    RCK(buf(nm("tbl", 1, cur).c_str(), (size_t)tiles_of(nuv1 - 1), &o1.lo));
+    RCK(buf(nm("tbh", 1, cur).c_str(), (size_t)tiles_of(nuv1 - 1), &o1.hi));
+    RCK(buf(nm("tbl", 2, cur).c_str(), (size_t)tiles_of(nuv2 - 1), &o2.lo));
+    RCK(buf(nm("tbh", 2, cur).c_str(), (size_t)tiles_of(nuv2 - 1), &o2.hi));
+    tb.lo1 = o1.lo;
+    tb.hi1 = o1.hi;
+    tb.lo2 = o2.lo;
+    tb.hi2 = o2.hi;
     if (s32_first) {
       RCK(buf("s32.C1", (size_t)(nuv1 - 1) * (nuv1 - 1), &f32c1));
       RCK(buf("s32.C2", (size_t)(nuv2 - 1) * (nuv2 - 1), &f32c2));
+      RCK(buf("s32.tbl1", (size_t)tiles_of(nuv1 - 1), &o1.lo32));
+      RCK(buf("s32.tbh1", (size_t)tiles_of(nuv1 - 1), &o1.hi32));
+      RCK(buf("s32.tbl2", (size_t)tiles_of(nuv2 - 1), &o2.lo32));
+      RCK(buf("s32.tbh2", (size_t)tiles_of(nuv2 - 1), &o2.hi32));
+      o1.C32 = f32c1;
+      o2.C32 = f32c2;
+      tb32.lo1 = o1.lo32;
+      tb32.hi1 = o1.hi32;
+      tb32.lo2 = o2.lo32;
+      tb32.hi2 = o2.hi32;
     }
-    RCK(eval_centres<T>(h1, reg1, 1, nuv1, C1, f32c1, amax));
-    RCK(eval_centres<T>(h2, reg2, 1, nuv2, C2, f32c2, amax));
+    o1.absmax = o2.absmax = amax;
+    RCK(eval_centres<T>(h1, reg1, 1, nuv1, o1));
+    RCK(eval_centres<T>(h2, reg2, 1, nuv2, o2));
   }
-  int* su1;
-  int* su2;
-  RCK(buf(nm("su", 1, cur).c_str(), 1, &su1));
-  RCK(buf(nm("su", 2, cur).c_str(), 1, &su2));
-  HCK(launch_fill_i32(su1, 0, 1, stream));
-  HCK(launch_fill_i32(su2, 0, 1, stream));
   if (prof) {
     HCK(hipStreamSynchronize(stream));
     res->ms_eval += now_ms() - te;
@@ -484,7 +520,8 @@ int jsi_ctx::min_distance(int h1, const jsi_region& r1, int h2, const jsi_region
         rr.C2 = C2;
         rr.absmax = amax;
         const int rc = dist_topk<float>(nslots, su1, su2, w1, nunits1, f32c1, w2, nunits2, f32c2, P.k, P.cull, &rec,
-                                        &n, &kk, rr, nullptr, (unsigned long long)P.cand_cap);
+                                        &n, &kk, rr, nullptr, (unsigned long long)P.cand_cap, tb32,
+                                        lastrec + (round - 1));
         if (rc == JSI_ERR_CAPACITY)
           s32 = 0;
         else if (rc != JSI_OK)
@@ -492,18 +529,17 @@ int jsi_ctx::min_distance(int h1, const jsi_region& r1, int h2, const jsi_region
       }
       if (!s32) {
         RCK(dist_topk<T>(nslots, su1, su2, w1, nunits1, C1, w2, nunits2, C2, P.k, P.cull, &rec, &n, &kk, Rerank(),
-                         amax, (unsigned long long)P.cand_cap));
+                         amax, (unsigned long long)P.cand_cap, tb, lastrec + (round - 1)));
       }
     } else {
       RCK(dist_topk<T>(nslots, su1, su2, w1, nunits1, C1, w2, nunits2, C2, P.k, P.cull, &rec, &n, &kk, Rerank(),
-                       nullptr, (unsigned long long)P.cand_cap));
+                       nullptr, (unsigned long long)P.cand_cap, tb, lastrec + (round - 1)));
     }
     (void)nunits_cur;
     jsi_round_info& ri = res->round[round - 1];
     ri.npairs = kk;
     ri.slots = nslots;
     ri.nuv = nuv1;
-    if (kk > 0) HCK(launch_copy_cand(lastrec + (round - 1), rec + (kk - 1), stream));
     if (prof) res->ms_screen += now_ms() - ts;
     if (P.trace && kk > 0) {
       host_rec.resize(kk);
@@ -537,7 +573,7 @@ int jsi_ctx::min_distance(int h1, const jsi_region& r1, int h2, const jsi_region
     unsigned char* dwork;
     RCK(buf_raw("d.dedupwork", dedup_work_bytes(), reinterpret_cast<void**>(&dwork), 0));
     HCK(launch_dist_dedup<T>(rec, (int)kk, w1 * w1, w2 * w2, su1, su2, reg1, reg2, nuv1, nuv2, fp1, fp2, nsu1, nsu2,
-                             nreg1, nreg2, nunits_dev, dwork, stream));
+                             nreg1, nreg2, nunits_dev, dwork, stream, amax, dbl ? kAmaxSlots : 0));
     // at most kk units per side; the exact counts stay on the device and the
     // evaluation skips the tiles beyond them
     const int nu_max = (int)kk;
@@ -545,9 +581,22 @@ int jsi_ctx::min_distance(int h1, const jsi_region& r1, int h2, const jsi_region
     const int wr = nref - 1;
     RCK(buf(nm("C", 1, nxt).c_str(), (size_t)nu_max * wr * wr, &nC1));
     RCK(buf(nm("C", 2, nxt).c_str(), (size_t)nu_max * wr * wr, &nC2));
-    if (dbl) HCK(launch_fill_u32(reinterpret_cast<unsigned*>(amax), 0u, 2 * kAmaxSlots, stream));
-    RCK(eval_centres<T>(h1, nreg1, nu_max, nref, nC1, nullptr, amax, nunits_dev));
-    RCK(eval_centres<T>(h2, nreg2, nu_max, nref, nC2, nullptr, amax, nunits_dev + 1));
+    EcOut<T> o1, o2;
+    o1.C = nC1;
+    o2.C = nC2;
+    RCK(buf(nm("tbl", 1, nxt).c_str(), (size_t)nu_max, &o1.lo));  // one tile per refinement unit
+    RCK(buf(nm("tbh", 1, nxt).c_str(), (size_t)nu_max, &o1.hi));
+    RCK(buf(nm("tbl", 2, nxt).c_str(), (size_t)nu_max, &o2.lo));
+    RCK(buf(nm("tbh", 2, nxt).c_str(), (size_t)nu_max, &o2.hi));
+    o1.absmax = o2.absmax = amax;
+    o1.nunits_dev = nunits_dev;
+    o2.nunits_dev = nunits_dev + 1;
+    RCK(eval_centres<T>(h1, nreg1, nu_max, nref, o1));
+    RCK(eval_centres<T>(h2, nreg2, nu_max, nref, o2));
+    tb.lo1 = o1.lo;
+    tb.hi1 = o1.hi;
+    tb.lo2 = o2.lo;
+    tb.hi2 = o2.hi;
     if (prof) {
       HCK(hipStreamSynchronize(stream));
       res->ms_eval += now_ms() - te;
@@ -1127,4 +1176,4 @@ int jsi_intersect(jsi_ctx* c, int h1, const jsi_region* r1, int h2, const jsi_re
 // used by the operator ABI (jsi_opabi.hip)
 template int jsi_ctx::dist_topk<float>(int, const int*, const int*, int, int, const float4*, int, int, const float4*,
                                        int, int, Cand**, long long*, long long*, Rerank, const unsigned long long*,
-                                       unsigned long long);
+                                       unsigned long long, TileBoxes<float4>, Cand*);

@@ -44,6 +44,15 @@ struct RerankSpec {
   const unsigned long long* absmax = nullptr;
 };
 
+// Per-tile boxes of both sides' centres, when the evaluation already wrote them.
+template <class V>
+struct TileBoxes {
+  const V* lo1 = nullptr;
+  const V* hi1 = nullptr;
+  const V* lo2 = nullptr;
+  const V* hi2 = nullptr;
+};
+
 struct KernelStat {
   long long launches = 0;
   double ms = 0.0;     // summed launch durations (HIP events on the context stream)
@@ -84,13 +93,13 @@ struct jsi_ctx {
                  unsigned long long* kmax, const char* tag);
   using Rerank = RerankSpec;
   template <class T>
-  int eval_centres(int h, const jsi::vec4_t<T>* reg, int nunits, int nuv, jsi::vec4_t<T>* C, float4* C32 = nullptr,
-                   unsigned long long* absmax = nullptr, const int* nunits_dev = nullptr);
+  int eval_centres(int h, const jsi::vec4_t<T>* reg, int nunits, int nuv, const jsi::EcOut<T>& o);
   template <class T>
   int dist_topk(int nslots, const int* su1, const int* su2, int w1, int nunits1, const jsi::vec4_t<T>* C1, int w2,
                 int nunits2, const jsi::vec4_t<T>* C2, int k, int cull, jsi::Cand** rec_out, long long* n_out,
                 long long* kk_out, Rerank rr = Rerank(), const unsigned long long* exp_absmax = nullptr,
-                unsigned long long cand_cap = 0);
+                unsigned long long cand_cap = 0, TileBoxes<jsi::vec4_t<T>> tbx = TileBoxes<jsi::vec4_t<T>>(),
+                jsi::Cand* last_out = nullptr);
   template <class T>
   int min_distance(int h1, const jsi_region& r1, int h2, const jsi_region& r2, const jsi_params& P,
                    jsi_result* res, jsi_pair* out, int cap_pairs);
