@@ -20,7 +20,66 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "queries/s + p99 latency at nuv=512, batched vs plain GPU, dist err ≤1e-9"
-PEAK = {"f32": 157.3, "f64": 78.6}  # TFLOP/s, MI355X published vector/matrix peaks
+PEAK = {"f32": 157.3, "f64": 78.6}  # TFLOP/s, MI355X published peaks (FP64: vector = matrix)
+FILTERS = {  # kernel statistics kept by the library -> kernel name
+    "dist_filter_tiles_f32": "k_dist_filter<float> (tile pairs)",
+    "dist_filter_tiles_f64": "k_dist_filter<double> (tile pairs)",
+    "dist_filter_units_f32": "k_dist_filter_units<float> (one tile per unit)",
+    "dist_filter_units_f64": "k_dist_filter_units<double> (one tile per unit)",
+}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_k_dist_filter_units_hardwave_nuv512_k4096.json")
+
+
+def hip_copy_calibration(mbytes, repeats=3):
+    """Plain device-to-device copies of a known size (the calibration kernel of
+    the FETCH_SIZE / WRITE_SIZE counters)."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    n = int(mbytes) << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(n)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(n)) == 0
+    for _ in range(repeats):
+        assert hip.hipMemcpy(b, a, C.c_size_t(n), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
+def pmc_summary(fetch_dir, write_dir, mbytes, kernel, out_path):
+    """HBM bytes per launch of `kernel` from a FETCH_SIZE run and a WRITE_SIZE run
+    (rocprofv3 --pmc, KiB per dispatch), each scaled by what the same counter
+    reads for the plain copy of `mbytes` MiB in the same run."""
+    import csv
+    import glob
+
+    def per_dispatch(d, counter):
+        rows = []
+        for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+            rows += [r for r in csv.DictReader(open(f)) if r["Counter_Name"] == counter]
+        got = {}
+        for r in rows:
+            got.setdefault((r["Kernel_Name"], r["Dispatch_Id"]), 0.0)
+            got[(r["Kernel_Name"], r["Dispatch_Id"])] += float(r["Counter_Value"])
+        kern = [v for (n, _), v in got.items() if kernel in n]
+        # the calibration copies are the largest copyBuffer dispatches (the
+        # library's own small device -> host reads use the same kernel)
+        copy = sorted(v for (n, _), v in got.items() if "copyBuffer" in n)[-3:]
+        return float(np.mean(kern)), float(np.mean(copy)), len(kern), len(copy)
+
+    fk, fc, nk, nc = per_dispatch(fetch_dir, "FETCH_SIZE")
+    wk, wc, _, _ = per_dispatch(write_dir, "WRITE_SIZE")
+    nbytes = float(int(mbytes) << 20)
+    ff, fw = nbytes / (fc * 1024.0), nbytes / (wc * 1024.0)  # true bytes per counted byte
+    out = {"kernel": kernel, "workload": "hard-wave pair, nuv=512, k=4096, FP32SCREEN (refinement rounds)",
+           "fetch_kib_per_launch": fk, "write_kib_per_launch": wk,
+           "copy_bytes": nbytes, "copy_fetch_kib": fc, "copy_write_kib": wc,
+           "factor_fetch": ff, "factor_write": fw,
+           "bytes_per_launch": fk * 1024.0 * ff + wk * 1024.0 * fw,
+           "launches": nk, "copy_launches": nc}
+    with open(out_path, "w") as f:
+        json.dump(out, f, indent=1)
+    print(json.dumps(out))
 
 
 def pct(xs, q):
@@ -99,7 +158,15 @@ def main():
     ap.add_argument("--cpu-queries", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--concurrency", type=int, default=8, help="worker streams of the served (batched) form")
+    ap.add_argument("--calib-copy-mb", type=int, default=0,
+                    help="also run plain device copies of this many MiB (calibrates the traffic counters)")
+    ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="write the traffic file from two rocprofv3 --pmc runs and exit")
     args = ap.parse_args()
+    if args.pmc_summary:
+        pmc_summary(args.pmc_summary[0], args.pmc_summary[1], args.calib_copy_mb or 256,
+                    "k_dist_filter_units<double", TRAFFIC_FILE)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each GPU would serve its own "
               "queries); run with --gpus 1", file=sys.stderr)
@@ -114,11 +181,14 @@ def main():
     ctx = Context(0)
     ctx.upload(a)
     ctx.upload(b)
+    if args.calib_copy_mb:
+        hip_copy_calibration(args.calib_copy_mb)
 
     # chosen form: FP32 screening with FP64 re-ranking, tile culling
     p = default_params(k=args.k, precision=args.precision, cull=1)
     ares, awall, alat = run_form(ctx, a, r1, b, r2, p, args.steps, args.warmup, profile=True)
-    stats = {name: ctx.kernel_stats(name) for name in ("dist_filter_f32", "dist_filter_f64")}
+    stats = {name: ctx.kernel_stats(name) for name in FILTERS}
+    stats = {n: v for n, v in stats.items() if v[0] > 0}
     ctx.profile(False)
     # served / batched: the same queries through the serving loop, several in flight
     res, wall, lat = run_served(ctx, ctx.query(0, a, r1, b, r2, p), args.steps, args.concurrency, 1)
@@ -126,21 +196,26 @@ def main():
     # drop-in EXACT32 mode (bit-exact with the reference, FP32 end to end)
     pe = default_params(k=args.k, precision="exact32", cull=1)
     eres, ewall, elat = run_form(ctx, a, r1, b, r2, pe, max(1, args.steps // 2), 1)
-    # plain FP64 GPU path: FP64 everywhere, one query at a time (the drop-in FP64 form)
-    pp = default_params(k=args.k, precision="fp64", cull=1)
+    # plain FP64 GPU path (the drop-in form the target is quoted against): FP64
+    # everywhere, every cell pair of every round evaluated, one query at a time
+    pp = default_params(k=args.k, precision="fp64", cull=0)
     pres, pwall, plat = run_form(ctx, a, r1, b, r2, pp, args.plain_steps, 1)
     pqps = args.plain_steps / pwall
-    # the same FP64 path with every cell pair evaluated (no tile or chunk culling)
-    xp = default_params(k=args.k, precision="fp64", cull=0)
-    xres, xwall, xlat = run_form(ctx, a, r1, b, r2, xp, 2, 1)
+    # the same FP64 path with this build's conservative tile culling (identical results)
+    cp = default_params(k=args.k, precision="fp64", cull=1)
+    cres, cwall, clat = run_form(ctx, a, r1, b, r2, cp, args.plain_steps, 1)
+    cqps = args.plain_steps / cwall
     dist_err = abs(res.distance - pres.distance)
 
     # dominant screen kernel of the chosen form (most summed time), timed with HIP events
     dom = max(stats, key=lambda n: stats[n][1])
     launches, kms, pairs = stats[dom]
     dt = "f32" if dom.endswith("f32") else "f64"
-    flops = 8.0 * pairs  # 3 sub + 3 mul + 2 add per cell pair
+    flops = 8.0 * pairs  # algorithmic: 3 sub + 3 mul + 2 add per cell pair
     achieved = flops / (kms * 1e-3) / 1e12 if kms > 0 else 0.0
+    traffic = None
+    if dom == "dist_filter_units_f64" and os.path.exists(TRAFFIC_FILE):
+        traffic = json.load(open(TRAFFIC_FILE))["bytes_per_launch"]  # rocprofv3 --pmc, separate runs
     cells = (args.nuv - 1) ** 2 * 2 + sum(
         (res.round[r].units1 + res.round[r].units2) * 15 * 15 for r in range(res.rounds))
     out = {
@@ -166,17 +241,22 @@ def main():
         "cells_per_s": cells * qps,
         "distance": res.distance,
         "dist_err_vs_fp64": dist_err,
-        "plain_fp64": {"queries_per_s": pqps, "p50_ms": pct(plat, 50), "p99_ms": pct(plat, 99),
+        "plain_fp64": {"form": "FP64, every cell pair evaluated, one query at a time",
+                       "queries_per_s": pqps, "p50_ms": pct(plat, 50), "p99_ms": pct(plat, 99),
                        "distance": pres.distance},
         "speedup_vs_plain": qps / pqps if pqps > 0 else None,
-        "exhaustive_fp64": {"queries_per_s": 2 / xwall, "p50_ms": pct(xlat, 50), "distance": xres.distance},
+        "fp64_culled": {"form": "FP64 with tile culling, one query at a time",
+                        "queries_per_s": cqps, "p50_ms": pct(clat, 50), "p99_ms": pct(clat, 99),
+                        "distance": cres.distance},
+        "speedup_vs_fp64_culled": qps / cqps if cqps > 0 else None,
         "exact32": {"queries_per_s": max(1, args.steps // 2) / ewall, "p50_ms": pct(elat, 50),
                     "p99_ms": pct(elat, 99), "distance": eres.distance},
-        "roofline": {"bound": "mfma", "kernel": f"k_dist_filter ({dom})",
+        "roofline": {"bound": "mfma", "kernel": FILTERS[dom],
                      "achieved": achieved, "peak": PEAK[dt], "unit": "TFLOP/s",
-                     "frac": achieved / PEAK[dt], "traffic": None,
+                     "frac": achieved / PEAK[dt], "traffic": traffic,
                      "launches": launches, "avg_launch_ms": kms / max(launches, 1),
-                     "pairs_per_launch": pairs / max(launches, 1)},
+                     "pairs_per_launch": pairs / max(launches, 1),
+                     "flop_per_pair": 8},
         "screen_kernels": {n: {"launches": v[0], "ms": v[1], "pairs": v[2]} for n, v in stats.items()},
     }
     if not args.no_cpu:

@@ -141,11 +141,9 @@ hipError_t launch_dist_filter(const uint64_t* list, const unsigned* list_count, 
 template <class T>
 hipError_t launch_set_thresh(T* thresh, const Cand* rec, int idx, int use_inf, hipStream_t s);
 
-// FP32SCREEN helpers: round FP64 centres to float (and max-reduce
-// |coordinate|), widen a float threshold by the rounding bound, and replace
+// FP32SCREEN helpers (the float centres and max |coordinate| come from the
+// evaluation): widen a float threshold by the rounding bound, and replace
 // candidate keys by their FP64 d2.
-hipError_t launch_narrow(const double4* src, float4* dst, long long n, unsigned long long* absmax_bits,
-                         hipStream_t s);
 hipError_t launch_inflate(float* thresh, const unsigned long long* absmax_bits, hipStream_t s);
 hipError_t launch_rerank(Cand* cand, const unsigned long long* n_dev, unsigned long long cap, const int* su1,
                          const int* su2, int cells1, int cells2, const double4* C1, const double4* C2, hipStream_t s);
@@ -175,8 +173,6 @@ hipError_t select_topk(const Cand* cand, unsigned long long n, unsigned long lon
 hipError_t launch_thresh_from_state_ru(float* thresh, const SelState* st, hipStream_t s);
 hipError_t launch_set_thresh_ru(float* thresh, const Cand* rec, int idx, hipStream_t s);
 hipError_t launch_copy_cand(Cand* dst, const Cand* src, hipStream_t s);
-template <class T>
-hipError_t launch_store_vec4(vec4_t<T>* dst, vec4_t<T> v, hipStream_t s);
 template <class T>
 hipError_t launch_query_init(vec4_t<T>* reg1, vec4_t<T> g1, vec4_t<T>* reg2, vec4_t<T> g2, int* su1, int* su2,
                              unsigned long long* amax, hipStream_t s);

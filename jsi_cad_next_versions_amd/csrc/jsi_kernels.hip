@@ -1419,35 +1419,6 @@ __global__ __launch_bounds__(256) void k_dist_filter_units(const uint64_t* list,
 // pair the FP64 ranking can select, and the candidates are re-ranked with
 // their FP64 d2.
 // ---------------------------------------------------------------------------
-__global__ void k_narrow(const double4* __restrict__ src, float4* __restrict__ dst, long long n,
-                         unsigned long long* absmax_bits) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  double m = 0.0;
-  if (i < n) {
-    const double4 c = src[i];
-    float4 f;
-    f.x = __double2float_rn(c.x);
-    f.y = __double2float_rn(c.y);
-    f.z = __double2float_rn(c.z);
-    f.w = 0.f;
-    dst[i] = f;
-    m = fmax(fabs(c.x), fmax(fabs(c.y), fabs(c.z)));
-  }
-  unsigned long long b = (unsigned long long)__double_as_longlong(m);
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    const unsigned long long o = __shfl_xor(b, off);
-    b = o > b ? o : b;
-  }
-  __shared__ unsigned long long red[4];
-  if (lane_id() == 0) red[threadIdx.x >> 6] = b;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) b = red[w] > b ? red[w] : b;
-    if (b) atomicMax(absmax_bits, b);
-  }
-}
-
 // T <- (sqrt(T)(1+16u) + 4 sqrt(3) E (1+16u))^2 (1+16u) + 1e-36, u = 2^-24,
 // E = 2^-24 max|centre coordinate| (the largest of the kAmaxSlots partial
 // maxima): one widening covers (a) the float
@@ -1781,14 +1752,11 @@ __global__ void k_query_init(vec4_t<T>* reg1, vec4_t<T> g1, vec4_t<T>* reg2, vec
   }
 }
 
-// Small device-side stores (no host round trip): one record, one region.
+// Small device-side store (no host round trip): one record.
 __global__ void k_copy_cand(Cand* dst, const Cand* src) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *dst = *src;
 }
-template <class T>
-__global__ void k_store_vec4(vec4_t<T>* dst, vec4_t<T> v) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *dst = v;
-}
+
 
 // ---------------------------------------------------------------------------
 // Bitonic sort of candidate records (power-of-two length, >= kSortBlock).
@@ -2512,13 +2480,6 @@ hipError_t launch_set_thresh(T* thresh, const Cand* rec, int idx, int use_inf, h
   return hipGetLastError();
 }
 
-hipError_t launch_narrow(const double4* src, float4* dst, long long n, unsigned long long* absmax_bits,
-                         hipStream_t s) {
-  if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_narrow, dim3(blocks_for(n, 256)), dim3(256), 0, s, src, dst, n, absmax_bits);
-  return hipGetLastError();
-}
-
 hipError_t launch_inflate(float* thresh, const unsigned long long* absmax_bits, hipStream_t s) {
   hipLaunchKernelGGL(k_inflate, dim3(1), dim3(64), 0, s, thresh, absmax_bits);
   return hipGetLastError();
@@ -2589,13 +2550,6 @@ hipError_t launch_copy_cand(Cand* dst, const Cand* src, hipStream_t s) {
   return hipGetLastError();
 }
 
-template <class T>
-hipError_t launch_store_vec4(vec4_t<T>* dst, vec4_t<T> v, hipStream_t s) {
-  hipLaunchKernelGGL(k_store_vec4<T>, dim3(1), dim3(64), 0, s, dst, v);
-  return hipGetLastError();
-}
-template hipError_t launch_store_vec4<float>(float4*, float4, hipStream_t);
-template hipError_t launch_store_vec4<double>(double4*, double4, hipStream_t);
 
 template <class T>
 hipError_t launch_query_init(vec4_t<T>* reg1, vec4_t<T> g1, vec4_t<T>* reg2, vec4_t<T> g2, int* su1, int* su2,

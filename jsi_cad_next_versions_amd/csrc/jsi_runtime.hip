@@ -314,7 +314,11 @@ int jsi_ctx::dist_topk(int nslots, const int* su1, const int* su2, int w1, int n
     if (prof) {
       float ms = 0.f;
       HCK(hipEventElapsedTime(&ms, ev[0], ev[1]));
-      KernelStat& kst = stats[std::is_same<T, float>::value ? "dist_filter_f32" : "dist_filter_f64"];
+      // k_dist_filter_units runs when every unit is a single tile (refinement rounds)
+      const bool units = w1 <= kTile && w2 <= kTile;
+      std::string key = std::string(units ? "dist_filter_units_" : "dist_filter_tiles_") +
+                        (std::is_same<T, float>::value ? "f32" : "f64");
+      KernelStat& kst = stats[key];
       kst.launches += 1;
       kst.ms += ms;
       kst.units += (double)cnt[2];
@@ -1042,8 +1046,10 @@ int jsi_ctx_profile(jsi_ctx* c, int enable) {
   if (!c) return JSI_ERR_INVALID;
   HCK(hipSetDevice(c->device));
   if (enable && !c->ev[0]) {
-    HCK(hipEventCreate(&c->ev[0]));
-    HCK(hipEventCreate(&c->ev[1]));
+    // device-scope release: no system-wide cache writeback inside the timed
+    // interval, so the pair brackets the kernel itself
+    HCK(hipEventCreateWithFlags(&c->ev[0], hipEventReleaseToDevice));
+    HCK(hipEventCreateWithFlags(&c->ev[1], hipEventReleaseToDevice));
   }
   c->prof = enable ? 1 : 0;
   c->stats.clear();
