@@ -71,7 +71,12 @@ typedef struct jsi_params {
   int cand_cap;          /* distance screen candidate capacity, 0 = default (4 Mi records), at least
                             2k is used; a smaller value exercises the overflow path (threshold
                             tightening and a repeated screen), results unchanged */
-  int reserved[4];
+  int polish;            /* distance: FP64 closest-point polish of the first `polish` records of the
+                            final top-k list (0 = off, at most JSI_MAX_K); fills jsi_result.closest */
+  int reserved_;
+  double param_tol;      /* distance, FP64 / FP32SCREEN only: > 0 replaces dist_max_rounds by the
+                            number of rounds after which a cell spans <= param_tol in u and v on both
+                            surfaces (at most JSI_MAX_ROUNDS); 0 = off */
 } jsi_params;
 
 typedef struct jsi_round_info {
@@ -85,6 +90,26 @@ typedef struct jsi_round_info {
   int pad_;
 } jsi_round_info;
 
+/* FP64 closest points of the two surfaces (jsi_params.polish > 0).  Each
+ * polished record starts at the parameter centres of its two cells and runs
+ * a damped Newton iteration on |S1(u1,v1) - S2(u2,v2)|^2 inside the query
+ * regions; the record with the smallest result wins (lowest rank on ties).
+ * No counterpart in the reference, whose result is the k-th centre distance. */
+typedef struct jsi_closest {
+  double distance;    /* |S1(u1,v1) - S2(u2,v2)| */
+  double u1, v1, u2, v2;
+  double p1[3];       /* S1(u1, v1) */
+  double p2[3];       /* S2(u2, v2) */
+  int32_t candidate;  /* rank of the winning record in the final top-k list */
+  int32_t iterations; /* iterations of the winner (accepted and rejected steps) */
+  int32_t polished;   /* records polished (0 = polish off) */
+  int32_t flags;      /* JSI_POLISH_* termination of the winner */
+} jsi_closest;
+#define JSI_POLISH_STATIONARY 1 /* gradient exactly zero */
+#define JSI_POLISH_STALLED 2    /* damping exceeded its cap without a decrease */
+#define JSI_POLISH_FIXED 4      /* the projected step left the parameters unchanged */
+#define JSI_POLISH_CONVERGED 8  /* accepted step <= 1e-15 (1 + max |parameter|) */
+
 typedef struct jsi_result {
   double distance;       /* distance: sqrt(k-th smallest centre d2) of the last round */
   int64_t npairs;        /* pairs of the last round */
@@ -94,6 +119,7 @@ typedef struct jsi_result {
   double ms_eval;        /* evaluation + AABB construction */
   double ms_screen;      /* screening + top-k selection */
   jsi_round_info round[JSI_MAX_ROUNDS];
+  jsi_closest closest;   /* distance with jsi_params.polish > 0 */
 } jsi_result;
 
 /* One candidate pair: slot and cell ids as in DistPairRef / IntersectPairRef
@@ -154,6 +180,30 @@ int jsi_min_distance(jsi_ctx* ctx, int h1, const jsi_region* r1, int h2, const j
  * pairs_out (optional, host) receives the last round's ordered pairs. */
 int jsi_intersect(jsi_ctx* ctx, int h1, const jsi_region* r1, int h2, const jsi_region* r2,
                   const jsi_params* params, jsi_result* result, jsi_pair* pairs_out, int pairs_cap);
+
+/* One sample of an intersection curve (jsi_intersect_curves). */
+typedef struct jsi_curve_point {
+  double x, y, z;        /* S1(u1, v1) */
+  double u1, v1, u2, v2; /* parameters on the two surfaces */
+  double residual;       /* |S1(u1,v1) - S2(u2,v2)| */
+  int32_t curve;         /* curve id, 0-based, in order of creation */
+  int32_t flags;         /* JSI_CURVE_CLOSED when the curve is closed */
+} jsi_curve_point;
+#define JSI_CURVE_CLOSED 1
+
+/* Intersection curves: the run_intersect rounds of jsi_intersect, then every
+ * pair of the last round is projected onto the intersection curve in FP64
+ * (Gauss-Newton on S1(u1,v1) - S2(u2,v2) = 0 from the centres of its two
+ * cells, inside the query regions); projections with residual <= tol
+ * (0 = 1e-9) are thinned to a spacing of `spacing` in 3-space (<= 0 = 1e-3 of
+ * their bounding-box diagonal), linked within 3 x spacing and ordered along
+ * each curve by its tangent.  pts (host) receives min(cap, *npts) samples,
+ * curve by curve in walk order; *npts and *ncurves receive the totals.  No
+ * counterpart in the reference, whose intersection() only prints pair counts
+ * (JSI_CAD_NEXT_VERSION5/src/intersection/intersection.cu:367). */
+int jsi_intersect_curves(jsi_ctx* ctx, int h1, const jsi_region* r1, int h2, const jsi_region* r2,
+                         const jsi_params* params, double spacing, double tol, jsi_result* result,
+                         jsi_curve_point* pts, int cap, int* npts, int* ncurves);
 
 /* ---- batches and serving ------------------------------------------------ */
 #define JSI_QUERY_DISTANCE 0

@@ -49,7 +49,9 @@ class Params(C.Structure):
         ("cull", C.c_int),
         ("trace", C.c_int),
         ("cand_cap", C.c_int),
-        ("reserved", C.c_int * 4),
+        ("polish", C.c_int),
+        ("reserved_", C.c_int),
+        ("param_tol", C.c_double),
     ]
 
 
@@ -66,6 +68,34 @@ class RoundInfo(C.Structure):
     ]
 
 
+class Closest(C.Structure):
+    """jsi_closest: FP64 closest points after the polish (jsi_params.polish > 0)."""
+    _fields_ = [
+        ("distance", C.c_double),
+        ("u1", C.c_double),
+        ("v1", C.c_double),
+        ("u2", C.c_double),
+        ("v2", C.c_double),
+        ("p1", C.c_double * 3),
+        ("p2", C.c_double * 3),
+        ("candidate", C.c_int32),
+        ("iterations", C.c_int32),
+        ("polished", C.c_int32),
+        ("flags", C.c_int32),
+    ]
+
+    def as_dict(self):
+        return dict(distance=self.distance, u1=self.u1, v1=self.v1, u2=self.u2, v2=self.v2, p1=tuple(self.p1),
+                    p2=tuple(self.p2), candidate=self.candidate, iterations=self.iterations,
+                    polished=self.polished, flags=self.flags)
+
+
+POLISH_STATIONARY = 1
+POLISH_STALLED = 2
+POLISH_FIXED = 4
+POLISH_CONVERGED = 8
+
+
 class Result(C.Structure):
     _fields_ = [
         ("distance", C.c_double),
@@ -76,6 +106,7 @@ class Result(C.Structure):
         ("ms_eval", C.c_double),
         ("ms_screen", C.c_double),
         ("round", RoundInfo * MAX_ROUNDS),
+        ("closest", Closest),
     ]
 
     def rounds_list(self):
@@ -106,10 +137,12 @@ def default_params(**overrides):
     p.cull = 1
     p.trace = 0
     p.cand_cap = 0
+    p.polish = 0
+    p.param_tol = 0.0
     for key, val in overrides.items():
         if key == "precision" and isinstance(val, str):
             val = PRECISIONS[val]
-        setattr(p, key, int(val))
+        setattr(p, key, float(val) if key == "param_tol" else int(val))
     return p
 
 
@@ -117,6 +150,24 @@ def pairs_to_numpy(buf, n):
     import numpy as np
     dt = np.dtype([("d2", "<f8"), ("slot", "<i4"), ("i", "<i4"), ("j", "<i4"), ("pad_", "<i4")])
     return np.frombuffer(bytes(memoryview(buf).cast("B")[: n * C.sizeof(Pair)]), dtype=dt).copy()
+
+
+class CurvePoint(C.Structure):
+    """jsi_curve_point: one ordered sample of an intersection curve."""
+    _fields_ = [("x", C.c_double), ("y", C.c_double), ("z", C.c_double), ("u1", C.c_double), ("v1", C.c_double),
+                ("u2", C.c_double), ("v2", C.c_double), ("residual", C.c_double), ("curve", C.c_int32),
+                ("flags", C.c_int32)]
+
+
+CURVE_CLOSED = 1
+CURVE_DTYPE = [("x", "<f8"), ("y", "<f8"), ("z", "<f8"), ("u1", "<f8"), ("v1", "<f8"), ("u2", "<f8"),
+               ("v2", "<f8"), ("residual", "<f8"), ("curve", "<i4"), ("flags", "<i4")]
+
+
+def curve_points_to_numpy(buf, n):
+    import numpy as np
+    return np.frombuffer(bytes(memoryview(buf).cast("B")[: n * C.sizeof(CurvePoint)]),
+                         dtype=np.dtype(CURVE_DTYPE)).copy()
 
 
 QUERY_DISTANCE = 0

@@ -5,6 +5,10 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
+#include "../../include/jsi_mi355x.h"
+
 namespace jsi {
 
 template <class T>
@@ -221,6 +225,62 @@ hipError_t launch_isect_units(const IPair* pairs, unsigned long long n, int cell
                               const unsigned* flag1, const unsigned* flag2, const unsigned long long* idx1,
                               const unsigned long long* idx2, int* firstpos2, int* nsu1, int* nsu2,
                               vec4_t<T>* nreg1, vec4_t<T>* nreg2, hipStream_t s);
+
+// FP64 solves on the final cells (jsi_refine.hip).  x = (u1, v1, u2, v2).
+struct PolishRec {  // closest-point polish of one top-k record
+  double x[4];
+  double f;         // |S1 - S2|^2 at x
+  double p1[3], p2[3];
+  int iters, flags; // JSI_POLISH_*
+};
+struct CurveRec {   // projection of one overlap pair onto the intersection curve
+  double x[4];
+  double p[3];      // S1(u1, v1)
+  double t[3];      // (S1u x S1v) x (S2u x S2v)
+  double r2;        // |S1 - S2|^2 at x
+  int iters, flags;
+};
+// box = (ulo, vlo, uhi, vhi) of the query region; TR = scalar of the unit regions.
+template <class TR>
+hipError_t launch_polish(const SurfDev<double>& s1, const SurfDev<double>& s2, const Cand* rec, int m, const int* su1,
+                         const int* su2, const vec4_t<TR>* reg1, const vec4_t<TR>* reg2, int nuv1, int nuv2,
+                         double4 box1, double4 box2, PolishRec* out, hipStream_t s);
+template <class TR>
+hipError_t launch_curve_proj(const SurfDev<double>& s1, const SurfDev<double>& s2, const IPair* pairs, long long n,
+                             const int* su1, const int* su2, const vec4_t<TR>* reg1, const vec4_t<TR>* reg2, int nuv1,
+                             int nuv2, double4 box1, double4 box2, CurveRec* out, hipStream_t s);
+// Curve tracing from the projections (jsi_refine.hip).
+struct MarchSeed {
+  double x[4];
+  double p[3];
+  double T[3];  // unit tangent
+  double r2;
+};
+struct MarchPt {
+  double x[4];
+  double p[3];
+  double r2;
+};
+struct MarchOut {
+  int succ;  // seed reached, -1 none
+  int n;     // points recorded
+  int flags; // JSI_MARCH_*
+  int pad;
+};
+constexpr int JSI_MARCH_SEED = 1, JSI_MARCH_EDGE = 2, JSI_MARCH_FAILED = 4, JSI_MARCH_CAP = 8;
+// Seeds: accepted projections thinned to spacing h; returns h.
+double curve_seeds(const CurveRec* recs, long long n, double tol, double spacing, std::vector<MarchSeed>& seeds);
+// Marches from seeds[starts[k]] along dir * T with step h (counting pass: pts
+// = nullptr; writing pass: points at pts + offs[k]).
+hipError_t launch_march(const SurfDev<double>& s1, const SurfDev<double>& s2, const MarchSeed* seeds, int nseeds,
+                        const int* starts, int nstart, double dir, double h, double4 box1, double4 box2,
+                        const long long* offs, MarchPt* pts, MarchOut* out, hipStream_t s);
+// Chains the marches into ordered curves.
+void assemble_curves(const std::vector<MarchSeed>& seeds, const std::vector<MarchOut>& fwd,
+                     const std::vector<long long>& fwd_off, const std::vector<MarchPt>& fwd_pts,
+                     const std::vector<int>& heads, const std::vector<MarchOut>& bwd,
+                     const std::vector<long long>& bwd_off, const std::vector<MarchPt>& bwd_pts,
+                     std::vector<jsi_curve_point>& out, int* ncurves);
 
 hipError_t launch_fill_i32(int* p, int value, unsigned long long n, hipStream_t s);
 hipError_t launch_fill_u32(unsigned* p, unsigned value, unsigned long long n, hipStream_t s);

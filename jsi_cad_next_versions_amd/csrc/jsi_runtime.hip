@@ -414,6 +414,73 @@ static double d2_from_bits(uint64_t b) {
   return d;
 }
 
+// Screening rounds of a distance query: dist_max_rounds, or with param_tol >
+// 0 the first round whose cells span <= param_tol in u and v on both
+// surfaces (round 1 cells span |stop - start| / nuv, each later round divides
+// by the refinement nuv).
+static int dist_rounds(const jsi_params& P, const jsi_region& r1, const jsi_region& r2) {
+  if (!(P.param_tol > 0.0)) return P.dist_max_rounds;
+  double w = 0.0;
+  for (const jsi_region* r : {&r1, &r2}) {
+    w = std::max(w, std::fabs((double)r->ustop - (double)r->ustart) / (double)r->nuv);
+    w = std::max(w, std::fabs((double)r->vstop - (double)r->vstart) / (double)r->nuv);
+  }
+  int rounds = 1;
+  while (w > P.param_tol && rounds < JSI_MAX_ROUNDS) {
+    w = w / (double)P.dist_refine_nuv;
+    ++rounds;
+  }
+  return rounds;
+}
+
+// Parameter box (ulo, vlo, uhi, vhi) of a query region.
+static double4 region_box(const jsi_region& r) {
+  double4 b;
+  b.x = std::min((double)r.ustart, (double)r.ustop);
+  b.y = std::min((double)r.vstart, (double)r.vstop);
+  b.z = std::max((double)r.ustart, (double)r.ustop);
+  b.w = std::max((double)r.vstart, (double)r.vstop);
+  return b;
+}
+
+// FP64 closest-point polish of the first P.polish records of the final
+// sorted top-k list (jsi_refine.hip); the smallest |S1 - S2| wins, the lowest
+// rank on ties.
+template <class T>
+int jsi_ctx::polish(int h1, const jsi_region& r1, int h2, const jsi_region& r2, const jsi_params& P, const Cand* rec,
+                    long long kk, const int* su1, const int* su2, const vec4_t<T>* reg1, const vec4_t<T>* reg2,
+                    int nuv1, int nuv2, jsi_result* res) {
+  const int m = (int)std::min<long long>(kk, P.polish);
+  SurfDev<double> s1, s2;
+  RCK(surf_dev<double>(h1, &s1));
+  RCK(surf_dev<double>(h2, &s2));
+  PolishRec* d_out;
+  RCK(buf("pol.out", (size_t)m, &d_out));
+  HCK(launch_polish<T>(s1, s2, rec, m, su1, su2, reg1, reg2, nuv1, nuv2, region_box(r1), region_box(r2), d_out,
+                       stream));
+  std::vector<PolishRec> h(m);
+  RCK(d2h(h.data(), d_out, sizeof(PolishRec) * m));
+  int best = 0;
+  for (int e = 1; e < m; ++e)
+    if (h[e].f < h[best].f) best = e;
+  jsi_closest& c = res->closest;
+  const PolishRec& b = h[best];
+  c.distance = std::sqrt(b.f);
+  c.u1 = b.x[0];
+  c.v1 = b.x[1];
+  c.u2 = b.x[2];
+  c.v2 = b.x[3];
+  for (int t = 0; t < 3; ++t) {
+    c.p1[t] = b.p1[t];
+    c.p2[t] = b.p2[t];
+  }
+  c.candidate = best;
+  c.iterations = b.iters;
+  c.polished = m;
+  c.flags = b.flags;
+  return JSI_OK;
+}
+
 template <class T>
 int jsi_ctx::min_distance(int h1, const jsi_region& r1, int h2, const jsi_region& r2, const jsi_params& P,
                            jsi_result* res, jsi_pair* out, int cap_pairs) {
@@ -506,6 +573,7 @@ int jsi_ctx::min_distance(int h1, const jsi_region& r1, int h2, const jsi_region
   }
   int nslots = 1;
   const int nref = P.dist_refine_nuv;
+  const int max_rounds = dist_rounds(P, r1, r2);
   std::vector<Cand> host_rec;
   Cand* rec = nullptr;
   long long kk = 0;
@@ -561,7 +629,7 @@ int jsi_ctx::min_distance(int h1, const jsi_region& r1, int h2, const jsi_region
     }
     res->rounds = round;
     res->npairs = kk;
-    if (kk == 0 || round == P.dist_max_rounds || round >= JSI_MAX_ROUNDS) break;
+    if (kk == 0 || round == max_rounds || round >= JSI_MAX_ROUNDS) break;
     // refinement units for the next round
     te = now_ms();
     const int nxt = cur ^ 1;
@@ -638,6 +706,7 @@ int jsi_ctx::min_distance(int h1, const jsi_region& r1, int h2, const jsi_region
       res->distance = 0.0;
     }
   }
+  if (P.polish > 0 && kk > 0) RCK(polish<T>(h1, r1, h2, r2, P, rec, kk, su1, su2, reg1, reg2, nuv1, nuv2, res));
   if (out && cap_pairs > 0 && kk > 0) {
     const long long m = std::min<long long>(kk, cap_pairs);
     host_rec.resize(m);
@@ -672,7 +741,7 @@ int jsi_ctx::isect_units(int h, const vec4_t<T>* reg, int nunits, int nuv, int u
 
 template <class T>
 int jsi_ctx::intersect(int h1, const jsi_region& r1, int h2, const jsi_region& r2, const jsi_params& P,
-                        jsi_result* res, jsi_pair* out, int cap_pairs) {
+                        jsi_result* res, jsi_pair* out, int cap_pairs, CurveReq* creq) {
   using V = vec4_t<T>;
   const double t0 = now_ms();
   auto smax_h = [](T a, T b) { return (a < b) ? b : a; };
@@ -847,6 +916,23 @@ int jsi_ctx::intersect(int h1, const jsi_region& r1, int h2, const jsi_region& r
     nuv1 = nuv2 = g_nuv;
   }
   res->isect_refine_nuv = g_nuv;
+  if (creq) {
+    // project every final pair onto the intersection curve, then order the points
+    creq->points.clear();
+    creq->ncurves = 0;
+    if (npairs > 0) {
+      SurfDev<double> s1, s2;
+      RCK(surf_dev<double>(h1, &s1));
+      RCK(surf_dev<double>(h2, &s2));
+      CurveRec* d_cr;
+      RCK(buf("crv.out", npairs, &d_cr));
+      HCK(launch_curve_proj<T>(s1, s2, pairs, (long long)npairs, su1, su2, reg1, reg2, nuv1, nuv2, region_box(r1),
+                               region_box(r2), d_cr, stream));
+      std::vector<CurveRec> hc(npairs);
+      RCK(d2h(hc.data(), d_cr, sizeof(CurveRec) * npairs));
+      RCK(trace_curves(hc.data(), (long long)npairs, s1, s2, region_box(r1), region_box(r2), creq));
+    }
+  }
   if (out && cap_pairs > 0 && npairs > 0) {
     const unsigned long long m = std::min<unsigned long long>(npairs, (unsigned long long)cap_pairs);
     hp.resize(m);
@@ -854,6 +940,64 @@ int jsi_ctx::intersect(int h1, const jsi_region& r1, int h2, const jsi_region& r
     for (unsigned long long e = 0; e < m; ++e) out[e] = jsi_pair{0.0, hp[e].slot, hp[e].i, hp[e].j, 0};
   }
   res->ms_total = now_ms() - t0;
+  return JSI_OK;
+}
+
+// Curves from the projected pairs: seeds (host), forward marches of every
+// seed and backward marches of the chain heads (device, a counting pass and a
+// writing pass each), chaining (host).
+int jsi_ctx::trace_curves(const CurveRec* recs, long long n, const SurfDev<double>& s1, const SurfDev<double>& s2,
+                          double4 box1, double4 box2, CurveReq* creq) {
+  std::vector<MarchSeed> seeds;
+  const double hstep = curve_seeds(recs, n, creq->tol, creq->spacing, seeds);
+  const int ns = (int)seeds.size();
+  if (ns == 0) return JSI_OK;
+  MarchSeed* d_seeds;
+  int* d_starts;
+  MarchOut* d_mo;
+  long long* d_off;
+  RCK(buf("crv.seeds", ns, &d_seeds));
+  RCK(buf("crv.starts", ns, &d_starts));
+  RCK(buf("crv.mo", ns, &d_mo));
+  RCK(buf("crv.off", ns, &d_off));
+  RCK(h2d(d_seeds, seeds.data(), sizeof(MarchSeed) * ns));
+  auto march = [&](const std::vector<int>& starts, double dir, std::vector<MarchOut>& mo, std::vector<long long>& off,
+                   std::vector<MarchPt>& pts) -> int {
+    const int m = (int)starts.size();
+    mo.assign(m, MarchOut());
+    off.assign(m, 0);
+    pts.clear();
+    if (m == 0) return JSI_OK;
+    RCK(h2d(d_starts, starts.data(), sizeof(int) * m));
+    HCK(launch_march(s1, s2, d_seeds, ns, d_starts, m, dir, hstep, box1, box2, nullptr, nullptr, d_mo, stream));
+    RCK(d2h(mo.data(), d_mo, sizeof(MarchOut) * m));
+    long long total = 0;
+    for (int k = 0; k < m; ++k) {
+      off[k] = total;
+      total += mo[k].n;
+    }
+    if (total == 0) return JSI_OK;
+    MarchPt* d_pts;
+    RCK(buf("crv.pts", (size_t)total, &d_pts));
+    RCK(h2d(d_off, off.data(), sizeof(long long) * m));
+    HCK(launch_march(s1, s2, d_seeds, ns, d_starts, m, dir, hstep, box1, box2, d_off, d_pts, d_mo, stream));
+    pts.resize(total);
+    RCK(d2h(pts.data(), d_pts, sizeof(MarchPt) * total));
+    return JSI_OK;
+  };
+  std::vector<int> all(ns);
+  for (int k = 0; k < ns; ++k) all[k] = k;
+  std::vector<MarchOut> fwd, bwd;
+  std::vector<long long> fwd_off, bwd_off;
+  std::vector<MarchPt> fwd_pts, bwd_pts;
+  RCK(march(all, 1.0, fwd, fwd_off, fwd_pts));
+  std::vector<int> indeg(ns, 0), heads;
+  for (int k = 0; k < ns; ++k)
+    if (fwd[k].succ >= 0) ++indeg[fwd[k].succ];
+  for (int k = 0; k < ns; ++k)
+    if (indeg[k] == 0) heads.push_back(k);
+  RCK(march(heads, -1.0, bwd, bwd_off, bwd_pts));
+  assemble_curves(seeds, fwd, fwd_off, fwd_pts, heads, bwd, bwd_off, bwd_pts, creq->points, &creq->ncurves);
   return JSI_OK;
 }
 
@@ -1153,8 +1297,13 @@ int jsi_min_distance(jsi_ctx* c, int h1, const jsi_region* r1, int h2, const jsi
                      jsi_result* res, jsi_pair* pairs_out, int pairs_cap) {
   RCK(validate_query(c, h1, r1, h2, r2, P));
   if (!res || P->k < 1 || P->k > JSI_MAX_K || P->dist_max_rounds < 1 || P->dist_refine_nuv < 2 ||
-      P->dist_refine_nuv > JSI_MAX_NUV) {
+      P->dist_refine_nuv > JSI_MAX_NUV || P->polish < 0 || P->polish > JSI_MAX_K || !(P->param_tol >= 0.0) ||
+      !std::isfinite(P->param_tol)) {
     set_err("invalid distance parameters");
+    return JSI_ERR_INVALID;
+  }
+  if (P->param_tol > 0.0 && P->precision == JSI_PREC_EXACT32) {
+    set_err("param_tol needs FP64 parameters (precision FP64 or FP32SCREEN)");
     return JSI_ERR_INVALID;
   }
   std::memset(res, 0, sizeof(*res));
@@ -1175,6 +1324,36 @@ int jsi_intersect(jsi_ctx* c, int h1, const jsi_region* r1, int h2, const jsi_re
   HCK(hipSetDevice(c->device));
   if (P->precision == JSI_PREC_EXACT32) return c->intersect<float>(h1, *r1, h2, *r2, *P, res, pairs_out, pairs_cap);
   return c->intersect<double>(h1, *r1, h2, *r2, *P, res, pairs_out, pairs_cap);
+}
+
+int jsi_intersect_curves(jsi_ctx* c, int h1, const jsi_region* r1, int h2, const jsi_region* r2,
+                         const jsi_params* P, double spacing, double tol, jsi_result* res, jsi_curve_point* pts,
+                         int cap, int* npts, int* ncurves) {
+  RCK(validate_query(c, h1, r1, h2, r2, P));
+  if (!res || !npts || !ncurves || (cap > 0 && !pts) || P->isect_max_rounds < 1 || P->isect_refine_nuv < 2 ||
+      P->isect_refine_nuv > JSI_MAX_NUV || P->isect_min_nuv < 1 || !std::isfinite(spacing) ||
+      !std::isfinite(tol) || tol < 0.0) {
+    set_err("invalid intersection-curve parameters");
+    return JSI_ERR_INVALID;
+  }
+  std::memset(res, 0, sizeof(*res));
+  *npts = 0;
+  *ncurves = 0;
+  HCK(hipSetDevice(c->device));
+  CurveReq req;
+  req.spacing = spacing;
+  req.tol = tol > 0.0 ? tol : 1.0e-9;
+  int rc;
+  if (P->precision == JSI_PREC_EXACT32)
+    rc = c->intersect<float>(h1, *r1, h2, *r2, *P, res, nullptr, 0, &req);
+  else
+    rc = c->intersect<double>(h1, *r1, h2, *r2, *P, res, nullptr, 0, &req);
+  if (rc != JSI_OK) return rc;
+  const size_t n = req.points.size();
+  *npts = (int)std::min<size_t>(n, INT_MAX);
+  *ncurves = req.ncurves;
+  if (cap > 0) std::memcpy(pts, req.points.data(), sizeof(jsi_curve_point) * std::min<size_t>(n, (size_t)cap));
+  return JSI_OK;
 }
 
 }  // extern "C"

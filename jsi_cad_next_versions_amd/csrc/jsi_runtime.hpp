@@ -53,6 +53,14 @@ struct TileBoxes {
   const V* hi2 = nullptr;
 };
 
+// Intersection-curve request of jsi_intersect_curves.
+struct CurveReq {
+  double spacing = 0.0;
+  double tol = 1.0e-9;
+  std::vector<jsi_curve_point> points;
+  int ncurves = 0;
+};
+
 struct KernelStat {
   long long launches = 0;
   double ms = 0.0;     // summed launch durations (HIP events on the context stream)
@@ -107,8 +115,14 @@ struct jsi_ctx {
   int isect_units(int h, const jsi::vec4_t<T>* reg, int nunits, int nuv, int use_k, T init_u, T init_v,
                   jsi::vec4_t<T>* C, jsi::vec4_t<T>* R, const char* tag);
   template <class T>
+  int polish(int h1, const jsi_region& r1, int h2, const jsi_region& r2, const jsi_params& P, const jsi::Cand* rec,
+             long long kk, const int* su1, const int* su2, const jsi::vec4_t<T>* reg1, const jsi::vec4_t<T>* reg2,
+             int nuv1, int nuv2, jsi_result* res);
+  template <class T>
   int intersect(int h1, const jsi_region& r1, int h2, const jsi_region& r2, const jsi_params& P, jsi_result* res,
-                jsi_pair* out, int cap_pairs);
+                jsi_pair* out, int cap_pairs, CurveReq* creq = nullptr);
+  int trace_curves(const jsi::CurveRec* recs, long long n, const jsi::SurfDev<double>& s1,
+                   const jsi::SurfDev<double>& s2, double4 box1, double4 box2, CurveReq* creq);
   template <class T>
   int eval_grid(int h, T ustep, T vstep, T ustart, T vstart, int nuv, T* host_out);
 };

@@ -14,8 +14,8 @@ import os
 
 import numpy as np
 
-from ._abi import (JSI_ERR_BUSY, JSI_OK, MAX_ROUNDS, QUERY_DISTANCE, QUERY_INTERSECT, Pair, Params,  # noqa: F401
-                   Query, Region, Result, default_params, pairs_to_numpy)
+from ._abi import (JSI_ERR_BUSY, JSI_OK, MAX_ROUNDS, QUERY_DISTANCE, QUERY_INTERSECT, CurvePoint,  # noqa: F401
+                   Pair, Params, Query, Region, Result, curve_points_to_numpy, default_params, pairs_to_numpy)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libjsi_mi355x.so")
@@ -28,7 +28,8 @@ _i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
 EXPORTS = [
     "jsi_params_default", "jsi_status_string", "jsi_last_error", "jsi_ctx_create", "jsi_ctx_destroy",
     "jsi_ctx_sync", "jsi_ctx_profile", "jsi_ctx_kernel_stats", "jsi_surface_upload", "jsi_surface_register_device", "jsi_surface_release",
-    "jsi_eval_grid_f32", "jsi_eval_grid_f64", "jsi_min_distance", "jsi_intersect", "jsi_run_batch",
+    "jsi_eval_grid_f32", "jsi_eval_grid_f64", "jsi_min_distance", "jsi_intersect", "jsi_intersect_curves",
+    "jsi_run_batch",
     "jsi_server_create", "jsi_server_destroy", "jsi_server_surface", "jsi_server_submit", "jsi_server_poll",
     "jsi_server_counters",
 ]
@@ -74,6 +75,8 @@ def load_library(path: str = LIB_PATH):
          C.c_void_p, C.c_int]
     L.jsi_min_distance.argtypes = q
     L.jsi_intersect.argtypes = q
+    L.jsi_intersect_curves.argtypes = q[:6] + [C.c_double, C.c_double, C.POINTER(Result), C.c_void_p, C.c_int,
+                                               C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.jsi_run_batch.argtypes = [C.c_void_p, C.c_int, C.POINTER(Query), C.POINTER(Result), C.POINTER(C.c_int),
                                 C.c_int]
     L.jsi_server_create.argtypes = [C.c_void_p, C.c_int, C.c_size_t, C.c_int, C.c_size_t, C.POINTER(C.c_void_p)]
@@ -177,6 +180,19 @@ class Context:
         """Returns (Result, last-round overlap pairs (up to cap))."""
         params = params or default_params()
         return self._query(self.L.jsi_intersect, s1, r1, s2, r2, params, cap)
+
+    def intersection_curves(self, s1, r1, s2, r2, params=None, spacing=0.0, tol=0.0, cap=1 << 20):
+        """Intersection rounds, then the FP64 curve samples: (Result, ordered
+        samples as a numpy record array, number of curves)."""
+        params = params or default_params()
+        h1, h2 = self.upload(s1), self.upload(s2)
+        res = Result()
+        buf = (CurvePoint * max(cap, 1))()
+        n, nc = C.c_int(), C.c_int()
+        _check(self.L, self.L.jsi_intersect_curves(self.h, h1, C.byref(r1), h2, C.byref(r2), C.byref(params),
+                                                   spacing, tol, C.byref(res), C.cast(buf, C.c_void_p), cap,
+                                                   C.byref(n), C.byref(nc)))
+        return res, curve_points_to_numpy(buf, min(cap, n.value)), nc.value
 
     def query(self, kind, s1, r1, s2, r2, params=None):
         """A jsi_query for run_batch / Server.submit (uploads the surfaces)."""

@@ -15,7 +15,9 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <array>
 #include <deque>
+#include <map>
 #include <limits>
 #include <queue>
 #include <unordered_map>
@@ -554,6 +556,577 @@ struct Dedup {
   }
 };
 
+
+// ---------------------------------------------------------------------------
+// FP64 solves on the final cells (no counterpart in the reference: its
+// minimum_distance returns the k-th centre distance, distance_analysis.cu:505,
+// and intersection() only prints pair counts, intersection.cu:367).
+//
+// Closest points: from the parameter centres of a record's two cells, a
+// damped Newton iteration on f(x) = |S1(u1,v1) - S2(u2,v2)|^2, x projected
+// onto the query boxes; a coordinate on a bound whose gradient points
+// outward is held (gradient entry 0, identity row and column).  With J = [S1u S1v -S2u -S2v] and d = S1 - S2:
+// gradient g = J^T d, Hessian H = J^T J + the d-weighted second derivatives
+// (+ for surface 1, - for surface 2), step z from (H + mu diag(J^T J + eps)) z
+// = g by LDL^T, x' = clamp(x - z); accepted when f decreases (mu /= 10),
+// otherwise mu *= 10.  Intersection points: Gauss-Newton with minimum-norm
+// steps, x' = clamp(x - J^T (J J^T)^-1 d).  Both in double, a fixed
+// operation order, no contraction.
+// ---------------------------------------------------------------------------
+struct Sol {
+  double x[4] = {0, 0, 0, 0};
+  double f = 0;          // |S1 - S2|^2
+  double p1[3] = {0, 0, 0}, p2[3] = {0, 0, 0};
+  double t[3] = {0, 0, 0};  // intersection: (S1u x S1v) x (S2u x S2v)
+  int iters = 0, flags = 0;
+};
+
+inline double dot(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+inline double bigger(double a, double b) { return b > a ? b : a; }
+inline double clip(double t, double lo, double hi) { return t < lo ? lo : (t > hi ? hi : t); }
+
+// LDL^T solve of a symmetric n x n system given by its lower triangle.
+bool ldlt(int n, const double A[4][4], const double* rhs, double* sol) {
+  double L[4][4] = {}, W[4][4] = {}, D[4] = {}, y[4] = {};
+  for (int j = 0; j < n; ++j) {
+    double dj = A[j][j];
+    for (int k = 0; k < j; ++k) dj = dj - W[j][k] * L[j][k];
+    if (!(dj > 0.0 && dj <= DBL_MAX)) return false;
+    D[j] = dj;
+    for (int i = j + 1; i < n; ++i) {
+      double v = A[i][j];
+      for (int k = 0; k < j; ++k) v = v - W[i][k] * L[j][k];
+      W[i][j] = v;
+      L[i][j] = v / dj;
+    }
+  }
+  for (int i = 0; i < n; ++i) {
+    double v = rhs[i];
+    for (int k = 0; k < i; ++k) v = v - L[i][k] * y[k];
+    y[i] = v;
+  }
+  for (int i = 0; i < n; ++i) y[i] = y[i] / D[i];
+  for (int i = n - 1; i >= 0; --i) {
+    double v = y[i];
+    for (int k = n - 1; k > i; --k) v = v - L[k][i] * sol[k];
+    sol[i] = v;
+  }
+  return true;
+}
+
+// Parameter centre of cell `cell` of a unit (cell iu spans [start + step iu,
+// start + step (iu + 1)], step = (stop - start)/nuv, distance_analysis.cu:92-94).
+template <class T>
+void cell_mid(const Region<T>& r, int cell, double* o) {
+  const int w = r.nuv - 1;
+  const int iu = cell % w, iv = cell / w;
+  const double us = (double(r.ustop) - double(r.ustart)) / double(r.nuv);
+  const double vs = (double(r.vstop) - double(r.vstart)) / double(r.nuv);
+  o[0] = double(r.ustart) + us * (double(iu) + 0.5);
+  o[1] = double(r.vstart) + vs * (double(iv) + 0.5);
+}
+
+struct Boxes {
+  double lo[4], hi[4];
+};
+Boxes boxes_of(const jsi_region& a, const jsi_region& b) {
+  Boxes bx;
+  const jsi_region* rs[2] = {&a, &b};
+  for (int s = 0; s < 2; ++s) {
+    const double us = rs[s]->ustart, ue = rs[s]->ustop, vs = rs[s]->vstart, ve = rs[s]->vstop;
+    bx.lo[2 * s] = std::min(us, ue);
+    bx.hi[2 * s] = std::max(us, ue);
+    bx.lo[2 * s + 1] = std::min(vs, ve);
+    bx.hi[2 * s + 1] = std::max(vs, ve);
+  }
+  return bx;
+}
+
+Sol newton_closest(const Net<double>& n1, const Net<double>& n2, const Boxes& bx, const double* start) {
+  Sol r;
+  double x[4], y[4];
+  for (int q = 0; q < 4; ++q) x[q] = clip(start[q], bx.lo[q], bx.hi[q]);
+  double A[18], B[18], An[18], Bn[18];
+  eval_point(n1, x[0], x[1], A);
+  eval_point(n2, x[2], x[3], B);
+  double d[3] = {A[0] - B[0], A[1] - B[1], A[2] - B[2]};
+  double f = dot(d, d);
+  double mu = 1.0e-3;
+  int it = 0;
+  while (it < 60) {
+    ++it;
+    const double *su1 = A + 3, *sv1 = A + 6, *su2 = B + 3, *sv2 = B + 6;
+    double g[4] = {dot(su1, d), dot(sv1, d), -dot(su2, d), -dot(sv2, d)};
+    bool held[4];
+    for (int q = 0; q < 4; ++q) {
+      held[q] = (x[q] <= bx.lo[q] && g[q] > 0.0) || (x[q] >= bx.hi[q] && g[q] < 0.0);
+      if (held[q]) g[q] = 0.0;
+    }
+    if (g[0] == 0.0 && g[1] == 0.0 && g[2] == 0.0 && g[3] == 0.0) {
+      r.flags |= JSI_POLISH_STATIONARY;
+      break;
+    }
+    const double jj[4] = {dot(su1, su1), dot(sv1, sv1), dot(su2, su2), dot(sv2, sv2)};
+    double H[4][4] = {};
+    H[0][0] = jj[0] + dot(d, A + 9);
+    H[1][0] = dot(su1, sv1) + dot(d, A + 12);
+    H[1][1] = jj[1] + dot(d, A + 15);
+    H[2][0] = -dot(su1, su2);
+    H[3][0] = -dot(su1, sv2);
+    H[2][1] = -dot(sv1, su2);
+    H[3][1] = -dot(sv1, sv2);
+    H[2][2] = jj[2] - dot(d, B + 9);
+    H[3][2] = dot(su2, sv2) - dot(d, B + 12);
+    H[3][3] = jj[3] - dot(d, B + 15);
+    const double eps = 1.0e-12 * bigger(bigger(jj[0], jj[1]), bigger(jj[2], jj[3]));
+    for (int q = 0; q < 4; ++q) H[q][q] = H[q][q] + mu * (jj[q] + eps);
+    for (int q = 0; q < 4; ++q) {
+      if (!held[q]) continue;
+      for (int r = 0; r < 4; ++r)
+        if (r < q)
+          H[q][r] = 0.0;
+        else if (r > q)
+          H[r][q] = 0.0;
+      H[q][q] = 1.0;
+    }
+    double z[4];
+    if (!ldlt(4, H, g, z)) {
+      mu = mu * 10.0;
+      if (mu > 1.0e12) {
+        r.flags |= JSI_POLISH_STALLED;
+        break;
+      }
+      continue;
+    }
+    bool moved = false;
+    for (int q = 0; q < 4; ++q) {
+      y[q] = clip(x[q] - z[q], bx.lo[q], bx.hi[q]);
+      moved = moved || y[q] != x[q];
+    }
+    if (!moved) {
+      r.flags |= JSI_POLISH_FIXED;
+      break;
+    }
+    eval_point(n1, y[0], y[1], An);
+    eval_point(n2, y[2], y[3], Bn);
+    const double dn[3] = {An[0] - Bn[0], An[1] - Bn[1], An[2] - Bn[2]};
+    const double fn = dot(dn, dn);
+    if (!(fn < f)) {
+      mu = mu * 10.0;
+      if (mu > 1.0e12) {
+        r.flags |= JSI_POLISH_STALLED;
+        break;
+      }
+      continue;
+    }
+    double step = 0.0, mag = 0.0;
+    for (int q = 0; q < 4; ++q) {
+      step = bigger(step, std::fabs(y[q] - x[q]));
+      mag = bigger(mag, std::fabs(y[q]));
+      x[q] = y[q];
+    }
+    std::memcpy(A, An, sizeof(A));
+    std::memcpy(B, Bn, sizeof(B));
+    for (int c = 0; c < 3; ++c) d[c] = dn[c];
+    f = fn;
+    mu = mu * 0.1;
+    if (mu < 1.0e-15) mu = 1.0e-15;
+    if (step <= 1.0e-15 * (1.0 + mag)) {
+      r.flags |= JSI_POLISH_CONVERGED;
+      break;
+    }
+  }
+  for (int q = 0; q < 4; ++q) r.x[q] = x[q];
+  r.f = f;
+  for (int c = 0; c < 3; ++c) {
+    r.p1[c] = A[c];
+    r.p2[c] = B[c];
+  }
+  r.iters = it;
+  return r;
+}
+
+Sol newton_meet(const Net<double>& n1, const Net<double>& n2, const Boxes& bx, const double* start) {
+  Sol r;
+  double x[4];
+  for (int q = 0; q < 4; ++q) x[q] = clip(start[q], bx.lo[q], bx.hi[q]);
+  double A[18], B[18];
+  eval_point(n1, x[0], x[1], A);
+  eval_point(n2, x[2], x[3], B);
+  double d[3] = {A[0] - B[0], A[1] - B[1], A[2] - B[2]};
+  double f = dot(d, d);
+  const double scale = 1.0 + bigger(bigger(std::fabs(A[0]), std::fabs(A[1])), std::fabs(A[2]));
+  const double tl = 1.0e-14 * scale;
+  const double tol2 = tl * tl;
+  int it = 0;
+  while (true) {
+    if (f <= tol2) {
+      r.flags |= JSI_POLISH_CONVERGED;
+      break;
+    }
+    if (it == 40) break;
+    ++it;
+    const double *su1 = A + 3, *sv1 = A + 6, *su2 = B + 3, *sv2 = B + 6;
+    double G[4][4] = {};
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j <= i; ++j) G[i][j] = su1[i] * su1[j] + sv1[i] * sv1[j] + su2[i] * su2[j] + sv2[i] * sv2[j];
+    double lam[3];
+    if (!ldlt(3, G, d, lam)) {
+      r.flags |= JSI_POLISH_STALLED;
+      break;
+    }
+    const double step[4] = {-dot(su1, lam), -dot(sv1, lam), dot(su2, lam), dot(sv2, lam)};
+    double y[4];
+    bool moved = false;
+    for (int q = 0; q < 4; ++q) {
+      y[q] = clip(x[q] + step[q], bx.lo[q], bx.hi[q]);
+      moved = moved || y[q] != x[q];
+    }
+    if (!moved) {
+      r.flags |= JSI_POLISH_FIXED;
+      break;
+    }
+    std::memcpy(x, y, sizeof(x));
+    eval_point(n1, x[0], x[1], A);
+    eval_point(n2, x[2], x[3], B);
+    for (int c = 0; c < 3; ++c) d[c] = A[c] - B[c];
+    f = dot(d, d);
+  }
+  for (int q = 0; q < 4; ++q) r.x[q] = x[q];
+  r.f = f;
+  for (int c = 0; c < 3; ++c) {
+    r.p1[c] = A[c];
+    r.p2[c] = B[c];
+  }
+  const double n1v[3] = {A[4] * A[8] - A[5] * A[7], A[5] * A[6] - A[3] * A[8], A[3] * A[7] - A[4] * A[6]};
+  const double n2v[3] = {B[4] * B[8] - B[5] * B[7], B[5] * B[6] - B[3] * B[8], B[3] * B[7] - B[4] * B[6]};
+  r.t[0] = n1v[1] * n2v[2] - n1v[2] * n2v[1];
+  r.t[1] = n1v[2] * n2v[0] - n1v[0] * n2v[2];
+  r.t[2] = n1v[0] * n2v[1] - n1v[1] * n2v[0];
+  r.iters = it;
+  return r;
+}
+
+struct Cell3 {
+  long long a, b, c;
+  bool operator<(const Cell3& o) const { return a != o.a ? a < o.a : (b != o.b ? b < o.b : c < o.c); }
+};
+Cell3 cell3(const double* p, double s) {
+  return Cell3{(long long)std::floor(p[0] / s), (long long)std::floor(p[1] / s), (long long)std::floor(p[2] / s)};
+}
+double sqdist(const double* p, const double* q) {
+  const double e[3] = {p[0] - q[0], p[1] - q[1], p[2] - q[2]};
+  return dot(e, e);
+}
+
+// Tracing of the intersection curves from the projected points.
+//  * seeds: projections with residual^2 <= tol^2, in pair order, thinned to
+//    spacing h (dropped when a kept seed is closer than h; h = spacing, else
+//    1e-3 of the accepted points' bounding-box diagonal, 1 if that is 0);
+//    seeds with a zero tangent t = (S1u x S1v) x (S2u x S2v) are left out;
+//    T = t / |t|;
+//  * march from a seed along +-T with step h: predictor = per-surface
+//    least-squares parameter step for a move of h T; corrector = Gauss-Newton
+//    (minimum norm) on S1 - S2 = 0 and T.(S1 - p_prev) = h from the predicted
+//    point clamped into the query boxes; when a corrector iterate leaves the
+//    boxes, it is clamped, the coordinate furthest outside (relative to its
+//    box width) is fixed, S1 - S2 = 0 is solved over the other three and the
+//    march ends at that edge point; after an interior step the tangent is
+//    re-oriented along the previous one; a new point closer than h/2 to a
+//    seed (lowest index; the start seed from the second step on) ends the
+//    march at that seed;
+//  * curves: every seed that no march reaches heads an open curve (backward
+//    march reversed, then the chain of seeds and forward points); leftover
+//    seeds form closed curves when their chain returns to its start.
+struct Seed {
+  double x[4], p[3], T[3], r2;
+};
+struct Step {
+  double x[4], p[3], r2;
+};
+struct Trace {
+  int succ = -1;
+  int flags = 0;
+  std::vector<Step> pts;
+};
+
+enum { kConv, kFail, kLeft, kNoConv };
+
+int corrector(const Net<double>& n1, const Net<double>& n2, const Boxes& bx, double* x, double* A, double* B, int freem,
+              bool plane, const double* T, const double* p0, double h, double tol2) {
+  for (int it = 0;; ++it) {
+    double F[4] = {A[0] - B[0], A[1] - B[1], A[2] - B[2], 0.0};
+    double r2 = dot(F, F);
+    if (plane) {
+      const double q[3] = {A[0] - p0[0], A[1] - p0[1], A[2] - p0[2]};
+      F[3] = dot(T, q) - h;
+      r2 = r2 + F[3] * F[3];
+    }
+    if (r2 <= tol2) return kConv;
+    if (it == 20) return kNoConv;
+    double J[4][4];
+    for (int r = 0; r < 3; ++r) {
+      J[r][0] = A[3 + r];
+      J[r][1] = A[6 + r];
+      J[r][2] = -B[3 + r];
+      J[r][3] = -B[6 + r];
+    }
+    J[3][0] = dot(T, A + 3);
+    J[3][1] = dot(T, A + 6);
+    J[3][2] = 0.0;
+    J[3][3] = 0.0;
+    for (int q = 0; q < 4; ++q)
+      if (!(freem & (1 << q)))
+        for (int r = 0; r < 4; ++r) J[r][q] = 0.0;
+    const int m = plane ? 4 : 3;
+    double G[4][4] = {};
+    for (int i = 0; i < m; ++i)
+      for (int j = 0; j <= i; ++j) G[i][j] = J[i][0] * J[j][0] + J[i][1] * J[j][1] + J[i][2] * J[j][2] + J[i][3] * J[j][3];
+    double lam[4];
+    if (!ldlt(m, G, F, lam)) return kFail;
+    bool outside = false;
+    for (int q = 0; q < 4; ++q) {
+      double corr = J[0][q] * lam[0] + J[1][q] * lam[1] + J[2][q] * lam[2];
+      if (plane) corr = corr + J[3][q] * lam[3];
+      const double v = x[q] - corr;
+      const bool out = v < bx.lo[q] || v > bx.hi[q];
+      outside = outside || (plane && out);
+      x[q] = (out && !plane) ? clip(v, bx.lo[q], bx.hi[q]) : v;
+    }
+    if (outside) return kLeft;
+    eval_point(n1, x[0], x[1], A);
+    eval_point(n2, x[2], x[3], B);
+  }
+}
+
+// Clamps into the boxes; returns the bit of the coordinate that was furthest
+// outside relative to its box width (first on ties), 0 if none was outside.
+int pin_to_box(double* x, const Boxes& bx) {
+  int pick = -1;
+  double far = 0.0;
+  for (int q = 0; q < 4; ++q) {
+    const double width = bx.hi[q] - bx.lo[q];
+    const double over = x[q] < bx.lo[q] ? (bx.lo[q] - x[q]) / width : (x[q] > bx.hi[q] ? (x[q] - bx.hi[q]) / width : 0.0);
+    if (over > far) {
+      far = over;
+      pick = q;
+    }
+    x[q] = clip(x[q], bx.lo[q], bx.hi[q]);
+  }
+  return pick < 0 ? 0 : 1 << pick;
+}
+
+Trace march(const Net<double>& n1, const Net<double>& n2, const Boxes& bx, const std::vector<Seed>& seeds, int from,
+            double dir, double h) {
+  Trace tr;
+  const Seed& sd = seeds[from];
+  double x[4], T[3], p[3], A[18], B[18];
+  std::memcpy(x, sd.x, sizeof(x));
+  for (int c = 0; c < 3; ++c) {
+    T[c] = dir * sd.T[c];
+    p[c] = sd.p[c];
+  }
+  eval_point(n1, x[0], x[1], A);
+  eval_point(n2, x[2], x[3], B);
+  const double scale = 1.0 + bigger(bigger(std::fabs(p[0]), std::fabs(p[1])), std::fabs(p[2]));
+  const double tl = 1.0e-14 * scale;
+  const double tol2 = tl * tl;
+  const double half = 0.5 * h;
+  const double half2 = half * half;
+  auto record = [&]() {
+    Step st;
+    std::memcpy(st.x, x, sizeof(x));
+    for (int c = 0; c < 3; ++c) st.p[c] = A[c];
+    const double d[3] = {A[0] - B[0], A[1] - B[1], A[2] - B[2]};
+    st.r2 = dot(d, d);
+    tr.pts.push_back(st);
+  };
+  int steps = 0;
+  for (; steps < 8192; ++steps) {
+    const double mv[3] = {h * T[0], h * T[1], h * T[2]};
+    double G1[4][4] = {}, G2[4][4] = {};
+    G1[0][0] = dot(A + 3, A + 3);
+    G1[1][0] = dot(A + 3, A + 6);
+    G1[1][1] = dot(A + 6, A + 6);
+    G2[0][0] = dot(B + 3, B + 3);
+    G2[1][0] = dot(B + 3, B + 6);
+    G2[1][1] = dot(B + 6, B + 6);
+    const double b1[2] = {dot(A + 3, mv), dot(A + 6, mv)};
+    const double b2[2] = {dot(B + 3, mv), dot(B + 6, mv)};
+    double d1[2], d2[2];
+    if (!ldlt(2, G1, b1, d1) || !ldlt(2, G2, b2, d2)) {
+      tr.flags |= 4;
+      break;
+    }
+    x[0] = x[0] + d1[0];
+    x[1] = x[1] + d1[1];
+    x[2] = x[2] + d2[0];
+    x[3] = x[3] + d2[1];
+    pin_to_box(x, bx);
+    eval_point(n1, x[0], x[1], A);
+    eval_point(n2, x[2], x[3], B);
+    const int st = corrector(n1, n2, bx, x, A, B, 15, true, T, p, h, tol2);
+    int pinned = 0;
+    if (st == kLeft) {
+      pinned = pin_to_box(x, bx);
+      eval_point(n1, x[0], x[1], A);
+      eval_point(n2, x[2], x[3], B);
+      if (corrector(n1, n2, bx, x, A, B, 15 & ~pinned, false, T, p, h, tol2) != kConv) {
+        tr.flags |= 4;
+        break;
+      }
+    } else if (st != kConv) {
+      tr.flags |= 4;
+      break;
+    }
+    if (!pinned) {
+      const double c1[3] = {A[4] * A[8] - A[5] * A[7], A[5] * A[6] - A[3] * A[8], A[3] * A[7] - A[4] * A[6]};
+      const double c2[3] = {B[4] * B[8] - B[5] * B[7], B[5] * B[6] - B[3] * B[8], B[3] * B[7] - B[4] * B[6]};
+      const double t[3] = {c1[1] * c2[2] - c1[2] * c2[1], c1[2] * c2[0] - c1[0] * c2[2],
+                           c1[0] * c2[1] - c1[1] * c2[0]};
+      const double tt = dot(t, t);
+      if (!(tt > 0.0)) {
+        tr.flags |= 4;
+        break;
+      }
+      const double inv = 1.0 / std::sqrt(tt);
+      double u[3] = {t[0] * inv, t[1] * inv, t[2] * inv};
+      if (dot(u, T) < 0.0)
+        for (double& v : u) v = -v;
+      for (int c = 0; c < 3; ++c) T[c] = u[c];
+    }
+    for (int c = 0; c < 3; ++c) p[c] = A[c];
+    int hit = -1;
+    for (int q = 0; q < (int)seeds.size() && hit < 0; ++q)
+      if (!(q == from && steps == 0) && sqdist(p, seeds[q].p) < half2) hit = q;
+    if (hit >= 0) {
+      tr.succ = hit;
+      tr.flags |= 1;
+      break;
+    }
+    record();
+    if (pinned) {
+      tr.flags |= 2;
+      break;
+    }
+  }
+  if (steps == 8192) tr.flags |= 8;
+  return tr;
+}
+
+void trace_curves(const Net<double>& n1, const Net<double>& n2, const Boxes& bx, const std::vector<Sol>& sols,
+                  double tol, double spacing, std::vector<jsi_curve_point>& out, int* ncurves) {
+  out.clear();
+  *ncurves = 0;
+  std::vector<const Sol*> acc;
+  for (const Sol& s : sols)
+    if (s.f <= tol * tol) acc.push_back(&s);
+  if (acc.empty()) return;
+  double h = spacing;
+  if (!(h > 0.0)) {
+    double lo[3], hi[3];
+    for (int c = 0; c < 3; ++c) lo[c] = hi[c] = acc[0]->p1[c];
+    for (const Sol* s : acc)
+      for (int c = 0; c < 3; ++c) {
+        lo[c] = std::min(lo[c], s->p1[c]);
+        hi[c] = std::max(hi[c], s->p1[c]);
+      }
+    const double ext[3] = {hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]};
+    h = 1.0e-3 * std::sqrt(dot(ext, ext));
+    if (!(h > 0.0)) h = 1.0;
+  }
+  std::vector<Seed> seeds;
+  {
+    std::map<Cell3, std::vector<int>> grid;
+    for (const Sol* s : acc) {
+      const Cell3 k = cell3(s->p1, h);
+      bool close = false;
+      for (int a = -1; a <= 1; ++a)
+        for (int b = -1; b <= 1; ++b)
+          for (int c = -1; c <= 1; ++c) {
+            auto it = grid.find(Cell3{k.a + a, k.b + b, k.c + c});
+            if (it == grid.end()) continue;
+            for (int q : it->second) close = close || sqdist(s->p1, seeds[q].p) < h * h;
+          }
+      const double tt = dot(s->t, s->t);
+      if (close || !(tt > 0.0)) continue;
+      Seed sd;
+      std::memcpy(sd.x, s->x, sizeof(sd.x));
+      std::memcpy(sd.p, s->p1, sizeof(sd.p));
+      const double inv = 1.0 / std::sqrt(tt);
+      for (int c = 0; c < 3; ++c) sd.T[c] = s->t[c] * inv;
+      sd.r2 = s->f;
+      grid[k].push_back((int)seeds.size());
+      seeds.push_back(sd);
+    }
+  }
+  const int ns = (int)seeds.size();
+  std::vector<Trace> fwd(ns);
+#pragma omp parallel for schedule(dynamic)
+  for (int s = 0; s < ns; ++s) fwd[s] = march(n1, n2, bx, seeds, s, 1.0, h);
+  std::vector<int> into(ns, 0);
+  for (const Trace& t : fwd)
+    if (t.succ >= 0) ++into[t.succ];
+  std::vector<int> heads;
+  for (int s = 0; s < ns; ++s)
+    if (into[s] == 0) heads.push_back(s);
+  std::vector<Trace> bwd(heads.size());
+#pragma omp parallel for schedule(dynamic)
+  for (long long e = 0; e < (long long)heads.size(); ++e) bwd[e] = march(n1, n2, bx, seeds, heads[e], -1.0, h);
+  std::vector<char> done(ns, 0);
+  int curve = 0;
+  auto put = [&](const double* x, const double* p, double r2, int flags) {
+    out.push_back(jsi_curve_point{p[0], p[1], p[2], x[0], x[1], x[2], x[3], std::sqrt(r2), curve, flags});
+  };
+  auto follow = [&](int s, int flags) {
+    for (; s >= 0 && !done[s]; s = fwd[s].succ) {
+      done[s] = 1;
+      put(seeds[s].x, seeds[s].p, seeds[s].r2, flags);
+      for (const Step& st : fwd[s].pts) put(st.x, st.p, st.r2, flags);
+    }
+  };
+  for (size_t e = 0; e < heads.size(); ++e) {
+    if (done[heads[e]]) continue;
+    for (auto it = bwd[e].pts.rbegin(); it != bwd[e].pts.rend(); ++it) put(it->x, it->p, it->r2, 0);
+    follow(heads[e], 0);
+    ++curve;
+  }
+  for (int s = 0; s < ns; ++s) {
+    if (done[s]) continue;
+    // closed when following the successors from s comes back to s before
+    // meeting a seed already on a curve
+    std::vector<char> mark(ns, 0);
+    int t = s;
+    while (t >= 0 && !done[t] && !mark[t]) {
+      mark[t] = 1;
+      t = fwd[t].succ;
+    }
+    follow(s, t == s ? JSI_CURVE_CLOSED : 0);
+    ++curve;
+  }
+  *ncurves = curve;
+}
+
+// Rounds of a distance query: dist_max_rounds, or with param_tol > 0 the
+// first round whose cells span <= param_tol in u and v on both surfaces.
+int rounds_for(const jsi_params& P, const jsi_region& r1, const jsi_region& r2) {
+  if (!(P.param_tol > 0.0)) return P.dist_max_rounds;
+  double span = 0.0;
+  for (const jsi_region* r : {&r1, &r2}) {
+    span = std::max(span, std::fabs(double(r->ustop) - double(r->ustart)) / double(r->nuv));
+    span = std::max(span, std::fabs(double(r->vstop) - double(r->vstart)) / double(r->nuv));
+  }
+  int n = 1;
+  for (; span > P.param_tol && n < JSI_MAX_ROUNDS; ++n) span = span / double(P.dist_refine_nuv);
+  return n;
+}
+
+struct CurveOut {
+  double spacing = 0.0, tol = 1.0e-9;
+  std::vector<jsi_curve_point> pts;
+  int ncurves = 0;
+};
+
 double now_ms() {
   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
@@ -580,6 +1153,7 @@ int min_distance(const jsi_params& P, const int* iv1, const float* d1, const jsi
   slot2.push_back(&units2.back());
   std::deque<std::vector<Unit<T>>> keep;  // owns refinement units of every round (stable references)
   int nuv_ref = P.dist_refine_nuv;
+  const int max_rounds = rounds_for(P, r1, r2);
   int round = 0;
   std::vector<DPair<T>> pairs;
   int cur_nuv = r1.nuv;
@@ -607,7 +1181,7 @@ int min_distance(const jsi_params& P, const int* iv1, const float* d1, const jsi
     res->distance = pairs.empty() ? 0.0 : double(std::sqrt(pairs.back().d2));
     res->rounds = round;
     res->npairs = (int64_t)pairs.size();
-    if (pairs.empty() || round == P.dist_max_rounds || round >= JSI_MAX_ROUNDS) break;
+    if (pairs.empty() || round == max_rounds || round >= JSI_MAX_ROUNDS) break;
     Dedup u1, u2;
     std::vector<std::pair<int, int>> jobs;
     for (const auto& p : pairs) jobs.push_back({u1.add(p.slot, p.i), u2.add(p.slot, p.j)});
@@ -634,6 +1208,36 @@ int min_distance(const jsi_params& P, const int* iv1, const float* d1, const jsi
     slot2.swap(ns2);
     cur_nuv = nuv_ref;
   }
+  if (P.polish > 0 && !pairs.empty()) {
+    const Net<double> m1 = make_net<double>(iv1, d1), m2 = make_net<double>(iv2, d2);
+    const Boxes bx = boxes_of(r1, r2);
+    const int m = std::min<int>(P.polish, (int)pairs.size());
+    std::vector<Sol> sols(m);
+#pragma omp parallel for schedule(dynamic)
+    for (int e = 0; e < m; ++e) {
+      double x0[4];
+      cell_mid(slot1[pairs[e].slot]->reg, pairs[e].i, x0);
+      cell_mid(slot2[pairs[e].slot]->reg, pairs[e].j, x0 + 2);
+      sols[e] = newton_closest(m1, m2, bx, x0);
+    }
+    int win = 0;
+    for (int e = 1; e < m; ++e)
+      if (sols[e].f < sols[win].f) win = e;
+    jsi_closest& c = res->closest;
+    c.distance = std::sqrt(sols[win].f);
+    c.u1 = sols[win].x[0];
+    c.v1 = sols[win].x[1];
+    c.u2 = sols[win].x[2];
+    c.v2 = sols[win].x[3];
+    for (int q = 0; q < 3; ++q) {
+      c.p1[q] = sols[win].p1[q];
+      c.p2[q] = sols[win].p2[q];
+    }
+    c.candidate = win;
+    c.iterations = sols[win].iters;
+    c.polished = m;
+    c.flags = sols[win].flags;
+  }
   if (out) {
     const int n = std::min<int>(cap, (int)pairs.size());
     for (int e = 0; e < n; ++e) out[e] = jsi_pair{double(pairs[e].d2), pairs[e].slot, pairs[e].i, pairs[e].j, 0};
@@ -645,7 +1249,8 @@ int min_distance(const jsi_params& P, const int* iv1, const float* d1, const jsi
 // intersection, intersection.cu:338-475 (JSI_CAD_PRESERVE_PAIR_RELATION=1).
 template <class T>
 int intersect(const jsi_params& P, const int* iv1, const float* d1, const jsi_region& r1, const int* iv2,
-              const float* d2, const jsi_region& r2, jsi_result* res, jsi_pair* out, int cap) {
+              const float* d2, const jsi_region& r2, jsi_result* res, jsi_pair* out, int cap,
+              CurveOut* co = nullptr) {
   const double t0 = now_ms();
   const Net<T> n1 = make_net<T>(iv1, d1), n2 = make_net<T>(iv2, d2);
   const Region<T> g1 = to_region<T>(r1), g2 = to_region<T>(r2);
@@ -722,6 +1327,19 @@ int intersect(const jsi_params& P, const int* iv1, const float* d1, const jsi_re
     cur_nuv = g_nuv;
   }
   res->isect_refine_nuv = g_nuv;
+  if (co) {
+    const Net<double> m1 = make_net<double>(iv1, d1), m2 = make_net<double>(iv2, d2);
+    const Boxes bx = boxes_of(r1, r2);
+    std::vector<Sol> sols(pairs.size());
+#pragma omp parallel for schedule(dynamic, 256)
+    for (long long e = 0; e < (long long)pairs.size(); ++e) {
+      double x0[4];
+      cell_mid(slot1[pairs[e].slot]->reg, pairs[e].i, x0);
+      cell_mid(slot2[pairs[e].slot]->reg, pairs[e].j, x0 + 2);
+      sols[e] = newton_meet(m1, m2, bx, x0);
+    }
+    trace_curves(m1, m2, bx, sols, co->tol, co->spacing, co->pts, &co->ncurves);
+  }
   if (out) {
     const long long n = std::min<long long>(cap, (long long)pairs.size());
     for (long long e = 0; e < n; ++e) out[e] = jsi_pair{0.0, pairs[e].slot, pairs[e].i, pairs[e].j, 0};
@@ -810,7 +1428,8 @@ double orc_compute_k_f64(const double* pts18, int nuv) { return compute_k(pts18,
 int orc_min_distance(const jsi_params* P, const int* iv1, const float* d1, const jsi_region* r1, const int* iv2,
                      const float* d2, const jsi_region* r2, jsi_result* res, jsi_pair* out, int cap) {
   if (!valid_query(P, iv1, d1, r1, iv2, d2, r2) || !res || P->k < 1 || P->dist_max_rounds < 1 ||
-      P->dist_refine_nuv < 2)
+      P->dist_refine_nuv < 2 || P->polish < 0 || !(P->param_tol >= 0.0) ||
+      (P->param_tol > 0.0 && P->precision == JSI_PREC_EXACT32))
     return JSI_ERR_INVALID;
   std::memset(res, 0, sizeof(*res));
   if (P->precision == JSI_PREC_EXACT32) return min_distance<float>(*P, iv1, d1, *r1, iv2, d2, *r2, res, out, cap);
@@ -824,6 +1443,28 @@ int orc_intersect(const jsi_params* P, const int* iv1, const float* d1, const js
   std::memset(res, 0, sizeof(*res));
   if (P->precision == JSI_PREC_EXACT32) return intersect<float>(*P, iv1, d1, *r1, iv2, d2, *r2, res, out, cap);
   return intersect<double>(*P, iv1, d1, *r1, iv2, d2, *r2, res, out, cap);
+}
+
+int orc_intersect_curves(const jsi_params* P, const int* iv1, const float* d1, const jsi_region* r1,
+                         const int* iv2, const float* d2, const jsi_region* r2, double spacing, double tol,
+                         jsi_result* res, jsi_curve_point* pts, int cap, int* npts, int* ncurves) {
+  if (!valid_query(P, iv1, d1, r1, iv2, d2, r2) || !res || !npts || !ncurves || P->isect_max_rounds < 1 ||
+      P->isect_refine_nuv < 2 || tol < 0.0)
+    return JSI_ERR_INVALID;
+  std::memset(res, 0, sizeof(*res));
+  CurveOut co;
+  co.spacing = spacing;
+  co.tol = tol > 0.0 ? tol : 1.0e-9;
+  int rc;
+  if (P->precision == JSI_PREC_EXACT32)
+    rc = intersect<float>(*P, iv1, d1, *r1, iv2, d2, *r2, res, nullptr, 0, &co);
+  else
+    rc = intersect<double>(*P, iv1, d1, *r1, iv2, d2, *r2, res, nullptr, 0, &co);
+  if (rc != JSI_OK) return rc;
+  *npts = (int)co.pts.size();
+  *ncurves = co.ncurves;
+  if (pts) std::memcpy(pts, co.pts.data(), sizeof(jsi_curve_point) * std::min<size_t>(co.pts.size(), size_t(cap)));
+  return JSI_OK;
 }
 
 int orc_topk_f32(int n1, const float* c1, int n2, const float* c2, int k, jsi_pair* out) {

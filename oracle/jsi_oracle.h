@@ -39,13 +39,21 @@ float orc_compute_k_f32(const float* pts18, int nuv);
 double orc_compute_k_f64(const double* pts18, int nuv);
 
 /* Whole queries.  params->precision selects float (EXACT32) or double (FP64,
- * FP32SCREEN).  pairs_out receives the last round's pair list. */
+ * FP32SCREEN).  pairs_out receives the last round's pair list.  params->polish
+ * and params->param_tol as in jsi_min_distance (FP64 closest points restated). */
 int orc_min_distance(const jsi_params* params, const int* ivec1, const float* data1, const jsi_region* r1,
                      const int* ivec2, const float* data2, const jsi_region* r2, jsi_result* result,
                      jsi_pair* pairs_out, int pairs_cap);
 int orc_intersect(const jsi_params* params, const int* ivec1, const float* data1, const jsi_region* r1,
                   const int* ivec2, const float* data2, const jsi_region* r2, jsi_result* result,
                   jsi_pair* pairs_out, int pairs_cap);
+
+/* Intersection curves: orc_intersect's rounds, then the FP64 projection of
+ * the last round's pairs onto the curve and their ordering (the steps of
+ * jsi_intersect_curves, restated). */
+int orc_intersect_curves(const jsi_params* params, const int* ivec1, const float* data1, const jsi_region* r1,
+                         const int* ivec2, const float* data2, const jsi_region* r2, double spacing, double tol,
+                         jsi_result* result, jsi_curve_point* pts, int cap, int* npts, int* ncurves);
 
 /* Screen only: top-k of one slot of distance centres (distance_analysis.cu:202-338)
  * given as float AoS arrays; for brute-force checks. */

@@ -18,7 +18,8 @@ ROOT = os.path.dirname(HERE)
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
-from jsi_cad_next_versions_amd._abi import Pair, Params, Region, Result, default_params  # noqa: E402
+from jsi_cad_next_versions_amd._abi import (CurvePoint, Pair, Params, Region, Result,  # noqa: E402
+                                            curve_points_to_numpy, default_params)
 
 ORACLE_SO = os.path.join(HERE, "liboracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libjsi_v5_ref.so")
@@ -57,6 +58,8 @@ class Oracle:
              C.c_void_p, C.c_int]
         L.orc_min_distance.argtypes = q
         L.orc_intersect.argtypes = q
+        L.orc_intersect_curves.argtypes = q[:7] + [C.c_double, C.c_double, C.POINTER(Result), C.c_void_p, C.c_int,
+                                                   C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.orc_topk_f32.argtypes = [C.c_int, _f32p, C.c_int, _f32p, C.c_int, C.c_void_p]
         L.orc_num_threads.restype = C.c_int
         L.orc_distance_topk_mask.argtypes = [_f32p, C.c_size_t, _f32p, C.c_size_t, C.c_int, _i32p, _i32p,
@@ -119,6 +122,19 @@ class Oracle:
     def intersect(self, s1, r1, s2, r2, params=None, cap=0):
         params = params or default_params()
         return self._query(self.lib.orc_intersect, s1, r1, s2, r2, params, cap)
+
+    def intersect_curves(self, s1, r1, s2, r2, params=None, spacing=0.0, tol=0.0, cap=1 << 20):
+        """(Result, ordered curve samples as a numpy record array, number of curves)."""
+        params = params or default_params()
+        res = Result()
+        buf = (CurvePoint * max(cap, 1))()
+        n, nc = C.c_int(), C.c_int()
+        rc = self.lib.orc_intersect_curves(C.byref(params), s1.ivec, s1.data, C.byref(r1), s2.ivec, s2.data,
+                                           C.byref(r2), spacing, tol, C.byref(res), C.cast(buf, C.c_void_p), cap,
+                                           C.byref(n), C.byref(nc))
+        if rc != 0:
+            raise RuntimeError(f"oracle query failed: {rc}")
+        return res, curve_points_to_numpy(buf, min(cap, n.value)), nc.value
 
     def distance_topk_mask(self, c1, c2, topk):
         c1 = np.ascontiguousarray(c1, np.float32).reshape(-1)
@@ -311,6 +327,19 @@ class RefHotspots:
         L.jsi_ascend_distance_topk_mask.argtypes = [_f32p, C.c_size_t, _f32p, C.c_size_t, C.c_int, _i32p, _i32p,
                                                     C.POINTER(C.c_float)]
         L.jsi_ascend_intersection_mask.argtypes = [_f32p, _f32p, C.c_int, _f32p, _f32p, C.c_int, _i32p, _i32p]
+
+    def intersect_curves(self, s1, r1, s2, r2, params=None, spacing=0.0, tol=0.0, cap=1 << 20):
+        """(Result, ordered curve samples as a numpy record array, number of curves)."""
+        params = params or default_params()
+        res = Result()
+        buf = (CurvePoint * max(cap, 1))()
+        n, nc = C.c_int(), C.c_int()
+        rc = self.lib.orc_intersect_curves(C.byref(params), s1.ivec, s1.data, C.byref(r1), s2.ivec, s2.data,
+                                           C.byref(r2), spacing, tol, C.byref(res), C.cast(buf, C.c_void_p), cap,
+                                           C.byref(n), C.byref(nc))
+        if rc != 0:
+            raise RuntimeError(f"oracle query failed: {rc}")
+        return res, curve_points_to_numpy(buf, min(cap, n.value)), nc.value
 
     def distance_topk_mask(self, c1, c2, topk):
         c1 = np.ascontiguousarray(c1, np.float32).reshape(-1)

@@ -48,6 +48,6 @@ def test_params_default_matches_version5():
     L.jsi_params_default(ctypes.byref(p))
     d = runtime.default_params()
     for f, _ in runtime.Params._fields_:
-        if f != "reserved":
+        if f != "reserved_":
             assert getattr(p, f) == getattr(d, f), f
     assert L.jsi_status_string(-4) == b"capacity exceeded"
