@@ -2,9 +2,12 @@
 
 Workload: two rational bicubic 64x64 'hard wave' surfaces (weights U[0.8,1.2],
 z = 0.05 sin(16 pi x) sin(16 pi y) and the same wave shifted by pi/7 in x,
-lifted to a ~1e-3 gap), initial grid nuv=512 (261,121 cells per surface,
-6.8e10 cell pairs in round 1), top-k = 4096, refinement nuv = 16, 4 rounds.
-One step = one query.  Inputs are resident in HBM before the timed region.
+lifted so the control nets clear by 1e-3; the surfaces are 2.1e-3 apart),
+initial grid nuv=512 (261,121 cells per surface, 6.8e10 cell pairs in round
+1), top-k = 4096, refinement nuv = 16, 4 rounds, then the FP64 closest-point
+polish of the best 64 records.  One step = one query.  Inputs are resident in
+HBM before the timed region.  The intersection-curve path (saddle / wave
+plane, nuv=512) and the oracle on the host cores are reported beside it.
 
 Prints ONE JSON line (see the keys below).
 """
@@ -131,7 +134,10 @@ def run_served(ctx, q, n, slots, warmup):
     return res, wall, lat
 
 
-def cpu_baseline(a, r1, b, r2, k, queries):
+def cpu_baseline(a, r1, b, r2, k, queries, polish):
+    """The OpenMP oracle on the host cores: EXACT32 queries (the reference's
+    algorithm) timed, plus one FP64 query with the closest-point polish whose
+    distance the GPU forms are checked against."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from jsi_cad_next_versions_amd._abi import default_params
     from oracle import Oracle
@@ -141,9 +147,10 @@ def cpu_baseline(a, r1, b, r2, k, queries):
     for _ in range(queries):
         o.min_distance(a, r1, b, r2, p)
     dt = time.perf_counter() - t0
+    ores, _ = o.min_distance(a, r1, b, r2, default_params(k=k, precision="fp64", polish=polish))
     return {"value": queries / dt, "unit": "queries/s", "cores": o.num_threads(), "kind": "port",
             "sample": f"{queries} full query(ies) of the same workload (nuv={r1.nuv}, k={k}, EXACT32) "
-                      f"on the OpenMP oracle, {dt:.1f} s"}
+                      f"on the OpenMP oracle, {dt:.1f} s"}, ores.closest.distance
 
 
 def main():
@@ -157,6 +164,8 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=20)
     ap.add_argument("--cpu-queries", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--polish", type=int, default=64, help="top-k records polished to FP64 closest points")
+    ap.add_argument("--isect-steps", type=int, default=5, help="intersection-curve queries (saddle / wave plane)")
     ap.add_argument("--concurrency", type=int, default=8, help="worker streams of the served (batched) form")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="also run plain device copies of this many MiB (calibrates the traffic counters)")
@@ -185,7 +194,7 @@ def main():
         hip_copy_calibration(args.calib_copy_mb)
 
     # chosen form: FP32 screening with FP64 re-ranking, tile culling
-    p = default_params(k=args.k, precision=args.precision, cull=1)
+    p = default_params(k=args.k, precision=args.precision, cull=1, polish=args.polish)
     ares, awall, alat = run_form(ctx, a, r1, b, r2, p, args.steps, args.warmup, profile=True)
     stats = {name: ctx.kernel_stats(name) for name in FILTERS}
     stats = {n: v for n, v in stats.items() if v[0] > 0}
@@ -198,14 +207,33 @@ def main():
     eres, ewall, elat = run_form(ctx, a, r1, b, r2, pe, max(1, args.steps // 2), 1)
     # plain FP64 GPU path (the drop-in form the target is quoted against): FP64
     # everywhere, every cell pair of every round evaluated, one query at a time
-    pp = default_params(k=args.k, precision="fp64", cull=0)
+    pp = default_params(k=args.k, precision="fp64", cull=0, polish=args.polish)
     pres, pwall, plat = run_form(ctx, a, r1, b, r2, pp, args.plain_steps, 1)
     pqps = args.plain_steps / pwall
     # the same FP64 path with this build's conservative tile culling (identical results)
-    cp = default_params(k=args.k, precision="fp64", cull=1)
+    cp = default_params(k=args.k, precision="fp64", cull=1, polish=args.polish)
     cres, cwall, clat = run_form(ctx, a, r1, b, r2, cp, args.plain_steps, 1)
     cqps = args.plain_steps / cwall
     dist_err = abs(res.distance - pres.distance)
+    closest_err = abs(res.closest.distance - pres.closest.distance)
+    # intersection curves (run_intersect rounds + FP64 curve tracing): saddle /
+    # wave-plane pair, nuv=512, 6 rounds
+    isect = None
+    if args.isect_steps > 0:
+        sa, sb = S.saddle_wave_pair()
+        q1, q2 = S.full_region(sa, 512, 1), S.full_region(sb, 512, 2)
+        ip = default_params(precision="fp64")
+        ctx.intersection_curves(sa, q1, sb, q2, ip)
+        ctx.sync()
+        ilat = []
+        for _ in range(args.isect_steps):
+            t1 = time.perf_counter()
+            ires, ipts, ncur = ctx.intersection_curves(sa, q1, sb, q2, ip)
+            ilat.append((time.perf_counter() - t1) * 1e3)
+        isect = {"workload": "saddle / wave-plane pair, nuv=512, 6 rounds, FP64, curve tracing",
+                 "queries_per_s": 1e3 / float(np.mean(ilat)), "p50_ms": pct(ilat, 50), "p99_ms": pct(ilat, 99),
+                 "pairs_per_round": [r["npairs"] for r in ires.rounds_list()], "curves": ncur,
+                 "samples": int(len(ipts)), "max_residual": float(ipts["residual"].max()) if len(ipts) else None}
 
     # dominant screen kernel of the chosen form (most summed time), timed with HIP events
     dom = max(stats, key=lambda n: stats[n][1])
@@ -232,7 +260,8 @@ def main():
         "dtype": "f32" if args.precision != "fp64" else "f64",
         "data": "synthetic (seeded SplitMix64 surfaces)",
         "config": {"workload": f"min-distance, rational bicubic 64x64 hard-wave pair, nuv={args.nuv}, "
-                               f"k={args.k}, refine nuv=16, 4 rounds",
+                               f"k={args.k}, refine nuv=16, 4 rounds, FP64 closest-point polish of "
+                               f"{args.polish} records",
                    "precision": args.precision, "cull": 1, "served_streams": args.concurrency},
         "p50_ms": pct(lat, 50),
         "p99_ms": pct(lat, 99),
@@ -241,9 +270,11 @@ def main():
         "cells_per_s": cells * qps,
         "distance": res.distance,
         "dist_err_vs_fp64": dist_err,
+        "closest": {"distance": res.closest.distance, "polished": res.closest.polished,
+                    "err_vs_plain_fp64": closest_err, "flags": res.closest.flags},
         "plain_fp64": {"form": "FP64, every cell pair evaluated, one query at a time",
                        "queries_per_s": pqps, "p50_ms": pct(plat, 50), "p99_ms": pct(plat, 99),
-                       "distance": pres.distance},
+                       "distance": pres.distance, "closest_distance": pres.closest.distance},
         "speedup_vs_plain": qps / pqps if pqps > 0 else None,
         "fp64_culled": {"form": "FP64 with tile culling, one query at a time",
                         "queries_per_s": cqps, "p50_ms": pct(clat, 50), "p99_ms": pct(clat, 99),
@@ -259,8 +290,12 @@ def main():
                      "flop_per_pair": 8},
         "screen_kernels": {n: {"launches": v[0], "ms": v[1], "pairs": v[2]} for n, v in stats.items()},
     }
+    if isect:
+        out["intersection_curves"] = isect
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(a, r1, b, r2, args.k, args.cpu_queries)
+        out["cpu_baseline"], odist = cpu_baseline(a, r1, b, r2, args.k, args.cpu_queries, args.polish)
+        out["closest"]["oracle_fp64_distance"] = odist
+        out["closest"]["err_vs_oracle"] = abs(res.closest.distance - odist)
     ctx.close()
     print(json.dumps(out), flush=True)
 

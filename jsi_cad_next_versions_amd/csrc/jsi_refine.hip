@@ -53,9 +53,9 @@ __host__ __device__ inline void cross3(const double* a, const double* b, double*
 }
 
 // S, Su, Sv, Suu, Suv, Svv at (u, v): EvaluateOneUV (evaluation.cu:259-356)
-// and the quotient rule (:77-128) in double.
+// and the quotient rule (:77-128) in double, any degree <= MAXP.
 template <int MAXP>
-__device__ void eval_point64(const SurfDev<double>& sd, double u, double v, double out[18]) {
+__device__ void eval_generic(const SurfDev<double>& sd, double u, double v, double out[18]) {
 #pragma unroll
   for (int e = 0; e < 18; ++e) out[e] = 0.0;
   if (sd.zero) return;
@@ -119,6 +119,184 @@ __device__ void eval_point64(const SurfDev<double>& sd, double u, double v, doub
     for (int c = 0; c < 3; ++c) out[3 * m + c] = h[m][c];
 }
 
+// A2.3 for a compile-time degree P (derivatives up to min(ORD, P)): the
+// operations of basis_ders with every loop unrolled, so the tables stay in
+// registers; results are bit-identical (the rows of order <= ORD do not
+// depend on the higher ones).
+template <int P, int ORD>
+__device__ __forceinline__ void basis_fixed(int span, double u, const double* U, double (&ders)[3][P + 1]) {
+  constexpr int D = P < ORD ? P : ORD;
+  double ndu[P + 1][P + 1], left[P + 1], right[P + 1], a[2][P + 1];
+  ndu[0][0] = 1.0;
+#pragma unroll
+  for (int j = 1; j <= P; ++j) {
+    left[j] = u - U[span + 1 - j];
+    right[j] = U[span + j] - u;
+    double saved = 0.0;
+#pragma unroll
+    for (int r = 0; r < j; ++r) {
+      ndu[j][r] = right[r + 1] + left[j - r];
+      const double temp = ndu[r][j - 1] / ndu[j][r];
+      ndu[r][j] = saved + right[r + 1] * temp;
+      saved = left[j - r] * temp;
+    }
+    ndu[j][j] = saved;
+  }
+#pragma unroll
+  for (int j = 0; j <= P; ++j) ders[0][j] = ndu[j][P];
+#pragma unroll
+  for (int r = 0; r <= P; ++r) {
+    a[0][0] = 1.0;
+#pragma unroll
+    for (int k = 1; k <= D; ++k) {
+      const int s1 = (k - 1) & 1, s2 = k & 1;  // the row swap of A2.3 after each k
+      double d = 0.0;
+      const int rk = r - k, pk = P - k;
+      if (r >= k) {
+        a[s2][0] = a[s1][0] / ndu[pk + 1][rk];
+        d = a[s2][0] * ndu[rk][pk];
+      }
+      const int j1 = (rk >= -1) ? 1 : -rk;
+      const int j2 = (r - 1 <= pk) ? k - 1 : P - r;
+#pragma unroll
+      for (int j = 1; j <= P; ++j) {
+        if (j < j1 || j > j2) continue;
+        a[s2][j] = (a[s1][j] - a[s1][j - 1]) / ndu[pk + 1][rk + j];
+        d += a[s2][j] * ndu[rk + j][pk];
+      }
+      if (r <= pk) {
+        a[s2][k] = -a[s1][k - 1] / ndu[pk + 1][r];
+        d += a[s2][k] * ndu[r][pk];
+      }
+      ders[k][r] = d;
+    }
+  }
+  int f = P;
+#pragma unroll
+  for (int k = 1; k <= D; ++k) {
+#pragma unroll
+    for (int j = 0; j <= P; ++j) ders[k][j] *= (double)f;
+    f *= P - k;
+  }
+}
+
+// eval_generic for compile-time degrees (PU, PV).  Each control row is read
+// once: its u-sums for the derivative orders k (the same sums EvaluateOneUV
+// forms for every l) feed the accumulators of all (k, l), in the same order.
+template <int PU, int PV, int ORD>
+__device__ __forceinline__ void eval_fixed(const SurfDev<double>& sd, double u, double v, double out[18]) {
+  constexpr int DU = PU < ORD ? PU : ORD, DV = PV < ORD ? PV : ORD;
+  constexpr int NR = ORD == 1 ? 3 : 6;  // output rows: S, Su, Sv (, Suu, Suv, Svv)
+  double Nu[3][PU + 1], Nv[3][PV + 1];
+  const int su = find_span<double>(sd.nku, PU, u, sd.U);
+  basis_fixed<PU, ORD>(su, u, sd.U, Nu);
+  const int sv = find_span<double>(sd.nkv, PV, v, sd.V);
+  basis_fixed<PV, ORD>(sv, v, sd.V, Nv);
+  double acc[NR][4];
+#pragma unroll
+  for (int m = 0; m < NR; ++m)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc[m][c] = 0.0;
+  const bool rat = sd.rational != 0;
+#pragma unroll
+  for (int s = 0; s <= PV; ++s) {
+    const int base = sd.ncpu * (sv - PV + s) + su - PU;
+    double P[PU + 1][4];
+#pragma unroll
+    for (int j = 0; j <= PU; ++j) {
+      P[j][0] = sd.CP[3 * (base + j) + 0];
+      P[j][1] = sd.CP[3 * (base + j) + 1];
+      P[j][2] = sd.CP[3 * (base + j) + 2];
+      P[j][3] = rat ? sd.W[base + j] : 0.0;
+    }
+    double row[DU + 1][4];
+#pragma unroll
+    for (int k = 0; k <= DU; ++k) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) row[k][c] = 0.0;
+#pragma unroll
+      for (int j = 0; j <= PU; ++j) {
+        if (rat) {
+          const double nw = Nu[k][j] * P[j][3];
+          row[k][0] += nw * P[j][0];
+          row[k][1] += nw * P[j][1];
+          row[k][2] += nw * P[j][2];
+          row[k][3] += nw;
+        } else {
+          row[k][0] += Nu[k][j] * P[j][0];
+          row[k][1] += Nu[k][j] * P[j][1];
+          row[k][2] += Nu[k][j] * P[j][2];
+        }
+      }
+    }
+#pragma unroll
+    for (int l = 0; l <= DV; ++l)
+#pragma unroll
+      for (int k = 0; k <= ORD - l; ++k) {
+        if (k > DU) continue;
+        const int raw = (l == 0) ? k : (l == 1 ? 3 + k : 5);
+        const int m = raw == 2 ? 3 : (raw == 3 ? 2 : raw);
+        if (m >= NR) continue;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[m][c] += Nv[l][s] * row[k][c];
+      }
+  }
+  if (rat) {
+    const double F = 1.0 / acc[0][3];
+#pragma unroll
+    for (int m = 0; m < NR; ++m)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) acc[m][c] *= F;
+    const double ws = -acc[1][3], wt = -acc[2][3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      acc[1][c] += ws * acc[0][c];
+      acc[2][c] += wt * acc[0][c];
+    }
+    if constexpr (NR == 6) {
+      const double wss = -acc[3][3], wst = -acc[4][3], wtt = -acc[5][3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const double S = acc[0][c], Su = acc[1][c], Sv = acc[2][c];
+        acc[3][c] += wss * S + 2.0 * ws * Su;
+        acc[4][c] += wst * S + wt * Su + ws * Sv;
+        acc[5][c] += wtt * S + 2.0 * wt * Sv;
+      }
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < NR; ++m)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) out[3 * m + c] = acc[m][c];
+}
+
+// Degrees 2 and 3 in each direction take the register-resident evaluator
+// (FAST: the host checked that both surfaces are of these degrees); others the
+// generic one.
+__host__ __device__ inline bool fast_degrees(const SurfDev<double>& sd) {
+  return !sd.zero && sd.degu >= 2 && sd.degu <= 3 && sd.degv >= 2 && sd.degv <= 3;
+}
+// ORD = 1: only S, Su, Sv (out[0..8]) are written on the register path (the
+// generic path writes all 18).
+template <int MAXP, bool FAST, int ORD = 2>
+__device__ __forceinline__ void eval_point64(const SurfDev<double>& sd, double u, double v, double out[18]) {
+  if (FAST) {
+    if (sd.degu == 3) {
+      if (sd.degv == 3)
+        eval_fixed<3, 3, ORD>(sd, u, v, out);
+      else
+        eval_fixed<3, 2, ORD>(sd, u, v, out);
+    } else {
+      if (sd.degv == 3)
+        eval_fixed<2, 3, ORD>(sd, u, v, out);
+      else
+        eval_fixed<2, 2, ORD>(sd, u, v, out);
+    }
+  } else {
+    eval_generic<MAXP>(sd, u, v, out);
+  }
+}
+
 // Solves M z = r for symmetric positive definite M (lower triangle used) by
 // LDL^T without square roots; false when a pivot is not positive and finite.
 template <int N>
@@ -164,7 +342,7 @@ __device__ inline void cell_centre(const vec4_t<TR>& R, int nuv, int cell, doubl
   o[1] = (double)R.y + vs * ((double)iv + 0.5);
 }
 
-template <class TR, int MAXP>
+template <class TR, int MAXP, bool FAST>
 __global__ __launch_bounds__(64) void k_polish(SurfDev<double> s1, SurfDev<double> s2, const Cand* rec, int m,
                                                const int* su1, const int* su2, const vec4_t<TR>* reg1,
                                                const vec4_t<TR>* reg2, int nuv1, int nuv2, double4 box1,
@@ -180,8 +358,8 @@ __global__ __launch_bounds__(64) void k_polish(SurfDev<double> s1, SurfDev<doubl
   cell_centre<TR>(reg2[su2[slot]], nuv2, cj, x + 2);
   for (int q = 0; q < 4; ++q) x[q] = clampd(x[q], lo[q], hi[q]);
   double E1[18], E2[18], F1[18], F2[18];
-  eval_point64<MAXP>(s1, x[0], x[1], E1);
-  eval_point64<MAXP>(s2, x[2], x[3], E2);
+  eval_point64<MAXP, FAST>(s1, x[0], x[1], E1);
+  eval_point64<MAXP, FAST>(s2, x[2], x[3], E2);
   double d[3] = {E1[0] - E2[0], E1[1] - E2[1], E1[2] - E2[2]};
   double f = dot3(d, d);
   double mu = kMuStart;
@@ -240,6 +418,16 @@ __global__ __launch_bounds__(64) void k_polish(SurfDev<double> s1, SurfDev<doubl
       }
       continue;
     }
+    // a step below the rounding of the parameters ends the iteration
+    double zm = 0.0, xm0 = 0.0;
+    for (int q = 0; q < 4; ++q) {
+      zm = dmax(zm, fabs(z[q]));
+      xm0 = dmax(xm0, fabs(x[q]));
+    }
+    if (zm <= 1.0e-15 * (1.0 + xm0)) {
+      flags |= JSI_POLISH_CONVERGED;
+      break;
+    }
     bool same = true;
     for (int q = 0; q < 4; ++q) {
       xn[q] = clampd(x[q] - z[q], lo[q], hi[q]);
@@ -249,8 +437,8 @@ __global__ __launch_bounds__(64) void k_polish(SurfDev<double> s1, SurfDev<doubl
       flags |= JSI_POLISH_FIXED;
       break;
     }
-    eval_point64<MAXP>(s1, xn[0], xn[1], F1);
-    eval_point64<MAXP>(s2, xn[2], xn[3], F2);
+    eval_point64<MAXP, FAST>(s1, xn[0], xn[1], F1);
+    eval_point64<MAXP, FAST>(s2, xn[2], xn[3], F2);
     const double dn[3] = {F1[0] - F2[0], F1[1] - F2[1], F1[2] - F2[2]};
     const double fn = dot3(dn, dn);
     if (fn < f) {
@@ -292,7 +480,7 @@ __global__ __launch_bounds__(64) void k_polish(SurfDev<double> s1, SurfDev<doubl
   out[e] = o;
 }
 
-template <class TR, int MAXP>
+template <class TR, int MAXP, bool FAST>
 __global__ __launch_bounds__(64) void k_curve_proj(SurfDev<double> s1, SurfDev<double> s2, const IPair* pairs,
                                                    long long n, const int* su1, const int* su2,
                                                    const vec4_t<TR>* reg1, const vec4_t<TR>* reg2, int nuv1,
@@ -307,8 +495,8 @@ __global__ __launch_bounds__(64) void k_curve_proj(SurfDev<double> s1, SurfDev<d
   cell_centre<TR>(reg2[su2[pr.slot]], nuv2, pr.j, x + 2);
   for (int q = 0; q < 4; ++q) x[q] = clampd(x[q], lo[q], hi[q]);
   double E1[18], E2[18];
-  eval_point64<MAXP>(s1, x[0], x[1], E1);
-  eval_point64<MAXP>(s2, x[2], x[3], E2);
+  eval_point64<MAXP, FAST, 1>(s1, x[0], x[1], E1);
+  eval_point64<MAXP, FAST, 1>(s2, x[2], x[3], E2);
   double d[3] = {E1[0] - E2[0], E1[1] - E2[1], E1[2] - E2[2]};
   double r2 = dot3(d, d);
   const double sc = 1.0 + dmax(dmax(fabs(E1[0]), fabs(E1[1])), fabs(E1[2]));
@@ -346,8 +534,8 @@ __global__ __launch_bounds__(64) void k_curve_proj(SurfDev<double> s1, SurfDev<d
       break;
     }
     for (int q = 0; q < 4; ++q) x[q] = xn[q];
-    eval_point64<MAXP>(s1, x[0], x[1], E1);
-    eval_point64<MAXP>(s2, x[2], x[3], E2);
+    eval_point64<MAXP, FAST, 1>(s1, x[0], x[1], E1);
+    eval_point64<MAXP, FAST, 1>(s2, x[2], x[3], E2);
     for (int t = 0; t < 3; ++t) d[t] = E1[t] - E2[t];
     r2 = dot3(d, d);
   }
@@ -364,23 +552,27 @@ __global__ __launch_bounds__(64) void k_curve_proj(SurfDev<double> s1, SurfDev<d
   out[e] = o;
 }
 
-template <int MAXP>
-struct PolishLaunch {
-  template <class TR>
-  static hipError_t run(dim3 g, hipStream_t s, const SurfDev<double>& s1, const SurfDev<double>& s2, const Cand* rec,
-                        int m, const int* su1, const int* su2, const vec4_t<TR>* reg1, const vec4_t<TR>* reg2,
-                        int nuv1, int nuv2, double4 box1, double4 box2, PolishRec* out) {
-    hipLaunchKernelGGL((k_polish<TR, MAXP>), g, dim3(64), 0, s, s1, s2, rec, m, su1, su2, reg1, reg2, nuv1, nuv2,
-                       box1, box2, out);
-    return hipGetLastError();
-  }
-};
-
 inline int max_degree(const SurfDev<double>& a, const SurfDev<double>& b) {
   return std::max(std::max(a.degu, a.degv), std::max(b.degu, b.degv));
 }
 
 }  // namespace
+
+// Kernel variant of a surface pair: 0 = both of degrees 2-3 (register
+// evaluator), else the generic evaluator sized for the highest degree.
+static int variant(const SurfDev<double>& s1, const SurfDev<double>& s2) {
+  if (fast_degrees(s1) && fast_degrees(s2)) return 0;
+  const int md = max_degree(s1, s2);
+  return md <= 3 ? 1 : (md <= 7 ? 2 : 3);
+}
+
+#define JSI_REFINE_DISPATCH(KERNEL, ...)                                   \
+  switch (variant(s1, s2)) {                                               \
+    case 0: hipLaunchKernelGGL((KERNEL<3, true>), __VA_ARGS__); break;     \
+    case 1: hipLaunchKernelGGL((KERNEL<3, false>), __VA_ARGS__); break;    \
+    case 2: hipLaunchKernelGGL((KERNEL<7, false>), __VA_ARGS__); break;    \
+    default: hipLaunchKernelGGL((KERNEL<kMaxDegP, false>), __VA_ARGS__); break; \
+  }
 
 template <class TR>
 hipError_t launch_polish(const SurfDev<double>& s1, const SurfDev<double>& s2, const Cand* rec, int m, const int* su1,
@@ -388,10 +580,24 @@ hipError_t launch_polish(const SurfDev<double>& s1, const SurfDev<double>& s2, c
                          double4 box1, double4 box2, PolishRec* out, hipStream_t s) {
   if (m <= 0) return hipSuccess;
   const dim3 g((unsigned)((m + 63) / 64));
-  const int md = max_degree(s1, s2);
-  if (md <= 3) return PolishLaunch<3>::run<TR>(g, s, s1, s2, rec, m, su1, su2, reg1, reg2, nuv1, nuv2, box1, box2, out);
-  if (md <= 7) return PolishLaunch<7>::run<TR>(g, s, s1, s2, rec, m, su1, su2, reg1, reg2, nuv1, nuv2, box1, box2, out);
-  return PolishLaunch<kMaxDegP>::run<TR>(g, s, s1, s2, rec, m, su1, su2, reg1, reg2, nuv1, nuv2, box1, box2, out);
+  switch (variant(s1, s2)) {
+    case 0:
+      hipLaunchKernelGGL((k_polish<TR, 3, true>), g, dim3(64), 0, s, s1, s2, rec, m, su1, su2, reg1, reg2, nuv1, nuv2,
+                         box1, box2, out);
+      break;
+    case 1:
+      hipLaunchKernelGGL((k_polish<TR, 3, false>), g, dim3(64), 0, s, s1, s2, rec, m, su1, su2, reg1, reg2, nuv1,
+                         nuv2, box1, box2, out);
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_polish<TR, 7, false>), g, dim3(64), 0, s, s1, s2, rec, m, su1, su2, reg1, reg2, nuv1,
+                         nuv2, box1, box2, out);
+      break;
+    default:
+      hipLaunchKernelGGL((k_polish<TR, kMaxDegP, false>), g, dim3(64), 0, s, s1, s2, rec, m, su1, su2, reg1, reg2,
+                         nuv1, nuv2, box1, box2, out);
+  }
+  return hipGetLastError();
 }
 
 template <class TR>
@@ -400,16 +606,23 @@ hipError_t launch_curve_proj(const SurfDev<double>& s1, const SurfDev<double>& s
                              int nuv2, double4 box1, double4 box2, CurveRec* out, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   const dim3 g((unsigned)((n + 63) / 64));
-  const int md = max_degree(s1, s2);
-  if (md <= 3)
-    hipLaunchKernelGGL((k_curve_proj<TR, 3>), g, dim3(64), 0, s, s1, s2, pairs, n, su1, su2, reg1, reg2, nuv1, nuv2,
-                       box1, box2, out);
-  else if (md <= 7)
-    hipLaunchKernelGGL((k_curve_proj<TR, 7>), g, dim3(64), 0, s, s1, s2, pairs, n, su1, su2, reg1, reg2, nuv1, nuv2,
-                       box1, box2, out);
-  else
-    hipLaunchKernelGGL((k_curve_proj<TR, kMaxDegP>), g, dim3(64), 0, s, s1, s2, pairs, n, su1, su2, reg1, reg2, nuv1,
-                       nuv2, box1, box2, out);
+  switch (variant(s1, s2)) {
+    case 0:
+      hipLaunchKernelGGL((k_curve_proj<TR, 3, true>), g, dim3(64), 0, s, s1, s2, pairs, n, su1, su2, reg1, reg2, nuv1,
+                         nuv2, box1, box2, out);
+      break;
+    case 1:
+      hipLaunchKernelGGL((k_curve_proj<TR, 3, false>), g, dim3(64), 0, s, s1, s2, pairs, n, su1, su2, reg1, reg2,
+                         nuv1, nuv2, box1, box2, out);
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_curve_proj<TR, 7, false>), g, dim3(64), 0, s, s1, s2, pairs, n, su1, su2, reg1, reg2,
+                         nuv1, nuv2, box1, box2, out);
+      break;
+    default:
+      hipLaunchKernelGGL((k_curve_proj<TR, kMaxDegP, false>), g, dim3(64), 0, s, s1, s2, pairs, n, su1, su2, reg1,
+                         reg2, nuv1, nuv2, box1, box2, out);
+  }
   return hipGetLastError();
 }
 
@@ -465,8 +678,8 @@ constexpr int kCorrIters = 20;
 constexpr int kMarchCap = 8192;
 enum { kCorrConv = 0, kCorrFail = 1, kCorrLeft = 2, kCorrNoConv = 3 };
 
-template <int MAXP>
-__device__ int correct(const SurfDev<double>& s1, const SurfDev<double>& s2, double* x, double* E1, double* E2,
+template <int MAXP, bool FAST>
+__device__ __forceinline__ int correct(const SurfDev<double>& s1, const SurfDev<double>& s2, double* x, double* E1, double* E2,
                        const double* lo, const double* hi, int freem, bool plane, const double* T, const double* p0,
                        double h, double tol2) {
   for (int it = 0;; ++it) {
@@ -519,8 +732,8 @@ __device__ int correct(const SurfDev<double>& s1, const SurfDev<double>& s2, dou
       x[q] = xn;
     }
     if (left) return kCorrLeft;
-    eval_point64<MAXP>(s1, x[0], x[1], E1);
-    eval_point64<MAXP>(s2, x[2], x[3], E2);
+    eval_point64<MAXP, FAST, 1>(s1, x[0], x[1], E1);
+    eval_point64<MAXP, FAST, 1>(s2, x[2], x[3], E2);
   }
 }
 
@@ -548,7 +761,7 @@ __device__ inline int clamp_fix(double* x, const double* lo, const double* hi) {
 // One thread per march.  Counting pass (pts == nullptr): out[k] receives the
 // successor, the number of recorded points and the end flags; writing pass:
 // the same march again, recording its points at pts + offs[k].
-template <int MAXP>
+template <int MAXP, bool FAST>
 __global__ __launch_bounds__(64) void k_march(SurfDev<double> s1, SurfDev<double> s2, const MarchSeed* seeds,
                                               int nseeds, const int* starts, int nstart, double dir, double h,
                                               double4 box1, double4 box2, const long long* offs, MarchPt* pts,
@@ -566,8 +779,8 @@ __global__ __launch_bounds__(64) void k_march(SurfDev<double> s1, SurfDev<double
     p[c] = sd.p[c];
   }
   double E1[18], E2[18];
-  eval_point64<MAXP>(s1, x[0], x[1], E1);
-  eval_point64<MAXP>(s2, x[2], x[3], E2);
+  eval_point64<MAXP, FAST, 1>(s1, x[0], x[1], E1);
+  eval_point64<MAXP, FAST, 1>(s2, x[2], x[3], E2);
   const double sc = 1.0 + dmax(dmax(fabs(p[0]), fabs(p[1])), fabs(p[2]));
   const double tl = 1.0e-14 * sc;
   const double tol2 = tl * tl;
@@ -602,15 +815,15 @@ __global__ __launch_bounds__(64) void k_march(SurfDev<double> s1, SurfDev<double
     x[2] = x[2] + y2[0];
     x[3] = x[3] + y2[1];
     (void)clamp_fix(x, lo, hi);
-    eval_point64<MAXP>(s1, x[0], x[1], E1);
-    eval_point64<MAXP>(s2, x[2], x[3], E2);
+    eval_point64<MAXP, FAST, 1>(s1, x[0], x[1], E1);
+    eval_point64<MAXP, FAST, 1>(s2, x[2], x[3], E2);
     int fixed = 0;
-    const int st = correct<MAXP>(s1, s2, x, E1, E2, lo, hi, 15, true, T, p, h, tol2);
+    const int st = correct<MAXP, FAST>(s1, s2, x, E1, E2, lo, hi, 15, true, T, p, h, tol2);
     if (st == kCorrLeft) {
       fixed = clamp_fix(x, lo, hi);
-      eval_point64<MAXP>(s1, x[0], x[1], E1);
-      eval_point64<MAXP>(s2, x[2], x[3], E2);
-      if (correct<MAXP>(s1, s2, x, E1, E2, lo, hi, 15 & ~fixed, false, T, p, h, tol2) != kCorrConv) {
+      eval_point64<MAXP, FAST, 1>(s1, x[0], x[1], E1);
+      eval_point64<MAXP, FAST, 1>(s2, x[2], x[3], E2);
+      if (correct<MAXP, FAST>(s1, s2, x, E1, E2, lo, hi, 15 & ~fixed, false, T, p, h, tol2) != kCorrConv) {
         flags |= JSI_MARCH_FAILED;
         break;
       }
@@ -678,16 +891,8 @@ hipError_t launch_march(const SurfDev<double>& s1, const SurfDev<double>& s2, co
                         const long long* offs, MarchPt* pts, MarchOut* out, hipStream_t s) {
   if (nstart <= 0) return hipSuccess;
   const dim3 g((unsigned)((nstart + 63) / 64));
-  const int md = max_degree(s1, s2);
-  if (md <= 3)
-    hipLaunchKernelGGL((k_march<3>), g, dim3(64), 0, s, s1, s2, seeds, nseeds, starts, nstart, dir, h, box1, box2,
-                       offs, pts, out);
-  else if (md <= 7)
-    hipLaunchKernelGGL((k_march<7>), g, dim3(64), 0, s, s1, s2, seeds, nseeds, starts, nstart, dir, h, box1, box2,
-                       offs, pts, out);
-  else
-    hipLaunchKernelGGL((k_march<kMaxDegP>), g, dim3(64), 0, s, s1, s2, seeds, nseeds, starts, nstart, dir, h, box1,
-                       box2, offs, pts, out);
+  JSI_REFINE_DISPATCH(k_march, g, dim3(64), 0, s, s1, s2, seeds, nseeds, starts, nstart, dir, h, box1, box2, offs,
+                      pts, out)
   return hipGetLastError();
 }
 

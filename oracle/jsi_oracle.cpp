@@ -698,6 +698,15 @@ Sol newton_closest(const Net<double>& n1, const Net<double>& n2, const Boxes& bx
       }
       continue;
     }
+    double zbig = 0.0, xbig = 0.0;
+    for (int q = 0; q < 4; ++q) {
+      zbig = bigger(zbig, std::fabs(z[q]));
+      xbig = bigger(xbig, std::fabs(x[q]));
+    }
+    if (zbig <= 1.0e-15 * (1.0 + xbig)) {  // step below the parameters' rounding
+      r.flags |= JSI_POLISH_CONVERGED;
+      break;
+    }
     bool moved = false;
     for (int q = 0; q < 4; ++q) {
       y[q] = clip(x[q] - z[q], bx.lo[q], bx.hi[q]);
