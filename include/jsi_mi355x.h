@@ -108,7 +108,8 @@ typedef struct jsi_closest {
 #define JSI_POLISH_STATIONARY 1 /* gradient exactly zero */
 #define JSI_POLISH_STALLED 2    /* damping exceeded its cap without a decrease */
 #define JSI_POLISH_FIXED 4      /* the projected step left the parameters unchanged */
-#define JSI_POLISH_CONVERGED 8  /* Newton step <= 1e-15 (1 + max |parameter|) */
+#define JSI_POLISH_CONVERGED 8  /* Newton step <= 1e-15 (1 + max |parameter|), or a step
+                                   <= 1e-9 (1 + max |parameter|) that does not decrease f */
 
 typedef struct jsi_result {
   double distance;       /* distance: sqrt(k-th smallest centre d2) of the last round */

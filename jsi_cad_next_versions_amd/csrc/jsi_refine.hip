@@ -461,6 +461,12 @@ __global__ __launch_bounds__(64) void k_polish(SurfDev<double> s1, SurfDev<doubl
         break;
       }
     } else {
+      // f no longer decreases: a rejected step this small means the minimum
+      // is resolved to the precision f allows (~sqrt(eps) in the parameters)
+      if (zm <= 1.0e-9 * (1.0 + xm0)) {
+        flags |= JSI_POLISH_CONVERGED;
+        break;
+      }
       mu = mu * 10.0;
       if (mu > kMuMax) {
         flags |= JSI_POLISH_STALLED;

@@ -721,6 +721,10 @@ Sol newton_closest(const Net<double>& n1, const Net<double>& n2, const Boxes& bx
     const double dn[3] = {An[0] - Bn[0], An[1] - Bn[1], An[2] - Bn[2]};
     const double fn = dot(dn, dn);
     if (!(fn < f)) {
+      if (zbig <= 1.0e-9 * (1.0 + xbig)) {  // no decrease from a step at f's resolution
+        r.flags |= JSI_POLISH_CONVERGED;
+        break;
+      }
       mu = mu * 10.0;
       if (mu > 1.0e12) {
         r.flags |= JSI_POLISH_STALLED;
