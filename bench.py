@@ -28,9 +28,9 @@ FILTERS = {  # kernel statistics kept by the library -> kernel name
     "dist_filter_tiles_f32": "k_dist_filter<float> (tile pairs)",
     "dist_filter_tiles_f64": "k_dist_filter<double> (tile pairs)",
     "dist_filter_units_f32": "k_dist_filter_units<float> (one tile per unit)",
-    "dist_filter_units_f64": "k_dist_filter_units<double> (one tile per unit)",
+    "dist_filter_units_f64": "k_dist_filter_mfma (FP64, one tile per unit, v_mfma_f64_16x16x4f64)",
 }
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_k_dist_filter_units_hardwave_nuv512_k4096.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_k_dist_filter_mfma_hardwave_nuv512_k4096.json")
 
 
 def hip_copy_calibration(mbytes, repeats=3):
@@ -134,6 +134,64 @@ def run_served(ctx, q, n, slots, warmup):
     return res, wall, lat
 
 
+def uneven_queries(ctx, n, k, polish, seed0=0):
+    """n queries of the uneven workload: random_pair's distribution (70%
+    distance / 30% intersection, degrees 2..5, nets 8..96, weights U[0.7,1.3],
+    nuv in {64, 128, 256, 512})."""
+    from jsi_cad_next_versions_amd import surfaces as S
+    from jsi_cad_next_versions_amd._abi import QUERY_DISTANCE, QUERY_INTERSECT, default_params
+    qs, cells = [], 0
+    for i in range(n):
+        kind = "isect" if (i * 7 + seed0) % 10 < 3 else "dist"
+        a, b, nuv = S.random_pair(seed0 + i, kind)
+        r1, r2 = S.full_region(a, nuv, 1), S.full_region(b, nuv, 2)
+        if kind == "dist":
+            q = ctx.query(QUERY_DISTANCE, a, r1, b, r2, default_params(k=k, precision="fp32screen", polish=polish))
+        else:
+            q = ctx.query(QUERY_INTERSECT, a, r1, b, r2, default_params(precision="fp64"))
+        qs.append(q)
+        cells += 2 * (nuv - 1) ** 2
+    return qs, cells
+
+
+def run_poisson(ctx, queries, rate, seconds, slots, seed=5):
+    """Open loop: request i arrives at t_i (exponential gaps at `rate`), is
+    submitted when due (re-tried while admission refuses it) and its latency
+    is result time - arrival time."""
+    from collections import deque
+
+    from jsi_cad_next_versions_amd.runtime import Server
+    rng = np.random.default_rng(seed)
+    n = max(1, int(rate * seconds))
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n))
+    lat, pending, waiting = [], {}, deque()
+    refused = 0
+    with Server(ctx, slots=slots) as srv:
+        for t in [srv.submit(q) for q in queries[:slots]]:  # warm-up: worker buffers
+            srv.poll(t)
+        i = 0
+        t0 = time.perf_counter()
+        while i < n or pending or waiting:
+            now = time.perf_counter() - t0
+            while i < n and arrivals[i] <= now:
+                waiting.append((queries[i % len(queries)], arrivals[i]))
+                i += 1
+            while waiting:
+                tk = srv.submit(waiting[0][0])
+                if tk is None:
+                    refused += 1
+                    break
+                pending[tk] = waiting.popleft()[1]
+            for tk in list(pending):
+                if srv.poll(tk, wait=False) is not None:
+                    lat.append((time.perf_counter() - t0 - pending.pop(tk)) * 1e3)
+            time.sleep(5e-5)
+        wall = time.perf_counter() - t0
+    return {"offered_rate": rate, "requests": n, "queries_per_s": n / wall, "p50_ms": pct(lat, 50),
+            "p99_ms": pct(lat, 99), "max_ms": max(lat) if lat else None, "admission_retries": refused,
+            "slots": slots}
+
+
 def cpu_baseline(a, r1, b, r2, k, queries, polish):
     """The OpenMP oracle on the host cores: EXACT32 queries (the reference's
     algorithm) timed, plus one FP64 query with the closest-point polish whose
@@ -166,6 +224,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--polish", type=int, default=64, help="top-k records polished to FP64 closest points")
     ap.add_argument("--isect-steps", type=int, default=5, help="intersection-curve queries (saddle / wave plane)")
+    ap.add_argument("--batch", type=int, default=256, help="queries of the uneven batch (0 = skip)")
+    ap.add_argument("--serve-rate", type=float, default=1000.0, help="Poisson arrival rate of the serving run")
+    ap.add_argument("--serve-seconds", type=float, default=2.0, help="arrival window of the serving run (0 = skip)")
     ap.add_argument("--concurrency", type=int, default=8, help="worker streams of the served (batched) form")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="also run plain device copies of this many MiB (calibrates the traffic counters)")
@@ -174,7 +235,7 @@ def main():
     args = ap.parse_args()
     if args.pmc_summary:
         pmc_summary(args.pmc_summary[0], args.pmc_summary[1], args.calib_copy_mb or 256,
-                    "k_dist_filter_units<double", TRAFFIC_FILE)
+                    "k_dist_filter_mfma", TRAFFIC_FILE)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each GPU would serve its own "
@@ -292,6 +353,30 @@ def main():
     }
     if isect:
         out["intersection_curves"] = isect
+    if args.batch > 0:
+        # uneven batch: many surface pairs of different degree, net size and nuv in flight together
+        bq, bcells = uneven_queries(ctx, args.batch, 512, args.polish)
+        ctx.run_batch(bq[: min(16, len(bq))], concurrency=args.concurrency)  # warm-up
+        t1 = time.perf_counter()
+        bres, bst = ctx.run_batch(bq, concurrency=args.concurrency)
+        bwall = time.perf_counter() - t1
+        out["batch_uneven"] = {
+            "workload": f"{args.batch} pairs, 70% distance (FP32SCREEN, k=512, polish {args.polish}) / 30% "
+                        "intersection (FP64, 6 rounds); degrees 2-5, nets 8-96, weights U[0.7,1.3], nuv 64-512",
+            "queries_per_s": args.batch / bwall, "cells_per_s": bcells / bwall, "seconds": bwall,
+            "failed": int(sum(1 for x in bst if x != 0)), "concurrency": args.concurrency}
+    if args.serve_seconds > 0:
+        # serving: Poisson arrivals of the uneven mix plus 10% 4x-scaled hard-wave pairs at nuv=1024
+        sq, _ = uneven_queries(ctx, 90, 512, args.polish, seed0=10000)
+        ha, hb = S.hard_wave_pair(scale=4.0)
+        hq = ctx.query(0, ha, S.full_region(ha, 1024, 1), hb, S.full_region(hb, 1024, 2),
+                       default_params(k=args.k, precision="fp32screen", cull=1, polish=args.polish))
+        mix = []
+        for i in range(100):
+            mix.append(hq if i % 10 == 9 else sq[i - i // 10])
+        out["serving_poisson"] = run_poisson(ctx, mix, args.serve_rate, args.serve_seconds, args.concurrency)
+        out["serving_poisson"]["workload"] = ("uneven mix + 10% 4x-scaled hard-wave pairs (nuv=1024, k=4096), "
+                                              "Poisson arrivals, latency = arrival -> result")
     if not args.no_cpu:
         out["cpu_baseline"], odist = cpu_baseline(a, r1, b, r2, args.k, args.cpu_queries, args.polish)
         out["closest"]["oracle_fp64_distance"] = odist

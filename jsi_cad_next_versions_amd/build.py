@@ -10,9 +10,12 @@ SOURCES = ["jsi_kernels.hip", "jsi_runtime.hip", "jsi_legacy.hip", "jsi_opabi.hi
 
 # -ffp-contract=off: no fused multiply-add, so float results match VERSION5
 # (built without FMA on x86-64) and double results match the FP64 oracle.
-# Packed FP32 VALU ops are disabled (a known-bad instruction family on this pool).
+# Packed FP32 VALU ops are disabled: some of their forms have given wrong results.
+# -amdgpu-mfma-vgpr-form: matrix-core results go to ordinary VGPRs, so the
+# comparisons read them without accumulator-register copies.
 HIPFLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math",
-            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wno-unused-command-line-argument"]
+            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-mllvm", "-amdgpu-mfma-vgpr-form",
+            "-Wno-unused-command-line-argument"]
 
 
 def build(verbose=False, jobs=3):

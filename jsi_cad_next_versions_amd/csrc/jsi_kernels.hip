@@ -1343,6 +1343,205 @@ __global__ __launch_bounds__(256) void k_dist_filter_units(const uint64_t* list,
   if (pairs_done && threadIdx.x == 0 && done) atomicAdd(pairs_done, done);
 }
 
+// Distance filter of the FP64 refinement rounds on the matrix cores.  The
+// expanded test value s = a.(-2b) + |b|^2 (see k_dist_filter) of a cell pair is
+// the product of the side-2 row [-2bx -2by -2bz |b|^2] and the side-1 column
+// [ax ay az 1], so one v_mfma_f64_16x16x4f64 forms the values of a 16 x 16
+// block of cell pairs (8 FLOP per pair, all on the matrix core); the vector
+// ALU only compares them with the thresholds T - |a|^2 + delta, which depend on
+// the column.  Operand layouts of the instruction: A lane l = (row l%16, k
+// l/16), B lane l = (k l/16, column l%16), result register r of lane l = (row
+// 4r + l/16, column l%16): a lane's results share its column, hence one
+// threshold per lane and side-1 block, and "any pass" is min(results) <= thr
+// (15 minima and one compare for 64 pairs per lane).  One workgroup per slot:
+// the side-1 B operands and thresholds are staged in LDS (16 blocks of 16
+// cells), wavefront w keeps the A operands of side-2 blocks w, w+4, w+8, w+12
+// in registers and, for each side-1 block, issues its four products before
+// testing them.  Padded cells carry NaN, which never passes (and min ignores).
+// delta covers the rounding of a 4-term dot product in any order (at most
+// 36 u m^2 here) and the other terms of the expanded test (the bound is
+// checked by the screen-bound tests).  A passing pair's key is the reference
+// d2 recomputed from a and b, so the records equal the direct test's.
+typedef double mfma_v4d __attribute__((ext_vector_type(4)));
+
+// v_min_f64 on values that are finite or quiet NaN (products of the matrix
+// core): the smaller operand, or the other one when one is NaN.  Written as the
+// instruction because fmin() canonicalises both operands first (two extra
+// instructions per minimum) to quiet signalling NaNs, which cannot occur here.
+__device__ __forceinline__ double min_nan(double a, double b) {
+  double r;
+  asm("v_min_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+
+__global__ __launch_bounds__(256) void k_dist_filter_mfma(const uint64_t* list, const unsigned* list_count,
+                                                          int nslots, const int* su1, const int* su2, int w1, int w2,
+                                                          const double4* C1, const double4* C2, const double* thresh,
+                                                          Cand* out, unsigned long long* out_count,
+                                                          unsigned long long cap, unsigned long long* pairs_done,
+                                                          const unsigned long long* absmax) {
+  constexpr int kStage = 128;
+  __shared__ double sB[17][64];  // one padding block: the prefetch past the last block stays in bounds
+  __shared__ double sT[17 * 16];
+  __shared__ double sA[16][64];  // side-2 A operands (read by the rare hit path)
+  __shared__ Cand stage[4][kStage];
+  unsigned long long done = 0;
+  const unsigned n = *list_count;
+  const double tv = *thresh;
+  const int lane = lane_id();
+  const int wv = threadIdx.x >> 6;
+  const int col = lane & 15, kq = lane >> 4;
+  unsigned long long mb = absmax[lane];  // kAmaxSlots == wavefront size
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long o = __shfl_xor(mb, off);
+    mb = o > mb ? o : mb;
+  }
+  const double mm = __longlong_as_double((long long)mb);
+  const double delta = 32.0 * 1.1102230246251565e-16 * (tv + 4.0 * mm * mm);
+  const int nA = w1 * w1, nB = w2 * w2;
+  const int nAb = (nA + 15) >> 4, nBb = (nB + 15) >> 4;  // side-1 (columns) and side-2 (rows) blocks
+  const double qnan = quiet_nan<double>();
+  int wcount = 0;
+  auto flush = [&]() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    unsigned long long base = 0;
+    if (lane == 0) base = atomicAdd(out_count, (unsigned long long)wcount);
+    base = __shfl(base, 0);
+    for (int e = lane; e < wcount; e += 64)
+      if (base + e < cap) out[base + e] = stage[wv][e];
+    __builtin_amdgcn_wave_barrier();
+    wcount = 0;
+  };
+  // The workgroup walks slots it = blockIdx.x + k gridDim.x (at most kMaxWalk of
+  // them): their unit ids are read up front, and the centres of the next slot
+  // are loaded into registers while the current one computes.  When every slot
+  // is listed, the list is the identity (records are sorted later, so the
+  // order of the slots does not matter).
+  constexpr int kMaxWalk = 8;
+  const bool all = n == (unsigned)nslots;
+  int wslot[kMaxWalk], wu1[kMaxWalk], wu2[kMaxWalk];
+#pragma unroll
+  for (int k = 0; k < kMaxWalk; ++k) {
+    const unsigned i = blockIdx.x + (unsigned)k * gridDim.x;
+    wslot[k] = -1;
+    wu1[k] = wu2[k] = 0;
+    if (i < n) {
+      wslot[k] = all ? (int)i : (int)(list[i] >> 32);
+      wu1[k] = su1[wslot[k]];
+      wu2[k] = su2[wslot[k]];
+    }
+  }
+  const int t = threadIdx.x;
+  auto load_a = [&](int v1) {  // side-1 cell t
+    double4 a;
+    a.x = a.y = a.z = a.w = qnan;
+    if (t < nA) a = C1[(size_t)v1 * nA + t];
+    return a;
+  };
+  auto load_b = [&](int v2, double (&o)[4]) {  // A operands of this wavefront's side-2 blocks
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int cell = 16 * (wv + 4 * q) + col;
+      o[q] = qnan;
+      if (wv + 4 * q < nBb && cell < nB) {
+        const double4 bb = C2[(size_t)v2 * nB + cell];
+        o[q] = kq == 0 ? -2.0 * bb.x : (kq == 1 ? -2.0 * bb.y : (kq == 2 ? -2.0 * bb.z : bb.x * bb.x + bb.y * bb.y + bb.z * bb.z));
+      }
+    }
+  };
+  double4 a_cur = load_a(wu1[0]);
+  double b_cur[4];
+  load_b(wu2[0], b_cur);
+#pragma unroll
+  for (int k = 0; k < kMaxWalk; ++k) {
+    if (wslot[k] < 0) break;
+    const int slot = wslot[k];
+    double4 a_nxt;
+    double b_nxt[4];
+    if (k + 1 < kMaxWalk && wslot[k + 1] >= 0) {
+      a_nxt = load_a(wu1[k + 1]);
+      load_b(wu2[k + 1], b_nxt);
+    }
+    __syncthreads();  // the previous slot's operands are no longer read
+    {
+      const double4 av = a_cur;
+      const bool in = t < nA;
+      sB[t >> 4][t & 15] = av.x;
+      sB[t >> 4][16 + (t & 15)] = av.y;
+      sB[t >> 4][32 + (t & 15)] = av.z;
+      sB[t >> 4][48 + (t & 15)] = in ? 1.0 : qnan;
+      sT[t] = in ? (tv - (av.x * av.x + av.y * av.y + av.z * av.z)) + delta : -infinity_v<double>();
+    }
+    double aop[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      aop[q] = b_cur[q];
+      sA[wv + 4 * q][lane] = aop[q];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) done += (unsigned long long)nA * (unsigned long long)nB;
+    const uint64_t key_s = pack_idx((uint32_t)slot, 0u, 0u);
+    double bop = sB[0][lane], thr = sT[col];
+    const mfma_v4d z = {0.0, 0.0, 0.0, 0.0};
+    for (int jb = 0; jb < nAb; ++jb) {
+      const double bop_n = sB[jb + 1][lane];  // next block (row 16 of sB is never written: read, unused)
+      const double thr_n = sT[16 * (jb + 1) + col];
+      // four independent products (a wavefront's absent side-2 blocks have NaN operands)
+      mfma_v4d d[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) d[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(aop[q], bop, z, 0, 0, 0);
+      const double m01 = min_nan(min_nan(min_nan(d[0][0], d[0][1]), min_nan(d[0][2], d[0][3])),
+                                 min_nan(min_nan(d[1][0], d[1][1]), min_nan(d[1][2], d[1][3])));
+      const double m23 = min_nan(min_nan(min_nan(d[2][0], d[2][1]), min_nan(d[2][2], d[2][3])),
+                                 min_nan(min_nan(d[3][0], d[3][1]), min_nan(d[3][2], d[3][3])));
+      if (__ballot(min_nan(m01, m23) <= thr) != 0ull) {
+        // rare: append the passing pairs, one per lane per step, keys recomputed from the cells
+        unsigned mask = 0;  // bit 4q + r: result r of product q passes
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) mask |= (d[q][r] <= thr ? 1u : 0u) << (4 * q + r);
+        for (;;) {
+          const bool has = mask != 0u;
+          const unsigned long long mk = __ballot(has);
+          if (!mk) break;
+          const int cnt = __popcll(mk);
+          if (wcount + cnt > kStage) flush();
+          if (has) {
+            const int bit = __builtin_ctz(mask);
+            mask &= mask - 1u;
+            const int ci = 16 * jb + col;                  // side-1 cell (column)
+            const int rb = wv + 4 * (bit >> 2);             // side-2 block
+            const int rr = 4 * (bit & 3) + kq;              // row in the block
+            const int cj = 16 * rb + rr;                    // side-2 cell
+            // a from the B operands, b = -(1/2)(-2b) from the A operands (exact)
+            const double ax = sB[jb][col], ay = sB[jb][16 + col], az = sB[jb][32 + col];
+            const double bx = -0.5 * sA[rb][rr], by = -0.5 * sA[rb][16 + rr], bz = -0.5 * sA[rb][32 + rr];
+            const double dq = pair_d2<double, false>(ax, ay, az, bx, by, bz);
+            Cand c;
+            c.hi = ord_bits(dq);
+            c.lo = key_s | ((uint64_t)(uint32_t)ci << 20) | (uint64_t)(uint32_t)cj;
+            stage[wv][wcount + __popcll(mk & ((1ull << lane) - 1))] = c;
+          }
+          wcount += cnt;
+        }
+      }
+      bop = bop_n;
+      thr = thr_n;
+    }
+    if (k + 1 < kMaxWalk && wslot[k + 1] >= 0) {
+      a_cur = a_nxt;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) b_cur[q] = b_nxt[q];
+    }
+  }
+  if (wcount) flush();
+  if (pairs_done && threadIdx.x == 0 && done) atomicAdd(pairs_done, done);
+}
+
 // ---------------------------------------------------------------------------
 // FP32 screening with an FP64 result (FP32SCREEN).  Centres are evaluated in
 // FP64 and rounded to float for the screen; the screen's threshold is widened
@@ -2375,11 +2574,17 @@ hipError_t launch_dist_filter(const uint64_t* list, const unsigned* list_count, 
   if (w1 <= kTile && w2 <= kTile) {  // every unit is one tile: one slot per list entry
     const unsigned gu = list_cap;     // one workgroup per entry: setup latency overlaps across slots
     if (absmax) {
-      if constexpr (std::is_same<T, double>::value)
-        hipLaunchKernelGGL((k_dist_filter_units<T, false, true>), dim3(gu), dim3(256), 0, s, list, list_count, su1,
+      if constexpr (std::is_same<T, double>::value) {
+        // one resident round of workgroups (4 per CU; 5 fit at this kernel's register use, but 4096 slots
+        // then split 3/4 per workgroup and the last quarter runs a fifth full) walking the slots: a
+        // workgroup lives a few microseconds per slot, too short to launch one per slot
+        // (at most 8 slots per workgroup: the grid grows for more slots)
+        const unsigned gp = std::max(std::max(1u, (unsigned)grid / 2u), (gu + 7u) / 8u);
+        hipLaunchKernelGGL(k_dist_filter_mfma, dim3(std::min(gu, gp)), dim3(256), 0, s, list, list_count, (int)gu, su1,
                            su2, w1, w2, C1, C2, thresh, out, out_count, cap, pairs_done, absmax);
-      else
+      } else {
         return hipErrorInvalidValue;
+      }
     } else if (fma) {
       hipLaunchKernelGGL((k_dist_filter_units<T, true, false>), dim3(gu), dim3(256), 0, s, list, list_count, su1, su2,
                          w1, w2, C1, C2, thresh, out, out_count, cap, pairs_done, absmax);
