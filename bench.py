@@ -225,7 +225,7 @@ def main():
     ap.add_argument("--polish", type=int, default=64, help="top-k records polished to FP64 closest points")
     ap.add_argument("--isect-steps", type=int, default=5, help="intersection-curve queries (saddle / wave plane)")
     ap.add_argument("--batch", type=int, default=256, help="queries of the uneven batch (0 = skip)")
-    ap.add_argument("--serve-rate", type=float, default=1000.0, help="Poisson arrival rate of the serving run")
+    ap.add_argument("--serve-rates", default="500,800", help="Poisson arrival rates of the serving runs (req/s)")
     ap.add_argument("--serve-seconds", type=float, default=2.0, help="arrival window of the serving run (0 = skip)")
     ap.add_argument("--concurrency", type=int, default=8, help="worker streams of the served (batched) form")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
@@ -374,9 +374,11 @@ def main():
         mix = []
         for i in range(100):
             mix.append(hq if i % 10 == 9 else sq[i - i // 10])
-        out["serving_poisson"] = run_poisson(ctx, mix, args.serve_rate, args.serve_seconds, args.concurrency)
-        out["serving_poisson"]["workload"] = ("uneven mix + 10% 4x-scaled hard-wave pairs (nuv=1024, k=4096), "
-                                              "Poisson arrivals, latency = arrival -> result")
+        out["serving_poisson"] = {
+            "workload": "uneven mix + 10% 4x-scaled hard-wave pairs (nuv=1024, k=4096), Poisson arrivals, "
+                        "latency = arrival -> result",
+            "runs": [run_poisson(ctx, mix, float(r), args.serve_seconds, args.concurrency)
+                     for r in args.serve_rates.split(",") if r.strip()]}
     if not args.no_cpu:
         out["cpu_baseline"], odist = cpu_baseline(a, r1, b, r2, args.k, args.cpu_queries, args.polish)
         out["closest"]["oracle_fp64_distance"] = odist

@@ -214,6 +214,11 @@ hipError_t launch_isect_pass(int write, const uint64_t* list, const unsigned lon
                              const unsigned long long* rowoff, unsigned* rowcursor, IPair* out, int grid,
                              hipStream_t s);
 hipError_t launch_row_sort(const unsigned long long* rowoff, unsigned long long nrows, IPair* out, hipStream_t s);
+// Long rows: segmented radix sort of the cell ids j (jsi_sort.hip); tmp of
+// segmented_sort_tmp_bytes(n, nrows) bytes.
+size_t segmented_sort_tmp_bytes(unsigned long long n, unsigned long long nrows);
+hipError_t segmented_sort_pairs(IPair* pairs, unsigned long long n, const unsigned long long* rowoff,
+                                unsigned long long nrows, void* tmp, size_t tmp_bytes, hipStream_t s);
 hipError_t scan_u32_to_u64(const unsigned* in, unsigned long long* out, unsigned long long n, void* tmp,
                            size_t tmp_bytes, hipStream_t s);
 size_t scan_tmp_bytes(unsigned long long n);

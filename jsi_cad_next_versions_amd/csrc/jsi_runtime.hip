@@ -835,7 +835,14 @@ int jsi_ctx::intersect(int h1, const jsi_region& r1, int h2, const jsi_region& r
       HCK(launch_fill_u32(rowcursor, 0u, nrows, stream));
       HCK(launch_isect_pass<T>(1, list, small, su1, su2, w1, w2, C1, R1, C2, R2, rowcount, rowoff, rowcursor, pairs,
                                grid, stream));
-      HCK(launch_row_sort(rowoff, nrows, pairs, stream));
+      if (npairs > 16ull * nrows) {  // long rows: segmented radix sort
+        const size_t tb = segmented_sort_tmp_bytes(npairs, nrows);
+        void* st;
+        RCK(buf_raw("i.segsort", tb, &st, 0));
+        HCK(segmented_sort_pairs(pairs, npairs, rowoff, nrows, st, tb, stream));
+      } else {
+        HCK(launch_row_sort(rowoff, nrows, pairs, stream));
+      }
     }
     HCK(hipStreamSynchronize(stream));
     res->ms_screen += now_ms() - ts;
