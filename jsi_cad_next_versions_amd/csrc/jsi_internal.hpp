@@ -97,9 +97,17 @@ struct EcOut {
   unsigned long long* absmax = nullptr;
   const int* nunits_dev = nullptr;
 };
+// One surface's regions for the fused evaluation + distance centres.
 template <class T>
-hipError_t launch_eval_centres(const SurfDev<T>& sd, const vec4_t<T>* reg, int nunits, int nuv, const EcOut<T>& o,
-                               hipStream_t s);
+struct EcJob {
+  SurfDev<T> sd;
+  const vec4_t<T>* reg = nullptr;
+  int nunits = 0, nuv = 0;
+  EcOut<T> o;
+};
+// b may be null; two jobs of the same kernel variant share one launch.
+template <class T>
+hipError_t launch_eval_centres(const EcJob<T>& a, const EcJob<T>* b, hipStream_t s);
 
 // Cell AABBs of every unit from its SoA points.  isect = 0: distance centres
 // (construct_aabbs_distance_host); isect = 1: centre + padded radius
@@ -124,16 +132,17 @@ hipError_t launch_dist_list(int nslots, const int* su1, const int* su2, int w1, 
                             const vec4_t<T>* hi1, const vec4_t<T>* lo2, const vec4_t<T>* hi2, const T* thresh,
                             int cull, uint64_t* list, unsigned* list_count, hipStream_t s, int fma = 0);
 // Seed threshold: picks up to nseed tile pairs from best[] (npos =
-// nslots * tiles per side-1 unit; ordered = take the first slots), histograms
-// the d2 of their cell pairs and writes the upper edge of the bin holding
-// rank k to *thresh (+inf if they hold fewer than k pairs).
+// nslots * tiles per side-1 unit; ordered = the first slots, one tile per
+// unit, best[] unused), histograms the d2 of their cell pairs and writes the
+// upper edge of the bin holding rank k to *thresh (+inf if they hold fewer
+// than k pairs).
 template <class T>
 hipError_t launch_seed_thresh(const unsigned long long* best, int npos, int tpu1, int nseed, int ordered,
                               uint64_t* list, unsigned* list_count, unsigned long long* range,
                               unsigned long long* work, const int* su1, const int* su2, int w1, int w2,
                               const vec4_t<T>* C1, const vec4_t<T>* C2, int k, int fma, T* thresh, hipStream_t s,
                               unsigned* clear = nullptr, int nclear = 0);
-size_t seed_work_bytes();  // size of `work` for launch_seed_thresh; clear: words zeroed at the end
+size_t seed_work_bytes();
 template <class T>
 hipError_t launch_dist_filter(const uint64_t* list, const unsigned* list_count, unsigned list_cap, const int* su1,
                               const int* su2, int w1, int w2, const vec4_t<T>* C1, const vec4_t<T>* C2,

@@ -307,8 +307,8 @@ struct EcRows {  // control rows per chunk of u-contractions
 };
 
 template <class T, int MAXD, bool RAT, bool KL>
-__global__ __launch_bounds__(64) void k_eval_centres(SurfDev<T> sd, const vec4_t<T>* regions, int nunits, int nuv,
-                                                     EcOut<T> out) {
+__device__ __forceinline__ void ec_tile(const SurfDev<T>& sd, const vec4_t<T>* regions, int nunits, int nuv,
+                                        const EcOut<T>& out, const int bid) {
   vec4_t<T>* const C = out.C;
   float4* const C32 = out.C32;
   unsigned long long* const absmax = out.absmax;
@@ -336,7 +336,7 @@ __global__ __launch_bounds__(64) void k_eval_centres(SurfDev<T> sd, const vec4_t
   const int lane = threadIdx.x;
   const int w = nuv - 1;
   const int tw = (w + kTile - 1) / kTile, tpu = tw * tw;
-  const int unit = blockIdx.x / tpu, t = blockIdx.x % tpu;
+  const int unit = bid / tpu, t = bid % tpu;
   if (unit >= nunits || (nunits_dev && unit >= *nunits_dev)) return;  // whole wavefront
   const int row0 = (t / tw) * kTile, col0 = (t % tw) * kTile;
   const int ht = min(kTile, w - row0), wd = min(kTile, w - col0);  // cells
@@ -526,8 +526,8 @@ __global__ __launch_bounds__(64) void k_eval_centres(SurfDev<T> sd, const vec4_t
       h.y = bmx[1];
       h.z = bmx[2];
       h.w = T(0);
-      out.lo[blockIdx.x] = l;  // block = unit * tiles per unit + tile, the tile box index
-      out.hi[blockIdx.x] = h;
+      out.lo[bid] = l;  // bid = unit * tiles per unit + tile, the tile box index
+      out.hi[bid] = h;
     }
   }
   if (out.lo32) {
@@ -539,8 +539,8 @@ __global__ __launch_bounds__(64) void k_eval_centres(SurfDev<T> sd, const vec4_t
         fmx[a] = fmaxf(fmx[a], __shfl_xor(fmx[a], off));
       }
     if (lane == 0) {
-      out.lo32[blockIdx.x] = make_float4(fmn[0], fmn[1], fmn[2], 0.f);
-      out.hi32[blockIdx.x] = make_float4(fmx[0], fmx[1], fmx[2], 0.f);
+      out.lo32[bid] = make_float4(fmn[0], fmn[1], fmn[2], 0.f);
+      out.hi32[bid] = make_float4(fmx[0], fmx[1], fmx[2], 0.f);
     }
   }
   if constexpr (std::is_same<T, double>::value) {
@@ -551,9 +551,22 @@ __global__ __launch_bounds__(64) void k_eval_centres(SurfDev<T> sd, const vec4_t
         const unsigned long long o = __shfl_xor(b, off);
         b = o > b ? o : b;
       }
-      if (lane == 0 && b) atomicMax(&absmax[blockIdx.x % kAmaxSlots], b);
+      if (lane == 0 && b) atomicMax(&absmax[bid % kAmaxSlots], b);
     }
   }
+}
+
+// Both surfaces of a query in one launch when they share a variant: blocks
+// [0, nb0) are tiles of j0, the rest tiles of j1.  Twice the wavefronts of
+// one surface's launch keep more loads in flight per CU.
+template <class T, int MAXD, bool RAT, bool KL>
+__global__ __launch_bounds__(64) void k_eval_centres(EcJob<T> j0, EcJob<T> j1, int nb0) {
+  const int b = (int)blockIdx.x;
+  const bool second = b >= nb0;  // uniform: one body, its arguments selected
+  const SurfDev<T> sd = second ? j1.sd : j0.sd;
+  const EcOut<T> o = second ? j1.o : j0.o;
+  ec_tile<T, MAXD, RAT, KL>(sd, second ? j1.reg : j0.reg, second ? j1.nunits : j0.nunits, second ? j1.nuv : j0.nuv, o,
+                            second ? b - nb0 : b);
 }
 
 // Cell AABBs (construct_aabbs_distance_host, distance_analysis.cu:45-86;
@@ -769,14 +782,12 @@ __global__ __launch_bounds__(256) void k_dist_seed(int nslots, const int* su1, c
   }
 }
 
-// Single workgroup: the seed tile pairs.  ordered = 1 (refinement rounds, one
-// tile pair per slot, slots in rank order of their parent pairs): the first
-// nseed slots.  Otherwise the nseed positions with the smallest upper bound.
-// range[0..1] is reset for the min/max pass of the seed histogram.
+// Single workgroup: the seed tile pairs, the nseed positions with the
+// smallest upper bound (ordered = 1, one tile pair per slot and slots in rank
+// order of their parent pairs: the first nseed slots).
 template <class T>
 __global__ __launch_bounds__(1024) void k_seed_list(const unsigned long long* best, int npos, int tpu1, int nseed,
-                                                    int ordered, uint64_t* list, unsigned* list_count,
-                                                    unsigned long long* range) {
+                                                    int ordered, uint64_t* list, unsigned* list_count) {
   __shared__ unsigned long long s[kSeedSortMax];
   __shared__ unsigned umax;
   const int m = nseed < npos ? nseed : npos;
@@ -812,11 +823,7 @@ __global__ __launch_bounds__(1024) void k_seed_list(const unsigned long long* be
     atomicMax(&umax, (unsigned)(b >> 32));
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    *list_count = (unsigned)m;
-    range[0] = ~0ull;  // min / max d2 bits over the seed pairs (k_seed_hist<..., true>)
-    range[1] = 0ull;
-  }
+  if (threadIdx.x == 0) *list_count = (unsigned)m;
 }
 
 constexpr int kSeedSplitMax = 16;    // workgroups per seed tile pair at most
@@ -824,20 +831,20 @@ constexpr int kSeedHistBins = 1024;  // bins of the seed histogram
 
 // Two passes over the d2 of the seed tile pairs (no candidate records are
 // written).  Each workgroup handles one seed tile pair against a 1/split
-// slice of its side-2 cells.  MINMAX: per-workgroup smallest and largest key
+// slice of its side-2 cells; seed tile pair `it` is list[it], or with
+// ordered (one tile per unit, slots in parent rank order) slot `it`.  MINMAX: per-workgroup smallest and largest key
 // -> mm[2*block].  Otherwise: a histogram in LDS, bins linear in the key bits
 // over [min, max] (its resolution follows the spread of the values: refined
 // cells have d2 within 1e-5 of each other), whose non-zero bins are added to
 // the global histogram gh.
 template <class T, bool FMA, bool MINMAX>
-__global__ __launch_bounds__(256) void k_seed_hist(const uint64_t* list, const unsigned* list_count, const int* su1,
+__global__ __launch_bounds__(256) void k_seed_hist(const uint64_t* list, int n, int ordered, const int* su1,
                                                    const int* su2, int w1, int w2, const vec4_t<T>* C1,
                                                    const vec4_t<T>* C2, int split, const unsigned long long* range,
                                                    unsigned long long* mm, unsigned* gh) {
   __shared__ vec4_t<T> sB[kTileCells];
   __shared__ unsigned h[MINMAX ? 1 : kSeedHistBins];
   __shared__ unsigned long long rmin[4], rmax[4];
-  const unsigned n = *list_count;
   unsigned long long lo = 0, width = 1;
   if (!MINMAX) {
     lo = range[0];
@@ -846,10 +853,10 @@ __global__ __launch_bounds__(256) void k_seed_hist(const uint64_t* list, const u
     for (int b = threadIdx.x; b < kSeedHistBins; b += blockDim.x) h[b] = 0;
   }
   unsigned long long kmin = ~0ull, kmax = 0ull;
-  const unsigned it = blockIdx.x / split;
-  const int part = blockIdx.x % split;
+  const int it = (int)(blockIdx.x / split);
+  const int part = (int)(blockIdx.x % split);
   if (it < n) {
-    const uint64_t e = list[it];
+    const uint64_t e = ordered ? tp_encode(it, 0, 0) : list[it];
     const int slot = (int)(e >> 32), t1 = (int)((e >> 16) & 0xffff), t2 = (int)(e & 0xffff);
     const int u1 = su1[slot], u2 = su2[slot];
     const TileGeo g1 = tile_geo(t1, w1), g2 = tile_geo(t2, w2);
@@ -2427,39 +2434,67 @@ hipError_t launch_points(const SurfDev<T>& sd, const T* tabU, const int* spanU, 
   return hipGetLastError();
 }
 
+template <class T>
+static int ec_blocks(const EcJob<T>& j) {
+  if (j.nunits <= 0 || j.nuv < 2) return 0;
+  const int tw = (j.nuv - 1 + kTile - 1) / kTile;
+  return j.nunits * tw * tw;
+}
+
+// variant of a job: 0-2 = degree class (3, 7, kMaxDeg - 1), +3 rational, +6 knots in LDS
+template <class T>
+static int ec_variant(const EcJob<T>& j) {
+  const int d = std::max(j.sd.degu, j.sd.degv);
+  const int dc = d <= 3 ? 0 : d <= 7 ? 1 : 2;
+  const bool kl = j.sd.nku <= kEcKnots && j.sd.nkv <= kEcKnots;
+  return dc + (j.sd.rational ? 3 : 0) + (kl ? 6 : 0);
+}
+
 template <class T, int MAXD>
-static void eval_centres_rat(const SurfDev<T>& sd, const vec4_t<T>* reg, int nunits, int nuv, const EcOut<T>& o,
-                             hipStream_t s) {
-  const int w = nuv - 1, tw = (w + kTile - 1) / kTile;
-  const dim3 g((unsigned)((long long)nunits * tw * tw)), b(64);
-  const bool kl = sd.nku <= kEcKnots && sd.nkv <= kEcKnots;
-  if (sd.rational) {
+static void ec_launch_rat(int var, const EcJob<T>& j0, const EcJob<T>& j1, int nb0, int nb, hipStream_t s) {
+  const dim3 g((unsigned)nb), b(64);
+  const bool rat = (var / 3) % 2 == 1, kl = var >= 6;
+  if (rat) {
     if (kl)
-      hipLaunchKernelGGL((k_eval_centres<T, MAXD, true, true>), g, b, 0, s, sd, reg, nunits, nuv, o);
+      hipLaunchKernelGGL((k_eval_centres<T, MAXD, true, true>), g, b, 0, s, j0, j1, nb0);
     else
-      hipLaunchKernelGGL((k_eval_centres<T, MAXD, true, false>), g, b, 0, s, sd, reg, nunits, nuv, o);
+      hipLaunchKernelGGL((k_eval_centres<T, MAXD, true, false>), g, b, 0, s, j0, j1, nb0);
   } else {
     if (kl)
-      hipLaunchKernelGGL((k_eval_centres<T, MAXD, false, true>), g, b, 0, s, sd, reg, nunits, nuv, o);
+      hipLaunchKernelGGL((k_eval_centres<T, MAXD, false, true>), g, b, 0, s, j0, j1, nb0);
     else
-      hipLaunchKernelGGL((k_eval_centres<T, MAXD, false, false>), g, b, 0, s, sd, reg, nunits, nuv, o);
+      hipLaunchKernelGGL((k_eval_centres<T, MAXD, false, false>), g, b, 0, s, j0, j1, nb0);
   }
 }
 
 template <class T>
-hipError_t launch_eval_centres(const SurfDev<T>& sd, const vec4_t<T>* reg, int nunits, int nuv, const EcOut<T>& o,
-                               hipStream_t s) {
-  if (nunits <= 0 || nuv < 2) return hipSuccess;
-  if ((o.C32 || o.absmax || o.lo32) && !std::is_same<T, double>::value) return hipErrorInvalidValue;
-  if ((o.C32 && !o.absmax) || (o.lo32 && !o.C32) || (!o.lo != !o.hi) || (!o.lo32 != !o.hi32))
-    return hipErrorInvalidValue;
-  const int d = std::max(sd.degu, sd.degv);
-  if (d <= 3)
-    eval_centres_rat<T, 3>(sd, reg, nunits, nuv, o, s);
-  else if (d <= 7)
-    eval_centres_rat<T, 7>(sd, reg, nunits, nuv, o, s);
-  else
-    eval_centres_rat<T, kMaxDeg - 1>(sd, reg, nunits, nuv, o, s);
+static void ec_launch(int var, const EcJob<T>& j0, const EcJob<T>& j1, int nb0, int nb, hipStream_t s) {
+  if (nb <= 0) return;
+  switch (var % 3) {
+    case 0: ec_launch_rat<T, 3>(var, j0, j1, nb0, nb, s); break;
+    case 1: ec_launch_rat<T, 7>(var, j0, j1, nb0, nb, s); break;
+    default: ec_launch_rat<T, kMaxDeg - 1>(var, j0, j1, nb0, nb, s); break;
+  }
+}
+
+template <class T>
+static bool ec_valid(const EcOut<T>& o) {
+  if ((o.C32 || o.absmax || o.lo32) && !std::is_same<T, double>::value) return false;
+  return !((o.C32 && !o.absmax) || (o.lo32 && !o.C32) || (!o.lo != !o.hi) || (!o.lo32 != !o.hi32));
+}
+
+template <class T>
+hipError_t launch_eval_centres(const EcJob<T>& a, const EcJob<T>* b, hipStream_t s) {
+  if (!ec_valid(a.o) || (b && !ec_valid(b->o))) return hipErrorInvalidValue;
+  const int na = ec_blocks(a), nb = b ? ec_blocks(*b) : 0;
+  const int va = ec_variant(a);
+  if (b && nb > 0 && ec_variant(*b) == va) {
+    if ((long long)na + nb > INT_MAX) return hipErrorInvalidValue;
+    ec_launch(va, a, *b, na, na + nb, s);
+  } else {
+    ec_launch(va, a, a, na, na, s);
+    if (b) ec_launch(ec_variant(*b), *b, *b, nb, nb, s);
+  }
   return hipGetLastError();
 }
 
@@ -2519,26 +2554,29 @@ hipError_t launch_seed_thresh(const unsigned long long* best, int npos, int tpu1
                               const vec4_t<T>* C1, const vec4_t<T>* C2, int k, int fma, T* thresh, hipStream_t s,
                               unsigned* clear, int nclear) {
   if (npos > kSeedSortMax || nseed > kSeedMaxTiles || nclear > kSeedHistBins) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_seed_list<T>, dim3(1), dim3(1024), 0, s, best, npos, tpu1, nseed, ordered, list, list_count,
-                     range);
+  if (ordered && tpu1 != 1) return hipErrorInvalidValue;
+  // ordered: seed tile pair i is slot i, nothing to pick
+  if (!ordered)
+    hipLaunchKernelGGL(k_seed_list<T>, dim3(1), dim3(1024), 0, s, best, npos, tpu1, nseed, ordered, list,
+                       list_count);
   const int m = nseed < npos ? nseed : npos;
   // about 1024 workgroups: fill the chip without splitting tiles finely
   int split = m > 0 ? 1024 / m : 1;
   split = split < 1 ? 1 : (split > kSeedSplitMax ? kSeedSplitMax : split);
-  const int nb = m * split;
+  const int nb = m > 0 ? m * split : 1;
   unsigned* gh = reinterpret_cast<unsigned*>(work + 2 * (size_t)kSeedMaxTiles * kSeedSplitMax);
   if (fma) {
-    hipLaunchKernelGGL((k_seed_hist<T, true, true>), dim3(nb), dim3(256), 0, s, list, list_count, su1, su2, w1, w2,
+    hipLaunchKernelGGL((k_seed_hist<T, true, true>), dim3(nb), dim3(256), 0, s, list, m, ordered, su1, su2, w1, w2,
                        C1, C2, split, range, work, gh);
     hipLaunchKernelGGL(k_seed_range, dim3(1), dim3(1024), 0, s, work, nb, range, gh);
-    hipLaunchKernelGGL((k_seed_hist<T, true, false>), dim3(nb), dim3(256), 0, s, list, list_count, su1, su2, w1, w2,
+    hipLaunchKernelGGL((k_seed_hist<T, true, false>), dim3(nb), dim3(256), 0, s, list, m, ordered, su1, su2, w1, w2,
                        C1, C2, split, range, work, gh);
   } else {
-    hipLaunchKernelGGL((k_seed_hist<T, false, true>), dim3(nb), dim3(256), 0, s, list, list_count, su1, su2, w1, w2,
-                       C1, C2, split, range, work, gh);
+    hipLaunchKernelGGL((k_seed_hist<T, false, true>), dim3(nb), dim3(256), 0, s, list, m, ordered, su1, su2, w1,
+                       w2, C1, C2, split, range, work, gh);
     hipLaunchKernelGGL(k_seed_range, dim3(1), dim3(1024), 0, s, work, nb, range, gh);
-    hipLaunchKernelGGL((k_seed_hist<T, false, false>), dim3(nb), dim3(256), 0, s, list, list_count, su1, su2, w1, w2,
-                       C1, C2, split, range, work, gh);
+    hipLaunchKernelGGL((k_seed_hist<T, false, false>), dim3(nb), dim3(256), 0, s, list, m, ordered, su1, su2, w1,
+                       w2, C1, C2, split, range, work, gh);
   }
   hipLaunchKernelGGL(k_seed_thresh<T>, dim3(1), dim3(kSeedHistBins), 0, s, gh, range, k, thresh, clear, nclear);
   return hipGetLastError();
@@ -2886,8 +2924,7 @@ hipError_t launch_fill_u32(unsigned* p, unsigned value, unsigned long long n, hi
                                             unsigned long long*, unsigned long long, int, hipStream_t,             \
                                             unsigned long long*, int, int, const unsigned long long*);             \
   template hipError_t launch_set_thresh<T>(T*, const Cand*, int, int, hipStream_t);                                \
-  template hipError_t launch_eval_centres<T>(const SurfDev<T>&, const vec4_t<T>*, int, int, const EcOut<T>&,       \
-                                             hipStream_t);                                                         \
+  template hipError_t launch_eval_centres<T>(const EcJob<T>&, const EcJob<T>*, hipStream_t);                      \
   template hipError_t launch_seed_thresh<T>(const unsigned long long*, int, int, int, int, uint64_t*, unsigned*,   \
                                             unsigned long long*, unsigned long long*, const int*, const int*, int, \
                                             int, const vec4_t<T>*, const vec4_t<T>*, int, int, T*, hipStream_t,    \

@@ -342,11 +342,28 @@ __device__ inline void cell_centre(const vec4_t<TR>& R, int nuv, int cell, doubl
   o[1] = (double)R.y + vs * ((double)iv + 0.5);
 }
 
+constexpr int kPolishKnots = 256;  // knots per vector staged in LDS
+
 template <class TR, int MAXP, bool FAST>
-__global__ __launch_bounds__(64) void k_polish(SurfDev<double> s1, SurfDev<double> s2, const Cand* rec, int m,
+__global__ __launch_bounds__(64) void k_polish(SurfDev<double> s1g, SurfDev<double> s2g, const Cand* rec, int m,
                                                const int* su1, const int* su2, const vec4_t<TR>* reg1,
                                                const vec4_t<TR>* reg2, int nuv1, int nuv2, double4 box1,
                                                double4 box2, PolishRec* out) {
+  // the four knot vectors in LDS when they fit: the span searches of every
+  // evaluation are chains of dependent loads
+  __shared__ double sk[4][kPolishKnots];
+  SurfDev<double> s1 = s1g, s2 = s2g;
+  if (s1.nku <= kPolishKnots && s1.nkv <= kPolishKnots && s2.nku <= kPolishKnots && s2.nkv <= kPolishKnots) {
+    for (int i = threadIdx.x; i < s1.nku; i += blockDim.x) sk[0][i] = s1g.U[i];
+    for (int i = threadIdx.x; i < s1.nkv; i += blockDim.x) sk[1][i] = s1g.V[i];
+    for (int i = threadIdx.x; i < s2.nku; i += blockDim.x) sk[2][i] = s2g.U[i];
+    for (int i = threadIdx.x; i < s2.nkv; i += blockDim.x) sk[3][i] = s2g.V[i];
+    __syncthreads();
+    s1.U = sk[0];
+    s1.V = sk[1];
+    s2.U = sk[2];
+    s2.V = sk[3];
+  }
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= m) return;
   const Cand c = rec[e];

@@ -189,12 +189,23 @@ int jsi_ctx::eval_units(int h, const vec4_t<T>* reg, int nunits, int nuv, int mo
   return JSI_OK;
 }
 
-// Distance cell centres of `nunits` regions, evaluation fused with the AABB.
+// Distance cell centres of both sides' regions, evaluation fused with the
+// AABB (one launch when the two surfaces share a kernel variant).
 template <class T>
-int jsi_ctx::eval_centres(int h, const vec4_t<T>* reg, int nunits, int nuv, const EcOut<T>& o) {
-  SurfDev<T> sd;
-  RCK(surf_dev<T>(h, &sd));
-  HCK(launch_eval_centres<T>(sd, reg, nunits, nuv, o, stream));
+int jsi_ctx::eval_centres(int h1, const vec4_t<T>* reg1, int nunits1, int nuv1, const EcOut<T>& o1, int h2,
+                          const vec4_t<T>* reg2, int nunits2, int nuv2, const EcOut<T>& o2) {
+  EcJob<T> a, b;
+  RCK(surf_dev<T>(h1, &a.sd));
+  RCK(surf_dev<T>(h2, &b.sd));
+  a.reg = reg1;
+  a.nunits = nunits1;
+  a.nuv = nuv1;
+  a.o = o1;
+  b.reg = reg2;
+  b.nunits = nunits2;
+  b.nuv = nuv2;
+  b.o = o2;
+  HCK(launch_eval_centres<T>(a, &b, stream));
   return JSI_OK;
 }
 
@@ -266,12 +277,14 @@ int jsi_ctx::dist_topk(int nslots, const int* su1, const int* su2, int w1, int n
   const unsigned long long want = (32ull * (unsigned long long)k + pairs_tp - 1) / (unsigned long long)pairs_tp;
   const int nseed = nslots == 1 ? std::min(256, tpu1) : (int)std::max(4ull, std::min(64ull, want));
   if (npos <= 8192) {
-    unsigned long long *bestv, *range, *work;
-    RCK(buf("d.best", (size_t)npos, &bestv));
+    unsigned long long *bestv = nullptr, *range, *work;
     RCK(buf("d.range", 2, &range));
     RCK(buf_raw("d.seedwork", seed_work_bytes(), reinterpret_cast<void**>(&work), 0));
     const int ordered = (nslots > 1 && tpu1 == 1 && tpu2 == 1) ? 1 : 0;
-    HCK(launch_dist_seed<T>(nslots, su1, su2, w1, w2, lo1, hi1, lo2, hi2, k, bestv, stream));
+    if (!ordered) {  // ordered: the seed tile pairs are the first slots' only tile pairs
+      RCK(buf("d.best", (size_t)npos, &bestv));
+      HCK(launch_dist_seed<T>(nslots, su1, su2, w1, w2, lo1, hi1, lo2, hi2, k, bestv, stream));
+    }
     // the seed's last kernel also clears the screen's counters (words 0-5 of
     // d.small: list count, unused, candidate count)
     HCK(launch_seed_thresh<T>(bestv, (int)npos, tpu1, nseed, ordered, list, list_count, range, work, su1, su2, w1,
@@ -564,8 +577,7 @@ int jsi_ctx::min_distance(int h1, const jsi_region& r1, int h2, const jsi_region
       tb32.hi2 = o2.hi32;
     }
     o1.absmax = o2.absmax = amax;
-    RCK(eval_centres<T>(h1, reg1, 1, nuv1, o1));
-    RCK(eval_centres<T>(h2, reg2, 1, nuv2, o2));
+    RCK(eval_centres<T>(h1, reg1, 1, nuv1, o1, h2, reg2, 1, nuv2, o2));
   }
   if (prof) {
     HCK(hipStreamSynchronize(stream));
@@ -663,8 +675,7 @@ int jsi_ctx::min_distance(int h1, const jsi_region& r1, int h2, const jsi_region
     o1.absmax = o2.absmax = amax;
     o1.nunits_dev = nunits_dev;
     o2.nunits_dev = nunits_dev + 1;
-    RCK(eval_centres<T>(h1, nreg1, nu_max, nref, o1));
-    RCK(eval_centres<T>(h2, nreg2, nu_max, nref, o2));
+    RCK(eval_centres<T>(h1, nreg1, nu_max, nref, o1, h2, nreg2, nu_max, nref, o2));
     tb.lo1 = o1.lo;
     tb.hi1 = o1.hi;
     tb.lo2 = o2.lo;

@@ -101,7 +101,8 @@ struct jsi_ctx {
                  unsigned long long* kmax, const char* tag);
   using Rerank = RerankSpec;
   template <class T>
-  int eval_centres(int h, const jsi::vec4_t<T>* reg, int nunits, int nuv, const jsi::EcOut<T>& o);
+  int eval_centres(int h1, const jsi::vec4_t<T>* reg1, int nunits1, int nuv1, const jsi::EcOut<T>& o1, int h2,
+                   const jsi::vec4_t<T>* reg2, int nunits2, int nuv2, const jsi::EcOut<T>& o2);
   template <class T>
   int dist_topk(int nslots, const int* su1, const int* su2, int w1, int nunits1, const jsi::vec4_t<T>* C1, int w2,
                 int nunits2, const jsi::vec4_t<T>* C2, int k, int cull, jsi::Cand** rec_out, long long* n_out,
