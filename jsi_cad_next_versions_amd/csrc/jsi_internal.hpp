@@ -68,17 +68,16 @@ enum EvalMode { kEvalFull18 = 0, kEvalPoint = 1, kEvalPointK = 2 };
 template <class T>
 hipError_t launch_promote(const float* src, T* dst, int n, hipStream_t s);
 
+// Evaluates nunits x nuv x nuv points, one workgroup per 16 x 16-point tile
+// (spans and basis rows computed in the kernel).  mode kEvalFull18: out = AoS
+// 18 per point (evalWithCuda layout, unit-major).  kEvalPoint / kEvalPointK:
+// out = SoA xyz per unit [unit][3][nuv*nuv]; kEvalPointK also max-reduces
+// |Suu|,|Suv|,|Svv| into kmax[unit*3 + {0,1,2}] (ordered bit patterns,
+// cleared by the caller).  steps_given: regions hold (ustart, vstart, ustep,
+// vstep) instead of (ustart, vstart, ustop, vstop).
 template <class T>
-hipError_t launch_basis(const T* knots, int nk, int p, int der, const vec4_t<T>* regions, int nunits, int nuv,
-                        int dir, T* table, int* spans, hipStream_t s);
-
-// Evaluates nunits x nuv x nuv points.  mode kEvalFull18: out = AoS 18 per
-// point (evalWithCuda layout, unit-major).  kEvalPoint / kEvalPointK: out =
-// SoA xyz per unit [unit][3][nuv*nuv]; kEvalPointK also max-reduces
-// |Suu|,|Suv|,|Svv| into kmax[unit*3 + {0,1,2}] (ordered bit patterns).
-template <class T>
-hipError_t launch_points(const SurfDev<T>& sd, const T* tabU, const int* spanU, const T* tabV, const int* spanV,
-                         int nunits, int nuv, int mode, T* out, unsigned long long* kmax, hipStream_t s);
+hipError_t launch_points_tile(const SurfDev<T>& sd, const vec4_t<T>* reg, int nunits, int nuv, int mode,
+                              int steps_given, T* out, unsigned long long* kmax, hipStream_t s);
 
 // Outputs of the fused evaluation + distance centres.  C as launch_aabb(isect
 // = 0) writes it; optional: lo/hi, the per-tile box of the centres

@@ -98,185 +98,12 @@ __device__ __forceinline__ void basis_vals(int span, int p, T u, const T* U, T (
   }
 }
 
-// One thread per (unit, sample index, direction).  The basis of a grid point
-// depends only on its column (u) or row (v), so it is computed once per
-// column/row and shared by the nuv points that use it.  `regions` holds
-// (ustart, vstart, ustop, vstop); the sample parameter follows
-// run_evaluation_and_construction_task + the grid driver
-// (distance_analysis.cu:92-93, evaluation.cu:366): step = (stop-start)/nuv,
-// t = step*idx + start.  steps_given = 1: regions hold (ustart, vstart,
-// ustep, vstep) directly (evalWithCuda).
-template <class T, int MAXP>
-__global__ void k_basis(const T* knots, int nk, int p, int der, const vec4_t<T>* regions, int nunits, int nuv,
-                        int dir, int steps_given, T* table, int* spans) {
-  const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= (long long)nunits * nuv) return;
-  const int unit = (int)(gid / nuv), idx = (int)(gid % nuv);
-  const vec4_t<T> r = regions[unit];
-  const T start = dir == 0 ? r.x : r.y;
-  T step;
-  if (steps_given)
-    step = dir == 0 ? r.z : r.w;
-  else
-    step = ((dir == 0 ? r.z : r.w) - start) / T(nuv);
-  const T t = step * T(idx) + start;
-  const int span = find_span(nk, p, t, knots);
-  T d[3][MAXP + 1];
-  basis_ders<T, MAXP>(span, p, t, knots, der, d);
-  T* o = table + (size_t)gid * 3 * (p + 1);
-  for (int k = 0; k < 3; ++k)
-    for (int j = 0; j <= p; ++j) o[k * (p + 1) + j] = (k <= der) ? d[k][j] : T(0);
-  spans[gid] = span;
-}
-
 // Position of derivative (k = u order, l = v order) in the output
 // [S, Su, Sv, Suu, Suv, Svv] (EvaluateOneUV loop order + row swap,
 // evaluation.cu:281-310).
 __device__ __forceinline__ int out_row(int k, int l) {
   const int m = (l == 0) ? k : (l == 1 ? 3 + k : 5);
   return m == 2 ? 3 : (m == 3 ? 2 : m);
-}
-
-// Grid points: EvaluateOneUV (evaluation.cu:259-356) on precomputed basis
-// rows.  The u-direction partial sums R[k][s] = sum_j Nu[k][j]*(w)*P are
-// identical for every l, so they are formed once per (k, s).
-template <class T, int MAXD, bool RAT, int MODE>
-__global__ __launch_bounds__(256) void k_points(SurfDev<T> sd, const T* tabU, const int* spanU, const T* tabV,
-                                                const int* spanV, int nunits, int nuv, T* out,
-                                                unsigned long long* kmax) {
-  const int npts = nuv * nuv;
-  const int bpu = (npts + 255) / 256;  // blocks per unit: a block never spans two units
-  const int unit = blockIdx.x / bpu;
-  const int pt = (blockIdx.x % bpu) * 256 + threadIdx.x;
-  const bool active = unit < nunits && pt < npts;
-  const int idx = pt % nuv, idy = pt / nuv;
-  const int p = sd.degu, q = sd.degv;
-  const int du = p < 2 ? p : 2, dv = q < 2 ? q : 2;
-  T res[18];
-#pragma unroll
-  for (int e = 0; e < 18; ++e) res[e] = T(0);
-  if (active && !sd.zero) {
-    const int su = spanU[unit * nuv + idx], sv = spanV[unit * nuv + idy];
-    const T* Nu = tabU + (size_t)(unit * nuv + idx) * 3 * (p + 1);
-    const T* Nv = tabV + (size_t)(unit * nuv + idy) * 3 * (q + 1);
-    constexpr int NK = (MODE == kEvalPoint) ? 1 : 3;
-    const int kmax_u = (MODE == kEvalPoint) ? 0 : du;
-    constexpr int NC = RAT ? 4 : 3;
-    T R[NK][MAXD + 1][NC];
-    for (int s = 0; s <= q; ++s) {
-      const int base = sd.ncpu * (sv - q + s) + su - p;
-      for (int k = 0; k <= kmax_u; ++k) {
-#pragma unroll
-        for (int c = 0; c < NC; ++c) R[k][s][c] = T(0);
-        for (int j = 0; j <= p; ++j) {
-          const int id = base + j;
-          const T n = Nu[k * (p + 1) + j];
-          if (RAT) {
-            const T nw = n * sd.W[id];
-            R[k][s][0] += nw * sd.CP[3 * id + 0];
-            R[k][s][1] += nw * sd.CP[3 * id + 1];
-            R[k][s][2] += nw * sd.CP[3 * id + 2];
-            R[k][s][NC - 1] += nw;
-          } else {
-            R[k][s][0] += n * sd.CP[3 * id + 0];
-            R[k][s][1] += n * sd.CP[3 * id + 1];
-            R[k][s][2] += n * sd.CP[3 * id + 2];
-          }
-        }
-      }
-    }
-    T h[6][4];
-#pragma unroll
-    for (int m = 0; m < 6; ++m)
-#pragma unroll
-      for (int c = 0; c < 4; ++c) h[m][c] = T(0);
-    const int lmax = (MODE == kEvalPoint) ? 0 : dv;
-    for (int l = 0; l <= lmax; ++l) {
-      for (int k = 0; k <= 2 - l; ++k) {
-        if (k > kmax_u) continue;
-        T acc[NC];
-#pragma unroll
-        for (int c = 0; c < NC; ++c) acc[c] = T(0);
-        for (int s = 0; s <= q; ++s) {
-          const T nv = Nv[l * (q + 1) + s];
-#pragma unroll
-          for (int c = 0; c < NC; ++c) acc[c] += nv * R[k][s][c];
-        }
-        const int m = out_row(k, l);
-#pragma unroll
-        for (int c = 0; c < NC; ++c) h[m][c] = acc[c];
-      }
-    }
-    if (RAT) {
-      // EG_EvaluateQuotientRule2 for dim 3, der 2, stride 4 (evaluation.cu:77-128).
-      const T F = T(1) / h[0][3];
-      if (MODE == kEvalPoint) {
-#pragma unroll
-        for (int c = 0; c < 3; ++c) h[0][c] *= F;
-      } else {
-#pragma unroll
-        for (int m = 0; m < 6; ++m)
-#pragma unroll
-          for (int c = 0; c < 4; ++c) h[m][c] *= F;
-        const T ws = -h[1][3], wt = -h[2][3];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          h[1][c] += ws * h[0][c];
-          h[2][c] += wt * h[0][c];
-        }
-        const T wss = -h[3][3], wst = -h[4][3], wtt = -h[5][3];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          const T S = h[0][c], Su = h[1][c], Sv = h[2][c];
-          h[3][c] += wss * S + T(2) * ws * Su;
-          h[4][c] += wst * S + wt * Su + ws * Sv;
-          h[5][c] += wtt * S + T(2) * wt * Sv;
-        }
-      }
-    }
-#pragma unroll
-    for (int m = 0; m < 6; ++m)
-#pragma unroll
-      for (int c = 0; c < 3; ++c) res[3 * m + c] = h[m][c];
-  }
-  if (MODE == kEvalFull18) {
-    if (active) {
-      T* o = out + ((size_t)unit * npts + pt) * 18;
-#pragma unroll
-      for (int e = 0; e < 18; ++e) o[e] = res[e];
-    }
-    return;
-  }
-  if (active) {
-    T* o = out + (size_t)unit * 3 * npts;
-    o[pt] = res[0];
-    o[npts + pt] = res[1];
-    o[2 * npts + pt] = res[2];
-  }
-  if (MODE == kEvalPointK) {
-    // compute_K maxima (common.cu:155-168); max is exact and order-free.
-    unsigned long long mk[3];
-#pragma unroll
-    for (int g = 0; g < 3; ++g) {
-      const T v = cmax(fabs(res[9 + 3 * g]), cmax(fabs(res[10 + 3 * g]), fabs(res[11 + 3 * g])));
-      mk[g] = active ? ord_bits(v) : 0ull;
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) {
-        const unsigned long long o = __shfl_xor(mk[g], off);
-        mk[g] = o > mk[g] ? o : mk[g];
-      }
-    }
-    __shared__ unsigned long long sk[4][3];
-    const int wv = threadIdx.x >> 6;
-    if (lane_id() == 0)
-      for (int g = 0; g < 3; ++g) sk[wv][g] = mk[g];
-    __syncthreads();
-    if (threadIdx.x < 3 && unit < nunits) {
-      unsigned long long m = 0;
-      for (int w2 = 0; w2 < 4; ++w2) m = sk[w2][threadIdx.x] > m ? sk[w2][threadIdx.x] : m;
-      atomicMax(&kmax[unit * 3 + threadIdx.x], m);
-    }
-  }
 }
 
 // Fused evaluation + distance AABB centres.  One wavefront per 16x16-cell
@@ -2375,62 +2202,299 @@ hipError_t launch_promote(const float* src, T* dst, int n, hipStream_t s) {
   return hipGetLastError();
 }
 
-template <class T>
-hipError_t launch_basis(const T* knots, int nk, int p, int der, const vec4_t<T>* regions, int nunits, int nuv,
-                        int dir, T* table, int* spans, hipStream_t s) {
-  const long long n = (long long)nunits * nuv;
-  if (n <= 0) return hipSuccess;
-  const int steps_given = dir >> 1;
-  dir &= 1;
-  const dim3 g(blocks_for(n, 128)), b(128);
-  if (p <= 3)
-    hipLaunchKernelGGL((k_basis<T, 3>), g, b, 0, s, knots, nk, p, der, regions, nunits, nuv, dir, steps_given,
-                       table, spans);
-  else if (p <= 7)
-    hipLaunchKernelGGL((k_basis<T, 7>), g, b, 0, s, knots, nk, p, der, regions, nunits, nuv, dir, steps_given,
-                       table, spans);
-  else
-    hipLaunchKernelGGL((k_basis<T, kMaxDeg - 1>), g, b, 0, s, knots, nk, p, der, regions, nunits, nuv, dir,
-                       steps_given, table, spans);
-  return hipGetLastError();
+// Grid points, one workgroup per 16 x 16-point tile of a unit's nuv x nuv grid
+// (EvaluateOneUV, evaluation.cu:259-356; all 18 values, or the positions and
+// the compute_K maxima).  The tile's 16 columns and 16 rows get their spans
+// and A2.3 basis rows (derivatives up to min(2, degree); knots in LDS when
+// KL), the u-contractions R[k][s] = sum_j Nu[k][j] (w) P of each (column,
+// control row) are formed once into LDS and shared by the column's points, and
+// each point adds its q+1 rows in the reference order for every (k, l), then
+// applies the quotient rule.  The operations on every value are those of
+// k_points' single-point recipe, so the results are bit-identical; the
+// contraction is shared, the control points are read once per column and row
+// instead of once per point and derivative order, and no basis tables go
+// through HBM.  Control rows are processed in chunks of PtRows.  BICUBIC
+// (degree 3 in u and v) uses the fixed-degree basis recurrence, so the
+// kernel needs no scratch memory.
+template <int MAXD>
+struct PtRows {
+  static constexpr int value = MAXD <= 3 ? 8 : 4;
+};
+
+template <class T, int MAXD, bool RAT, bool KL, int MODE, bool BICUBIC>
+__global__ __launch_bounds__(256) void k_points_tile(SurfDev<T> sd, const vec4_t<T>* regions, int nunits, int nuv,
+                                                     int steps_given, T* out, unsigned long long* kmax) {
+  constexpr int TP = kTile;  // points per tile edge
+  constexpr int NB = MAXD + 1;
+  constexpr int NC = RAT ? 4 : 3;
+  constexpr int NK = MODE == kEvalPoint ? 1 : 3;  // derivative orders kept per direction
+  constexpr int kRows = PtRows<MAXD>::value;
+  constexpr int kKn = KL ? kEcKnots : 1;
+  __shared__ T sKu[kKn], sKv[kKn];
+  __shared__ T sNu[TP][NK][NB], sNv[TP][NK][NB];
+  __shared__ int sSu[TP], sSv[TP];
+  __shared__ T sR[TP][kRows][NK][NC];
+  __shared__ int sV0, sNr;
+  __shared__ unsigned long long sk[4][3];
+  const int tid = threadIdx.x;
+  const int tw = (nuv + TP - 1) / TP, tpu = tw * tw;
+  const int unit = blockIdx.x / tpu, t = blockIdx.x % tpu;
+  if (unit >= nunits) return;  // whole workgroup
+  const int row0 = (t / tw) * TP, col0 = (t % tw) * TP;
+  const int ht = min(TP, nuv - row0), wd = min(TP, nuv - col0);  // points of the tile
+  const int p = sd.degu, q = sd.degv;
+  const int du = p < 2 ? p : 2, dv = q < 2 ? q : 2;
+  const int ku = MODE == kEvalPoint ? 0 : du;  // u orders formed
+  const int lv = MODE == kEvalPoint ? 0 : dv;  // v orders formed
+  const vec4_t<T> r = regions[unit];
+  const int ex = tid % TP, ey = tid / TP;
+  const bool active = ex < wd && ey < ht;
+  const int npts = nuv * nuv;
+  const int pt = (row0 + ey) * nuv + col0 + ex;
+  // phase 0: knot vectors in LDS
+  if (KL && !sd.zero) {
+    for (int x = tid; x < sd.nku; x += 256) sKu[x] = sd.U[x];
+    for (int x = tid; x < sd.nkv; x += 256) sKv[x] = sd.V[x];
+  }
+  __syncthreads();
+  // phase 1: spans and basis rows of the tile's columns (u) and rows (v);
+  // the parameter follows the grid driver (evaluation.cu:366)
+  if (!sd.zero && tid < 2 * TP) {
+    const int dir = tid >= TP;
+    const int e = dir ? tid - TP : tid;
+    if (e < (dir ? ht : wd)) {
+      const T start = dir ? r.y : r.x;
+      const T step = steps_given ? (dir ? r.w : r.z) : ((dir ? r.w : r.z) - start) / T(nuv);
+      const int idx = (dir ? row0 : col0) + e;
+      const T tt = step * T(idx) + start;
+      const int deg = dir ? q : p, der = dir ? dv : du;
+      const T* knots = KL ? (dir ? sKv : sKu) : (dir ? sd.V : sd.U);
+      const int span = find_span(dir ? sd.nkv : sd.nku, deg, tt, knots);
+      T(*const nb)[NK][NB] = dir ? sNv : sNu;
+      if (BICUBIC) {  // degree 3 both ways: the register-resident recurrence (no scratch)
+        T d[3][4];
+        basis_ders_p<T, 3>(span, tt, knots, der, d);
+#pragma unroll
+        for (int k = 0; k < NK; ++k)
+#pragma unroll
+          for (int j = 0; j <= 3; ++j) nb[e][k][j] = k <= der ? d[k][j] : T(0);
+      } else {
+        T d[3][MAXD + 1];
+        basis_ders<T, MAXD>(span, deg, tt, knots, der, d);
+        for (int k = 0; k < NK; ++k)
+          for (int j = 0; j <= deg; ++j) nb[e][k][j] = k <= der ? d[k][j] : T(0);
+      }
+      (dir ? sSv : sSu)[e] = span;
+    }
+  }
+  __syncthreads();
+  // control rows the tile reads: [vlo - q, vhi]
+  if (tid == 0) {
+    int vlo = INT_MAX, vhi = INT_MIN;
+    if (!sd.zero)
+      for (int e = 0; e < ht; ++e) {
+        vlo = min(vlo, sSv[e]);
+        vhi = max(vhi, sSv[e]);
+      }
+    sV0 = vlo - q;
+    sNr = sd.zero ? 0 : vhi - vlo + q + 1;
+  }
+  __syncthreads();
+  const int v0 = sV0, nrows = sNr;
+  T h[6][4];
+#pragma unroll
+  for (int m = 0; m < 6; ++m)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) h[m][c] = T(0);
+  const int sv = active && !sd.zero ? sSv[ey] : 0;
+  for (int r0 = 0; r0 < nrows; r0 += kRows) {
+    const int nr = min(kRows, nrows - r0);
+    // phase 2: u-contractions of rows [r0, r0 + nr) for every column, all u
+    // orders from one read of each control point
+    for (int task = tid; task < wd * nr; task += 256) {
+      const int rr = task % nr, cx = task / nr;
+      const int base = sd.ncpu * (v0 + r0 + rr) + sSu[cx] - p;
+      T acc[NK][NC];
+#pragma unroll
+      for (int k = 0; k < NK; ++k)
+#pragma unroll
+        for (int c = 0; c < NC; ++c) acc[k][c] = T(0);
+      constexpr int UR = MAXD <= 7 ? MAXD + 1 : 1;
+#pragma unroll UR
+      for (int j = 0; j <= MAXD; ++j) {
+        if (j > p) break;
+        const int id = base + j;
+        const T wj = RAT ? sd.W[id] : T(1);
+        const T px = sd.CP[3 * id + 0], py = sd.CP[3 * id + 1], pz = sd.CP[3 * id + 2];
+#pragma unroll
+        for (int k = 0; k < NK; ++k) {
+          if (k > ku) break;
+          const T n = sNu[cx][k][j];
+          if (RAT) {
+            const T nw = n * wj;
+            acc[k][0] += nw * px;
+            acc[k][1] += nw * py;
+            acc[k][2] += nw * pz;
+            acc[k][NC - 1] += nw;
+          } else {
+            acc[k][0] += n * px;
+            acc[k][1] += n * py;
+            acc[k][2] += n * pz;
+          }
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < NK; ++k)
+        if (k <= ku)
+#pragma unroll
+          for (int c = 0; c < NC; ++c) sR[cx][rr][k][c] = acc[k][c];
+    }
+    __syncthreads();
+    // phase 3: each point adds this chunk's rows, in s order, for every (k, l)
+    if (active && !sd.zero) {
+#pragma unroll
+      for (int l = 0; l < NK; ++l)
+#pragma unroll
+        for (int k = 0; k < NK - l; ++k) {  // unrolled: h stays in registers
+          if (l > lv || k > ku) continue;
+          const int m = out_row(k, l);
+          constexpr int UR = MAXD <= 7 ? MAXD + 1 : 1;
+#pragma unroll UR
+          for (int s2 = 0; s2 <= MAXD; ++s2) {
+            if (s2 > q) break;
+            const int rr = sv - q + s2 - v0 - r0;
+            if (rr < 0 || rr >= nr) continue;
+            const T nv = sNv[ey][l][s2];
+#pragma unroll
+            for (int c = 0; c < NC; ++c) h[m][c] += nv * sR[ex][rr][k][c];
+          }
+        }
+    }
+    __syncthreads();  // sR is rewritten by the next chunk
+  }
+  if (RAT && active && !sd.zero) {
+    // EG_EvaluateQuotientRule2 for dim 3, der 2, stride 4 (evaluation.cu:77-128).
+    const T F = T(1) / h[0][3];
+    if (MODE == kEvalPoint) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) h[0][c] *= F;
+    } else {
+#pragma unroll
+      for (int m = 0; m < 6; ++m)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) h[m][c] *= F;
+      const T ws = -h[1][3], wt = -h[2][3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        h[1][c] += ws * h[0][c];
+        h[2][c] += wt * h[0][c];
+      }
+      const T wss = -h[3][3], wst = -h[4][3], wtt = -h[5][3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const T S = h[0][c], Su = h[1][c], Sv = h[2][c];
+        h[3][c] += wss * S + T(2) * ws * Su;
+        h[4][c] += wst * S + wt * Su + ws * Sv;
+        h[5][c] += wtt * S + T(2) * wt * Sv;
+      }
+    }
+  }
+  if (MODE == kEvalFull18) {
+    if (active) {
+      T* o = out + ((size_t)unit * npts + pt) * 18;
+#pragma unroll
+      for (int m = 0; m < 6; ++m)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) o[3 * m + c] = h[m][c];
+    }
+    return;
+  }
+  if (active) {
+    T* o = out + (size_t)unit * 3 * npts;
+    o[pt] = h[0][0];
+    o[npts + pt] = h[0][1];
+    o[2 * npts + pt] = h[0][2];
+  }
+  if (MODE == kEvalPointK) {
+    // compute_K maxima (common.cu:155-168); max is exact and order-free.
+    unsigned long long mk[3];
+#pragma unroll
+    for (int g = 0; g < 3; ++g) {
+      const T v = cmax(fabs(h[3 + g][0]), cmax(fabs(h[3 + g][1]), fabs(h[3 + g][2])));
+      mk[g] = active ? ord_bits(v) : 0ull;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_xor(mk[g], off);
+        mk[g] = o > mk[g] ? o : mk[g];
+      }
+    }
+    const int wv = tid >> 6;
+    if (lane_id() == 0)
+      for (int g = 0; g < 3; ++g) sk[wv][g] = mk[g];
+    __syncthreads();
+    if (tid < 3) {
+      unsigned long long m = 0;
+      for (int w2 = 0; w2 < 4; ++w2) m = sk[w2][tid] > m ? sk[w2][tid] : m;
+      atomicMax(&kmax[unit * 3 + tid], m);
+    }
+  }
 }
 
-template <class T, int MAXD, bool RAT>
-static void points_mode(const SurfDev<T>& sd, const T* tU, const int* sU, const T* tV, const int* sV, int nunits,
-                        int nuv, int mode, T* out, unsigned long long* kmax, hipStream_t s) {
-  const int bpu = (nuv * nuv + 255) / 256;
-  const dim3 g((unsigned)((long long)bpu * nunits)), b(256);
+template <class T, int MAXD, bool RAT, bool KL, bool BIC>
+static void points_tile_mode(const SurfDev<T>& sd, const vec4_t<T>* reg, int nunits, int nuv, int mode,
+                             int steps_given, T* out, unsigned long long* kmax, hipStream_t s) {
+  const int tw = (nuv + kTile - 1) / kTile;
+  const dim3 g((unsigned)((long long)nunits * tw * tw)), b(256);
   if (mode == kEvalFull18)
-    hipLaunchKernelGGL((k_points<T, MAXD, RAT, kEvalFull18>), g, b, 0, s, sd, tU, sU, tV, sV, nunits, nuv, out,
-                       kmax);
+    hipLaunchKernelGGL((k_points_tile<T, MAXD, RAT, KL, kEvalFull18, BIC>), g, b, 0, s, sd, reg, nunits, nuv,
+                       steps_given, out, kmax);
   else if (mode == kEvalPoint)
-    hipLaunchKernelGGL((k_points<T, MAXD, RAT, kEvalPoint>), g, b, 0, s, sd, tU, sU, tV, sV, nunits, nuv, out,
-                       kmax);
+    hipLaunchKernelGGL((k_points_tile<T, MAXD, RAT, KL, kEvalPoint, BIC>), g, b, 0, s, sd, reg, nunits, nuv,
+                       steps_given, out, kmax);
   else
-    hipLaunchKernelGGL((k_points<T, MAXD, RAT, kEvalPointK>), g, b, 0, s, sd, tU, sU, tV, sV, nunits, nuv, out,
-                       kmax);
+    hipLaunchKernelGGL((k_points_tile<T, MAXD, RAT, KL, kEvalPointK, BIC>), g, b, 0, s, sd, reg, nunits, nuv,
+                       steps_given, out, kmax);
+}
+
+template <class T, int MAXD, bool RAT, bool KL>
+static void points_tile_bic(const SurfDev<T>& sd, const vec4_t<T>* reg, int nunits, int nuv, int mode,
+                            int steps_given, T* out, unsigned long long* kmax, hipStream_t s) {
+  if (MAXD == 3 && sd.degu == 3 && sd.degv == 3 && !sd.zero)
+    points_tile_mode<T, MAXD, RAT, KL, true>(sd, reg, nunits, nuv, mode, steps_given, out, kmax, s);
+  else
+    points_tile_mode<T, MAXD, RAT, KL, false>(sd, reg, nunits, nuv, mode, steps_given, out, kmax, s);
 }
 
 template <class T, int MAXD>
-static void points_rat(const SurfDev<T>& sd, const T* tU, const int* sU, const T* tV, const int* sV, int nunits,
-                       int nuv, int mode, T* out, unsigned long long* kmax, hipStream_t s) {
-  if (sd.rational)
-    points_mode<T, MAXD, true>(sd, tU, sU, tV, sV, nunits, nuv, mode, out, kmax, s);
-  else
-    points_mode<T, MAXD, false>(sd, tU, sU, tV, sV, nunits, nuv, mode, out, kmax, s);
+static void points_tile_rat(const SurfDev<T>& sd, const vec4_t<T>* reg, int nunits, int nuv, int mode,
+                            int steps_given, T* out, unsigned long long* kmax, hipStream_t s) {
+  const bool kl = sd.nku <= kEcKnots && sd.nkv <= kEcKnots;
+  if (sd.rational) {
+    if (kl)
+      points_tile_bic<T, MAXD, true, true>(sd, reg, nunits, nuv, mode, steps_given, out, kmax, s);
+    else
+      points_tile_bic<T, MAXD, true, false>(sd, reg, nunits, nuv, mode, steps_given, out, kmax, s);
+  } else {
+    if (kl)
+      points_tile_bic<T, MAXD, false, true>(sd, reg, nunits, nuv, mode, steps_given, out, kmax, s);
+    else
+      points_tile_bic<T, MAXD, false, false>(sd, reg, nunits, nuv, mode, steps_given, out, kmax, s);
+  }
 }
 
 template <class T>
-hipError_t launch_points(const SurfDev<T>& sd, const T* tabU, const int* spanU, const T* tabV, const int* spanV,
-                         int nunits, int nuv, int mode, T* out, unsigned long long* kmax, hipStream_t s) {
+hipError_t launch_points_tile(const SurfDev<T>& sd, const vec4_t<T>* reg, int nunits, int nuv, int mode,
+                              int steps_given, T* out, unsigned long long* kmax, hipStream_t s) {
   if (nunits <= 0 || nuv <= 0) return hipSuccess;
-  const int q = sd.degv;
-  if (q <= 3)
-    points_rat<T, 3>(sd, tabU, spanU, tabV, spanV, nunits, nuv, mode, out, kmax, s);
-  else if (q <= 7)
-    points_rat<T, 7>(sd, tabU, spanU, tabV, spanV, nunits, nuv, mode, out, kmax, s);
+  const long long tw = (nuv + kTile - 1) / kTile;
+  if ((long long)nunits * tw * tw > 0x7fffffffll) return hipErrorInvalidValue;
+  const int d = std::max(sd.degu, sd.degv);
+  if (d <= 3)
+    points_tile_rat<T, 3>(sd, reg, nunits, nuv, mode, steps_given, out, kmax, s);
+  else if (d <= 7)
+    points_tile_rat<T, 7>(sd, reg, nunits, nuv, mode, steps_given, out, kmax, s);
   else
-    points_rat<T, kMaxDeg - 1>(sd, tabU, spanU, tabV, spanV, nunits, nuv, mode, out, kmax, s);
+    points_tile_rat<T, kMaxDeg - 1>(sd, reg, nunits, nuv, mode, steps_given, out, kmax, s);
   return hipGetLastError();
 }
 
@@ -2905,10 +2969,6 @@ hipError_t launch_fill_u32(unsigned* p, unsigned value, unsigned long long n, hi
 // ---- explicit instantiations ----------------------------------------------
 #define JSI_INST(T)                                                                                                 \
   template hipError_t launch_promote<T>(const float*, T*, int, hipStream_t);                                       \
-  template hipError_t launch_basis<T>(const T*, int, int, int, const vec4_t<T>*, int, int, int, T*, int*,          \
-                                      hipStream_t);                                                                \
-  template hipError_t launch_points<T>(const SurfDev<T>&, const T*, const int*, const T*, const int*, int, int,    \
-                                       int, T*, unsigned long long*, hipStream_t);                                 \
   template hipError_t launch_aabb<T>(const T*, int, int, int, int, const unsigned long long*, const vec4_t<T>*, T, \
                                      T, vec4_t<T>*, vec4_t<T>*, hipStream_t);                                      \
   template hipError_t launch_tilebox<T>(const vec4_t<T>*, const vec4_t<T>*, int, int, int, vec4_t<T>*,             \
@@ -2925,6 +2985,8 @@ hipError_t launch_fill_u32(unsigned* p, unsigned value, unsigned long long n, hi
                                             unsigned long long*, int, int, const unsigned long long*);             \
   template hipError_t launch_set_thresh<T>(T*, const Cand*, int, int, hipStream_t);                                \
   template hipError_t launch_eval_centres<T>(const EcJob<T>&, const EcJob<T>*, hipStream_t);                      \
+  template hipError_t launch_points_tile<T>(const SurfDev<T>&, const vec4_t<T>*, int, int, int, int, T*,           \
+                                            unsigned long long*, hipStream_t);                                     \
   template hipError_t launch_seed_thresh<T>(const unsigned long long*, int, int, int, int, uint64_t*, unsigned*,   \
                                             unsigned long long*, unsigned long long*, const int*, const int*, int, \
                                             int, const vec4_t<T>*, const vec4_t<T>*, int, int, T*, hipStream_t,    \

@@ -27,6 +27,68 @@ __device__ int find_span(int nk, int p, T u, const T* U) {
   return mid;
 }
 
+// basis_ders with the degree P fixed at compile time: every loop unrolls and
+// the arrays stay in registers; the operations and their order are those of
+// basis_ders (der <= 2).
+template <class T, int P>
+__device__ __forceinline__ void basis_ders_p(int span, T u, const T* U, int der, T (&ders)[3][P + 1]) {
+  T ndu[P + 1][P + 1];
+  T left[P + 1], right[P + 1];
+  T a[2][P + 1];
+  ndu[0][0] = T(1);
+#pragma unroll
+  for (int j = 1; j <= P; ++j) {
+    left[j] = u - U[span + 1 - j];
+    right[j] = U[span + j] - u;
+    T saved = T(0);
+#pragma unroll
+    for (int r = 0; r < j; ++r) {
+      ndu[j][r] = right[r + 1] + left[j - r];
+      const T temp = ndu[r][j - 1] / ndu[j][r];
+      ndu[r][j] = saved + right[r + 1] * temp;
+      saved = left[j - r] * temp;
+    }
+    ndu[j][j] = saved;
+  }
+#pragma unroll
+  for (int j = 0; j <= P; ++j) ders[0][j] = ndu[j][P];
+#pragma unroll
+  for (int r = 0; r <= P; ++r) {
+    a[0][0] = T(1);
+#pragma unroll
+    for (int k = 1; k <= 2; ++k) {
+      if (k > der) break;
+      const int s1 = (k - 1) & 1, s2 = k & 1;  // the rows swap after every order
+      T d = T(0);
+      const int rk = r - k, pk = P - k;
+      if (r >= k) {
+        a[s2][0] = a[s1][0] / ndu[pk + 1][rk];
+        d = a[s2][0] * ndu[rk][pk];
+      }
+      const int j1 = (rk >= -1) ? 1 : -rk;
+      const int j2 = (r - 1 <= pk) ? k - 1 : P - r;
+#pragma unroll
+      for (int j = j1; j <= j2; ++j) {
+        a[s2][j] = (a[s1][j] - a[s1][j - 1]) / ndu[pk + 1][rk + j];
+        d += a[s2][j] * ndu[rk + j][pk];
+      }
+      if (r <= pk) {
+        a[s2][k] = -a[s1][k - 1] / ndu[pk + 1][r];
+        d += a[s2][k] * ndu[r][pk];
+      }
+      ders[k][r] = d;
+    }
+  }
+  int r = P;
+#pragma unroll
+  for (int k = 1; k <= 2; ++k) {
+    if (k > der) break;
+#pragma unroll
+    for (int j = 0; j <= P; ++j) ders[k][j] *= T(r);
+    r *= P - k;
+  }
+}
+
 template <class T, int MAXP>
 __device__ void basis_ders(int span, int p, T u, const T* U, int der, T (&ders)[3][MAXP + 1]) {
   T ndu[MAXP + 1][MAXP + 1];

@@ -164,28 +164,12 @@ int jsi_ctx::surf_dev(int h, SurfDev<T>* out) {
 template <class T>
 int jsi_ctx::eval_units(int h, const vec4_t<T>* reg, int nunits, int nuv, int mode, int steps_given, T* out,
                          unsigned long long* kmax, const char* tag) {
+  (void)tag;
   SurfDev<T> sd;
   RCK(surf_dev<T>(h, &sd));
-  if (sd.zero) {
-    // periodic / unsupported degree: all-zero output, as EvaluateOneUV (evaluation.cu:271-272)
-  }
-  const int p = std::max(sd.degu, 1), q = std::max(sd.degv, 1);
-  std::string nu = std::string(tag) + ".tabU", nv = std::string(tag) + ".tabV";
-  std::string su = std::string(tag) + ".spanU", sv = std::string(tag) + ".spanV";
-  T *tU, *tV;
-  int *sU, *sV;
-  const size_t nn = (size_t)nunits * nuv;
-  RCK(buf(nu.c_str(), nn * 3 * (p + 1), &tU));
-  RCK(buf(nv.c_str(), nn * 3 * (q + 1), &tV));
-  RCK(buf(su.c_str(), nn, &sU));
-  RCK(buf(sv.c_str(), nn, &sV));
-  const int der_u = std::min(2, sd.degu), der_v = std::min(2, sd.degv);
-  if (!sd.zero) {
-    HCK(launch_basis<T>(sd.U, sd.nku, sd.degu, der_u, reg, nunits, nuv, 0 | (steps_given << 1), tU, sU, stream));
-    HCK(launch_basis<T>(sd.V, sd.nkv, sd.degv, der_v, reg, nunits, nuv, 1 | (steps_given << 1), tV, sV, stream));
-  }
+  // sd.zero (periodic / unsupported degree): all-zero output, as EvaluateOneUV (evaluation.cu:271-272)
   if (mode == kEvalPointK) HCK(launch_fill_u32(reinterpret_cast<unsigned*>(kmax), 0u, (size_t)nunits * 6, stream));
-  HCK(launch_points<T>(sd, tU, sU, tV, sV, nunits, nuv, mode, out, kmax, stream));
+  HCK(launch_points_tile<T>(sd, reg, nunits, nuv, mode, steps_given, out, kmax, stream));
   return JSI_OK;
 }
 
