@@ -224,6 +224,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--polish", type=int, default=64, help="top-k records polished to FP64 closest points")
     ap.add_argument("--isect-steps", type=int, default=5, help="intersection-curve queries (saddle / wave plane)")
+    ap.add_argument("--eval-steps", type=int, default=5, help="grid evaluations (evalWithCuda, nuv=1024) per dtype")
     ap.add_argument("--batch", type=int, default=256, help="queries of the uneven batch (0 = skip)")
     ap.add_argument("--serve-rates", default="500,800", help="Poisson arrival rates of the serving runs (req/s)")
     ap.add_argument("--serve-seconds", type=float, default=2.0, help="arrival window of the serving run (0 = skip)")
@@ -296,6 +297,25 @@ def main():
                  "pairs_per_round": [r["npairs"] for r in ires.rounds_list()], "curves": ncur,
                  "samples": int(len(ipts)), "max_residual": float(ipts["residual"].max()) if len(ipts) else None}
 
+    # grid evaluation (evalWithCuda: 18 values per point to host memory) of surface A
+    # at nuv=1024; the kernel is timed with HIP events, its bytes are the 18
+    # values per point it writes to HBM (the device -> host copy is not timed)
+    evaluation = None
+    if args.eval_steps > 0:
+        evaluation = {"workload": "evalWithCuda grid of the hard-wave surface, nuv=1024 (1,048,576 points, 18 values each)",
+                      "peak_GBps": 8000.0}
+        for dt, name in ((np.float32, "f32"), (np.float64, "f64")):
+            ctx.eval_grid(a, 1.0 / 1024, 1.0 / 1024, 0.0, 0.0, 1024, dtype=dt)  # warm-up
+            ctx.profile(True)
+            for _ in range(args.eval_steps):
+                ctx.eval_grid(a, 1.0 / 1024, 1.0 / 1024, 0.0, 0.0, 1024, dtype=dt)
+            n, ms, nbytes = ctx.kernel_stats("eval_points18_" + name)
+            ctx.profile(False)
+            gbps = nbytes / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+            evaluation[name] = {"kernel": "k_points_tile", "avg_launch_ms": ms / max(n, 1),
+                                "bytes_per_launch": nbytes / max(n, 1), "achieved_GBps": gbps,
+                                "frac": gbps / 8000.0}
+
     # dominant screen kernel of the chosen form (most summed time), timed with HIP events
     dom = max(stats, key=lambda n: stats[n][1])
     launches, kms, pairs = stats[dom]
@@ -353,6 +373,8 @@ def main():
     }
     if isect:
         out["intersection_curves"] = isect
+    if evaluation:
+        out["evaluation"] = evaluation
     if args.batch > 0:
         # uneven batch: many surface pairs of different degree, net size and nuv in flight together
         bq, bcells = uneven_queries(ctx, args.batch, 512, args.polish)

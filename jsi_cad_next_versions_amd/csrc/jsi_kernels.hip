@@ -2400,12 +2400,24 @@ __global__ __launch_bounds__(256) void k_points_tile(SurfDev<T> sd, const vec4_t
     }
   }
   if (MODE == kEvalFull18) {
-    if (active) {
-      T* o = out + ((size_t)unit * npts + pt) * 18;
+    // 18 values per point, point-major: a tile row is one contiguous run of
+    // 18 wd values, so half a tile at a time goes through LDS and is written
+    // with consecutive lanes on consecutive addresses
+    __shared__ T sOut[TP / 2][TP * 18];
 #pragma unroll
-      for (int m = 0; m < 6; ++m)
+    for (int half = 0; half < 2; ++half) {
+      if (active && (ey >> 3) == half)
 #pragma unroll
-        for (int c = 0; c < 3; ++c) o[3 * m + c] = h[m][c];
+        for (int m = 0; m < 6; ++m)
+#pragma unroll
+          for (int c = 0; c < 3; ++c) sOut[ey & 7][ex * 18 + 3 * m + c] = h[m][c];
+      __syncthreads();
+      const int rows = min(TP / 2, ht - (TP / 2) * half), run = wd * 18;
+      for (int e = tid; e < rows * run; e += 256) {
+        const int ry = e / run, off = e - ry * run;
+        out[((size_t)unit * npts + (size_t)(row0 + (TP / 2) * half + ry) * nuv + col0) * 18 + off] = sOut[ry][off];
+      }
+      __syncthreads();
     }
     return;
   }

@@ -169,7 +169,20 @@ int jsi_ctx::eval_units(int h, const vec4_t<T>* reg, int nunits, int nuv, int mo
   RCK(surf_dev<T>(h, &sd));
   // sd.zero (periodic / unsupported degree): all-zero output, as EvaluateOneUV (evaluation.cu:271-272)
   if (mode == kEvalPointK) HCK(launch_fill_u32(reinterpret_cast<unsigned*>(kmax), 0u, (size_t)nunits * 6, stream));
+  if (prof) HCK(hipEventRecord(ev[0], stream));
   HCK(launch_points_tile<T>(sd, reg, nunits, nuv, mode, steps_given, out, kmax, stream));
+  if (prof) {  // bytes written per launch: 18 values (grid) or xyz (units) per point
+    HCK(hipEventRecord(ev[1], stream));
+    HCK(hipEventSynchronize(ev[1]));
+    float ms = 0.f;
+    HCK(hipEventElapsedTime(&ms, ev[0], ev[1]));
+    const std::string key = std::string(mode == kEvalFull18 ? "eval_points18_" : "eval_points_") +
+                            (std::is_same<T, float>::value ? "f32" : "f64");
+    KernelStat& kst = stats[key];
+    kst.launches += 1;
+    kst.ms += ms;
+    kst.units += (double)nunits * nuv * nuv * (mode == kEvalFull18 ? 18 : 3) * sizeof(T);
+  }
   return JSI_OK;
 }
 
