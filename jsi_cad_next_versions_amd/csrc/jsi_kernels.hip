@@ -1234,7 +1234,7 @@ __global__ __launch_bounds__(256) void k_dist_filter_mfma(const uint64_t* list, 
   const double mm = __longlong_as_double((long long)mb);
   const double delta = 32.0 * 1.1102230246251565e-16 * (tv + 4.0 * mm * mm);
   const int nA = w1 * w1, nB = w2 * w2;
-  const int nAb = (nA + 15) >> 4, nBb = (nB + 15) >> 4;  // side-1 (columns) and side-2 (rows) blocks
+  const int nAb = (nA + 15) >> 4;  // side-1 blocks (columns)
   const double qnan = quiet_nan<double>();
   int wcount = 0;
   auto flush = [&]() {
@@ -1275,29 +1275,24 @@ __global__ __launch_bounds__(256) void k_dist_filter_mfma(const uint64_t* list, 
     if (t < nA) a = C1[(size_t)v1 * nA + t];
     return a;
   };
-  auto load_b = [&](int v2, double (&o)[4]) {  // A operands of this wavefront's side-2 blocks
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int cell = 16 * (wv + 4 * q) + col;
-      o[q] = qnan;
-      if (wv + 4 * q < nBb && cell < nB) {
-        const double4 bb = C2[(size_t)v2 * nB + cell];
-        o[q] = kq == 0 ? -2.0 * bb.x : (kq == 1 ? -2.0 * bb.y : (kq == 2 ? -2.0 * bb.z : bb.x * bb.x + bb.y * bb.y + bb.z * bb.z));
-      }
-    }
+  auto load_b = [&](int v2) {  // side-2 cell t (its A operand column is formed at staging)
+    double4 b;
+    b.x = b.y = b.z = b.w = qnan;
+    if (t < nB) b = C2[(size_t)v2 * nB + t];
+    return b;
   };
+  // the next slot's cells are loaded as they are stored (no arithmetic on
+  // them until the staging), so the loads stay in flight across the products
   double4 a_cur = load_a(wu1[0]);
-  double b_cur[4];
-  load_b(wu2[0], b_cur);
+  double4 b_cur = load_b(wu2[0]);
 #pragma unroll
   for (int k = 0; k < kMaxWalk; ++k) {
     if (wslot[k] < 0) break;
     const int slot = wslot[k];
-    double4 a_nxt;
-    double b_nxt[4];
+    double4 a_nxt, b_nxt;
     if (k + 1 < kMaxWalk && wslot[k + 1] >= 0) {
       a_nxt = load_a(wu1[k + 1]);
-      load_b(wu2[k + 1], b_nxt);
+      b_nxt = load_b(wu2[k + 1]);
     }
     __syncthreads();  // the previous slot's operands are no longer read
     {
@@ -1309,13 +1304,17 @@ __global__ __launch_bounds__(256) void k_dist_filter_mfma(const uint64_t* list, 
       sB[t >> 4][48 + (t & 15)] = in ? 1.0 : qnan;
       sT[t] = in ? (tv - (av.x * av.x + av.y * av.y + av.z * av.z)) + delta : -infinity_v<double>();
     }
-    double aop[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      aop[q] = b_cur[q];
-      sA[wv + 4 * q][lane] = aop[q];
+    {  // A operand column of side-2 cell t: lane (row t%16, k) of block t/16 holds [-2bx -2by -2bz |b|^2][k]
+      const double4 bv = b_cur;
+      sA[t >> 4][t & 15] = -2.0 * bv.x;
+      sA[t >> 4][16 + (t & 15)] = -2.0 * bv.y;
+      sA[t >> 4][32 + (t & 15)] = -2.0 * bv.z;
+      sA[t >> 4][48 + (t & 15)] = bv.x * bv.x + bv.y * bv.y + bv.z * bv.z;
     }
     __syncthreads();
+    double aop[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) aop[q] = sA[wv + 4 * q][lane];
     if (threadIdx.x == 0) done += (unsigned long long)nA * (unsigned long long)nB;
     const uint64_t key_s = pack_idx((uint32_t)slot, 0u, 0u);
     double bop = sB[0][lane], thr = sT[col];
@@ -1327,6 +1326,8 @@ __global__ __launch_bounds__(256) void k_dist_filter_mfma(const uint64_t* list, 
       mfma_v4d d[4];
 #pragma unroll
       for (int q = 0; q < 4; ++q) d[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(aop[q], bop, z, 0, 0, 0);
+      // all four products issue before the first minimum waits on a result
+      __builtin_amdgcn_sched_barrier(0);
       const double m01 = min_nan(min_nan(min_nan(d[0][0], d[0][1]), min_nan(d[0][2], d[0][3])),
                                  min_nan(min_nan(d[1][0], d[1][1]), min_nan(d[1][2], d[1][3])));
       const double m23 = min_nan(min_nan(min_nan(d[2][0], d[2][1]), min_nan(d[2][2], d[2][3])),
@@ -1368,8 +1369,7 @@ __global__ __launch_bounds__(256) void k_dist_filter_mfma(const uint64_t* list, 
     }
     if (k + 1 < kMaxWalk && wslot[k + 1] >= 0) {
       a_cur = a_nxt;
-#pragma unroll
-      for (int q = 0; q < 4; ++q) b_cur[q] = b_nxt[q];
+      b_cur = b_nxt;
     }
   }
   if (wcount) flush();
