@@ -838,6 +838,51 @@ __global__ void k_dist_list(int nslots, const int* su1, const int* su2, int w1, 
   }
 }
 
+// The same test with one workgroup per (slot, side-1 tile) and its threads
+// over the side-2 tiles (many tiles per unit, round 1): the side-1 box and
+// the unit ids are uniform loads, the side-2 boxes are read coalesced, and
+// each workgroup appends its kept tile pairs with one atomic per 256 tiles.
+template <class T, bool FMA>
+__global__ __launch_bounds__(256) void k_dist_list_rows(int nslots, const int* su1, const int* su2, int w1, int w2,
+                                                        const vec4_t<T>* lo1, const vec4_t<T>* hi1,
+                                                        const vec4_t<T>* lo2, const vec4_t<T>* hi2,
+                                                        const T* thresh, int cull, uint64_t* list,
+                                                        unsigned* list_count) {
+  __shared__ unsigned wsum[4], sbase;
+  const int tpu1 = tiles_per_unit(w1), tpu2 = tiles_per_unit(w2);
+  const int pos = blockIdx.x;  // slot * tpu1 + t1
+  if (pos >= nslots * tpu1) return;  // whole workgroup
+  const int slot = pos / tpu1, t1 = pos - slot * tpu1;
+  const int a = su1[slot] * tpu1 + t1, ub = su2[slot] * tpu2;
+  const vec4_t<T> l1 = lo1[a], h1 = hi1[a];
+  const T th = *thresh;
+  const int lane = lane_id(), wv = threadIdx.x >> 6;
+  for (int c0 = 0; c0 < tpu2; c0 += 256) {
+    const int t2 = c0 + (int)threadIdx.x;
+    bool keep = t2 < tpu2;
+    if (keep && cull) {
+      const vec4_t<T> l2 = lo2[ub + t2], h2 = hi2[ub + t2];
+      const T gx = smax(T(0), smax(l1.x - h2.x, l2.x - h1.x));
+      const T gy = smax(T(0), smax(l1.y - h2.y, l2.y - h1.y));
+      const T gz = smax(T(0), smax(l1.z - h2.z, l2.z - h1.z));
+      const T lb2 = FMA ? fma(gx, gx, fma(gy, gy, gz * gz)) : gx * gx + gy * gy + gz * gz;
+      keep = !(lb2 > th);
+    }
+    const unsigned long long m = __ballot(keep);
+    if (lane == 0) wsum[wv] = (unsigned)__popcll(m);
+    __syncthreads();
+    const unsigned total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    if (total) {  // uniform
+      if (threadIdx.x == 0) sbase = atomicAdd(list_count, total);
+      __syncthreads();
+      unsigned off = sbase;
+      for (int v = 0; v < wv; ++v) off += wsum[v];
+      if (keep) list[off + __popcll(m & ((1ull << lane) - 1))] = tp_encode(slot, t1, t2);
+    }
+    __syncthreads();  // wsum / sbase are rewritten by the next chunk
+  }
+}
+
 // d2 of two centres.  FMA = false: center_distance_sq's order
 // (distance_analysis.cu:202-207), no contraction (EXACT32 / FP64).  FMA = true:
 // fma(dx, dx, fma(dy, dy, dz*dz)) for the FP32SCREEN screen, whose threshold
@@ -2668,6 +2713,16 @@ hipError_t launch_dist_list(int nslots, const int* su1, const int* su2, int w1, 
                             int cull, uint64_t* list, unsigned* list_count, hipStream_t s, int fma) {
   const long long n = n_tile_pairs(nslots, w1, w2);
   if (n <= 0) return hipSuccess;
+  const long long rows = (long long)nslots * tiles_per_unit_h(w1);
+  if (tiles_per_unit_h(w2) >= 64 && rows <= 0x7fffffffll) {  // many side-2 tiles: a workgroup per side-1 tile
+    if (fma)
+      hipLaunchKernelGGL((k_dist_list_rows<T, true>), dim3((unsigned)rows), dim3(256), 0, s, nslots, su1, su2, w1, w2,
+                         lo1, hi1, lo2, hi2, thresh, cull, list, list_count);
+    else
+      hipLaunchKernelGGL((k_dist_list_rows<T, false>), dim3((unsigned)rows), dim3(256), 0, s, nslots, su1, su2, w1,
+                         w2, lo1, hi1, lo2, hi2, thresh, cull, list, list_count);
+    return hipGetLastError();
+  }
   if (fma)
     hipLaunchKernelGGL((k_dist_list<T, true>), dim3(blocks_for(n, 256)), dim3(256), 0, s, nslots, su1, su2, w1, w2,
                        lo1, hi1, lo2, hi2, thresh, cull, list, list_count);
