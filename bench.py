@@ -228,7 +228,11 @@ def main():
     ap.add_argument("--batch", type=int, default=256, help="queries of the uneven batch (0 = skip)")
     ap.add_argument("--serve-rates", default="500,800", help="Poisson arrival rates of the serving runs (req/s)")
     ap.add_argument("--serve-seconds", type=float, default=2.0, help="arrival window of the serving run (0 = skip)")
-    ap.add_argument("--concurrency", type=int, default=8, help="worker streams of the served (batched) form")
+    ap.add_argument("--concurrency", type=int, default=16,
+                    help="worker streams of the served (batched) form (16: the GPU is saturated)")
+    ap.add_argument("--mix-slots", type=int, default=8,
+                    help="worker streams of the uneven batch and the Poisson serving runs (the 4x-scaled pairs "
+                         "of the mix hold more device memory each; 8 serves them faster than 16)")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="also run plain device copies of this many MiB (calibrates the traffic counters)")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
@@ -264,6 +268,13 @@ def main():
     # served / batched: the same queries through the serving loop, several in flight
     res, wall, lat = run_served(ctx, ctx.query(0, a, r1, b, r2, p), args.steps, args.concurrency, 1)
     qps = args.steps / wall
+    # the same with fewer streams: throughput against latency
+    served_by_streams = {}
+    for nst in (4, 8):
+        if nst < args.concurrency:
+            _, w2, l2 = run_served(ctx, ctx.query(0, a, r1, b, r2, p), args.steps, nst, 1)
+            served_by_streams[str(nst)] = {"queries_per_s": args.steps / w2, "p50_ms": pct(l2, 50),
+                                           "p99_ms": pct(l2, 99)}
     # drop-in EXACT32 mode (bit-exact with the reference, FP32 end to end)
     pe = default_params(k=args.k, precision="exact32", cull=1)
     eres, ewall, elat = run_form(ctx, a, r1, b, r2, pe, max(1, args.steps // 2), 1)
@@ -347,6 +358,7 @@ def main():
         "p50_ms": pct(lat, 50),
         "p99_ms": pct(lat, 99),
         "concurrency": args.concurrency,
+        "served_by_streams": served_by_streams,
         "alone": {"queries_per_s": args.steps / awall, "p50_ms": pct(alat, 50), "p99_ms": pct(alat, 99)},
         "cells_per_s": cells * qps,
         "distance": res.distance,
@@ -378,15 +390,15 @@ def main():
     if args.batch > 0:
         # uneven batch: many surface pairs of different degree, net size and nuv in flight together
         bq, bcells = uneven_queries(ctx, args.batch, 512, args.polish)
-        ctx.run_batch(bq[: min(16, len(bq))], concurrency=args.concurrency)  # warm-up
+        ctx.run_batch(bq[: min(16, len(bq))], concurrency=args.mix_slots)  # warm-up
         t1 = time.perf_counter()
-        bres, bst = ctx.run_batch(bq, concurrency=args.concurrency)
+        bres, bst = ctx.run_batch(bq, concurrency=args.mix_slots)
         bwall = time.perf_counter() - t1
         out["batch_uneven"] = {
             "workload": f"{args.batch} pairs, 70% distance (FP32SCREEN, k=512, polish {args.polish}) / 30% "
                         "intersection (FP64, 6 rounds); degrees 2-5, nets 8-96, weights U[0.7,1.3], nuv 64-512",
             "queries_per_s": args.batch / bwall, "cells_per_s": bcells / bwall, "seconds": bwall,
-            "failed": int(sum(1 for x in bst if x != 0)), "concurrency": args.concurrency}
+            "failed": int(sum(1 for x in bst if x != 0)), "concurrency": args.mix_slots}
     if args.serve_seconds > 0:
         # serving: Poisson arrivals of the uneven mix plus 10% 4x-scaled hard-wave pairs at nuv=1024
         sq, _ = uneven_queries(ctx, 90, 512, args.polish, seed0=10000)
@@ -399,7 +411,7 @@ def main():
         out["serving_poisson"] = {
             "workload": "uneven mix + 10% 4x-scaled hard-wave pairs (nuv=1024, k=4096), Poisson arrivals, "
                         "latency = arrival -> result",
-            "runs": [run_poisson(ctx, mix, float(r), args.serve_seconds, args.concurrency)
+            "runs": [run_poisson(ctx, mix, float(r), args.serve_seconds, args.mix_slots)
                      for r in args.serve_rates.split(",") if r.strip()]}
     if not args.no_cpu:
         out["cpu_baseline"], odist = cpu_baseline(a, r1, b, r2, args.k, args.cpu_queries, args.polish)
