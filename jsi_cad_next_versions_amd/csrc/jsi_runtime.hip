@@ -943,8 +943,10 @@ int jsi_ctx::intersect(int h1, const jsi_region& r1, int h2, const jsi_region& r
       RCK(buf("crv.out", npairs, &d_cr));
       HCK(launch_curve_proj<T>(s1, s2, pairs, (long long)npairs, su1, su2, reg1, reg2, nuv1, nuv2, region_box(r1),
                                region_box(r2), d_cr, stream));
+      const double tc0 = now_ms();
       std::vector<CurveRec> hc(npairs);
       RCK(d2h(hc.data(), d_cr, sizeof(CurveRec) * npairs));
+      if (prof) host_stat("curve_records_d2h", now_ms() - tc0);
       RCK(trace_curves(hc.data(), (long long)npairs, s1, s2, region_box(r1), region_box(r2), creq));
     }
   }
@@ -964,7 +966,9 @@ int jsi_ctx::intersect(int h1, const jsi_region& r1, int h2, const jsi_region& r
 int jsi_ctx::trace_curves(const CurveRec* recs, long long n, const SurfDev<double>& s1, const SurfDev<double>& s2,
                           double4 box1, double4 box2, CurveReq* creq) {
   std::vector<MarchSeed> seeds;
+  double th = now_ms();
   const double hstep = curve_seeds(recs, n, creq->tol, creq->spacing, seeds);
+  if (prof) host_stat("curve_seeds", now_ms() - th);
   const int ns = (int)seeds.size();
   if (ns == 0) return JSI_OK;
   MarchSeed* d_seeds;
@@ -1005,14 +1009,20 @@ int jsi_ctx::trace_curves(const CurveRec* recs, long long n, const SurfDev<doubl
   std::vector<MarchOut> fwd, bwd;
   std::vector<long long> fwd_off, bwd_off;
   std::vector<MarchPt> fwd_pts, bwd_pts;
+  th = now_ms();
   RCK(march(all, 1.0, fwd, fwd_off, fwd_pts));
+  if (prof) host_stat("curve_march_fwd", now_ms() - th);
   std::vector<int> indeg(ns, 0), heads;
   for (int k = 0; k < ns; ++k)
     if (fwd[k].succ >= 0) ++indeg[fwd[k].succ];
   for (int k = 0; k < ns; ++k)
     if (indeg[k] == 0) heads.push_back(k);
+  th = now_ms();
   RCK(march(heads, -1.0, bwd, bwd_off, bwd_pts));
+  if (prof) host_stat("curve_march_bwd", now_ms() - th);
+  th = now_ms();
   assemble_curves(seeds, fwd, fwd_off, fwd_pts, heads, bwd, bwd_off, bwd_pts, creq->points, &creq->ncurves);
+  if (prof) host_stat("curve_assemble", now_ms() - th);
   return JSI_OK;
 }
 

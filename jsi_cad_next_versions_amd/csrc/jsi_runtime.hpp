@@ -83,6 +83,12 @@ struct jsi_ctx {
   void* thresh_ptr_ = nullptr;  // device threshold of the screen in flight
   bool debug_ = false;          // JSI_DEBUG: per-screen counts on stderr
 
+  // profiling: host-side time of a named step (launches = calls, ms = summed time)
+  void host_stat(const char* name, double ms) {
+    KernelStat& st = stats[name];
+    st.launches += 1;
+    st.ms += ms;
+  }
   int buf_raw(const char* name, size_t bytes, void** out, int fill_int);
   template <class X>
   int buf(const char* name, size_t count, X** out) {

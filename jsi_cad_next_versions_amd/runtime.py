@@ -14,8 +14,9 @@ import os
 
 import numpy as np
 
-from ._abi import (JSI_ERR_BUSY, JSI_OK, MAX_ROUNDS, QUERY_DISTANCE, QUERY_INTERSECT, CurvePoint,  # noqa: F401
-                   Pair, Params, Query, Region, Result, curve_points_to_numpy, default_params, pairs_to_numpy)
+from ._abi import (CURVE_DTYPE, JSI_ERR_BUSY, JSI_OK, MAX_ROUNDS, QUERY_DISTANCE, QUERY_INTERSECT,  # noqa: F401
+                   CurvePoint, Pair, Params, Query, Region, Result, curve_points_to_numpy, default_params,
+                   pairs_to_numpy)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libjsi_mi355x.so")
@@ -187,12 +188,14 @@ class Context:
         params = params or default_params()
         h1, h2 = self.upload(s1), self.upload(s2)
         res = Result()
-        buf = (CurvePoint * max(cap, 1))()
+        dt = np.dtype(CURVE_DTYPE)
+        assert dt.itemsize == C.sizeof(CurvePoint)
+        buf = np.empty(max(cap, 1), dtype=dt)  # not cleared: the library writes the first min(cap, n) records
         n, nc = C.c_int(), C.c_int()
         _check(self.L, self.L.jsi_intersect_curves(self.h, h1, C.byref(r1), h2, C.byref(r2), C.byref(params),
-                                                   spacing, tol, C.byref(res), C.cast(buf, C.c_void_p), cap,
+                                                   spacing, tol, C.byref(res), C.c_void_p(buf.ctypes.data), cap,
                                                    C.byref(n), C.byref(nc)))
-        return res, curve_points_to_numpy(buf, min(cap, n.value)), nc.value
+        return res, buf[: min(cap, n.value)].copy(), nc.value
 
     def query(self, kind, s1, r1, s2, r2, params=None):
         """A jsi_query for run_batch / Server.submit (uploads the surfaces)."""
