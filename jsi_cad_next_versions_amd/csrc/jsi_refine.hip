@@ -27,7 +27,6 @@
 #include <cfloat>
 #include <cmath>
 #include <cstdint>
-#include <unordered_map>
 #include <vector>
 
 #include "../../include/jsi_mi355x.h"
@@ -781,15 +780,20 @@ __device__ inline int clamp_fix(double* x, const double* lo, const double* hi) {
   return worst < 0 ? 0 : 1 << worst;
 }
 
-// One thread per march.  Counting pass (pts == nullptr): out[k] receives the
-// successor, the number of recorded points and the end flags; writing pass:
-// the same march again, recording its points at pts + offs[k].
+// One wave per march: every lane steps the same march (the values are the
+// same in all lanes), and the search for a seed within h/2 of the new point
+// runs over 64 seeds at a time (the lowest index with a hit wins, as in a
+// sequential scan; seeds are at least h apart, so at most one can hit).
+// Counting pass (pts == nullptr): out[k] receives the successor, the number of
+// recorded points and the end flags; writing pass: the same march again,
+// recording its points at pts + offs[k].  Lane 0 writes.
 template <int MAXP, bool FAST>
 __global__ __launch_bounds__(64) void k_march(SurfDev<double> s1, SurfDev<double> s2, const MarchSeed* seeds,
                                               int nseeds, const int* starts, int nstart, double dir, double h,
                                               double4 box1, double4 box2, const long long* offs, MarchPt* pts,
                                               MarchOut* out) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = blockIdx.x;
+  const int lane = threadIdx.x;
   if (k >= nstart) return;
   const int s = starts[k];
   const double lo[4] = {box1.x, box1.y, box2.x, box2.y};
@@ -872,11 +876,17 @@ __global__ __launch_bounds__(64) void k_march(SurfDev<double> s1, SurfDev<double
     }
     for (int q = 0; q < 3; ++q) p[q] = E1[q];
     int hit = -1;
-    for (int q = 0; q < nseeds; ++q) {
-      if (q == s && step == 0) continue;  // the start seed counts only once the march has left it
-      const double dq[3] = {p[0] - seeds[q].p[0], p[1] - seeds[q].p[1], p[2] - seeds[q].p[2]};
-      if (dot3(dq, dq) < rr2) {
-        hit = q;
+    for (int q0 = 0; q0 < nseeds; q0 += 64) {
+      const int q = q0 + lane;
+      bool near = false;
+      // the start seed counts only once the march has left it
+      if (q < nseeds && !(q == s && step == 0)) {
+        const double dq[3] = {p[0] - seeds[q].p[0], p[1] - seeds[q].p[1], p[2] - seeds[q].p[2]};
+        near = dot3(dq, dq) < rr2;
+      }
+      const unsigned long long m = __ballot(near);
+      if (m != 0ull) {
+        hit = q0 + __ffsll((long long)m) - 1;
         break;
       }
     }
@@ -885,7 +895,7 @@ __global__ __launch_bounds__(64) void k_march(SurfDev<double> s1, SurfDev<double
       flags |= JSI_MARCH_SEED;
       break;
     }
-    if (dst && n < room) {
+    if (dst && n < room && lane == 0) {
       MarchPt& o = dst[n];
       for (int q = 0; q < 4; ++q) o.x[q] = x[q];
       for (int q = 0; q < 3; ++q) o.p[q] = E1[q];
@@ -904,7 +914,7 @@ __global__ __launch_bounds__(64) void k_march(SurfDev<double> s1, SurfDev<double
   o.n = n;
   o.flags = flags;
   o.pad = 0;
-  out[k] = o;
+  if (lane == 0) out[k] = o;
 }
 
 }  // namespace
@@ -913,7 +923,7 @@ hipError_t launch_march(const SurfDev<double>& s1, const SurfDev<double>& s2, co
                         const int* starts, int nstart, double dir, double h, double4 box1, double4 box2,
                         const long long* offs, MarchPt* pts, MarchOut* out, hipStream_t s) {
   if (nstart <= 0) return hipSuccess;
-  const dim3 g((unsigned)((nstart + 63) / 64));
+  const dim3 g((unsigned)nstart);
   JSI_REFINE_DISPATCH(k_march, g, dim3(64), 0, s, s1, s2, seeds, nseeds, starts, nstart, dir, h, box1, box2, offs,
                       pts, out)
   return hipGetLastError();
@@ -943,12 +953,55 @@ inline double dist2(const double* p, const double* q) {
   return dot3(d, d);
 }
 
+// Cells of the kept seeds: open addressing with linear probing over a
+// power-of-two table (at most half full); each cell holds the index of its
+// last seed, earlier ones are chained through `next`.
+struct SeedCells {
+  std::vector<CellKey> key;
+  std::vector<int> head;  // -1: empty slot
+  std::vector<int> next;  // per seed
+  size_t used = 0;
+  SeedCells() : key(1024), head(1024, -1) {}
+  size_t slot(const CellKey& k) const {
+    const size_t mask = head.size() - 1;
+    size_t i = CellHash()(k) & mask;
+    while (head[i] >= 0 && !(key[i] == k)) i = (i + 1) & mask;
+    return i;
+  }
+  int first(const CellKey& k) const { return head[slot(k)]; }
+  void add(const CellKey& k, int seed) {
+    if (2 * (used + 1) > head.size()) grow();
+    const size_t i = slot(k);
+    if (head[i] < 0) {
+      key[i] = k;
+      ++used;
+    }
+    next.push_back(head[i]);
+    head[i] = seed;
+  }
+  void grow() {
+    std::vector<CellKey> ok;
+    std::vector<int> oh;
+    ok.swap(key);
+    oh.swap(head);
+    key.assign(ok.size() * 2, CellKey{0, 0, 0});
+    head.assign(oh.size() * 2, -1);
+    for (size_t i = 0; i < oh.size(); ++i)
+      if (oh[i] >= 0) {
+        const size_t j = slot(ok[i]);
+        key[j] = ok[i];
+        head[j] = oh[i];
+      }
+  }
+};
+
 }  // namespace
 
 double curve_seeds(const CurveRec* recs, long long n, double tol, double spacing, std::vector<MarchSeed>& seeds) {
   seeds.clear();
   const double tol2 = tol * tol;
   std::vector<long long> acc;
+  acc.reserve((size_t)n);
   for (long long e = 0; e < n; ++e)
     if (recs[e].r2 <= tol2) acc.push_back(e);
   double h = spacing;
@@ -967,23 +1020,28 @@ double curve_seeds(const CurveRec* recs, long long n, double tol, double spacing
     if (!(h > 0.0)) h = 1.0;
   }
   const double h2 = h * h;
-  std::unordered_map<CellKey, std::vector<int>, CellHash> grid;
+  SeedCells grid;
+  // The seed that dropped the previous record is tried first: records in pair
+  // order mostly lie close to each other.  Whether a record is dropped depends
+  // only on whether some kept seed lies closer than h, not on which one.
+  int last = -1;
   for (long long e : acc) {
     const CurveRec& r = recs[e];
+    if (last >= 0 && dist2(r.p, seeds[last].p) < h2) continue;
     const CellKey k = cell_of(r.p, h);
-    bool near = false;
-    for (int da = -1; da <= 1 && !near; ++da)
-      for (int db = -1; db <= 1 && !near; ++db)
-        for (int dc = -1; dc <= 1 && !near; ++dc) {
-          auto it = grid.find(CellKey{k.a + da, k.b + db, k.c + dc});
-          if (it == grid.end()) continue;
-          for (int q : it->second)
+    int near = -1;
+    for (int da = -1; da <= 1 && near < 0; ++da)
+      for (int db = -1; db <= 1 && near < 0; ++db)
+        for (int dc = -1; dc <= 1 && near < 0; ++dc)
+          for (int q = grid.first(CellKey{k.a + da, k.b + db, k.c + dc}); q >= 0; q = grid.next[q])
             if (dist2(r.p, seeds[q].p) < h2) {
-              near = true;
+              near = q;
               break;
             }
-        }
-    if (near) continue;
+    if (near >= 0) {
+      last = near;
+      continue;
+    }
     const double tt = dot3(r.t, r.t);
     if (!(tt > 0.0)) continue;  // tangential contact: no direction to march
     MarchSeed sd;
@@ -992,7 +1050,7 @@ double curve_seeds(const CurveRec* recs, long long n, double tol, double spacing
     const double inv = 1.0 / std::sqrt(tt);
     for (int c = 0; c < 3; ++c) sd.T[c] = r.t[c] * inv;
     sd.r2 = r.r2;
-    grid[k].push_back((int)seeds.size());
+    grid.add(k, (int)seeds.size());
     seeds.push_back(sd);
   }
   return h;

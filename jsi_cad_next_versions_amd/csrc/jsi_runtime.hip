@@ -13,6 +13,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -944,10 +945,10 @@ int jsi_ctx::intersect(int h1, const jsi_region& r1, int h2, const jsi_region& r
       HCK(launch_curve_proj<T>(s1, s2, pairs, (long long)npairs, su1, su2, reg1, reg2, nuv1, nuv2, region_box(r1),
                                region_box(r2), d_cr, stream));
       const double tc0 = now_ms();
-      std::vector<CurveRec> hc(npairs);
-      RCK(d2h(hc.data(), d_cr, sizeof(CurveRec) * npairs));
+      std::unique_ptr<CurveRec[]> hc(new CurveRec[npairs]);  // filled by the copy
+      RCK(d2h(hc.get(), d_cr, sizeof(CurveRec) * npairs));
       if (prof) host_stat("curve_records_d2h", now_ms() - tc0);
-      RCK(trace_curves(hc.data(), (long long)npairs, s1, s2, region_box(r1), region_box(r2), creq));
+      RCK(trace_curves(hc.get(), (long long)npairs, s1, s2, region_box(r1), region_box(r2), creq));
     }
   }
   if (out && cap_pairs > 0 && npairs > 0) {
