@@ -13,7 +13,6 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
-#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -103,6 +102,19 @@ int jsi_ctx::buf_raw(const char* name, size_t bytes, void** out, int fill_int) {
     }
   }
   *out = b.p;
+  return JSI_OK;
+}
+
+int jsi_ctx::pinned(size_t bytes, void** out) {
+  if (bytes > pinned_bytes_) {
+    if (pinned_) HCK(hipHostFree(pinned_));
+    pinned_ = nullptr;
+    pinned_bytes_ = 0;
+    const size_t want = std::max<size_t>(bytes, (size_t)1 << 20);
+    HCK(hipHostMalloc(&pinned_, want, hipHostMallocDefault));
+    pinned_bytes_ = want;
+  }
+  *out = pinned_;
   return JSI_OK;
 }
 
@@ -942,13 +954,25 @@ int jsi_ctx::intersect(int h1, const jsi_region& r1, int h2, const jsi_region& r
       RCK(surf_dev<double>(h2, &s2));
       CurveRec* d_cr;
       RCK(buf("crv.out", npairs, &d_cr));
+      if (prof) HCK(hipEventRecord(ev[0], stream));
       HCK(launch_curve_proj<T>(s1, s2, pairs, (long long)npairs, su1, su2, reg1, reg2, nuv1, nuv2, region_box(r1),
                                region_box(r2), d_cr, stream));
+      if (prof) {
+        HCK(hipEventRecord(ev[1], stream));
+        HCK(hipEventSynchronize(ev[1]));
+        float ms = 0.f;
+        HCK(hipEventElapsedTime(&ms, ev[0], ev[1]));
+        KernelStat& kst = stats["curve_proj"];
+        kst.launches += 1;
+        kst.ms += ms;
+        kst.units += (double)npairs;
+      }
       const double tc0 = now_ms();
-      std::unique_ptr<CurveRec[]> hc(new CurveRec[npairs]);  // filled by the copy
-      RCK(d2h(hc.get(), d_cr, sizeof(CurveRec) * npairs));
+      void* hc;
+      RCK(pinned(sizeof(CurveRec) * npairs, &hc));
+      RCK(d2h(hc, d_cr, sizeof(CurveRec) * npairs));
       if (prof) host_stat("curve_records_d2h", now_ms() - tc0);
-      RCK(trace_curves(hc.get(), (long long)npairs, s1, s2, region_box(r1), region_box(r2), creq));
+      RCK(trace_curves(static_cast<const CurveRec*>(hc), (long long)npairs, s1, s2, region_box(r1), region_box(r2), creq));
     }
   }
   if (out && cap_pairs > 0 && npairs > 0) {
@@ -1207,6 +1231,7 @@ int jsi_ctx_destroy(jsi_ctx* c) {
   }
   for (hipEvent_t e : c->ev)
     if (e) (void)hipEventDestroy(e);
+  if (c->pinned_) (void)hipHostFree(c->pinned_);
   (void)hipStreamDestroy(c->stream);
   delete c;
   return JSI_OK;

@@ -97,6 +97,10 @@ struct jsi_ctx {
     *out = static_cast<X*>(p);
     return r;
   }
+  // page-locked host staging (grown on demand, freed with the context)
+  void* pinned_ = nullptr;
+  size_t pinned_bytes_ = 0;
+  int pinned(size_t bytes, void** out);
   int d2h(void* dst, const void* src, size_t bytes);
   int h2d(void* dst, const void* src, size_t bytes);
 
