@@ -1231,29 +1231,34 @@ __global__ __launch_bounds__(256) void k_dist_filter_units(const uint64_t* list,
 // the column.  Operand layouts of the instruction: A lane l = (row l%16, k
 // l/16), B lane l = (k l/16, column l%16), result register r of lane l = (row
 // 4r + l/16, column l%16): a lane's results share its column, hence one
-// threshold per lane and side-1 block, and "any pass" is min(results) <= thr
-// (15 minima and one compare for 64 pairs per lane).  One workgroup per slot:
-// the side-1 B operands and thresholds are staged in LDS (16 blocks of 16
-// cells), wavefront w keeps the A operands of side-2 blocks w, w+4, w+8, w+12
-// in registers and, for each side-1 block, issues its four products before
-// testing them.  Padded cells carry NaN, which never passes (and min ignores).
+// threshold per lane and side-1 block, and "any pass" is min(results) <= thr.
+// One workgroup per slot: the side-1 B operands and thresholds are staged in
+// LDS (blocks of 16 cells).  Only full blocks go to the matrix cores (a unit of
+// 225 cells has 14 full blocks and one cell): wavefront w keeps the A operands
+// of side-2 blocks w, w+4, .. in registers and the products of the leftover
+// full side-2 blocks are shared out so that the four wavefronts do equal work
+// (49 products each for 14 x 14 blocks, where padding both sides to 16-cell
+// blocks and splitting by side-2 block gave one wavefront 60); for each side-1
+// block a wavefront issues its products before testing them.  Pairs with a
+// cell of a partial block take the reference test d2 <= T on the vector ALU.
 // delta covers the rounding of a 4-term dot product in any order (at most
 // 36 u m^2 here) and the other terms of the expanded test (the bound is
 // checked by the screen-bound tests).  A passing pair's key is the reference
 // d2 recomputed from a and b, so the records equal the direct test's.
 typedef double mfma_v4d __attribute__((ext_vector_type(4)));
 
-// v_min_f64 on values that are finite or quiet NaN (products of the matrix
-// core): the smaller operand, or the other one when one is NaN.  Written as the
-// instruction because fmin() canonicalises both operands first (two extra
-// instructions per minimum) to quiet signalling NaNs, which cannot occur here.
+// v_min_f64 on finite values (products of full blocks): written as the
+// instruction because fmin() first canonicalises both operands (two extra
+// instructions per minimum).  The compiler does not see inline asm as a reader
+// of a matrix-core result, so callers insert the wait states themselves.
 __device__ __forceinline__ double min_nan(double a, double b) {
   double r;
   asm("v_min_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
   return r;
 }
 
-__global__ __launch_bounds__(256) void k_dist_filter_mfma(const uint64_t* list, const unsigned* list_count,
+template <int NQ>  // NQ = (side-2 cells / 16) / 4: full side-2 blocks each wavefront holds in registers
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NQ < 4 ? 4 : 3))) void k_dist_filter_mfma(const uint64_t* list, const unsigned* list_count,
                                                           int nslots, const int* su1, const int* su2, int w1, int w2,
                                                           const double4* C1, const double4* C2, const double* thresh,
                                                           Cand* out, unsigned long long* out_count,
@@ -1279,7 +1284,8 @@ __global__ __launch_bounds__(256) void k_dist_filter_mfma(const uint64_t* list, 
   const double mm = __longlong_as_double((long long)mb);
   const double delta = 32.0 * 1.1102230246251565e-16 * (tv + 4.0 * mm * mm);
   const int nA = w1 * w1, nB = w2 * w2;
-  const int nAb = (nA + 15) >> 4;  // side-1 blocks (columns)
+  const int nAf = nA >> 4, nBf = nB >> 4;  // full blocks of 16 cells (side-1: columns, side-2: rows)
+  const int nL = nBf - 4 * NQ;              // full side-2 blocks beyond the wavefronts' NQ each (0..3)
   const double qnan = quiet_nan<double>();
   int wcount = 0;
   auto flush = [&]() {
@@ -1301,18 +1307,19 @@ __global__ __launch_bounds__(256) void k_dist_filter_mfma(const uint64_t* list, 
   // order of the slots does not matter).
   constexpr int kMaxWalk = 8;
   const bool all = n == (unsigned)nslots;
-  int wslot[kMaxWalk], wu1[kMaxWalk], wu2[kMaxWalk];
-#pragma unroll
-  for (int k = 0; k < kMaxWalk; ++k) {
+  struct WalkSlot {
+    int slot, u1, u2;  // slot -1: past the end of the walk
+  };
+  auto walk_slot = [&](int k) {
+    WalkSlot w{-1, 0, 0};
     const unsigned i = blockIdx.x + (unsigned)k * gridDim.x;
-    wslot[k] = -1;
-    wu1[k] = wu2[k] = 0;
-    if (i < n) {
-      wslot[k] = all ? (int)i : (int)(list[i] >> 32);
-      wu1[k] = su1[wslot[k]];
-      wu2[k] = su2[wslot[k]];
+    if (k < kMaxWalk && i < n) {
+      w.slot = all ? (int)i : (int)(list[i] >> 32);
+      w.u1 = su1[w.slot];
+      w.u2 = su2[w.slot];
     }
-  }
+    return w;
+  };
   const int t = threadIdx.x;
   auto load_a = [&](int v1) {  // side-1 cell t
     double4 a;
@@ -1326,19 +1333,23 @@ __global__ __launch_bounds__(256) void k_dist_filter_mfma(const uint64_t* list, 
     if (t < nB) b = C2[(size_t)v2 * nB + t];
     return b;
   };
-  // the next slot's cells are loaded as they are stored (no arithmetic on
-  // them until the staging), so the loads stay in flight across the products
-  double4 a_cur = load_a(wu1[0]);
-  double4 b_cur = load_b(wu2[0]);
-#pragma unroll
-  for (int k = 0; k < kMaxWalk; ++k) {
-    if (wslot[k] < 0) break;
-    const int slot = wslot[k];
+  // The slot ids are read two slots ahead and the next slot's cells are
+  // loaded as they are stored (no arithmetic on them until the staging), so
+  // the loads stay in flight across the products.  The walk is a rolled loop
+  // (its body holds the product loops of every segment: unrolled copies would
+  // outgrow the instruction cache).
+  WalkSlot cur = walk_slot(0), nxt = walk_slot(1);
+  double4 a_cur = load_a(cur.u1);
+  double4 b_cur = load_b(cur.u2);
+#pragma unroll 1
+  for (int k = 0; cur.slot >= 0; ++k) {
+    const int slot = cur.slot;
     double4 a_nxt, b_nxt;
-    if (k + 1 < kMaxWalk && wslot[k + 1] >= 0) {
-      a_nxt = load_a(wu1[k + 1]);
-      b_nxt = load_b(wu2[k + 1]);
+    if (nxt.slot >= 0) {
+      a_nxt = load_a(nxt.u1);
+      b_nxt = load_b(nxt.u2);
     }
+    const WalkSlot nxt2 = walk_slot(k + 2);
     __syncthreads();  // the previous slot's operands are no longer read
     {
       const double4 av = a_cur;
@@ -1357,50 +1368,133 @@ __global__ __launch_bounds__(256) void k_dist_filter_mfma(const uint64_t* list, 
       sA[t >> 4][48 + (t & 15)] = bv.x * bv.x + bv.y * bv.y + bv.z * bv.z;
     }
     __syncthreads();
-    double aop[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) aop[q] = sA[wv + 4 * q][lane];
     if (threadIdx.x == 0) done += (unsigned long long)nA * (unsigned long long)nB;
     const uint64_t key_s = pack_idx((uint32_t)slot, 0u, 0u);
-    double bop = sB[0][lane], thr = sT[col];
+    // Appends the passing pairs of up to four products: product q covers side-1
+    // block jq[q] and side-2 block rq[q]; bit 4q + r of `mask` = result r of q.
+    auto append = [&](const mfma_v4d* d, const double* th, const int* jq, const int* rq, int cnt) {
+      unsigned mask = 0;
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        if (q < cnt)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) mask |= (d[q][r] <= th[q] ? 1u : 0u) << (4 * q + r);
+      for (;;) {
+        const bool has = mask != 0u;
+        const unsigned long long mk = __ballot(has);
+        if (!mk) break;
+        const int cnt2 = __popcll(mk);
+        if (wcount + cnt2 > kStage) flush();
+        if (has) {
+          const int bit = __builtin_ctz(mask);
+          mask &= mask - 1u;
+          const int q = bit >> 2;
+          const int jb = q == 0 ? jq[0] : q == 1 ? jq[1] : q == 2 ? jq[2] : jq[3];
+          const int rb = q == 0 ? rq[0] : q == 1 ? rq[1] : q == 2 ? rq[2] : rq[3];
+          const int ci = 16 * jb + col;        // side-1 cell (column)
+          const int rr = 4 * (bit & 3) + kq;   // row in the side-2 block
+          const int cj = 16 * rb + rr;         // side-2 cell
+          // a from the B operands, b = -(1/2)(-2b) from the A operands (exact)
+          const double ax = sB[jb][col], ay = sB[jb][16 + col], az = sB[jb][32 + col];
+          const double bx = -0.5 * sA[rb][rr], by = -0.5 * sA[rb][16 + rr], bz = -0.5 * sA[rb][32 + rr];
+          const double dq = pair_d2<double, false>(ax, ay, az, bx, by, bz);
+          Cand c;
+          c.hi = ord_bits(dq);
+          c.lo = key_s | ((uint64_t)(uint32_t)ci << 20) | (uint64_t)(uint32_t)cj;
+          stage[wv][wcount + __popcll(mk & ((1ull << lane) - 1))] = c;
+        }
+        wcount += cnt2;
+      }
+    };
     const mfma_v4d z = {0.0, 0.0, 0.0, 0.0};
-    for (int jb = 0; jb < nAb; ++jb) {
-      const double bop_n = sB[jb + 1][lane];  // next block (row 16 of sB is never written: read, unused)
-      const double thr_n = sT[16 * (jb + 1) + col];
-      // four independent products (a wavefront's absent side-2 blocks have NaN operands)
-      mfma_v4d d[4];
+    // Products of full blocks.  Wavefront w holds side-2 blocks w, w+4, ..
+    // (NQ of them) and multiplies them with every full side-1 block.  The
+    // products of the nL leftover side-2 blocks 4 NQ + i with the side-1
+    // blocks, taken in the order (i, jb), are cut into four contiguous runs,
+    // one per wavefront: a run covers at most two leftover blocks over disjoint
+    // ranges of jb, so the wavefront walks jb in up to four segments, each a
+    // loop with a fixed number of products (NQ, or NQ + 1 with that leftover
+    // block), and every wavefront does the same number of products to within one.
+    if (nBf > 0) {
+      double aop[NQ > 0 ? NQ : 1];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) d[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(aop[q], bop, z, 0, 0, 0);
-      // all four products issue before the first minimum waits on a result
-      __builtin_amdgcn_sched_barrier(0);
-      const double m01 = min_nan(min_nan(min_nan(d[0][0], d[0][1]), min_nan(d[0][2], d[0][3])),
-                                 min_nan(min_nan(d[1][0], d[1][1]), min_nan(d[1][2], d[1][3])));
-      const double m23 = min_nan(min_nan(min_nan(d[2][0], d[2][1]), min_nan(d[2][2], d[2][3])),
-                                 min_nan(min_nan(d[3][0], d[3][1]), min_nan(d[3][2], d[3][3])));
-      if (__ballot(min_nan(m01, m23) <= thr) != 0ull) {
-        // rare: append the passing pairs, one per lane per step, keys recomputed from the cells
-        unsigned mask = 0;  // bit 4q + r: result r of product q passes
+      for (int q = 0; q < NQ; ++q) aop[q] = sA[wv + 4 * q][lane];
+      auto run = [&](auto extra_c, int jb0, int jb1, int rbx) {
+        constexpr bool EXTRA = decltype(extra_c)::value;
+        if (NQ + (EXTRA ? 1 : 0) == 0 || jb0 >= jb1) return;
+        const double ax = EXTRA ? sA[rbx][lane] : 0.0;
+        double bop = sB[jb0][lane], thr = sT[16 * jb0 + col];
+        for (int jb = jb0; jb < jb1; ++jb) {
+          const double bop_n = sB[jb + 1][lane];  // next block (row 16 of sB is never written: read, unused)
+          const double thr_n = sT[16 * (jb + 1) + col];
+          mfma_v4d d[4];  // NQ + EXTRA <= 4 (append reads up to four)
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
+          for (int q = 0; q < NQ; ++q) d[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(aop[q], bop, z, 0, 0, 0);
+          if (EXTRA) d[NQ < 4 ? NQ : 3] = __builtin_amdgcn_mfma_f64_16x16x4f64(ax, bop, z, 0, 0, 0);
+          // all products issue before the minima; the minima are written as the
+          // instruction, so the compiler does not count them as readers of the
+          // products: 19 wait states (a DGEMM 16x16 result read by the vector
+          // ALU) are inserted by hand after the last product
+          __builtin_amdgcn_sched_barrier(0);
+          asm volatile("s_nop 15\n\ts_nop 2" ::: "memory");
+          __builtin_amdgcn_sched_barrier(0);
+          double m = infinity_v<double>();
 #pragma unroll
-          for (int r = 0; r < 4; ++r) mask |= (d[q][r] <= thr ? 1u : 0u) << (4 * q + r);
-        for (;;) {
-          const bool has = mask != 0u;
-          const unsigned long long mk = __ballot(has);
-          if (!mk) break;
+          for (int q = 0; q < NQ + (EXTRA ? 1 : 0); ++q)
+            m = min_nan(m, min_nan(min_nan(d[q][0], d[q][1]), min_nan(d[q][2], d[q][3])));
+          if (__ballot(m <= thr) != 0ull) {  // rare
+            const double th[4] = {thr, thr, thr, thr};
+            const int jq[4] = {jb, jb, jb, jb};
+            int rq[4] = {wv, wv + 4, wv + 8, wv + 12};
+            if (EXTRA && NQ < 4) rq[NQ < 4 ? NQ : 0] = rbx;
+            append(d, th, jq, rq, NQ + (EXTRA ? 1 : 0));
+          }
+          bop = bop_n;
+          thr = thr_n;
+        }
+      };
+      using plain_t = std::integral_constant<bool, false>;
+      using extra_t = std::integral_constant<bool, true>;
+      const int nE = nL * nAf;
+      const int e0 = (nE * wv) >> 2, e1 = (nE * (wv + 1)) >> 2;
+      if (e1 <= e0) {
+        run(plain_t(), 0, nAf, 0);
+      } else {
+        // run [e0, e1): leftover block i0 over jb [j0, min(nAf, j0 + len)), then block i0 + 1 over jb [0, r)
+        const int i0 = e0 / nAf, j0 = e0 - i0 * nAf;
+        const int j1 = min(nAf, j0 + (e1 - e0));
+        const int r = (e1 - e0) - (j1 - j0);
+        run(extra_t(), 0, r, 4 * NQ + i0 + 1);
+        run(plain_t(), r, j0, 0);
+        run(extra_t(), j0, j1, 4 * NQ + i0);
+        run(plain_t(), j1, nAf, 0);
+      }
+    }
+    // Pairs with a cell of a partial block (side-1 cells 16 nAf.., or side-2
+    // cells 16 nBf..): the reference test d2 <= T on the vector ALU, one pair
+    // per thread: each partial side-1 cell against side-2 cell t, then each
+    // partial side-2 cell against full-block side-1 cell t.
+    {
+      const int rA = nA - 16 * nAf, rB = nB - 16 * nBf;
+      for (int e = 0; e < rA + rB; ++e) {
+        const bool s1p = e < rA;
+        const int ci = s1p ? 16 * nAf + e : t;
+        const int cj = s1p ? t : 16 * nBf + (e - rA);
+        const bool valid = s1p ? t < nB : t < 16 * nAf;
+        bool hit = false;
+        double dq = 0.0;
+        if (valid) {
+          const double ax = sB[ci >> 4][ci & 15], ay = sB[ci >> 4][16 + (ci & 15)], az = sB[ci >> 4][32 + (ci & 15)];
+          const double bx = -0.5 * sA[cj >> 4][cj & 15], by = -0.5 * sA[cj >> 4][16 + (cj & 15)],
+                       bz = -0.5 * sA[cj >> 4][32 + (cj & 15)];
+          dq = pair_d2<double, false>(ax, ay, az, bx, by, bz);
+          hit = dq <= tv;
+        }
+        const unsigned long long mk = __ballot(hit);
+        if (mk) {
           const int cnt = __popcll(mk);
           if (wcount + cnt > kStage) flush();
-          if (has) {
-            const int bit = __builtin_ctz(mask);
-            mask &= mask - 1u;
-            const int ci = 16 * jb + col;                  // side-1 cell (column)
-            const int rb = wv + 4 * (bit >> 2);             // side-2 block
-            const int rr = 4 * (bit & 3) + kq;              // row in the block
-            const int cj = 16 * rb + rr;                    // side-2 cell
-            // a from the B operands, b = -(1/2)(-2b) from the A operands (exact)
-            const double ax = sB[jb][col], ay = sB[jb][16 + col], az = sB[jb][32 + col];
-            const double bx = -0.5 * sA[rb][rr], by = -0.5 * sA[rb][16 + rr], bz = -0.5 * sA[rb][32 + rr];
-            const double dq = pair_d2<double, false>(ax, ay, az, bx, by, bz);
+          if (hit) {
             Cand c;
             c.hi = ord_bits(dq);
             c.lo = key_s | ((uint64_t)(uint32_t)ci << 20) | (uint64_t)(uint32_t)cj;
@@ -1409,13 +1503,13 @@ __global__ __launch_bounds__(256) void k_dist_filter_mfma(const uint64_t* list, 
           wcount += cnt;
         }
       }
-      bop = bop_n;
-      thr = thr_n;
     }
-    if (k + 1 < kMaxWalk && wslot[k + 1] >= 0) {
+    if (nxt.slot >= 0) {
       a_cur = a_nxt;
       b_cur = b_nxt;
     }
+    cur = nxt;
+    nxt = nxt2;
   }
   if (wcount) flush();
   if (pairs_done && threadIdx.x == 0 && done) atomicAdd(pairs_done, done);
@@ -2749,8 +2843,18 @@ hipError_t launch_dist_filter(const uint64_t* list, const unsigned* list_count, 
         // workgroup lives a few microseconds per slot, too short to launch one per slot
         // (at most 8 slots per workgroup: the grid grows for more slots)
         const unsigned gp = std::max(std::max(1u, (unsigned)grid / 2u), (gu + 7u) / 8u);
-        hipLaunchKernelGGL(k_dist_filter_mfma, dim3(std::min(gu, gp)), dim3(256), 0, s, list, list_count, (int)gu, su1,
-                           su2, w1, w2, C1, C2, thresh, out, out_count, cap, pairs_done, absmax);
+        const int nq = (w2 * w2 / 16) / 4;
+#define JSI_MFMA_FILTER(NQ)                                                                                       \
+  hipLaunchKernelGGL(k_dist_filter_mfma<NQ>, dim3(std::min(gu, gp)), dim3(256), 0, s, list, list_count, (int)gu, su1, \
+                     su2, w1, w2, C1, C2, thresh, out, out_count, cap, pairs_done, absmax)
+        switch (nq) {
+          case 0: JSI_MFMA_FILTER(0); break;
+          case 1: JSI_MFMA_FILTER(1); break;
+          case 2: JSI_MFMA_FILTER(2); break;
+          case 3: JSI_MFMA_FILTER(3); break;
+          default: JSI_MFMA_FILTER(4); break;
+        }
+#undef JSI_MFMA_FILTER
       } else {
         return hipErrorInvalidValue;
       }

@@ -87,6 +87,25 @@ def test_min_distance_hardwave_matches_oracle(ctx, orc, prec, k):
     np.testing.assert_array_equal(pairs, opairs)
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32screen"])
+@pytest.mark.parametrize("nref", [3, 4, 5, 9, 13, 16, 17])
+def test_min_distance_refine_grids_match_oracle(ctx, orc, prec, nref):
+    """Refinement units of (nref - 1)^2 cells: the refinement filter splits
+    them into full 16-cell blocks (matrix-core products dealt to the four
+    wavefronts) and partial ones (vector test); nref = 3 and 4 have no full
+    block, 5 one, 9 four, 13 nine, 16 fourteen plus one cell, 17 sixteen."""
+    a, b = S.hard_wave_pair()
+    r1, r2 = regions(a, b, 64)
+    p = default_params(k=512, precision=prec, trace=1)
+    p.dist_refine_nuv = nref
+    p.dist_max_rounds = 3
+    res, pairs = ctx.minimum_distance(a, r1, b, r2, p)
+    ores, opairs = orc.min_distance(a, r1, b, r2, p)
+    same_rounds(res, ores)
+    assert res.distance == ores.distance
+    np.testing.assert_array_equal(pairs, opairs)
+
+
 @pytest.mark.parametrize("prec", ["exact32", "fp32screen"])
 @pytest.mark.parametrize("cull", [0, 1])
 def test_culling_leaves_results_unchanged(ctx, orc, cull, prec):
