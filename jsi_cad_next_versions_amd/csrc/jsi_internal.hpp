@@ -42,6 +42,7 @@ struct IPair {
 
 constexpr int kTile = 16;          // cells per tile edge in the screens
 constexpr int kTileCells = kTile * kTile;
+constexpr int kPairsSlots = 32;      // profiling counters of evaluated cell pairs (summed on the host)
 constexpr int kSortBlock = 2048;   // records per LDS bitonic block
 constexpr int kAmaxSlots = 64;     // slots of a max |coordinate| reduction (read as their max)
 

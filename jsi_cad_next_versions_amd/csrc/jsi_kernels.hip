@@ -925,6 +925,23 @@ __device__ __forceinline__ double quiet_nan<double>() {
 // the reference d2 recomputed from a and b = -(1/2)(-2b) (exact), so records
 // are identical to the direct test's; pairs that pass only by the margin carry
 // their true key and cannot displace a smaller one.
+// Profiling count of evaluated cell pairs: the workgroup's wavefront counts
+// (held by lane 0 of each wavefront) are summed in LDS and added to one of
+// kPairsSlots counters picked by workgroup index (thousands of atomics on a
+// single address serialise and cost tens of microseconds per launch).  Every
+// thread of the workgroup calls it.
+__device__ __forceinline__ void add_pairs_done(unsigned long long* pairs_done, unsigned long long v) {
+  __shared__ unsigned long long s_done[8];
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) s_done[w] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long sum = 0;
+    for (int q = 0; q < (int)((blockDim.x + 63) >> 6); ++q) sum += s_done[q];
+    if (sum) atomicAdd(&pairs_done[blockIdx.x & (kPairsSlots - 1)], sum);
+  }
+}
+
 template <class T, bool FMA, bool EXP>
 __global__ __launch_bounds__(256) void k_dist_filter(const uint64_t* list, const unsigned* list_count,
                                                      const int* su1, const int* su2, int w1, int w2,
@@ -1087,7 +1104,7 @@ __global__ __launch_bounds__(256) void k_dist_filter(const uint64_t* list, const
     }
   }
   if (wcount) flush();
-  if (pairs_done && lane == 0 && done) atomicAdd(pairs_done, done);
+  if (pairs_done) add_pairs_done(pairs_done, done);
 }
 
 // Distance filter for slots whose units are single tiles (w1, w2 <= 16: the
@@ -1219,7 +1236,7 @@ __global__ __launch_bounds__(256) void k_dist_filter_units(const uint64_t* list,
     }
   }
   if (wcount) flush();
-  if (pairs_done && threadIdx.x == 0 && done) atomicAdd(pairs_done, done);
+  if (pairs_done) add_pairs_done(pairs_done, done);
 }
 
 // Distance filter of the FP64 refinement rounds on the matrix cores.  The
@@ -1512,7 +1529,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NQ < 4 ? 4 
     nxt = nxt2;
   }
   if (wcount) flush();
-  if (pairs_done && threadIdx.x == 0 && done) atomicAdd(pairs_done, done);
+  if (pairs_done) add_pairs_done(pairs_done, done);
 }
 
 // ---------------------------------------------------------------------------

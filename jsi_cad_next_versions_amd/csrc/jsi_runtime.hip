@@ -320,9 +320,10 @@ int jsi_ctx::dist_topk(int nslots, const int* su1, const int* su2, int w1, int n
                             fma));
     if (!counters_clear) HCK(launch_fill_u32(reinterpret_cast<unsigned*>(out_count), 0u, 2, stream));
     counters_clear = false;  // a repeated screen clears them itself
-    unsigned long long* pairs_done = prof ? reinterpret_cast<unsigned long long*>(list_count + 8) : nullptr;
+    unsigned long long* pairs_done = nullptr;
     if (prof) {
-      HCK(launch_fill_u32(reinterpret_cast<unsigned*>(pairs_done), 0u, 2, stream));
+      RCK(buf("d.pairs", kPairsSlots, &pairs_done));
+      HCK(launch_fill_u32(reinterpret_cast<unsigned*>(pairs_done), 0u, 2 * kPairsSlots, stream));
       HCK(hipEventRecord(ev[0], stream));
     }
     HCK(launch_dist_filter<T>(list, list_count, (unsigned)ntp, su1, su2, w1, w2, C1, C2, thresh, cand, out_count, cap,
@@ -330,7 +331,7 @@ int jsi_ctx::dist_topk(int nslots, const int* su1, const int* su2, int w1, int n
     if (prof) HCK(hipEventRecord(ev[1], stream));
     if (rerank) HCK(launch_rerank(cand, out_count, cap, su1, su2, w1 * w1, w2 * w2, rr.C1, rr.C2, stream));
     HCK(select_topk(cand, cap, k, maxkey, st, hist, small, small_count, small_cap, stream, 2, false, out_count));
-    unsigned long long cnt[4];  // out_count, thresh, pairs_done, small_count (adjacent in d.small)
+    unsigned long long cnt[4];  // out_count, thresh, unused, small_count (adjacent in d.small)
     RCK(d2h(cnt, out_count, sizeof(cnt)));
     n = cnt[0];
     m = cnt[3];
@@ -344,7 +345,11 @@ int jsi_ctx::dist_topk(int nslots, const int* su1, const int* su2, int w1, int n
       KernelStat& kst = stats[key];
       kst.launches += 1;
       kst.ms += ms;
-      kst.units += (double)cnt[2];
+      unsigned long long pd[kPairsSlots];
+      RCK(d2h(pd, pairs_done, sizeof(pd)));
+      double pairs = 0.0;
+      for (int q = 0; q < kPairsSlots; ++q) pairs += (double)pd[q];
+      kst.units += pairs;
     }
     if (debug_) {
       unsigned lc = 0;
