@@ -2855,10 +2855,9 @@ hipError_t launch_dist_filter(const uint64_t* list, const unsigned* list_count, 
     const unsigned gu = list_cap;     // one workgroup per entry: setup latency overlaps across slots
     if (absmax) {
       if constexpr (std::is_same<T, double>::value) {
-        // one resident round of workgroups (4 per CU; 5 fit at this kernel's register use, but 4096 slots
-        // then split 3/4 per workgroup and the last quarter runs a fifth full) walking the slots: a
-        // workgroup lives a few microseconds per slot, too short to launch one per slot
-        // (at most 8 slots per workgroup: the grid grows for more slots)
+        // one resident round of workgroups (4 per CU: the kernel is held to 128 registers for four
+        // wavefronts per SIMD) walking the slots: a workgroup lives a few microseconds per slot, too
+        // short to launch one per slot (at most 8 slots per workgroup: the grid grows for more slots)
         const unsigned gp = std::max(std::max(1u, (unsigned)grid / 2u), (gu + 7u) / 8u);
         const int nq = (w2 * w2 / 16) / 4;
 #define JSI_MFMA_FILTER(NQ)                                                                                       \

@@ -13,7 +13,8 @@
 //    S1(u1,v1) - S2(u2,v2) = 0.
 //  * k_march: tracing of the intersection curves from the projected points
 //    (seeds) by predictor-corrector steps of a fixed length; each seed marches
-//    until the next seed along the curve, a domain edge or a failure, and the
+//    (one wavefront per march) until the next seed along the curve, a domain
+//    edge or a failure, and the
 //    host chains seeds by these successor links into ordered curves
 //    (curve_seeds, assemble_curves).
 //
