@@ -45,11 +45,12 @@ CASES = {
     "hardwave": lambda: S.hard_wave_pair(),
     "paraboloid_plane": lambda: (shapes.quadric(1, 1), shapes.plane(0.5, -0.25, -0.09375)),
     "saddle_edge": lambda: (shapes.quadric(1, -1), shapes.plane(0, 0, 1.25)),
+    "spheres": lambda: (shapes.sphere_patch((0.0, 0.0, 0.0), 1.0, 0), shapes.sphere_patch((2.5, 2.5, 0.0), 0.5, 180)),
 }
 
 
 @pytest.mark.parametrize("prec", ["exact32", "fp64", "fp32screen"])
-@pytest.mark.parametrize("case", ["hardwave", "paraboloid_plane", "saddle_edge"])
+@pytest.mark.parametrize("case", ["hardwave", "paraboloid_plane", "saddle_edge", "spheres"])
 def test_polish_matches_oracle(ctx, orc, case, prec):
     a, b = CASES[case]()
     r1, r2 = regions(a, b, 64)

@@ -1,9 +1,10 @@
 """Pins the oracle's FP64 solves on the final cells with closed forms:
-closest points of quadrics and planes, parallel planes, a minimum on the
+closest points of quadrics and planes, of two spheres, parallel planes, a minimum on the
 domain edge, the parameter-tolerance round count, and intersection curves
 (hyperbola branches, circles) whose points, end points, order and topology
 are known exactly.  The surfaces are dyadic Bezier patches, exact in the
-float32 input format."""
+float32 input format, and rational sphere patches (within float32 rounding of
+the spheres)."""
 import math
 
 import numpy as np
@@ -60,6 +61,34 @@ def test_opposed_paraboloids(orc):
     assert abs(c.distance - 0.125) <= 1e-12 * 0.125
     np.testing.assert_allclose(c.p1, [0, 0, 0], atol=1e-7)
     np.testing.assert_allclose(c.p2, [0, 0, -0.125], atol=1e-7)
+
+
+def test_sphere_patches(orc):
+    """Spheres of radius 1 at the origin and 1/2 at (5/2, 5/2, 0), patches facing
+    each other (rational weights, so the polish runs the full quotient rule):
+    distance |c1 - c2| - r1 - r2 between the points on the centre line.  The
+    float32 weights and coordinates put the surfaces within ~1e-7 of the
+    spheres, hence the tolerances."""
+    s1 = shapes.sphere_patch((0.0, 0.0, 0.0), 1.0, 0)
+    s2 = shapes.sphere_patch((2.5, 2.5, 0.0), 0.5, 180)
+    for s, cen, r in ((s1, (0.0, 0.0, 0.0), 1.0), (s2, (2.5, 2.5, 0.0), 0.5)):  # the patches are spheres
+        nuv = 9
+        g = orc.eval_grid(s, 1.0 / nuv, 1.0 / nuv, 0.0, 0.0, nuv, np.float64).reshape(nuv, nuv, 6, 3)
+        np.testing.assert_allclose(np.linalg.norm(g[..., 0, :] - np.array(cen), axis=-1), r, atol=2e-7)
+    # one screening round: the k-th centre distance is ~1e-3 off, the polish closes the gap
+    r1, r2 = regions(s1, s2, 64)
+    p = default_params(k=64, polish=16, precision="fp64")
+    p.dist_max_rounds = 1
+    res, _ = orc.min_distance(s1, r1, s2, r2, p)
+    c = res.closest
+    exact = 2.5 * math.sqrt(2.0) - 1.5
+    assert res.distance - exact > 1e-4
+    assert abs(c.distance - exact) <= 1e-6
+    d = np.array([1.0, 1.0, 0.0]) / math.sqrt(2.0)
+    np.testing.assert_allclose(c.p1, d, atol=1e-6)
+    np.testing.assert_allclose(c.p2, np.array([2.5, 2.5, 0.0]) - 0.5 * d, atol=1e-6)
+    np.testing.assert_allclose([c.u1, c.v1, c.u2, c.v2], 0.5, atol=1e-6)  # centres of the symmetric arcs
+    assert abs(np.linalg.norm(np.array(c.p1) - np.array(c.p2)) - c.distance) < 1e-14
 
 
 def test_minimum_on_the_domain_edge(orc):
