@@ -113,11 +113,13 @@ def same_curves(g, o):
 
 
 @pytest.mark.parametrize("case,nuv,rounds", [("hyperbola", 64, 4), ("circle", 32, 2), ("tilted_circle", 32, 2),
-                                             ("waveplane", 128, 6), ("waveplane", 512, 6)])
+                                             ("spheres", 64, 3), ("waveplane", 128, 6), ("waveplane", 512, 6)])
 def test_curves_match_oracle(ctx, orc, case, nuv, rounds):
     a, b = {"hyperbola": lambda: (shapes.quadric(1, -1), shapes.plane(0, 0, 0.25)),
             "circle": lambda: (shapes.quadric(1, 1), shapes.plane(0, 0, 0.25)),
             "tilted_circle": lambda: (shapes.quadric(1, 1), shapes.plane(0.5, -0.25, 0.25)),
+            "spheres": lambda: (shapes.sphere_patch((0.0, 0.0, 0.0), 1.0, 0),
+                                shapes.sphere_patch((1.0, 1.0, 0.0), 1.0, 180)),
             "waveplane": S.saddle_wave_pair}[case]()
     r1, r2 = regions(a, b, nuv)
     for prec in ("fp64", "exact32"):
@@ -125,7 +127,7 @@ def test_curves_match_oracle(ctx, orc, case, nuv, rounds):
         g = ctx.intersection_curves(a, r1, b, r2, p)
         o = orc.intersect_curves(a, r1, b, r2, p)
         same_curves(g, o)
-    expect = {"hyperbola": 2, "circle": 1, "tilted_circle": 1, "waveplane": 2}[case]
+    expect = {"hyperbola": 2, "circle": 1, "tilted_circle": 1, "spheres": 2, "waveplane": 2}[case]
     assert g[2] == expect
 
 

@@ -174,6 +174,34 @@ def test_saddle_plane_hyperbola(orc, nuv, rounds):
         assert st.max() < 1.55 * h and st[1:-1].min() > 0.45 * h
 
 
+def test_sphere_sphere_circle_arcs(orc):
+    """Unit spheres at the origin and at (1, 1, 0) meet in the circle of radius
+    sqrt(1/2) about (1/2, 1/2, 0) in the plane x + y = 1.  The facing patches
+    (latitude within +-30 deg) hold the two arcs of it with |z| <= 1/2: two open
+    curves from z = -1/2 to z = 1/2 (or back), each a quarter of the circle,
+    monotone in z, ending at x = 1/2 +- sqrt(2)/4 on the patch edges."""
+    a = shapes.sphere_patch((0.0, 0.0, 0.0), 1.0, 0)
+    b = shapes.sphere_patch((1.0, 1.0, 0.0), 1.0, 180)
+    pts, nc = curves(orc, a, b, 64, 3)
+    assert nc == 2
+    assert not np.any(pts["flags"] & CURVE_CLOSED)
+    P = np.stack([pts["x"], pts["y"], pts["z"]], -1)
+    assert np.abs(np.linalg.norm(P, axis=1) - 1.0).max() < 1e-7  # float32 weights: ~1e-8
+    assert np.abs(np.linalg.norm(P - np.array([1.0, 1.0, 0.0]), axis=1) - 1.0).max() < 1e-7
+    assert np.abs(P[:, 0] + P[:, 1] - 1.0).max() < 1e-7
+    ends = []
+    for c in range(nc):
+        q = pts[pts["curve"] == c]
+        assert abs(abs(q["z"][0]) - 0.5) < 1e-7 and abs(abs(q["z"][-1]) - 0.5) < 1e-7
+        assert q["z"][0] * q["z"][-1] < 0  # from one edge of the band to the other
+        dz = np.diff(q["z"])
+        assert np.all(dz > 0) or np.all(dz < 0)
+        ends.append(round(float(q["x"][0]), 6))
+        arc = steps(q).sum()  # a quarter of the circle of radius sqrt(1/2), polygonal
+        assert abs(arc - math.sqrt(0.5) * math.pi / 2) < 1e-3
+    np.testing.assert_allclose(sorted(ends), [0.5 - math.sqrt(2) / 4, 0.5 + math.sqrt(2) / 4], atol=1e-6)
+
+
 @pytest.mark.parametrize("plane,centre,r2", [((0, 0, 0.25), (0, 0), 0.25),
                                              ((0.5, -0.25, 0.25), (0.25, -0.125), 0.328125)])
 def test_paraboloid_plane_circle(orc, plane, centre, r2):
