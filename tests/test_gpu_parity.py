@@ -224,6 +224,25 @@ def test_intersection_dense_overlap_matches_oracle(ctx, orc, seed, prec):
     np.testing.assert_array_equal(pairs, opairs)
 
 
+@pytest.mark.parametrize("k", [64, 8192])
+def test_min_distance_maximum_grid_fp32screen_equals_fp64(ctx, k):
+    """nuv = 1025 (the largest grid: 1024 x 1024 cells per side, cell indices
+    of 20 bits) and k up to the largest k: beyond what the oracle finishes in
+    seconds, so the check is the property FP32SCREEN is built on, identical
+    pair lists and distance to the FP64 path, plus the order of the list."""
+    a, b = S.hard_wave_pair()
+    r1, r2 = regions(a, b, 1025)
+    g64, p64 = ctx.minimum_distance(a, r1, b, r2, default_params(k=k, precision="fp64", trace=1))
+    g32, p32 = ctx.minimum_distance(a, r1, b, r2, default_params(k=k, precision="fp32screen", trace=1))
+    same_rounds(g64, g32)
+    assert g64.distance == g32.distance
+    np.testing.assert_array_equal(p64, p32)
+    assert len(p64) == k
+    key = list(zip(p64["d2"], p64["slot"], p64["i"], p64["j"]))
+    assert key == sorted(key)  # total order (d2, slot, i, j)
+    assert p64["i"].max() < 1024 * 1024 and p64["j"].max() < 1024 * 1024
+
+
 def test_invalid_arguments_raise(ctx):
     from jsi_cad_next_versions_amd.runtime import JsiError
     a, b = S.wave_pair_small()
@@ -233,3 +252,8 @@ def test_invalid_arguments_raise(ctx):
     r1, r2 = regions(a, b, 8)
     with pytest.raises(JsiError):
         ctx.minimum_distance(a, r1, b, r2, default_params(k=0))
+    with pytest.raises(JsiError):
+        ctx.minimum_distance(a, r1, b, r2, default_params(k=8193))
+    r1, r2 = regions(a, b, 1026)  # one past the largest grid
+    with pytest.raises(JsiError):
+        ctx.minimum_distance(a, r1, b, r2)

@@ -131,6 +131,20 @@ def test_curves_match_oracle(ctx, orc, case, nuv, rounds):
     assert g[2] == expect
 
 
+@pytest.mark.parametrize("prec", ["exact32", "fp64"])
+def test_intersection_of_disjoint_surfaces_is_empty(ctx, orc, prec):
+    """Planes z = 0 and z = 4: no overlap pair in the first round, no curve."""
+    a, b = shapes.plane(0, 0, 0), shapes.plane(0, 0, 4.0)
+    r1, r2 = regions(a, b, 32)
+    p = default_params(precision=prec, trace=1)
+    res, pairs = ctx.intersection(a, r1, b, r2, p, cap=1 << 16)
+    ores, opairs = orc.intersect(a, r1, b, r2, p, cap=1 << 16)
+    assert res.rounds_list() == ores.rounds_list()
+    assert len(pairs) == 0 and len(opairs) == 0
+    g = ctx.intersection_curves(a, r1, b, r2, default_params(precision="fp64"))
+    assert g[2] == 0 and len(g[1]) == 0
+
+
 def test_curves_explicit_spacing_and_capacity(ctx, orc):
     a, b = shapes.quadric(1, 1), shapes.plane(0, 0, 0.25)
     r1, r2 = regions(a, b, 32)
