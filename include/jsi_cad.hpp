@@ -1,21 +1,23 @@
 // jsi_cad.hpp - the JSI_CAD_NEXT VERSION5 query API, served by
-// libjsi_mi355x.so on an MI355X.  Drivers written against
-// JSI_CAD_NEXT_VERSION5/include/jsi_cad.hpp and src/common/common.cuh
-// compile and link against this header unchanged: the names, argument
-// meaning and (C++) signatures are the same.  Differences:
+// libjsi_mi355x.so on an MI355X.  Replaces JSI_CAD_NEXT_VERSION5/include/jsi_cad.hpp:1-48
+// with the same include guard, names, argument meaning and C++ signatures, so
+// the reference's drivers (examples/*_example.cu, which include this header,
+// nurbs_data.hpp and common/common.cuh) compile against this include/
+// directory alone, or beside the reference's own headers (whichever copy is
+// included first wins, the guards being the same).  The memory helpers
+// (cadMalloc, cadMemcpy, CadMemcpyKind ...) live in common/common.cuh, as in
+// the reference.  Differences:
 //   * the workspace is device memory (HBM): d_ivec/d_data of a task and all
 //     pointers from allocate_from_workspace are device pointers, filled with
 //     cadMemcpy(..., CadMemcpyKind::HostToDevice) exactly as the drivers do;
 //   * results are computed by the HIP kernels in EXACT32 mode, bit-exact with
 //     the reference's float path.
-#ifndef JSI_MI355X_JSI_CAD_HPP
-#define JSI_MI355X_JSI_CAD_HPP
+#ifndef _JSI_CAD_HPP
+#define _JSI_CAD_HPP
 
-#include <cstddef>
-#include <cstdint>
+#include <string>
 #include <cstdio>
 #include <cstdlib>
-#include <string>
 
 using cudaError_t = int;
 inline constexpr int cudaSuccess = 0;
@@ -36,7 +38,9 @@ struct EvalAndConstructTask {
 };
 
 void init_cad();
+
 void deinit_cad();
+
 void* allocate_from_workspace(size_t size);
 
 // Returns sqrt of the k-th (k = 8) smallest AABB-centre squared distance of
@@ -60,13 +64,5 @@ inline void checkError(cudaError_t error, std::string msg) {
 // results receives 18*nuv*nuv floats; all pointers may be host or device.
 void evalWithCuda(int* ivec, float* data, int ndata, float ustep, float vstep, float ustart, float vstart, int nuv,
                   float* results);
-
-// Memory helpers of common.cuh:17-27 (0 = ok, -1 = error).
-enum class CadMemcpyKind { HostToDevice, DeviceToHost, DeviceToDevice };
-int cadMalloc(void** ptr, size_t size);
-int cadFree(void* ptr);
-int cadMemcpy(void* dst, const void* src, size_t size, CadMemcpyKind kind);
-int cadMemset(void* ptr, int value, size_t size);
-bool cadIsAclEnabled();
 
 #endif

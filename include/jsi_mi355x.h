@@ -121,6 +121,12 @@ typedef struct jsi_result {
   double ms_screen;      /* screening + top-k selection */
   jsi_round_info round[JSI_MAX_ROUNDS];
   jsi_closest closest;   /* distance with jsi_params.polish > 0 */
+  /* intersection with use_compute_k: compute_K of the two initial regions
+   * (side 1, side 2) and their patch scale max(|du|/U0, |dv|/V0), the values
+   * VERSION5 prints as "Compute_K initial side%d raw=%g scale=%g padding=%g"
+   * (intersection.cu:187-205); 0 otherwise */
+  double isect_k_raw[2];
+  double isect_k_scale[2];
 } jsi_result;
 
 /* One candidate pair: slot and cell ids as in DistPairRef / IntersectPairRef

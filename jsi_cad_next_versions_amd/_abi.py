@@ -107,6 +107,8 @@ class Result(C.Structure):
         ("ms_screen", C.c_double),
         ("round", RoundInfo * MAX_ROUNDS),
         ("closest", Closest),
+        ("isect_k_raw", C.c_double * 2),
+        ("isect_k_scale", C.c_double * 2),
     ]
 
     def rounds_list(self):

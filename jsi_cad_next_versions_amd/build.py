@@ -4,6 +4,7 @@ import subprocess
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libjsi_mi355x.so")
 SOURCES = ["jsi_kernels.hip", "jsi_runtime.hip", "jsi_legacy.hip", "jsi_opabi.hip", "jsi_serve.hip", "jsi_refine.hip", "jsi_sort.hip"]
@@ -37,7 +38,23 @@ def build(verbose=False, jobs=3):
     cmd = ["hipcc", "--offload-arch=gfx950", "-shared", "-fPIC", "-Wl,--no-undefined", "-o", LIB + ".tmp", *objs]
     subprocess.run(cmd, check=True)
     os.replace(LIB + ".tmp", LIB)
+    build_legacy_driver(verbose)
     return LIB
+
+
+def build_legacy_driver(verbose=False):
+    """tests/legacy_driver: a C++ program written like the VERSION5 example
+    drivers, compiled with g++ against include/ alone (no HIP headers) and
+    linked to the library, as a maintainer's relinked driver would be."""
+    src = os.path.join(ROOT, "tests", "legacy_driver.cpp")
+    out = os.path.join(ROOT, "tests", "legacy_driver")
+    cmd = ["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), src, "-o", out + ".tmp",
+           "-L", HERE, "-ljsi_mi355x", "-Wl,-rpath,$ORIGIN/../jsi_cad_next_versions_amd"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(out + ".tmp", out)
+    return out
 
 
 if __name__ == "__main__":

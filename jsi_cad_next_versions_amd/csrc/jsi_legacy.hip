@@ -5,13 +5,21 @@
 #include <hip/hip_runtime.h>
 
 #include <atomic>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
 #include <vector>
 
+#include "../../include/common/common.cuh"
 #include "../../include/jsi_cad.hpp"
 #include "../../include/jsi_mi355x.h"
+
+// common.cu:21-22: the reference's stream bookkeeping; kept for drivers that
+// name it.  Every query runs on the context's own HIP stream.
+int current_stream_id = 0;
+CadStream cuda_streams[num_cuda_streams] = {0, 1, 2, 3};
 
 namespace {
 
@@ -138,6 +146,17 @@ void intersection(EvalAndConstructTask t1, EvalAndConstructTask t2) {
     std::exit(-1);
   }
   g_isect_nuv = res.isect_refine_nuv;
+  if (p.use_compute_k) {
+    // the first two evaluations of a call are the initial regions of side 1
+    // and side 2 (intersection.cu:187-205)
+    for (int side = 1; side <= 2; ++side) {
+      const float raw = (float)res.isect_k_raw[side - 1];
+      const float scale = (float)res.isect_k_scale[side - 1];
+      float pad = raw * scale * scale;
+      if (!std::isfinite(pad) || pad < 0.0f) pad = 0.0f;
+      printf("Compute_K initial side%d raw=%g scale=%g padding=%g\n", side, raw, scale, pad);
+    }
+  }
   printf("Compute_K AABB padding: %s\n", p.use_compute_k ? "on" : "off");
   for (int r = 0; r < res.rounds; ++r) {
     printf("Round: %d (pair refine, aligned_slots=%lld)\n", r + 1, (long long)res.round[r].slots);
@@ -203,3 +222,143 @@ int cadMemset(void* ptr, int value, size_t size) {
 }
 
 bool cadIsAclEnabled() { return false; }
+
+// ---------------------------------------------------------------------------
+// compute_K and PrefixSumAdd (common.cu:137-172) on the GPU.  Their pointers
+// may be host or device memory (the reference passes host vectors); inputs are
+// staged into device buffers and results copied back with hipMemcpyDefault.
+// ---------------------------------------------------------------------------
+namespace {
+
+// max(a, b) as CadMax (common.cu:132-135): a > b ? a : b
+__device__ __forceinline__ float cad_max(float a, float b) { return a > b ? a : b; }
+
+// Per point: the largest |coordinate| of Suu, Suv, Svv (values 9-17 of the 18),
+// written to deriv[3*i + {0,1,2}] and max-reduced into m[0..2] as float bits
+// (non-negative floats order as their bit patterns).
+__global__ __launch_bounds__(256) void k_legacy_second_deriv(const float* __restrict__ ev, long long npts,
+                                                           float* __restrict__ deriv, unsigned* m) {
+  __shared__ unsigned red[3][256];
+  unsigned best[3] = {0u, 0u, 0u};
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < npts;
+       i += (long long)gridDim.x * blockDim.x) {
+    const float* p = ev + i * 18;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const float v = cad_max(fabsf(p[9 + 3 * q]), cad_max(fabsf(p[10 + 3 * q]), fabsf(p[11 + 3 * q])));
+      deriv[i * 3 + q] = v;
+      best[q] = max(best[q], __float_as_uint(v));
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 3; ++q) red[q][threadIdx.x] = best[q];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+#pragma unroll
+      for (int q = 0; q < 3; ++q) red[q][threadIdx.x] = max(red[q][threadIdx.x], red[q][threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < 3; ++q) atomicMax(&m[q], red[q][0]);
+  }
+}
+
+// K = (Muu/(s*s) + 2*Muv/(s*s) + Mvv/(s*s)) / 8 with s = n - 1 (1 when n <= 1),
+// the maxima starting at -FLT_MAX as in the reference (so an empty grid gives
+// its value too).
+__global__ void k_legacy_k(const unsigned* m, long long npts, int n, float* out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  float mx[3];
+  for (int q = 0; q < 3; ++q) mx[q] = npts > 0 ? __uint_as_float(m[q]) : -3.402823466e+38f;
+  const int side = n > 1 ? (n - 1) : 1;
+  const float s2 = (float)(side * side);
+  *out = (mx[0] / s2 + 2.0f * mx[1] / s2 + mx[2] / s2) / 8.0f;
+}
+
+// Inclusive scan of n int32 values by one workgroup: each thread scans a
+// contiguous chunk, the chunk totals are scanned in LDS, then each chunk adds
+// its offset.  Sums wrap modulo 2^32 (the reference's int accumulator).
+__global__ __launch_bounds__(1024) void k_legacy_prefix(const int32_t* __restrict__ in, int32_t* __restrict__ out,
+                                                       long long n) {
+  __shared__ unsigned tot[1024];
+  const long long chunk = (n + blockDim.x - 1) / blockDim.x;
+  const long long b = (long long)threadIdx.x * chunk, e = b + chunk < n ? b + chunk : n;
+  unsigned acc = 0u;
+  for (long long i = b; i < e; ++i) acc += (unsigned)in[i];
+  tot[threadIdx.x] = acc;
+  __syncthreads();
+  for (unsigned w = 1; w < blockDim.x; w <<= 1) {  // Hillis-Steele scan of the chunk totals
+    const unsigned v = threadIdx.x >= w ? tot[threadIdx.x - w] : 0u;
+    __syncthreads();
+    tot[threadIdx.x] += v;
+    __syncthreads();
+  }
+  unsigned run = threadIdx.x > 0 ? tot[threadIdx.x - 1] : 0u;
+  for (long long i = b; i < e; ++i) {
+    run += (unsigned)in[i];
+    out[i] = (int32_t)run;
+  }
+}
+
+template <class X>
+struct DevTmp {  // scoped device allocation of the legacy helpers
+  X* p = nullptr;
+  explicit DevTmp(size_t n) {
+    if (n && hipMalloc(&p, n * sizeof(X)) != hipSuccess) p = nullptr;
+  }
+  ~DevTmp() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+}  // namespace
+
+void compute_K(float* d_evalresults, float* d_deriv, float* d_out, int n) {
+  ensure_ctx();
+  const long long npts = n > 0 ? (long long)n * n : 0;
+  DevTmp<float> ev(npts * 18), der(npts * 3), k(1);
+  DevTmp<unsigned> m(3);
+  if ((npts && (!ev.p || !der.p)) || !k.p || !m.p) {
+    printf("compute_K: device allocation failed\n");
+    std::exit(1);
+  }
+  bool ok = hipMemset(m.p, 0, 3 * sizeof(unsigned)) == hipSuccess;
+  if (npts) {
+    ok = ok && hipMemcpy(ev.p, d_evalresults, npts * 18 * sizeof(float), hipMemcpyDefault) == hipSuccess;
+    const long long blocks = std::min<long long>((npts + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_legacy_second_deriv, dim3((unsigned)blocks), dim3(256), 0, 0, ev.p, npts, der.p, m.p);
+    ok = ok && hipGetLastError() == hipSuccess;
+  }
+  hipLaunchKernelGGL(k_legacy_k, dim3(1), dim3(64), 0, 0, m.p, npts, n, k.p);
+  ok = ok && hipGetLastError() == hipSuccess;
+  if (npts) ok = ok && hipMemcpy(d_deriv, der.p, npts * 3 * sizeof(float), hipMemcpyDefault) == hipSuccess;
+  ok = ok && hipMemcpy(d_out, k.p, sizeof(float), hipMemcpyDefault) == hipSuccess;
+  if (!ok) {
+    printf("compute_K: HIP error\n");
+    std::exit(1);
+  }
+}
+
+int PrefixSumAdd(const int32_t* d_input, int32_t* d_output, size_t size) {
+  if (size == 0) return 0;
+  ensure_ctx();
+  DevTmp<int32_t> in(size), out(size);
+  if (!in.p || !out.p) {
+    printf("PrefixSumAdd: device allocation failed\n");
+    std::exit(1);
+  }
+  bool ok = hipMemcpy(in.p, d_input, size * sizeof(int32_t), hipMemcpyDefault) == hipSuccess;
+  hipLaunchKernelGGL(k_legacy_prefix, dim3(1), dim3(1024), 0, 0, in.p, out.p, (long long)size);
+  ok = ok && hipGetLastError() == hipSuccess;
+  ok = ok && hipMemcpy(d_output, out.p, size * sizeof(int32_t), hipMemcpyDefault) == hipSuccess;
+  int32_t last = 0;
+  ok = ok && hipMemcpy(&last, out.p + (size - 1), sizeof(int32_t), hipMemcpyDeviceToHost) == hipSuccess;
+  if (!ok) {
+    printf("PrefixSumAdd: HIP error\n");
+    std::exit(1);
+  }
+  return last;
+}

@@ -759,9 +759,42 @@ int jsi_ctx::isect_units(int h, const vec4_t<T>* reg, int nunits, int nuv, int u
   T* pts;
   unsigned long long* kmax;
   RCK(buf("ipts", (size_t)3 * nunits * nuv * nuv, &pts));
-  RCK(buf("kmax", (size_t)3 * nunits, &kmax));
+  RCK(buf((std::string("kmax.") + tag).c_str(), (size_t)3 * nunits, &kmax));
   RCK(eval_units<T>(h, reg, nunits, nuv, use_k ? kEvalPointK : kEvalPoint, 0, pts, kmax, tag));
   HCK(launch_aabb<T>(pts, nunits, nuv, 1, use_k, kmax, reg, init_u, init_v, C, R, stream));
+  return JSI_OK;
+}
+
+// compute_K of the two initial regions and their patch scale, from the
+// maxima the round-1 evaluation left in kmax.i1 / kmax.i2 (same arithmetic as
+// k_aabb, so the float values are the ones the padding used).
+template <class T>
+int jsi_ctx::initial_k(int nuv1, int nuv2, const jsi_region& r1, const jsi_region& r2, const T* iu, const T* iv,
+                       jsi_result* res) {
+  for (int side = 1; side <= 2; ++side) {
+    unsigned long long* kmax;
+    RCK(buf(side == 1 ? "kmax.i1" : "kmax.i2", 3, &kmax));
+    unsigned long long m[3];
+    RCK(d2h(m, kmax, sizeof(m)));
+    T v[3];
+    for (int c = 0; c < 3; ++c) {
+      if (std::is_same<T, float>::value) {
+        const uint32_t b = (uint32_t)m[c];
+        std::memcpy(&v[c], &b, sizeof(float));
+      } else {
+        std::memcpy(&v[c], &m[c], sizeof(double));
+      }
+    }
+    const int nuv = side == 1 ? nuv1 : nuv2;
+    const jsi_region& r = side == 1 ? r1 : r2;
+    const int sd = nuv > 1 ? nuv - 1 : 1;
+    const T s2 = T(sd * sd);
+    const T raw = (v[0] / s2 + T(2) * v[1] / s2 + v[2] / s2) / T(8);
+    const T us = std::fabs(T(r.ustop) - T(r.ustart)) / iu[side];
+    const T vs = std::fabs(T(r.vstop) - T(r.vstart)) / iv[side];
+    res->isect_k_raw[side - 1] = (double)raw;
+    res->isect_k_scale[side - 1] = (double)(us < vs ? vs : us);
+  }
   return JSI_OK;
 }
 
@@ -856,6 +889,7 @@ int jsi_ctx::intersect(int h1, const jsi_region& r1, int h2, const jsi_region& r
                              grid, stream));
     HCK(scan_u32_to_u64(rowcount, rowoff, nrows, scantmp, stb, stream));
     RCK(d2h(&npairs, rowoff + nrows, sizeof(npairs)));
+    if (round == 1 && use_k) RCK(initial_k<T>(nuv1, nuv2, r1, r2, iu, iv, res));
     RCK(buf(nm("pairs", 1, cur).c_str(), std::max(npairs, 1ull), &pairs));
     if (npairs > 0) {
       HCK(launch_fill_u32(rowcursor, 0u, nrows, stream));

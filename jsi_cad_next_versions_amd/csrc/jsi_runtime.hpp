@@ -130,6 +130,9 @@ struct jsi_ctx {
              long long kk, const int* su1, const int* su2, const jsi::vec4_t<T>* reg1, const jsi::vec4_t<T>* reg2,
              int nuv1, int nuv2, jsi_result* res);
   template <class T>
+  int initial_k(int nuv1, int nuv2, const jsi_region& r1, const jsi_region& r2, const T* iu, const T* iv,
+                jsi_result* res);
+  template <class T>
   int intersect(int h1, const jsi_region& r1, int h2, const jsi_region& r2, const jsi_params& P, jsi_result* res,
                 jsi_pair* out, int cap_pairs, CurveReq* creq = nullptr);
   int trace_curves(const jsi::CurveRec* recs, long long n, const jsi::SurfDev<double>& s1,

@@ -328,19 +328,6 @@ class RefHotspots:
                                                     C.POINTER(C.c_float)]
         L.jsi_ascend_intersection_mask.argtypes = [_f32p, _f32p, C.c_int, _f32p, _f32p, C.c_int, _i32p, _i32p]
 
-    def intersect_curves(self, s1, r1, s2, r2, params=None, spacing=0.0, tol=0.0, cap=1 << 20):
-        """(Result, ordered curve samples as a numpy record array, number of curves)."""
-        params = params or default_params()
-        res = Result()
-        buf = (CurvePoint * max(cap, 1))()
-        n, nc = C.c_int(), C.c_int()
-        rc = self.lib.orc_intersect_curves(C.byref(params), s1.ivec, s1.data, C.byref(r1), s2.ivec, s2.data,
-                                           C.byref(r2), spacing, tol, C.byref(res), C.cast(buf, C.c_void_p), cap,
-                                           C.byref(n), C.byref(nc))
-        if rc != 0:
-            raise RuntimeError(f"oracle query failed: {rc}")
-        return res, curve_points_to_numpy(buf, min(cap, n.value)), nc.value
-
     def distance_topk_mask(self, c1, c2, topk):
         c1 = np.ascontiguousarray(c1, np.float32).reshape(-1)
         c2 = np.ascontiguousarray(c2, np.float32).reshape(-1)

@@ -6,10 +6,13 @@ exist on the GPU box).  Inputs are the reference's sample surfaces
 synthetic surfaces of jsi_cad_next_versions_amd.surfaces.
 
 Outputs (data only): eval_f32.npz (grid evaluations), golden.json (distances,
-compute_K values, intersection round counts, all floats as exact bit patterns).
+compute_K values, intersection round counts, all floats as exact bit patterns;
+the printed output of call sequences run in a fresh process, where the
+reference's intersection refinement nuv starts at 16 and halves across calls).
 """
 import json
 import os
+import subprocess
 import sys
 
 import numpy as np
@@ -63,10 +66,14 @@ def main():
         key = f"centers_{name}_n{nuv}"
         arrays[key] = c
         gold["dist_centers"].append({"surface": name, "nuv": nuv, "key": key})
-    # end-to-end minimum_distance (k = 8, 4 rounds, refinement nuv 16)
+    # end-to-end minimum_distance (k = 8, 4 rounds, refinement nuv 16); the
+    # degree-11 CAD faces (inter_case1/2) at the intersection driver's nuv 16
+    # and above
     for a, b, nuv in [("wave16_A", "wave16_B", 64), ("hardwave_A", "hardwave_B", 64),
                       ("hardwave_A", "hardwave_B", 128), ("saddle", "waveplane", 48),
-                      ("inter_case3_obj1", "inter_case3_obj2", 40), ("inter_case4_obj1", "inter_case4_obj2", 33)]:
+                      ("inter_case3_obj1", "inter_case3_obj2", 40), ("inter_case4_obj1", "inter_case4_obj2", 33),
+                      ("inter_case1_obj1", "inter_case1_obj2", 16), ("inter_case1_obj1", "inter_case1_obj2", 48),
+                      ("inter_case2_obj1", "inter_case2_obj2", 16), ("inter_case2_obj1", "inter_case2_obj2", 40)]:
         s1, s2 = surf[a], surf[b]
         d, _ = ref.min_distance(s1, S.full_region(s1, nuv, 1), s2, S.full_region(s2, nuv, 2))
         gold["min_distance"].append({"a": a, "b": b, "nuv": nuv, "distance": f32bits(d)})
@@ -87,16 +94,82 @@ def main():
                 break
             if c > 512 and g > 4:
                 g //= 2
-    np.savez_compressed(os.path.join(HERE, "eval_f32.npz"), **arrays)
+    # call sequences in a fresh process each: the printed output of every
+    # call (the reference's only intersection output) and the distances
+    gold["sequences"] = [run_fresh(seq) for seq in SEQUENCES]
+    save_npz(os.path.join(HERE, "eval_f32.npz"), arrays)
     # the sample surfaces (data only) so the fixtures are self-contained on machines without the reference
     small = {k: v for k, v in surf.items() if k.startswith(("inter_case3", "inter_case4"))}
-    np.savez_compressed(os.path.join(HERE, "sample_surfaces.npz"),
-                        **{f"{k}_ivec": v.ivec for k, v in small.items()},
-                        **{f"{k}_data": v.data for k, v in small.items()})
+    save_npz(os.path.join(HERE, "sample_surfaces.npz"), {**{f"{k}_ivec": v.ivec for k, v in small.items()},
+                                                          **{f"{k}_data": v.data for k, v in small.items()}})
     with open(os.path.join(HERE, "golden.json"), "w") as f:
         json.dump(gold, f, indent=1)
     print("wrote", len(arrays), "arrays;", {k: len(v) for k, v in gold.items()})
 
 
+def save_npz(path, arrays):
+    """Rewrites an archive only when its arrays change (zip entries carry timestamps)."""
+    if os.path.exists(path):
+        old = dict(np.load(path, allow_pickle=False))
+        if old.keys() == arrays.keys() and all(np.array_equal(old[k], arrays[k]) and old[k].dtype == arrays[k].dtype
+                                               for k in old):
+            return
+    np.savez_compressed(path, **arrays)
+
+
+# Call sequences of the VERSION5 API, each run in a fresh process, as the
+# reference's drivers run (examples/intersection_example.cu:89 makes one
+# intersection() call per process, refinement nuv starting at 16).  op:
+# "isect" (intersection with CAD_INTERSECT_USE_K on / off) or "dist"
+# (minimum_distance); full knot domains, side 1 / 2.
+SEQUENCES = [
+    {"name": "driver_case1", "calls": [{"op": "isect", "a": "inter_case1_obj1", "b": "inter_case1_obj2", "nuv": 16,
+                                        "use_k": True}]},
+    {"name": "driver_case2", "calls": [{"op": "isect", "a": "inter_case2_obj1", "b": "inter_case2_obj2", "nuv": 16,
+                                        "use_k": True}]},
+    {"name": "case1_no_k", "calls": [{"op": "isect", "a": "inter_case1_obj1", "b": "inter_case1_obj2", "nuv": 16,
+                                      "use_k": False}]},
+    # case 1 halves the refinement nuv to 4; case 2 then starts there
+    {"name": "case1_then_case2", "calls": [
+        {"op": "isect", "a": "inter_case1_obj1", "b": "inter_case1_obj2", "nuv": 16, "use_k": True},
+        {"op": "isect", "a": "inter_case2_obj1", "b": "inter_case2_obj2", "nuv": 16, "use_k": True},
+        {"op": "dist", "a": "inter_case2_obj1", "b": "inter_case2_obj2", "nuv": 16}]},
+    # surfaces that travel with the fixtures (sample_surfaces.npz, generators)
+    {"name": "portable", "calls": [
+        {"op": "isect", "a": "inter_case3_obj1", "b": "inter_case3_obj2", "nuv": 16, "use_k": True},
+        {"op": "dist", "a": "hardwave_A", "b": "hardwave_B", "nuv": 64},
+        {"op": "isect", "a": "inter_case4_obj1", "b": "inter_case4_obj2", "nuv": 16, "use_k": True},
+        {"op": "isect", "a": "saddle", "b": "waveplane", "nuv": 32, "use_k": False},
+        {"op": "dist", "a": "inter_case3_obj1", "b": "inter_case3_obj2", "nuv": 40}]},
+]
+
+
+def run_fresh(seq):
+    out = subprocess.run([sys.executable, os.path.abspath(__file__), "--sequence", json.dumps(seq)],
+                         capture_output=True, text=True, check=True).stdout
+    return json.loads(out.strip().splitlines()[-1])
+
+
+def sequence_main(seq):
+    """Runs one call sequence in this (fresh) process; prints one JSON line."""
+    ref = RefV5()
+    surf = surfaces(ref)
+    logs, results = [], []
+    for c in seq["calls"]:
+        s1, s2 = surf[c["a"]], surf[c["b"]]
+        r1, r2 = S.full_region(s1, c["nuv"], 1), S.full_region(s2, c["nuv"], 2)
+        if c["op"] == "isect":
+            counts, log = ref.intersect(s1, r1, s2, r2, use_k=c["use_k"])
+            results.append(counts)
+        else:
+            d, log = ref.min_distance(s1, r1, s2, r2)
+            results.append(f32bits(d))
+        logs.append(log)
+    print(json.dumps({**seq, "logs": logs, "results": results}))
+
+
 if __name__ == "__main__":
-    main()
+    if len(sys.argv) == 3 and sys.argv[1] == "--sequence":
+        sequence_main(json.loads(sys.argv[2]))
+    else:
+        main()

@@ -17,7 +17,8 @@ HEADER = os.path.join(ROOT, "include", "jsi_mi355x.h")
 LEGACY = ["_Z8init_cadv", "_Z10deinit_cadv", "_Z23allocate_from_workspacem",
           "_Z16minimum_distanceRK20EvalAndConstructTaskS1_", "_Z12intersection20EvalAndConstructTaskS_",
           "_Z12evalWithCudaPiPfiffffiS0_", "_Z9cadMallocPPvm", "_Z7cadFreePv",
-          "_Z9cadMemcpyPvPKvm13CadMemcpyKind", "_Z9cadMemsetPvim", "_Z15cadIsAclEnabledv"]
+          "_Z9cadMemcpyPvPKvm13CadMemcpyKind", "_Z9cadMemsetPvim", "_Z15cadIsAclEnabledv",
+          "_Z12PrefixSumAddPKiPim", "_Z9compute_KPfS_S_i"]
 
 
 def header_functions():
