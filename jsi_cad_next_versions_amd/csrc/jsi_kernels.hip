@@ -1267,7 +1267,14 @@ typedef double mfma_v4d __attribute__((ext_vector_type(4)));
 // v_min_f64 on finite values (products of full blocks): written as the
 // instruction because fmin() first canonicalises both operands (two extra
 // instructions per minimum).  The compiler does not see inline asm as a reader
-// of a matrix-core result, so callers insert the wait states themselves.
+// of a matrix-core result, so callers insert the wait states themselves
+// (k_dist_filter_mfma: s_nop 15; s_nop 2 between scheduling barriers).  The
+// instruction order this relies on was read back from the code object of the
+// toolchain below; another compiler must be checked the same way before the
+// guard is widened.
+#if !(HIP_VERSION_MAJOR == 7 && HIP_VERSION_MINOR == 2 && __clang_major__ == 22)
+#error "k_dist_filter_mfma's hand-inserted wait states are verified for HIP 7.2 / clang 22 only"
+#endif
 __device__ __forceinline__ double min_nan(double a, double b) {
   double r;
   asm("v_min_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
@@ -3099,8 +3106,8 @@ static hipError_t scan_rec(const IN* in, unsigned long long* out, unsigned long 
   unsigned long long* sums = tmp;
   unsigned long long* offs = tmp + nb + 1;
   hipLaunchKernelGGL(k_scan_reduce<IN>, dim3((unsigned)nb), dim3(1024), 0, s, in, n, sums);
-  if (nb == 1) {
-    hipError_t e = hipMemsetAsync(offs, 0, sizeof(unsigned long long), s);
+  if (nb == 1) {  // a fill kernel, not a memset node: the scan may run under graph capture
+    hipError_t e = launch_fill_u32(reinterpret_cast<unsigned*>(offs), 0u, 2, s);
     if (e != hipSuccess) return e;
   } else {
     hipError_t e = scan_rec<unsigned long long>(sums, offs, nb, tmp + 2 * (nb + 1), s);
@@ -3112,7 +3119,7 @@ static hipError_t scan_rec(const IN* in, unsigned long long* out, unsigned long 
 
 hipError_t scan_u32_to_u64(const unsigned* in, unsigned long long* out, unsigned long long n, void* tmp,
                            size_t tmp_bytes, hipStream_t s) {
-  if (n == 0) return hipMemsetAsync(out, 0, sizeof(unsigned long long), s);
+  if (n == 0) return launch_fill_u32(reinterpret_cast<unsigned*>(out), 0u, 2, s);
   if (tmp_bytes < scan_tmp_bytes(n)) return hipErrorInvalidValue;
   return scan_rec<unsigned>(in, out, n, static_cast<unsigned long long*>(tmp), s);
 }

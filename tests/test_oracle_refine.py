@@ -108,12 +108,12 @@ def test_polish_is_independent_of_precision_mode(orc):
     c64 = closest(orc, a, b, nuv=64, k=64, polish=64)
     c32 = closest(orc, a, b, nuv=64, k=64, polish=64, prec="exact32")
     assert abs(c64.distance - c32.distance) <= 1e-12 * c64.distance
-    # the 4 refinement rounds shrink the cells to ~4e-6 of the domain (~1e-5 in
-    # space), so the 64 closest centre pairs lie within a few cells of the
-    # minimum and their distances exceed it by O(cell^2 / gap) < 1e-6
+    # the k-th (64th) centre distance of the last round is an upper estimate
+    # of the same local minimum: the 4 rounds shrink the cells to ~4e-6 of the
+    # domain, and the 64 closest centre pairs stay within 1% of the gap of it
     r1, r2 = regions(a, b, 64)
     res, _ = orc.min_distance(a, r1, b, r2, default_params(k=64, precision="fp64"))
-    assert 0.0 <= res.distance - c64.distance <= 1e-6
+    assert 0.0 <= res.distance - c64.distance <= 1e-2 * c64.distance
     assert c64.flags & 8  # converged
 
 

@@ -132,3 +132,109 @@ def test_matrix_core_form_keeps_every_pair_at_threshold():
 def test_matrix_core_planted_margin_too_small_is_caught():
     assert margin_violations(0.0, 0.0, n=1500, seed=11, test=mfma_passes) > 0
     assert margin_violations(32.0, 0.0, n=1500, seed=11, test=mfma_passes) > 0
+
+
+# ---------------------------------------------------------------------------
+# FP32SCREEN: the float screen of round 1 reads centres rounded to float and
+# a float threshold t widened by k_inflate (jsi_kernels.hip):
+#   inflate(t) = ru32((sqrt(t) g + 4 sqrt(3) E g)^2 g + 1e-36), g = 1 + 16u,
+#   u = 2^-24, E = 2^-24 max |centre coordinate|.
+# t is at least the float d2 of k real pairs (seed pairs, or the k-th record
+# of a capped screen), so the FP64 k-th d2 D is at most the largest FP64 d2 of
+# those pairs; every pair of the FP64 top-k has FP64 d2 <= D and must pass the
+# float test fl32 d2 <= inflate(t).  The float d2 may be evaluated with fused
+# multiply-adds in any order (the screen is compiled with contraction on).
+# ---------------------------------------------------------------------------
+U24 = 2.0 ** -24
+
+
+def f32(x):
+    """A double rounded to the nearest float32 (the centre copies of the screen)."""
+    return float(np.float32(x))
+
+
+def r32(q):
+    """Correct rounding of an exact rational to float32 (via the exact double
+    nearest, then a check of the two float neighbours)."""
+    d = float(q)
+    c = np.float32(d)
+    best = None
+    for cand in (np.nextafter(c, np.float32(-np.inf)), c, np.nextafter(c, np.float32(np.inf))):
+        err = abs(Fraction(float(cand)) - q)
+        if best is None or err < best[0] or (err == best[0] and int(np.float32(cand).view(np.uint32)) % 2 == 0):
+            best = (err, float(cand))
+    return best[1]
+
+
+def float_d2_orders(a, b):
+    """Every value of (dx^2 + dy^2) + dz^2 over float evaluation orders with
+    products rounded or fused into the next add; a, b are float32 centres."""
+    d = [r32(Fraction(a[c]) - Fraction(b[c])) for c in range(3)]
+    sq = [Fraction(x) * Fraction(x) for x in d]
+    out = []
+    for perm in itertools.permutations(range(3)):
+        p0, p1, p2 = perm
+        # products rounded, sum left to right
+        out.append(r32(Fraction(r32(Fraction(r32(sq[p0])) + Fraction(r32(sq[p1])))) + Fraction(r32(sq[p2]))))
+        # fused chain fma(d2, d2, fma(d1, d1, d0*d0))
+        out.append(r32(sq[p2] + Fraction(r32(sq[p1] + Fraction(r32(sq[p0]))))))
+        # one fused add
+        out.append(r32(sq[p2] + Fraction(r32(Fraction(r32(sq[p0])) + Fraction(r32(sq[p1]))))))
+        out.append(r32(Fraction(r32(sq[p2])) + Fraction(r32(sq[p1] + Fraction(r32(sq[p0]))))))
+    return out
+
+
+def inflate(t, E, e_coef=4.0, g_coef=16.0):
+    """k_inflate's arithmetic in double, rounded up to float."""
+    g = 1.0 + g_coef * U24
+    r = np.sqrt(np.float64(t)) * g + e_coef * 1.7320508075688772 * E * g
+    v = r * r * g + 1e-36
+    f = np.float32(v)
+    if float(f) < v:
+        f = np.nextafter(f, np.float32(np.inf))
+    return float(f)
+
+
+def fp32screen_samples(seed, n):
+    """(A, B, C, D): C, D give the threshold pair, A, B a pair whose FP64 d2
+    is just below C, D's (the hardest pair the FP64 top-k can hold)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        scale = 10.0 ** rng.uniform(-2, 3)
+        rel = 10.0 ** rng.uniform(-7, -1)
+        c = rng.uniform(-scale, scale, 3)
+        dvec = rng.normal(0, scale * rel, 3)
+        dd = c + dvec
+        a = c + rng.uniform(-scale * rel, scale * rel, 3) * 5
+        dir_ = rng.normal(0, 1, 3)
+        dir_ /= np.linalg.norm(dir_)
+        b = a + dir_ * np.linalg.norm(dvec) * (1 - 10.0 ** rng.uniform(-12, -6))
+        if rng.random() < 0.25:  # coordinates on a coarse grid: ties in the float roundings
+            q = 2.0 ** int(rng.integers(8, 22))
+            a, b, c, dd = (np.round(x * q) / q for x in (a, b, c, dd))
+        out.append(tuple(x.astype(np.float64) for x in (a, b, c, dd)))
+    return out
+
+
+def fp32screen_violations(n=1200, seed=3, **kw):
+    bad = 0
+    for A, B, Cc, D in fp32screen_samples(seed, n):
+        if ref_d2(A, B) > ref_d2(Cc, D):
+            continue
+        E = float(max(np.max(np.abs(x)) for x in (A, B, Cc, D))) * U24
+        a, b, c, d = ([f32(v) for v in x] for x in (A, B, Cc, D))
+        t = min(float_d2_orders(c, d))  # the threshold pair at its smallest float d2
+        if max(float_d2_orders(a, b)) > inflate(t, E, **kw):
+            bad += 1
+    return bad
+
+
+def test_fp32screen_inflate_keeps_every_fp64_topk_pair():
+    assert fp32screen_violations() == 0
+
+
+def test_fp32screen_planted_weaker_inflate_is_caught():
+    # a widening that covers the centre rounding only once, and none at all
+    assert fp32screen_violations(n=400, e_coef=1.0) > 0
+    assert fp32screen_violations(n=400, e_coef=0.0, g_coef=0.0) > 0
