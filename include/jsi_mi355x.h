@@ -182,6 +182,18 @@ int jsi_eval_grid_f64(jsi_ctx* ctx, int handle, double ustep, double vstep, doub
 int jsi_min_distance(jsi_ctx* ctx, int h1, const jsi_region* r1, int h2, const jsi_region* r2,
                      const jsi_params* params, jsi_result* result, jsi_pair* pairs_out, int pairs_cap);
 
+/* jsi_min_distance for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * No counterpart in the reference.  Same arguments, results and errors as
+ * jsi_min_distance; no device memory is allocated: every buffer of the query
+ * comes from a 256 MiB scratch array that the library holds on each GPU it
+ * runs on until the process ends (queries up to nuv 512 with k 4096 fit; a
+ * larger one fails with JSI_ERR_NOMEM and a message naming the limit, before
+ * or while it runs).  Calls on one GPU share that array: calls that overlap
+ * on one GPU race. */
+int jsi_min_distance_noalloc(jsi_ctx* ctx, int h1, const jsi_region* r1, int h2, const jsi_region* r2,
+                             const jsi_params* params, jsi_result* result, jsi_pair* pairs_out, int pairs_cap);
+
 /* Surface-surface intersection screening, the run_intersect path:
  * intersection (JSI_CAD_NEXT_VERSION5/src/intersection/intersection.cu:338-475).
  * pairs_out (optional, host) receives the last round's ordered pairs. */

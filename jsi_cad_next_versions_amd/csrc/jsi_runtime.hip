@@ -59,7 +59,6 @@ unsigned long long next_pow2(unsigned long long x) {
   return p;
 }
 
-constexpr unsigned long long kCandCap = 1ull << 22;  // candidate records kept per screening pass
 constexpr unsigned kSmallCap = 16384;               // records sorted after selection
 
 
@@ -69,6 +68,30 @@ constexpr unsigned kSmallCap = 16384;               // records sorted after sele
 // Context
 // ---------------------------------------------------------------------------
 int jsi_ctx::buf_raw(const char* name, size_t bytes, void** out, int fill_int) {
+  if (scratch_.base) {
+    DevBuf& b = scratch_.bufs[name];
+    if (b.bytes < bytes) {
+      const size_t off = (scratch_.off + 255) & ~(size_t)255;
+      const size_t want = std::max(bytes, (size_t)256);
+      if (off + want > scratch_.cap) {
+        set_err("scratch of %zu bytes exceeded: buffer %s needs %zu bytes at offset %zu (this query is larger "
+                "than the allocation-free entry point supports)", scratch_.cap, name, want, off);
+        return JSI_ERR_NOMEM;
+      }
+      b.p = scratch_.base + off;
+      b.bytes = want;
+      scratch_.off = off + want;
+      if (fill_int != 0) {
+        hipError_t f = launch_fill_i32(static_cast<int*>(b.p), fill_int, want / sizeof(int), stream);
+        if (f != hipSuccess) {
+          set_err("fill %s: %s", name, hipGetErrorString(f));
+          return JSI_ERR_DEVICE;
+        }
+      }
+    }
+    *out = b.p;
+    return JSI_OK;
+  }
   DevBuf& b = bufs[name];
   if (b.bytes < bytes) {
     if (b.p) {
@@ -144,6 +167,18 @@ int jsi_ctx::surf_dev(int h, SurfDev<T>* out) {
   const T* base;
   if (std::is_same<T, float>::value) {
     base = reinterpret_cast<const T*>(s.d_data);
+  } else if (!s.d_data64 && scratch_.base) {
+    // no resident FP64 copy and no allocation allowed: promote into the scratch
+    // (on this stream, ahead of the kernels that read it)
+    double* p64 = nullptr;
+    const std::string nm = "s64." + std::to_string(h);
+    {
+      void* v = nullptr;
+      RCK(buf_raw(nm.c_str(), sizeof(double) * (size_t)s.ndata, &v, 0));
+      p64 = static_cast<double*>(v);
+      HCK(launch_promote<double>(s.d_data, p64, s.ndata, stream));
+    }
+    base = reinterpret_cast<const T*>(p64);
   } else {
     if (!s.d_data64) {
       // promoted once, synchronously, so every stream sees the finished copy
@@ -266,7 +301,7 @@ int jsi_ctx::dist_topk(int nslots, const int* su1, const int* su2, int w1, int n
   out_count = reinterpret_cast<unsigned long long*>(list_count + 4);
   thresh = reinterpret_cast<T*>(list_count + 6);
   small_count = reinterpret_cast<unsigned long long*>(list_count + 10);
-  RCK(buf("d.cand", kCandCap, &cand));
+  RCK(buf("d.cand", cand_cap_max_, &cand));
   RCK(buf("d.topk", kSmallCap, &small));
   RCK(buf("d.selstate", 1, &st));
   RCK(buf("d.hist", 4096, &hist));
@@ -309,8 +344,8 @@ int jsi_ctx::dist_topk(int nslots, const int* su1, const int* su2, int w1, int n
   // one copy brings back both the candidate count and the selected count
   const unsigned long long cap =
       cand_cap ? std::min<unsigned long long>(std::max<unsigned long long>(cand_cap, 2ull * (unsigned long long)k),
-                                              kCandCap)
-               : kCandCap;
+                                              cand_cap_max_)
+               : cand_cap_max_;
   const unsigned small_cap = (unsigned)std::max(2048, std::min((int)kSmallCap, 2 * k));
   const unsigned long long maxkey = rerank ? 0x7ff0000000000000ull : maxkey_t;
   unsigned long long n = 0, m = 0;
@@ -398,7 +433,7 @@ int jsi_ctx::dist_topk(int nslots, const int* su1, const int* su2, int w1, int n
   if (m > small_cap) {
     // more than kSmallCap candidates tie with the k-th key: sort them all
     const unsigned long long n2 = std::max(next_pow2(std::max(n, 1ull)), (unsigned long long)kSortBlock);
-    if (n2 > kCandCap) {
+    if (n2 > cand_cap_max_) {
       set_err("distance screen: %llu candidates tie at the k-th key", n);
       return JSI_ERR_CAPACITY;
     }
@@ -1383,8 +1418,8 @@ int jsi_eval_grid_f64(jsi_ctx* c, int h, double ustep, double vstep, double usta
   return c->eval_grid<double>(h, ustep, vstep, ustart, vstart, nuv, out);
 }
 
-int jsi_min_distance(jsi_ctx* c, int h1, const jsi_region* r1, int h2, const jsi_region* r2, const jsi_params* P,
-                     jsi_result* res, jsi_pair* pairs_out, int pairs_cap) {
+static int validate_distance(jsi_ctx* c, int h1, const jsi_region* r1, int h2, const jsi_region* r2,
+                             const jsi_params* P, jsi_result* res) {
   RCK(validate_query(c, h1, r1, h2, r2, P));
   if (!res || P->k < 1 || P->k > JSI_MAX_K || P->dist_max_rounds < 1 || P->dist_refine_nuv < 2 ||
       P->dist_refine_nuv > JSI_MAX_NUV || P->polish < 0 || P->polish > JSI_MAX_K || !(P->param_tol >= 0.0) ||
@@ -1396,6 +1431,37 @@ int jsi_min_distance(jsi_ctx* c, int h1, const jsi_region* r1, int h2, const jsi
     set_err("param_tol needs FP64 parameters (precision FP64 or FP32SCREEN)");
     return JSI_ERR_INVALID;
   }
+  return JSI_OK;
+}
+
+extern "C++" int min_distance_in_scratch(jsi_ctx* c, int h1, const jsi_region* r1, int h2, const jsi_region* r2,
+                            const jsi_params* P, jsi_result* res, jsi_pair* pairs_out, int pairs_cap, void* scratch,
+                            size_t bytes, unsigned long long cand_cap) {
+  RCK(validate_distance(c, h1, r1, h2, r2, P, res));
+  std::memset(res, 0, sizeof(*res));
+  HCK(hipSetDevice(c->device));
+  c->scratch_.base = static_cast<char*>(scratch);
+  c->scratch_.cap = bytes;
+  c->scratch_.off = 0;
+  c->scratch_.bufs.clear();
+  const unsigned long long saved_cap = c->cand_cap_max_;
+  c->cand_cap_max_ = cand_cap;
+  int rc = P->precision == JSI_PREC_EXACT32 ? c->min_distance<float>(h1, *r1, h2, *r2, *P, res, pairs_out, pairs_cap)
+                                            : c->min_distance<double>(h1, *r1, h2, *r2, *P, res, pairs_out, pairs_cap);
+  // the stream has drained on success (results were read back); after a
+  // failure, wait for what was launched before handing the scratch back
+  if (rc != JSI_OK) (void)hipStreamSynchronize(c->stream);
+  if (c->debug_)
+    fprintf(stderr, "[jsi] scratch: %zu of %zu bytes used\n", c->scratch_.off, c->scratch_.cap);
+  c->scratch_.base = nullptr;
+  c->scratch_.bufs.clear();
+  c->cand_cap_max_ = saved_cap;
+  return rc;
+}
+
+int jsi_min_distance(jsi_ctx* c, int h1, const jsi_region* r1, int h2, const jsi_region* r2, const jsi_params* P,
+                     jsi_result* res, jsi_pair* pairs_out, int pairs_cap) {
+  RCK(validate_distance(c, h1, r1, h2, r2, P, res));
   std::memset(res, 0, sizeof(*res));
   HCK(hipSetDevice(c->device));
   if (P->precision == JSI_PREC_EXACT32) return c->min_distance<float>(h1, *r1, h2, *r2, *P, res, pairs_out, pairs_cap);

@@ -80,6 +80,16 @@ struct jsi_ctx {
   std::shared_ptr<SurfaceTable> surfaces = std::make_shared<SurfaceTable>();
   bool worker = false;  // shares its parent's surfaces; destroy frees only its own buffers
   std::map<std::string, DevBuf> bufs;
+  // Caller-provided scratch (the allocation-free distance entry point): while
+  // `base` is set, buf_raw carves every named buffer of the query from
+  // [base, base + cap) in request order (a buffer requested again reuses its
+  // piece when it is large enough), and no device memory is allocated.
+  struct Scratch {
+    char* base = nullptr;
+    size_t cap = 0, off = 0;
+    std::map<std::string, DevBuf> bufs;
+  } scratch_;
+  unsigned long long cand_cap_max_ = 1ull << 22;  // candidate records per screening pass (power of two)
   void* thresh_ptr_ = nullptr;  // device threshold of the screen in flight
   bool debug_ = false;          // JSI_DEBUG: per-screen counts on stderr
 
@@ -143,3 +153,13 @@ struct jsi_ctx {
 
 // Worker context sharing the parent's surfaces (batch runner / server).
 extern "C" int jsi_ctx_create_worker(jsi_ctx* parent, jsi_ctx** out);
+
+// jsi_min_distance with every device buffer carved from `scratch` (device
+// memory of `bytes` bytes that the caller owns); allocates no device memory.
+// Candidate records per screening pass are capped at `cand_cap` (a power of
+// two; more candidates take the overflow path, results unchanged).
+__attribute__((visibility("hidden"))) int min_distance_in_scratch(jsi_ctx* c, int h1, const jsi_region* r1, int h2,
+                                                                  const jsi_region* r2, const jsi_params* P,
+                                                                  jsi_result* res, jsi_pair* pairs_out,
+                                                                  int pairs_cap, void* scratch, size_t bytes,
+                                                                  unsigned long long cand_cap);

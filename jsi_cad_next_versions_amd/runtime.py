@@ -29,8 +29,8 @@ _i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
 EXPORTS = [
     "jsi_params_default", "jsi_status_string", "jsi_last_error", "jsi_ctx_create", "jsi_ctx_destroy",
     "jsi_ctx_sync", "jsi_ctx_profile", "jsi_ctx_kernel_stats", "jsi_surface_upload", "jsi_surface_register_device", "jsi_surface_release",
-    "jsi_eval_grid_f32", "jsi_eval_grid_f64", "jsi_min_distance", "jsi_intersect", "jsi_intersect_curves",
-    "jsi_run_batch",
+    "jsi_eval_grid_f32", "jsi_eval_grid_f64", "jsi_min_distance", "jsi_min_distance_noalloc", "jsi_intersect",
+    "jsi_intersect_curves", "jsi_run_batch",
     "jsi_server_create", "jsi_server_destroy", "jsi_server_surface", "jsi_server_submit", "jsi_server_poll",
     "jsi_server_counters",
 ]
@@ -75,6 +75,7 @@ def load_library(path: str = LIB_PATH):
     q = [C.c_void_p, C.c_int, C.POINTER(Region), C.c_int, C.POINTER(Region), C.POINTER(Params), C.POINTER(Result),
          C.c_void_p, C.c_int]
     L.jsi_min_distance.argtypes = q
+    L.jsi_min_distance_noalloc.argtypes = q
     L.jsi_intersect.argtypes = q
     L.jsi_intersect_curves.argtypes = q[:6] + [C.c_double, C.c_double, C.POINTER(Result), C.c_void_p, C.c_int,
                                                C.POINTER(C.c_int), C.POINTER(C.c_int)]
@@ -176,6 +177,12 @@ class Context:
         params = params or default_params()
         cap = params.k if cap is None else cap
         return self._query(self.L.jsi_min_distance, s1, r1, s2, r2, params, cap)
+
+    def minimum_distance_noalloc(self, s1, r1, s2, r2, params=None, cap=None):
+        """minimum_distance through the allocation-free entry point."""
+        params = params or default_params()
+        cap = params.k if cap is None else cap
+        return self._query(self.L.jsi_min_distance_noalloc, s1, r1, s2, r2, params, cap)
 
     def intersection(self, s1, r1, s2, r2, params=None, cap=0):
         """Returns (Result, last-round overlap pairs (up to cap))."""
