@@ -154,7 +154,11 @@ int jsi_ctx_destroy(jsi_ctx* ctx);
 int jsi_ctx_sync(jsi_ctx* ctx);
 /* Per-kernel timing (HIP events on the context stream) of the screen
  * kernels: enable = 1 clears the counters and starts recording.  Names:
- * "dist_filter_f32", "dist_filter_f64".  units = cell pairs evaluated. */
+ * "dist_filter_f32", "dist_filter_f64".  units = cell pairs evaluated.
+ * Always kept, for the context and every batch runner / server made from it:
+ * "batches" (launches = batches run, units = queries in them) and
+ * "batch_fallbacks" (units = batched queries rerun alone after a bounded
+ * buffer of the batch overflowed). */
 int jsi_ctx_profile(jsi_ctx* ctx, int enable);
 int jsi_ctx_kernel_stats(jsi_ctx* ctx, const char* name, int64_t* launches, double* ms, double* units);
 
@@ -246,10 +250,14 @@ int jsi_run_batch(jsi_ctx* ctx, int n, const jsi_query* queries, jsi_result* res
                   int concurrency);
 
 /* Serving loop: `slots` worker streams take admitted queries as they free up
- * (continuous batching); submissions are refused with JSI_ERR_BUSY when the
- * estimated device memory of the queries in flight would exceed hbm_cap_bytes
- * (0 = no limit) or max_pending requests wait.  Surfaces can be cached by a
- * caller key with LRU eviction beyond cache_cap_bytes. */
+ * (continuous batching: a free worker takes the oldest request and every
+ * queued distance request of the same shape, up to 64, and runs them as one
+ * batch); submissions are refused with JSI_ERR_BUSY when the estimated device
+ * memory of the queries in flight would exceed hbm_cap_bytes (0 = what the
+ * context's device memory cap leaves free when the server starts; a request
+ * is always admitted when nothing is in flight) or max_pending requests wait.
+ * Surfaces can be cached by a caller key with LRU eviction beyond
+ * cache_cap_bytes. */
 typedef struct jsi_server jsi_server;
 int jsi_server_create(jsi_ctx* ctx, int slots, size_t hbm_cap_bytes, int max_pending, size_t cache_cap_bytes,
                       jsi_server** out);

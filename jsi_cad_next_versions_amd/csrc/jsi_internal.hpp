@@ -5,6 +5,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <cstring>
+
+#include <tuple>
 #include <vector>
 
 #include "../../include/jsi_mi355x.h"
@@ -298,5 +301,112 @@ void assemble_curves(const std::vector<MarchSeed>& seeds, const std::vector<Marc
 
 hipError_t launch_fill_i32(int* p, int value, unsigned long long n, hipStream_t s);
 hipError_t launch_fill_u32(unsigned* p, unsigned value, unsigned long long n, hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// Batched distance queries.  A batch is a set of queries of one shape (same
+// precision, grids, k, rounds, polish); every kernel of the query runs once
+// for the whole batch with blockIdx.y = query, reading that query's arguments
+// from a device array.  A query whose screen overflowed its candidate buffer,
+// or whose selection could not bound its top-k in two narrowing steps, sets
+// its failure flag; every later kernel of that query returns at once and the
+// host runs the query again on the single-query path.
+// ---------------------------------------------------------------------------
+template <class... A>
+struct BArgs {  // one query's arguments of a batched kernel
+  std::tuple<A...> t;
+  const unsigned* fail;
+};
+
+// Launches recorded for a batch and the host image of their arguments; the
+// image is copied to `dev` (arena) before the launches run, so the same
+// program runs again (or replays as a graph) after the image is rewritten.
+// Every recorded kernel takes one argument: a pointer into the arena.
+struct BatchLaunch {
+  const void* func;  // the kernel (host-side handle)
+  dim3 grid, block;
+  size_t off;        // its argument array in the arena
+};
+struct BatchProgram {
+  std::vector<unsigned char> arena;
+  std::vector<BatchLaunch> launches;
+  template <class X>
+  size_t put(const std::vector<X>& v) {
+    size_t off = (arena.size() + 63) & ~(size_t)63;
+    arena.resize(off + sizeof(X) * v.size());
+    std::memcpy(arena.data() + off, v.data(), sizeof(X) * v.size());
+    return off;
+  }
+  void add(const void* func, dim3 grid, dim3 block, size_t off) { launches.push_back({func, grid, block, off}); }
+  hipError_t run(const unsigned char* dev, hipStream_t s) const {
+    for (const BatchLaunch& l : launches) {
+      const void* a = dev + l.off;
+      void* args[] = {&a};
+      const hipError_t e = hipLaunchKernel(l.func, l.grid, l.block, args, 0, s);
+      if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+  }
+};
+
+struct DistBatchShape {
+  int prec = 0;                       // JSI_PREC_*
+  int nuv1 = 0, nuv2 = 0;             // round-1 grids
+  int nref = 16;                      // refinement grid
+  int rounds = 4;
+  int k = 8;
+  int cull = 1;
+  int polish = 0;                     // records polished (0 = none)
+  unsigned long long cand_cap = 0;    // candidate records per query and screen
+  unsigned small_cap = 0;             // records compacted by the selection
+  int grid = 0;                       // resident workgroups of the chip
+  int nq = 0;                         // queries
+};
+
+// Per-query inputs and device buffers of a distance batch.  [s][p]: side s,
+// ping-pong buffer p (round r reads p = (r - 1) % 2).
+template <class T>
+struct DistBatchQ {
+  SurfDev<T> sd1, sd2;
+  SurfDev<double> p1, p2;  // FP64 surfaces of the polish
+  vec4_t<T> g1, g2;        // round-1 regions (ustart, vstart, ustop, vstop)
+  double4 box1, box2;      // parameter boxes of the polish
+  vec4_t<T>* reg[2][2];
+  int* su[2][2];
+  vec4_t<T>* C[2][2];
+  vec4_t<T>* tbl[2][2];
+  vec4_t<T>* tbh[2][2];
+  float4* c32[2];          // FP32SCREEN round 1: centres rounded to float, their tile boxes
+  float4* tb32l[2];
+  float4* tb32h[2];
+  unsigned long long* amax;
+  Cand* lastrec;           // [rounds]
+  int* nunits;             // [2 rounds]
+  uint64_t* list;
+  unsigned* small;         // counters (list count, candidate count, threshold, compacted count)
+  Cand* cand;
+  Cand* topk;
+  SelState* st;
+  unsigned* hist;
+  unsigned long long* range;
+  unsigned long long* seedwork;
+  unsigned long long* best;
+  Cand* sorted[JSI_MAX_ROUNDS];  // each round's ordered top-k (k records)
+  int* fp1;
+  int* fp2;
+  unsigned char* dwork;
+  PolishRec* pol;
+  unsigned* fail;
+};
+
+// Records every launch of a distance batch (all rounds, not the polish).
+template <class T>
+int record_dist_batch(BatchProgram& prog, const DistBatchShape& sh, const std::vector<DistBatchQ<T>>& qs);
+// Records the polish of every query of a batch (jsi_refine.hip).
+template <class T>
+int record_polish_batch(BatchProgram& prog, const DistBatchShape& sh, const std::vector<DistBatchQ<T>>& qs);
+// Kernel variants a query's launches depend on (batches group equal keys).
+template <class T>
+int eval_variant(const SurfDev<T>& sd);
+int polish_variant(const SurfDev<double>& s1, const SurfDev<double>& s2);
 
 }  // namespace jsi

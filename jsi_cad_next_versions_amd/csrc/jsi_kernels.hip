@@ -572,7 +572,7 @@ constexpr int kSeedSortMax = 8192;
 // One workgroup per (slot, side-1 tile): the side-2 tile with the smallest
 // upper bound, packed as (upper-bound bits << 32 | side-2 tile).
 template <class T>
-__global__ __launch_bounds__(256) void k_dist_seed(int nslots, const int* su1, const int* su2, int w1, int w2,
+__device__ __forceinline__ void k_dist_seed_body(int nslots, const int* su1, const int* su2, int w1, int w2,
                                                    const vec4_t<T>* lo1, const vec4_t<T>* hi1,
                                                    const vec4_t<T>* lo2, const vec4_t<T>* hi2, int k,
                                                    unsigned long long* best) {
@@ -608,12 +608,19 @@ __global__ __launch_bounds__(256) void k_dist_seed(int nslots, const int* su1, c
     best[pos] = m;
   }
 }
+template <class T>
+__global__ __launch_bounds__(256) void k_dist_seed(int nslots, const int* su1, const int* su2, int w1, int w2,
+                                                   const vec4_t<T>* lo1, const vec4_t<T>* hi1,
+                                                   const vec4_t<T>* lo2, const vec4_t<T>* hi2, int k,
+                                                   unsigned long long* best) {
+  k_dist_seed_body<T>(nslots, su1, su2, w1, w2, lo1, hi1, lo2, hi2, k, best);
+}
 
 // Single workgroup: the seed tile pairs, the nseed positions with the
 // smallest upper bound (ordered = 1, one tile pair per slot and slots in rank
 // order of their parent pairs: the first nseed slots).
 template <class T>
-__global__ __launch_bounds__(1024) void k_seed_list(const unsigned long long* best, int npos, int tpu1, int nseed,
+__device__ __forceinline__ void k_seed_list_body(const unsigned long long* best, int npos, int tpu1, int nseed,
                                                     int ordered, uint64_t* list, unsigned* list_count) {
   __shared__ unsigned long long s[kSeedSortMax];
   __shared__ unsigned umax;
@@ -652,6 +659,11 @@ __global__ __launch_bounds__(1024) void k_seed_list(const unsigned long long* be
   __syncthreads();
   if (threadIdx.x == 0) *list_count = (unsigned)m;
 }
+template <class T>
+__global__ __launch_bounds__(1024) void k_seed_list(const unsigned long long* best, int npos, int tpu1, int nseed,
+                                                    int ordered, uint64_t* list, unsigned* list_count) {
+  k_seed_list_body<T>(best, npos, tpu1, nseed, ordered, list, list_count);
+}
 
 constexpr int kSeedSplitMax = 16;    // workgroups per seed tile pair at most
 constexpr int kSeedHistBins = 1024;  // bins of the seed histogram
@@ -665,7 +677,7 @@ constexpr int kSeedHistBins = 1024;  // bins of the seed histogram
 // cells have d2 within 1e-5 of each other), whose non-zero bins are added to
 // the global histogram gh.
 template <class T, bool FMA, bool MINMAX>
-__global__ __launch_bounds__(256) void k_seed_hist(const uint64_t* list, int n, int ordered, const int* su1,
+__device__ __forceinline__ void k_seed_hist_body(const uint64_t* list, int n, int ordered, const int* su1,
                                                    const int* su2, int w1, int w2, const vec4_t<T>* C1,
                                                    const vec4_t<T>* C2, int split, const unsigned long long* range,
                                                    unsigned long long* mm, unsigned* gh) {
@@ -738,10 +750,17 @@ __global__ __launch_bounds__(256) void k_seed_hist(const uint64_t* list, int n, 
   for (int b = threadIdx.x; b < kSeedHistBins; b += blockDim.x)
     if (h[b]) atomicAdd(&gh[b], h[b]);
 }
+template <class T, bool FMA, bool MINMAX>
+__global__ __launch_bounds__(256) void k_seed_hist(const uint64_t* list, int n, int ordered, const int* su1,
+                                                   const int* su2, int w1, int w2, const vec4_t<T>* C1,
+                                                   const vec4_t<T>* C2, int split, const unsigned long long* range,
+                                                   unsigned long long* mm, unsigned* gh) {
+  k_seed_hist_body<T, FMA, MINMAX>(list, n, ordered, su1, su2, w1, w2, C1, C2, split, range, mm, gh);
+}
 
 // Single workgroup: reduce the per-workgroup min/max into range[0..1] and
 // clear the global histogram for the second pass.
-__global__ __launch_bounds__(1024) void k_seed_range(const unsigned long long* mm, int nblocks,
+__device__ __forceinline__ void k_seed_range_body(const unsigned long long* mm, int nblocks,
                                                      unsigned long long* range, unsigned* gh) {
   __shared__ unsigned long long smin[1024], smax[1024];
   unsigned long long a = ~0ull, b = 0ull;
@@ -765,12 +784,16 @@ __global__ __launch_bounds__(1024) void k_seed_range(const unsigned long long* m
     range[1] = smax[0];
   }
 }
+__global__ __launch_bounds__(1024) void k_seed_range(const unsigned long long* mm, int nblocks,
+                                                     unsigned long long* range, unsigned* gh) {
+  k_seed_range_body(mm, nblocks, range, gh);
+}
 
 // Single workgroup: threshold = upper edge of the histogram bin holding rank
 // k (at least k seed pairs lie at or below it); +inf when the seeds hold
 // fewer than k pairs.
 template <class T>
-__global__ __launch_bounds__(1024) void k_seed_thresh(const unsigned* gh, const unsigned long long* range, int k,
+__device__ __forceinline__ void k_seed_thresh_body(const unsigned* gh, const unsigned long long* range, int k,
                                                       T* thresh, unsigned* clear, int nclear) {
   if ((int)threadIdx.x < nclear) clear[threadIdx.x] = 0u;  // the screen's counters, free again after the seed
   __shared__ unsigned long long cum[kSeedHistBins];
@@ -801,6 +824,11 @@ __global__ __launch_bounds__(1024) void k_seed_thresh(const unsigned* gh, const 
     }
   }
 }
+template <class T>
+__global__ __launch_bounds__(1024) void k_seed_thresh(const unsigned* gh, const unsigned long long* range, int k,
+                                                      T* thresh, unsigned* clear, int nclear) {
+  k_seed_thresh_body<T>(gh, range, k, thresh, clear, nclear);
+}
 
 // Active tile pairs of the distance screen.  Lower bound on the centre d2 of
 // any cell pair of a tile pair: per axis the gap between the two centre boxes.
@@ -808,7 +836,7 @@ __global__ __launch_bounds__(1024) void k_seed_thresh(const unsigned* gh, const 
 // bound computed in T never exceeds any computed cell-pair d2; a tile pair is
 // skipped only when the bound is strictly above the threshold.
 template <class T, bool FMA>
-__global__ void k_dist_list(int nslots, const int* su1, const int* su2, int w1, int w2, const vec4_t<T>* lo1,
+__device__ __forceinline__ void k_dist_list_body(int nslots, const int* su1, const int* su2, int w1, int w2, const vec4_t<T>* lo1,
                             const vec4_t<T>* hi1, const vec4_t<T>* lo2, const vec4_t<T>* hi2, const T* thresh,
                             int cull, uint64_t* list, unsigned* list_count) {
   const int tpu1 = tiles_per_unit(w1), tpu2 = tiles_per_unit(w2);
@@ -837,13 +865,19 @@ __global__ void k_dist_list(int nslots, const int* su1, const int* su2, int w1, 
     if (keep) list[base + __popcll(m & ((1ull << lane_id()) - 1))] = tp_encode(x.slot, x.t1, x.t2);
   }
 }
+template <class T, bool FMA>
+__global__ void k_dist_list(int nslots, const int* su1, const int* su2, int w1, int w2, const vec4_t<T>* lo1,
+                            const vec4_t<T>* hi1, const vec4_t<T>* lo2, const vec4_t<T>* hi2, const T* thresh,
+                            int cull, uint64_t* list, unsigned* list_count) {
+  k_dist_list_body<T, FMA>(nslots, su1, su2, w1, w2, lo1, hi1, lo2, hi2, thresh, cull, list, list_count);
+}
 
 // The same test with one workgroup per (slot, side-1 tile) and its threads
 // over the side-2 tiles (many tiles per unit, round 1): the side-1 box and
 // the unit ids are uniform loads, the side-2 boxes are read coalesced, and
 // each workgroup appends its kept tile pairs with one atomic per 256 tiles.
 template <class T, bool FMA>
-__global__ __launch_bounds__(256) void k_dist_list_rows(int nslots, const int* su1, const int* su2, int w1, int w2,
+__device__ __forceinline__ void k_dist_list_rows_body(int nslots, const int* su1, const int* su2, int w1, int w2,
                                                         const vec4_t<T>* lo1, const vec4_t<T>* hi1,
                                                         const vec4_t<T>* lo2, const vec4_t<T>* hi2,
                                                         const T* thresh, int cull, uint64_t* list,
@@ -881,6 +915,14 @@ __global__ __launch_bounds__(256) void k_dist_list_rows(int nslots, const int* s
     }
     __syncthreads();  // wsum / sbase are rewritten by the next chunk
   }
+}
+template <class T, bool FMA>
+__global__ __launch_bounds__(256) void k_dist_list_rows(int nslots, const int* su1, const int* su2, int w1, int w2,
+                                                        const vec4_t<T>* lo1, const vec4_t<T>* hi1,
+                                                        const vec4_t<T>* lo2, const vec4_t<T>* hi2,
+                                                        const T* thresh, int cull, uint64_t* list,
+                                                        unsigned* list_count) {
+  k_dist_list_rows_body<T, FMA>(nslots, su1, su2, w1, w2, lo1, hi1, lo2, hi2, thresh, cull, list, list_count);
 }
 
 // d2 of two centres.  FMA = false: center_distance_sq's order
@@ -943,7 +985,7 @@ __device__ __forceinline__ void add_pairs_done(unsigned long long* pairs_done, u
 }
 
 template <class T, bool FMA, bool EXP>
-__global__ __launch_bounds__(256) void k_dist_filter(const uint64_t* list, const unsigned* list_count,
+__device__ __forceinline__ void k_dist_filter_body(const uint64_t* list, const unsigned* list_count,
                                                      const int* su1, const int* su2, int w1, int w2,
                                                      const vec4_t<T>* C1, const vec4_t<T>* C2, const T* thresh,
                                                      Cand* out, unsigned long long* out_count,
@@ -1106,6 +1148,15 @@ __global__ __launch_bounds__(256) void k_dist_filter(const uint64_t* list, const
   if (wcount) flush();
   if (pairs_done) add_pairs_done(pairs_done, done);
 }
+template <class T, bool FMA, bool EXP>
+__global__ __launch_bounds__(256) void k_dist_filter(const uint64_t* list, const unsigned* list_count,
+                                                     const int* su1, const int* su2, int w1, int w2,
+                                                     const vec4_t<T>* C1, const vec4_t<T>* C2, const T* thresh,
+                                                     Cand* out, unsigned long long* out_count,
+                                                     unsigned long long cap, unsigned long long* pairs_done,
+                                                     int cull, const unsigned long long* absmax) {
+  k_dist_filter_body<T, FMA, EXP>(list, list_count, su1, su2, w1, w2, C1, C2, thresh, out, out_count, cap, pairs_done, cull, absmax);
+}
 
 // Distance filter for slots whose units are single tiles (w1, w2 <= 16: the
 // refinement rounds).  Same records as k_dist_filter (including the EXP
@@ -1117,7 +1168,7 @@ __global__ __launch_bounds__(256) void k_dist_filter(const uint64_t* list, const
 // values first.  No chunk culling: refined units are near each other
 // everywhere, so it would prune nothing.
 template <class T, bool FMA, bool EXP>
-__global__ __launch_bounds__(256) void k_dist_filter_units(const uint64_t* list, const unsigned* list_count,
+__device__ __forceinline__ void k_dist_filter_units_body(const uint64_t* list, const unsigned* list_count,
                                                            const int* su1, const int* su2, int w1, int w2,
                                                            const vec4_t<T>* C1, const vec4_t<T>* C2,
                                                            const T* thresh, Cand* out, unsigned long long* out_count,
@@ -1238,6 +1289,15 @@ __global__ __launch_bounds__(256) void k_dist_filter_units(const uint64_t* list,
   if (wcount) flush();
   if (pairs_done) add_pairs_done(pairs_done, done);
 }
+template <class T, bool FMA, bool EXP>
+__global__ __launch_bounds__(256) void k_dist_filter_units(const uint64_t* list, const unsigned* list_count,
+                                                           const int* su1, const int* su2, int w1, int w2,
+                                                           const vec4_t<T>* C1, const vec4_t<T>* C2,
+                                                           const T* thresh, Cand* out, unsigned long long* out_count,
+                                                           unsigned long long cap, unsigned long long* pairs_done,
+                                                           const unsigned long long* absmax) {
+  k_dist_filter_units_body<T, FMA, EXP>(list, list_count, su1, su2, w1, w2, C1, C2, thresh, out, out_count, cap, pairs_done, absmax);
+}
 
 // Distance filter of the FP64 refinement rounds on the matrix cores.  The
 // expanded test value s = a.(-2b) + |b|^2 (see k_dist_filter) of a cell pair is
@@ -1282,7 +1342,7 @@ __device__ __forceinline__ double min_nan(double a, double b) {
 }
 
 template <int NQ>  // NQ = (side-2 cells / 16) / 4: full side-2 blocks each wavefront holds in registers
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NQ < 4 ? 4 : 3))) void k_dist_filter_mfma(const uint64_t* list, const unsigned* list_count,
+__device__ __forceinline__ void k_dist_filter_mfma_body(const uint64_t* list, const unsigned* list_count,
                                                           int nslots, const int* su1, const int* su2, int w1, int w2,
                                                           const double4* C1, const double4* C2, const double* thresh,
                                                           Cand* out, unsigned long long* out_count,
@@ -1538,6 +1598,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NQ < 4 ? 4 
   if (wcount) flush();
   if (pairs_done) add_pairs_done(pairs_done, done);
 }
+template <int NQ>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NQ < 4 ? 4 : 3))) void k_dist_filter_mfma(const uint64_t* list, const unsigned* list_count,
+                                                          int nslots, const int* su1, const int* su2, int w1, int w2,
+                                                          const double4* C1, const double4* C2, const double* thresh,
+                                                          Cand* out, unsigned long long* out_count,
+                                                          unsigned long long cap, unsigned long long* pairs_done,
+                                                          const unsigned long long* absmax) {
+  k_dist_filter_mfma_body<NQ>(list, list_count, nslots, su1, su2, w1, w2, C1, C2, thresh, out, out_count, cap, pairs_done, absmax);
+}
 
 // ---------------------------------------------------------------------------
 // FP32 screening with an FP64 result (FP32SCREEN).  Centres are evaluated in
@@ -1553,7 +1622,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NQ < 4 ? 4 
 // (b) the float evaluation of d2 (<= 5u relative) and (c) the double
 // evaluation of d2 (<= 5 * 2^-53 relative), applied once to bound the FP64
 // k-th d2 from k float values and once more to reach every pair below it.
-__global__ void k_inflate(float* thresh, const unsigned long long* absmax_bits) {
+__device__ __forceinline__ void k_inflate_body(float* thresh, const unsigned long long* absmax_bits) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   const float t = *thresh;
   if (!(t < __uint_as_float(0x7f800000u))) return;  // +inf stays +inf
@@ -1565,18 +1634,25 @@ __global__ void k_inflate(float* thresh, const unsigned long long* absmax_bits) 
   const double r = sqrt((double)t) * g + 4.0 * 1.7320508075688772 * E * g;
   *thresh = __double2float_ru(r * r * g + 1e-36);
 }
+__global__ void k_inflate(float* thresh, const unsigned long long* absmax_bits) {
+  k_inflate_body(thresh, absmax_bits);
+}
 
 __device__ __forceinline__ void rerank_one(Cand* cand, unsigned long long e, const int* su1, const int* su2,
                                            int cells1, int cells2, const double4* C1, const double4* C2);
 
 // Candidate records get their FP64 d2 (same recipe as the FP64 screen).  The
 // record count is min(*n_dev, cap).
-__global__ void k_rerank(Cand* cand, const unsigned long long* n_dev, unsigned long long cap, const int* su1,
+__device__ __forceinline__ void k_rerank_body(Cand* cand, const unsigned long long* n_dev, unsigned long long cap, const int* su1,
                          const int* su2, int cells1, int cells2, const double4* C1, const double4* C2) {
   const unsigned long long n = min(*n_dev, cap);
   for (unsigned long long e = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; e < n;
        e += (unsigned long long)gridDim.x * blockDim.x)
     rerank_one(cand, e, su1, su2, cells1, cells2, C1, C2);
+}
+__global__ void k_rerank(Cand* cand, const unsigned long long* n_dev, unsigned long long cap, const int* su1,
+                         const int* su2, int cells1, int cells2, const double4* C1, const double4* C2) {
+  k_rerank_body(cand, n_dev, cap, su1, su2, cells1, cells2, C1, C2);
 }
 
 __device__ __forceinline__ void rerank_one(Cand* cand, unsigned long long e, const int* su1, const int* su2,
@@ -1681,8 +1757,12 @@ __global__ __launch_bounds__(256) void k_pair_mask(const uint64_t* list, const u
 }
 
 template <class T>
-__global__ void k_set_thresh(T* thresh, const Cand* rec, int idx, int use_inf) {
+__device__ __forceinline__ void k_set_thresh_body(T* thresh, const Cand* rec, int idx, int use_inf) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *thresh = use_inf ? infinity_v<T>() : from_bits<T>(rec[idx].hi);
+}
+template <class T>
+__global__ void k_set_thresh(T* thresh, const Cand* rec, int idx, int use_inf) {
+  k_set_thresh_body<T>(thresh, rec, idx, use_inf);
 }
 
 // ---------------------------------------------------------------------------
@@ -1704,7 +1784,7 @@ __device__ __forceinline__ unsigned long long sel_n(unsigned long long n, const 
   return n_dev ? min(*n_dev, n) : n;
 }
 
-__global__ __launch_bounds__(256) void k_sel_init(SelState* st, unsigned long long n,
+__device__ __forceinline__ void k_sel_init_body(SelState* st, unsigned long long n,
                                                   const unsigned long long* n_dev, unsigned long long k,
                                                   unsigned long long maxkey, unsigned small_cap, unsigned* hist,
                                                   unsigned long long* out_count) {
@@ -1722,8 +1802,14 @@ __global__ __launch_bounds__(256) void k_sel_init(SelState* st, unsigned long lo
   s.done = (n <= small_cap || s.need == 0) ? 1 : 0;
   *st = s;
 }
+__global__ __launch_bounds__(256) void k_sel_init(SelState* st, unsigned long long n,
+                                                  const unsigned long long* n_dev, unsigned long long k,
+                                                  unsigned long long maxkey, unsigned small_cap, unsigned* hist,
+                                                  unsigned long long* out_count) {
+  k_sel_init_body(st, n, n_dev, k, maxkey, small_cap, hist, out_count);
+}
 
-__global__ __launch_bounds__(256) void k_sel_minmax(const Cand* cand, unsigned long long n,
+__device__ __forceinline__ void k_sel_minmax_body(const Cand* cand, unsigned long long n,
                                                     const unsigned long long* n_dev, SelState* st) {
   __shared__ unsigned long long rmin[4], rmax[4];
   if (st->done) return;
@@ -1756,8 +1842,12 @@ __global__ __launch_bounds__(256) void k_sel_minmax(const Cand* cand, unsigned l
     atomicMax(&st->hi, b);
   }
 }
+__global__ __launch_bounds__(256) void k_sel_minmax(const Cand* cand, unsigned long long n,
+                                                    const unsigned long long* n_dev, SelState* st) {
+  k_sel_minmax_body(cand, n, n_dev, st);
+}
 
-__global__ __launch_bounds__(256) void k_sel_hist(const Cand* cand, unsigned long long n,
+__device__ __forceinline__ void k_sel_hist_body(const Cand* cand, unsigned long long n,
                                                   const unsigned long long* n_dev, const SelState* st,
                                                   unsigned* hist) {
   __shared__ unsigned h[kSelBins];
@@ -1776,8 +1866,13 @@ __global__ __launch_bounds__(256) void k_sel_hist(const Cand* cand, unsigned lon
   for (int b = threadIdx.x; b < kSelBins; b += blockDim.x)
     if (h[b]) atomicAdd(&hist[b], h[b]);
 }
+__global__ __launch_bounds__(256) void k_sel_hist(const Cand* cand, unsigned long long n,
+                                                  const unsigned long long* n_dev, const SelState* st,
+                                                  unsigned* hist) {
+  k_sel_hist_body(cand, n, n_dev, st, hist);
+}
 
-__global__ __launch_bounds__(1024) void k_sel_step(SelState* st, unsigned* hist, unsigned small_cap) {
+__device__ __forceinline__ void k_sel_step_body(SelState* st, unsigned* hist, unsigned small_cap) {
   __shared__ unsigned long long cum[1024];
   __shared__ int found;
   __shared__ unsigned long long found_before;
@@ -1824,8 +1919,11 @@ __global__ __launch_bounds__(1024) void k_sel_step(SelState* st, unsigned* hist,
   __syncthreads();
   for (int b = threadIdx.x; b < kSelBins; b += 1024) hist[b] = 0;
 }
+__global__ __launch_bounds__(1024) void k_sel_step(SelState* st, unsigned* hist, unsigned small_cap) {
+  k_sel_step_body(st, hist, small_cap);
+}
 
-__global__ __launch_bounds__(256) void k_sel_compact(const Cand* cand, unsigned long long n,
+__device__ __forceinline__ void k_sel_compact_body(const Cand* cand, unsigned long long n,
                                                      const unsigned long long* n_dev, const SelState* st, Cand* out,
                                                      unsigned long long* out_count, unsigned long long cap) {
   n = sel_n(n, n_dev);
@@ -1850,6 +1948,11 @@ __global__ __launch_bounds__(256) void k_sel_compact(const Cand* cand, unsigned 
     }
   }
 }
+__global__ __launch_bounds__(256) void k_sel_compact(const Cand* cand, unsigned long long n,
+                                                     const unsigned long long* n_dev, const SelState* st, Cand* out,
+                                                     unsigned long long* out_count, unsigned long long cap) {
+  k_sel_compact_body(cand, n, n_dev, st, out, out_count, cap);
+}
 
 template <class T>
 __global__ void k_thresh_from_state(T* thresh, const SelState* st) {
@@ -1868,7 +1971,7 @@ __global__ void k_set_thresh_ru(float* thresh, const Cand* rec, int idx) {
 // Start of a distance query: both sides' round-1 regions, slot 0 -> unit 0,
 // and the max |coordinate| slots cleared (one launch instead of five).
 template <class T>
-__global__ void k_query_init(vec4_t<T>* reg1, vec4_t<T> g1, vec4_t<T>* reg2, vec4_t<T> g2, int* su1, int* su2,
+__device__ __forceinline__ void k_query_init_body(vec4_t<T>* reg1, vec4_t<T> g1, vec4_t<T>* reg2, vec4_t<T> g2, int* su1, int* su2,
                              unsigned long long* amax) {
   if (amax && threadIdx.x < kAmaxSlots) amax[threadIdx.x] = 0ull;
   if (threadIdx.x == 0) {
@@ -1877,6 +1980,11 @@ __global__ void k_query_init(vec4_t<T>* reg1, vec4_t<T> g1, vec4_t<T>* reg2, vec
     *su1 = 0;
     *su2 = 0;
   }
+}
+template <class T>
+__global__ void k_query_init(vec4_t<T>* reg1, vec4_t<T> g1, vec4_t<T>* reg2, vec4_t<T> g2, int* su1, int* su2,
+                             unsigned long long* amax) {
+  k_query_init_body<T>(reg1, g1, reg2, g2, su1, su2, amax);
 }
 
 // Small device-side store (no host round trip): one record.
@@ -1959,7 +2067,7 @@ __global__ __launch_bounds__(1024) void k_bsort_single(Cand* a, int n2) {
 // records at a time in LDS (broadcast reads), and the partial ranks are
 // summed in LDS.  O(m^2) compares spread over m/64 * 16 wavefronts.
 constexpr int kRankWaves = 16;
-__global__ __launch_bounds__(1024) void k_rank_sort(const Cand* in, const unsigned long long* m_dev, unsigned m_cap,
+__device__ __forceinline__ void k_rank_sort_body(const Cand* in, const unsigned long long* m_dev, unsigned m_cap,
                                                     unsigned keep, Cand* out, Cand* last_out) {
   __shared__ Cand stage[kRankWaves][64];
   __shared__ unsigned part[kRankWaves][64];
@@ -1993,6 +2101,10 @@ __global__ __launch_bounds__(1024) void k_rank_sort(const Cand* in, const unsign
     if (r < keep) out[r] = mine;
     if (last_out && r == min(keep, m) - 1) *last_out = mine;  // the last kept record
   }
+}
+__global__ __launch_bounds__(1024) void k_rank_sort(const Cand* in, const unsigned long long* m_dev, unsigned m_cap,
+                                                    unsigned keep, Cand* out, Cand* last_out) {
+  k_rank_sort_body(in, m_dev, m_cap, keep, out, last_out);
 }
 
 __global__ void k_bsort_global(Cand* a, unsigned long long n2, unsigned long long k, unsigned long long j) {
@@ -2042,7 +2154,7 @@ __device__ __forceinline__ void dedup_decode(const Cand* rec, int e, int* s, int
 
 // Pass 1: firstpos[slot * cells + cell] = smallest record index with that
 // (slot, cell), per side (tables hold INT_MAX between uses).
-__global__ __launch_bounds__(256) void k_dedup_mark(const Cand* rec, int n, int cells1, int cells2, int* fp1,
+__device__ __forceinline__ void k_dedup_mark_body(const Cand* rec, int n, int cells1, int cells2, int* fp1,
                                                     int* fp2) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
@@ -2051,12 +2163,16 @@ __global__ __launch_bounds__(256) void k_dedup_mark(const Cand* rec, int n, int 
   atomicMin(&fp1[(long long)s * cells1 + i], e);
   atomicMin(&fp2[(long long)s * cells2 + j], e);
 }
+__global__ __launch_bounds__(256) void k_dedup_mark(const Cand* rec, int n, int cells1, int cells2, int* fp1,
+                                                    int* fp2) {
+  k_dedup_mark_body(rec, n, cells1, cells2, fp1, fp2);
+}
 
 // Pass 2 (single workgroup, contiguous chunks of <= 8 records per thread, all
 // loads issued up front): a record whose firstpos is itself opens a new unit;
 // unit ids follow record order (first appearance); every record's next-round
 // slot map entry is the unit of its first appearance.
-__global__ __launch_bounds__(1024) void k_dedup_units(const Cand* rec, int n, int cells1, int cells2,
+__device__ __forceinline__ void k_dedup_units_body(const Cand* rec, int n, int cells1, int cells2,
                                                       const int* fp1, const int* fp2, int* nsu1, int* nsu2,
                                                       unsigned char* isfirst, int* nunits, unsigned long long* zero,
                                                       int nzero) {
@@ -2108,10 +2224,16 @@ __global__ __launch_bounds__(1024) void k_dedup_units(const Cand* rec, int n, in
     __syncthreads();  // sUnit is reused by the next side
   }
 }
+__global__ __launch_bounds__(1024) void k_dedup_units(const Cand* rec, int n, int cells1, int cells2,
+                                                      const int* fp1, const int* fp2, int* nsu1, int* nsu2,
+                                                      unsigned char* isfirst, int* nunits, unsigned long long* zero,
+                                                      int nzero) {
+  k_dedup_units_body(rec, n, cells1, cells2, fp1, fp2, nsu1, nsu2, isfirst, nunits, zero, nzero);
+}
 
 // Pass 3: child regions of the new units, and firstpos back to INT_MAX.
 template <class T>
-__global__ __launch_bounds__(256) void k_dedup_apply(const Cand* rec, int n, int cells1, int cells2, const int* su1,
+__device__ __forceinline__ void k_dedup_apply_body(const Cand* rec, int n, int cells1, int cells2, const int* su1,
                                                      const int* su2, const vec4_t<T>* reg1, const vec4_t<T>* reg2,
                                                      int pnuv1, int pnuv2, const unsigned char* isfirst,
                                                      const int* nsu1, const int* nsu2, vec4_t<T>* nreg1,
@@ -2124,6 +2246,14 @@ __global__ __launch_bounds__(256) void k_dedup_apply(const Cand* rec, int n, int
   if (isfirst[kDedupMax + e]) nreg2[nsu2[e]] = child_region<T>(reg2[su2[s]], pnuv2, j);
   fp1[(long long)s * cells1 + i] = INT_MAX;
   fp2[(long long)s * cells2 + j] = INT_MAX;
+}
+template <class T>
+__global__ __launch_bounds__(256) void k_dedup_apply(const Cand* rec, int n, int cells1, int cells2, const int* su1,
+                                                     const int* su2, const vec4_t<T>* reg1, const vec4_t<T>* reg2,
+                                                     int pnuv1, int pnuv2, const unsigned char* isfirst,
+                                                     const int* nsu1, const int* nsu2, vec4_t<T>* nreg1,
+                                                     vec4_t<T>* nreg2, int* fp1, int* fp2) {
+  k_dedup_apply_body<T>(rec, n, cells1, cells2, su1, su2, reg1, reg2, pnuv1, pnuv2, isfirst, nsu1, nsu2, nreg1, nreg2, fp1, fp2);
 }
 
 // ---------------------------------------------------------------------------
@@ -2348,9 +2478,12 @@ __global__ void k_fill_i32(int* p, int v, unsigned long long n) {
   const unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
 }
-__global__ void k_fill_u32(unsigned* p, unsigned v, unsigned long long n) {
+__device__ __forceinline__ void k_fill_u32_body(unsigned* p, unsigned v, unsigned long long n) {
   const unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
+}
+__global__ void k_fill_u32(unsigned* p, unsigned v, unsigned long long n) {
+  k_fill_u32_body(p, v, n);
 }
 
 }  // namespace
@@ -3204,5 +3337,558 @@ hipError_t launch_fill_u32(unsigned* p, unsigned value, unsigned long long n, hi
 JSI_INST(float)
 JSI_INST(double)
 #undef JSI_INST
+
+
+// ---------------------------------------------------------------------------
+// Batched distance queries (jsi_internal.hpp).  Each kernel of the query runs
+// once for the whole batch: blockIdx.y selects the query, whose arguments are
+// one tuple of the body's parameters in a device array; a query whose failure
+// flag is set returns at once.  The bodies are the single-query kernels'
+// bodies, so every query computes exactly what it computes alone.
+// ---------------------------------------------------------------------------
+template <auto F>
+struct BodyArgs;
+template <class... P, void (*F)(P...)>
+struct BodyArgs<F> {
+  using tuple = std::tuple<std::decay_t<P>...>;
+  using type = BArgs<std::decay_t<P>...>;
+};
+
+template <int LB, auto F>
+__global__ __launch_bounds__(LB) void k_batched(const typename BodyArgs<F>::type* args) {
+  const auto& x = args[blockIdx.y];
+  if (x.fail && *x.fail) return;
+  std::apply([](auto... a) { F(a...); }, x.t);
+}
+template <int LB, int WPE, auto F>
+__global__ __launch_bounds__(LB) __attribute__((amdgpu_waves_per_eu(WPE))) void k_batched_wpe(
+    const typename BodyArgs<F>::type* args) {
+  const auto& x = args[blockIdx.y];
+  if (x.fail && *x.fail) return;
+  std::apply([](auto... a) { F(a...); }, x.t);
+}
+
+template <auto F, class... X>
+static typename BodyArgs<F>::type bargs(const unsigned* fail, X... x) {
+  return typename BodyArgs<F>::type{typename BodyArgs<F>::tuple(x...), fail};
+}
+
+// Records one batched launch: grid (gx, queries), `block` threads.
+template <int LB, auto F>
+static void brec(BatchProgram& p, unsigned gx, unsigned block, const std::vector<typename BodyArgs<F>::type>& v) {
+  if (v.empty() || gx == 0) return;
+  const size_t off = p.put(v);
+  p.add(reinterpret_cast<const void*>(&k_batched<LB, F>), dim3(gx, (unsigned)v.size()), dim3(block), off);
+}
+template <int LB, int WPE, auto F>
+static void brec_wpe(BatchProgram& p, unsigned gx, unsigned block, const std::vector<typename BodyArgs<F>::type>& v) {
+  if (v.empty() || gx == 0) return;
+  const size_t off = p.put(v);
+  p.add(reinterpret_cast<const void*>(&k_batched_wpe<LB, WPE, F>), dim3(gx, (unsigned)v.size()), dim3(block), off);
+}
+
+// Per query: the failure flag after a screen and its selection.  A query
+// fails when its candidates overflowed the buffer (flag bit 1), when fewer
+// than k pairs exist (bit 2), or when two narrowing steps left more records
+// than the compaction holds (bit 4: many keys tied with the k-th); the
+// single-query path handles all three by repeating steps.
+__device__ __forceinline__ void k_batch_check_body(const unsigned long long* out_count, unsigned long long cap,
+                                                   const unsigned long long* small_count, unsigned small_cap,
+                                                   unsigned long long k, unsigned* fail) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const unsigned long long n = *out_count, m = *small_count;
+  const unsigned why = (n > cap ? 1u : 0u) | (n < k ? 2u : 0u) | (m > (unsigned long long)small_cap ? 4u : 0u);
+  if (why) *fail = why;
+}
+
+template <class T>
+__device__ __forceinline__ int ec_blocks_d(const EcJob<T>& j) {
+  if (j.nunits <= 0 || j.nuv < 2) return 0;
+  const int tw = (j.nuv - 1 + kTile - 1) / kTile;
+  return j.nunits * tw * tw;
+}
+
+// Fused evaluation + centres of many jobs: blockIdx.y = job.
+template <class T>
+struct EcBJob {
+  EcJob<T> j;
+  const unsigned* fail;
+};
+template <class T, int MAXD, bool RAT, bool KL>
+__global__ __launch_bounds__(64) void k_eval_centres_b(const EcBJob<T>* jobs) {
+  const EcBJob<T>& x = jobs[blockIdx.y];
+  if (x.fail && *x.fail) return;
+  if ((int)blockIdx.x >= ec_blocks_d(x.j)) return;
+  ec_tile<T, MAXD, RAT, KL>(x.j.sd, x.j.reg, x.j.nunits, x.j.nuv, x.j.o, (int)blockIdx.x);
+}
+
+template <class T>
+int eval_variant(const SurfDev<T>& sd) {
+  EcJob<T> j;
+  j.sd = sd;
+  return ec_variant(j);
+}
+template int eval_variant<float>(const SurfDev<float>&);
+template int eval_variant<double>(const SurfDev<double>&);
+
+template <class T, int MAXD>
+static void ec_brec_rat(BatchProgram& p, int var, unsigned gx, const std::vector<EcBJob<T>>& v) {
+  const size_t off = p.put(v);
+  const bool rat = (var / 3) % 2 == 1, kl = var >= 6;
+  const void* f = rat ? (kl ? reinterpret_cast<const void*>(&k_eval_centres_b<T, MAXD, true, true>)
+                            : reinterpret_cast<const void*>(&k_eval_centres_b<T, MAXD, true, false>))
+                      : (kl ? reinterpret_cast<const void*>(&k_eval_centres_b<T, MAXD, false, true>)
+                            : reinterpret_cast<const void*>(&k_eval_centres_b<T, MAXD, false, false>));
+  p.add(f, dim3(gx, (unsigned)v.size()), dim3(64), off);
+}
+
+// Jobs grouped by kernel variant, one launch per variant present.
+template <class T>
+static void ec_brec(BatchProgram& p, const std::vector<EcBJob<T>>& jobs) {
+  std::vector<int> vars;
+  for (const auto& x : jobs) {
+    const int v = ec_variant(x.j);
+    if (std::find(vars.begin(), vars.end(), v) == vars.end()) vars.push_back(v);
+  }
+  for (int v : vars) {
+    std::vector<EcBJob<T>> g;
+    int gx = 0;
+    for (const auto& x : jobs)
+      if (ec_variant(x.j) == v) {
+        g.push_back(x);
+        gx = std::max(gx, ec_blocks(x.j));
+      }
+    if (gx <= 0) continue;
+    switch (v % 3) {
+      case 0: ec_brec_rat<T, 3>(p, v, (unsigned)gx, g); break;
+      case 1: ec_brec_rat<T, 7>(p, v, (unsigned)gx, g); break;
+      default: ec_brec_rat<T, kMaxDeg - 1>(p, v, (unsigned)gx, g); break;
+    }
+  }
+}
+
+// One query's view of a screening round.
+template <class S>
+struct TopkQ {
+  const int *su1, *su2;
+  const vec4_t<S>*C1, *C2, *lo1, *hi1, *lo2, *hi2;
+  const double4 *R1, *R2;              // FP64 centres of the re-ranking (FP32SCREEN)
+  const unsigned long long* exp_absmax;  // FP64: the expanded test's max |coordinate|
+  const unsigned long long* absmax;      // FP32SCREEN: the widening's max |coordinate|
+  uint64_t* list;
+  unsigned* small;
+  Cand* cand;
+  Cand* topk;
+  SelState* st;
+  unsigned* hist;
+  unsigned long long* range;
+  unsigned long long* work;
+  unsigned long long* best;
+  Cand* sorted;
+  Cand* last;
+  unsigned* fail;
+};
+
+// The launches of one screen (dist_topk's sequence without its host round
+// trips): seed threshold, tile-pair list, filter, re-rank, two narrowing
+// steps, compaction, the failure check, rank sort.
+template <class S>
+static void topk_brec(BatchProgram& p, const DistBatchShape& sh, const std::vector<TopkQ<S>>& tq, int nslots,
+                      int w1, int w2, bool rerank, bool exp) {
+  const int B = (int)tq.size();
+  const int fma = rerank ? 1 : 0;
+  const int k = sh.k;
+  const int tpu1 = tiles_per_unit_h(w1), tpu2 = tiles_per_unit_h(w2);
+  const unsigned long long ntp = (unsigned long long)nslots * tpu1 * tpu2;
+  const long long npos = (long long)nslots * tpu1;
+  const int pairs_tp = std::min(w1 * w1, kTileCells) * std::min(w2 * w2, kTileCells);
+  const unsigned long long want = (32ull * (unsigned long long)k + pairs_tp - 1) / (unsigned long long)pairs_tp;
+  const int nseed = nslots == 1 ? std::min(256, tpu1) : (int)std::max(4ull, std::min(64ull, want));
+  const unsigned long long cap = sh.cand_cap;
+  const unsigned small_cap = sh.small_cap;
+  auto thr = [](const TopkQ<S>& q) { return reinterpret_cast<S*>(q.small + 6); };
+  auto out_count = [](const TopkQ<S>& q) { return reinterpret_cast<unsigned long long*>(q.small + 4); };
+  auto small_count = [](const TopkQ<S>& q) { return reinterpret_cast<unsigned long long*>(q.small + 10); };
+  bool counters_clear = false;
+  if (npos <= 8192) {
+    const int ordered = (nslots > 1 && tpu1 == 1 && tpu2 == 1) ? 1 : 0;
+    if (!ordered) {
+      using FT = std::integral_constant<decltype(&k_dist_seed_body<S>), &k_dist_seed_body<S>>;
+      std::vector<typename BodyArgs<FT::value>::type> a;
+      for (const auto& q : tq) a.push_back(bargs<FT::value>(q.fail, nslots, q.su1, q.su2, w1, w2, q.lo1, q.hi1, q.lo2, q.hi2, k, q.best));
+      brec<256, FT::value>(p, (unsigned)npos, 256, a);
+      using GT = std::integral_constant<decltype(&k_seed_list_body<S>), &k_seed_list_body<S>>;
+      std::vector<typename BodyArgs<GT::value>::type> b;
+      for (const auto& q : tq) b.push_back(bargs<GT::value>(q.fail, q.best, (int)npos, tpu1, nseed, ordered, q.list, q.small));
+      brec<1024, GT::value>(p, 1, 1024, b);
+    }
+    const int m = nseed < npos ? nseed : (int)npos;
+    int split = m > 0 ? 1024 / m : 1;
+    split = split < 1 ? 1 : (split > kSeedSplitMax ? kSeedSplitMax : split);
+    const int nb = m > 0 ? m * split : 1;
+    auto gh = [](const TopkQ<S>& q) {
+      return reinterpret_cast<unsigned*>(q.work + 2 * (size_t)kSeedMaxTiles * kSeedSplitMax);
+    };
+    auto hist_pass = [&](auto fh) {
+      using FT = decltype(fh);
+      std::vector<typename BodyArgs<FT::value>::type> a;
+      for (const auto& q : tq)
+        a.push_back(bargs<FT::value>(q.fail, (const uint64_t*)q.list, m, ordered, q.su1, q.su2, w1, w2, q.C1, q.C2, split,
+                             (const unsigned long long*)q.range, q.work, gh(q)));
+      brec<256, FT::value>(p, (unsigned)nb, 256, a);
+    };
+    if (fma)
+      hist_pass(std::integral_constant<decltype(&k_seed_hist_body<S, true, true>), &k_seed_hist_body<S, true, true>>());
+    else
+      hist_pass(
+          std::integral_constant<decltype(&k_seed_hist_body<S, false, true>), &k_seed_hist_body<S, false, true>>());
+    {
+      using FT = std::integral_constant<decltype(&k_seed_range_body), &k_seed_range_body>;
+      std::vector<typename BodyArgs<FT::value>::type> a;
+      for (const auto& q : tq) a.push_back(bargs<FT::value>(q.fail, (const unsigned long long*)q.work, nb, q.range, gh(q)));
+      brec<1024, FT::value>(p, 1, 1024, a);
+    }
+    if (fma)
+      hist_pass(
+          std::integral_constant<decltype(&k_seed_hist_body<S, true, false>), &k_seed_hist_body<S, true, false>>());
+    else
+      hist_pass(
+          std::integral_constant<decltype(&k_seed_hist_body<S, false, false>), &k_seed_hist_body<S, false, false>>());
+    {
+      using FT = std::integral_constant<decltype(&k_seed_thresh_body<S>), &k_seed_thresh_body<S>>;
+      std::vector<typename BodyArgs<FT::value>::type> a;
+      for (const auto& q : tq)
+        a.push_back(bargs<FT::value>(q.fail, (const unsigned*)gh(q), (const unsigned long long*)q.range, k, thr(q), q.small, 6));
+      brec<1024, FT::value>(p, 1, kSeedHistBins, a);
+    }
+    if (rerank) {
+      using FT = std::integral_constant<decltype(&k_inflate_body), &k_inflate_body>;
+      std::vector<typename BodyArgs<FT::value>::type> a;
+      for (const auto& q : tq) a.push_back(bargs<FT::value>(q.fail, reinterpret_cast<float*>(thr(q)), q.absmax));
+      brec<1024, FT::value>(p, 1, 64, a);
+    }
+    counters_clear = true;
+  } else {
+    using FT = std::integral_constant<decltype(&k_set_thresh_body<S>), &k_set_thresh_body<S>>;
+    std::vector<typename BodyArgs<FT::value>::type> a;
+    for (const auto& q : tq) a.push_back(bargs<FT::value>(q.fail, thr(q), (const Cand*)q.cand, 0, 1));
+    brec<1024, FT::value>(p, 1, 64, a);
+  }
+  auto fill = [&](auto ptr_of, unsigned long long n) {
+    using FT = std::integral_constant<decltype(&k_fill_u32_body), &k_fill_u32_body>;
+    std::vector<typename BodyArgs<FT::value>::type> a;
+    for (const auto& q : tq) a.push_back(bargs<FT::value>(q.fail, ptr_of(q), 0u, n));
+    brec<1024, FT::value>(p, 1, 64, a);
+  };
+  if (!counters_clear) fill([](const TopkQ<S>& q) { return q.small; }, 1);
+  {  // tile-pair list
+    const long long rows = npos;
+    auto list_pass = [&](auto fl, unsigned gx) {
+      using FT = decltype(fl);
+      std::vector<typename BodyArgs<FT::value>::type> a;
+      for (const auto& q : tq)
+        a.push_back(bargs<FT::value>(q.fail, nslots, q.su1, q.su2, w1, w2, q.lo1, q.hi1, q.lo2, q.hi2, (const S*)thr(q),
+                             sh.cull, q.list, q.small));
+      brec<1024, FT::value>(p, gx, 256, a);
+    };
+    if (tpu2 >= 64) {
+      if (fma)
+        list_pass(std::integral_constant<decltype(&k_dist_list_rows_body<S, true>), &k_dist_list_rows_body<S, true>>(),
+                  (unsigned)rows);
+      else
+        list_pass(
+            std::integral_constant<decltype(&k_dist_list_rows_body<S, false>), &k_dist_list_rows_body<S, false>>(),
+            (unsigned)rows);
+    } else {
+      const unsigned g = blocks_for((long long)ntp, 256);
+      if (fma)
+        list_pass(std::integral_constant<decltype(&k_dist_list_body<S, true>), &k_dist_list_body<S, true>>(), g);
+      else
+        list_pass(std::integral_constant<decltype(&k_dist_list_body<S, false>), &k_dist_list_body<S, false>>(), g);
+    }
+  }
+  if (!counters_clear) fill([](const TopkQ<S>& q) { return q.small + 4; }, 2);
+  // the filter
+  if (w1 <= kTile && w2 <= kTile) {
+    const unsigned gu = (unsigned)ntp;
+    if (exp) {
+      if constexpr (std::is_same<S, double>::value) {
+        // as the single-query launch: workgroups walk at most 8 slots each
+        const unsigned gx = std::min(gu, std::max(1u, (gu + 7u) / 8u));
+        const int nq = (w2 * w2 / 16) / 4;
+        auto mf = [&](auto fm, auto wpe) {
+          using FT = decltype(fm);
+          constexpr int W = decltype(wpe)::value;
+          std::vector<typename BodyArgs<FT::value>::type> a;
+          for (const auto& q : tq)
+            a.push_back(bargs<FT::value>(q.fail, (const uint64_t*)q.list, (const unsigned*)q.small, (int)gu, q.su1, q.su2,
+                                 w1, w2, q.C1, q.C2, (const double*)thr(q), q.cand, out_count(q), cap,
+                                 (unsigned long long*)nullptr, q.exp_absmax));
+          brec_wpe<256, W, FT::value>(p, gx, 256, a);
+        };
+#define JSI_MF(NQ)                                                                              \
+  mf(std::integral_constant<decltype(&k_dist_filter_mfma_body<NQ>), &k_dist_filter_mfma_body<NQ>>(), \
+     std::integral_constant<int, (NQ < 4 ? 4 : 3)>())
+        switch (nq) {
+          case 0: JSI_MF(0); break;
+          case 1: JSI_MF(1); break;
+          case 2: JSI_MF(2); break;
+          case 3: JSI_MF(3); break;
+          default: JSI_MF(4); break;
+        }
+#undef JSI_MF
+      }
+    } else {
+      auto fu = [&](auto ff) {
+        using FT = decltype(ff);
+        std::vector<typename BodyArgs<FT::value>::type> a;
+        for (const auto& q : tq)
+          a.push_back(bargs<FT::value>(q.fail, (const uint64_t*)q.list, (const unsigned*)q.small, q.su1, q.su2, w1, w2, q.C1,
+                               q.C2, (const S*)thr(q), q.cand, out_count(q), cap, (unsigned long long*)nullptr,
+                               (const unsigned long long*)nullptr));
+        brec<256, FT::value>(p, gu, 256, a);
+      };
+      if (fma)
+        fu(std::integral_constant<decltype(&k_dist_filter_units_body<S, true, false>),
+                                  &k_dist_filter_units_body<S, true, false>>());
+      else
+        fu(std::integral_constant<decltype(&k_dist_filter_units_body<S, false, false>),
+                                  &k_dist_filter_units_body<S, false, false>>());
+    }
+  } else {
+    // round 1: persistent workgroups walking the query's listed tile pairs,
+    // about twice the resident workgroups over the whole batch
+    const unsigned gx = (unsigned)std::min<unsigned long long>(
+        ntp, std::max(16ull, (2ull * (unsigned long long)sh.grid + B - 1) / (unsigned long long)B));
+    auto ft = [&](auto ff) {
+      using FT = decltype(ff);
+      std::vector<typename BodyArgs<FT::value>::type> a;
+      for (const auto& q : tq)
+        a.push_back(bargs<FT::value>(q.fail, (const uint64_t*)q.list, (const unsigned*)q.small, q.su1, q.su2, w1, w2, q.C1,
+                             q.C2, (const S*)thr(q), q.cand, out_count(q), cap, (unsigned long long*)nullptr, sh.cull,
+                             exp ? q.exp_absmax : (const unsigned long long*)nullptr));
+      brec<256, FT::value>(p, gx, 256, a);
+    };
+    if (exp) {
+      if constexpr (std::is_same<S, double>::value)
+        ft(std::integral_constant<decltype(&k_dist_filter_body<S, false, true>), &k_dist_filter_body<S, false, true>>());
+    } else if (fma) {
+      ft(std::integral_constant<decltype(&k_dist_filter_body<S, true, false>), &k_dist_filter_body<S, true, false>>());
+    } else {
+      ft(std::integral_constant<decltype(&k_dist_filter_body<S, false, false>),
+                                &k_dist_filter_body<S, false, false>>());
+    }
+  }
+  // selection over the query's own records (grid-stride over at most cap)
+  const unsigned gs = (unsigned)std::max(1ull, std::min<unsigned long long>(64, (cap + 255) / 256));
+  if (rerank) {
+    using FT = std::integral_constant<decltype(&k_rerank_body), &k_rerank_body>;
+    std::vector<typename BodyArgs<FT::value>::type> a;
+    for (const auto& q : tq)
+      a.push_back(bargs<FT::value>(q.fail, q.cand, (const unsigned long long*)out_count(q), cap, q.su1, q.su2, w1 * w1, w2 * w2,
+                           q.R1, q.R2));
+    brec<1024, FT::value>(p, gs, 256, a);
+  }
+  const unsigned long long maxkey =
+      rerank ? 0x7ff0000000000000ull : (std::is_same<S, float>::value ? 0x7f800000ull : 0x7ff0000000000000ull);
+  {
+    using FT = std::integral_constant<decltype(&k_sel_init_body), &k_sel_init_body>;
+    std::vector<typename BodyArgs<FT::value>::type> a;
+    for (const auto& q : tq)
+      a.push_back(bargs<FT::value>(q.fail, q.st, cap, (const unsigned long long*)out_count(q), (unsigned long long)k, maxkey,
+                           small_cap, q.hist, small_count(q)));
+    brec<256, FT::value>(p, 1, 256, a);
+  }
+  {
+    using FT = std::integral_constant<decltype(&k_sel_minmax_body), &k_sel_minmax_body>;
+    std::vector<typename BodyArgs<FT::value>::type> a;
+    for (const auto& q : tq) a.push_back(bargs<FT::value>(q.fail, (const Cand*)q.cand, cap, (const unsigned long long*)out_count(q), q.st));
+    brec<256, FT::value>(p, gs, 256, a);
+  }
+  for (int it = 0; it < 2; ++it) {
+    {
+      using FT = std::integral_constant<decltype(&k_sel_hist_body), &k_sel_hist_body>;
+      std::vector<typename BodyArgs<FT::value>::type> a;
+      for (const auto& q : tq)
+        a.push_back(bargs<FT::value>(q.fail, (const Cand*)q.cand, cap, (const unsigned long long*)out_count(q),
+                             (const SelState*)q.st, q.hist));
+      brec<256, FT::value>(p, gs, 256, a);
+    }
+    {
+      using FT = std::integral_constant<decltype(&k_sel_step_body), &k_sel_step_body>;
+      std::vector<typename BodyArgs<FT::value>::type> a;
+      for (const auto& q : tq) a.push_back(bargs<FT::value>(q.fail, q.st, q.hist, small_cap));
+      brec<1024, FT::value>(p, 1, 1024, a);
+    }
+  }
+  {
+    using FT = std::integral_constant<decltype(&k_sel_compact_body), &k_sel_compact_body>;
+    std::vector<typename BodyArgs<FT::value>::type> a;
+    for (const auto& q : tq)
+      a.push_back(bargs<FT::value>(q.fail, (const Cand*)q.cand, cap, (const unsigned long long*)out_count(q),
+                           (const SelState*)q.st, q.topk, small_count(q), (unsigned long long)small_cap));
+    brec<256, FT::value>(p, gs, 256, a);
+  }
+  {
+    using FT = std::integral_constant<decltype(&k_batch_check_body), &k_batch_check_body>;
+    std::vector<typename BodyArgs<FT::value>::type> a;
+    for (const auto& q : tq)
+      a.push_back(bargs<FT::value>(q.fail, (const unsigned long long*)out_count(q), cap,
+                           (const unsigned long long*)small_count(q), small_cap, (unsigned long long)k, q.fail));
+    brec<1024, FT::value>(p, 1, 64, a);
+  }
+  {
+    using FT = std::integral_constant<decltype(&k_rank_sort_body), &k_rank_sort_body>;
+    std::vector<typename BodyArgs<FT::value>::type> a;
+    for (const auto& q : tq)
+      a.push_back(bargs<FT::value>(q.fail, (const Cand*)q.topk, (const unsigned long long*)small_count(q), small_cap,
+                           (unsigned)k, q.sorted, q.last));
+    brec<1024, FT::value>(p, blocks_for(small_cap, 64), 64 * kRankWaves, a);
+  }
+}
+
+template <class T>
+int record_dist_batch(BatchProgram& p, const DistBatchShape& sh, const std::vector<DistBatchQ<T>>& qs) {
+  using V = vec4_t<T>;
+  const bool dbl = std::is_same<T, double>::value;
+  const bool s32 = dbl && sh.prec == JSI_PREC_FP32SCREEN;
+  const int k = sh.k;
+  if (qs.empty()) return JSI_OK;
+  if (k < 1 || k > kDedupMax || sh.rounds < 1 || sh.nuv1 < 2 || sh.nuv2 < 2 || sh.nref < 2) return JSI_ERR_INVALID;
+  {
+    using FT = std::integral_constant<decltype(&k_query_init_body<T>), &k_query_init_body<T>>;
+    std::vector<typename BodyArgs<FT::value>::type> a;
+    for (const auto& q : qs)
+      a.push_back(bargs<FT::value>(q.fail, q.reg[0][0], q.g1, q.reg[1][0], q.g2, q.su[0][0], q.su[1][0],
+                           dbl ? q.amax : (unsigned long long*)nullptr));
+    brec<1024, FT::value>(p, 1, 64, a);
+  }
+  {  // round-1 evaluation of both sides
+    std::vector<EcBJob<T>> jobs;
+    for (int s = 0; s < 2; ++s)
+      for (const auto& q : qs) {
+        EcBJob<T> x;
+        x.fail = q.fail;
+        x.j.sd = s ? q.sd2 : q.sd1;
+        x.j.reg = q.reg[s][0];
+        x.j.nunits = 1;
+        x.j.nuv = s ? sh.nuv2 : sh.nuv1;
+        x.j.o.C = q.C[s][0];
+        x.j.o.lo = q.tbl[s][0];
+        x.j.o.hi = q.tbh[s][0];
+        if (s32) {
+          x.j.o.C32 = q.c32[s];
+          x.j.o.lo32 = q.tb32l[s];
+          x.j.o.hi32 = q.tb32h[s];
+        }
+        x.j.o.absmax = dbl ? q.amax : nullptr;
+        jobs.push_back(x);
+      }
+    ec_brec<T>(p, jobs);
+  }
+  for (int round = 1; round <= sh.rounds; ++round) {
+    const int pp = (round - 1) % 2, nx = round % 2;
+    const int nuv1 = round == 1 ? sh.nuv1 : sh.nref, nuv2 = round == 1 ? sh.nuv2 : sh.nref;
+    const int w1 = nuv1 - 1, w2 = nuv2 - 1;
+    const int nslots = round == 1 ? 1 : k;
+    if (s32 && round == 1) {
+      std::vector<TopkQ<float>> tq;
+      for (const auto& q : qs) {
+        TopkQ<float> t{};
+        t.su1 = q.su[0][pp];
+        t.su2 = q.su[1][pp];
+        t.C1 = q.c32[0];
+        t.C2 = q.c32[1];
+        t.lo1 = q.tb32l[0];
+        t.hi1 = q.tb32h[0];
+        t.lo2 = q.tb32l[1];
+        t.hi2 = q.tb32h[1];
+        t.R1 = reinterpret_cast<const double4*>(q.C[0][pp]);
+        t.R2 = reinterpret_cast<const double4*>(q.C[1][pp]);
+        t.absmax = q.amax;
+        t.list = q.list; t.small = q.small; t.cand = q.cand; t.topk = q.topk; t.st = q.st; t.hist = q.hist;
+        t.range = q.range; t.work = q.seedwork; t.best = q.best;
+        t.sorted = q.sorted[round - 1];
+        t.last = q.lastrec + (round - 1);
+        t.fail = q.fail;
+        tq.push_back(t);
+      }
+      topk_brec<float>(p, sh, tq, nslots, w1, w2, true, false);
+    } else {
+      std::vector<TopkQ<T>> tq;
+      for (const auto& q : qs) {
+        TopkQ<T> t{};
+        t.su1 = q.su[0][pp];
+        t.su2 = q.su[1][pp];
+        t.C1 = q.C[0][pp];
+        t.C2 = q.C[1][pp];
+        t.lo1 = q.tbl[0][pp];
+        t.hi1 = q.tbh[0][pp];
+        t.lo2 = q.tbl[1][pp];
+        t.hi2 = q.tbh[1][pp];
+        t.exp_absmax = dbl ? q.amax : nullptr;
+        t.list = q.list; t.small = q.small; t.cand = q.cand; t.topk = q.topk; t.st = q.st; t.hist = q.hist;
+        t.range = q.range; t.work = q.seedwork; t.best = q.best;
+        t.sorted = q.sorted[round - 1];
+        t.last = q.lastrec + (round - 1);
+        t.fail = q.fail;
+        tq.push_back(t);
+      }
+      topk_brec<T>(p, sh, tq, nslots, w1, w2, false, dbl);
+    }
+    if (round == sh.rounds) break;
+    // refinement units of the next round (launch_dist_dedup's sequence)
+    {
+      using FT = std::integral_constant<decltype(&k_dedup_mark_body), &k_dedup_mark_body>;
+      std::vector<typename BodyArgs<FT::value>::type> a;
+      for (const auto& q : qs)
+        a.push_back(bargs<FT::value>(q.fail, (const Cand*)(q.sorted[round - 1]), k, w1 * w1, w2 * w2, q.fp1, q.fp2));
+      brec<256, FT::value>(p, blocks_for(k, 256), 256, a);
+    }
+    {
+      using FT = std::integral_constant<decltype(&k_dedup_units_body), &k_dedup_units_body>;
+      std::vector<typename BodyArgs<FT::value>::type> a;
+      for (const auto& q : qs)
+        a.push_back(bargs<FT::value>(q.fail, (const Cand*)(q.sorted[round - 1]), k, w1 * w1, w2 * w2,
+                             (const int*)q.fp1, (const int*)q.fp2, q.su[0][nx], q.su[1][nx], q.dwork,
+                             q.nunits + 2 * (round - 1), dbl ? q.amax : (unsigned long long*)nullptr,
+                             dbl ? kAmaxSlots : 0));
+      brec<1024, FT::value>(p, 1, 1024, a);
+    }
+    {
+      using FT = std::integral_constant<decltype(&k_dedup_apply_body<T>), &k_dedup_apply_body<T>>;
+      std::vector<typename BodyArgs<FT::value>::type> a;
+      for (const auto& q : qs)
+        a.push_back(bargs<FT::value>(q.fail, (const Cand*)(q.sorted[round - 1]), k, w1 * w1, w2 * w2,
+                             (const int*)q.su[0][pp], (const int*)q.su[1][pp], (const V*)q.reg[0][pp],
+                             (const V*)q.reg[1][pp], nuv1, nuv2, (const unsigned char*)q.dwork,
+                             (const int*)q.su[0][nx], (const int*)q.su[1][nx], q.reg[0][nx], q.reg[1][nx], q.fp1,
+                             q.fp2));
+      brec<256, FT::value>(p, blocks_for(k, 256), 256, a);
+    }
+    {
+      std::vector<EcBJob<T>> jobs;
+      for (int s = 0; s < 2; ++s)
+        for (const auto& q : qs) {
+          EcBJob<T> x;
+          x.fail = q.fail;
+          x.j.sd = s ? q.sd2 : q.sd1;
+          x.j.reg = q.reg[s][nx];
+          x.j.nunits = k;
+          x.j.nuv = sh.nref;
+          x.j.o.C = q.C[s][nx];
+          x.j.o.lo = q.tbl[s][nx];
+          x.j.o.hi = q.tbh[s][nx];
+          x.j.o.absmax = dbl ? q.amax : nullptr;
+          x.j.o.nunits_dev = q.nunits + 2 * (round - 1) + s;
+          jobs.push_back(x);
+        }
+      ec_brec<T>(p, jobs);
+    }
+  }
+  return JSI_OK;
+}
+template int record_dist_batch<float>(BatchProgram&, const DistBatchShape&, const std::vector<DistBatchQ<float>>&);
+template int record_dist_batch<double>(BatchProgram&, const DistBatchShape&, const std::vector<DistBatchQ<double>>&);
 
 }  // namespace jsi

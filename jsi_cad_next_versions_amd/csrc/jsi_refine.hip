@@ -345,7 +345,7 @@ __device__ inline void cell_centre(const vec4_t<TR>& R, int nuv, int cell, doubl
 constexpr int kPolishKnots = 256;  // knots per vector staged in LDS
 
 template <class TR, int MAXP, bool FAST>
-__global__ __launch_bounds__(64) void k_polish(SurfDev<double> s1g, SurfDev<double> s2g, const Cand* rec, int m,
+__device__ __forceinline__ void k_polish_body(SurfDev<double> s1g, SurfDev<double> s2g, const Cand* rec, int m,
                                                const int* su1, const int* su2, const vec4_t<TR>* reg1,
                                                const vec4_t<TR>* reg2, int nuv1, int nuv2, double4 box1,
                                                double4 box2, PolishRec* out) {
@@ -501,6 +501,13 @@ __global__ __launch_bounds__(64) void k_polish(SurfDev<double> s1g, SurfDev<doub
   o.iters = it;
   o.flags = flags;
   out[e] = o;
+}
+template <class TR, int MAXP, bool FAST>
+__global__ __launch_bounds__(64) void k_polish(SurfDev<double> s1g, SurfDev<double> s2g, const Cand* rec, int m,
+                                               const int* su1, const int* su2, const vec4_t<TR>* reg1,
+                                               const vec4_t<TR>* reg2, int nuv1, int nuv2, double4 box1,
+                                               double4 box2, PolishRec* out) {
+  k_polish_body<TR, MAXP, FAST>(s1g, s2g, rec, m, su1, su2, reg1, reg2, nuv1, nuv2, box1, box2, out);
 }
 
 template <class TR, int MAXP, bool FAST>
@@ -1125,4 +1132,74 @@ void assemble_curves(const std::vector<MarchSeed>& seeds, const std::vector<Marc
   }
   *ncurves = cid;
 }
+
+// ---------------------------------------------------------------------------
+// Polish of a batch of distance queries (jsi_internal.hpp): one launch per
+// kernel variant present, blockIdx.y = query, each query's records polished
+// as alone.
+// ---------------------------------------------------------------------------
+int polish_variant(const SurfDev<double>& s1, const SurfDev<double>& s2) { return variant(s1, s2); }
+
+template <class TR>
+struct PolishBArgs {
+  SurfDev<double> s1, s2;
+  const Cand* rec;
+  int m;
+  const int *su1, *su2;
+  const vec4_t<TR>*reg1, *reg2;
+  int nuv1, nuv2;
+  double4 box1, box2;
+  PolishRec* out;
+  const unsigned* fail;
+};
+
+template <class TR, int MAXP, bool FAST>
+__global__ __launch_bounds__(64) void k_polish_b(const PolishBArgs<TR>* args) {
+  const PolishBArgs<TR>& x = args[blockIdx.y];
+  if (x.fail && *x.fail) return;
+  if ((int)blockIdx.x * 64 >= x.m) return;
+  k_polish_body<TR, MAXP, FAST>(x.s1, x.s2, x.rec, x.m, x.su1, x.su2, x.reg1, x.reg2, x.nuv1, x.nuv2, x.box1, x.box2,
+                                x.out);
+}
+
+template <class T>
+int record_polish_batch(BatchProgram& p, const DistBatchShape& sh, const std::vector<DistBatchQ<T>>& qs) {
+  if (sh.polish <= 0 || qs.empty()) return JSI_OK;
+  const int m = std::min(sh.k, sh.polish);
+  const int last = (sh.rounds - 1) % 2;
+  const int nuv1 = sh.rounds == 1 ? sh.nuv1 : sh.nref, nuv2 = sh.rounds == 1 ? sh.nuv2 : sh.nref;
+  for (int v = 0; v < 4; ++v) {
+    std::vector<PolishBArgs<T>> a;
+    for (const auto& q : qs) {
+      if (variant(q.p1, q.p2) != v) continue;
+      PolishBArgs<T> x;
+      x.s1 = q.p1;
+      x.s2 = q.p2;
+      x.rec = q.sorted[sh.rounds - 1];
+      x.m = m;
+      x.su1 = q.su[0][last];
+      x.su2 = q.su[1][last];
+      x.reg1 = q.reg[0][last];
+      x.reg2 = q.reg[1][last];
+      x.nuv1 = nuv1;
+      x.nuv2 = nuv2;
+      x.box1 = q.box1;
+      x.box2 = q.box2;
+      x.out = q.pol;
+      x.fail = q.fail;
+      a.push_back(x);
+    }
+    if (a.empty()) continue;
+    const size_t off = p.put(a);
+    const void* f = v == 0   ? reinterpret_cast<const void*>(&k_polish_b<T, 3, true>)
+                    : v == 1 ? reinterpret_cast<const void*>(&k_polish_b<T, 3, false>)
+                    : v == 2 ? reinterpret_cast<const void*>(&k_polish_b<T, 7, false>)
+                             : reinterpret_cast<const void*>(&k_polish_b<T, kMaxDegP, false>);
+    p.add(f, dim3((unsigned)((m + 63) / 64), (unsigned)a.size()), dim3(64), off);
+  }
+  return JSI_OK;
+}
+template int record_polish_batch<float>(BatchProgram&, const DistBatchShape&, const std::vector<DistBatchQ<float>>&);
+template int record_polish_batch<double>(BatchProgram&, const DistBatchShape&, const std::vector<DistBatchQ<double>>&);
+
 }  // namespace jsi

@@ -97,25 +97,28 @@ int jsi_ctx::buf_raw(const char* name, size_t bytes, void** out, int fill_int) {
     if (b.p) {
       HCK(hipStreamSynchronize(stream));
       HCK(hipFree(b.p));
-      used -= b.bytes;
+      pool->release(b.bytes);
       b.p = nullptr;
       b.bytes = 0;
     }
     size_t want = std::max(bytes + bytes / 2, (size_t)4096);
-    if (used + want > cap) want = std::max(bytes, (size_t)256);
-    if (used + want > cap) {
-      set_err("device arena cap exceeded: buffer %s needs %zu bytes, %zu of %zu in use", name, want, used, cap);
-      return JSI_ERR_NOMEM;
+    if (!pool->reserve(want)) {
+      want = std::max(bytes, (size_t)256);
+      if (!pool->reserve(want)) {
+        set_err("device memory cap exceeded: buffer %s needs %zu bytes, %zu of %zu in use", name, want,
+                pool->used.load(), pool->cap);
+        return JSI_ERR_NOMEM;
+      }
     }
     hipError_t e = hipMalloc(&b.p, want);
     if (e != hipSuccess) {
       (void)hipGetLastError();
+      pool->release(want);
       set_err("hipMalloc(%zu) for %s failed: %s", want, name, hipGetErrorString(e));
       b.p = nullptr;
       return JSI_ERR_NOMEM;
     }
     b.bytes = want;
-    used += want;
     if (fill_int != 0) {
       hipError_t f = launch_fill_i32(static_cast<int*>(b.p), fill_int, want / sizeof(int), stream);
       if (f != hipSuccess) {
@@ -182,8 +185,11 @@ int jsi_ctx::surf_dev(int h, SurfDev<T>* out) {
   } else {
     if (!s.d_data64) {
       // promoted once, synchronously, so every stream sees the finished copy
+      if (!pool->reserve(sizeof(double) * (size_t)s.ndata)) {
+        set_err("device memory cap exceeded promoting surface %d to FP64", h);
+        return JSI_ERR_NOMEM;
+      }
       HCK(hipMalloc(&s.d_data64, sizeof(double) * (size_t)s.ndata));
-      used += sizeof(double) * (size_t)s.ndata;
       HCK(launch_promote<double>(s.d_data, s.d_data64, s.ndata, stream));
       HCK(hipStreamSynchronize(stream));
     }
@@ -1258,7 +1264,7 @@ int jsi_ctx_create(int device, size_t hbm_cap_bytes, jsi_ctx** out) {
   }
   size_t free_b = 0, total_b = 0;
   (void)hipMemGetInfo(&free_b, &total_b);
-  c->cap = hbm_cap_bytes ? hbm_cap_bytes : (free_b / 4) * 3;
+  c->pool->cap = hbm_cap_bytes ? hbm_cap_bytes : (free_b / 4) * 3;
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->ncu = prop.multiProcessorCount;
   if (c->ncu <= 0) c->ncu = 256;
@@ -1276,7 +1282,8 @@ int jsi_ctx_create_worker(jsi_ctx* parent, jsi_ctx** out) {
   c->debug_ = parent->debug_;
   c->worker = true;
   c->surfaces = parent->surfaces;
-  c->cap = parent->cap;
+  c->pool = parent->pool;  // one account for the parent and all its workers
+  c->root = parent->root ? parent->root : parent;
   c->ncu = parent->ncu;
   c->grid = parent->grid;
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
@@ -1293,8 +1300,12 @@ int jsi_ctx_destroy(jsi_ctx* c) {
   if (!c) return JSI_ERR_INVALID;
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
+  jsi_ctx_drop_batcher(c);  // its plans release their memory to the pool first
   for (auto& kv : c->bufs)
-    if (kv.second.p) (void)hipFree(kv.second.p);
+    if (kv.second.p) {
+      (void)hipFree(kv.second.p);
+      c->pool->release(kv.second.bytes);
+    }
   if (!c->worker) {
     std::lock_guard<std::mutex> lock(c->surfaces->mu);
     for (auto& s : c->surfaces->recs) {
@@ -1327,6 +1338,7 @@ int jsi_ctx_profile(jsi_ctx* c, int enable) {
 
 int jsi_ctx_kernel_stats(jsi_ctx* c, const char* name, int64_t* launches, double* ms, double* units) {
   if (!c || !name) return JSI_ERR_INVALID;
+  std::lock_guard<std::mutex> lock(c->stats_mu);
   auto it = c->stats.find(name);
   const KernelStat st = it == c->stats.end() ? KernelStat() : it->second;
   if (launches) *launches = st.launches;
@@ -1352,12 +1364,19 @@ int jsi_surface_upload(jsi_ctx* c, const int ivec[7], const float* data, int nda
   s.ndata = ndata;
   s.owned = true;
   const size_t bytes = sizeof(float) * (size_t)ndata;
-  if (c->used + bytes > c->cap) {
+  if (!c->pool->reserve(bytes)) {
     set_err("HBM cap reached uploading a surface");
     return JSI_ERR_NOMEM;
   }
-  HCK(hipMalloc(&s.d_data, bytes));
-  c->used += bytes;
+  {
+    hipError_t e = hipMalloc(&s.d_data, bytes);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      c->pool->release(bytes);
+      set_err("hipMalloc(%zu) for a surface failed: %s", bytes, hipGetErrorString(e));
+      return JSI_ERR_NOMEM;
+    }
+  }
   HCK(hipMemcpy(s.d_data, data, bytes, hipMemcpyHostToDevice));
   std::lock_guard<std::mutex> lock(c->surfaces->mu);
   c->surfaces->recs.push_back(s);
@@ -1395,11 +1414,11 @@ int jsi_surface_release(jsi_ctx* c, int h) {
   SurfaceRec& s = tab[h];
   if (s.owned && s.d_data) {
     HCK(hipFree(s.d_data));
-    c->used -= sizeof(float) * (size_t)s.ndata;
+    c->pool->release(sizeof(float) * (size_t)s.ndata);
   }
   if (s.d_data64) {
     HCK(hipFree(s.d_data64));
-    c->used -= sizeof(double) * (size_t)s.ndata;
+    c->pool->release(sizeof(double) * (size_t)s.ndata);
   }
   s = SurfaceRec();
   return JSI_OK;
@@ -1518,3 +1537,11 @@ int jsi_intersect_curves(jsi_ctx* c, int h1, const jsi_region* r1, int h2, const
 template int jsi_ctx::dist_topk<float>(int, const int*, const int*, int, int, const float4*, int, int, const float4*,
                                        int, int, Cand**, long long*, long long*, Rerank, const unsigned long long*,
                                        unsigned long long, TileBoxes<float4>, Cand*);
+
+// Used by the batched distance path (jsi_batch.hip).
+template int jsi_ctx::surf_dev<float>(int, SurfDev<float>*);
+template int jsi_ctx::surf_dev<double>(int, SurfDev<double>*);
+template int jsi_ctx::min_distance<float>(int, const jsi_region&, int, const jsi_region&, const jsi_params&,
+                                          jsi_result*, jsi_pair*, int);
+template int jsi_ctx::min_distance<double>(int, const jsi_region&, int, const jsi_region&, const jsi_params&,
+                                           jsi_result*, jsi_pair*, int);

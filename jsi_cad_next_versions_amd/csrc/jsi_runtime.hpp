@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -61,6 +62,28 @@ struct CurveReq {
   int ncurves = 0;
 };
 
+// Device bytes held by a context and every worker context made from it (one
+// counter for all of them), bounded by `cap`.
+struct HbmPool {
+  std::atomic<size_t> used{0};
+  size_t cap = 0;
+  bool reserve(size_t bytes) {
+    size_t u = used.load();
+    do {
+      if (u + bytes > cap) return false;
+    } while (!used.compare_exchange_weak(u, u + bytes));
+    return true;
+  }
+  void release(size_t bytes) { used.fetch_sub(bytes); }
+};
+
+namespace jsi {
+class DistBatcher;
+struct DistBatcherDelete {  // defined where DistBatcher is complete (jsi_batch.hip)
+  void operator()(DistBatcher* b) const;
+};
+}
+
 struct KernelStat {
   long long launches = 0;
   double ms = 0.0;     // summed launch durations (HIP events on the context stream)
@@ -73,12 +96,12 @@ struct jsi_ctx {
   hipEvent_t ev[2] = {nullptr, nullptr};
   std::map<std::string, KernelStat> stats;
   hipStream_t stream = nullptr;
-  size_t cap = 0;
-  size_t used = 0;
+  std::shared_ptr<HbmPool> pool = std::make_shared<HbmPool>();
   int ncu = 0;
   int grid = 0;  // persistent-kernel grid size
   std::shared_ptr<SurfaceTable> surfaces = std::make_shared<SurfaceTable>();
   bool worker = false;  // shares its parent's surfaces; destroy frees only its own buffers
+  std::unique_ptr<jsi::DistBatcher, jsi::DistBatcherDelete> batcher;  // batched distance queries, made on first use
   std::map<std::string, DevBuf> bufs;
   // Caller-provided scratch (the allocation-free distance entry point): while
   // `base` is set, buf_raw carves every named buffer of the query from
@@ -93,6 +116,17 @@ struct jsi_ctx {
   void* thresh_ptr_ = nullptr;  // device threshold of the screen in flight
   bool debug_ = false;          // JSI_DEBUG: per-screen counts on stderr
 
+  // Counters shared by a context and its workers (kept on the root context,
+  // read with jsi_ctx_kernel_stats): batched queries, batches, fallbacks.
+  jsi_ctx* root = nullptr;
+  std::mutex stats_mu;
+  void shared_stat(const char* name, double units) {
+    jsi_ctx* r = root ? root : this;
+    std::lock_guard<std::mutex> lock(r->stats_mu);
+    KernelStat& st = r->stats[name];
+    st.launches += 1;
+    st.units += units;
+  }
   // profiling: host-side time of a named step (launches = calls, ms = summed time)
   void host_stat(const char* name, double ms) {
     KernelStat& st = stats[name];
@@ -153,6 +187,17 @@ struct jsi_ctx {
 
 // Worker context sharing the parent's surfaces (batch runner / server).
 extern "C" int jsi_ctx_create_worker(jsi_ctx* parent, jsi_ctx** out);
+
+// Runs n queries on one context: distance queries of equal shape as batches
+// (jsi_batch.hip), the others one at a time.
+__attribute__((visibility("hidden"))) int jsi_ctx_run_distance_batch(jsi_ctx* c, int n, const jsi_query* const* qs,
+                                                                     jsi_result* const* res, int* status);
+// Shape key of a distance query that can run batched ("" = it runs alone).
+__attribute__((visibility("hidden"))) std::string jsi_ctx_batch_key(jsi_ctx* c, const jsi_query& q);
+// Frees the context's batch plans.
+__attribute__((visibility("hidden"))) void jsi_ctx_drop_batcher(jsi_ctx* c);
+// Device bytes one batched distance query holds.
+__attribute__((visibility("hidden"))) size_t jsi_batch_query_bytes(const jsi_query& q);
 
 // jsi_min_distance with every device buffer carved from `scratch` (device
 // memory of `bytes` bytes that the caller owns); allocates no device memory.

@@ -12,6 +12,7 @@
 #include <deque>
 #include <memory>
 #include <mutex>
+#include <string>
 #include <thread>
 #include <unordered_map>
 #include <vector>
@@ -28,17 +29,23 @@ struct Job {
   int status = JSI_OK;
   bool done = false;
   size_t est_bytes = 0;
+  std::string key;  // shape of a batchable distance query ("" = runs alone)
 };
 
-// Device bytes a query may hold while it runs (grid points, cells, tile
-// lists, the 64 MiB candidate buffer and, for intersections, pair lists).
-size_t estimate_bytes(const jsi_query& q) {
+// Device bytes a query may hold while it runs: a batched distance query its
+// share of the batch's buffers (cells of both ping-pong rounds, tile lists,
+// candidates, first-appearance tables); otherwise grid points, cells, tile
+// lists, the 64 MiB candidate buffer and, for intersections, pair lists.
+size_t estimate_bytes(const jsi_query& q, bool batched) {
+  if (batched) return jsi_batch_query_bytes(q);
   const size_t n1 = (size_t)q.r1.nuv * q.r1.nuv, n2 = (size_t)q.r2.nuv * q.r2.nuv;
   const size_t scal = q.params.precision == JSI_PREC_EXACT32 ? 4 : 8;
   size_t b = (n1 + n2) * scal * 3 * 2 + (n1 + n2) * scal * 8 + ((size_t)64 << 20);
   if (q.kind == JSI_QUERY_INTERSECT) b += (size_t)256 << 20;
   return b;
 }
+
+constexpr int kMaxBatch = 64;  // distance queries a worker takes in one batch
 
 }  // namespace
 
@@ -76,31 +83,58 @@ struct jsi_server {
     }
   }
 
+  // Continuous batching: a free worker takes the oldest request and, when it
+  // is a batchable distance query, every queued request of the same shape
+  // (up to kMaxBatch), so the requests that arrived while the workers were
+  // busy run as one batch on the next free worker.
   void worker_loop(int w) {
     jsi_ctx* c = workers[w];
     (void)hipSetDevice(c->device);
     while (true) {
-      std::shared_ptr<Job> job;
+      std::vector<std::shared_ptr<Job>> take;
       {
         std::unique_lock<std::mutex> lock(mu);
         cv_work.wait(lock, [&] { return stop || !queue.empty(); });
         if (stop && queue.empty()) return;
-        job = queue.front();
+        take.push_back(queue.front());
         queue.pop_front();
+        const std::string& key = take[0]->key;
+        if (!key.empty())
+          for (auto it = queue.begin(); it != queue.end() && (int)take.size() < kMaxBatch;) {
+            if ((*it)->key == key) {
+              take.push_back(*it);
+              it = queue.erase(it);
+            } else {
+              ++it;
+            }
+          }
       }
-      int rc;
-      if (job->q.kind == JSI_QUERY_DISTANCE)
-        rc = jsi_min_distance(c, job->q.h1, &job->q.r1, job->q.h2, &job->q.r2, &job->q.params, &job->res, nullptr,
-                              0);
-      else
-        rc = jsi_intersect(c, job->q.h1, &job->q.r1, job->q.h2, &job->q.r2, &job->q.params, &job->res, nullptr, 0);
+      if (take.size() == 1 && take[0]->key.empty()) {
+        Job* job = take[0].get();
+        job->status = job->q.kind == JSI_QUERY_DISTANCE
+                          ? jsi_min_distance(c, job->q.h1, &job->q.r1, job->q.h2, &job->q.r2, &job->q.params,
+                                             &job->res, nullptr, 0)
+                          : jsi_intersect(c, job->q.h1, &job->q.r1, job->q.h2, &job->q.r2, &job->q.params, &job->res,
+                                          nullptr, 0);
+      } else {
+        std::vector<const jsi_query*> qs;
+        std::vector<jsi_result*> rs;
+        std::vector<int> st(take.size(), JSI_OK);
+        for (auto& j : take) {
+          qs.push_back(&j->q);
+          rs.push_back(&j->res);
+        }
+        (void)jsi_ctx_run_distance_batch(c, (int)take.size(), qs.data(), rs.data(), st.data());
+        for (size_t i = 0; i < take.size(); ++i) take[i]->status = st[i];
+      }
       {
         std::lock_guard<std::mutex> lock(mu);
-        job->status = rc;
-        job->done = true;
-        inflight -= job->est_bytes;
-        touch_refs(job->q, -1);
-        ++n_completed;
+        for (auto& job : take) {
+          job->done = true;
+          inflight -= job->est_bytes;
+          touch_refs(job->q, -1);
+          ++n_completed;
+        }
       }
       cv_done.notify_all();
     }
@@ -114,7 +148,12 @@ int jsi_server_create(jsi_ctx* ctx, int slots, size_t hbm_cap_bytes, int max_pen
   if (!ctx || !out || slots < 1 || slots > 64) return JSI_ERR_INVALID;
   auto* s = new jsi_server();
   s->parent = ctx;
-  s->hbm_cap = hbm_cap_bytes;
+  // admission: the caller's cap, or what the context's device memory account
+  // has left when the server starts
+  {
+    const size_t used = ctx->pool->used.load(), cap = ctx->pool->cap;
+    s->hbm_cap = hbm_cap_bytes ? hbm_cap_bytes : (cap > used ? cap - used : 0);
+  }
   s->max_pending = max_pending > 0 ? max_pending : 1 << 20;
   s->cache_cap = cache_cap_bytes;
   for (int w = 0; w < slots; ++w) {
@@ -183,10 +222,13 @@ int jsi_server_submit(jsi_server* s, const jsi_query* q, int64_t* ticket) {
   auto job = std::make_shared<Job>();
   job->q = *q;
   std::memset(&job->res, 0, sizeof(job->res));
-  job->est_bytes = estimate_bytes(*q);
+  job->key = jsi_ctx_batch_key(s->parent, *q);
+  job->est_bytes = estimate_bytes(*q, !job->key.empty());
   {
     std::lock_guard<std::mutex> lock(s->mu);
-    if ((int)s->queue.size() >= s->max_pending || (s->hbm_cap && s->inflight + job->est_bytes > s->hbm_cap)) {
+    // a request larger than the whole cap is still admitted when nothing else runs
+    if ((int)s->queue.size() >= s->max_pending ||
+        (s->hbm_cap && s->inflight > 0 && s->inflight + job->est_bytes > s->hbm_cap)) {
       ++s->n_refused;
       return JSI_ERR_BUSY;
     }

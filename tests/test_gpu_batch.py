@@ -1,0 +1,147 @@
+"""Batched distance queries: queries of one shape run as one batch (every
+kernel launched once for the batch, captured as a HIP graph and replayed).
+Each batched result must equal the same query run alone on the single-query
+path: every round's ordered list (hashes), k-th distances, unit counts, the
+distance and the polished closest points.  A query that overflows a bounded
+buffer of the batch (here: fewer cell pairs than k) is rerun alone."""
+import os
+
+import numpy as np
+import pytest
+
+from jsi_cad_next_versions_amd import surfaces as S
+from jsi_cad_next_versions_amd._abi import QUERY_DISTANCE, default_params
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from jsi_cad_next_versions_amd.runtime import Context
+    c = Context(0)
+    yield c
+    c.close()
+
+
+def wavy(n, deg, seed, lift=0.0, periods=3.0):
+    k = S.clamped_uniform_knots(n, deg)
+    g = np.arange(n) / (n - 1)
+    X, Y = np.meshgrid(g, g)
+    rng = S.Rng(seed)
+    Z = 0.05 * np.sin(2 * np.pi * periods * X) * np.sin(2 * np.pi * periods * Y)
+    Z = Z + rng.normal(n * n, 0.002).reshape(n, n) + lift
+    w = rng.uniform(n * n, 0.8, 1.2).reshape(n, n)
+    return S.make_surface(deg, deg, k, k, np.stack([X, Y, Z], -1), w, f"wavy{n}_d{deg}_s{seed}")
+
+
+def same(a, b):
+    assert a.rounds == b.rounds and a.npairs == b.npairs and a.distance == b.distance
+    assert a.rounds_list() == b.rounds_list()
+    assert a.closest.distance == b.closest.distance and a.closest.candidate == b.closest.candidate
+    assert a.closest.polished == b.closest.polished
+
+
+def run_batched(ctx, metas):
+    """All queries through one worker, so the queued ones form batches."""
+    qs = [ctx.query(QUERY_DISTANCE, a, r1, b, r2, p) for a, b, r1, r2, p in metas]
+    res, st = ctx.run_batch(qs, concurrency=1)
+    assert st == [0] * len(qs)
+    return res
+
+
+def batch_stats(ctx):
+    nb, _, nq = ctx.kernel_stats("batches")
+    _, _, nf = ctx.kernel_stats("batch_fallbacks")
+    return nb, nq, nf
+
+
+def check(ctx, metas, fallbacks=0):
+    b0, q0, f0 = batch_stats(ctx)
+    res = run_batched(ctx, metas)
+    b1, q1, f1 = batch_stats(ctx)
+    # every query ran on the batched path (some fell back as stated; None:
+    # any number may)
+    assert q1 - q0 == len(metas) and b1 - b0 >= 1
+    assert fallbacks is None or f1 - f0 == fallbacks
+    for r, (a, b, r1, r2, p) in zip(res, metas):
+        same(r, ctx.minimum_distance(a, r1, b, r2, p)[0])
+
+
+@pytest.mark.parametrize("prec", ["fp32screen", "fp64", "exact32"])
+def test_different_surfaces_one_shape(ctx, prec):
+    """12 pairs of different nets (same degree and grid): one batch."""
+    metas = []
+    for i in range(12):
+        a, b = wavy(24 + i, 3, 100 + i), wavy(30 - i // 2, 3, 200 + i, lift=0.03)
+        metas.append((a, b, S.full_region(a, 96, 1), S.full_region(b, 80, 2),
+                      default_params(k=256, precision=prec, polish=16, trace=1)))
+    check(ctx, metas)
+
+
+def test_benchmark_query_batched(ctx):
+    """The benchmark query (hard-wave pair, nuv 512, k 4096, FP32SCREEN,
+    polish 64) 20 times in one batch."""
+    a, b = S.hard_wave_pair()
+    p = default_params(k=4096, precision="fp32screen", polish=64, trace=1)
+    metas = [(a, b, S.full_region(a, 512, 1), S.full_region(b, 512, 2), p)] * 20
+    check(ctx, metas)
+
+
+def test_degrees_and_refinement_grids(ctx):
+    """Degrees 2 and 5 with refinement grids 5 and 17: a batch per shape."""
+    m5, m17 = [], []
+    for i in range(4):
+        a, b = wavy(20, 2, 300 + i), wavy(14, 5, 400 + i, lift=0.02)
+        m5.append((a, b, S.full_region(a, 64, 1), S.full_region(b, 64, 2),
+                   default_params(k=128, precision="fp64", dist_refine_nuv=5, trace=1)))
+        m17.append((a, b, S.full_region(a, 48, 1), S.full_region(b, 48, 2),
+                    default_params(k=64, precision="fp32screen", dist_refine_nuv=17, trace=1, polish=8)))
+    check(ctx, m5)
+    check(ctx, m17)
+
+
+def test_param_tol_rounds(ctx):
+    """param_tol = 1e-9 from nuv 64: 7 rounds.  Deep rounds can leave more
+    records tied with the k-th key than the batch's compaction holds; such a
+    query is rerun alone, so only the results are checked."""
+    metas = []
+    for i in range(4):
+        a, b = wavy(20, 2, 300 + i), wavy(14, 5, 400 + i, lift=0.02)
+        metas.append((a, b, S.full_region(a, 64, 1), S.full_region(b, 64, 2),
+                      default_params(k=32, precision="fp64", param_tol=1e-9, trace=1)))
+    check(ctx, metas, fallbacks=None)
+
+
+def test_fallback_for_too_few_pairs(ctx):
+    """nuv 5 gives 16 x 16 round-1 pairs, fewer than k = 512: the batch marks
+    the query failed and it runs alone; its neighbours in the batch are not
+    affected."""
+    a, b = wavy(20, 3, 7), wavy(20, 3, 8, lift=0.02)
+    metas = [(a, b, S.full_region(a, 5, 1), S.full_region(b, 5, 2), default_params(k=512, precision="fp64", trace=1)),
+             (a, b, S.full_region(a, 5, 1), S.full_region(b, 5, 2), default_params(k=8, precision="fp64", trace=1))]
+    metas += [(a, b, S.full_region(a, 5, 1), S.full_region(b, 5, 2), default_params(k=512, precision="fp64", trace=1))] * 3
+    check(ctx, metas, fallbacks=4)
+
+
+def test_without_graphs(ctx):
+    """JSI_BATCH_GRAPH=0: the same launches issued one by one."""
+    os.environ["JSI_BATCH_GRAPH"] = "0"
+    try:
+        a, b = S.hard_wave_pair()
+        metas = [(a, b, S.full_region(a, 128, 1), S.full_region(b, 128, 2),
+                  default_params(k=512, precision="fp32screen", polish=16, trace=1))] * 5
+        check(ctx, metas)
+    finally:
+        del os.environ["JSI_BATCH_GRAPH"]
+
+
+def test_graph_replays_with_new_inputs(ctx):
+    """The same shape twice with different surfaces: the second batch replays
+    the first one's graph with a rewritten argument image."""
+    for rep in range(3):
+        metas = []
+        for i in range(6):
+            a, b = wavy(26, 3, 500 + 10 * rep + i), wavy(26, 3, 600 + 10 * rep + i, lift=0.025)
+            metas.append((a, b, S.full_region(a, 72, 1), S.full_region(b, 72, 2),
+                          default_params(k=200, precision="fp64", polish=4, trace=1)))
+        check(ctx, metas)

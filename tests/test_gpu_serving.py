@@ -73,7 +73,9 @@ def test_server_admission_refuses_beyond_cap(ctx):
     from jsi_cad_next_versions_amd.runtime import Server
     a, b = S.hard_wave_pair()
     q = ctx.query(QUERY_DISTANCE, a, S.full_region(a, 256, 1), b, S.full_region(b, 256, 2), default_params(k=64))
-    with Server(ctx, slots=1, hbm_cap_bytes=200 << 20) as srv:
+    # a cap below one query's estimate: the first is admitted (nothing in
+    # flight), the next ones are refused while it runs
+    with Server(ctx, slots=1, hbm_cap_bytes=1 << 20) as srv:
         t1 = srv.submit(q)
         assert t1 is not None
         refused = srv.submit(q) is None or srv.submit(q) is None
