@@ -107,31 +107,39 @@ def run_form(ctx, a, r1, b, r2, params, steps, warmup, profile=False):
     return res, wall, lat
 
 
-def run_served(ctx, q, n, slots, warmup):
-    """n copies of the query through the serving loop, closed loop: `slots`
-    requests stay in flight (a new one is admitted as each completes);
-    latency = submit -> result."""
+def submit_all(srv, queries):
+    """Submits the queries in order (while admission refuses one, the oldest
+    result is collected first) and waits for all.  Returns (last result, wall
+    seconds, statuses)."""
     from collections import deque
+    t0 = time.perf_counter()
+    pending, res, ok = deque(), None, []
+    for q in queries:
+        t = srv.submit(q)
+        while t is None:
+            res = srv.poll(pending.popleft())
+            ok.append(res is not None)
+            t = srv.submit(q)
+        pending.append(t)
+    while pending:
+        res = srv.poll(pending.popleft())
+        ok.append(res is not None)
+    return res, time.perf_counter() - t0, ok
 
-    from jsi_cad_next_versions_amd.runtime import Server
-    with Server(ctx, slots=slots) as srv:
-        for t in [srv.submit(q) for _ in range(warmup * slots)]:
-            srv.poll(t)
-        t0 = time.perf_counter()
-        inflight = deque()
-        lat, res, sent = [], None, 0
-        while sent < n and len(inflight) < slots:
-            inflight.append((srv.submit(q), time.perf_counter()))
-            sent += 1
-        while inflight:
-            t, ts = inflight.popleft()
-            res = srv.poll(t)
-            lat.append((time.perf_counter() - ts) * 1e3)
-            if sent < n:
-                inflight.append((srv.submit(q), time.perf_counter()))
-                sent += 1
-        wall = time.perf_counter() - t0
-    return res, wall, lat
+
+def warm_server(srv, queries, slots, rounds=2):
+    """Bursts of queued requests so every worker builds its batch plans (64-,
+    16- and 4-query batches) before anything is timed."""
+    for burst in [64 * slots] * rounds + [16 * slots, 4 * slots, slots]:
+        submit_all(srv, [queries[i % len(queries)] for i in range(burst)])
+
+
+def run_served(srv, q, n):
+    """n copies of the query submitted together; the workers take them in
+    batches as they free up (continuous batching).  Returns (last result,
+    wall seconds)."""
+    res, wall, _ = submit_all(srv, [q] * n)
+    return res, wall
 
 
 def uneven_queries(ctx, n, k, polish, seed0=0):
@@ -152,42 +160,37 @@ def uneven_queries(ctx, n, k, polish, seed0=0):
     return qs, cells
 
 
-def run_poisson(ctx, queries, rate, seconds, slots, seed=5):
-    """Open loop: request i arrives at t_i (exponential gaps at `rate`), is
-    submitted when due (re-tried while admission refuses it) and its latency
-    is result time - arrival time."""
+def run_poisson(srv, queries, rate, seconds, seed=5):
+    """Open loop on a warm server: request i arrives at t_i (exponential gaps
+    at `rate`), is submitted when due (again while admission refuses it) and
+    its latency is result time - arrival time."""
     from collections import deque
-
-    from jsi_cad_next_versions_amd.runtime import Server
     rng = np.random.default_rng(seed)
     n = max(1, int(rate * seconds))
     arrivals = np.cumsum(rng.exponential(1.0 / rate, n))
     lat, pending, waiting = [], {}, deque()
     refused = 0
-    with Server(ctx, slots=slots) as srv:
-        for t in [srv.submit(q) for q in queries[:slots]]:  # warm-up: worker buffers
-            srv.poll(t)
-        i = 0
-        t0 = time.perf_counter()
-        while i < n or pending or waiting:
-            now = time.perf_counter() - t0
-            while i < n and arrivals[i] <= now:
-                waiting.append((queries[i % len(queries)], arrivals[i]))
-                i += 1
-            while waiting:
-                tk = srv.submit(waiting[0][0])
-                if tk is None:
-                    refused += 1
-                    break
-                pending[tk] = waiting.popleft()[1]
-            for tk in list(pending):
-                if srv.poll(tk, wait=False) is not None:
-                    lat.append((time.perf_counter() - t0 - pending.pop(tk)) * 1e3)
-            time.sleep(5e-5)
-        wall = time.perf_counter() - t0
-    return {"offered_rate": rate, "requests": n, "queries_per_s": n / wall, "p50_ms": pct(lat, 50),
-            "p99_ms": pct(lat, 99), "max_ms": max(lat) if lat else None, "admission_retries": refused,
-            "slots": slots}
+    i = 0
+    t0 = time.perf_counter()
+    while i < n or pending or waiting:
+        now = time.perf_counter() - t0
+        while i < n and arrivals[i] <= now:
+            waiting.append((queries[i % len(queries)], arrivals[i]))
+            i += 1
+        while waiting:
+            tk = srv.submit(waiting[0][0])
+            if tk is None:
+                refused += 1
+                break
+            pending[tk] = waiting.popleft()[1]
+        for tk in list(pending):
+            if srv.poll(tk, wait=False) is not None:
+                lat.append((time.perf_counter() - t0 - pending.pop(tk)) * 1e3)
+        time.sleep(5e-5)
+    wall = time.perf_counter() - t0
+    return {"offered_rate": rate, "requests": n, "seconds": float(arrivals[-1]), "queries_per_s": n / wall,
+            "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99), "max_ms": max(lat) if lat else None,
+            "admission_retries": refused}
 
 
 def cpu_baseline(a, r1, b, r2, k, queries, polish):
@@ -214,8 +217,9 @@ def cpu_baseline(a, r1, b, r2, k, queries, polish):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--steps", type=int, default=2048, help="queries of the timed served run")
     ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--alone-steps", type=int, default=50, help="queries of each one-at-a-time form")
     ap.add_argument("--nuv", type=int, default=512)
     ap.add_argument("--k", type=int, default=4096)
     ap.add_argument("--precision", default="fp32screen")
@@ -225,14 +229,18 @@ def main():
     ap.add_argument("--polish", type=int, default=64, help="top-k records polished to FP64 closest points")
     ap.add_argument("--isect-steps", type=int, default=5, help="intersection-curve queries (saddle / wave plane)")
     ap.add_argument("--eval-steps", type=int, default=5, help="grid evaluations (evalWithCuda, nuv=1024) per dtype")
-    ap.add_argument("--batch", type=int, default=256, help="queries of the uneven batch (0 = skip)")
-    ap.add_argument("--serve-rates", default="500,800", help="Poisson arrival rates of the serving runs (req/s)")
-    ap.add_argument("--serve-seconds", type=float, default=2.0, help="arrival window of the serving run (0 = skip)")
-    ap.add_argument("--concurrency", type=int, default=16,
-                    help="worker streams of the served (batched) form (16: the GPU is saturated)")
+    ap.add_argument("--batch", type=int, default=4096, help="queries of the uneven batch (0 = skip)")
+    ap.add_argument("--latency-rate", type=float, default=3000.0,
+                    help="offered rate (req/s) of the open-loop run whose p50/p99 head the JSON line")
+    ap.add_argument("--rates", default="1000,2000,3000,4000",
+                    help="Poisson arrival rates (req/s) of the open-loop runs of the benchmark query")
+    ap.add_argument("--rate-seconds", type=float, default=1.0, help="arrival window of each open-loop run")
+    ap.add_argument("--serve-rates", default="500,800", help="Poisson arrival rates of the uneven-mix runs (req/s)")
+    ap.add_argument("--serve-seconds", type=float, default=2.0, help="arrival window of the mix runs (0 = skip)")
+    ap.add_argument("--concurrency", type=int, default=4,
+                    help="server workers of the served form (each takes batches of up to 64 queued queries)")
     ap.add_argument("--mix-slots", type=int, default=8,
-                    help="worker streams of the uneven batch and the Poisson serving runs (the 4x-scaled pairs "
-                         "of the mix hold more device memory each; 8 serves them faster than 16)")
+                    help="server workers of the uneven batch and the mixed serving runs")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="also run plain device copies of this many MiB (calibrates the traffic counters)")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
@@ -264,35 +272,48 @@ def main():
     # the same form one query at a time without profiling (profiling waits on
     # each timed kernel), then profiled: the profiled run's HIP events give the
     # per-kernel times
-    ures, uwall, ulat = run_form(ctx, a, r1, b, r2, p, args.steps, args.warmup)
-    ares, awall, alat = run_form(ctx, a, r1, b, r2, p, args.steps, 0, profile=True)
+    ures, uwall, ulat = run_form(ctx, a, r1, b, r2, p, args.alone_steps, args.warmup)
+    ares, awall, alat = run_form(ctx, a, r1, b, r2, p, args.alone_steps, 0, profile=True)
     stats = {name: ctx.kernel_stats(name) for name in FILTERS}
     stats = {n: v for n, v in stats.items() if v[0] > 0}
     ctx.profile(False)
-    # served / batched: the same queries through the serving loop, several in flight
-    res, wall, lat = run_served(ctx, ctx.query(0, a, r1, b, r2, p), args.steps, args.concurrency, 1)
-    qps = args.steps / wall
-    # the same with fewer streams: throughput against latency
-    served_by_streams = {}
-    for nst in (4, 8):
-        if nst < args.concurrency:
-            _, w2, l2 = run_served(ctx, ctx.query(0, a, r1, b, r2, p), args.steps, nst, 1)
-            served_by_streams[str(nst)] = {"queries_per_s": args.steps / w2, "p50_ms": pct(l2, 50),
-                                           "p99_ms": pct(l2, 99)}
     # drop-in EXACT32 mode (bit-exact with the reference, FP32 end to end)
     pe = default_params(k=args.k, precision="exact32", cull=1)
-    eres, ewall, elat = run_form(ctx, a, r1, b, r2, pe, max(1, args.steps // 2), 1)
-    # plain FP64 GPU path (the drop-in form the target is quoted against): FP64
-    # everywhere, every cell pair of every round evaluated, one query at a time
-    pp = default_params(k=args.k, precision="fp64", cull=0, polish=args.polish)
-    pres, pwall, plat = run_form(ctx, a, r1, b, r2, pp, args.plain_steps, 1)
-    pqps = args.plain_steps / pwall
-    # the same FP64 path with this build's conservative tile culling (identical results)
+    eres, ewall, elat = run_form(ctx, a, r1, b, r2, pe, max(1, args.alone_steps // 2), 1)
+    # the plain FP64 GPU path: the shipped FP64 per-query path (its default
+    # tile culling on, identical results), one query at a time
     cp = default_params(k=args.k, precision="fp64", cull=1, polish=args.polish)
     cres, cwall, clat = run_form(ctx, a, r1, b, r2, cp, args.plain_steps, 1)
     cqps = args.plain_steps / cwall
-    dist_err = abs(res.distance - pres.distance)
-    closest_err = abs(res.closest.distance - pres.closest.distance)
+    # the same with culling off: every cell pair of every round evaluated
+    pp = default_params(k=args.k, precision="fp64", cull=0, polish=args.polish)
+    pres, pwall, plat = run_form(ctx, a, r1, b, r2, pp, max(2, args.plain_steps // 4), 1)
+    pqps = max(2, args.plain_steps // 4) / pwall
+    # served: requests queue at a server whose workers take every queued
+    # request of the same shape as one batch (continuous batching)
+    from jsi_cad_next_versions_amd.runtime import Server
+    q = ctx.query(0, a, r1, b, r2, p)
+    with Server(ctx, slots=args.concurrency, max_pending=1 << 20) as srv:
+        warm_server(srv, [q], args.concurrency)
+        nb1, _, nq1 = ctx.kernel_stats("batches")
+        _, _, nl1 = ctx.kernel_stats("batch_launches")
+        res, wall = run_served(srv, q, args.steps)
+        nb2, _, nq2 = ctx.kernel_stats("batches")
+        _, _, nl2 = ctx.kernel_stats("batch_launches")
+        _, _, nf = ctx.kernel_stats("batch_fallbacks")
+        qps = args.steps / wall
+        # one request in flight at a time on the batched path (a batch of one
+        # replayed as a graph: one launch for the host, no round trips)
+        blat = []
+        for _ in range(args.alone_steps):
+            t1 = time.perf_counter()
+            srv.poll(srv.submit(q))
+            blat.append((time.perf_counter() - t1) * 1e3)
+        # open loop: Poisson arrivals of the same query at stated rates
+        open_loop = [run_poisson(srv, [q], float(r), args.rate_seconds) for r in args.rates.split(",") if r.strip()]
+    headline = min(open_loop, key=lambda r: abs(r["offered_rate"] - args.latency_rate))
+    dist_err = abs(res.distance - cres.distance)
+    closest_err = abs(res.closest.distance - cres.closest.distance)
     # intersection curves (run_intersect rounds + FP64 curve tracing): saddle /
     # wave-plane pair, nuv=512, 6 rounds
     isect = None
@@ -358,13 +379,26 @@ def main():
         "config": {"workload": f"min-distance, rational bicubic 64x64 hard-wave pair, nuv={args.nuv}, "
                                f"k={args.k}, refine nuv=16, 4 rounds, FP64 closest-point polish of "
                                f"{args.polish} records",
-                   "precision": args.precision, "cull": 1, "served_streams": args.concurrency},
-        "p50_ms": pct(lat, 50),
-        "p99_ms": pct(lat, 99),
-        "concurrency": args.concurrency,
-        "served_by_streams": served_by_streams,
-        "alone": {"queries_per_s": args.steps / uwall, "p50_ms": pct(ulat, 50), "p99_ms": pct(ulat, 99)},
-        "alone_profiled": {"queries_per_s": args.steps / awall, "p50_ms": pct(alat, 50), "p99_ms": pct(alat, 99)},
+                   "precision": args.precision, "cull": 1, "server_workers": args.concurrency,
+                   "max_batch": 64},
+        "p50_ms": headline["p50_ms"],
+        "p99_ms": headline["p99_ms"],
+        "latency_at_rate": headline["offered_rate"],
+        "latency_note": "p50/p99 of the open-loop Poisson run (arrival -> result) at the stated offered rate; "
+                        "every open-loop run is listed under open_loop",
+        "open_loop": open_loop,
+        "served": {"form": f"{args.steps} requests queued at once; {args.concurrency} server workers, each "
+                           "taking the queued requests of one shape as a batch of up to 64 (one launch per "
+                           "kernel for the batch, replayed as a HIP graph)",
+                   "queries_per_s": qps, "batches": nb2 - nb1, "mean_batch": (nq2 - nq1) / max(1, nb2 - nb1),
+                   "fallbacks": nf, "kernel_launches_per_batch": (nl2 - nl1) / max(1, nb2 - nb1),
+                   "kernel_launches_per_query": (nl2 - nl1) / max(1, nq2 - nq1)},
+        "alone_batched": {"form": "one request in flight, served as a batch of one (graph replay)",
+                          "queries_per_s": 1e3 / float(np.mean(blat)), "p50_ms": pct(blat, 50),
+                          "p99_ms": pct(blat, 99)},
+        "alone": {"queries_per_s": args.alone_steps / uwall, "p50_ms": pct(ulat, 50), "p99_ms": pct(ulat, 99)},
+        "alone_profiled": {"queries_per_s": args.alone_steps / awall, "p50_ms": pct(alat, 50),
+                           "p99_ms": pct(alat, 99)},
         "cells_per_s": cells * qps,
         "distance": res.distance,
         "dist_err_vs_fp64": dist_err,
@@ -379,7 +413,7 @@ def main():
                                     "at a time", "queries_per_s": pqps, "p50_ms": pct(plat, 50),
                             "p99_ms": pct(plat, 99), "distance": pres.distance},
         "speedup_vs_fp64_every_pair": qps / pqps if pqps > 0 else None,
-        "exact32": {"queries_per_s": max(1, args.steps // 2) / ewall, "p50_ms": pct(elat, 50),
+        "exact32": {"queries_per_s": max(1, args.alone_steps // 2) / ewall, "p50_ms": pct(elat, 50),
                     "p99_ms": pct(elat, 99), "distance": eres.distance},
         "roofline": {"bound": "mfma", "kernel": FILTERS[dom],
                      "achieved": achieved, "peak": PEAK[dt], "unit": "TFLOP/s",
@@ -396,15 +430,18 @@ def main():
     if args.batch > 0:
         # uneven batch: many surface pairs of different degree, net size and nuv in flight together
         bq, bcells = uneven_queries(ctx, args.batch, 512, args.polish)
-        ctx.run_batch(bq[: min(16, len(bq))], concurrency=args.mix_slots)  # warm-up
-        t1 = time.perf_counter()
-        bres, bst = ctx.run_batch(bq, concurrency=args.mix_slots)
-        bwall = time.perf_counter() - t1
+        with Server(ctx, slots=args.mix_slots, max_pending=1 << 20) as srv:
+            submit_all(srv, bq[: min(256, len(bq))])  # warm-up: plans of the common shapes
+            nb1, _, nq1 = ctx.kernel_stats("batches")
+            _, bwall, bok = submit_all(srv, bq)
+            bst = [0 if x else -1 for x in bok]
+            nb2, _, nq2 = ctx.kernel_stats("batches")
         out["batch_uneven"] = {
             "workload": f"{args.batch} pairs, 70% distance (FP32SCREEN, k=512, polish {args.polish}) / 30% "
                         "intersection (FP64, 6 rounds); degrees 2-5, nets 8-96, weights U[0.7,1.3], nuv 64-512",
             "queries_per_s": args.batch / bwall, "cells_per_s": bcells / bwall, "seconds": bwall,
-            "failed": int(sum(1 for x in bst if x != 0)), "concurrency": args.mix_slots}
+            "failed": int(sum(1 for x in bst if x != 0)), "server_workers": args.mix_slots,
+            "distance_batches": nb2 - nb1, "batched_distance_queries": nq2 - nq1}
     if args.serve_seconds > 0:
         # serving: Poisson arrivals of the uneven mix plus 10% 4x-scaled hard-wave pairs at nuv=1024
         sq, _ = uneven_queries(ctx, 90, 512, args.polish, seed0=10000)
@@ -414,11 +451,13 @@ def main():
         mix = []
         for i in range(100):
             mix.append(hq if i % 10 == 9 else sq[i - i // 10])
+        with Server(ctx, slots=args.mix_slots, max_pending=1 << 20) as srv:
+            warm_server(srv, mix, args.mix_slots, rounds=1)
+            runs = [run_poisson(srv, mix, float(r), args.serve_seconds) for r in args.serve_rates.split(",")
+                    if r.strip()]
         out["serving_poisson"] = {
             "workload": "uneven mix + 10% 4x-scaled hard-wave pairs (nuv=1024, k=4096), Poisson arrivals, "
-                        "latency = arrival -> result",
-            "runs": [run_poisson(ctx, mix, float(r), args.serve_seconds, args.mix_slots)
-                     for r in args.serve_rates.split(",") if r.strip()]}
+                        "latency = arrival -> result", "server_workers": args.mix_slots, "runs": runs}
     if not args.no_cpu:
         out["cpu_baseline"], odist = cpu_baseline(a, r1, b, r2, args.k, args.cpu_queries, args.polish)
         out["closest"]["oracle_fp64_distance"] = odist

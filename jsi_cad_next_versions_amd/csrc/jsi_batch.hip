@@ -22,6 +22,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <set>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -119,6 +120,7 @@ struct DistPlan {
   bool no_graph = false;           // capture failed: launch directly
   HbmPool* pool = nullptr;
   size_t reserved = 0;
+  uint64_t last = 0;               // LRU clock of the plan cache
 
   ~DistPlan() {
     if (exec) (void)hipGraphExecDestroy(exec);
@@ -186,13 +188,60 @@ static QLayout layout_of(const DistBatchShape& sh, size_t vsz) {
   return L;
 }
 
+// Batch plans shared by a context and all its workers: a worker checks a plan
+// out for one batch and returns it; idle plans beyond kIdleBudget bytes are
+// freed, least recently used first.
+class PlanCache {
+ public:
+  std::unique_ptr<DistPlan> checkout(const std::string& key) {
+    std::lock_guard<std::mutex> lock(mu_);
+    auto it = idle_.find(key);
+    if (it == idle_.end()) return nullptr;
+    std::unique_ptr<DistPlan> p = std::move(it->second);
+    idle_bytes_ -= p->dev_bytes;
+    idle_.erase(it);
+    return p;
+  }
+  void checkin(const std::string& key, std::unique_ptr<DistPlan> p) {
+    std::vector<std::unique_ptr<DistPlan>> drop;
+    {
+      std::lock_guard<std::mutex> lock(mu_);
+      seen_.insert(key);
+      p->last = ++clock_;
+      idle_bytes_ += p->dev_bytes;
+      idle_.emplace(key, std::move(p));
+      while (idle_bytes_ > kIdleBudget && idle_.size() > 1) {
+        auto victim = idle_.begin();
+        for (auto e = idle_.begin(); e != idle_.end(); ++e)
+          if (e->second->last < victim->second->last) victim = e;
+        idle_bytes_ -= victim->second->dev_bytes;
+        drop.push_back(std::move(victim->second));
+        idle_.erase(victim);
+      }
+    }
+    // freed outside the lock (hipFree synchronises the device)
+  }
+  bool seen(const std::string& key) {
+    std::lock_guard<std::mutex> lock(mu_);
+    return seen_.count(key) != 0;
+  }
+
+ private:
+  static constexpr size_t kIdleBudget = size_t(48) << 30;
+  std::mutex mu_;
+  std::multimap<std::string, std::unique_ptr<DistPlan>> idle_;
+  std::set<std::string> seen_;
+  size_t idle_bytes_ = 0;
+  uint64_t clock_ = 0;
+};
+
 class DistBatcher {
  public:
-  explicit DistBatcher(jsi_ctx* c) : c_(c) {
+  DistBatcher(jsi_ctx* c, std::shared_ptr<PlanCache> cache) : c_(c), cache_(std::move(cache)) {
     const char* g = std::getenv("JSI_BATCH_GRAPH");
     graphs_ = !(g && std::strcmp(g, "0") == 0);
   }
-  ~DistBatcher() { plans_.clear(); }
+  const std::shared_ptr<PlanCache>& cache() const { return cache_; }
 
   // Runs n distance queries of one class (batch_key equal); results and
   // statuses per query.
@@ -213,8 +262,8 @@ class DistBatcher {
   void fill_q(const DistPlan& pl, const QLayout& L, int q, DistBatchQ<T>* d);
 
   jsi_ctx* c_;
+  std::shared_ptr<PlanCache> cache_;
   bool graphs_ = true;
-  std::map<std::string, std::unique_ptr<DistPlan>> plans_;
 };
 
 std::string DistBatcher::key(const jsi_query& q) {
@@ -339,10 +388,23 @@ void DistBatcher::fill_q(const DistPlan& pl, const QLayout& L, int q, DistBatchQ
   d->fail = pl.fail + q;
 }
 
+// Device bytes a plan of one shape may hold; larger shapes run in smaller
+// batches (the 4x-scaled nuv 1024 query holds ~250 MB, most of it the
+// round-1 tile-pair list).
+constexpr size_t kPlanBudget = size_t(8) << 30;
+
 int DistBatcher::run(const jsi_query* const* qs, int n, jsi_result* const* res, int* status) {
   if (n <= 0) return JSI_OK;
-  if (qs[0]->params.precision == JSI_PREC_EXACT32) return run_t<float>(qs, n, res, status);
-  return run_t<double>(qs, n, res, status);
+  const size_t per = jsi_batch_query_bytes(*qs[0]);
+  const int cap = per * 64 <= kPlanBudget ? 64 : (per * 16 <= kPlanBudget ? 16 : 4);
+  if (n == 1 && !cache_->seen(key(*qs[0]) + " q4")) return JSI_ERR_UNSUPPORTED;  // alone: not worth a plan
+  for (int a = 0; a < n; a += cap) {
+    const int m = std::min(cap, n - a);
+    const int rc = qs[0]->params.precision == JSI_PREC_EXACT32 ? run_t<float>(qs + a, m, res + a, status + a)
+                                                               : run_t<double>(qs + a, m, res + a, status + a);
+    if (rc != JSI_OK) return rc;
+  }
+  return JSI_OK;
 }
 
 template <class T>
@@ -368,14 +430,21 @@ int DistBatcher::run_t(const jsi_query* const* qs, int n, jsi_result* const* res
   if (n > 64) return JSI_ERR_INVALID;
   sh.nq = nq;
   const std::string pk = key(q0) + " q" + std::to_string(nq);
-  auto it = plans_.find(pk);
-  if (it == plans_.end()) {
-    std::unique_ptr<DistPlan> pl;
-    const int rc = make_plan<T>(sh, nq, &pl);
+  std::unique_ptr<DistPlan> plp = cache_->checkout(pk);
+  if (!plp) {
+    const int rc = make_plan<T>(sh, nq, &plp);
     if (rc != JSI_OK) return rc;
-    it = plans_.emplace(pk, std::move(pl)).first;
   }
-  DistPlan& pl = *it->second;
+  // back to the cache when this batch is done, whatever happens
+  struct Return {
+    PlanCache* c;
+    const std::string& k;
+    std::unique_ptr<DistPlan>& p;
+    ~Return() {
+      if (p) c->checkin(k, std::move(p));
+    }
+  } ret{cache_.get(), pk, plp};
+  DistPlan& pl = *plp;
   const QLayout L = layout_of(sh, sizeof(vec4_t<T>));
   std::vector<DistBatchQ<T>> dq(nq);
   for (int q = 0; q < nq; ++q) {
@@ -496,6 +565,7 @@ int DistBatcher::run_t(const jsi_query* const* qs, int n, jsi_result* const* res
   const int* hnu = reinterpret_cast<const int*>(ho + o_nu);
   const PolishRec* hpol = reinterpret_cast<const PolishRec*>(ho + o_pol);
   const double ms = now_ms() - t0;
+  c_->shared_stat("batch_launches", (double)prog.launches.size());
   {
     int nfail = 0;
     for (int q = 0; q < n; ++q) {
@@ -583,18 +653,30 @@ int DistBatcher::run_t(const jsi_query* const* qs, int n, jsi_result* const* res
 
 void jsi::DistBatcherDelete::operator()(jsi::DistBatcher* b) const { delete b; }
 
+// The context's batcher; a worker's shares its root context's plan cache.
+static jsi::DistBatcher* batcher_of(jsi_ctx* c) {
+  static std::mutex mu;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!c->batcher) {
+    jsi_ctx* r = c->root ? c->root : c;
+    if (!r->batcher) r->batcher.reset(new jsi::DistBatcher(r, std::make_shared<jsi::PlanCache>()));
+    if (r != c) c->batcher.reset(new jsi::DistBatcher(c, r->batcher->cache()));
+  }
+  return c->batcher.get();
+}
+
 void jsi_ctx_drop_batcher(jsi_ctx* c) { c->batcher.reset(); }
 
 // Batched distance queries of one context: groups the eligible queries by
 // shape and runs each group (up to 64 queries at a time) as one batch; other
 // queries run one at a time.  Used by jsi_run_batch and the server.
 int jsi_ctx_run_distance_batch(jsi_ctx* c, int n, const jsi_query* const* qs, jsi_result* const* res, int* status) {
-  if (!c->batcher) c->batcher.reset(new jsi::DistBatcher(c));
+  jsi::DistBatcher* const bat = batcher_of(c);
   if (hipSetDevice(c->device) != hipSuccess) return JSI_ERR_DEVICE;
   std::map<std::string, std::vector<int>> groups;
   for (int i = 0; i < n; ++i) {
     if (jsi::DistBatcher::eligible(*qs[i])) {
-      const std::string k = c->batcher->key(*qs[i]);
+      const std::string k = bat->key(*qs[i]);
       if (!k.empty()) {
         groups[k].push_back(i);
         continue;
@@ -619,7 +701,7 @@ int jsi_ctx_run_distance_batch(jsi_ctx* c, int n, const jsi_query* const* qs, js
         bq[j] = qs[idx[a + j]];
         br[j] = res[idx[a + j]];
       }
-      const int rc = c->batcher->run(bq.data(), cnt, br.data(), bs.data());
+      const int rc = bat->run(bq.data(), cnt, br.data(), bs.data());
       for (int j = 0; j < cnt; ++j) {
         int st = rc == JSI_OK ? bs[j] : rc;
         if (rc != JSI_OK) {  // the batch could not run (e.g. memory): each query alone
@@ -638,8 +720,7 @@ int jsi_ctx_run_distance_batch(jsi_ctx* c, int n, const jsi_query* const* qs, js
 std::string jsi_ctx_batch_key(jsi_ctx* c, const jsi_query& q) {
   if (!jsi::DistBatcher::eligible(q)) return std::string();
   if (std::getenv("JSI_BATCH") && std::strcmp(std::getenv("JSI_BATCH"), "0") == 0) return std::string();
-  if (!c->batcher) c->batcher.reset(new jsi::DistBatcher(c));
-  return c->batcher->key(q);
+  return batcher_of(c)->key(q);
 }
 
 size_t jsi_batch_query_bytes(const jsi_query& q) {

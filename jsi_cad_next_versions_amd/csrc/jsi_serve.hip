@@ -217,28 +217,40 @@ int jsi_server_surface(jsi_server* s, uint64_t key, const int ivec[7], const flo
   return JSI_OK;
 }
 
-int jsi_server_submit(jsi_server* s, const jsi_query* q, int64_t* ticket) {
-  if (!s || !q || !ticket) return JSI_ERR_INVALID;
+static std::shared_ptr<Job> make_job(jsi_server* s, const jsi_query* q) {
   auto job = std::make_shared<Job>();
   job->q = *q;
   std::memset(&job->res, 0, sizeof(job->res));
   job->key = jsi_ctx_batch_key(s->parent, *q);
   job->est_bytes = estimate_bytes(*q, !job->key.empty());
+  return job;
+}
+
+// Admission and queueing (caller holds s->mu).
+static int enqueue_locked(jsi_server* s, const std::shared_ptr<Job>& job, int64_t* ticket) {
+  // a request larger than the whole cap is still admitted when nothing else runs
+  if ((int)s->queue.size() >= s->max_pending ||
+      (s->hbm_cap && s->inflight > 0 && s->inflight + job->est_bytes > s->hbm_cap)) {
+    ++s->n_refused;
+    return JSI_ERR_BUSY;
+  }
+  job->ticket = s->next_ticket++;
+  s->inflight += job->est_bytes;
+  s->touch_refs(job->q, +1);
+  s->jobs[job->ticket] = job;
+  s->queue.push_back(job);
+  ++s->n_admitted;
+  *ticket = job->ticket;
+  return JSI_OK;
+}
+
+int jsi_server_submit(jsi_server* s, const jsi_query* q, int64_t* ticket) {
+  if (!s || !q || !ticket) return JSI_ERR_INVALID;
+  auto job = make_job(s, q);
   {
     std::lock_guard<std::mutex> lock(s->mu);
-    // a request larger than the whole cap is still admitted when nothing else runs
-    if ((int)s->queue.size() >= s->max_pending ||
-        (s->hbm_cap && s->inflight > 0 && s->inflight + job->est_bytes > s->hbm_cap)) {
-      ++s->n_refused;
-      return JSI_ERR_BUSY;
-    }
-    job->ticket = s->next_ticket++;
-    s->inflight += job->est_bytes;
-    s->touch_refs(*q, +1);
-    s->jobs[job->ticket] = job;
-    s->queue.push_back(job);
-    ++s->n_admitted;
-    *ticket = job->ticket;
+    const int rc = enqueue_locked(s, job, ticket);
+    if (rc != JSI_OK) return rc;
   }
   s->cv_work.notify_one();
   return JSI_OK;
@@ -278,12 +290,20 @@ int jsi_run_batch(jsi_ctx* ctx, int n, const jsi_query* queries, jsi_result* res
   int rc = jsi_server_create(ctx, concurrency, 0, 0, 0, &s);
   if (rc != JSI_OK) return rc;
   std::vector<int64_t> tickets(n);
-  for (int i = 0; i < n; ++i) {
-    rc = jsi_server_submit(s, &queries[i], &tickets[i]);
-    if (rc != JSI_OK) {
-      jsi_server_destroy(s);
-      return rc;
-    }
+  {
+    // every query is queued before a worker takes any, so equal shapes batch
+    std::vector<std::shared_ptr<Job>> jobs;
+    for (int i = 0; i < n; ++i) jobs.push_back(make_job(s, &queries[i]));
+    std::lock_guard<std::mutex> lock(s->mu);
+    const size_t cap = s->hbm_cap;
+    s->hbm_cap = 0;  // a batch run admits everything it was given
+    for (int i = 0; i < n && rc == JSI_OK; ++i) rc = enqueue_locked(s, jobs[i], &tickets[i]);
+    s->hbm_cap = cap;
+  }
+  s->cv_work.notify_all();
+  if (rc != JSI_OK) {
+    jsi_server_destroy(s);
+    return rc;
   }
   int first = JSI_OK;
   for (int i = 0; i < n; ++i) {

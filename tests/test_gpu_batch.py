@@ -3,7 +3,8 @@ kernel launched once for the batch, captured as a HIP graph and replayed).
 Each batched result must equal the same query run alone on the single-query
 path: every round's ordered list (hashes), k-th distances, unit counts, the
 distance and the polished closest points.  A query that overflows a bounded
-buffer of the batch (here: fewer cell pairs than k) is rerun alone."""
+buffer of the batch (here: fewer cell pairs than k) is rerun alone.  A lone
+query of a shape never batched before runs alone (no plan is built for it)."""
 import os
 
 import numpy as np
@@ -120,6 +121,7 @@ def test_fallback_for_too_few_pairs(ctx):
     metas = [(a, b, S.full_region(a, 5, 1), S.full_region(b, 5, 2), default_params(k=512, precision="fp64", trace=1)),
              (a, b, S.full_region(a, 5, 1), S.full_region(b, 5, 2), default_params(k=8, precision="fp64", trace=1))]
     metas += [(a, b, S.full_region(a, 5, 1), S.full_region(b, 5, 2), default_params(k=512, precision="fp64", trace=1))] * 3
+    metas += metas[1:2]  # the k = 8 shape twice, so it forms a batch
     check(ctx, metas, fallbacks=4)
 
 
