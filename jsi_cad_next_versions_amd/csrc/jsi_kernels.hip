@@ -1608,6 +1608,307 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NQ < 4 ? 4 
   k_dist_filter_mfma_body<NQ>(list, list_count, nslots, su1, su2, w1, w2, C1, C2, thresh, out, out_count, cap, pairs_done, absmax);
 }
 
+// FP32 screen of the refinement rounds (FP32SCREEN) on the matrix cores: the
+// work split of k_dist_filter_mfma (same units, same records) with the
+// products formed by v_mfma_f32_16x16x4f32, twice the FP64 rate, and the
+// threshold folded into the product.  Each side is taken relative to an
+// origin of its own (o1: side-1 cell 0, o2: side-2 cell 0; any points would
+// do): a' = fl32(a - o1), b' = fl32(b - o2), c = fl64(o1 - o2), so that
+//   |a' - b' + c|^2 = |c|^2 + P(a') + s,  P(a') = 2 c.a' + |a'|^2,
+//   s = [-2b' q(b')] . [a' 1],  q(b') = |b'|^2 - 2 c.b':
+// only s and P, of the size of the units times |c| (not |c|^2), are rounded to
+// float.  With u = 2^-24, A, B = the slot's largest |a'|, |b'|,
+//   R = sqrt(T (1 + 1e-12)) (1 + 1e-14) + 1.01 u (A + B) + 1.01 u64 |c|,
+//   K = R^2 - |c|^2,  kd = fl32 rounded up (K + delta),
+//   delta = (8 u (2 A B + B^2 + 2 |c| B + 2 |c| A + A^2 + |K|)
+//            + 8 u64 (R^2 + |c|^2 + 2 |c| A + A^2)) (1 + 1e-6) + 1e-30,
+// the instruction forms D = s' + C with the accumulator C = fl32(fl32(P) - kd)
+// in every result register of the column, and a pair passes when D <= 0.
+// Every pair whose FP64 d2 is at most T (1 + 1e-13) passes: the roundings of
+// a - o1, b - o2 and o1 - o2 move |a' - b' + c| from |a - b| by at most
+// 1.01 u (|a'| + |b'|) + 1.01 u64 |c|, so |a' - b' + c| <= R and
+// s + P <= K; the float q(b'), P and C and the 5-term sum in any order and
+// rounding add at most 8 u (2 A B + B^2 + 2 |c| B + 2 |c| A + A^2 + |K| +
+// delta) (|C| <= |P| + |K| + delta), the FP64 parts a few u64 of their terms
+// (1e-30: flushed denormals).  "Some result <= 0" is read from the integer
+// minimum of the results' bits (negative floats are negative integers; a
+// NaN's sign is not used, so a NaN may send a product to the exact test).  A
+// passing pair's record holds its FP64 d2 from the FP64 centres (kept in
+// LDS), so the records are those of k_dist_filter_mfma except near-threshold
+// extras above T, which never reach the k smallest (at least k seed pairs
+// have d2 <= T).  Result register r of lane l of this instruction is
+// (row 4 (l/16) + r, column l%16).
+template <int NQ>
+__device__ __forceinline__ void k_dist_filter_mfma32_body(const uint64_t* list, const unsigned* list_count,
+                                                            int nslots, const int* su1, const int* su2, int w1,
+                                                            int w2, const double4* C1, const double4* C2,
+                                                            const double* thresh, Cand* out,
+                                                            unsigned long long* out_count, unsigned long long cap,
+                                                            unsigned long long* pairs_done) {
+  typedef float v4f __attribute__((ext_vector_type(4)));
+  constexpr int kStage = 128;
+  __shared__ float sB[17][64];  // side-1 B operands [16 k + column]; one padding block for the prefetch
+  __shared__ float sP[17 * 16];  // side-1 fl32(P(a'))
+  __shared__ float sA[16][64];  // side-2 A operands [16 k + row]
+  __shared__ float sMax[8];     // per wavefront: largest |a'| (0-3), |b'| (4-7)
+  __shared__ double sCa[3][256], sCb[3][256];  // FP64 centres (records and partial blocks)
+  __shared__ Cand stage[4][kStage];
+  unsigned long long done = 0;
+  const unsigned n = *list_count;
+  const double tv = *thresh;
+  const int lane = lane_id();
+  const int wv = threadIdx.x >> 6;
+  const int col = lane & 15, kq = lane >> 4;
+  constexpr double u = 5.9604644775390625e-08;    // 2^-24
+  constexpr double u64 = 1.1102230246251565e-16;  // 2^-53
+  const double sqt = sqrt(tv * (1.0 + 1e-12)) * (1.0 + 1e-14);
+  const int nA = w1 * w1, nB = w2 * w2;
+  const int nAf = nA >> 4, nBf = nB >> 4;  // full blocks of 16 cells (side-1: columns, side-2: rows)
+  const int nL = nBf - 4 * NQ;              // full side-2 blocks beyond the wavefronts' NQ each (0..3)
+  const double qnan = quiet_nan<double>();
+  const float qnanf = quiet_nan<float>();
+  int wcount = 0;
+  auto flush = [&]() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    unsigned long long base = 0;
+    if (lane == 0) base = atomicAdd(out_count, (unsigned long long)wcount);
+    base = __shfl(base, 0);
+    for (int e = lane; e < wcount; e += 64)
+      if (base + e < cap) out[base + e] = stage[wv][e];
+    __builtin_amdgcn_wave_barrier();
+    wcount = 0;
+  };
+  // the slot walk of k_dist_filter_mfma (slot ids two ahead, centres one ahead)
+  constexpr int kMaxWalk = 8;
+  const bool all = n == (unsigned)nslots;
+  struct WalkSlot {
+    int slot, u1, u2;  // slot -1: past the end of the walk
+  };
+  auto walk_slot = [&](int k) {
+    WalkSlot w{-1, 0, 0};
+    const unsigned i = blockIdx.x + (unsigned)k * gridDim.x;
+    if (k < kMaxWalk && i < n) {
+      w.slot = all ? (int)i : (int)(list[i] >> 32);
+      w.u1 = su1[w.slot];
+      w.u2 = su2[w.slot];
+    }
+    return w;
+  };
+  const int t = threadIdx.x;
+  auto load_a = [&](int v1) {
+    double4 a;
+    a.x = a.y = a.z = a.w = qnan;
+    if (t < nA) a = C1[(size_t)v1 * nA + t];
+    return a;
+  };
+  auto load_b = [&](int v2) {
+    double4 b;
+    b.x = b.y = b.z = b.w = qnan;
+    if (t < nB) b = C2[(size_t)v2 * nB + t];
+    return b;
+  };
+  // the slot's origins (one address per wavefront)
+  auto load_o1 = [&](int v1) { return C1[(size_t)v1 * nA]; };
+  auto load_o2 = [&](int v2) { return C2[(size_t)v2 * nB]; };
+  WalkSlot cur = walk_slot(0), nxt = walk_slot(1);
+  double4 a_cur = load_a(cur.u1), b_cur = load_b(cur.u2), o1_cur, o2_cur;
+  if (cur.slot >= 0) {
+    o1_cur = load_o1(cur.u1);
+    o2_cur = load_o2(cur.u2);
+  }
+#pragma unroll 1
+  for (int k = 0; cur.slot >= 0; ++k) {
+    const int slot = cur.slot;
+    double4 a_nxt, b_nxt, o1_nxt, o2_nxt;
+    if (nxt.slot >= 0) {
+      a_nxt = load_a(nxt.u1);
+      b_nxt = load_b(nxt.u2);
+      o1_nxt = load_o1(nxt.u1);
+      o2_nxt = load_o2(nxt.u2);
+    }
+    const WalkSlot nxt2 = walk_slot(k + 2);
+    __syncthreads();  // the previous slot's operands are no longer read
+    const double cx = o1_cur.x - o2_cur.x, cy = o1_cur.y - o2_cur.y, cz = o1_cur.z - o2_cur.z;
+    {
+      const double4 av = a_cur;
+      const bool in = t < nA;
+      const float ax = __double2float_rn(av.x - o1_cur.x), ay = __double2float_rn(av.y - o1_cur.y),
+                  az = __double2float_rn(av.z - o1_cur.z);
+      sB[t >> 4][t & 15] = ax;
+      sB[t >> 4][16 + (t & 15)] = ay;
+      sB[t >> 4][32 + (t & 15)] = az;
+      sB[t >> 4][48 + (t & 15)] = in ? 1.f : qnanf;
+      sCa[0][t] = av.x;
+      sCa[1][t] = av.y;
+      sCa[2][t] = av.z;
+      const double a2 = (double)ax * ax + (double)ay * ay + (double)az * az;
+      sP[t] = in ? __double2float_rn(2.0 * (cx * ax + cy * ay + cz * az) + a2) : infinity_v<float>();
+      float m = in ? __double2float_ru(sqrt(a2) * (1.0 + 1e-15)) : 0.f;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+      if (lane == 0) sMax[wv] = m;
+    }
+    {  // A operand column of side-2 cell t: lane (row t%16, k) of block t/16 holds [-2bx' -2by' -2bz' q(b')][k]
+      const double4 bv = b_cur;
+      const bool in = t < nB;
+      const float bx = __double2float_rn(bv.x - o2_cur.x), by = __double2float_rn(bv.y - o2_cur.y),
+                  bz = __double2float_rn(bv.z - o2_cur.z);
+      const double b2 = (double)bx * bx + (double)by * by + (double)bz * bz;
+      sA[t >> 4][t & 15] = -2.f * bx;
+      sA[t >> 4][16 + (t & 15)] = -2.f * by;
+      sA[t >> 4][32 + (t & 15)] = -2.f * bz;
+      sA[t >> 4][48 + (t & 15)] = __double2float_rn(b2 - 2.0 * (cx * bx + cy * by + cz * bz));
+      sCb[0][t] = bv.x;
+      sCb[1][t] = bv.y;
+      sCb[2][t] = bv.z;
+      float m = in ? __double2float_ru(sqrt(b2) * (1.0 + 1e-15)) : 0.f;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+      if (lane == 0) sMax[4 + wv] = m;
+    }
+    __syncthreads();
+    float kd;  // K + delta of the slot, rounded up to float
+    {
+      const double A = fmax(fmax((double)sMax[0], (double)sMax[1]), fmax((double)sMax[2], (double)sMax[3]));
+      const double B = fmax(fmax((double)sMax[4], (double)sMax[5]), fmax((double)sMax[6], (double)sMax[7]));
+      const double c2 = cx * cx + cy * cy + cz * cz, cn = sqrt(c2);
+      const double R = sqt + 1.01 * u * (A + B) + 1.01 * u64 * cn;
+      const double R2 = R * R, K = R2 - c2;
+      const double delta = (8.0 * u * (2.0 * A * B + B * B + 2.0 * cn * B + 2.0 * cn * A + A * A + fabs(K)) +
+                            8.0 * u64 * (R2 + c2 + 2.0 * cn * A + A * A)) *
+                               (1.0 + 1e-6) +
+                           1e-30;
+      kd = __double2float_ru(K + delta);
+    }
+    if (threadIdx.x == 0) done += (unsigned long long)nA * (unsigned long long)nB;
+    const uint64_t key_s = pack_idx((uint32_t)slot, 0u, 0u);
+    auto record = [&](int ci, int cj, const unsigned long long mk, bool hit) {
+      // FP64 d2 of cells ci, cj from the FP64 centres
+      double dq = 0.0;
+      if (hit) dq = pair_d2<double, false>(sCa[0][ci], sCa[1][ci], sCa[2][ci], sCb[0][cj], sCb[1][cj], sCb[2][cj]);
+      const int cnt = __popcll(mk);
+      if (wcount + cnt > kStage) flush();
+      if (hit) {
+        Cand c;
+        c.hi = ord_bits(dq);
+        c.lo = key_s | ((uint64_t)(uint32_t)ci << 20) | (uint64_t)(uint32_t)cj;
+        stage[wv][wcount + __popcll(mk & ((1ull << lane) - 1))] = c;
+      }
+      wcount += cnt;
+    };
+    // Appends the passing pairs of up to four products (product q: side-1
+    // block jb, side-2 block rq[q]; bit 4q + r of `mask` = result r of q).
+    auto append = [&](const v4f* d, int jb, const int* rq, int cnt) {
+      unsigned mask = 0;
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        if (q < cnt)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) mask |= (d[q][r] <= 0.f ? 1u : 0u) << (4 * q + r);
+      for (;;) {
+        const bool has = mask != 0u;
+        const unsigned long long mk = __ballot(has);
+        if (!mk) break;
+        int ci = 0, cj = 0;
+        if (has) {
+          const int bit = __builtin_ctz(mask);
+          mask &= mask - 1u;
+          const int q = bit >> 2;
+          const int rb = q == 0 ? rq[0] : q == 1 ? rq[1] : q == 2 ? rq[2] : rq[3];
+          ci = 16 * jb + col;                 // side-1 cell (column)
+          cj = 16 * rb + 4 * kq + (bit & 3);  // side-2 cell (row 4 (l/16) + r)
+        }
+        record(ci, cj, mk, has);
+      }
+    };
+    if (nBf > 0) {
+      float aop[NQ > 0 ? NQ : 1];
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) aop[q] = sA[wv + 4 * q][lane];
+      auto run = [&](auto extra_c, int jb0, int jb1, int rbx) {
+        // (NQ = 4 leaves no leftover side-2 blocks: that form never runs with EXTRA)
+        constexpr bool EXTRA = decltype(extra_c)::value && NQ < 4;
+        constexpr int NP = NQ + (EXTRA ? 1 : 0);  // products per side-1 block, <= 4
+        if (NP == 0 || jb0 >= jb1) return;
+        const float ax = EXTRA ? sA[rbx][lane] : 0.f;
+        float bop = sB[jb0][lane], pc = sP[16 * jb0 + col];
+        for (int jb = jb0; jb < jb1; ++jb) {
+          const float bop_n = sB[jb + 1][lane];  // next block (row 16 of sB, entries 256.. of sP: read, unused)
+          const float pc_n = sP[16 * (jb + 1) + col];
+          const float cc = pc - kd;  // the accumulator: -(threshold) of this lane's column
+          const v4f cv = {cc, cc, cc, cc};
+          v4f d[4];
+#pragma unroll
+          for (int q = 0; q < NQ; ++q) d[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(aop[q], bop, cv, 0, 0, 0);
+          if (EXTRA) d[NP - 1] = __builtin_amdgcn_mfma_f32_16x16x4f32(ax, bop, cv, 0, 0, 0);
+          int mn = INT_MAX;
+#pragma unroll
+          for (int q = 0; q < NP; ++q)
+            mn = min(mn, min(min(__float_as_int(d[q][0]), __float_as_int(d[q][1])),
+                             min(__float_as_int(d[q][2]), __float_as_int(d[q][3]))));
+          if (__ballot(mn <= 0) != 0ull) {  // rare
+            int rq[4] = {wv, wv + 4, wv + 8, wv + 12};
+            if (EXTRA) rq[NP - 1] = rbx;
+            append(d, jb, rq, NP);
+          }
+          bop = bop_n;
+          pc = pc_n;
+        }
+      };
+      using plain_t = std::integral_constant<bool, false>;
+      using extra_t = std::integral_constant<bool, true>;
+      const int nE = nL * nAf;
+      const int e0 = (nE * wv) >> 2, e1 = (nE * (wv + 1)) >> 2;
+      if (e1 <= e0) {
+        run(plain_t(), 0, nAf, 0);
+      } else {
+        const int i0 = e0 / nAf, j0 = e0 - i0 * nAf;
+        const int j1 = min(nAf, j0 + (e1 - e0));
+        const int r = (e1 - e0) - (j1 - j0);
+        run(extra_t(), 0, r, 4 * NQ + i0 + 1);
+        run(plain_t(), r, j0, 0);
+        run(extra_t(), j0, j1, 4 * NQ + i0);
+        run(plain_t(), j1, nAf, 0);
+      }
+    }
+    // pairs with a cell of a partial block: the reference test d2 <= T in FP64
+    {
+      const int rA = nA - 16 * nAf, rB = nB - 16 * nBf;
+      for (int e = 0; e < rA + rB; ++e) {
+        const bool s1p = e < rA;
+        const int ci = s1p ? 16 * nAf + e : t;
+        const int cj = s1p ? t : 16 * nBf + (e - rA);
+        const bool valid = s1p ? t < nB : t < 16 * nAf;
+        bool hit = false;
+        if (valid)
+          hit = pair_d2<double, false>(sCa[0][ci], sCa[1][ci], sCa[2][ci], sCb[0][cj], sCb[1][cj], sCb[2][cj]) <= tv;
+        const unsigned long long mk = __ballot(hit);
+        if (mk) record(ci, cj, mk, hit);
+      }
+    }
+    if (nxt.slot >= 0) {
+      a_cur = a_nxt;
+      b_cur = b_nxt;
+      o1_cur = o1_nxt;
+      o2_cur = o2_nxt;
+    }
+    cur = nxt;
+    nxt = nxt2;
+  }
+  if (wcount) flush();
+  if (pairs_done) add_pairs_done(pairs_done, done);
+}
+template <int NQ>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_dist_filter_mfma32(
+    const uint64_t* list, const unsigned* list_count, int nslots, const int* su1, const int* su2, int w1, int w2,
+    const double4* C1, const double4* C2, const double* thresh, Cand* out, unsigned long long* out_count,
+    unsigned long long cap, unsigned long long* pairs_done) {
+  k_dist_filter_mfma32_body<NQ>(list, list_count, nslots, su1, su2, w1, w2, C1, C2, thresh, out, out_count, cap,
+                                pairs_done);
+}
+
 // ---------------------------------------------------------------------------
 // FP32 screening with an FP64 result (FP32SCREEN).  Centres are evaluated in
 // FP64 and rounded to float for the screen; the screen's threshold is widened
@@ -2988,7 +3289,7 @@ hipError_t launch_dist_filter(const uint64_t* list, const unsigned* list_count, 
                               const int* su2, int w1, int w2, const vec4_t<T>* C1, const vec4_t<T>* C2,
                               const T* thresh, Cand* out, unsigned long long* out_count, unsigned long long cap,
                               int grid, hipStream_t s, unsigned long long* pairs_done, int fma, int cull,
-                              const unsigned long long* absmax) {
+                              const unsigned long long* absmax, int screen32) {
   if (list_cap == 0) return hipSuccess;
   const unsigned g = (unsigned)std::min<long long>(grid, (long long)list_cap);
   if (w1 <= kTile && w2 <= kTile) {  // every unit is one tile: one slot per list entry
@@ -3000,9 +3301,13 @@ hipError_t launch_dist_filter(const uint64_t* list, const unsigned* list_count, 
         // short to launch one per slot (at most 8 slots per workgroup: the grid grows for more slots)
         const unsigned gp = std::max(std::max(1u, (unsigned)grid / 2u), (gu + 7u) / 8u);
         const int nq = (w2 * w2 / 16) / 4;
-#define JSI_MFMA_FILTER(NQ)                                                                                       \
-  hipLaunchKernelGGL(k_dist_filter_mfma<NQ>, dim3(std::min(gu, gp)), dim3(256), 0, s, list, list_count, (int)gu, su1, \
-                     su2, w1, w2, C1, C2, thresh, out, out_count, cap, pairs_done, absmax)
+#define JSI_MFMA_FILTER(NQ)                                                                                         \
+  if (screen32)                                                                                                     \
+    hipLaunchKernelGGL(k_dist_filter_mfma32<NQ>, dim3(std::min(gu, gp)), dim3(256), 0, s, list, list_count, (int)gu, \
+                       su1, su2, w1, w2, C1, C2, thresh, out, out_count, cap, pairs_done);                          \
+  else                                                                                                              \
+    hipLaunchKernelGGL(k_dist_filter_mfma<NQ>, dim3(std::min(gu, gp)), dim3(256), 0, s, list, list_count, (int)gu,  \
+                       su1, su2, w1, w2, C1, C2, thresh, out, out_count, cap, pairs_done, absmax)
         switch (nq) {
           case 0: JSI_MFMA_FILTER(0); break;
           case 1: JSI_MFMA_FILTER(1); break;
@@ -3309,7 +3614,7 @@ hipError_t launch_fill_u32(unsigned* p, unsigned value, unsigned long long n, hi
   template hipError_t launch_dist_filter<T>(const uint64_t*, const unsigned*, unsigned, const int*, const int*,    \
                                             int, int, const vec4_t<T>*, const vec4_t<T>*, const T*, Cand*,         \
                                             unsigned long long*, unsigned long long, int, hipStream_t,             \
-                                            unsigned long long*, int, int, const unsigned long long*);             \
+                                            unsigned long long*, int, int, const unsigned long long*, int);        \
   template hipError_t launch_set_thresh<T>(T*, const Cand*, int, int, hipStream_t);                                \
   template hipError_t launch_eval_centres<T>(const EcJob<T>&, const EcJob<T>*, hipStream_t);                      \
   template hipError_t launch_points_tile<T>(const SurfDev<T>&, const vec4_t<T>*, int, int, int, int, T*,           \
@@ -3616,6 +3921,15 @@ static void topk_brec(BatchProgram& p, const DistBatchShape& sh, const std::vect
         // as the single-query launch: workgroups walk at most 8 slots each
         const unsigned gx = std::min(gu, std::max(1u, (gu + 7u) / 8u));
         const int nq = (w2 * w2 / 16) / 4;
+        auto mf32 = [&](auto fm) {
+          using FT = decltype(fm);
+          std::vector<typename BodyArgs<FT::value>::type> a;
+          for (const auto& q : tq)
+            a.push_back(bargs<FT::value>(q.fail, (const uint64_t*)q.list, (const unsigned*)q.small, (int)gu, q.su1,
+                                         q.su2, w1, w2, q.C1, q.C2, (const double*)thr(q), q.cand, out_count(q), cap,
+                                         (unsigned long long*)nullptr));
+          brec_wpe<256, 4, FT::value>(p, gx, 256, a);
+        };
         auto mf = [&](auto fm, auto wpe) {
           using FT = decltype(fm);
           constexpr int W = decltype(wpe)::value;
@@ -3626,9 +3940,12 @@ static void topk_brec(BatchProgram& p, const DistBatchShape& sh, const std::vect
                                  (unsigned long long*)nullptr, q.exp_absmax));
           brec_wpe<256, W, FT::value>(p, gx, 256, a);
         };
-#define JSI_MF(NQ)                                                                              \
-  mf(std::integral_constant<decltype(&k_dist_filter_mfma_body<NQ>), &k_dist_filter_mfma_body<NQ>>(), \
-     std::integral_constant<int, (NQ < 4 ? 4 : 3)>())
+#define JSI_MF(NQ)                                                                                         \
+  if (sh.prec == JSI_PREC_FP32SCREEN)                                                                          \
+    mf32(std::integral_constant<decltype(&k_dist_filter_mfma32_body<NQ>), &k_dist_filter_mfma32_body<NQ>>()); \
+  else                                                                                                         \
+    mf(std::integral_constant<decltype(&k_dist_filter_mfma_body<NQ>), &k_dist_filter_mfma_body<NQ>>(),        \
+       std::integral_constant<int, (NQ < 4 ? 4 : 3)>())
         switch (nq) {
           case 0: JSI_MF(0); break;
           case 1: JSI_MF(1); break;

@@ -267,7 +267,7 @@ template <class T>
 int jsi_ctx::dist_topk(int nslots, const int* su1, const int* su2, int w1, int nunits1, const vec4_t<T>* C1, int w2,
                         int nunits2, const vec4_t<T>* C2, int k, int cull, Cand** rec_out, long long* n_out,
                         long long* kk_out, Rerank rr, const unsigned long long* exp_absmax,
-                        unsigned long long cand_cap, TileBoxes<vec4_t<T>> tbx, Cand* last_out) {
+                        unsigned long long cand_cap, TileBoxes<vec4_t<T>> tbx, Cand* last_out, int screen32) {
   using V = vec4_t<T>;
   const bool rerank = rr.C1 != nullptr;
   const int fma = rerank ? 1 : 0;  // the FP32SCREEN screen may fuse; its margin covers the rounding
@@ -368,7 +368,7 @@ int jsi_ctx::dist_topk(int nslots, const int* su1, const int* su2, int w1, int n
       HCK(hipEventRecord(ev[0], stream));
     }
     HCK(launch_dist_filter<T>(list, list_count, (unsigned)ntp, su1, su2, w1, w2, C1, C2, thresh, cand, out_count, cap,
-                              grid, stream, pairs_done, fma, cull, exp_absmax));
+                              grid, stream, pairs_done, fma, cull, exp_absmax, screen32));
     if (prof) HCK(hipEventRecord(ev[1], stream));
     if (rerank) HCK(launch_rerank(cand, out_count, cap, su1, su2, w1 * w1, w2 * w2, rr.C1, rr.C2, stream));
     HCK(select_topk(cand, cap, k, maxkey, st, hist, small, small_count, small_cap, stream, 2, false, out_count));
@@ -651,8 +651,10 @@ int jsi_ctx::min_distance(int h1, const jsi_region& r1, int h2, const jsi_region
     long long n = 0;
     if constexpr (std::is_same<T, double>::value) {
       // FP32SCREEN screens round 1 (6.8e10 pairs at nuv=512) in float.  Refined
-      // cells shrink below float resolution of the coordinates, so later rounds
-      // (and a round whose float candidate set cannot be bounded) run in FP64.
+      // cells shrink below float resolution of the world coordinates, so the
+      // later rounds screen on the FP32 matrix cores relative to an origin per
+      // slot (screen32) with FP64 keys; a round 1 whose float candidate set
+      // cannot be bounded runs in FP64.
       int s32 = (P.precision == JSI_PREC_FP32SCREEN && round == 1) ? 1 : 0;
       if (s32) {
         Rerank rr;
@@ -669,7 +671,8 @@ int jsi_ctx::min_distance(int h1, const jsi_region& r1, int h2, const jsi_region
       }
       if (!s32) {
         RCK(dist_topk<T>(nslots, su1, su2, w1, nunits1, C1, w2, nunits2, C2, P.k, P.cull, &rec, &n, &kk, Rerank(),
-                         amax, (unsigned long long)P.cand_cap, tb, lastrec + (round - 1)));
+                         amax, (unsigned long long)P.cand_cap, tb, lastrec + (round - 1),
+                         P.precision == JSI_PREC_FP32SCREEN ? 1 : 0));
       }
     } else {
       RCK(dist_topk<T>(nslots, su1, su2, w1, nunits1, C1, w2, nunits2, C2, P.k, P.cull, &rec, &n, &kk, Rerank(),
@@ -1536,7 +1539,7 @@ int jsi_intersect_curves(jsi_ctx* c, int h1, const jsi_region* r1, int h2, const
 // used by the operator ABI (jsi_opabi.hip)
 template int jsi_ctx::dist_topk<float>(int, const int*, const int*, int, int, const float4*, int, int, const float4*,
                                        int, int, Cand**, long long*, long long*, Rerank, const unsigned long long*,
-                                       unsigned long long, TileBoxes<float4>, Cand*);
+                                       unsigned long long, TileBoxes<float4>, Cand*, int);
 
 // Used by the batched distance path (jsi_batch.hip).
 template int jsi_ctx::surf_dev<float>(int, SurfDev<float>*);

@@ -162,7 +162,7 @@ struct jsi_ctx {
                 int nunits2, const jsi::vec4_t<T>* C2, int k, int cull, jsi::Cand** rec_out, long long* n_out,
                 long long* kk_out, Rerank rr = Rerank(), const unsigned long long* exp_absmax = nullptr,
                 unsigned long long cand_cap = 0, TileBoxes<jsi::vec4_t<T>> tbx = TileBoxes<jsi::vec4_t<T>>(),
-                jsi::Cand* last_out = nullptr);
+                jsi::Cand* last_out = nullptr, int screen32 = 0);
   template <class T>
   int min_distance(int h1, const jsi_region& r1, int h2, const jsi_region& r2, const jsi_params& P,
                    jsi_result* res, jsi_pair* out, int cap_pairs);
