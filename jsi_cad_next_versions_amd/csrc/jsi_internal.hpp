@@ -151,11 +151,14 @@ hipError_t launch_dist_filter(const uint64_t* list, const unsigned* list_count, 
                               const int* su2, int w1, int w2, const vec4_t<T>* C1, const vec4_t<T>* C2,
                               const T* thresh, Cand* out, unsigned long long* out_count, unsigned long long cap,
                               int grid, hipStream_t s, unsigned long long* pairs_done = nullptr, int fma = 0,
-                              int cull = 1, const unsigned long long* absmax = nullptr, int screen32 = 0);
+                              int cull = 1, const unsigned long long* absmax = nullptr, int screen32 = 0,
+                              const vec4_t<T>* blo1 = nullptr, const vec4_t<T>* bhi1 = nullptr,
+                              const vec4_t<T>* blo2 = nullptr, const vec4_t<T>* bhi2 = nullptr);
 // absmax (double only): the expanded-form test (three FMAs per pair with a
 // rounding margin from the largest |centre coordinate|, exact keys).
 // screen32 (with absmax, units of one tile): the FP32 matrix-core screen of
-// the refinement rounds (FP32SCREEN), same records.
+// the refinement rounds (FP32SCREEN), same records; blo1 .. bhi2: the units'
+// centre boxes (one tile per unit).
 template <class T>
 hipError_t launch_set_thresh(T* thresh, const Cand* rec, int idx, int use_inf, hipStream_t s);
 

@@ -1611,25 +1611,30 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NQ < 4 ? 4 
 // FP32 screen of the refinement rounds (FP32SCREEN) on the matrix cores: the
 // work split of k_dist_filter_mfma (same units, same records) with the
 // products formed by v_mfma_f32_16x16x4f32, twice the FP64 rate, and the
-// threshold folded into the product.  Each side is taken relative to an
+// whole pass test folded into the product.  Each side is taken relative to an
 // origin of its own (o1: side-1 cell 0, o2: side-2 cell 0; any points would
 // do): a' = fl32(a - o1), b' = fl32(b - o2), c = fl64(o1 - o2), so that
-//   |a' - b' + c|^2 = |c|^2 + P(a') + s,  P(a') = 2 c.a' + |a'|^2,
-//   s = [-2b' q(b')] . [a' 1],  q(b') = |b'|^2 - 2 c.b':
-// only s and P, of the size of the units times |c| (not |c|^2), are rounded to
-// float.  With u = 2^-24, A, B = the slot's largest |a'|, |b'|,
+//   |a' - b' + c|^2 - |c|^2 = P(a') + Q(b') - 2 a'.b',
+//   P(a') = 2 c.a' + |a'|^2,  Q(b') = |b'|^2 - 2 c.b':
+// only P, Q and the cross term, of the size of the units times |c| (not
+// |c|^2), are rounded to float.  The instruction forms
+//   D = [-2b' 1] . [a' fl32(P(a') - kd)] + fl32(Q(b'))
+// (the side-2 row's Q in the accumulator: each lane holds four rows of one
+// side-2 block, so its four accumulator values change only with that block),
+// and a pair passes when D <= 0.  With u = 2^-24, u64 = 2^-53, the slot's
+// bounds A >= |a'|, B >= |b'| (diagonals of the units' centre boxes: o1 and
+// every side-1 centre lie in unit 1's box; L1 lengths),
 //   R = sqrt(T (1 + 1e-12)) (1 + 1e-14) + 1.01 u (A + B) + 1.01 u64 |c|,
 //   K = R^2 - |c|^2,  kd = fl32 rounded up (K + delta),
-//   delta = (8 u (2 A B + B^2 + 2 |c| B + 2 |c| A + A^2 + |K|)
-//            + 8 u64 (R^2 + |c|^2 + 2 |c| A + A^2)) (1 + 1e-6) + 1e-30,
-// the instruction forms D = s' + C with the accumulator C = fl32(fl32(P) - kd)
-// in every result register of the column, and a pair passes when D <= 0.
+//   delta = (16 u (2 A B + B^2 + 2 |c| B + 2 |c| A + A^2 + |K|)
+//            + 8 u64 (R^2 + |c|^2 + 2 |c| A + A^2)) (1 + 1e-6) + 1e-30.
 // Every pair whose FP64 d2 is at most T (1 + 1e-13) passes: the roundings of
 // a - o1, b - o2 and o1 - o2 move |a' - b' + c| from |a - b| by at most
 // 1.01 u (|a'| + |b'|) + 1.01 u64 |c|, so |a' - b' + c| <= R and
-// s + P <= K; the float q(b'), P and C and the 5-term sum in any order and
-// rounding add at most 8 u (2 A B + B^2 + 2 |c| B + 2 |c| A + A^2 + |K| +
-// delta) (|C| <= |P| + |K| + delta), the FP64 parts a few u64 of their terms
+// P + Q - 2 a'.b' <= K; P, Q and P - kd in float (c rounded to float, 3-term
+// sums) and the 5-term sum of the instruction in any order and rounding add at
+// most 16 u (2 A B + B^2 + 2 |c| B + 2 |c| A + A^2 + |K| + delta) (|P - kd| <=
+// 2 |c| A + A^2 + |K| + delta), the FP64 parts a few u64 of their terms
 // (1e-30: flushed denormals).  "Some result <= 0" is read from the integer
 // minimum of the results' bits (negative floats are negative integers; a
 // NaN's sign is not used, so a NaN may send a product to the exact test).  A
@@ -1642,15 +1647,16 @@ template <int NQ>
 __device__ __forceinline__ void k_dist_filter_mfma32_body(const uint64_t* list, const unsigned* list_count,
                                                             int nslots, const int* su1, const int* su2, int w1,
                                                             int w2, const double4* C1, const double4* C2,
+                                                            const double4* lo1, const double4* hi1,
+                                                            const double4* lo2, const double4* hi2,
                                                             const double* thresh, Cand* out,
                                                             unsigned long long* out_count, unsigned long long cap,
                                                             unsigned long long* pairs_done) {
   typedef float v4f __attribute__((ext_vector_type(4)));
   constexpr int kStage = 128;
   __shared__ float sB[17][64];  // side-1 B operands [16 k + column]; one padding block for the prefetch
-  __shared__ float sP[17 * 16];  // side-1 fl32(P(a'))
   __shared__ float sA[16][64];  // side-2 A operands [16 k + row]
-  __shared__ float sMax[8];     // per wavefront: largest |a'| (0-3), |b'| (4-7)
+  __shared__ float sQ[256];     // side-2 fl32(Q(b'))
   __shared__ double sCa[3][256], sCb[3][256];  // FP64 centres (records and partial blocks)
   __shared__ Cand stage[4][kStage];
   unsigned long long done = 0;
@@ -1709,79 +1715,70 @@ __device__ __forceinline__ void k_dist_filter_mfma32_body(const uint64_t* list, 
     if (t < nB) b = C2[(size_t)v2 * nB + t];
     return b;
   };
-  // the slot's origins (one address per wavefront)
-  auto load_o1 = [&](int v1) { return C1[(size_t)v1 * nA]; };
-  auto load_o2 = [&](int v2) { return C2[(size_t)v2 * nB]; };
+  // bound on the diagonal of a unit's centre box (its L1 length, rounded up)
+  auto diag = [](const double4& l, const double4& h) {
+    return ((h.x - l.x) + (h.y - l.y) + (h.z - l.z)) * (1.0 + 1e-15);
+  };
   WalkSlot cur = walk_slot(0), nxt = walk_slot(1);
-  double4 a_cur = load_a(cur.u1), b_cur = load_b(cur.u2), o1_cur, o2_cur;
-  if (cur.slot >= 0) {
-    o1_cur = load_o1(cur.u1);
-    o2_cur = load_o2(cur.u2);
-  }
+  double4 a_cur = load_a(cur.u1), b_cur = load_b(cur.u2);
 #pragma unroll 1
   for (int k = 0; cur.slot >= 0; ++k) {
-    const int slot = cur.slot;
-    double4 a_nxt, b_nxt, o1_nxt, o2_nxt;
+    const int slot = cur.slot, u1 = cur.u1, u2 = cur.u2;
+    double4 a_nxt, b_nxt;
     if (nxt.slot >= 0) {
       a_nxt = load_a(nxt.u1);
       b_nxt = load_b(nxt.u2);
-      o1_nxt = load_o1(nxt.u1);
-      o2_nxt = load_o2(nxt.u2);
     }
     const WalkSlot nxt2 = walk_slot(k + 2);
-    __syncthreads();  // the previous slot's operands are no longer read
-    const double cx = o1_cur.x - o2_cur.x, cy = o1_cur.y - o2_cur.y, cz = o1_cur.z - o2_cur.z;
+    // the slot's constants (one address per wavefront)
+    const double4 o1 = C1[(size_t)u1 * nA], o2 = C2[(size_t)u2 * nB];
+    const double A = diag(lo1[u1], hi1[u1]), B = diag(lo2[u2], hi2[u2]);
+    const double cx = o1.x - o2.x, cy = o1.y - o2.y, cz = o1.z - o2.z;
+    float kd;  // K + delta, rounded up to float
     {
-      const double4 av = a_cur;
-      const bool in = t < nA;
-      const float ax = __double2float_rn(av.x - o1_cur.x), ay = __double2float_rn(av.y - o1_cur.y),
-                  az = __double2float_rn(av.z - o1_cur.z);
-      sB[t >> 4][t & 15] = ax;
-      sB[t >> 4][16 + (t & 15)] = ay;
-      sB[t >> 4][32 + (t & 15)] = az;
-      sB[t >> 4][48 + (t & 15)] = in ? 1.f : qnanf;
-      sCa[0][t] = av.x;
-      sCa[1][t] = av.y;
-      sCa[2][t] = av.z;
-      const double a2 = (double)ax * ax + (double)ay * ay + (double)az * az;
-      sP[t] = in ? __double2float_rn(2.0 * (cx * ax + cy * ay + cz * az) + a2) : infinity_v<float>();
-      float m = in ? __double2float_ru(sqrt(a2) * (1.0 + 1e-15)) : 0.f;
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
-      if (lane == 0) sMax[wv] = m;
-    }
-    {  // A operand column of side-2 cell t: lane (row t%16, k) of block t/16 holds [-2bx' -2by' -2bz' q(b')][k]
-      const double4 bv = b_cur;
-      const bool in = t < nB;
-      const float bx = __double2float_rn(bv.x - o2_cur.x), by = __double2float_rn(bv.y - o2_cur.y),
-                  bz = __double2float_rn(bv.z - o2_cur.z);
-      const double b2 = (double)bx * bx + (double)by * by + (double)bz * bz;
-      sA[t >> 4][t & 15] = -2.f * bx;
-      sA[t >> 4][16 + (t & 15)] = -2.f * by;
-      sA[t >> 4][32 + (t & 15)] = -2.f * bz;
-      sA[t >> 4][48 + (t & 15)] = __double2float_rn(b2 - 2.0 * (cx * bx + cy * by + cz * bz));
-      sCb[0][t] = bv.x;
-      sCb[1][t] = bv.y;
-      sCb[2][t] = bv.z;
-      float m = in ? __double2float_ru(sqrt(b2) * (1.0 + 1e-15)) : 0.f;
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
-      if (lane == 0) sMax[4 + wv] = m;
-    }
-    __syncthreads();
-    float kd;  // K + delta of the slot, rounded up to float
-    {
-      const double A = fmax(fmax((double)sMax[0], (double)sMax[1]), fmax((double)sMax[2], (double)sMax[3]));
-      const double B = fmax(fmax((double)sMax[4], (double)sMax[5]), fmax((double)sMax[6], (double)sMax[7]));
-      const double c2 = cx * cx + cy * cy + cz * cz, cn = sqrt(c2);
+      const double c2 = cx * cx + cy * cy + cz * cz;
+      const double cn = (fabs(cx) + fabs(cy) + fabs(cz)) * (1.0 + 1e-15);  // >= |c|
       const double R = sqt + 1.01 * u * (A + B) + 1.01 * u64 * cn;
       const double R2 = R * R, K = R2 - c2;
-      const double delta = (8.0 * u * (2.0 * A * B + B * B + 2.0 * cn * B + 2.0 * cn * A + A * A + fabs(K)) +
+      const double delta = (16.0 * u * (2.0 * A * B + B * B + 2.0 * cn * B + 2.0 * cn * A + A * A + fabs(K)) +
                             8.0 * u64 * (R2 + c2 + 2.0 * cn * A + A * A)) *
                                (1.0 + 1e-6) +
                            1e-30;
       kd = __double2float_ru(K + delta);
     }
+    const float cxf = (float)cx, cyf = (float)cy, czf = (float)cz;
+    __syncthreads();  // the previous slot's operands are no longer read
+    {
+      const double4 av = a_cur;
+      const bool in = t < nA;
+      const float ax = __double2float_rn(av.x - o1.x), ay = __double2float_rn(av.y - o1.y),
+                  az = __double2float_rn(av.z - o1.z);
+      const float a2 = fmaf(ax, ax, fmaf(ay, ay, az * az));
+      const float pa = fmaf(2.f * cxf, ax, fmaf(2.f * cyf, ay, fmaf(2.f * czf, az, a2)));
+      sB[t >> 4][t & 15] = ax;
+      sB[t >> 4][16 + (t & 15)] = ay;
+      sB[t >> 4][32 + (t & 15)] = az;
+      sB[t >> 4][48 + (t & 15)] = in ? pa - kd : qnanf;
+      sCa[0][t] = av.x;
+      sCa[1][t] = av.y;
+      sCa[2][t] = av.z;
+    }
+    {  // A operand column of side-2 cell t: lane (row t%16, k) of block t/16 holds [-2bx' -2by' -2bz' 1][k]
+      const double4 bv = b_cur;
+      const bool in = t < nB;
+      const float bx = __double2float_rn(bv.x - o2.x), by = __double2float_rn(bv.y - o2.y),
+                  bz = __double2float_rn(bv.z - o2.z);
+      const float b2 = fmaf(bx, bx, fmaf(by, by, bz * bz));
+      sA[t >> 4][t & 15] = -2.f * bx;
+      sA[t >> 4][16 + (t & 15)] = -2.f * by;
+      sA[t >> 4][32 + (t & 15)] = -2.f * bz;
+      sA[t >> 4][48 + (t & 15)] = in ? 1.f : qnanf;
+      sQ[t] = fmaf(-2.f * cxf, bx, fmaf(-2.f * cyf, by, fmaf(-2.f * czf, bz, b2)));
+      sCb[0][t] = bv.x;
+      sCb[1][t] = bv.y;
+      sCb[2][t] = bv.z;
+    }
+    __syncthreads();
     if (threadIdx.x == 0) done += (unsigned long long)nA * (unsigned long long)nB;
     const uint64_t key_s = pack_idx((uint32_t)slot, 0u, 0u);
     auto record = [&](int ci, int cj, const unsigned long long mk, bool hit) {
@@ -1823,26 +1820,34 @@ __device__ __forceinline__ void k_dist_filter_mfma32_body(const uint64_t* list, 
         record(ci, cj, mk, has);
       }
     };
+    // this lane's four rows of side-2 block rb: their Q, the accumulator
+    auto acc_of = [&](int rb) {
+      const float* q = &sQ[16 * rb + 4 * kq];
+      const v4f c = {q[0], q[1], q[2], q[3]};
+      return c;
+    };
     if (nBf > 0) {
       float aop[NQ > 0 ? NQ : 1];
+      v4f cq[NQ > 0 ? NQ : 1];
 #pragma unroll
-      for (int q = 0; q < NQ; ++q) aop[q] = sA[wv + 4 * q][lane];
+      for (int q = 0; q < NQ; ++q) {
+        aop[q] = sA[wv + 4 * q][lane];
+        cq[q] = acc_of(wv + 4 * q);
+      }
       auto run = [&](auto extra_c, int jb0, int jb1, int rbx) {
         // (NQ = 4 leaves no leftover side-2 blocks: that form never runs with EXTRA)
         constexpr bool EXTRA = decltype(extra_c)::value && NQ < 4;
         constexpr int NP = NQ + (EXTRA ? 1 : 0);  // products per side-1 block, <= 4
         if (NP == 0 || jb0 >= jb1) return;
         const float ax = EXTRA ? sA[rbx][lane] : 0.f;
-        float bop = sB[jb0][lane], pc = sP[16 * jb0 + col];
+        const v4f cx4 = EXTRA ? acc_of(rbx) : v4f{0.f, 0.f, 0.f, 0.f};
+        float bop = sB[jb0][lane];
         for (int jb = jb0; jb < jb1; ++jb) {
-          const float bop_n = sB[jb + 1][lane];  // next block (row 16 of sB, entries 256.. of sP: read, unused)
-          const float pc_n = sP[16 * (jb + 1) + col];
-          const float cc = pc - kd;  // the accumulator: -(threshold) of this lane's column
-          const v4f cv = {cc, cc, cc, cc};
+          const float bop_n = sB[jb + 1][lane];  // next block (row 16 of sB: read, unused)
           v4f d[4];
 #pragma unroll
-          for (int q = 0; q < NQ; ++q) d[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(aop[q], bop, cv, 0, 0, 0);
-          if (EXTRA) d[NP - 1] = __builtin_amdgcn_mfma_f32_16x16x4f32(ax, bop, cv, 0, 0, 0);
+          for (int q = 0; q < NQ; ++q) d[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(aop[q], bop, cq[q], 0, 0, 0);
+          if (EXTRA) d[EXTRA ? NP - 1 : 0] = __builtin_amdgcn_mfma_f32_16x16x4f32(ax, bop, cx4, 0, 0, 0);
           int mn = INT_MAX;
 #pragma unroll
           for (int q = 0; q < NP; ++q)
@@ -1850,11 +1855,10 @@ __device__ __forceinline__ void k_dist_filter_mfma32_body(const uint64_t* list, 
                              min(__float_as_int(d[q][2]), __float_as_int(d[q][3]))));
           if (__ballot(mn <= 0) != 0ull) {  // rare
             int rq[4] = {wv, wv + 4, wv + 8, wv + 12};
-            if (EXTRA) rq[NP - 1] = rbx;
+            if (EXTRA) rq[EXTRA ? NP - 1 : 0] = rbx;
             append(d, jb, rq, NP);
           }
           bop = bop_n;
-          pc = pc_n;
         }
       };
       using plain_t = std::integral_constant<bool, false>;
@@ -1891,8 +1895,6 @@ __device__ __forceinline__ void k_dist_filter_mfma32_body(const uint64_t* list, 
     if (nxt.slot >= 0) {
       a_cur = a_nxt;
       b_cur = b_nxt;
-      o1_cur = o1_nxt;
-      o2_cur = o2_nxt;
     }
     cur = nxt;
     nxt = nxt2;
@@ -1903,10 +1905,11 @@ __device__ __forceinline__ void k_dist_filter_mfma32_body(const uint64_t* list, 
 template <int NQ>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_dist_filter_mfma32(
     const uint64_t* list, const unsigned* list_count, int nslots, const int* su1, const int* su2, int w1, int w2,
-    const double4* C1, const double4* C2, const double* thresh, Cand* out, unsigned long long* out_count,
-    unsigned long long cap, unsigned long long* pairs_done) {
-  k_dist_filter_mfma32_body<NQ>(list, list_count, nslots, su1, su2, w1, w2, C1, C2, thresh, out, out_count, cap,
-                                pairs_done);
+    const double4* C1, const double4* C2, const double4* lo1, const double4* hi1, const double4* lo2,
+    const double4* hi2, const double* thresh, Cand* out, unsigned long long* out_count, unsigned long long cap,
+    unsigned long long* pairs_done) {
+  k_dist_filter_mfma32_body<NQ>(list, list_count, nslots, su1, su2, w1, w2, C1, C2, lo1, hi1, lo2, hi2, thresh,
+                                out, out_count, cap, pairs_done);
 }
 
 // ---------------------------------------------------------------------------
@@ -3289,7 +3292,8 @@ hipError_t launch_dist_filter(const uint64_t* list, const unsigned* list_count, 
                               const int* su2, int w1, int w2, const vec4_t<T>* C1, const vec4_t<T>* C2,
                               const T* thresh, Cand* out, unsigned long long* out_count, unsigned long long cap,
                               int grid, hipStream_t s, unsigned long long* pairs_done, int fma, int cull,
-                              const unsigned long long* absmax, int screen32) {
+                              const unsigned long long* absmax, int screen32, const vec4_t<T>* blo1,
+                              const vec4_t<T>* bhi1, const vec4_t<T>* blo2, const vec4_t<T>* bhi2) {
   if (list_cap == 0) return hipSuccess;
   const unsigned g = (unsigned)std::min<long long>(grid, (long long)list_cap);
   if (w1 <= kTile && w2 <= kTile) {  // every unit is one tile: one slot per list entry
@@ -3304,7 +3308,7 @@ hipError_t launch_dist_filter(const uint64_t* list, const unsigned* list_count, 
 #define JSI_MFMA_FILTER(NQ)                                                                                         \
   if (screen32)                                                                                                     \
     hipLaunchKernelGGL(k_dist_filter_mfma32<NQ>, dim3(std::min(gu, gp)), dim3(256), 0, s, list, list_count, (int)gu, \
-                       su1, su2, w1, w2, C1, C2, thresh, out, out_count, cap, pairs_done);                          \
+                       su1, su2, w1, w2, C1, C2, blo1, bhi1, blo2, bhi2, thresh, out, out_count, cap, pairs_done);  \
   else                                                                                                              \
     hipLaunchKernelGGL(k_dist_filter_mfma<NQ>, dim3(std::min(gu, gp)), dim3(256), 0, s, list, list_count, (int)gu,  \
                        su1, su2, w1, w2, C1, C2, thresh, out, out_count, cap, pairs_done, absmax)
@@ -3614,7 +3618,8 @@ hipError_t launch_fill_u32(unsigned* p, unsigned value, unsigned long long n, hi
   template hipError_t launch_dist_filter<T>(const uint64_t*, const unsigned*, unsigned, const int*, const int*,    \
                                             int, int, const vec4_t<T>*, const vec4_t<T>*, const T*, Cand*,         \
                                             unsigned long long*, unsigned long long, int, hipStream_t,             \
-                                            unsigned long long*, int, int, const unsigned long long*, int);        \
+                                            unsigned long long*, int, int, const unsigned long long*, int,         \
+                                            const vec4_t<T>*, const vec4_t<T>*, const vec4_t<T>*, const vec4_t<T>*); \
   template hipError_t launch_set_thresh<T>(T*, const Cand*, int, int, hipStream_t);                                \
   template hipError_t launch_eval_centres<T>(const EcJob<T>&, const EcJob<T>*, hipStream_t);                      \
   template hipError_t launch_points_tile<T>(const SurfDev<T>&, const vec4_t<T>*, int, int, int, int, T*,           \
@@ -3926,8 +3931,8 @@ static void topk_brec(BatchProgram& p, const DistBatchShape& sh, const std::vect
           std::vector<typename BodyArgs<FT::value>::type> a;
           for (const auto& q : tq)
             a.push_back(bargs<FT::value>(q.fail, (const uint64_t*)q.list, (const unsigned*)q.small, (int)gu, q.su1,
-                                         q.su2, w1, w2, q.C1, q.C2, (const double*)thr(q), q.cand, out_count(q), cap,
-                                         (unsigned long long*)nullptr));
+                                         q.su2, w1, w2, q.C1, q.C2, q.lo1, q.hi1, q.lo2, q.hi2, (const double*)thr(q),
+                                         q.cand, out_count(q), cap, (unsigned long long*)nullptr));
           brec_wpe<256, 4, FT::value>(p, gx, 256, a);
         };
         auto mf = [&](auto fm, auto wpe) {

@@ -368,7 +368,7 @@ int jsi_ctx::dist_topk(int nslots, const int* su1, const int* su2, int w1, int n
       HCK(hipEventRecord(ev[0], stream));
     }
     HCK(launch_dist_filter<T>(list, list_count, (unsigned)ntp, su1, su2, w1, w2, C1, C2, thresh, cand, out_count, cap,
-                              grid, stream, pairs_done, fma, cull, exp_absmax, screen32));
+                              grid, stream, pairs_done, fma, cull, exp_absmax, screen32, lo1, hi1, lo2, hi2));
     if (prof) HCK(hipEventRecord(ev[1], stream));
     if (rerank) HCK(launch_rerank(cand, out_count, cap, su1, su2, w1 * w1, w2 * w2, rr.C1, rr.C2, stream));
     HCK(select_topk(cand, cap, k, maxkey, st, hist, small, small_count, small_cap, stream, 2, false, out_count));
