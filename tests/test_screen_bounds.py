@@ -238,3 +238,114 @@ def test_fp32screen_planted_weaker_inflate_is_caught():
     # a widening that covers the centre rounding only once, and none at all
     assert fp32screen_violations(n=400, e_coef=1.0) > 0
     assert fp32screen_violations(n=400, e_coef=0.0, g_coef=0.0) > 0
+
+
+# ---------------------------------------------------------------------------
+# FP32SCREEN refinement rounds (k_dist_filter_mfma32, jsi_kernels.hip): each
+# side relative to its own origin, a' = fl32(a - o1), b' = fl32(b - o2),
+# c = fl64(o1 - o2); the matrix core forms
+#   D = [-2b' 1] . [a' fl32(P - kd)] + fl32(Q),
+#   P = 2 c.a' + |a'|^2, Q = |b'|^2 - 2 c.b' (float fused chains, c in float),
+# and a pair passes when D <= 0.  kd = ru32(K + delta) with
+#   R = sqrt(T (1 + 1e-12)) (1 + 1e-14) + 1.01 u (A + B) + 1.01 u64 |c|_1,
+#   K = R^2 - |c|^2, delta = (16 u (2AB + B^2 + 2|c|B + 2|c|A + A^2 + |K|)
+#                             + 8 u64 (R^2 + |c|^2 + 2|c|A + A^2)) (1 + 1e-6) + 1e-30.
+# Every pair whose FP64 d2 is at most T (1 + 1e-13) must pass for every
+# evaluation order of the 5-term sum.  A, B are the smallest bounds the kernel
+# could use (|a'|, |b'| themselves; the kernel's box diagonals are larger).
+# ---------------------------------------------------------------------------
+
+
+def fmaf(x, y, z):
+    return r32(Fraction(x) * Fraction(y) + Fraction(z))
+
+
+def ru32(v):
+    f = np.float32(v)
+    if float(f) < v:
+        f = np.nextafter(f, np.float32(np.inf))
+    return float(f)
+
+
+def refine32_kd(T, A, B, c, delta_coef=16.0, eps=True):
+    cx, cy, cz = c
+    sqt = np.sqrt(T * (1.0 + 1e-12)) * (1.0 + 1e-14)
+    cn = (abs(cx) + abs(cy) + abs(cz)) * (1.0 + 1e-15)
+    R = sqt + ((1.01 * U24 * (A + B) + 1.01 * U53 * cn) if eps else 0.0)
+    R2 = R * R
+    c2 = cx * cx + cy * cy + cz * cz
+    K = R2 - c2
+    delta = (delta_coef * U24 * (2.0 * A * B + B * B + 2.0 * cn * B + 2.0 * cn * A + A * A + abs(K)) +
+             8.0 * U53 * (R2 + c2 + 2.0 * cn * A + A * A)) * (1.0 + 1e-6) + 1e-30
+    return ru32(K + delta)
+
+
+def refine32_worst(a, b, o1, o2, T, **kw):
+    """Largest computed D of the pair over the orders a matrix core may use."""
+    c = tuple(float(Fraction(o1[k]) - Fraction(o2[k])) for k in range(3))
+    ap = [f32(float(Fraction(a[k]) - Fraction(o1[k]))) for k in range(3)]
+    bp = [f32(float(Fraction(b[k]) - Fraction(o2[k]))) for k in range(3)]
+    cf = [f32(x) for x in c]
+    A = float(np.sqrt(float(sum(Fraction(x) * Fraction(x) for x in ap)))) * (1.0 + 1e-15)
+    B = float(np.sqrt(float(sum(Fraction(x) * Fraction(x) for x in bp)))) * (1.0 + 1e-15)
+    kd = refine32_kd(T, A, B, c, **kw)
+    a2 = fmaf(ap[0], ap[0], fmaf(ap[1], ap[1], r32(Fraction(ap[2]) * Fraction(ap[2]))))
+    pa = fmaf(2.0 * cf[0], ap[0], fmaf(2.0 * cf[1], ap[1], fmaf(2.0 * cf[2], ap[2], a2)))
+    b2 = fmaf(bp[0], bp[0], fmaf(bp[1], bp[1], r32(Fraction(bp[2]) * Fraction(bp[2]))))
+    q = fmaf(-2.0 * cf[0], bp[0], fmaf(-2.0 * cf[1], bp[1], fmaf(-2.0 * cf[2], bp[2], b2)))
+    x = [ap[0], ap[1], ap[2], r32(Fraction(pa) - Fraction(kd))]
+    y = [-2.0 * bp[0], -2.0 * bp[1], -2.0 * bp[2], 1.0]
+    items = [Fraction(x[k]) * Fraction(y[k]) for k in range(4)] + [Fraction(q)]
+    rounded = [r32(v) for v in items]
+    worst = r32(sum(items))  # one rounding
+    for perm in itertools.permutations(range(5)):
+        acc = rounded[perm[0]]  # rounded terms, summed in this order
+        for k in perm[1:]:
+            acc = r32(Fraction(acc) + Fraction(rounded[k]))
+        worst = max(worst, acc)
+        acc = rounded[perm[0]]  # each product fused into the running sum
+        for k in perm[1:]:
+            acc = r32(items[k] + Fraction(acc))
+        worst = max(worst, acc)
+    return worst
+
+
+def refine32_samples(seed, n):
+    """Two refined units at distance |c| with cells of extent `ext` around
+    their origins, world coordinates up to 1e3; a pair and its FP64 d2."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        world = 10.0 ** rng.uniform(-2, 3)
+        gap = world * 10.0 ** rng.uniform(-6, -1)
+        ext = gap * 10.0 ** rng.uniform(-6, 0.3)
+        o1 = rng.uniform(-world, world, 3)
+        d = rng.normal(0, 1, 3)
+        o2 = o1 + d / np.linalg.norm(d) * gap
+        a = o1 + rng.uniform(-ext, ext, 3)
+        b = o2 + rng.uniform(-ext, ext, 3)
+        if rng.random() < 0.25:  # coordinates on a grid: ties in the roundings
+            s = 2.0 ** int(rng.integers(10, 30))
+            o1, o2, a, b = (np.round(v * s) / s for v in (o1, o2, a, b))
+        out.append(tuple(tuple(float(x) for x in v) for v in (a, b, o1, o2)))
+    return out
+
+
+def refine32_violations(n=300, seed=5, rel=(1.0, 1.0 - 1e-13), **kw):
+    bad = 0
+    for a, b, o1, o2 in refine32_samples(seed, n):
+        d2 = ref_d2(a, b)
+        for r in rel:  # threshold at the pair's own d2 and just below (pairs up to T (1 + 1e-13))
+            if refine32_worst(a, b, o1, o2, d2 * r, **kw) > 0.0:
+                bad += 1
+    return bad
+
+
+def test_refine32_screen_keeps_every_pair_at_threshold():
+    assert refine32_violations() == 0
+
+
+def test_refine32_planted_weaker_margins_are_caught():
+    # no rounding margin at all, and the margin without the widening of R
+    assert refine32_violations(n=120, delta_coef=0.0, eps=False) > 0
+    assert refine32_violations(n=120, delta_coef=0.0) > 0
