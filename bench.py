@@ -29,8 +29,16 @@ FILTERS = {  # kernel statistics kept by the library -> kernel name
     "dist_filter_tiles_f64": "k_dist_filter<double> (tile pairs)",
     "dist_filter_units_f32": "k_dist_filter_units<float> (one tile per unit)",
     "dist_filter_units_f64": "k_dist_filter_mfma (FP64, one tile per unit, v_mfma_f64_16x16x4f64)",
+    "dist_filter_units_mfma32": "k_dist_filter_mfma32 (FP32 screen of the refinement rounds, one tile per unit, "
+                                "v_mfma_f32_16x16x4f32)",
 }
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_k_dist_filter_mfma_hardwave_nuv512_k4096.json")
+FILTER_DTYPE = {"dist_filter_tiles_f32": "f32", "dist_filter_tiles_f64": "f64", "dist_filter_units_f32": "f32",
+                "dist_filter_units_f64": "f64", "dist_filter_units_mfma32": "f32"}
+# HBM bytes per launch of the matrix-core filters (rocprofv3 --pmc, separate runs): kernel -> file
+TRAFFIC = {
+    "dist_filter_units_f64": ("k_dist_filter_mfma<", "traffic_k_dist_filter_mfma_hardwave_nuv512_k4096.json"),
+    "dist_filter_units_mfma32": ("k_dist_filter_mfma32<", "traffic_k_dist_filter_mfma32_hardwave_nuv512_k4096.json"),
+}
 
 
 def hip_copy_calibration(mbytes, repeats=3):
@@ -245,10 +253,13 @@ def main():
                     help="also run plain device copies of this many MiB (calibrates the traffic counters)")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="write the traffic file from two rocprofv3 --pmc runs and exit")
+    ap.add_argument("--pmc-kernel", default="dist_filter_units_mfma32", choices=sorted(TRAFFIC),
+                    help="the filter whose traffic file --pmc-summary writes")
     args = ap.parse_args()
     if args.pmc_summary:
-        pmc_summary(args.pmc_summary[0], args.pmc_summary[1], args.calib_copy_mb or 256,
-                    "k_dist_filter_mfma", TRAFFIC_FILE)
+        kern, fname = TRAFFIC[args.pmc_kernel]
+        pmc_summary(args.pmc_summary[0], args.pmc_summary[1], args.calib_copy_mb or 256, kern,
+                    os.path.join(ROOT, "profiles", fname))
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each GPU would serve its own "
@@ -355,12 +366,13 @@ def main():
     # dominant screen kernel of the chosen form (most summed time), timed with HIP events
     dom = max(stats, key=lambda n: stats[n][1])
     launches, kms, pairs = stats[dom]
-    dt = "f32" if dom.endswith("f32") else "f64"
+    dt = FILTER_DTYPE[dom]
     flops = 8.0 * pairs  # algorithmic: 3 sub + 3 mul + 2 add per cell pair
     achieved = flops / (kms * 1e-3) / 1e12 if kms > 0 else 0.0
     traffic = None
-    if dom == "dist_filter_units_f64" and os.path.exists(TRAFFIC_FILE):
-        traffic = json.load(open(TRAFFIC_FILE))["bytes_per_launch"]  # rocprofv3 --pmc, separate runs
+    if dom in TRAFFIC and os.path.exists(os.path.join(ROOT, "profiles", TRAFFIC[dom][1])):
+        # rocprofv3 --pmc, separate runs
+        traffic = json.load(open(os.path.join(ROOT, "profiles", TRAFFIC[dom][1])))["bytes_per_launch"]
     cells = (args.nuv - 1) ** 2 * 2 + sum(
         (res.round[r].units1 + res.round[r].units2) * 15 * 15 for r in range(res.rounds))
     out = {

@@ -382,7 +382,7 @@ int jsi_ctx::dist_topk(int nslots, const int* su1, const int* su2, int w1, int n
       // k_dist_filter_units runs when every unit is a single tile (refinement rounds)
       const bool units = w1 <= kTile && w2 <= kTile;
       std::string key = std::string(units ? "dist_filter_units_" : "dist_filter_tiles_") +
-                        (std::is_same<T, float>::value ? "f32" : "f64");
+                        (units && screen32 ? "mfma32" : std::is_same<T, float>::value ? "f32" : "f64");
       KernelStat& kst = stats[key];
       kst.launches += 1;
       kst.ms += ms;
