@@ -343,6 +343,27 @@ def main():
                  "queries_per_s": 1e3 / float(np.mean(ilat)), "p50_ms": pct(ilat, 50), "p99_ms": pct(ilat, 99),
                  "pairs_per_round": [r["npairs"] for r in ires.rounds_list()], "curves": ncur,
                  "samples": int(len(ipts)), "max_residual": float(ipts["residual"].max()) if len(ipts) else None}
+        # the six screening rounds alone, FP64 and FP32SCREEN (float screen + FP64 re-test: same lists),
+        # and the float screen's round-1 candidates against the FP64 and EXACT32 lists
+        rounds = {}
+        for prec in ("fp64", "fp32screen"):
+            rp = default_params(precision=prec)
+            ctx.intersection(sa, q1, sb, q2, rp)
+            ctx.sync()
+            rl = []
+            for _ in range(max(3, args.isect_steps)):
+                t1 = time.perf_counter()
+                rr, _ = ctx.intersection(sa, q1, sb, q2, rp)
+                rl.append((time.perf_counter() - t1) * 1e3)
+            rounds[prec] = {"queries_per_s": 1e3 / float(np.mean(rl)), "p50_ms": pct(rl, 50),
+                            "pairs_per_round": [r["npairs"] for r in rr.rounds_list()]}
+        one = {"isect_max_rounds": 1}
+        rounds["round1_pairs"] = {
+            "fp64": ctx.intersection(sa, q1, sb, q2, default_params(precision="fp64", **one))[0].npairs,
+            "exact32": ctx.intersection(sa, q1, sb, q2, default_params(precision="exact32", **one))[0].npairs,
+            "fp32_candidates": ctx.intersection(sa, q1, sb, q2, default_params(precision="fp32screen",
+                                                                               screen_out=1, **one))[0].npairs}
+        isect["rounds_only"] = rounds
 
     # grid evaluation (evalWithCuda: 18 values per point to host memory) of surface A
     # at nuv=1024; the kernel is timed with HIP events, its bytes are the 18

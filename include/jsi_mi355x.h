@@ -35,9 +35,13 @@ extern "C" {
 /* EXACT32: FP32 with the reference's operation order and no contraction;
  *          bit-exact with VERSION5 (which computes in float throughout).
  * FP64:    the same recipe in double ("plain FP64").
- * FP32SCREEN: FP32 evaluation + screening with a rounding-error margin that
+ * FP32SCREEN: FP64 evaluation, screening in FP32 with a rounding margin that
  *          makes the candidate set a superset of the FP64 one, then FP64
- *          re-ranking and FP64 refinement; results equal FP64 mode.          */
+ *          re-ranking / re-testing; results equal FP64 mode.  Distance: round
+ *          1 on float copies of the centres, rounds 2.. on the FP32 matrix
+ *          cores relative to per-slot origins.  Intersection: float records
+ *          covering the FP64 acceptance intervals (round 1: also VERSION5's
+ *          float ones), so the candidates include VERSION5's round-1 pairs.   */
 #define JSI_PREC_EXACT32 0
 #define JSI_PREC_FP64 1
 #define JSI_PREC_FP32SCREEN 2
@@ -73,7 +77,9 @@ typedef struct jsi_params {
                             tightening and a repeated screen), results unchanged */
   int polish;            /* distance: FP64 closest-point polish of the first `polish` records of the
                             final top-k list (0 = off, at most JSI_MAX_K); fills jsi_result.closest */
-  int reserved_;
+  int screen_out;        /* intersection, FP32SCREEN only: 1 = pairs_out and jsi_result.npairs hold the
+                            float screen's candidates of the last round instead of its FP64-tested
+                            pairs (a check of the screen's superset property; rounds unchanged) */
   double param_tol;      /* distance, FP64 / FP32SCREEN only: > 0 replaces dist_max_rounds by the
                             number of rounds after which a cell spans <= param_tol in u and v on both
                             surfaces (at most JSI_MAX_ROUNDS); 0 = off */

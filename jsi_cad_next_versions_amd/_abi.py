@@ -50,7 +50,7 @@ class Params(C.Structure):
         ("trace", C.c_int),
         ("cand_cap", C.c_int),
         ("polish", C.c_int),
-        ("reserved_", C.c_int),
+        ("screen_out", C.c_int),
         ("param_tol", C.c_double),
     ]
 

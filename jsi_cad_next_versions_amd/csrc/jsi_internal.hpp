@@ -231,6 +231,16 @@ hipError_t launch_isect_pass(int write, const uint64_t* list, const unsigned lon
                              const unsigned long long* rowoff, unsigned* rowcursor, IPair* out, int grid,
                              hipStream_t s);
 hipError_t launch_row_sort(const unsigned long long* rowoff, unsigned long long nrows, IPair* out, hipStream_t s);
+// FP32SCREEN intersection: float records covering the FP64 (and, given Cv/Rv,
+// the VERSION5 float) acceptance intervals of each cell; FP64 re-test of the
+// float candidates and the order-keeping compaction of the accepted ones.
+hipError_t launch_isect_boxes32(const double4* C, const double4* R, const float4* Cv, const float4* Rv,
+                                unsigned long long n, float4* X, float4* Y, hipStream_t s);
+hipError_t launch_isect_retest(const IPair* cand, unsigned long long n, const int* su1, const int* su2, int cells1,
+                               int cells2, const double4* C1, const double4* R1, const double4* C2, const double4* R2,
+                               unsigned* flag, hipStream_t s);
+hipError_t launch_isect_scatter(const IPair* cand, const unsigned* flag, const unsigned long long* idx,
+                                unsigned long long n, IPair* out, hipStream_t s);
 // Long rows: segmented radix sort of the cell ids j (jsi_sort.hip); tmp of
 // segmented_sort_tmp_bytes(n, nrows) bytes.
 size_t segmented_sort_tmp_bytes(unsigned long long n, unsigned long long nrows);

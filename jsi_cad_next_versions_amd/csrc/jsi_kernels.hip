@@ -2655,6 +2655,74 @@ __global__ __launch_bounds__(256) void k_isect_pass(const uint64_t* list, const 
   }
 }
 
+// FP32SCREEN intersection screen.  Float (centre, radius) records, one per
+// cell, whose strict float overlap test (k_isect_pass<float>) accepts every
+// cell pair that the FP64 test accepts and, given the VERSION5 float values of
+// the same grid (round 1), every pair that VERSION5's own float test accepts.
+// Per axis: fl64(|c1 - c2|) < fl64(r1 + r2) implies |c1 - c2| < (r1 + r2)(1 +
+// 2.0001 u64), so the FP64 test accepts only pairs whose closed intervals
+// [c - r (1 + 3 u64), c + r (1 + 3 u64)] meet; the float test likewise with
+// [c32 - r32 (1 + 3u), c32 + r32 (1 + 3u)], u = 2^-24.  The record covers the
+// hull [lo, hi] of those intervals (widened by 4 u64 (|c| + r) for the FP64
+// arithmetic): c = fl32(mid), r = ru32((half + |c - mid|)(1 + 3u) + 4 u64
+// (|mid| + half) + 1e-30).  Two records whose hulls meet satisfy |c1 - c2| <=
+// r1 + r2 - m with m >= 2.99u (r1 + r2), and then fl32(|c1 - c2|) <=
+// |c1 - c2|(1 + u) < (r1 + r2)(1 - u) <= fl32(r1 + r2): the float test accepts.
+__global__ void k_isect_boxes32(const double4* C, const double4* R, const float4* Cv, const float4* Rv,
+                                unsigned long long n, float4* X, float4* Y) {
+  const unsigned long long e = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  constexpr double u = 5.9604644775390625e-08, u64 = 1.1102230246251565e-16;
+  const double4 c = C[e], r = R[e];
+  float4 cv = make_float4(0.f, 0.f, 0.f, 0.f), rv = cv;
+  if (Cv) {
+    cv = Cv[e];
+    rv = Rv[e];
+  }
+  auto axis = [&](double c64, double r64, float c32, float r32, float* xo, float* yo) {
+    const double w = r64 * (1.0 + 4.0 * u64) + 4.0 * u64 * (fabs(c64) + r64);
+    double lo = c64 - w, hi = c64 + w;
+    if (Cv) {
+      const double w32 = (double)r32 * (1.0 + 3.0 * u) + 4.0 * u64 * (fabs((double)c32) + (double)r32);
+      lo = fmin(lo, (double)c32 - w32);
+      hi = fmax(hi, (double)c32 + w32);
+    }
+    const double mid = 0.5 * (lo + hi), half = 0.5 * (hi - lo);
+    const float cf = __double2float_rn(mid);
+    *xo = cf;
+    *yo = __double2float_ru((half + fabs((double)cf - mid)) * (1.0 + 3.0 * u) + 4.0 * u64 * (fabs(mid) + half) +
+                            1e-30);
+  };
+  float4 x, y;
+  axis(c.x, r.x, cv.x, rv.x, &x.x, &y.x);
+  axis(c.y, r.y, cv.y, rv.y, &x.y, &y.y);
+  axis(c.z, r.z, cv.z, rv.z, &x.z, &y.z);
+  x.w = y.w = 0.f;
+  X[e] = x;
+  Y[e] = y;
+}
+
+// FP64 test of the float screen's candidates (the expression of
+// k_isect_pass<double>, so the decisions are the FP64 screen's), then the
+// order-keeping compaction of the accepted ones.
+__global__ void k_isect_retest(const IPair* cand, unsigned long long n, const int* su1, const int* su2, int cells1,
+                               int cells2, const double4* C1, const double4* R1, const double4* C2,
+                               const double4* R2, unsigned* flag) {
+  const unsigned long long e = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const IPair p = cand[e];
+  const size_t a = (size_t)su1[p.slot] * cells1 + p.i, b = (size_t)su2[p.slot] * cells2 + p.j;
+  const double4 c1 = C1[a], r1 = R1[a], c2 = C2[b], r2 = R2[b];
+  const bool hit = fabs(c1.x - c2.x) < (r1.x + r2.x) && fabs(c1.y - c2.y) < (r1.y + r2.y) &&
+                   fabs(c1.z - c2.z) < (r1.z + r2.z);
+  flag[e] = hit ? 1u : 0u;
+}
+__global__ void k_isect_scatter(const IPair* cand, const unsigned* flag, const unsigned long long* idx,
+                                unsigned long long n, IPair* out) {
+  const unsigned long long e = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n && flag[e]) out[idx[e]] = cand[e];
+}
+
 // Insertion sort of each row's pairs by j (rows are short).
 __global__ void k_row_sort(const unsigned long long* rowoff, unsigned long long nrows, IPair* out) {
   const unsigned long long r = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -3526,6 +3594,27 @@ hipError_t launch_isect_pass(int write, const uint64_t* list, const unsigned lon
 hipError_t launch_row_sort(const unsigned long long* rowoff, unsigned long long nrows, IPair* out, hipStream_t s) {
   if (nrows == 0) return hipSuccess;
   hipLaunchKernelGGL(k_row_sort, dim3(blocks_for(nrows, 256)), dim3(256), 0, s, rowoff, nrows, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_isect_boxes32(const double4* C, const double4* R, const float4* Cv, const float4* Rv,
+                                unsigned long long n, float4* X, float4* Y, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_isect_boxes32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, C, R, Cv, Rv, n, X, Y);
+  return hipGetLastError();
+}
+hipError_t launch_isect_retest(const IPair* cand, unsigned long long n, const int* su1, const int* su2, int cells1,
+                               int cells2, const double4* C1, const double4* R1, const double4* C2, const double4* R2,
+                               unsigned* flag, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_isect_retest, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, cand, n, su1, su2, cells1,
+                     cells2, C1, R1, C2, R2, flag);
+  return hipGetLastError();
+}
+hipError_t launch_isect_scatter(const IPair* cand, const unsigned* flag, const unsigned long long* idx,
+                                unsigned long long n, IPair* out, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_isect_scatter, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, cand, flag, idx, n, out);
   return hipGetLastError();
 }
 

@@ -842,6 +842,98 @@ int jsi_ctx::initial_k(int nuv1, int nuv2, const jsi_region& r1, const jsi_regio
   return JSI_OK;
 }
 
+// FP32SCREEN screen of one intersection round: float records covering each
+// cell's FP64 acceptance interval (and, given V*, VERSION5's float one), the
+// float tile cull and overlap test, rows sorted by j, then the FP64 test of
+// every candidate and the order-keeping compaction: the result is the FP64
+// screen's list.  The candidates stay in *cand (screen_out).
+int jsi_ctx::isect_screen32(int round, int nslots, const int* su1, const int* su2, int nunits1, int w1, int nunits2,
+                            int w2, const double4* C1, const double4* R1, const double4* C2, const double4* R2,
+                            const float4* V1c, const float4* V1r, const float4* V2c, const float4* V2r, int cull,
+                            const char* pairs_name, IPair** pairs_out, unsigned long long* npairs_out,
+                            IPair** cand_out, unsigned long long* ncand_out) {
+  const int tpu1 = ((w1 + kTile - 1) / kTile) * ((w1 + kTile - 1) / kTile);
+  const int tpu2 = ((w2 + kTile - 1) / kTile) * ((w2 + kTile - 1) / kTile);
+  const size_t n1 = (size_t)nunits1 * w1 * w1, n2 = (size_t)nunits2 * w2 * w2;
+  float4 *X1, *Y1, *X2, *Y2, *lo1, *hi1, *lo2, *hi2;
+  RCK(buf("i32.X1", n1, &X1));
+  RCK(buf("i32.Y1", n1, &Y1));
+  RCK(buf("i32.X2", n2, &X2));
+  RCK(buf("i32.Y2", n2, &Y2));
+  HCK(launch_isect_boxes32(C1, R1, V1c, V1r, n1, X1, Y1, stream));
+  HCK(launch_isect_boxes32(C2, R2, V2c, V2r, n2, X2, Y2, stream));
+  RCK(buf("tb32.lo1", (size_t)nunits1 * tpu1, &lo1));
+  RCK(buf("tb32.hi1", (size_t)nunits1 * tpu1, &hi1));
+  RCK(buf("tb32.lo2", (size_t)nunits2 * tpu2, &lo2));
+  RCK(buf("tb32.hi2", (size_t)nunits2 * tpu2, &hi2));
+  HCK(launch_tilebox<float>(X1, Y1, nunits1, w1, 1, lo1, hi1, stream));
+  HCK(launch_tilebox<float>(X2, Y2, nunits2, w2, 1, lo2, hi2, stream));
+  const unsigned long long ntp = (unsigned long long)nslots * tpu1 * tpu2;
+  if (ntp > (1ull << 40)) {
+    set_err("intersection round %d: %d slots exceed the supported range", round, nslots);
+    return JSI_ERR_CAPACITY;
+  }
+  uint64_t* list;
+  unsigned long long* small;
+  RCK(buf("i.list", ntp, &list));
+  RCK(buf("i.small", 8, &small));
+  HCK(launch_fill_u32(reinterpret_cast<unsigned*>(small), 0u, 2, stream));
+  HCK(launch_isect_list<float>(nslots, su1, su2, w1, w2, lo1, hi1, lo2, hi2, cull, list, small, ntp, stream));
+  const unsigned long long nrows = (unsigned long long)nslots * w1 * w1;
+  unsigned *rowcount, *rowcursor;
+  unsigned long long* rowoff;
+  void* scantmp;
+  RCK(buf("i.rowcount", nrows, &rowcount));
+  RCK(buf("i.rowcursor", nrows, &rowcursor));
+  RCK(buf("i.rowoff", nrows + 1, &rowoff));
+  const size_t stb = scan_tmp_bytes(nrows);
+  RCK(buf_raw("i.scantmp", stb, &scantmp, 0));
+  HCK(launch_fill_u32(rowcount, 0u, nrows, stream));
+  HCK(launch_isect_pass<float>(0, list, small, su1, su2, w1, w2, X1, Y1, X2, Y2, rowcount, rowoff, rowcursor, nullptr,
+                               grid, stream));
+  HCK(scan_u32_to_u64(rowcount, rowoff, nrows, scantmp, stb, stream));
+  unsigned long long ncand = 0;
+  RCK(d2h(&ncand, rowoff + nrows, sizeof(ncand)));
+  IPair* cand;
+  RCK(buf("i32.cand", std::max(ncand, 1ull), &cand));
+  unsigned long long npairs = 0;
+  if (ncand > 0) {
+    HCK(launch_fill_u32(rowcursor, 0u, nrows, stream));
+    HCK(launch_isect_pass<float>(1, list, small, su1, su2, w1, w2, X1, Y1, X2, Y2, rowcount, rowoff, rowcursor, cand,
+                                 grid, stream));
+    if (ncand > 16ull * nrows) {
+      const size_t tb = segmented_sort_tmp_bytes(ncand, nrows);
+      void* st;
+      RCK(buf_raw("i.segsort", tb, &st, 0));
+      HCK(segmented_sort_pairs(cand, ncand, rowoff, nrows, st, tb, stream));
+    } else {
+      HCK(launch_row_sort(rowoff, nrows, cand, stream));
+    }
+    unsigned* flag;
+    unsigned long long* idx;
+    RCK(buf("i32.flag", ncand, &flag));
+    RCK(buf("i32.idx", ncand + 1, &idx));
+    HCK(launch_isect_retest(cand, ncand, su1, su2, w1 * w1, w2 * w2, C1, R1, C2, R2, flag, stream));
+    const size_t stb2 = scan_tmp_bytes(ncand);
+    RCK(buf_raw("i.scantmp2", stb2, &scantmp, 0));
+    HCK(scan_u32_to_u64(flag, idx, ncand, scantmp, stb2, stream));
+    RCK(d2h(&npairs, idx + ncand, sizeof(npairs)));
+    RCK(buf(pairs_name, std::max(npairs, 1ull), pairs_out));
+    HCK(launch_isect_scatter(cand, flag, idx, ncand, *pairs_out, stream));
+  } else {
+    RCK(buf(pairs_name, 1, pairs_out));
+  }
+  if (prof) {
+    KernelStat& kst = stats["isect_fp32_candidates"];
+    kst.launches += 1;
+    kst.units += (double)ncand;
+  }
+  *npairs_out = npairs;
+  *cand_out = cand;
+  *ncand_out = ncand;
+  return JSI_OK;
+}
+
 template <class T>
 int jsi_ctx::intersect(int h1, const jsi_region& r1, int h2, const jsi_region& r2, const jsi_params& P,
                         jsi_result* res, jsi_pair* out, int cap_pairs, CurveReq* creq) {
@@ -890,62 +982,97 @@ int jsi_ctx::intersect(int h1, const jsi_region& r1, int h2, const jsi_region& r
     HCK(hipStreamSynchronize(stream));
     res->ms_eval += now_ms() - te;
   }
+  // FP32SCREEN: the float screen's round-1 records also cover VERSION5's own
+  // float boxes of the same grid (EXACT32 evaluation of the two regions)
+  const bool s32 = std::is_same<T, double>::value && P.precision == JSI_PREC_FP32SCREEN;
+  float4 *V1c = nullptr, *V1r = nullptr, *V2c = nullptr, *V2r = nullptr;
+  if (s32) {
+    const float tinyf = 1.0e-12f;
+    auto ext = [&](float a, float b) { return std::max(std::fabs(b - a), tinyf); };
+    const float4 f1 = make_float4(r1.ustart, r1.vstart, r1.ustop, r1.vstop);
+    const float4 f2 = make_float4(r2.ustart, r2.vstart, r2.ustop, r2.vstop);
+    float4 *regf1, *regf2;
+    RCK(buf("i32.reg1", 1, &regf1));
+    RCK(buf("i32.reg2", 1, &regf2));
+    RCK(h2d(regf1, &f1, sizeof(float4)));
+    RCK(h2d(regf2, &f2, sizeof(float4)));
+    RCK(buf("i32.V1c", (size_t)(nuv1 - 1) * (nuv1 - 1), &V1c));
+    RCK(buf("i32.V1r", (size_t)(nuv1 - 1) * (nuv1 - 1), &V1r));
+    RCK(buf("i32.V2c", (size_t)(nuv2 - 1) * (nuv2 - 1), &V2c));
+    RCK(buf("i32.V2r", (size_t)(nuv2 - 1) * (nuv2 - 1), &V2r));
+    RCK(isect_units<float>(h1, regf1, 1, nuv1, use_k, ext(r1.ustart, r1.ustop), ext(r1.vstart, r1.vstop), V1c, V1r,
+                           "x1"));
+    RCK(isect_units<float>(h2, regf2, 1, nuv2, use_k, ext(r2.ustart, r2.ustop), ext(r2.vstart, r2.vstop), V2c, V2r,
+                           "x2"));
+  }
   int nunits1 = 1, nunits2 = 1;
   long long nslots = 1;
   int g_nuv = P.isect_refine_nuv;
   IPair* pairs = nullptr;
   unsigned long long npairs = 0;
+  IPair* cand_out = nullptr;  // screen_out: the last round's float candidates
+  unsigned long long ncand_out = 0;
   std::vector<IPair> hp;
   for (int round = 1;; ++round) {
     const double ts = now_ms();
     const int w1 = nuv1 - 1, w2 = nuv2 - 1;
     const int tpu1 = ((w1 + kTile - 1) / kTile) * ((w1 + kTile - 1) / kTile);
     const int tpu2 = ((w2 + kTile - 1) / kTile) * ((w2 + kTile - 1) / kTile);
-    V *lo1, *hi1, *lo2, *hi2;
-    RCK(buf("tb.lo1", (size_t)nunits1 * tpu1, &lo1));
-    RCK(buf("tb.hi1", (size_t)nunits1 * tpu1, &hi1));
-    RCK(buf("tb.lo2", (size_t)nunits2 * tpu2, &lo2));
-    RCK(buf("tb.hi2", (size_t)nunits2 * tpu2, &hi2));
-    HCK(launch_tilebox<T>(C1, R1, nunits1, w1, 1, lo1, hi1, stream));
-    HCK(launch_tilebox<T>(C2, R2, nunits2, w2, 1, lo2, hi2, stream));
-    const unsigned long long ntp = (unsigned long long)nslots * tpu1 * tpu2;
-    if (nslots > INT_MAX || ntp > (1ull << 40)) {
-      set_err("intersection round %d: %lld slots exceed the supported range", round, nslots);
-      return JSI_ERR_CAPACITY;
-    }
-    uint64_t* list;
-    unsigned long long* small;
-    RCK(buf("i.list", ntp, &list));
-    RCK(buf("i.small", 8, &small));
-    HCK(launch_fill_u32(reinterpret_cast<unsigned*>(small), 0u, 2, stream));
-    HCK(launch_isect_list<T>((int)nslots, su1, su2, w1, w2, lo1, hi1, lo2, hi2, P.cull, list, small, ntp, stream));
-    const unsigned long long nrows = (unsigned long long)nslots * w1 * w1;
-    unsigned *rowcount, *rowcursor;
-    unsigned long long* rowoff;
-    void* scantmp;
-    RCK(buf("i.rowcount", nrows, &rowcount));
-    RCK(buf("i.rowcursor", nrows, &rowcursor));
-    RCK(buf("i.rowoff", nrows + 1, &rowoff));
-    const size_t stb = scan_tmp_bytes(nrows);
-    RCK(buf_raw("i.scantmp", stb, &scantmp, 0));
-    HCK(launch_fill_u32(rowcount, 0u, nrows, stream));
-    HCK(launch_isect_pass<T>(0, list, small, su1, su2, w1, w2, C1, R1, C2, R2, rowcount, rowoff, rowcursor, nullptr,
-                             grid, stream));
-    HCK(scan_u32_to_u64(rowcount, rowoff, nrows, scantmp, stb, stream));
-    RCK(d2h(&npairs, rowoff + nrows, sizeof(npairs)));
-    if (round == 1 && use_k) RCK(initial_k<T>(nuv1, nuv2, r1, r2, iu, iv, res));
-    RCK(buf(nm("pairs", 1, cur).c_str(), std::max(npairs, 1ull), &pairs));
-    if (npairs > 0) {
-      HCK(launch_fill_u32(rowcursor, 0u, nrows, stream));
-      HCK(launch_isect_pass<T>(1, list, small, su1, su2, w1, w2, C1, R1, C2, R2, rowcount, rowoff, rowcursor, pairs,
+    if (s32) {
+      if constexpr (std::is_same<T, double>::value) {
+        RCK(isect_screen32(round, (int)nslots, su1, su2, nunits1, w1, nunits2, w2, C1, R1, C2, R2,
+                           round == 1 ? V1c : nullptr, round == 1 ? V1r : nullptr, round == 1 ? V2c : nullptr,
+                           round == 1 ? V2r : nullptr, P.cull, nm("pairs", 1, cur).c_str(), &pairs, &npairs,
+                           &cand_out, &ncand_out));
+        if (round == 1 && use_k) RCK(initial_k<T>(nuv1, nuv2, r1, r2, iu, iv, res));
+      }
+    } else {
+      V *lo1, *hi1, *lo2, *hi2;
+      RCK(buf("tb.lo1", (size_t)nunits1 * tpu1, &lo1));
+      RCK(buf("tb.hi1", (size_t)nunits1 * tpu1, &hi1));
+      RCK(buf("tb.lo2", (size_t)nunits2 * tpu2, &lo2));
+      RCK(buf("tb.hi2", (size_t)nunits2 * tpu2, &hi2));
+      HCK(launch_tilebox<T>(C1, R1, nunits1, w1, 1, lo1, hi1, stream));
+      HCK(launch_tilebox<T>(C2, R2, nunits2, w2, 1, lo2, hi2, stream));
+      const unsigned long long ntp = (unsigned long long)nslots * tpu1 * tpu2;
+      if (nslots > INT_MAX || ntp > (1ull << 40)) {
+        set_err("intersection round %d: %lld slots exceed the supported range", round, nslots);
+        return JSI_ERR_CAPACITY;
+      }
+      uint64_t* list;
+      unsigned long long* small;
+      RCK(buf("i.list", ntp, &list));
+      RCK(buf("i.small", 8, &small));
+      HCK(launch_fill_u32(reinterpret_cast<unsigned*>(small), 0u, 2, stream));
+      HCK(launch_isect_list<T>((int)nslots, su1, su2, w1, w2, lo1, hi1, lo2, hi2, P.cull, list, small, ntp, stream));
+      const unsigned long long nrows = (unsigned long long)nslots * w1 * w1;
+      unsigned *rowcount, *rowcursor;
+      unsigned long long* rowoff;
+      void* scantmp;
+      RCK(buf("i.rowcount", nrows, &rowcount));
+      RCK(buf("i.rowcursor", nrows, &rowcursor));
+      RCK(buf("i.rowoff", nrows + 1, &rowoff));
+      const size_t stb = scan_tmp_bytes(nrows);
+      RCK(buf_raw("i.scantmp", stb, &scantmp, 0));
+      HCK(launch_fill_u32(rowcount, 0u, nrows, stream));
+      HCK(launch_isect_pass<T>(0, list, small, su1, su2, w1, w2, C1, R1, C2, R2, rowcount, rowoff, rowcursor, nullptr,
                                grid, stream));
-      if (npairs > 16ull * nrows) {  // long rows: segmented radix sort
-        const size_t tb = segmented_sort_tmp_bytes(npairs, nrows);
-        void* st;
-        RCK(buf_raw("i.segsort", tb, &st, 0));
-        HCK(segmented_sort_pairs(pairs, npairs, rowoff, nrows, st, tb, stream));
-      } else {
-        HCK(launch_row_sort(rowoff, nrows, pairs, stream));
+      HCK(scan_u32_to_u64(rowcount, rowoff, nrows, scantmp, stb, stream));
+      RCK(d2h(&npairs, rowoff + nrows, sizeof(npairs)));
+      if (round == 1 && use_k) RCK(initial_k<T>(nuv1, nuv2, r1, r2, iu, iv, res));
+      RCK(buf(nm("pairs", 1, cur).c_str(), std::max(npairs, 1ull), &pairs));
+      if (npairs > 0) {
+        HCK(launch_fill_u32(rowcursor, 0u, nrows, stream));
+        HCK(launch_isect_pass<T>(1, list, small, su1, su2, w1, w2, C1, R1, C2, R2, rowcount, rowoff, rowcursor, pairs,
+                                 grid, stream));
+        if (npairs > 16ull * nrows) {  // long rows: segmented radix sort
+          const size_t tb = segmented_sort_tmp_bytes(npairs, nrows);
+          void* st;
+          RCK(buf_raw("i.segsort", tb, &st, 0));
+          HCK(segmented_sort_pairs(pairs, npairs, rowoff, nrows, st, tb, stream));
+        } else {
+          HCK(launch_row_sort(rowoff, nrows, pairs, stream));
+        }
       }
     }
     HCK(hipStreamSynchronize(stream));
@@ -986,6 +1113,7 @@ int jsi_ctx::intersect(int h1, const jsi_region& r1, int h2, const jsi_region& r
     RCK(buf_raw("i.fp2", sizeof(int) * (size_t)nslots * w2 * w2, reinterpret_cast<void**>(&fp2), INT_MAX));
     HCK(launch_isect_dedup_flags(pairs, npairs, w2 * w2, fp2, flag1, flag2, stream));
     const size_t stb2 = scan_tmp_bytes(npairs);
+    void* scantmp;
     RCK(buf_raw("i.scantmp2", stb2, &scantmp, 0));
     HCK(scan_u32_to_u64(flag1, idx1, npairs, scantmp, stb2, stream));
     HCK(scan_u32_to_u64(flag2, idx2, npairs, scantmp, stb2, stream));
@@ -1057,6 +1185,11 @@ int jsi_ctx::intersect(int h1, const jsi_region& r1, int h2, const jsi_region& r
       if (prof) host_stat("curve_records_d2h", now_ms() - tc0);
       RCK(trace_curves(static_cast<const CurveRec*>(hc), (long long)npairs, s1, s2, region_box(r1), region_box(r2), creq));
     }
+  }
+  if (s32 && P.screen_out) {  // report the last round's float candidates instead of the tested pairs
+    pairs = cand_out;
+    npairs = ncand_out;
+    res->npairs = (int64_t)ncand_out;
   }
   if (out && cap_pairs > 0 && npairs > 0) {
     const unsigned long long m = std::min<unsigned long long>(npairs, (unsigned long long)cap_pairs);

@@ -157,6 +157,11 @@ struct jsi_ctx {
   template <class T>
   int eval_centres(int h1, const jsi::vec4_t<T>* reg1, int nunits1, int nuv1, const jsi::EcOut<T>& o1, int h2,
                    const jsi::vec4_t<T>* reg2, int nunits2, int nuv2, const jsi::EcOut<T>& o2);
+  int isect_screen32(int round, int nslots, const int* su1, const int* su2, int nunits1, int w1, int nunits2, int w2,
+                     const double4* C1, const double4* R1, const double4* C2, const double4* R2,
+                     const float4* V1c, const float4* V1r, const float4* V2c, const float4* V2r, int cull,
+                     const char* pairs_name, jsi::IPair** pairs_out, unsigned long long* npairs_out,
+                     jsi::IPair** cand_out, unsigned long long* ncand_out);
   template <class T>
   int dist_topk(int nslots, const int* su1, const int* su2, int w1, int nunits1, const jsi::vec4_t<T>* C1, int w2,
                 int nunits2, const jsi::vec4_t<T>* C2, int k, int cull, jsi::Cand** rec_out, long long* n_out,

@@ -49,6 +49,5 @@ def test_params_default_matches_version5():
     L.jsi_params_default(ctypes.byref(p))
     d = runtime.default_params()
     for f, _ in runtime.Params._fields_:
-        if f != "reserved_":
-            assert getattr(p, f) == getattr(d, f), f
+        assert getattr(p, f) == getattr(d, f), f
     assert L.jsi_status_string(-4) == b"capacity exceeded"

@@ -349,3 +349,80 @@ def test_refine32_planted_weaker_margins_are_caught():
     # no rounding margin at all, and the margin without the widening of R
     assert refine32_violations(n=120, delta_coef=0.0, eps=False) > 0
     assert refine32_violations(n=120, delta_coef=0.0) > 0
+
+
+# ---------------------------------------------------------------------------
+# FP32SCREEN intersection (k_isect_boxes32, jsi_kernels.hip): per axis, the
+# float record (c, r) of a cell covers the hull of the FP64 acceptance
+# interval [c64 -+ (r64 (1 + 4 u64) + 4 u64 (|c64| + r64))] and, in round 1,
+# VERSION5's float one [c32 -+ (r32 (1 + 3u) + 4 u64 (|c32| + r32))]:
+#   c = fl32(mid), r = ru32((half + |c - mid|)(1 + 3u) + 4 u64 (|mid| + half) + 1e-30).
+# Every pair that the FP64 test fl64(|c1 - c2|) < fl64(r1 + r2) accepts, or
+# that the float test accepts on the VERSION5 values, must pass the float test
+# on the records.
+# ---------------------------------------------------------------------------
+
+
+def isect_record(c64, r64, c32=None, r32=None, k=3.0, a=4.0):
+    w = r64 * (1.0 + 4.0 * U53) + a * U53 * (abs(c64) + r64)
+    lo, hi = c64 - w, c64 + w
+    if c32 is not None:
+        w32 = r32 * (1.0 + 3.0 * U24) + a * U53 * (abs(c32) + r32)
+        lo, hi = min(lo, c32 - w32), max(hi, c32 + w32)
+    mid, half = 0.5 * (lo + hi), 0.5 * (hi - lo)
+    cf = f32(mid)
+    return cf, ru32((half + abs(cf - mid)) * (1.0 + k * U24) + a * U53 * (abs(mid) + half) + 1e-30)
+
+
+def float_overlap(c1, r1, c2, r2):
+    """k_isect_pass<float>: fl32(|c1 - c2|) < fl32(r1 + r2) on float values."""
+    return r32(abs(Fraction(c1) - Fraction(c2))) < r32(Fraction(r1) + Fraction(r2))
+
+
+def isect_samples(seed, n):
+    """Cells at the edge of the overlap test: centres |c1 - c2| just inside
+    r1 + r2 (relative gap 1e-1 .. 1e-16), world coordinates 1e-3 .. 1e3,
+    radii 1e-9 .. 1e-1 of them; each with a nearby float (VERSION5-like) copy."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        world = 10.0 ** rng.uniform(-3, 3)
+        r1 = world * 10.0 ** rng.uniform(-9, -1)
+        r2 = r1 * 10.0 ** rng.uniform(-1, 1)
+        c1 = rng.uniform(-world, world)
+        c2 = c1 + rng.choice([-1.0, 1.0]) * (r1 + r2) * (1.0 - 10.0 ** rng.uniform(-16, -1))
+        if rng.random() < 0.25:  # grid values: ties in the roundings
+            q = 2.0 ** int(rng.integers(10, 40))
+            c1, c2, r1, r2 = (np.round(x * q) / q for x in (c1, c2, r1, r2))
+        noise = lambda x: f32(x * (1.0 + rng.normal(0, 1e-6)))
+        out.append(tuple(float(x) for x in (c1, r1, c2, r2)) + (noise(c1), noise(r1), noise(c2), noise(r2)))
+    return out
+
+
+def isect_violations(n=4000, seed=17, **kw):
+    bad = 0
+    for c1, r1, c2, r2, v1, w1, v2, w2 in isect_samples(seed, n):
+        fp64 = abs(c1 - c2) < (r1 + r2)
+        v5 = float_overlap(v1, w1, v2, w2)
+        if not (fp64 or v5):
+            continue
+        x1, y1 = isect_record(c1, r1, v1, w1, **kw)
+        x2, y2 = isect_record(c2, r2, v2, w2, **kw)
+        if not float_overlap(x1, y1, x2, y2):
+            bad += 1
+        if fp64:  # later rounds: records from the FP64 values alone
+            x1, y1 = isect_record(c1, r1, **kw)
+            x2, y2 = isect_record(c2, r2, **kw)
+            if not float_overlap(x1, y1, x2, y2):
+                bad += 1
+    return bad
+
+
+def test_isect32_records_keep_every_accepted_pair():
+    assert isect_violations() == 0
+
+
+def test_isect32_planted_weaker_records_are_caught():
+    # records without the relative widening, and without any widening
+    assert isect_violations(n=1500, k=0.0) > 0
+    assert isect_violations(n=1500, k=0.0, a=0.0) > 0
