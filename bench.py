@@ -249,6 +249,8 @@ def main():
                     help="server workers of the served form (each takes batches of up to 64 queued queries)")
     ap.add_argument("--mix-slots", type=int, default=8,
                     help="server workers of the uneven batch and the mixed serving runs")
+    ap.add_argument("--tuning-steps", type=int, default=1024,
+                    help="served queries per run of the tuned-vs-default comparison (0 = skip)")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="also run plain device copies of this many MiB (calibrates the traffic counters)")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
@@ -322,6 +324,26 @@ def main():
             blat.append((time.perf_counter() - t1) * 1e3)
         # open loop: Poisson arrivals of the same query at stated rates
         open_loop = [run_poisson(srv, [q], float(r), args.rate_seconds) for r in args.rates.split(",") if r.strip()]
+        # the autotuned launch table (loaded by Context) against the built-in choices, same served form
+        tuning = None
+        if args.tuning_steps > 0:
+            from jsi_cad_next_versions_amd.runtime import TUNING_PATH
+            tuned = os.path.exists(TUNING_PATH) and os.environ.get("JSI_TUNING", "1") != "0"
+            rates = {}
+            for form in ("tuned", "default", "tuned_again"):
+                if form == "default":
+                    ctx.tune_clear()
+                elif form == "tuned_again" and tuned:
+                    ctx.load_tuning(TUNING_PATH)
+                warm_server(srv, [q], args.concurrency, rounds=1)
+                _, w = run_served(srv, q, args.tuning_steps)
+                rates[form] = args.tuning_steps / w
+            tq = 0.5 * (rates["tuned"] + rates["tuned_again"])
+            tuning = {"table": os.path.relpath(TUNING_PATH, ROOT) if tuned else None,
+                      "form": f"served, {args.tuning_steps} queued queries per run; the tuned rate is the mean of "
+                              "a run before and after the default run",
+                      "tuned_queries_per_s": tq, "default_queries_per_s": rates["default"],
+                      "gain": tq / rates["default"]}
     headline = min(open_loop, key=lambda r: abs(r["offered_rate"] - args.latency_rate))
     dist_err = abs(res.distance - cres.distance)
     closest_err = abs(res.closest.distance - cres.closest.distance)
@@ -456,6 +478,8 @@ def main():
                      "flop_per_pair": 8},
         "screen_kernels": {n: {"launches": v[0], "ms": v[1], "pairs": v[2]} for n, v in stats.items()},
     }
+    if tuning:
+        out["tuning"] = tuning
     if isect:
         out["intersection_curves"] = isect
     if evaluation:

@@ -204,6 +204,18 @@ int jsi_min_distance(jsi_ctx* ctx, int h1, const jsi_region* r1, int h2, const j
 int jsi_min_distance_noalloc(jsi_ctx* ctx, int h1, const jsi_region* r1, int h2, const jsi_region* r2,
                              const jsi_params* params, jsi_result* result, jsi_pair* pairs_out, int pairs_cap);
 
+/* Autotuned launch and seed choices of the distance screens, per (degree,
+ * nuv): a query uses the entry of the larger degree of its surfaces with the
+ * nearest round-1 nuv (the defaults when its degree has none).  knob:
+ * "seed_pairs" (refinement seed cell pairs per unit of k, 1..256, default
+ * 32), "filter_walk" (slots a refinement-filter workgroup walks, 1..16,
+ * default 8), "r1_scale" (round-1 screen workgroups, percent of the default
+ * grid, 10..1000).  None of them changes a result.  The table is shared by a
+ * context and its server workers.  No counterpart in the reference (which
+ * has fixed launch shapes); autotune.py writes the table. */
+int jsi_tune_set(jsi_ctx* ctx, int degree, int nuv, const char* knob, int value);
+int jsi_tune_clear(jsi_ctx* ctx); /* back to the defaults everywhere */
+
 /* Surface-surface intersection screening, the run_intersect path:
  * intersection (JSI_CAD_NEXT_VERSION5/src/intersection/intersection.cu:338-475).
  * pairs_out (optional, host) receives the last round's ordered pairs. */

@@ -260,6 +260,7 @@ class DistBatcher {
   int make_plan(const DistBatchShape& sh, int nq, std::unique_ptr<DistPlan>* out);
   template <class T>
   void fill_q(const DistPlan& pl, const QLayout& L, int q, DistBatchQ<T>* d);
+  Tuning tuning_of(const jsi_query& q);
 
   jsi_ctx* c_;
   std::shared_ptr<PlanCache> cache_;
@@ -288,10 +289,20 @@ std::string DistBatcher::key(const jsi_query& q) {
       pv = polish_variant(pa, pb);
     }
   }
+  const Tuning tn = tuning_of(q);
   char buf[256];
-  snprintf(buf, sizeof(buf), "p%d n%d,%d r%d R%d k%d c%d m%d v%d,%d,%d", P.precision, q.r1.nuv, q.r2.nuv,
-           P.dist_refine_nuv, query_rounds(P, q.r1, q.r2), P.k, P.cull ? 1 : 0, std::min(P.k, P.polish), v1, v2, pv);
+  snprintf(buf, sizeof(buf), "p%d n%d,%d r%d R%d k%d c%d m%d v%d,%d,%d t%d,%d,%d", P.precision, q.r1.nuv, q.r2.nuv,
+           P.dist_refine_nuv, query_rounds(P, q.r1, q.r2), P.k, P.cull ? 1 : 0, std::min(P.k, P.polish), v1, v2, pv,
+           tn.seed_pairs, tn.filter_walk, tn.r1_scale);
   return buf;
+}
+
+// The autotuned choices for a query: by the larger degree of its surfaces
+// and its round-1 nuv.
+Tuning DistBatcher::tuning_of(const jsi_query& q) {
+  SurfDev<float> a, b;
+  if (c_->surf_dev<float>(q.h1, &a) != JSI_OK || c_->surf_dev<float>(q.h2, &b) != JSI_OK) return Tuning();
+  return c_->tuning_for(std::max(std::max(a.degu, a.degv), std::max(b.degu, b.degv)), q.r1.nuv);
 }
 
 template <class T>
@@ -424,6 +435,7 @@ int DistBatcher::run_t(const jsi_query* const* qs, int n, jsi_result* const* res
   sh.small_cap = (unsigned)std::max(2048, std::min(16384, 2 * sh.k));
   sh.cand_cap = std::max<unsigned long long>(1ull << 19, 128ull * (unsigned long long)sh.k);
   sh.grid = c_->grid;
+  sh.tune = tuning_of(q0);
   // plans hold 4, 16 or 64 queries; slots past n are marked failed from the
   // start, so every kernel skips them
   const int nq = n <= 4 ? 4 : (n <= 16 ? 16 : 64);

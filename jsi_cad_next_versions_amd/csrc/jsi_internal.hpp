@@ -153,7 +153,7 @@ hipError_t launch_dist_filter(const uint64_t* list, const unsigned* list_count, 
                               int grid, hipStream_t s, unsigned long long* pairs_done = nullptr, int fma = 0,
                               int cull = 1, const unsigned long long* absmax = nullptr, int screen32 = 0,
                               const vec4_t<T>* blo1 = nullptr, const vec4_t<T>* bhi1 = nullptr,
-                              const vec4_t<T>* blo2 = nullptr, const vec4_t<T>* bhi2 = nullptr);
+                              const vec4_t<T>* blo2 = nullptr, const vec4_t<T>* bhi2 = nullptr, int walk = 8);
 // absmax (double only): the expanded-form test (three FMAs per pair with a
 // rounding margin from the largest |centre coordinate|, exact keys).
 // screen32 (with absmax, units of one tile): the FP32 matrix-core screen of
@@ -363,6 +363,15 @@ struct BatchProgram {
   }
 };
 
+// Launch and seed choices of the distance screens, picked per (degree, nuv)
+// by the autotuner (autotune.py) and set through jsi_tune_set; the defaults
+// are the hand-set values.  None of them changes a result.
+struct Tuning {
+  int seed_pairs = 32;  // refinement rounds: seed cell pairs per unit of k (from the first slots)
+  int filter_walk = 8;  // refinement filter: slots a workgroup walks (1..16)
+  int r1_scale = 100;   // round-1 screen: workgroups, percent of the default grid
+};
+
 struct DistBatchShape {
   int prec = 0;                       // JSI_PREC_*
   int nuv1 = 0, nuv2 = 0;             // round-1 grids
@@ -375,6 +384,7 @@ struct DistBatchShape {
   unsigned small_cap = 0;             // records compacted by the selection
   int grid = 0;                       // resident workgroups of the chip
   int nq = 0;                         // queries
+  Tuning tune;                        // launch and seed choices
 };
 
 // Per-query inputs and device buffers of a distance batch.  [s][p]: side s,

@@ -1385,11 +1385,12 @@ __device__ __forceinline__ void k_dist_filter_mfma_body(const uint64_t* list, co
     wcount = 0;
   };
   // The workgroup walks slots it = blockIdx.x + k gridDim.x (at most kMaxWalk of
-  // them): their unit ids are read up front, and the centres of the next slot
-  // are loaded into registers while the current one computes.  When every slot
-  // is listed, the list is the identity (records are sorted later, so the
-  // order of the slots does not matter).
-  constexpr int kMaxWalk = 8;
+  // them; grids hold at least one workgroup per kMaxWalk slots): their unit
+  // ids are read up front, and the centres of the next slot are loaded into
+  // registers while the current one computes.  When every slot is listed, the
+  // list is the identity (records are sorted later, so the order of the slots
+  // does not matter).
+  constexpr int kMaxWalk = 16;
   const bool all = n == (unsigned)nslots;
   struct WalkSlot {
     int slot, u1, u2;  // slot -1: past the end of the walk
@@ -1687,7 +1688,7 @@ __device__ __forceinline__ void k_dist_filter_mfma32_body(const uint64_t* list, 
     wcount = 0;
   };
   // the slot walk of k_dist_filter_mfma (slot ids two ahead, centres one ahead)
-  constexpr int kMaxWalk = 8;
+  constexpr int kMaxWalk = 16;
   const bool all = n == (unsigned)nslots;
   struct WalkSlot {
     int slot, u1, u2;  // slot -1: past the end of the walk
@@ -3361,7 +3362,7 @@ hipError_t launch_dist_filter(const uint64_t* list, const unsigned* list_count, 
                               const T* thresh, Cand* out, unsigned long long* out_count, unsigned long long cap,
                               int grid, hipStream_t s, unsigned long long* pairs_done, int fma, int cull,
                               const unsigned long long* absmax, int screen32, const vec4_t<T>* blo1,
-                              const vec4_t<T>* bhi1, const vec4_t<T>* blo2, const vec4_t<T>* bhi2) {
+                              const vec4_t<T>* bhi1, const vec4_t<T>* blo2, const vec4_t<T>* bhi2, int walk) {
   if (list_cap == 0) return hipSuccess;
   const unsigned g = (unsigned)std::min<long long>(grid, (long long)list_cap);
   if (w1 <= kTile && w2 <= kTile) {  // every unit is one tile: one slot per list entry
@@ -3371,7 +3372,9 @@ hipError_t launch_dist_filter(const uint64_t* list, const unsigned* list_count, 
         // one resident round of workgroups (4 per CU: the kernel is held to 128 registers for four
         // wavefronts per SIMD) walking the slots: a workgroup lives a few microseconds per slot, too
         // short to launch one per slot (at most 8 slots per workgroup: the grid grows for more slots)
-        const unsigned gp = std::max(std::max(1u, (unsigned)grid / 2u), (gu + 7u) / 8u);
+        // (at least one workgroup per `walk` slots, walk <= 16 = the kernels' kMaxWalk)
+        const unsigned wk = (unsigned)std::min(16, std::max(1, walk));
+        const unsigned gp = std::max(std::max(1u, (unsigned)grid / 2u), (gu + wk - 1u) / wk);
         const int nq = (w2 * w2 / 16) / 4;
 #define JSI_MFMA_FILTER(NQ)                                                                                         \
   if (screen32)                                                                                                     \
@@ -3708,7 +3711,8 @@ hipError_t launch_fill_u32(unsigned* p, unsigned value, unsigned long long n, hi
                                             int, int, const vec4_t<T>*, const vec4_t<T>*, const T*, Cand*,         \
                                             unsigned long long*, unsigned long long, int, hipStream_t,             \
                                             unsigned long long*, int, int, const unsigned long long*, int,         \
-                                            const vec4_t<T>*, const vec4_t<T>*, const vec4_t<T>*, const vec4_t<T>*); \
+                                            const vec4_t<T>*, const vec4_t<T>*, const vec4_t<T>*, const vec4_t<T>*, \
+                                            int);                                                                  \
   template hipError_t launch_set_thresh<T>(T*, const Cand*, int, int, hipStream_t);                                \
   template hipError_t launch_eval_centres<T>(const EcJob<T>&, const EcJob<T>*, hipStream_t);                      \
   template hipError_t launch_points_tile<T>(const SurfDev<T>&, const vec4_t<T>*, int, int, int, int, T*,           \
@@ -3901,7 +3905,8 @@ static void topk_brec(BatchProgram& p, const DistBatchShape& sh, const std::vect
   const unsigned long long ntp = (unsigned long long)nslots * tpu1 * tpu2;
   const long long npos = (long long)nslots * tpu1;
   const int pairs_tp = std::min(w1 * w1, kTileCells) * std::min(w2 * w2, kTileCells);
-  const unsigned long long want = (32ull * (unsigned long long)k + pairs_tp - 1) / (unsigned long long)pairs_tp;
+  const unsigned long long want =
+      ((unsigned long long)sh.tune.seed_pairs * (unsigned long long)k + pairs_tp - 1) / (unsigned long long)pairs_tp;
   const int nseed = nslots == 1 ? std::min(256, tpu1) : (int)std::max(4ull, std::min(64ull, want));
   const unsigned long long cap = sh.cand_cap;
   const unsigned small_cap = sh.small_cap;
@@ -4013,7 +4018,8 @@ static void topk_brec(BatchProgram& p, const DistBatchShape& sh, const std::vect
     if (exp) {
       if constexpr (std::is_same<S, double>::value) {
         // as the single-query launch: workgroups walk at most 8 slots each
-        const unsigned gx = std::min(gu, std::max(1u, (gu + 7u) / 8u));
+        const unsigned wk = (unsigned)std::min(16, std::max(1, sh.tune.filter_walk));  // <= kMaxWalk
+        const unsigned gx = std::min(gu, std::max(1u, (gu + wk - 1u) / wk));
         const int nq = (w2 * w2 / 16) / 4;
         auto mf32 = [&](auto fm) {
           using FT = decltype(fm);
@@ -4069,8 +4075,9 @@ static void topk_brec(BatchProgram& p, const DistBatchShape& sh, const std::vect
   } else {
     // round 1: persistent workgroups walking the query's listed tile pairs,
     // about twice the resident workgroups over the whole batch
+    const unsigned long long g1 = 2ull * (unsigned long long)sh.grid * (unsigned long long)sh.tune.r1_scale / 100ull;
     const unsigned gx = (unsigned)std::min<unsigned long long>(
-        ntp, std::max(16ull, (2ull * (unsigned long long)sh.grid + B - 1) / (unsigned long long)B));
+        ntp, std::max(16ull, (g1 + B - 1) / (unsigned long long)B));
     auto ft = [&](auto ff) {
       using FT = decltype(ff);
       std::vector<typename BodyArgs<FT::value>::type> a;

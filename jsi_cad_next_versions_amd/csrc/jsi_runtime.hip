@@ -325,7 +325,8 @@ int jsi_ctx::dist_topk(int nslots, const int* su1, const int* su2, int w1, int n
   // round 1 (one slot of many tiles): up to 256 tile pairs, so the seeds
   // cover most regions that hold top-k pairs; refinement rounds (slots in
   // parent-rank order): about 32 k seed pairs from the first slots
-  const unsigned long long want = (32ull * (unsigned long long)k + pairs_tp - 1) / (unsigned long long)pairs_tp;
+  const unsigned long long want =
+      ((unsigned long long)tune_.seed_pairs * (unsigned long long)k + pairs_tp - 1) / (unsigned long long)pairs_tp;
   const int nseed = nslots == 1 ? std::min(256, tpu1) : (int)std::max(4ull, std::min(64ull, want));
   if (npos <= 8192) {
     unsigned long long *bestv = nullptr, *range, *work;
@@ -367,8 +368,11 @@ int jsi_ctx::dist_topk(int nslots, const int* su1, const int* su2, int w1, int n
       HCK(launch_fill_u32(reinterpret_cast<unsigned*>(pairs_done), 0u, 2 * kPairsSlots, stream));
       HCK(hipEventRecord(ev[0], stream));
     }
+    const bool units_f = w1 <= kTile && w2 <= kTile;
+    const int fgrid = units_f ? grid : std::max(1, (int)((long long)grid * tune_.r1_scale / 100));
     HCK(launch_dist_filter<T>(list, list_count, (unsigned)ntp, su1, su2, w1, w2, C1, C2, thresh, cand, out_count, cap,
-                              grid, stream, pairs_done, fma, cull, exp_absmax, screen32, lo1, hi1, lo2, hi2));
+                              fgrid, stream, pairs_done, fma, cull, exp_absmax, screen32, lo1, hi1, lo2, hi2,
+                              tune_.filter_walk));
     if (prof) HCK(hipEventRecord(ev[1], stream));
     if (rerank) HCK(launch_rerank(cand, out_count, cap, su1, su2, w1 * w1, w2 * w2, rr.C1, rr.C2, stream));
     HCK(select_topk(cand, cap, k, maxkey, st, hist, small, small_count, small_cap, stream, 2, false, out_count));
@@ -555,6 +559,12 @@ int jsi_ctx::min_distance(int h1, const jsi_region& r1, int h2, const jsi_region
                            jsi_result* res, jsi_pair* out, int cap_pairs) {
   using V = vec4_t<T>;
   const double t0 = now_ms();
+  {
+    SurfDev<float> a, b;
+    RCK(surf_dev<float>(h1, &a));
+    RCK(surf_dev<float>(h2, &b));
+    tune_ = tuning_for(std::max(std::max(a.degu, a.degv), std::max(b.degu, b.degv)), r1.nuv);
+  }
   // round-1 regions
   V g1, g2;
   g1.x = T(r1.ustart);
@@ -1681,3 +1691,30 @@ template int jsi_ctx::min_distance<float>(int, const jsi_region&, int, const jsi
                                           jsi_result*, jsi_pair*, int);
 template int jsi_ctx::min_distance<double>(int, const jsi_region&, int, const jsi_region&, const jsi_params&,
                                            jsi_result*, jsi_pair*, int);
+
+extern "C" int jsi_tune_set(jsi_ctx* ctx, int degree, int nuv, const char* knob, int value) {
+  if (!ctx || !knob || degree < 1 || degree > 24 || nuv < 2 || nuv > JSI_MAX_NUV) return JSI_ERR_INVALID;
+  jsi_ctx* r = ctx->root ? ctx->root : ctx;
+  std::lock_guard<std::mutex> lock(r->stats_mu);
+  jsi::Tuning& t = r->tune_table[std::make_pair(degree, nuv)];
+  const std::string k(knob);
+  if (k == "seed_pairs" && value >= 1 && value <= 256) {
+    t.seed_pairs = value;
+  } else if (k == "filter_walk" && value >= 1 && value <= 16) {
+    t.filter_walk = value;
+  } else if (k == "r1_scale" && value >= 10 && value <= 1000) {
+    t.r1_scale = value;
+  } else {
+    return JSI_ERR_INVALID;
+  }
+  return JSI_OK;
+}
+
+extern "C" int jsi_tune_clear(jsi_ctx* ctx) {
+  if (!ctx) return JSI_ERR_INVALID;
+  jsi_ctx* r = ctx->root ? ctx->root : ctx;
+  std::lock_guard<std::mutex> lock(r->stats_mu);
+  r->tune_table.clear();
+  return JSI_OK;
+}
+

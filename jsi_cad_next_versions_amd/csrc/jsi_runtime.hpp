@@ -120,6 +120,25 @@ struct jsi_ctx {
   // read with jsi_ctx_kernel_stats): batched queries, batches, fallbacks.
   jsi_ctx* root = nullptr;
   std::mutex stats_mu;
+  // Autotuned launch and seed choices per (degree, nuv) (kept on the root
+  // context, guarded by stats_mu), and the choices of the query in flight.
+  std::map<std::pair<int, int>, jsi::Tuning> tune_table;
+  jsi::Tuning tune_;
+  jsi::Tuning tuning_for(int degree, int nuv) {
+    jsi_ctx* r = root ? root : this;
+    std::lock_guard<std::mutex> lock(r->stats_mu);
+    const jsi::Tuning* best = nullptr;
+    long long bd = 0;
+    for (const auto& e : r->tune_table) {  // same degree, nearest nuv
+      if (e.first.first != degree) continue;
+      const long long d = std::llabs((long long)e.first.second - nuv);
+      if (!best || d < bd) {
+        best = &e.second;
+        bd = d;
+      }
+    }
+    return best ? *best : jsi::Tuning();
+  }
   void shared_stat(const char* name, double units) {
     jsi_ctx* r = root ? root : this;
     std::lock_guard<std::mutex> lock(r->stats_mu);

@@ -10,6 +10,7 @@ is present, loading raises.
 from __future__ import annotations
 
 import ctypes as C
+import json
 import os
 
 import numpy as np
@@ -20,6 +21,9 @@ from ._abi import (CURVE_DTYPE, JSI_ERR_BUSY, JSI_OK, MAX_ROUNDS, QUERY_DISTANCE
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libjsi_mi355x.so")
+# autotuned launch choices per (degree, nuv) (autotune.py writes it; jsi_tune_set)
+TUNING_PATH = os.path.join(HERE, "tuning.json")
+TUNING_KNOBS = ("seed_pairs", "filter_walk", "r1_scale")
 
 _f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
@@ -32,7 +36,7 @@ EXPORTS = [
     "jsi_eval_grid_f32", "jsi_eval_grid_f64", "jsi_min_distance", "jsi_min_distance_noalloc", "jsi_intersect",
     "jsi_intersect_curves", "jsi_run_batch",
     "jsi_server_create", "jsi_server_destroy", "jsi_server_surface", "jsi_server_submit", "jsi_server_poll",
-    "jsi_server_counters",
+    "jsi_server_counters", "jsi_tune_set", "jsi_tune_clear",
 ]
 # include/jsi_ascend_hotspots.h
 OP_EXPORTS = ["jsi_ascend_distance_topk_mask", "jsi_ascend_intersection_mask"]
@@ -87,6 +91,8 @@ def load_library(path: str = LIB_PATH):
     L.jsi_server_submit.argtypes = [C.c_void_p, C.POINTER(Query), C.POINTER(C.c_int64)]
     L.jsi_server_poll.argtypes = [C.c_void_p, C.c_int64, C.POINTER(Result), C.c_int]
     L.jsi_server_counters.argtypes = [C.c_void_p] + [C.POINTER(C.c_int64)] * 5
+    L.jsi_tune_set.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_char_p, C.c_int]
+    L.jsi_tune_clear.argtypes = [C.c_void_p]
     L.jsi_ascend_distance_topk_mask.argtypes = [_f32p, C.c_size_t, _f32p, C.c_size_t, C.c_int, _i32p, _i32p,
                                                 C.POINTER(C.c_float)]
     L.jsi_ascend_intersection_mask.argtypes = [_f32p, _f32p, C.c_int, _f32p, _f32p, C.c_int, _i32p, _i32p]
@@ -102,12 +108,33 @@ def _check(L, rc):
 class Context:
     """One device context with resident surfaces (jsi_ctx)."""
 
-    def __init__(self, device: int = 0, hbm_cap_bytes: int = 0):
+    def __init__(self, device: int = 0, hbm_cap_bytes: int = 0, tuning: bool = True):
+        """tuning: load the autotuned launch table (TUNING_PATH, written by
+        autotune.py) unless JSI_TUNING=0; it changes launch shapes only."""
         self.L = load_library()
         h = C.c_void_p()
         _check(self.L, self.L.jsi_ctx_create(device, hbm_cap_bytes, C.byref(h)))
         self.h = h
         self._handles = {}
+        if tuning and os.environ.get("JSI_TUNING", "1") != "0" and os.path.exists(TUNING_PATH):
+            self.load_tuning(TUNING_PATH)
+
+    def tune_set(self, degree, nuv, knob, value):
+        """One autotuned choice for queries of this (larger) degree and round-1 nuv."""
+        _check(self.L, self.L.jsi_tune_set(self.h, int(degree), int(nuv), knob.encode(), int(value)))
+
+    def tune_clear(self):
+        """Back to the built-in launch choices."""
+        _check(self.L, self.L.jsi_tune_clear(self.h))
+
+    def load_tuning(self, path):
+        """Loads a table written by autotune.py: entries of (degree, nuv, knobs)."""
+        with open(path) as f:
+            table = json.load(f)
+        for e in table["entries"]:
+            for knob in TUNING_KNOBS:
+                if knob in e["choice"]:
+                    self.tune_set(e["degree"], e["nuv"], knob, e["choice"][knob])
 
     def close(self):
         if self.h:
