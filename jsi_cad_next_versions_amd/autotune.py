@@ -38,11 +38,16 @@ CANDIDATES = {
 DEFAULTS = {"seed_pairs": 32, "filter_walk": 8, "r1_scale": 100}
 
 
+WORKERS = 4
+
+
 def served_rate(srv, q, n, reps):
     """Median queries/s over `reps` runs of n queued copies of q (after a
-    warm-up batch that builds the batch plan of the current choices)."""
-    for t in [srv.submit(q) for _ in range(128)]:
-        srv.poll(t)
+    warm-up that lets every worker build the batch plans of the current
+    choices)."""
+    for _ in range(2):
+        for t in [srv.submit(q) for _ in range(64 * WORKERS)]:
+            srv.poll(t)
     rates = []
     for _ in range(reps):
         t0 = time.perf_counter()
@@ -82,7 +87,11 @@ def tune_entry(ctx, srv, degree, nuv, n, reps, min_gain, log):
             apply(choice)
         if not changed:
             break
+    # the reported rates: both forms measured again, side by side
+    apply(DEFAULTS)
+    base = served_rate(srv, q, n, reps)
     apply(choice)
+    best = served_rate(srv, q, n, reps)
     return {"degree": degree, "nuv": nuv, "choice": choice, "default_queries_per_s": base,
             "tuned_queries_per_s": best, "gain": best / base, "trials": trials}
 
@@ -93,7 +102,7 @@ def main(argv=None):
     ap.add_argument("--degrees", default="2,3,5")
     ap.add_argument("--nuvs", default="128,256,512")
     ap.add_argument("--queries", type=int, default=320, help="queued copies per timed run")
-    ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--reps", type=int, default=5)
     ap.add_argument("--min-gain", type=float, default=0.02)
     args = ap.parse_args(argv)
 
@@ -102,7 +111,7 @@ def main(argv=None):
 
     ctx = Context(0, tuning=False)
     entries = []
-    with Server(ctx, slots=4, max_pending=1 << 20) as srv:
+    with Server(ctx, slots=WORKERS, max_pending=1 << 20) as srv:
         for d in (int(x) for x in args.degrees.split(",")):
             for nuv in (int(x) for x in args.nuvs.split(",")):
                 log(f"degree {d} nuv {nuv}")
