@@ -8,7 +8,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libjsi_mi355x.so")
 SOURCES = ["jsi_kernels.hip", "jsi_runtime.hip", "jsi_legacy.hip", "jsi_opabi.hip", "jsi_serve.hip", "jsi_refine.hip", "jsi_sort.hip",
-           "jsi_noalloc.hip", "jsi_batch.hip"]
+           "jsi_noalloc.hip", "jsi_batch.hip", "jsi_isect_cells.hip"]
 
 # -ffp-contract=off: no fused multiply-add, so float results match VERSION5
 # (built without FMA on x86-64) and double results match the FP64 oracle.

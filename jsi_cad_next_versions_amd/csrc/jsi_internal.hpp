@@ -258,6 +258,37 @@ hipError_t launch_isect_units(const IPair* pairs, unsigned long long n, int cell
                               const unsigned long long* idx2, int* firstpos2, int* nsu1, int* nsu2,
                               vec4_t<T>* nreg1, vec4_t<T>* nreg2, hipStream_t s);
 
+// One side of an intersection refinement round whose units have nuv <=
+// kIcMaxNuv (jsi_isect_cells.hip): evaluation, the compute_K padding, the
+// padded cell AABBs and each unit's tile box in one pass, several units per
+// wavefront.  lo / hi (one box per unit) may be null.
+constexpr int kIcMaxNuv = 8;
+template <class T>
+struct IcJob {
+  SurfDev<T> sd;
+  const vec4_t<T>* reg = nullptr;
+  int nunits = 0, nuv = 0;
+  T init_u = T(1), init_v = T(1);
+  vec4_t<T>* C = nullptr;
+  vec4_t<T>* R = nullptr;
+  vec4_t<T>* lo = nullptr;
+  vec4_t<T>* hi = nullptr;
+};
+// Overlap screen of a round whose side-1 units have nuv <= kIcMaxNuv and
+// side-2 units at most 64 cells: a hit mask and count per row (slot, i)
+// (cull: slots whose unit boxes lo/hi are disjoint have none); after the
+// scan of the counts, the pairs in (slot, i, j) order.
+template <class T>
+hipError_t launch_isect_masks(long long nslots, int w1, int w2, const int* su1, const int* su2, const vec4_t<T>* C1,
+                              const vec4_t<T>* R1, const vec4_t<T>* C2, const vec4_t<T>* R2, const vec4_t<T>* lo1,
+                              const vec4_t<T>* hi1, const vec4_t<T>* lo2, const vec4_t<T>* hi2, int cull,
+                              unsigned long long* masks, unsigned* rowcount, hipStream_t s);
+hipError_t launch_isect_emit(long long nslots, int w1, const unsigned long long* masks,
+                             const unsigned long long* rowoff, IPair* out, hipStream_t s);
+// b may be null; two jobs of the same kernel variant share one launch.
+template <class T>
+hipError_t launch_isect_cells(const IcJob<T>& a, const IcJob<T>* b, int use_k, hipStream_t s);
+
 // FP64 solves on the final cells (jsi_refine.hip).  x = (u1, v1, u2, v2).
 struct PolishRec {  // closest-point polish of one top-k record
   double x[4];

@@ -1,0 +1,536 @@
+// jsi_isect_cells.hip - the intersection refinement units of a round, for
+// grids of nuv <= 8 points per side: evaluation with second derivatives,
+// the unit's compute_K maxima, the padded cell AABBs and the unit's tile box
+// in one pass.  File:line citations refer to JSI_CAD_NEXT_VERSION5/.
+//
+// After the first round, the refinement units are cells of the round before,
+// sampled on nuv = 16 points per side, halved to 8 and then 4 once a round
+// passes 512 pairs (intersection.cu:338-475).  k_points_tile gives each unit
+// a 256-thread workgroup for a 16 x 16-point tile, so 16 or 64 of its threads
+// hold a point and the fixed costs of a workgroup (knot staging, five
+// dependent phases, barriers) set the speed; k_aabb and k_tilebox then read
+// the points back from HBM.  Here a wavefront holds several units (a power
+// of two lanes per unit, nuv^2 of them holding a point each) and carries
+// them from their regions to their cells in registers and LDS:
+//   1. spans and A2.3 rows (derivatives up to 2) of the units' columns and
+//      rows, one per lane (knots in LDS when KL);
+//   2. the u-contractions of every (column, control row) of a unit, by the
+//      unit's lanes, all u orders from one read of each control point;
+//   3. each point adds its q+1 rows in s order for every (k, l) and applies
+//      the quotient rule (EvaluateOneUV, evaluation.cu:259-356;
+//      EG_EvaluateQuotientRule2, evaluation.cu:77-128);
+//   4. the unit's compute_K maxima, a max over its lanes (common.cu:155-168);
+//   5. each cell's AABB from its four corner points with the compute_K
+//      padding (construct_aabbs_host, intersection.cu:106-148,187-206);
+//   6. the unit's tile box (a unit of nuv <= 8 has one 16 x 16-cell tile).
+// Every value goes through the operations k_points_tile, k_aabb and
+// k_tilebox apply to it, in their order, so the cells are bit-identical.
+#include <algorithm>
+#include <climits>
+#include <type_traits>
+
+#include "jsi_device.hpp"
+#include "jsi_internal.hpp"
+#include "jsi_nurbs.hpp"
+
+namespace jsi {
+
+namespace {
+
+constexpr int kMaxDeg = 25;     // MAXDEG, evaluation.cu:13
+constexpr int kIcMaxCols = 16;  // unit columns (and unit rows) per wavefront
+
+// Lanes per unit: nuv^2 rounded up to a power of two, so a unit's lanes are
+// an aligned group and xor shuffles below the group size stay inside it.
+__host__ __device__ inline int ic_lanes(int nuv) {
+  int l = 1;
+  while (l < nuv * nuv) l <<= 1;
+  return l;
+}
+// Units per wavefront: as many groups as fit, at most kIcMaxCols columns.
+__host__ __device__ inline int ic_units_per_wave(int nuv) {
+  const int a = 64 / ic_lanes(nuv), b = kIcMaxCols / nuv;
+  return a < b ? a : b;
+}
+
+// control rows per chunk of u-contractions: a degree-3 unit inside one knot
+// span reads 4 rows, one across a knot 5
+template <int MAXD>
+struct IcRows {
+  static constexpr int value = MAXD <= 3 ? MAXD + 2 : 4;
+};
+
+template <class T, int MAXD, bool RAT, bool KL, bool BIC, bool PADK>
+__global__ __launch_bounds__(64) void k_isect_cells(IcJob<T> j0, IcJob<T> j1, int nb0) {
+  constexpr int NB = MAXD + 1;
+  constexpr int NC = RAT ? 4 : 3;
+  constexpr int NK = PADK ? 3 : 1;  // derivative orders kept per direction
+  constexpr int kRows = IcRows<MAXD>::value;
+  constexpr int kKn = KL ? kEcKnots : 1;
+  constexpr int kRsz = kIcMaxCols * kRows * NK * NC;
+  constexpr int kA = kRsz > 3 * 64 ? kRsz : 3 * 64;
+  __shared__ T sKu[kKn], sKv[kKn];
+  __shared__ T sN[2 * kIcMaxCols][NK][NB];  // A2.3 rows: columns [0, 16), rows [16, 32)
+  __shared__ int sS[2 * kIcMaxCols];        // their spans
+  // two lives: u-contractions [column][row][k][c] (phases 2-3), then the
+  // points' coordinates [c][lane] (phase 5); a barrier separates them
+  __shared__ T sA[kA];
+  T(*const sR)[kRows][NK][NC] = reinterpret_cast<T(*)[kRows][NK][NC]>(sA);
+  const bool second = (int)blockIdx.x >= nb0;  // uniform: one body, its arguments selected
+  const SurfDev<T> sd = second ? j1.sd : j0.sd;
+  const vec4_t<T>* const regions = second ? j1.reg : j0.reg;
+  const int nunits = second ? j1.nunits : j0.nunits;
+  const int nuv = second ? j1.nuv : j0.nuv;
+  const int bid = second ? (int)blockIdx.x - nb0 : (int)blockIdx.x;
+  const int lane = threadIdx.x;
+  const int L = ic_lanes(nuv), upw = ic_units_per_wave(nuv);
+  const int sub = lane / L, e = lane % L;  // unit of the wavefront, lane in the unit
+  const int unit = bid * upw + sub;
+  const int npts = nuv * nuv, w = nuv - 1;
+  const bool live = sub < upw && unit < nunits;
+  const bool active = live && e < npts;  // the lane holds point e of its unit
+  const int iy = e / nuv, ix = e % nuv;
+  const int p = sd.degu, q = sd.degv;
+  const int du = p < 2 ? p : 2, dv = q < 2 ? q : 2;
+  const int ku = PADK ? du : 0;  // u orders formed
+  const int lv = PADK ? dv : 0;  // v orders formed
+  const int ncol = upw * nuv;    // columns (and rows) of the wavefront's units
+  // phase 0: knot vectors in LDS
+  if (KL && !sd.zero) {
+    for (int x = lane; x < sd.nku; x += 64) sKu[x] = sd.U[x];
+    for (int x = lane; x < sd.nkv; x += 64) sKv[x] = sd.V[x];
+  }
+  __syncthreads();
+  // phase 1: spans and basis rows; the parameter follows the grid driver
+  // (evaluation.cu:366) on the unit's own nuv x nuv grid
+  if (!sd.zero && lane < 2 * ncol) {
+    const int dir = lane >= ncol;
+    const int t = dir ? lane - ncol : lane;
+    const int un = bid * upw + t / nuv, idx = t % nuv;
+    if (un < nunits) {
+      const vec4_t<T> r = regions[un];
+      const T start = dir ? r.y : r.x;
+      const T step = ((dir ? r.w : r.z) - start) / T(nuv);
+      const T tt = step * T(idx) + start;
+      const int deg = dir ? q : p, der = PADK ? (dir ? dv : du) : 0;
+      const T* knots = KL ? (dir ? sKv : sKu) : (dir ? sd.V : sd.U);
+      const int span = find_span(dir ? sd.nkv : sd.nku, deg, tt, knots);
+      const int slot = (dir ? kIcMaxCols : 0) + t;
+      if (BIC) {  // degree 3 both ways: the register-resident recurrence
+        T d[3][4];
+        basis_ders_p<T, 3>(span, tt, knots, der, d);
+#pragma unroll
+        for (int k = 0; k < NK; ++k)
+#pragma unroll
+          for (int j = 0; j <= 3; ++j) sN[slot][k][j] = k <= der ? d[k][j] : T(0);
+      } else {
+        T d[3][MAXD + 1];
+        basis_ders<T, MAXD>(span, deg, tt, knots, der, d);
+        for (int k = 0; k < NK; ++k)
+          for (int j = 0; j <= deg; ++j) sN[slot][k][j] = k <= der ? d[k][j] : T(0);
+      }
+      sS[slot] = span;
+    }
+  }
+  __syncthreads();
+  // control rows a unit reads: [vlo - q, vhi] over its rows' spans (a
+  // reduction over the unit's lane group), and the wavefront's largest count
+  int vlo = INT_MAX, vhi = INT_MIN;
+  if (live && !sd.zero && e < nuv) vlo = vhi = sS[kIcMaxCols + sub * nuv + e];
+  for (int off = L >> 1; off > 0; off >>= 1) {
+    vlo = min(vlo, __shfl_xor(vlo, off));
+    vhi = max(vhi, __shfl_xor(vhi, off));
+  }
+  const int v0 = vlo - q;
+  const int nrows = (live && !sd.zero) ? vhi - vlo + q + 1 : 0;
+  int nrw = nrows;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) nrw = max(nrw, __shfl_xor(nrw, off));
+  T h[6][4];
+#pragma unroll
+  for (int m = 0; m < 6; ++m)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) h[m][c] = T(0);
+  const int cpt = sub * nuv + ix;  // the point's column slot
+  const int rpt = kIcMaxCols + sub * nuv + iy;
+  const int sv = active && !sd.zero ? sS[rpt] : 0;
+  for (int r0 = 0; r0 < nrw; r0 += kRows) {
+    const int nr = min(kRows, nrows - r0);  // <= 0 once the unit's rows are done
+    // phase 2: u-contractions of the unit's rows [r0, r0 + nr) for each of
+    // its columns, all u orders from one read of each control point
+    for (int task = e; task < nuv * nr; task += L) {
+      const int rr = task % nr, cx = task / nr;
+      const int cs = sub * nuv + cx;
+      const int base = sd.ncpu * (v0 + r0 + rr) + sS[cs] - p;
+      T acc[NK][NC];
+#pragma unroll
+      for (int k = 0; k < NK; ++k)
+#pragma unroll
+        for (int c = 0; c < NC; ++c) acc[k][c] = T(0);
+      constexpr int UR = MAXD <= 7 ? MAXD + 1 : 1;
+#pragma unroll UR
+      for (int j = 0; j <= MAXD; ++j) {
+        if (j > p) break;
+        const int id = base + j;
+        const T wj = RAT ? sd.W[id] : T(1);
+        const T px = sd.CP[3 * id + 0], py = sd.CP[3 * id + 1], pz = sd.CP[3 * id + 2];
+#pragma unroll
+        for (int k = 0; k < NK; ++k) {
+          if (k > ku) break;
+          const T n = sN[cs][k][j];
+          if (RAT) {
+            const T nw = n * wj;
+            acc[k][0] += nw * px;
+            acc[k][1] += nw * py;
+            acc[k][2] += nw * pz;
+            acc[k][NC - 1] += nw;
+          } else {
+            acc[k][0] += n * px;
+            acc[k][1] += n * py;
+            acc[k][2] += n * pz;
+          }
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < NK; ++k)
+        if (k <= ku)
+#pragma unroll
+          for (int c = 0; c < NC; ++c) sR[cs][rr][k][c] = acc[k][c];
+    }
+    __syncthreads();
+    // phase 3: each point adds this chunk's rows, in s order, for every (k, l)
+    if (active && !sd.zero) {
+#pragma unroll
+      for (int l = 0; l < NK; ++l)
+#pragma unroll
+        for (int k = 0; k < NK - l; ++k) {
+          if (l > lv || k > ku) continue;
+          const int m = out_row(k, l);
+          constexpr int UR = MAXD <= 7 ? MAXD + 1 : 1;
+#pragma unroll UR
+          for (int s2 = 0; s2 <= MAXD; ++s2) {
+            if (s2 > q) break;
+            const int rr = sv - q + s2 - v0 - r0;
+            if (rr < 0 || rr >= nr) continue;
+            const T nv = sN[rpt][l][s2];
+#pragma unroll
+            for (int c = 0; c < NC; ++c) h[m][c] += nv * sR[cpt][rr][k][c];
+          }
+        }
+    }
+    __syncthreads();  // sR is rewritten by the next chunk (or becomes the points)
+  }
+  if (RAT && active && !sd.zero) {
+    // EG_EvaluateQuotientRule2 for dim 3, der 2, stride 4 (evaluation.cu:77-128)
+    const T F = T(1) / h[0][3];
+    if (!PADK) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) h[0][c] *= F;
+    } else {
+#pragma unroll
+      for (int m = 0; m < 6; ++m)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) h[m][c] *= F;
+      const T ws = -h[1][3], wt = -h[2][3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        h[1][c] += ws * h[0][c];
+        h[2][c] += wt * h[0][c];
+      }
+      const T wss = -h[3][3], wst = -h[4][3], wtt = -h[5][3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const T S = h[0][c], Su = h[1][c], Sv = h[2][c];
+        h[3][c] += wss * S + T(2) * ws * Su;
+        h[4][c] += wst * S + wt * Su + ws * Sv;
+        h[5][c] += wtt * S + T(2) * wt * Sv;
+      }
+    }
+  }
+  // phase 4: compute_K maxima of the unit's points (max is exact and
+  // order-free) and the padding of its cells (common.cu:170-171)
+  T pad = T(0);
+  if (PADK) {
+    unsigned long long mk[3];
+#pragma unroll
+    for (int g = 0; g < 3; ++g) {
+      const T v = cmax(fabs(h[3 + g][0]), cmax(fabs(h[3 + g][1]), fabs(h[3 + g][2])));
+      mk[g] = active ? ord_bits(v) : 0ull;
+      for (int off = L >> 1; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_xor(mk[g], off);
+        mk[g] = o > mk[g] ? o : mk[g];
+      }
+    }
+    if (live) {
+      const T muu = from_bits<T>(mk[0]);
+      const T muv = from_bits<T>(mk[1]);
+      const T mvv = from_bits<T>(mk[2]);
+      const int side = nuv > 1 ? nuv - 1 : 1;
+      const T s2 = T(side * side);
+      const T raw = (muu / s2 + T(2) * muv / s2 + mvv / s2) / T(8);
+      const vec4_t<T> r = regions[unit];
+      const T init_u = second ? j1.init_u : j0.init_u, init_v = second ? j1.init_v : j0.init_v;
+      const T us = fabs(r.z - r.x) / init_u;
+      const T vs = fabs(r.w - r.y) / init_v;
+      const T ps = smax(us, vs);
+      pad = raw * ps * ps;
+      if (!isfinite(pad) || pad < T(0)) pad = T(0);
+    }
+  }
+  // phase 5: the points into LDS, then each cell's AABB from its corners
+  // (cell id = iv * (nuv - 1) + iu, corners (iv, iu), (iv, iu + 1),
+  // (iv + 1, iu), (iv + 1, iu + 1)) and the unit's box of its padded cells
+  T* const sP = sA;
+  if (active) {
+    sP[lane] = h[0][0];
+    sP[64 + lane] = h[0][1];
+    sP[128 + lane] = h[0][2];
+  }
+  __syncthreads();
+  const T inf = infinity_v<T>();
+  T mn[3] = {inf, inf, inf}, mx[3] = {-inf, -inf, -inf};
+  T amax = T(0);
+  if (active && ix < w && iy < w) {
+    const int p0 = lane, p1 = p0 + 1, p2 = p0 + nuv, p3 = p2 + 1;
+    T lo[3], hi[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const T* b = sP + 64 * c;
+      const T a0 = b[p0], a1 = b[p1], a2 = b[p2], a3 = b[p3];
+      lo[c] = smin(a0, smin(a1, smin(a2, a3)));
+      hi[c] = smax(a0, smax(a1, smax(a2, a3)));
+    }
+    vec4_t<T> ctr, rad;
+    ctr.x = (lo[0] + hi[0]) / T(2);
+    ctr.y = (lo[1] + hi[1]) / T(2);
+    ctr.z = (lo[2] + hi[2]) / T(2);
+    ctr.w = T(0);
+    rad.x = (hi[0] - lo[0]) / T(2) + pad;
+    rad.y = (hi[1] - lo[1]) / T(2) + pad;
+    rad.z = (hi[2] - lo[2]) / T(2) + pad;
+    rad.w = T(0);
+    vec4_t<T>* const C = second ? j1.C : j0.C;
+    vec4_t<T>* const R = second ? j1.R : j0.R;
+    const size_t o = (size_t)unit * w * w + iy * w + ix;
+    C[o] = ctr;
+    R[o] = rad;
+    const T v[3] = {ctr.x, ctr.y, ctr.z}, rr[3] = {rad.x, rad.y, rad.z};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      mn[a] = fmin(mn[a], v[a] - rr[a]);
+      mx[a] = fmax(mx[a], v[a] + rr[a]);
+      amax = fmax(amax, fabs(v[a]) + fabs(rr[a]));
+    }
+  }
+  vec4_t<T>* const lo = second ? j1.lo : j0.lo;
+  if (lo) {  // uniform: a kernel argument
+    for (int off = L >> 1; off > 0; off >>= 1) {
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        mn[a] = fmin(mn[a], __shfl_xor(mn[a], off));
+        mx[a] = fmax(mx[a], __shfl_xor(mx[a], off));
+      }
+      amax = fmax(amax, __shfl_xor(amax, off));
+    }
+    if (live && e == 0) {
+      // outward margin of k_tilebox: 2^-20 of the largest |coordinate|
+      const T delta = amax * T(9.5367431640625e-07) + T(1e-30);
+      vec4_t<T> l, hh;
+      l.x = mn[0] - delta;
+      l.y = mn[1] - delta;
+      l.z = mn[2] - delta;
+      l.w = T(0);
+      hh.x = mx[0] + delta;
+      hh.y = mx[1] + delta;
+      hh.z = mx[2] + delta;
+      hh.w = T(0);
+      vec4_t<T>* const hi = second ? j1.hi : j0.hi;
+      lo[unit] = l;
+      hi[unit] = hh;
+    }
+  }
+}
+
+// Overlap screen of a round whose units have at most 64 cells per side
+// (nuv <= 9, one tile per unit): one lane per row (slot, i) tests cell i of
+// the slot's side-1 unit against every cell j of its side-2 unit with the
+// strict 3-axis overlap test (overlap_aabb_host, intersection.cu:46-63) and
+// keeps the hits as a bit mask, j ascending; rowcount = its population.  A
+// slot whose unit boxes are disjoint has no hits (the tile-pair cull of
+// k_isect_list with one tile per unit).  The rows' offsets are a scan of the
+// counts, and k_isect_emit writes each row's pairs in j order, so the list
+// comes out in (slot, i, j) order with no atomics and no row sort.
+template <class T>
+__global__ __launch_bounds__(256) void k_isect_masks(long long nrows, int cells1, int cells2, const int* su1,
+                                                     const int* su2, const vec4_t<T>* C1, const vec4_t<T>* R1,
+                                                     const vec4_t<T>* C2, const vec4_t<T>* R2, const vec4_t<T>* lo1,
+                                                     const vec4_t<T>* hi1, const vec4_t<T>* lo2,
+                                                     const vec4_t<T>* hi2, int cull, unsigned long long* masks,
+                                                     unsigned* rowcount) {
+  const long long row = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= nrows) return;
+  const int slot = (int)(row / cells1), i = (int)(row % cells1);
+  const int u1 = su1[slot], u2 = su2[slot];
+  bool keep = true;
+  if (cull) {
+    const vec4_t<T> l1 = lo1[u1], h1 = hi1[u1], l2 = lo2[u2], h2 = hi2[u2];
+    keep = l1.x <= h2.x && l2.x <= h1.x && l1.y <= h2.y && l2.y <= h1.y && l1.z <= h2.z && l2.z <= h1.z;
+  }
+  unsigned long long m = 0;
+  if (keep) {
+    const vec4_t<T> c1 = C1[(size_t)u1 * cells1 + i], r1 = R1[(size_t)u1 * cells1 + i];
+    const vec4_t<T>* const c2p = C2 + (size_t)u2 * cells2;
+    const vec4_t<T>* const r2p = R2 + (size_t)u2 * cells2;
+    for (int j = 0; j < cells2; ++j) {
+      const vec4_t<T> c2 = c2p[j], r2 = r2p[j];
+      const bool hit = fabs(c1.x - c2.x) < (r1.x + r2.x) && fabs(c1.y - c2.y) < (r1.y + r2.y) &&
+                       fabs(c1.z - c2.z) < (r1.z + r2.z);
+      m |= (unsigned long long)hit << j;
+    }
+  }
+  masks[row] = m;
+  rowcount[row] = (unsigned)__popcll(m);
+}
+
+__global__ __launch_bounds__(256) void k_isect_emit(long long nrows, int cells1, const unsigned long long* masks,
+                                                    const unsigned long long* rowoff, IPair* out) {
+  const long long row = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= nrows) return;
+  unsigned long long m = masks[row];
+  if (!m) return;
+  unsigned long long o = rowoff[row];
+  IPair pr;
+  pr.slot = (int)(row / cells1);
+  pr.i = (int)(row % cells1);
+  while (m) {
+    pr.j = __ffsll((long long)m) - 1;
+    m &= m - 1;
+    out[o++] = pr;
+  }
+}
+
+template <class T>
+int ic_blocks(const IcJob<T>& j) {
+  if (j.nunits <= 0 || j.nuv < 2 || j.nuv > kIcMaxNuv) return 0;
+  const int upw = ic_units_per_wave(j.nuv);
+  return (j.nunits + upw - 1) / upw;
+}
+
+// variant of a job: 0-2 = degree class (3, 7, kMaxDeg - 1), +3 rational,
+// +6 knots in LDS, +12 bicubic (degree class 0 only)
+template <class T>
+int ic_variant(const IcJob<T>& j) {
+  const int d = std::max(j.sd.degu, j.sd.degv);
+  const int dc = d <= 3 ? 0 : d <= 7 ? 1 : 2;
+  const bool kl = j.sd.nku <= kEcKnots && j.sd.nkv <= kEcKnots;
+  const bool bic = j.sd.degu == 3 && j.sd.degv == 3 && !j.sd.zero;
+  return dc + (j.sd.rational ? 3 : 0) + (kl ? 6 : 0) + (bic ? 12 : 0);
+}
+
+template <class T, int MAXD, bool RAT, bool KL, bool BIC>
+void ic_launch_k(int use_k, const IcJob<T>& j0, const IcJob<T>& j1, int nb0, int nb, hipStream_t s) {
+  const dim3 g((unsigned)nb), b(64);
+  if (use_k)
+    hipLaunchKernelGGL((k_isect_cells<T, MAXD, RAT, KL, BIC, true>), g, b, 0, s, j0, j1, nb0);
+  else
+    hipLaunchKernelGGL((k_isect_cells<T, MAXD, RAT, KL, BIC, false>), g, b, 0, s, j0, j1, nb0);
+}
+
+template <class T, int MAXD>
+void ic_launch_rat(int var, int use_k, const IcJob<T>& j0, const IcJob<T>& j1, int nb0, int nb, hipStream_t s) {
+  const bool rat = (var / 3) % 2 == 1, kl = (var / 6) % 2 == 1, bic = var >= 12;
+  if (MAXD == 3 && bic) {
+    if (rat) {
+      if (kl)
+        ic_launch_k<T, MAXD, true, true, true>(use_k, j0, j1, nb0, nb, s);
+      else
+        ic_launch_k<T, MAXD, true, false, true>(use_k, j0, j1, nb0, nb, s);
+    } else {
+      if (kl)
+        ic_launch_k<T, MAXD, false, true, true>(use_k, j0, j1, nb0, nb, s);
+      else
+        ic_launch_k<T, MAXD, false, false, true>(use_k, j0, j1, nb0, nb, s);
+    }
+    return;
+  }
+  if (rat) {
+    if (kl)
+      ic_launch_k<T, MAXD, true, true, false>(use_k, j0, j1, nb0, nb, s);
+    else
+      ic_launch_k<T, MAXD, true, false, false>(use_k, j0, j1, nb0, nb, s);
+  } else {
+    if (kl)
+      ic_launch_k<T, MAXD, false, true, false>(use_k, j0, j1, nb0, nb, s);
+    else
+      ic_launch_k<T, MAXD, false, false, false>(use_k, j0, j1, nb0, nb, s);
+  }
+}
+
+template <class T>
+void ic_launch(int var, int use_k, const IcJob<T>& j0, const IcJob<T>& j1, int nb0, int nb, hipStream_t s) {
+  if (nb <= 0) return;
+  switch (var % 3) {
+    case 0: ic_launch_rat<T, 3>(var, use_k, j0, j1, nb0, nb, s); break;
+    case 1: ic_launch_rat<T, 7>(var, use_k, j0, j1, nb0, nb, s); break;
+    default: ic_launch_rat<T, kMaxDeg - 1>(var, use_k, j0, j1, nb0, nb, s); break;
+  }
+}
+
+}  // namespace
+
+template <class T>
+hipError_t launch_isect_cells(const IcJob<T>& a, const IcJob<T>* b, int use_k, hipStream_t s) {
+  for (const IcJob<T>* j : {&a, b}) {
+    if (!j || j->nunits <= 0) continue;
+    if (j->nuv < 2 || j->nuv > kIcMaxNuv || !j->C || !j->R || !j->lo != !j->hi) return hipErrorInvalidValue;
+  }
+  const int na = ic_blocks(a), nb = b ? ic_blocks(*b) : 0;
+  const int va = ic_variant(a);
+  if (b && nb > 0 && ic_variant(*b) == va) {
+    if ((long long)na + nb > INT_MAX) return hipErrorInvalidValue;
+    ic_launch(va, use_k, a, *b, na, na + nb, s);
+  } else {
+    ic_launch(va, use_k, a, a, na, na, s);
+    if (b) ic_launch(ic_variant(*b), use_k, *b, *b, nb, nb, s);
+  }
+  return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_isect_masks(long long nslots, int w1, int w2, const int* su1, const int* su2, const vec4_t<T>* C1,
+                              const vec4_t<T>* R1, const vec4_t<T>* C2, const vec4_t<T>* R2, const vec4_t<T>* lo1,
+                              const vec4_t<T>* hi1, const vec4_t<T>* lo2, const vec4_t<T>* hi2, int cull,
+                              unsigned long long* masks, unsigned* rowcount, hipStream_t s) {
+  const int cells1 = w1 * w1, cells2 = w2 * w2;
+  const long long nrows = nslots * cells1;
+  if (nrows <= 0) return hipSuccess;
+  if (cells2 > 64 || w1 > kIcMaxNuv || (cull && (!lo1 || !hi1 || !lo2 || !hi2)) || nrows > (1ll << 40))
+    return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_isect_masks<T>, dim3((unsigned)((nrows + 255) / 256)), dim3(256), 0, s, nrows, cells1, cells2,
+                     su1, su2, C1, R1, C2, R2, lo1, hi1, lo2, hi2, cull, masks, rowcount);
+  return hipGetLastError();
+}
+
+hipError_t launch_isect_emit(long long nslots, int w1, const unsigned long long* masks,
+                             const unsigned long long* rowoff, IPair* out, hipStream_t s) {
+  const long long nrows = nslots * w1 * w1;
+  if (nrows <= 0) return hipSuccess;
+  if (nrows > (1ll << 40)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_isect_emit, dim3((unsigned)((nrows + 255) / 256)), dim3(256), 0, s, nrows, w1 * w1, masks,
+                     rowoff, out);
+  return hipGetLastError();
+}
+
+#define JSI_IM(T)                                                                                                  \
+  template hipError_t launch_isect_masks<T>(long long, int, int, const int*, const int*, const vec4_t<T>*,         \
+                                            const vec4_t<T>*, const vec4_t<T>*, const vec4_t<T>*, const vec4_t<T>*, \
+                                            const vec4_t<T>*, const vec4_t<T>*, const vec4_t<T>*, int,              \
+                                            unsigned long long*, unsigned*, hipStream_t);
+JSI_IM(float)
+JSI_IM(double)
+#undef JSI_IM
+
+template hipError_t launch_isect_cells<float>(const IcJob<float>&, const IcJob<float>*, int, hipStream_t);
+template hipError_t launch_isect_cells<double>(const IcJob<double>&, const IcJob<double>*, int, hipStream_t);
+
+}  // namespace jsi

@@ -11,6 +11,7 @@
 #include <float.h>
 #include <limits.h>
 
+#include "jsi_device.hpp"
 #include "jsi_internal.hpp"
 #include "jsi_nurbs.hpp"
 
@@ -20,49 +21,8 @@ namespace {
 
 constexpr int kMaxDeg = 25;  // MAXDEG, evaluation.cu:13
 
-// std::min / std::max: the first argument wins unless the second is strictly
-// smaller / larger; the nesting order fixes the sign of zero.
-template <class T>
-__device__ __forceinline__ T smin(T a, T b) {
-  return (b < a) ? b : a;
-}
-template <class T>
-__device__ __forceinline__ T smax(T a, T b) {
-  return (a < b) ? b : a;
-}
-// CadMax (common.cu:132-135)
-template <class T>
-__device__ __forceinline__ T cmax(T a, T b) {
-  return a > b ? a : b;
-}
-
-__device__ __forceinline__ unsigned long long ord_bits(float x) { return (unsigned long long)__float_as_uint(x); }
-__device__ __forceinline__ unsigned long long ord_bits(double x) {
-  return (unsigned long long)__double_as_longlong(x);
-}
-template <class T>
-__device__ __forceinline__ T from_bits(unsigned long long b);
-template <>
-__device__ __forceinline__ float from_bits<float>(unsigned long long b) {
-  return __uint_as_float((unsigned)b);
-}
-template <>
-__device__ __forceinline__ double from_bits<double>(unsigned long long b) {
-  return __longlong_as_double((long long)b);
-}
 __device__ __forceinline__ unsigned ord32_up(float x) { return __float_as_uint(x); }
 __device__ __forceinline__ unsigned ord32_up(double x) { return __float_as_uint(__double2float_ru(x)); }
-
-template <class T>
-__device__ __forceinline__ T infinity_v();
-template <>
-__device__ __forceinline__ float infinity_v<float>() {
-  return __uint_as_float(0x7f800000u);
-}
-template <>
-__device__ __forceinline__ double infinity_v<double>() {
-  return __longlong_as_double(0x7ff0000000000000ll);
-}
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 
@@ -98,14 +58,6 @@ __device__ __forceinline__ void basis_vals(int span, int p, T u, const T* U, T (
   }
 }
 
-// Position of derivative (k = u order, l = v order) in the output
-// [S, Su, Sv, Suu, Suv, Svv] (EvaluateOneUV loop order + row swap,
-// evaluation.cu:281-310).
-__device__ __forceinline__ int out_row(int k, int l) {
-  const int m = (l == 0) ? k : (l == 1 ? 3 + k : 5);
-  return m == 2 ? 3 : (m == 3 ? 2 : m);
-}
-
 // Fused evaluation + distance AABB centres.  One wavefront per 16x16-cell
 // tile of a unit (a refinement unit is one tile).  The tile's 17 corner
 // columns and 17 corner rows get their spans and basis values (A2.1, A2.3
@@ -127,7 +79,6 @@ __device__ __forceinline__ int out_row(int k, int l) {
 // (rounding margins of the screens); with C32 the centres are also written
 // rounded to float (FP32 screening).
 static_assert(kAmaxSlots == 64, "the filter reduces the slots across one wavefront");
-constexpr int kEcKnots = 128;  // knots per direction staged in LDS when KL
 template <int MAXD>
 struct EcRows {  // control rows per chunk of u-contractions
   static constexpr int value = MAXD <= 3 ? 8 : 16;

@@ -1023,6 +1023,11 @@ int jsi_ctx::intersect(int h1, const jsi_region& r1, int h2, const jsi_region& r
   IPair* cand_out = nullptr;  // screen_out: the last round's float candidates
   unsigned long long ncand_out = 0;
   std::vector<IPair> hp;
+  // units of nuv <= kIcMaxNuv: evaluation, cells and tile boxes in one pass
+  // (JSI_ISECT_CELLS=0 keeps the grid-point kernel, the AABB and tile-box passes)
+  const char* icenv = std::getenv("JSI_ISECT_CELLS");
+  const bool fuse_cells = !(icenv && std::strcmp(icenv, "0") == 0);
+  V *tlo1 = nullptr, *thi1 = nullptr, *tlo2 = nullptr, *thi2 = nullptr;  // tile boxes of the current units
   for (int round = 1;; ++round) {
     const double ts = now_ms();
     const int w1 = nuv1 - 1, w2 = nuv2 - 1;
@@ -1037,51 +1042,73 @@ int jsi_ctx::intersect(int h1, const jsi_region& r1, int h2, const jsi_region& r
         if (round == 1 && use_k) RCK(initial_k<T>(nuv1, nuv2, r1, r2, iu, iv, res));
       }
     } else {
-      V *lo1, *hi1, *lo2, *hi2;
-      RCK(buf("tb.lo1", (size_t)nunits1 * tpu1, &lo1));
-      RCK(buf("tb.hi1", (size_t)nunits1 * tpu1, &hi1));
-      RCK(buf("tb.lo2", (size_t)nunits2 * tpu2, &lo2));
-      RCK(buf("tb.hi2", (size_t)nunits2 * tpu2, &hi2));
-      HCK(launch_tilebox<T>(C1, R1, nunits1, w1, 1, lo1, hi1, stream));
-      HCK(launch_tilebox<T>(C2, R2, nunits2, w2, 1, lo2, hi2, stream));
+      V *lo1 = tlo1, *hi1 = thi1, *lo2 = tlo2, *hi2 = thi2;
+      if (!lo1) {  // tile boxes not written by the evaluation
+        RCK(buf("tb.lo1", (size_t)nunits1 * tpu1, &lo1));
+        RCK(buf("tb.hi1", (size_t)nunits1 * tpu1, &hi1));
+        RCK(buf("tb.lo2", (size_t)nunits2 * tpu2, &lo2));
+        RCK(buf("tb.hi2", (size_t)nunits2 * tpu2, &hi2));
+        HCK(launch_tilebox<T>(C1, R1, nunits1, w1, 1, lo1, hi1, stream));
+        HCK(launch_tilebox<T>(C2, R2, nunits2, w2, 1, lo2, hi2, stream));
+      }
       const unsigned long long ntp = (unsigned long long)nslots * tpu1 * tpu2;
       if (nslots > INT_MAX || ntp > (1ull << 40)) {
         set_err("intersection round %d: %lld slots exceed the supported range", round, nslots);
         return JSI_ERR_CAPACITY;
       }
-      uint64_t* list;
-      unsigned long long* small;
-      RCK(buf("i.list", ntp, &list));
-      RCK(buf("i.small", 8, &small));
-      HCK(launch_fill_u32(reinterpret_cast<unsigned*>(small), 0u, 2, stream));
-      HCK(launch_isect_list<T>((int)nslots, su1, su2, w1, w2, lo1, hi1, lo2, hi2, P.cull, list, small, ntp, stream));
-      const unsigned long long nrows = (unsigned long long)nslots * w1 * w1;
-      unsigned *rowcount, *rowcursor;
-      unsigned long long* rowoff;
-      void* scantmp;
-      RCK(buf("i.rowcount", nrows, &rowcount));
-      RCK(buf("i.rowcursor", nrows, &rowcursor));
-      RCK(buf("i.rowoff", nrows + 1, &rowoff));
-      const size_t stb = scan_tmp_bytes(nrows);
-      RCK(buf_raw("i.scantmp", stb, &scantmp, 0));
-      HCK(launch_fill_u32(rowcount, 0u, nrows, stream));
-      HCK(launch_isect_pass<T>(0, list, small, su1, su2, w1, w2, C1, R1, C2, R2, rowcount, rowoff, rowcursor, nullptr,
-                               grid, stream));
-      HCK(scan_u32_to_u64(rowcount, rowoff, nrows, scantmp, stb, stream));
-      RCK(d2h(&npairs, rowoff + nrows, sizeof(npairs)));
-      if (round == 1 && use_k) RCK(initial_k<T>(nuv1, nuv2, r1, r2, iu, iv, res));
-      RCK(buf(nm("pairs", 1, cur).c_str(), std::max(npairs, 1ull), &pairs));
-      if (npairs > 0) {
-        HCK(launch_fill_u32(rowcursor, 0u, nrows, stream));
-        HCK(launch_isect_pass<T>(1, list, small, su1, su2, w1, w2, C1, R1, C2, R2, rowcount, rowoff, rowcursor, pairs,
+      if (fuse_cells && w1 <= kIcMaxNuv && w2 <= kIcMaxNuv) {
+        // units of at most 64 cells: hit masks per row, scan, pairs in order
+        const unsigned long long nrows = (unsigned long long)nslots * w1 * w1;
+        unsigned* rowcount;
+        unsigned long long *rowoff, *masks;
+        void* scantmp;
+        RCK(buf("i.rowcount", nrows, &rowcount));
+        RCK(buf("i.rowoff", nrows + 1, &rowoff));
+        RCK(buf("i.masks", nrows, &masks));
+        const size_t stb = scan_tmp_bytes(nrows);
+        RCK(buf_raw("i.scantmp", stb, &scantmp, 0));
+        HCK(launch_isect_masks<T>(nslots, w1, w2, su1, su2, C1, R1, C2, R2, lo1, hi1, lo2, hi2, P.cull, masks,
+                                  rowcount, stream));
+        HCK(scan_u32_to_u64(rowcount, rowoff, nrows, scantmp, stb, stream));
+        RCK(d2h(&npairs, rowoff + nrows, sizeof(npairs)));
+        if (round == 1 && use_k) RCK(initial_k<T>(nuv1, nuv2, r1, r2, iu, iv, res));
+        RCK(buf(nm("pairs", 1, cur).c_str(), std::max(npairs, 1ull), &pairs));
+        if (npairs > 0) HCK(launch_isect_emit(nslots, w1, masks, rowoff, pairs, stream));
+      } else {
+        uint64_t* list;
+        unsigned long long* small;
+        RCK(buf("i.list", ntp, &list));
+        RCK(buf("i.small", 8, &small));
+        HCK(launch_fill_u32(reinterpret_cast<unsigned*>(small), 0u, 2, stream));
+        HCK(launch_isect_list<T>((int)nslots, su1, su2, w1, w2, lo1, hi1, lo2, hi2, P.cull, list, small, ntp, stream));
+        const unsigned long long nrows = (unsigned long long)nslots * w1 * w1;
+        unsigned *rowcount, *rowcursor;
+        unsigned long long* rowoff;
+        void* scantmp;
+        RCK(buf("i.rowcount", nrows, &rowcount));
+        RCK(buf("i.rowcursor", nrows, &rowcursor));
+        RCK(buf("i.rowoff", nrows + 1, &rowoff));
+        const size_t stb = scan_tmp_bytes(nrows);
+        RCK(buf_raw("i.scantmp", stb, &scantmp, 0));
+        HCK(launch_fill_u32(rowcount, 0u, nrows, stream));
+        HCK(launch_isect_pass<T>(0, list, small, su1, su2, w1, w2, C1, R1, C2, R2, rowcount, rowoff, rowcursor, nullptr,
                                  grid, stream));
-        if (npairs > 16ull * nrows) {  // long rows: segmented radix sort
-          const size_t tb = segmented_sort_tmp_bytes(npairs, nrows);
-          void* st;
-          RCK(buf_raw("i.segsort", tb, &st, 0));
-          HCK(segmented_sort_pairs(pairs, npairs, rowoff, nrows, st, tb, stream));
-        } else {
-          HCK(launch_row_sort(rowoff, nrows, pairs, stream));
+        HCK(scan_u32_to_u64(rowcount, rowoff, nrows, scantmp, stb, stream));
+        RCK(d2h(&npairs, rowoff + nrows, sizeof(npairs)));
+        if (round == 1 && use_k) RCK(initial_k<T>(nuv1, nuv2, r1, r2, iu, iv, res));
+        RCK(buf(nm("pairs", 1, cur).c_str(), std::max(npairs, 1ull), &pairs));
+        if (npairs > 0) {
+          HCK(launch_fill_u32(rowcursor, 0u, nrows, stream));
+          HCK(launch_isect_pass<T>(1, list, small, su1, su2, w1, w2, C1, R1, C2, R2, rowcount, rowoff, rowcursor, pairs,
+                                   grid, stream));
+          if (npairs > 16ull * nrows) {  // long rows: segmented radix sort
+            const size_t tb = segmented_sort_tmp_bytes(npairs, nrows);
+            void* st;
+            RCK(buf_raw("i.segsort", tb, &st, 0));
+            HCK(segmented_sort_pairs(pairs, npairs, rowoff, nrows, st, tb, stream));
+          } else {
+            HCK(launch_row_sort(rowoff, nrows, pairs, stream));
+          }
         }
       }
     }
@@ -1146,8 +1173,50 @@ int jsi_ctx::intersect(int h1, const jsi_region& r1, int h2, const jsi_region& r
     RCK(buf(nm("R", 1, nxt).c_str(), (size_t)nu1 * wr * wr, &nR1));
     RCK(buf(nm("C", 2, nxt).c_str(), (size_t)nu2 * wr * wr, &nC2));
     RCK(buf(nm("R", 2, nxt).c_str(), (size_t)nu2 * wr * wr, &nR2));
-    RCK(isect_units<T>(h1, nreg1, (int)nu1, g_nuv, use_k, iu[1], iv[1], nC1, nR1, "i1"));
-    RCK(isect_units<T>(h2, nreg2, (int)nu2, g_nuv, use_k, iu[2], iv[2], nC2, nR2, "i2"));
+    tlo1 = thi1 = tlo2 = thi2 = nullptr;
+    if (fuse_cells && g_nuv >= 2 && g_nuv <= kIcMaxNuv) {
+      IcJob<T> j1, j2;
+      RCK(surf_dev<T>(h1, &j1.sd));
+      RCK(surf_dev<T>(h2, &j2.sd));
+      j1.reg = nreg1;
+      j2.reg = nreg2;
+      j1.nunits = (int)nu1;
+      j2.nunits = (int)nu2;
+      j1.nuv = j2.nuv = g_nuv;
+      j1.init_u = iu[1];
+      j1.init_v = iv[1];
+      j2.init_u = iu[2];
+      j2.init_v = iv[2];
+      j1.C = nC1;
+      j1.R = nR1;
+      j2.C = nC2;
+      j2.R = nR2;
+      if (!s32) {  // the FP32 screen builds its own boxes from its float records
+        RCK(buf("tb.lo1", (size_t)nu1, &tlo1));
+        RCK(buf("tb.hi1", (size_t)nu1, &thi1));
+        RCK(buf("tb.lo2", (size_t)nu2, &tlo2));
+        RCK(buf("tb.hi2", (size_t)nu2, &thi2));
+        j1.lo = tlo1;
+        j1.hi = thi1;
+        j2.lo = tlo2;
+        j2.hi = thi2;
+      }
+      if (prof) HCK(hipEventRecord(ev[0], stream));
+      HCK(launch_isect_cells<T>(j1, &j2, use_k, stream));
+      if (prof) {
+        HCK(hipEventRecord(ev[1], stream));
+        HCK(hipEventSynchronize(ev[1]));
+        float ms = 0.f;
+        HCK(hipEventElapsedTime(&ms, ev[0], ev[1]));
+        KernelStat& kst = stats[std::is_same<T, float>::value ? "isect_cells_f32" : "isect_cells_f64"];
+        kst.launches += 1;
+        kst.ms += ms;
+        kst.units += (double)(nu1 + nu2);
+      }
+    } else {
+      RCK(isect_units<T>(h1, nreg1, (int)nu1, g_nuv, use_k, iu[1], iv[1], nC1, nR1, "i1"));
+      RCK(isect_units<T>(h2, nreg2, (int)nu2, g_nuv, use_k, iu[2], iv[2], nC2, nR2, "i2"));
+    }
     HCK(hipStreamSynchronize(stream));
     res->ms_eval += now_ms() - te;
     cur = nxt;
