@@ -13,7 +13,9 @@
 // of two lanes per unit, nuv^2 of them holding a point each) and carries
 // them from their regions to their cells in registers and LDS:
 //   1. spans and A2.3 rows (derivatives up to 2) of the units' columns and
-//      rows, one per lane (knots in LDS when KL);
+//      rows, one per lane (the knots read through the L1 cache: staging
+//      them in LDS would cost a load round trip, a barrier and the LDS that
+//      limits the resident wavefronts);
 //   2. the u-contractions of every (column, control row) of a unit, by the
 //      unit's lanes, all u orders from one read of each control point;
 //   3. each point adds its q+1 rows in s order for every (k, l) and applies
@@ -54,22 +56,17 @@ __host__ __device__ inline int ic_units_per_wave(int nuv) {
 }
 
 // control rows per chunk of u-contractions: a degree-3 unit inside one knot
-// span reads 4 rows, one across a knot 5
-template <int MAXD>
-struct IcRows {
-  static constexpr int value = MAXD <= 3 ? MAXD + 2 : 4;
-};
+// span reads 4 rows (one across a knot takes a second chunk)
+constexpr int kIcRows = 4;
 
-template <class T, int MAXD, bool RAT, bool KL, bool BIC, bool PADK>
+template <class T, int MAXD, bool RAT, bool BIC, bool PADK>
 __global__ __launch_bounds__(64) void k_isect_cells(IcJob<T> j0, IcJob<T> j1, int nb0) {
   constexpr int NB = MAXD + 1;
   constexpr int NC = RAT ? 4 : 3;
   constexpr int NK = PADK ? 3 : 1;  // derivative orders kept per direction
-  constexpr int kRows = IcRows<MAXD>::value;
-  constexpr int kKn = KL ? kEcKnots : 1;
+  constexpr int kRows = kIcRows;
   constexpr int kRsz = kIcMaxCols * kRows * NK * NC;
   constexpr int kA = kRsz > 3 * 64 ? kRsz : 3 * 64;
-  __shared__ T sKu[kKn], sKv[kKn];
   __shared__ T sN[2 * kIcMaxCols][NK][NB];  // A2.3 rows: columns [0, 16), rows [16, 32)
   __shared__ int sS[2 * kIcMaxCols];        // their spans
   // two lives: u-contractions [column][row][k][c] (phases 2-3), then the
@@ -95,12 +92,6 @@ __global__ __launch_bounds__(64) void k_isect_cells(IcJob<T> j0, IcJob<T> j1, in
   const int ku = PADK ? du : 0;  // u orders formed
   const int lv = PADK ? dv : 0;  // v orders formed
   const int ncol = upw * nuv;    // columns (and rows) of the wavefront's units
-  // phase 0: knot vectors in LDS
-  if (KL && !sd.zero) {
-    for (int x = lane; x < sd.nku; x += 64) sKu[x] = sd.U[x];
-    for (int x = lane; x < sd.nkv; x += 64) sKv[x] = sd.V[x];
-  }
-  __syncthreads();
   // phase 1: spans and basis rows; the parameter follows the grid driver
   // (evaluation.cu:366) on the unit's own nuv x nuv grid
   if (!sd.zero && lane < 2 * ncol) {
@@ -113,7 +104,7 @@ __global__ __launch_bounds__(64) void k_isect_cells(IcJob<T> j0, IcJob<T> j1, in
       const T step = ((dir ? r.w : r.z) - start) / T(nuv);
       const T tt = step * T(idx) + start;
       const int deg = dir ? q : p, der = PADK ? (dir ? dv : du) : 0;
-      const T* knots = KL ? (dir ? sKv : sKu) : (dir ? sd.V : sd.U);
+      const T* knots = dir ? sd.V : sd.U;
       const int span = find_span(dir ? sd.nkv : sd.nku, deg, tt, knots);
       const int slot = (dir ? kIcMaxCols : 0) + t;
       if (BIC) {  // degree 3 both ways: the register-resident recurrence
@@ -417,53 +408,40 @@ int ic_blocks(const IcJob<T>& j) {
 }
 
 // variant of a job: 0-2 = degree class (3, 7, kMaxDeg - 1), +3 rational,
-// +6 knots in LDS, +12 bicubic (degree class 0 only)
+// +6 bicubic (degree class 0 only)
 template <class T>
 int ic_variant(const IcJob<T>& j) {
   const int d = std::max(j.sd.degu, j.sd.degv);
   const int dc = d <= 3 ? 0 : d <= 7 ? 1 : 2;
-  const bool kl = j.sd.nku <= kEcKnots && j.sd.nkv <= kEcKnots;
   const bool bic = j.sd.degu == 3 && j.sd.degv == 3 && !j.sd.zero;
-  return dc + (j.sd.rational ? 3 : 0) + (kl ? 6 : 0) + (bic ? 12 : 0);
+  return dc + (j.sd.rational ? 3 : 0) + (bic ? 6 : 0);
 }
 
-template <class T, int MAXD, bool RAT, bool KL, bool BIC>
+template <class T, int MAXD, bool RAT, bool BIC>
 void ic_launch_k(int use_k, const IcJob<T>& j0, const IcJob<T>& j1, int nb0, int nb, hipStream_t s) {
   const dim3 g((unsigned)nb), b(64);
   if (use_k)
-    hipLaunchKernelGGL((k_isect_cells<T, MAXD, RAT, KL, BIC, true>), g, b, 0, s, j0, j1, nb0);
+    hipLaunchKernelGGL((k_isect_cells<T, MAXD, RAT, BIC, true>), g, b, 0, s, j0, j1, nb0);
   else
-    hipLaunchKernelGGL((k_isect_cells<T, MAXD, RAT, KL, BIC, false>), g, b, 0, s, j0, j1, nb0);
+    hipLaunchKernelGGL((k_isect_cells<T, MAXD, RAT, BIC, false>), g, b, 0, s, j0, j1, nb0);
 }
 
 template <class T, int MAXD>
 void ic_launch_rat(int var, int use_k, const IcJob<T>& j0, const IcJob<T>& j1, int nb0, int nb, hipStream_t s) {
-  const bool rat = (var / 3) % 2 == 1, kl = (var / 6) % 2 == 1, bic = var >= 12;
-  if (MAXD == 3 && bic) {
-    if (rat) {
-      if (kl)
-        ic_launch_k<T, MAXD, true, true, true>(use_k, j0, j1, nb0, nb, s);
+  const bool rat = (var / 3) % 2 == 1, bic = var >= 6;
+  if constexpr (MAXD == 3) {
+    if (bic) {
+      if (rat)
+        ic_launch_k<T, MAXD, true, true>(use_k, j0, j1, nb0, nb, s);
       else
-        ic_launch_k<T, MAXD, true, false, true>(use_k, j0, j1, nb0, nb, s);
-    } else {
-      if (kl)
-        ic_launch_k<T, MAXD, false, true, true>(use_k, j0, j1, nb0, nb, s);
-      else
-        ic_launch_k<T, MAXD, false, false, true>(use_k, j0, j1, nb0, nb, s);
+        ic_launch_k<T, MAXD, false, true>(use_k, j0, j1, nb0, nb, s);
+      return;
     }
-    return;
   }
-  if (rat) {
-    if (kl)
-      ic_launch_k<T, MAXD, true, true, false>(use_k, j0, j1, nb0, nb, s);
-    else
-      ic_launch_k<T, MAXD, true, false, false>(use_k, j0, j1, nb0, nb, s);
-  } else {
-    if (kl)
-      ic_launch_k<T, MAXD, false, true, false>(use_k, j0, j1, nb0, nb, s);
-    else
-      ic_launch_k<T, MAXD, false, false, false>(use_k, j0, j1, nb0, nb, s);
-  }
+  if (rat)
+    ic_launch_k<T, MAXD, true, false>(use_k, j0, j1, nb0, nb, s);
+  else
+    ic_launch_k<T, MAXD, false, false>(use_k, j0, j1, nb0, nb, s);
 }
 
 template <class T>

@@ -2733,7 +2733,7 @@ __global__ __launch_bounds__(1024) void k_scan_apply(const IN* in, unsigned long
     s[threadIdx.x] += x;
     __syncthreads();
   }
-  unsigned long long run = block_off[blockIdx.x] + s[threadIdx.x] - t;
+  unsigned long long run = (block_off ? block_off[blockIdx.x] : 0ull) + s[threadIdx.x] - t;
   for (int e = 0; e < 4; ++e) {
     const unsigned long long i = base + threadIdx.x * 4 + e;
     if (i < n) out[i] = run;
@@ -3588,16 +3588,15 @@ template <class IN>
 static hipError_t scan_rec(const IN* in, unsigned long long* out, unsigned long long n, unsigned long long* tmp,
                            hipStream_t s) {
   const unsigned long long nb = (n + kScanBlock - 1) / kScanBlock;
+  if (nb == 1) {  // one block: the scan and the total in one launch
+    hipLaunchKernelGGL(k_scan_apply<IN>, dim3(1), dim3(1024), 0, s, in, n, nullptr, out);
+    return hipGetLastError();
+  }
   unsigned long long* sums = tmp;
   unsigned long long* offs = tmp + nb + 1;
   hipLaunchKernelGGL(k_scan_reduce<IN>, dim3((unsigned)nb), dim3(1024), 0, s, in, n, sums);
-  if (nb == 1) {  // a fill kernel, not a memset node: the scan may run under graph capture
-    hipError_t e = launch_fill_u32(reinterpret_cast<unsigned*>(offs), 0u, 2, s);
-    if (e != hipSuccess) return e;
-  } else {
-    hipError_t e = scan_rec<unsigned long long>(sums, offs, nb, tmp + 2 * (nb + 1), s);
-    if (e != hipSuccess) return e;
-  }
+  hipError_t e = scan_rec<unsigned long long>(sums, offs, nb, tmp + 2 * (nb + 1), s);
+  if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_scan_apply<IN>, dim3((unsigned)nb), dim3(1024), 0, s, in, n, offs, out);
   return hipGetLastError();
 }
