@@ -5,6 +5,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "jsi_internal.hpp"
+
 namespace jsi {
 
 // std::min / std::max: the first argument wins unless the second is strictly
@@ -57,6 +59,23 @@ __device__ __forceinline__ double infinity_v<double>() {
 __device__ __forceinline__ int out_row(int k, int l) {
   const int m = (l == 0) ? k : (l == 1 ? 3 + k : 5);
   return m == 2 ? 3 : (m == 3 ? 2 : m);
+}
+
+// Region of cell `bbox` (id = y * (pnuv - 1) + x) of a unit sampled on a
+// pnuv x pnuv grid: the refinement task of that cell (make_dist_refine_task,
+// distance_analysis.cu:340-354; the same split in intersection.cu:388-420).
+template <class T>
+__device__ __forceinline__ vec4_t<T> child_region(const vec4_t<T>& pr, int pnuv, int bbox) {
+  const int w = pnuv - 1;
+  const T ustep = (pr.z - pr.x) / T(pnuv);
+  const T vstep = (pr.w - pr.y) / T(pnuv);
+  const int x = bbox % w, y = bbox / w;
+  vec4_t<T> c;
+  c.x = pr.x + ustep * T(x);
+  c.z = c.x + ustep;
+  c.y = pr.y + vstep * T(y);
+  c.w = c.y + vstep;
+  return c;
 }
 
 constexpr int kEcKnots = 128;  // knots per direction staged in LDS (when both vectors fit)

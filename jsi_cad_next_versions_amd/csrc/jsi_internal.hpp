@@ -253,7 +253,7 @@ hipError_t launch_isect_dedup_flags(const IPair* pairs, unsigned long long n, in
                                     unsigned* flag1, unsigned* flag2, hipStream_t s);
 template <class T>
 hipError_t launch_isect_units(const IPair* pairs, unsigned long long n, int cells2, const int* su1, const int* su2,
-                              const vec4_t<T>* reg1, const vec4_t<T>* reg2, int parent_nuv, int parent_w,
+                              const vec4_t<T>* reg1, const vec4_t<T>* reg2, int parent_nuv1, int parent_nuv2,
                               const unsigned* flag1, const unsigned* flag2, const unsigned long long* idx1,
                               const unsigned long long* idx2, int* firstpos2, int* nsu1, int* nsu2,
                               vec4_t<T>* nreg1, vec4_t<T>* nreg2, hipStream_t s);
@@ -274,17 +274,30 @@ struct IcJob {
   vec4_t<T>* lo = nullptr;
   vec4_t<T>* hi = nullptr;
 };
-// Overlap screen of a round whose side-1 units have nuv <= kIcMaxNuv and
-// side-2 units at most 64 cells: a hit mask and count per row (slot, i)
-// (cull: slots whose unit boxes lo/hi are disjoint have none); after the
-// scan of the counts, the pairs in (slot, i, j) order.
+// Overlap screen of a round whose units have at most kIcMaxNuv^2 cells per
+// side, per slot (jsi_isect_cells.hip).  Count: per slot its pairs, its
+// non-empty rows and its distinct side-2 cells (cull: slots whose unit boxes
+// lo/hi are disjoint have none).  Emit, given the exclusive scans of the
+// three counts: the pairs in (slot, i, j) order and, with nsu1 non-null, the
+// next round's units (per pair its units, per unit its region: cells of the
+// parents reg1 / reg2 on pnuv1 / pnuv2 grids), numbered by first appearance.
 template <class T>
-hipError_t launch_isect_masks(long long nslots, int w1, int w2, const int* su1, const int* su2, const vec4_t<T>* C1,
-                              const vec4_t<T>* R1, const vec4_t<T>* C2, const vec4_t<T>* R2, const vec4_t<T>* lo1,
-                              const vec4_t<T>* hi1, const vec4_t<T>* lo2, const vec4_t<T>* hi2, int cull,
-                              unsigned long long* masks, unsigned* rowcount, hipStream_t s);
-hipError_t launch_isect_emit(long long nslots, int w1, const unsigned long long* masks,
-                             const unsigned long long* rowoff, IPair* out, hipStream_t s);
+hipError_t launch_isect_slot_count(long long nslots, int w1, int w2, const int* su1, const int* su2,
+                                   const vec4_t<T>* C1, const vec4_t<T>* R1, const vec4_t<T>* C2,
+                                   const vec4_t<T>* R2, const vec4_t<T>* lo1, const vec4_t<T>* hi1,
+                                   const vec4_t<T>* lo2, const vec4_t<T>* hi2, int cull, unsigned* npair,
+                                   unsigned* nrow, unsigned* ncol, hipStream_t s);
+template <class T>
+hipError_t launch_isect_slot_emit(long long nslots, int w1, int w2, const int* su1, const int* su2,
+                                  const vec4_t<T>* C1, const vec4_t<T>* R1, const vec4_t<T>* C2, const vec4_t<T>* R2,
+                                  const vec4_t<T>* lo1, const vec4_t<T>* hi1, const vec4_t<T>* lo2,
+                                  const vec4_t<T>* hi2, int cull, const unsigned long long* offp,
+                                  const unsigned long long* offr, const unsigned long long* offc, IPair* out,
+                                  const vec4_t<T>* reg1, const vec4_t<T>* reg2, int pnuv1, int pnuv2, int* nsu1,
+                                  int* nsu2, vec4_t<T>* nreg1, vec4_t<T>* nreg2, hipStream_t s);
+// out[0..2] = a[n], b[n], c[n] (the totals of three scans, one copy back)
+hipError_t launch_gather_totals(const unsigned long long* a, const unsigned long long* b,
+                                const unsigned long long* c, long long n, unsigned long long* out, hipStream_t s);
 // b may be null; two jobs of the same kernel variant share one launch.
 template <class T>
 hipError_t launch_isect_cells(const IcJob<T>& a, const IcJob<T>* b, int use_k, hipStream_t s);

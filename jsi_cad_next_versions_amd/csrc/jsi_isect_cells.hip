@@ -342,62 +342,176 @@ __global__ __launch_bounds__(64) void k_isect_cells(IcJob<T> j0, IcJob<T> j1, in
   }
 }
 
-// Overlap screen of a round whose units have at most 64 cells per side
-// (nuv <= 9, one tile per unit): one lane per row (slot, i) tests cell i of
-// the slot's side-1 unit against every cell j of its side-2 unit with the
-// strict 3-axis overlap test (overlap_aabb_host, intersection.cu:46-63) and
-// keeps the hits as a bit mask, j ascending; rowcount = its population.  A
-// slot whose unit boxes are disjoint has no hits (the tile-pair cull of
-// k_isect_list with one tile per unit).  The rows' offsets are a scan of the
-// counts, and k_isect_emit writes each row's pairs in j order, so the list
-// comes out in (slot, i, j) order with no atomics and no row sort.
+// ---------------------------------------------------------------------------
+// Overlap screen and refinement bookkeeping of a round whose units have at
+// most 64 cells per side (nuv <= 9, one tile per unit), per slot.
+//
+// After a dense round the next one has as many slots as the dense round had
+// pairs (10^7 after a 64 x 64 first grid whose padded cells mostly overlap),
+// nearly all of them without a hit, so nothing here is sized per row (slot,
+// i).  A group of lanes holds one slot, a lane per side-1 cell i (cells1
+// rounded up to a power of two lanes); a lane's hits over the side-2 cells j
+// form a bit mask, j ascending (the strict 3-axis test of overlap_aabb_host,
+// intersection.cu:46-63; a slot whose unit boxes are disjoint has none, the
+// tile-pair cull of k_isect_list with one tile per unit).  k_isect_slot_count
+// writes three counts per slot: its pairs, its non-empty rows (the side-1
+// units it creates) and its distinct j (its side-2 units).  After their
+// scans, k_isect_slot_emit recomputes the masks and writes the slot's pairs
+// in (i, j) order at the slot's offset, so the list is in (slot, i, j) order
+// with no atomics and no sort, and, when a round follows, the refinement
+// units: units are numbered by first appearance in pair order, (slot, i) on
+// side 1 and (slot, j) on side 2 (the reuse maps of intersection.cu:384-446),
+// so a side-1 unit is its slot's offset plus the non-empty rows before it,
+// and a side-2 unit its slot's offset plus the distinct j of the rows before
+// its first row plus the new j below it in that row.
+// ---------------------------------------------------------------------------
 template <class T>
-__global__ __launch_bounds__(256) void k_isect_masks(long long nrows, int cells1, int cells2, const int* su1,
-                                                     const int* su2, const vec4_t<T>* C1, const vec4_t<T>* R1,
-                                                     const vec4_t<T>* C2, const vec4_t<T>* R2, const vec4_t<T>* lo1,
-                                                     const vec4_t<T>* hi1, const vec4_t<T>* lo2,
-                                                     const vec4_t<T>* hi2, int cull, unsigned long long* masks,
-                                                     unsigned* rowcount) {
-  const long long row = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (row >= nrows) return;
-  const int slot = (int)(row / cells1), i = (int)(row % cells1);
-  const int u1 = su1[slot], u2 = su2[slot];
-  bool keep = true;
-  if (cull) {
-    const vec4_t<T> l1 = lo1[u1], h1 = hi1[u1], l2 = lo2[u2], h2 = hi2[u2];
-    keep = l1.x <= h2.x && l2.x <= h1.x && l1.y <= h2.y && l2.y <= h1.y && l1.z <= h2.z && l2.z <= h1.z;
-  }
-  unsigned long long m = 0;
-  if (keep) {
-    const vec4_t<T> c1 = C1[(size_t)u1 * cells1 + i], r1 = R1[(size_t)u1 * cells1 + i];
-    const vec4_t<T>* const c2p = C2 + (size_t)u2 * cells2;
-    const vec4_t<T>* const r2p = R2 + (size_t)u2 * cells2;
-    for (int j = 0; j < cells2; ++j) {
-      const vec4_t<T> c2 = c2p[j], r2 = r2p[j];
-      const bool hit = fabs(c1.x - c2.x) < (r1.x + r2.x) && fabs(c1.y - c2.y) < (r1.y + r2.y) &&
-                       fabs(c1.z - c2.z) < (r1.z + r2.z);
-      m |= (unsigned long long)hit << j;
-    }
-  }
-  masks[row] = m;
-  rowcount[row] = (unsigned)__popcll(m);
+struct SlotScreen {
+  long long nslots;
+  int cells1, cells2;
+  const int *su1, *su2;
+  const vec4_t<T>*C1, *R1, *C2, *R2;
+  const vec4_t<T>*lo1, *hi1, *lo2, *hi2;  // unit boxes (cull)
+  int cull;
+};
+template <class T>
+struct SlotUnits {
+  const vec4_t<T>*reg1, *reg2;  // the slots' parent regions
+  int pnuv1, pnuv2;             // their grids
+  int *nsu1, *nsu2;             // per pair: its units
+  vec4_t<T>*nreg1, *nreg2;      // per unit: its region
+};
+
+__host__ __device__ inline int slot_lanes(int cells1) {
+  int l = 1;
+  while (l < cells1) l <<= 1;
+  return l;
+}
+// slots per wavefront; the emit pass keeps cells2 unit ids per slot in LDS,
+// at most 1024 per wavefront
+__host__ __device__ inline int slot_groups(int cells1, int cells2) {
+  const int a = 64 / slot_lanes(cells1), b = 1024 / cells2;
+  return a < b ? a : (b > 0 ? b : 1);
 }
 
-__global__ __launch_bounds__(256) void k_isect_emit(long long nrows, int cells1, const unsigned long long* masks,
-                                                    const unsigned long long* rowoff, IPair* out) {
-  const long long row = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (row >= nrows) return;
-  unsigned long long m = masks[row];
-  if (!m) return;
-  unsigned long long o = rowoff[row];
-  IPair pr;
-  pr.slot = (int)(row / cells1);
-  pr.i = (int)(row % cells1);
-  while (m) {
-    pr.j = __ffsll((long long)m) - 1;
-    m &= m - 1;
-    out[o++] = pr;
+template <class T>
+__device__ __forceinline__ unsigned long long slot_row_mask(const SlotScreen<T>& s, long long slot, int i) {
+  const int u1 = s.su1[slot], u2 = s.su2[slot];
+  if (s.cull) {
+    const vec4_t<T> l1 = s.lo1[u1], h1 = s.hi1[u1], l2 = s.lo2[u2], h2 = s.hi2[u2];
+    if (!(l1.x <= h2.x && l2.x <= h1.x && l1.y <= h2.y && l2.y <= h1.y && l1.z <= h2.z && l2.z <= h1.z)) return 0;
   }
+  const vec4_t<T> c1 = s.C1[(size_t)u1 * s.cells1 + i], r1 = s.R1[(size_t)u1 * s.cells1 + i];
+  const vec4_t<T>* const c2p = s.C2 + (size_t)u2 * s.cells2;
+  const vec4_t<T>* const r2p = s.R2 + (size_t)u2 * s.cells2;
+  unsigned long long m = 0;
+  for (int j = 0; j < s.cells2; ++j) {
+    const vec4_t<T> c2 = c2p[j], r2 = r2p[j];
+    const bool hit = fabs(c1.x - c2.x) < (r1.x + r2.x) && fabs(c1.y - c2.y) < (r1.y + r2.y) &&
+                     fabs(c1.z - c2.z) < (r1.z + r2.z);
+    m |= (unsigned long long)hit << j;
+  }
+  return m;
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_isect_slot_count(SlotScreen<T> s, unsigned* npair, unsigned* nrow,
+                                                          unsigned* ncol) {
+  const int G = slot_lanes(s.cells1), spw = slot_groups(s.cells1, s.cells2);
+  const long long nwaves = (s.nslots + spw - 1) / spw;
+  const int lane = threadIdx.x & 63, grp = lane / G, i = lane % G;
+  for (long long base = (long long)blockIdx.x * 4; base < nwaves; base += (long long)gridDim.x * 4) {
+    const long long slot = (base + (threadIdx.x >> 6)) * spw + grp;
+    const bool valid = grp < spw && slot < s.nslots && i < s.cells1;
+    const unsigned long long m = valid ? slot_row_mask(s, slot, i) : 0ull;
+    unsigned pc = (unsigned)__popcll(m), nz = m != 0;
+    unsigned long long om = m;
+    for (int off = G >> 1; off > 0; off >>= 1) {
+      pc += __shfl_xor(pc, off);
+      nz += __shfl_xor(nz, off);
+      om |= __shfl_xor(om, off);
+    }
+    if (valid && i == 0) {
+      npair[slot] = pc;
+      nrow[slot] = nz;
+      ncol[slot] = (unsigned)__popcll(om);
+    }
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_isect_slot_emit(SlotScreen<T> s, const unsigned long long* offp,
+                                                         const unsigned long long* offr,
+                                                         const unsigned long long* offc, IPair* out, int units,
+                                                         SlotUnits<T> u) {
+  extern __shared__ int sTab[];  // per wavefront: spw slots x cells2 side-2 unit ids
+  const int G = slot_lanes(s.cells1), spw = slot_groups(s.cells1, s.cells2);
+  const long long nwaves = (s.nslots + spw - 1) / spw;
+  const int lane = threadIdx.x & 63, grp = lane / G, i = lane % G;
+  int* const tab = sTab + ((threadIdx.x >> 6) * spw + (grp < spw ? grp : 0)) * s.cells2;
+  for (long long base = (long long)blockIdx.x * 4; base < nwaves; base += (long long)gridDim.x * 4) {
+    const long long slot = (base + (threadIdx.x >> 6)) * spw + grp;
+    const bool valid = grp < spw && slot < s.nslots && i < s.cells1;
+    const unsigned long long m = valid ? slot_row_mask(s, slot, i) : 0ull;
+    // inclusive scans over the group's rows: pairs, non-empty rows, j seen
+    unsigned pc = (unsigned)__popcll(m), nz = m != 0;
+    unsigned long long om = m;
+    for (int off = 1; off < G; off <<= 1) {
+      const unsigned a = __shfl_up(pc, off, G), b = __shfl_up(nz, off, G);
+      const unsigned long long c = __shfl_up(om, off, G);
+      if (i >= off) {
+        pc += a;
+        nz += b;
+        om |= c;
+      }
+    }
+    unsigned long long prior = __shfl_up(om, 1, G);  // j of the rows before
+    if (i == 0) prior = 0;
+    const unsigned long long p0 = valid ? offp[slot] + (pc - (unsigned)__popcll(m)) : 0ull;
+    int u1 = 0;
+    if (valid && m) {
+      IPair pr;
+      pr.slot = (int)slot;
+      pr.i = i;
+      unsigned long long p = p0;
+      for (unsigned long long r = m; r; r &= r - 1) {
+        pr.j = __ffsll((long long)r) - 1;
+        out[p++] = pr;
+      }
+      if (units) {
+        u1 = (int)(offr[slot] + (nz - 1));
+        u.nreg1[u1] = child_region<T>(u.reg1[s.su1[slot]], u.pnuv1, i);
+        const unsigned long long nw = m & ~prior;
+        const unsigned long long ub = offc[slot] + (unsigned)__popcll(prior);
+        const vec4_t<T> preg2 = u.reg2[s.su2[slot]];
+        for (unsigned long long r = nw; r; r &= r - 1) {
+          const int j = __ffsll((long long)r) - 1;
+          const int idx = (int)(ub + (unsigned)__popcll(nw & ((1ull << j) - 1)));
+          u.nreg2[idx] = child_region<T>(preg2, u.pnuv2, j);
+          tab[j] = idx;
+        }
+      }
+    }
+    if (units) {  // uniform: a kernel argument
+      __syncthreads();
+      if (valid && m) {
+        unsigned long long p = p0;
+        for (unsigned long long r = m; r; r &= r - 1) {
+          u.nsu1[p] = u1;
+          u.nsu2[p] = tab[__ffsll((long long)r) - 1];
+          ++p;
+        }
+      }
+      __syncthreads();  // the table is rewritten by the next slots
+    }
+  }
+}
+
+__global__ void k_gather_totals(const unsigned long long* a, const unsigned long long* b,
+                                const unsigned long long* c, long long n, unsigned long long* out) {
+  out[0] = a[n];
+  out[1] = b[n];
+  out[2] = c[n];
 }
 
 template <class T>
@@ -475,38 +589,99 @@ hipError_t launch_isect_cells(const IcJob<T>& a, const IcJob<T>* b, int use_k, h
 }
 
 template <class T>
-hipError_t launch_isect_masks(long long nslots, int w1, int w2, const int* su1, const int* su2, const vec4_t<T>* C1,
-                              const vec4_t<T>* R1, const vec4_t<T>* C2, const vec4_t<T>* R2, const vec4_t<T>* lo1,
-                              const vec4_t<T>* hi1, const vec4_t<T>* lo2, const vec4_t<T>* hi2, int cull,
-                              unsigned long long* masks, unsigned* rowcount, hipStream_t s) {
-  const int cells1 = w1 * w1, cells2 = w2 * w2;
-  const long long nrows = nslots * cells1;
-  if (nrows <= 0) return hipSuccess;
-  if (cells2 > 64 || w1 > kIcMaxNuv || (cull && (!lo1 || !hi1 || !lo2 || !hi2)) || nrows > (1ll << 40))
+static SlotScreen<T> slot_screen(long long nslots, int w1, int w2, const int* su1, const int* su2, const vec4_t<T>* C1,
+                                 const vec4_t<T>* R1, const vec4_t<T>* C2, const vec4_t<T>* R2, const vec4_t<T>* lo1,
+                                 const vec4_t<T>* hi1, const vec4_t<T>* lo2, const vec4_t<T>* hi2, int cull) {
+  SlotScreen<T> s;
+  s.nslots = nslots;
+  s.cells1 = w1 * w1;
+  s.cells2 = w2 * w2;
+  s.su1 = su1;
+  s.su2 = su2;
+  s.C1 = C1;
+  s.R1 = R1;
+  s.C2 = C2;
+  s.R2 = R2;
+  s.lo1 = lo1;
+  s.hi1 = hi1;
+  s.lo2 = lo2;
+  s.hi2 = hi2;
+  s.cull = cull;
+  return s;
+}
+
+static unsigned slot_blocks(long long nslots, int cells1, int cells2) {
+  const int spw = slot_groups(cells1, cells2);
+  const long long waves = (nslots + spw - 1) / spw;
+  return (unsigned)std::min<long long>((waves + 3) / 4, 1ll << 20);
+}
+
+template <class T>
+hipError_t launch_isect_slot_count(long long nslots, int w1, int w2, const int* su1, const int* su2,
+                                   const vec4_t<T>* C1, const vec4_t<T>* R1, const vec4_t<T>* C2,
+                                   const vec4_t<T>* R2, const vec4_t<T>* lo1, const vec4_t<T>* hi1,
+                                   const vec4_t<T>* lo2, const vec4_t<T>* hi2, int cull, unsigned* npair,
+                                   unsigned* nrow, unsigned* ncol, hipStream_t s) {
+  if (nslots <= 0) return hipSuccess;
+  if (w1 < 1 || w2 < 1 || w1 > kIcMaxNuv || w2 > kIcMaxNuv || nslots > INT_MAX ||
+      (cull && (!lo1 || !hi1 || !lo2 || !hi2)))
     return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_isect_masks<T>, dim3((unsigned)((nrows + 255) / 256)), dim3(256), 0, s, nrows, cells1, cells2,
-                     su1, su2, C1, R1, C2, R2, lo1, hi1, lo2, hi2, cull, masks, rowcount);
+  const SlotScreen<T> sc = slot_screen<T>(nslots, w1, w2, su1, su2, C1, R1, C2, R2, lo1, hi1, lo2, hi2, cull);
+  hipLaunchKernelGGL(k_isect_slot_count<T>, dim3(slot_blocks(nslots, sc.cells1, sc.cells2)), dim3(256), 0, s, sc,
+                     npair, nrow, ncol);
   return hipGetLastError();
 }
 
-hipError_t launch_isect_emit(long long nslots, int w1, const unsigned long long* masks,
-                             const unsigned long long* rowoff, IPair* out, hipStream_t s) {
-  const long long nrows = nslots * w1 * w1;
-  if (nrows <= 0) return hipSuccess;
-  if (nrows > (1ll << 40)) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_isect_emit, dim3((unsigned)((nrows + 255) / 256)), dim3(256), 0, s, nrows, w1 * w1, masks,
-                     rowoff, out);
+template <class T>
+hipError_t launch_isect_slot_emit(long long nslots, int w1, int w2, const int* su1, const int* su2,
+                                  const vec4_t<T>* C1, const vec4_t<T>* R1, const vec4_t<T>* C2, const vec4_t<T>* R2,
+                                  const vec4_t<T>* lo1, const vec4_t<T>* hi1, const vec4_t<T>* lo2,
+                                  const vec4_t<T>* hi2, int cull, const unsigned long long* offp,
+                                  const unsigned long long* offr, const unsigned long long* offc, IPair* out,
+                                  const vec4_t<T>* reg1, const vec4_t<T>* reg2, int pnuv1, int pnuv2, int* nsu1,
+                                  int* nsu2, vec4_t<T>* nreg1, vec4_t<T>* nreg2, hipStream_t s) {
+  if (nslots <= 0) return hipSuccess;
+  if (w1 < 1 || w2 < 1 || w1 > kIcMaxNuv || w2 > kIcMaxNuv || nslots > INT_MAX ||
+      (cull && (!lo1 || !hi1 || !lo2 || !hi2)))
+    return hipErrorInvalidValue;
+  const SlotScreen<T> sc = slot_screen<T>(nslots, w1, w2, su1, su2, C1, R1, C2, R2, lo1, hi1, lo2, hi2, cull);
+  SlotUnits<T> su;
+  su.reg1 = reg1;
+  su.reg2 = reg2;
+  su.pnuv1 = pnuv1;
+  su.pnuv2 = pnuv2;
+  su.nsu1 = nsu1;
+  su.nsu2 = nsu2;
+  su.nreg1 = nreg1;
+  su.nreg2 = nreg2;
+  const int units = nsu1 != nullptr;
+  if (units && (!nsu2 || !nreg1 || !nreg2 || !reg1 || !reg2)) return hipErrorInvalidValue;
+  const size_t lds = units ? sizeof(int) * 4 * slot_groups(sc.cells1, sc.cells2) * sc.cells2 : 0;
+  hipLaunchKernelGGL(k_isect_slot_emit<T>, dim3(slot_blocks(nslots, sc.cells1, sc.cells2)), dim3(256), lds, s, sc,
+                     offp, offr, offc, out, units, su);
   return hipGetLastError();
 }
 
-#define JSI_IM(T)                                                                                                  \
-  template hipError_t launch_isect_masks<T>(long long, int, int, const int*, const int*, const vec4_t<T>*,         \
-                                            const vec4_t<T>*, const vec4_t<T>*, const vec4_t<T>*, const vec4_t<T>*, \
-                                            const vec4_t<T>*, const vec4_t<T>*, const vec4_t<T>*, int,              \
-                                            unsigned long long*, unsigned*, hipStream_t);
-JSI_IM(float)
-JSI_IM(double)
-#undef JSI_IM
+hipError_t launch_gather_totals(const unsigned long long* a, const unsigned long long* b,
+                                const unsigned long long* c, long long n, unsigned long long* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_gather_totals, dim3(1), dim3(1), 0, s, a, b, c, n, out);
+  return hipGetLastError();
+}
+
+#define JSI_IS(T)                                                                                                  \
+  template hipError_t launch_isect_slot_count<T>(long long, int, int, const int*, const int*, const vec4_t<T>*,    \
+                                                 const vec4_t<T>*, const vec4_t<T>*, const vec4_t<T>*,             \
+                                                 const vec4_t<T>*, const vec4_t<T>*, const vec4_t<T>*,             \
+                                                 const vec4_t<T>*, int, unsigned*, unsigned*, unsigned*,            \
+                                                 hipStream_t);                                                      \
+  template hipError_t launch_isect_slot_emit<T>(                                                                   \
+      long long, int, int, const int*, const int*, const vec4_t<T>*, const vec4_t<T>*, const vec4_t<T>*,           \
+      const vec4_t<T>*, const vec4_t<T>*, const vec4_t<T>*, const vec4_t<T>*, const vec4_t<T>*, int,               \
+      const unsigned long long*, const unsigned long long*, const unsigned long long*, IPair*, const vec4_t<T>*,   \
+      const vec4_t<T>*, int, int, int*, int*, vec4_t<T>*, vec4_t<T>*, hipStream_t);
+JSI_IS(float)
+JSI_IS(double)
+#undef JSI_IS
 
 template hipError_t launch_isect_cells<float>(const IcJob<float>&, const IcJob<float>*, int, hipStream_t);
 template hipError_t launch_isect_cells<double>(const IcJob<double>&, const IcJob<double>*, int, hipStream_t);

@@ -2382,21 +2382,8 @@ __global__ void k_bsort_global(Cand* a, unsigned long long n2, unsigned long lon
 // JSI_CAD_PRESERVE_PAIR_RELATION=1): jobs are the sorted top-k pairs; each side
 // keeps one unit per distinct (slot, cell) in order of first appearance; slot k
 // of the next round is (unit of job k's side-1 endpoint, unit of its side-2
-// endpoint).  Child region: make_dist_refine_task (distance_analysis.cu:340-354).
+// endpoint).  Child region: child_region (jsi_device.hpp).
 // ---------------------------------------------------------------------------
-template <class T>
-__device__ __forceinline__ vec4_t<T> child_region(const vec4_t<T>& pr, int pnuv, int bbox) {
-  const int w = pnuv - 1;
-  const T ustep = (pr.z - pr.x) / T(pnuv);
-  const T vstep = (pr.w - pr.y) / T(pnuv);
-  const int x = bbox % w, y = bbox / w;
-  vec4_t<T> c;
-  c.x = pr.x + ustep * T(x);
-  c.z = c.x + ustep;
-  c.y = pr.y + vstep * T(y);
-  c.w = c.y + vstep;
-  return c;
-}
 
 constexpr int kDedupMax = 8192;
 constexpr int kDedupPer = kDedupMax / 1024;  // records per thread of k_dedup_units
@@ -2771,7 +2758,7 @@ __global__ void k_isect_flags2(const IPair* pairs, unsigned long long n, int cel
 
 template <class T>
 __global__ void k_isect_units(const IPair* pairs, unsigned long long n, int cells2, const int* su1, const int* su2,
-                              const vec4_t<T>* reg1, const vec4_t<T>* reg2, int pnuv, const unsigned* flag1,
+                              const vec4_t<T>* reg1, const vec4_t<T>* reg2, int pnuv1, int pnuv2, const unsigned* flag1,
                               const unsigned* flag2, const unsigned long long* idx1,
                               const unsigned long long* idx2, const int* firstpos2, int* nsu1, int* nsu2,
                               vec4_t<T>* nreg1, vec4_t<T>* nreg2) {
@@ -2779,10 +2766,10 @@ __global__ void k_isect_units(const IPair* pairs, unsigned long long n, int cell
   if (p >= n) return;
   const IPair a = pairs[p];
   nsu1[p] = (int)(idx1[p] + flag1[p] - 1);
-  if (flag1[p]) nreg1[idx1[p]] = child_region<T>(reg1[su1[a.slot]], pnuv, a.i);
+  if (flag1[p]) nreg1[idx1[p]] = child_region<T>(reg1[su1[a.slot]], pnuv1, a.i);
   const int first = firstpos2[(size_t)a.slot * cells2 + a.j];
   nsu2[p] = (int)idx2[first];
-  if (flag2[p]) nreg2[idx2[p]] = child_region<T>(reg2[su2[a.slot]], pnuv, a.j);
+  if (flag2[p]) nreg2[idx2[p]] = child_region<T>(reg2[su2[a.slot]], pnuv2, a.j);
 }
 
 __global__ void k_isect_reset2(const IPair* pairs, unsigned long long n, int cells2, int* firstpos2) {
@@ -3620,15 +3607,14 @@ hipError_t launch_isect_dedup_flags(const IPair* pairs, unsigned long long n, in
 
 template <class T>
 hipError_t launch_isect_units(const IPair* pairs, unsigned long long n, int cells2, const int* su1, const int* su2,
-                              const vec4_t<T>* reg1, const vec4_t<T>* reg2, int parent_nuv, int parent_w,
+                              const vec4_t<T>* reg1, const vec4_t<T>* reg2, int parent_nuv1, int parent_nuv2,
                               const unsigned* flag1, const unsigned* flag2, const unsigned long long* idx1,
                               const unsigned long long* idx2, int* firstpos2, int* nsu1, int* nsu2,
                               vec4_t<T>* nreg1, vec4_t<T>* nreg2, hipStream_t s) {
-  (void)parent_w;
   if (n == 0) return hipSuccess;
   const dim3 g(blocks_for(n, 256)), b(256);
-  hipLaunchKernelGGL(k_isect_units<T>, g, b, 0, s, pairs, n, cells2, su1, su2, reg1, reg2, parent_nuv, flag1, flag2,
-                     idx1, idx2, firstpos2, nsu1, nsu2, nreg1, nreg2);
+  hipLaunchKernelGGL(k_isect_units<T>, g, b, 0, s, pairs, n, cells2, su1, su2, reg1, reg2, parent_nuv1, parent_nuv2,
+                     flag1, flag2, idx1, idx2, firstpos2, nsu1, nsu2, nreg1, nreg2);
   hipLaunchKernelGGL(k_isect_reset2, g, b, 0, s, pairs, n, cells2, firstpos2);
   return hipGetLastError();
 }

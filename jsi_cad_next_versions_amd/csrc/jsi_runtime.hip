@@ -1028,6 +1028,8 @@ int jsi_ctx::intersect(int h1, const jsi_region& r1, int h2, const jsi_region& r
   const char* icenv = std::getenv("JSI_ISECT_CELLS");
   const bool fuse_cells = !(icenv && std::strcmp(icenv, "0") == 0);
   V *tlo1 = nullptr, *thi1 = nullptr, *tlo2 = nullptr, *thi2 = nullptr;  // tile boxes of the current units
+  bool units_ready = false;  // the screen also wrote the next round's units
+  unsigned long long ready_nu1 = 0, ready_nu2 = 0;
   for (int round = 1;; ++round) {
     const double ts = now_ms();
     const int w1 = nuv1 - 1, w2 = nuv2 - 1;
@@ -1057,23 +1059,48 @@ int jsi_ctx::intersect(int h1, const jsi_region& r1, int h2, const jsi_region& r
         return JSI_ERR_CAPACITY;
       }
       if (fuse_cells && w1 <= kIcMaxNuv && w2 <= kIcMaxNuv) {
-        // units of at most 64 cells: hit masks per row, scan, pairs in order
-        const unsigned long long nrows = (unsigned long long)nslots * w1 * w1;
-        unsigned* rowcount;
-        unsigned long long *rowoff, *masks;
+        // units of at most 64 cells: per-slot counts of pairs and new units,
+        // their scans, then the ordered pairs and (when a round follows) the
+        // next round's units in one pass
+        const size_t ns = (size_t)nslots;
+        unsigned* cnt;
+        unsigned long long *off, *tot;
         void* scantmp;
-        RCK(buf("i.rowcount", nrows, &rowcount));
-        RCK(buf("i.rowoff", nrows + 1, &rowoff));
-        RCK(buf("i.masks", nrows, &masks));
-        const size_t stb = scan_tmp_bytes(nrows);
+        RCK(buf("i.scount", 3 * ns, &cnt));
+        RCK(buf("i.soff", 3 * (ns + 1), &off));
+        RCK(buf("i.stot", 3, &tot));
+        const size_t stb = scan_tmp_bytes(ns);
         RCK(buf_raw("i.scantmp", stb, &scantmp, 0));
-        HCK(launch_isect_masks<T>(nslots, w1, w2, su1, su2, C1, R1, C2, R2, lo1, hi1, lo2, hi2, P.cull, masks,
-                                  rowcount, stream));
-        HCK(scan_u32_to_u64(rowcount, rowoff, nrows, scantmp, stb, stream));
-        RCK(d2h(&npairs, rowoff + nrows, sizeof(npairs)));
+        unsigned *cp = cnt, *cr = cnt + ns, *cc = cnt + 2 * ns;
+        unsigned long long *op = off, *orr = off + ns + 1, *oc = off + 2 * (ns + 1);
+        HCK(launch_isect_slot_count<T>(nslots, w1, w2, su1, su2, C1, R1, C2, R2, lo1, hi1, lo2, hi2, P.cull, cp, cr,
+                                       cc, stream));
+        HCK(scan_u32_to_u64(cp, op, ns, scantmp, stb, stream));
+        HCK(scan_u32_to_u64(cr, orr, ns, scantmp, stb, stream));
+        HCK(scan_u32_to_u64(cc, oc, ns, scantmp, stb, stream));
+        HCK(launch_gather_totals(op, orr, oc, nslots, tot, stream));
+        unsigned long long t3[3];
+        RCK(d2h(t3, tot, sizeof(t3)));
+        npairs = t3[0];
         if (round == 1 && use_k) RCK(initial_k<T>(nuv1, nuv2, r1, r2, iu, iv, res));
         RCK(buf(nm("pairs", 1, cur).c_str(), std::max(npairs, 1ull), &pairs));
-        if (npairs > 0) HCK(launch_isect_emit(nslots, w1, masks, rowoff, pairs, stream));
+        const bool follows = round < P.isect_max_rounds && round < JSI_MAX_ROUNDS && npairs > 0 &&
+                             npairs <= (unsigned long long)INT_MAX;
+        int *nsu1 = nullptr, *nsu2 = nullptr;
+        V *nreg1 = nullptr, *nreg2 = nullptr;
+        if (follows) {
+          const int nxt = cur ^ 1;
+          RCK(buf(nm("su", 1, nxt).c_str(), npairs, &nsu1));
+          RCK(buf(nm("su", 2, nxt).c_str(), npairs, &nsu2));
+          RCK(buf(nm("reg", 1, nxt).c_str(), t3[1], &nreg1));
+          RCK(buf(nm("reg", 2, nxt).c_str(), t3[2], &nreg2));
+          units_ready = true;
+          ready_nu1 = t3[1];
+          ready_nu2 = t3[2];
+        }
+        if (npairs > 0)
+          HCK(launch_isect_slot_emit<T>(nslots, w1, w2, su1, su2, C1, R1, C2, R2, lo1, hi1, lo2, hi2, P.cull, op, orr,
+                                        oc, pairs, reg1, reg2, nuv1, nuv2, nsu1, nsu2, nreg1, nreg2, stream));
       } else {
         uint64_t* list;
         unsigned long long* small;
@@ -1140,33 +1167,43 @@ int jsi_ctx::intersect(int h1, const jsi_region& r1, int h2, const jsi_region& r
     }
     te = now_ms();
     const int nxt = cur ^ 1;
-    unsigned *flag1, *flag2;
-    unsigned long long *idx1, *idx2;
-    int* fp2;
-    RCK(buf("i.flag1", npairs, &flag1));
-    RCK(buf("i.flag2", npairs, &flag2));
-    RCK(buf("i.idx1", npairs + 1, &idx1));
-    RCK(buf("i.idx2", npairs + 1, &idx2));
-    RCK(buf_raw("i.fp2", sizeof(int) * (size_t)nslots * w2 * w2, reinterpret_cast<void**>(&fp2), INT_MAX));
-    HCK(launch_isect_dedup_flags(pairs, npairs, w2 * w2, fp2, flag1, flag2, stream));
-    const size_t stb2 = scan_tmp_bytes(npairs);
-    void* scantmp;
-    RCK(buf_raw("i.scantmp2", stb2, &scantmp, 0));
-    HCK(scan_u32_to_u64(flag1, idx1, npairs, scantmp, stb2, stream));
-    HCK(scan_u32_to_u64(flag2, idx2, npairs, scantmp, stb2, stream));
     unsigned long long nu1 = 0, nu2 = 0;
-    RCK(d2h(&nu1, idx1 + npairs, sizeof(nu1)));
-    RCK(d2h(&nu2, idx2 + npairs, sizeof(nu2)));
-    ri.units1 = (int64_t)nu1;
-    ri.units2 = (int64_t)nu2;
     int *nsu1, *nsu2;
     V *nreg1, *nreg2;
-    RCK(buf(nm("su", 1, nxt).c_str(), npairs, &nsu1));
-    RCK(buf(nm("su", 2, nxt).c_str(), npairs, &nsu2));
-    RCK(buf(nm("reg", 1, nxt).c_str(), nu1, &nreg1));
-    RCK(buf(nm("reg", 2, nxt).c_str(), nu2, &nreg2));
-    HCK(launch_isect_units<T>(pairs, npairs, w2 * w2, su1, su2, reg1, reg2, nuv1, w1, flag1, flag2, idx1, idx2, fp2,
-                              nsu1, nsu2, nreg1, nreg2, stream));
+    if (units_ready) {  // written by the screen
+      nu1 = ready_nu1;
+      nu2 = ready_nu2;
+      RCK(buf(nm("su", 1, nxt).c_str(), npairs, &nsu1));
+      RCK(buf(nm("su", 2, nxt).c_str(), npairs, &nsu2));
+      RCK(buf(nm("reg", 1, nxt).c_str(), nu1, &nreg1));
+      RCK(buf(nm("reg", 2, nxt).c_str(), nu2, &nreg2));
+      units_ready = false;
+    } else {
+      unsigned *flag1, *flag2;
+      unsigned long long *idx1, *idx2;
+      int* fp2;
+      RCK(buf("i.flag1", npairs, &flag1));
+      RCK(buf("i.flag2", npairs, &flag2));
+      RCK(buf("i.idx1", npairs + 1, &idx1));
+      RCK(buf("i.idx2", npairs + 1, &idx2));
+      RCK(buf_raw("i.fp2", sizeof(int) * (size_t)nslots * w2 * w2, reinterpret_cast<void**>(&fp2), INT_MAX));
+      HCK(launch_isect_dedup_flags(pairs, npairs, w2 * w2, fp2, flag1, flag2, stream));
+      const size_t stb2 = scan_tmp_bytes(npairs);
+      void* scantmp;
+      RCK(buf_raw("i.scantmp2", stb2, &scantmp, 0));
+      HCK(scan_u32_to_u64(flag1, idx1, npairs, scantmp, stb2, stream));
+      HCK(scan_u32_to_u64(flag2, idx2, npairs, scantmp, stb2, stream));
+      RCK(d2h(&nu1, idx1 + npairs, sizeof(nu1)));
+      RCK(d2h(&nu2, idx2 + npairs, sizeof(nu2)));
+      RCK(buf(nm("su", 1, nxt).c_str(), npairs, &nsu1));
+      RCK(buf(nm("su", 2, nxt).c_str(), npairs, &nsu2));
+      RCK(buf(nm("reg", 1, nxt).c_str(), nu1, &nreg1));
+      RCK(buf(nm("reg", 2, nxt).c_str(), nu2, &nreg2));
+      HCK(launch_isect_units<T>(pairs, npairs, w2 * w2, su1, su2, reg1, reg2, nuv1, nuv2, flag1, flag2, idx1, idx2,
+                                fp2, nsu1, nsu2, nreg1, nreg2, stream));
+    }
+    ri.units1 = (int64_t)nu1;
+    ri.units2 = (int64_t)nu2;
     const int wr = g_nuv - 1;
     V *nC1, *nR1, *nC2, *nR2;
     RCK(buf(nm("C", 1, nxt).c_str(), (size_t)nu1 * wr * wr, &nC1));

@@ -6,6 +6,8 @@ nuv must be identical.  The cases cover every unit grid from 2 x 2 to 8 x 8
 points (lane groups of 4 to 64, 1 to 8 units per wavefront), the compute_K
 padding on and off, the three precisions, degrees 2-5 and 11, rational and
 polynomial surfaces, and knot vectors longer than the on-chip copy."""
+import time
+
 import numpy as np
 import pytest
 
@@ -136,3 +138,66 @@ def test_small_first_round(ctx, orc, monkeypatch, nuv, prec):
     a, b = S.saddle_wave_pair()
     res = run_three(ctx, orc, a, b, nuv, monkeypatch, precision=prec)
     assert res.round[0].npairs > 0
+
+
+@pytest.mark.parametrize("nuv1,nuv2", [(40, 24), (9, 33)])
+@pytest.mark.parametrize("prec", ["exact32", "fp64"])
+def test_unequal_first_grids(ctx, orc, monkeypatch, nuv1, nuv2, prec):
+    """Different first grids on the two sides: each side's child regions
+    split its own parent's grid (make_intersect_refine_task uses the parent
+    AABB result's own nuv and width)."""
+    a, b = S.saddle_wave_pair()
+    r1, r2 = S.full_region(a, nuv1, 1), S.full_region(b, nuv2, 2)
+    p = default_params(trace=1, precision=prec)
+    res, pairs = ctx.intersection(a, r1, b, r2, p, cap=1 << 20)
+    monkeypatch.setenv("JSI_ISECT_CELLS", "0")
+    pres, ppairs = ctx.intersection(a, r1, b, r2, p, cap=1 << 20)
+    monkeypatch.delenv("JSI_ISECT_CELLS")
+    ores, opairs = orc.intersect(a, r1, b, r2, p, cap=1 << 20)
+    for other, opl in ((pres, ppairs), (ores, opairs)):
+        assert res.rounds_list() == other.rounds_list()
+        np.testing.assert_array_equal(pairs, opl)
+    assert res.rounds == 6 and res.round[1].units1 != res.round[1].units2
+
+
+def test_dense_first_round_ten_million_slots(ctx, monkeypatch):
+    """An uneven-workload pair whose 64 x 64 first grids overlap on 10.4M cell
+    pairs: the second round has 10.4M slots of 49 x 49 cells, almost all
+    without a hit (the per-slot screen keeps 12 bytes per slot)."""
+    kind, a, b, nuv = S.uneven_workload(37, 0)[36]
+    assert kind == "isect" and nuv == 64
+    r1, r2 = S.full_region(a, nuv, 1), S.full_region(b, nuv, 2)
+    p = default_params(trace=1, precision="fp64")
+    res, pairs = ctx.intersection(a, r1, b, r2, p, cap=1 << 16)
+    monkeypatch.setenv("JSI_ISECT_CELLS", "0")
+    pres, ppairs = ctx.intersection(a, r1, b, r2, p, cap=1 << 16)
+    monkeypatch.delenv("JSI_ISECT_CELLS")
+    assert res.rounds_list() == pres.rounds_list()
+    np.testing.assert_array_equal(pairs, ppairs)
+    assert [r["npairs"] for r in res.rounds_list()] == [10389328, 15836, 14360, 16715, 20375, 25402]
+
+
+def test_server_intersections_only(ctx):
+    """Every intersection query of the first 400 uneven-workload pairs (nuv
+    64-512, round-1 lists up to 10M pairs) queued at once on 8 server
+    workers: all succeed and equal the same query run alone."""
+    from jsi_cad_next_versions_amd._abi import QUERY_INTERSECT
+    from jsi_cad_next_versions_amd.runtime import Server
+    meta = [(a, b, nuv) for kind, a, b, nuv in S.uneven_workload(400, 0) if kind == "isect"]
+    p = default_params(precision="fp64", trace=1)
+    qs = [ctx.query(QUERY_INTERSECT, a, S.full_region(a, nuv, 1), b, S.full_region(b, nuv, 2), p)
+          for a, b, nuv in meta]
+    print(f"server: {len(qs)} intersection queries on 8 workers", flush=True)
+    with Server(ctx, slots=8) as srv:
+        tickets = []
+        for q in qs:
+            tk = srv.submit(q)
+            while tk is None:  # admission holds it back
+                time.sleep(1e-3)
+                tk = srv.submit(q)
+            tickets.append(tk)
+        results = [srv.poll(tk) for tk in tickets]
+    print("server: solo runs", flush=True)
+    for r, (a, b, nuv) in zip(results, meta):
+        s, _ = ctx.intersection(a, S.full_region(a, nuv, 1), b, S.full_region(b, nuv, 2), p)
+        assert r.rounds_list() == s.rounds_list()
