@@ -275,26 +275,36 @@ struct IcJob {
   vec4_t<T>* hi = nullptr;
 };
 // Overlap screen of a round whose units have at most kIcMaxNuv^2 cells per
-// side, per slot (jsi_isect_cells.hip).  Count: per slot its pairs, its
-// non-empty rows and its distinct side-2 cells (cull: slots whose unit boxes
-// lo/hi are disjoint have none).  Emit, given the exclusive scans of the
-// three counts: the pairs in (slot, i, j) order and, with nsu1 non-null, the
-// next round's units (per pair its units, per unit its region: cells of the
-// parents reg1 / reg2 on pnuv1 / pnuv2 grids), numbered by first appearance.
+// side, per slot (jsi_isect_cells.hip).  Cull (rounds with many slots): with
+// wcount, the number of slots whose unit boxes meet in each run of 64; with
+// woff (the exclusive scan of wcount) and list, those slots in order.  Count,
+// over n entries (slots 0..n-1, or list[0..n-1] when list is non-null): per
+// entry its pairs, its non-empty rows and its distinct side-2 cells (without
+// a list and with cull, slots whose unit boxes lo/hi are disjoint have none).
+// Emit, given the exclusive scans of the three counts: the pairs in (slot, i,
+// j) order and, with nsu1 non-null, the next round's units (per pair its
+// units, per unit its region: cells of the parents reg1 / reg2 on pnuv1 /
+// pnuv2 grids), numbered by first appearance.
+template <class T>
+hipError_t launch_isect_slot_cull(long long nslots, int w1, int w2, const int* su1, const int* su2,
+                                  const vec4_t<T>* lo1, const vec4_t<T>* hi1, const vec4_t<T>* lo2,
+                                  const vec4_t<T>* hi2, unsigned* wcount, const unsigned long long* woff, int* list,
+                                  hipStream_t s);
 template <class T>
 hipError_t launch_isect_slot_count(long long nslots, int w1, int w2, const int* su1, const int* su2,
                                    const vec4_t<T>* C1, const vec4_t<T>* R1, const vec4_t<T>* C2,
                                    const vec4_t<T>* R2, const vec4_t<T>* lo1, const vec4_t<T>* hi1,
-                                   const vec4_t<T>* lo2, const vec4_t<T>* hi2, int cull, unsigned* npair,
-                                   unsigned* nrow, unsigned* ncol, hipStream_t s);
+                                   const vec4_t<T>* lo2, const vec4_t<T>* hi2, int cull, long long n,
+                                   const int* list, unsigned* npair, unsigned* nrow, unsigned* ncol, hipStream_t s);
 template <class T>
 hipError_t launch_isect_slot_emit(long long nslots, int w1, int w2, const int* su1, const int* su2,
                                   const vec4_t<T>* C1, const vec4_t<T>* R1, const vec4_t<T>* C2, const vec4_t<T>* R2,
                                   const vec4_t<T>* lo1, const vec4_t<T>* hi1, const vec4_t<T>* lo2,
-                                  const vec4_t<T>* hi2, int cull, const unsigned long long* offp,
-                                  const unsigned long long* offr, const unsigned long long* offc, IPair* out,
-                                  const vec4_t<T>* reg1, const vec4_t<T>* reg2, int pnuv1, int pnuv2, int* nsu1,
-                                  int* nsu2, vec4_t<T>* nreg1, vec4_t<T>* nreg2, hipStream_t s);
+                                  const vec4_t<T>* hi2, int cull, long long n, const int* list,
+                                  const unsigned long long* offp, const unsigned long long* offr,
+                                  const unsigned long long* offc, IPair* out, const vec4_t<T>* reg1,
+                                  const vec4_t<T>* reg2, int pnuv1, int pnuv2, int* nsu1, int* nsu2,
+                                  vec4_t<T>* nreg1, vec4_t<T>* nreg2, hipStream_t s);
 // out[0..2] = a[n], b[n], c[n] (the totals of three scans, one copy back)
 hipError_t launch_gather_totals(const unsigned long long* a, const unsigned long long* b,
                                 const unsigned long long* c, long long n, unsigned long long* out, hipStream_t s);

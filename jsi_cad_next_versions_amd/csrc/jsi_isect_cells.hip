@@ -107,19 +107,10 @@ __global__ __launch_bounds__(64) void k_isect_cells(IcJob<T> j0, IcJob<T> j1, in
       const T* knots = dir ? sd.V : sd.U;
       const int span = find_span(dir ? sd.nkv : sd.nku, deg, tt, knots);
       const int slot = (dir ? kIcMaxCols : 0) + t;
-      if (BIC) {  // degree 3 both ways: the register-resident recurrence
-        T d[3][4];
-        basis_ders_p<T, 3>(span, tt, knots, der, d);
-#pragma unroll
-        for (int k = 0; k < NK; ++k)
-#pragma unroll
-          for (int j = 0; j <= 3; ++j) sN[slot][k][j] = k <= der ? d[k][j] : T(0);
-      } else {
-        T d[3][MAXD + 1];
-        basis_ders<T, MAXD>(span, deg, tt, knots, der, d);
-        for (int k = 0; k < NK; ++k)
-          for (int j = 0; j <= deg; ++j) sN[slot][k][j] = k <= der ? d[k][j] : T(0);
-      }
+      if (BIC)  // degree 3 both ways: one recurrence, no degree switch
+        basis_row_fixed<T, 3, NK, NB>(span, tt, knots, der, sN[slot]);
+      else
+        basis_row_any<T, MAXD, NK, NB>(span, deg, tt, knots, der, sN[slot]);
       sS[slot] = span;
     }
   }
@@ -349,11 +340,13 @@ __global__ __launch_bounds__(64) void k_isect_cells(IcJob<T> j0, IcJob<T> j1, in
 // After a dense round the next one has as many slots as the dense round had
 // pairs (10^7 after a 64 x 64 first grid whose padded cells mostly overlap),
 // nearly all of them without a hit, so nothing here is sized per row (slot,
-// i).  A group of lanes holds one slot, a lane per side-1 cell i (cells1
+// i), and most slots end at the unit-box test (the tile-pair cull of
+// k_isect_list with one tile per unit), which for such rounds runs first,
+// one slot per lane, and lists the slots that pass in order.  A group of
+// lanes then holds one slot, a lane per side-1 cell i (cells1
 // rounded up to a power of two lanes); a lane's hits over the side-2 cells j
 // form a bit mask, j ascending (the strict 3-axis test of overlap_aabb_host,
-// intersection.cu:46-63; a slot whose unit boxes are disjoint has none, the
-// tile-pair cull of k_isect_list with one tile per unit).  k_isect_slot_count
+// intersection.cu:46-63).  k_isect_slot_count
 // writes three counts per slot: its pairs, its non-empty rows (the side-1
 // units it creates) and its distinct j (its side-2 units).  After their
 // scans, k_isect_slot_emit recomputes the masks and writes the slot's pairs
@@ -387,20 +380,25 @@ __host__ __device__ inline int slot_lanes(int cells1) {
   while (l < cells1) l <<= 1;
   return l;
 }
-// slots per wavefront; the emit pass keeps cells2 unit ids per slot in LDS,
-// at most 1024 per wavefront
+// slot groups per wavefront; the emit pass keeps cells2 unit ids per group
+// in LDS, at most 1024 per wavefront
 __host__ __device__ inline int slot_groups(int cells1, int cells2) {
   const int a = 64 / slot_lanes(cells1), b = 1024 / cells2;
   return a < b ? a : (b > 0 ? b : 1);
 }
 
+// The unit-box test of one slot (the tile-pair cull of k_isect_list).
+template <class T>
+__device__ __forceinline__ bool slot_boxes_meet(const SlotScreen<T>& s, long long slot) {
+  if (!s.cull) return true;
+  const int u1 = s.su1[slot], u2 = s.su2[slot];
+  const vec4_t<T> l1 = s.lo1[u1], h1 = s.hi1[u1], l2 = s.lo2[u2], h2 = s.hi2[u2];
+  return l1.x <= h2.x && l2.x <= h1.x && l1.y <= h2.y && l2.y <= h1.y && l1.z <= h2.z && l2.z <= h1.z;
+}
+
 template <class T>
 __device__ __forceinline__ unsigned long long slot_row_mask(const SlotScreen<T>& s, long long slot, int i) {
   const int u1 = s.su1[slot], u2 = s.su2[slot];
-  if (s.cull) {
-    const vec4_t<T> l1 = s.lo1[u1], h1 = s.hi1[u1], l2 = s.lo2[u2], h2 = s.hi2[u2];
-    if (!(l1.x <= h2.x && l2.x <= h1.x && l1.y <= h2.y && l2.y <= h1.y && l1.z <= h2.z && l2.z <= h1.z)) return 0;
-  }
   const vec4_t<T> c1 = s.C1[(size_t)u1 * s.cells1 + i], r1 = s.R1[(size_t)u1 * s.cells1 + i];
   const vec4_t<T>* const c2p = s.C2 + (size_t)u2 * s.cells2;
   const vec4_t<T>* const r2p = s.R2 + (size_t)u2 * s.cells2;
@@ -414,15 +412,42 @@ __device__ __forceinline__ unsigned long long slot_row_mask(const SlotScreen<T>&
   return m;
 }
 
+// Rounds with many slots are culled first, one slot per lane: a wavefront
+// counts its 64 slots that pass (k_isect_slot_cull), and after a scan of the
+// counts lists them in order (k_isect_slot_list).  The count and emit passes
+// then run over the list (entry e holds slot list[e]; their counts and
+// offsets are per entry), or, without a list, over every slot with the
+// unit-box test in each group.
 template <class T>
-__global__ __launch_bounds__(256) void k_isect_slot_count(SlotScreen<T> s, unsigned* npair, unsigned* nrow,
-                                                          unsigned* ncol) {
+__global__ __launch_bounds__(256) void k_isect_slot_cull(SlotScreen<T> s, unsigned* wcount) {
+  const long long slot = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool pass = slot < s.nslots && slot_boxes_meet(s, slot);
+  const unsigned long long m = __ballot(pass);
+  if ((threadIdx.x & 63) == 0 && slot < s.nslots) wcount[slot >> 6] = (unsigned)__popcll(m);
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_isect_slot_list(SlotScreen<T> s, const unsigned long long* woff,
+                                                         int* list) {
+  const long long slot = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool pass = slot < s.nslots && slot_boxes_meet(s, slot);
+  const unsigned long long m = __ballot(pass);
+  const int lane = threadIdx.x & 63;
+  if (pass) list[woff[slot >> 6] + __popcll(m & ((1ull << lane) - 1))] = (int)slot;
+}
+
+// n entries: slots 0 .. n-1, or list[0 .. n-1]
+template <class T>
+__global__ __launch_bounds__(256) void k_isect_slot_count(SlotScreen<T> s, long long n, const int* list,
+                                                          unsigned* npair, unsigned* nrow, unsigned* ncol) {
   const int G = slot_lanes(s.cells1), spw = slot_groups(s.cells1, s.cells2);
-  const long long nwaves = (s.nslots + spw - 1) / spw;
+  const long long nwaves = (n + spw - 1) / spw;
   const int lane = threadIdx.x & 63, grp = lane / G, i = lane % G;
   for (long long base = (long long)blockIdx.x * 4; base < nwaves; base += (long long)gridDim.x * 4) {
-    const long long slot = (base + (threadIdx.x >> 6)) * spw + grp;
-    const bool valid = grp < spw && slot < s.nslots && i < s.cells1;
+    const long long e = (base + (threadIdx.x >> 6)) * spw + grp;
+    const bool in = grp < spw && e < n;
+    const long long slot = in ? (list ? (long long)list[e] : e) : 0;
+    const bool valid = in && i < s.cells1 && (list || slot_boxes_meet(s, slot));
     const unsigned long long m = valid ? slot_row_mask(s, slot, i) : 0ull;
     unsigned pc = (unsigned)__popcll(m), nz = m != 0;
     unsigned long long om = m;
@@ -431,27 +456,30 @@ __global__ __launch_bounds__(256) void k_isect_slot_count(SlotScreen<T> s, unsig
       nz += __shfl_xor(nz, off);
       om |= __shfl_xor(om, off);
     }
-    if (valid && i == 0) {
-      npair[slot] = pc;
-      nrow[slot] = nz;
-      ncol[slot] = (unsigned)__popcll(om);
+    if (in && i == 0) {
+      npair[e] = pc;
+      nrow[e] = nz;
+      ncol[e] = (unsigned)__popcll(om);
     }
   }
 }
 
 template <class T>
-__global__ __launch_bounds__(256) void k_isect_slot_emit(SlotScreen<T> s, const unsigned long long* offp,
+__global__ __launch_bounds__(256) void k_isect_slot_emit(SlotScreen<T> s, long long n, const int* list,
+                                                         const unsigned long long* offp,
                                                          const unsigned long long* offr,
                                                          const unsigned long long* offc, IPair* out, int units,
                                                          SlotUnits<T> u) {
-  extern __shared__ int sTab[];  // per wavefront: spw slots x cells2 side-2 unit ids
+  extern __shared__ int sTab[];  // per wavefront: spw groups x cells2 side-2 unit ids
   const int G = slot_lanes(s.cells1), spw = slot_groups(s.cells1, s.cells2);
-  const long long nwaves = (s.nslots + spw - 1) / spw;
+  const long long nwaves = (n + spw - 1) / spw;
   const int lane = threadIdx.x & 63, grp = lane / G, i = lane % G;
   int* const tab = sTab + ((threadIdx.x >> 6) * spw + (grp < spw ? grp : 0)) * s.cells2;
   for (long long base = (long long)blockIdx.x * 4; base < nwaves; base += (long long)gridDim.x * 4) {
-    const long long slot = (base + (threadIdx.x >> 6)) * spw + grp;
-    const bool valid = grp < spw && slot < s.nslots && i < s.cells1;
+    const long long e = (base + (threadIdx.x >> 6)) * spw + grp;
+    const bool in = grp < spw && e < n;
+    const long long slot = in ? (list ? (long long)list[e] : e) : 0;
+    const bool valid = in && i < s.cells1 && (list || slot_boxes_meet(s, slot));
     const unsigned long long m = valid ? slot_row_mask(s, slot, i) : 0ull;
     // inclusive scans over the group's rows: pairs, non-empty rows, j seen
     unsigned pc = (unsigned)__popcll(m), nz = m != 0;
@@ -467,7 +495,7 @@ __global__ __launch_bounds__(256) void k_isect_slot_emit(SlotScreen<T> s, const 
     }
     unsigned long long prior = __shfl_up(om, 1, G);  // j of the rows before
     if (i == 0) prior = 0;
-    const unsigned long long p0 = valid ? offp[slot] + (pc - (unsigned)__popcll(m)) : 0ull;
+    const unsigned long long p0 = valid ? offp[e] + (pc - (unsigned)__popcll(m)) : 0ull;
     int u1 = 0;
     if (valid && m) {
       IPair pr;
@@ -479,10 +507,10 @@ __global__ __launch_bounds__(256) void k_isect_slot_emit(SlotScreen<T> s, const 
         out[p++] = pr;
       }
       if (units) {
-        u1 = (int)(offr[slot] + (nz - 1));
+        u1 = (int)(offr[e] + (nz - 1));
         u.nreg1[u1] = child_region<T>(u.reg1[s.su1[slot]], u.pnuv1, i);
         const unsigned long long nw = m & ~prior;
-        const unsigned long long ub = offc[slot] + (unsigned)__popcll(prior);
+        const unsigned long long ub = offc[e] + (unsigned)__popcll(prior);
         const vec4_t<T> preg2 = u.reg2[s.su2[slot]];
         for (unsigned long long r = nw; r; r &= r - 1) {
           const int j = __ffsll((long long)r) - 1;
@@ -610,24 +638,45 @@ static SlotScreen<T> slot_screen(long long nslots, int w1, int w2, const int* su
   return s;
 }
 
-static unsigned slot_blocks(long long nslots, int cells1, int cells2) {
+static unsigned slot_blocks(long long n, int cells1, int cells2) {
   const int spw = slot_groups(cells1, cells2);
-  const long long waves = (nslots + spw - 1) / spw;
+  const long long waves = (n + spw - 1) / spw;
   return (unsigned)std::min<long long>((waves + 3) / 4, 1ll << 20);
+}
+
+static bool slot_args_ok(long long nslots, int w1, int w2, int cull, const void* lo1, const void* hi1,
+                         const void* lo2, const void* hi2) {
+  return w1 >= 1 && w2 >= 1 && w1 <= kIcMaxNuv && w2 <= kIcMaxNuv && nslots <= INT_MAX &&
+         !(cull && (!lo1 || !hi1 || !lo2 || !hi2));
+}
+
+template <class T>
+hipError_t launch_isect_slot_cull(long long nslots, int w1, int w2, const int* su1, const int* su2,
+                                  const vec4_t<T>* lo1, const vec4_t<T>* hi1, const vec4_t<T>* lo2,
+                                  const vec4_t<T>* hi2, unsigned* wcount, const unsigned long long* woff, int* list,
+                                  hipStream_t s) {
+  if (nslots <= 0) return hipSuccess;
+  if (!slot_args_ok(nslots, w1, w2, 1, lo1, hi1, lo2, hi2) || (!wcount == !list)) return hipErrorInvalidValue;
+  const SlotScreen<T> sc = slot_screen<T>(nslots, w1, w2, su1, su2, nullptr, nullptr, nullptr, nullptr, lo1, hi1,
+                                          lo2, hi2, 1);
+  const dim3 g((unsigned)((nslots + 255) / 256)), b(256);
+  if (wcount)
+    hipLaunchKernelGGL(k_isect_slot_cull<T>, g, b, 0, s, sc, wcount);
+  else
+    hipLaunchKernelGGL(k_isect_slot_list<T>, g, b, 0, s, sc, woff, list);
+  return hipGetLastError();
 }
 
 template <class T>
 hipError_t launch_isect_slot_count(long long nslots, int w1, int w2, const int* su1, const int* su2,
                                    const vec4_t<T>* C1, const vec4_t<T>* R1, const vec4_t<T>* C2,
                                    const vec4_t<T>* R2, const vec4_t<T>* lo1, const vec4_t<T>* hi1,
-                                   const vec4_t<T>* lo2, const vec4_t<T>* hi2, int cull, unsigned* npair,
-                                   unsigned* nrow, unsigned* ncol, hipStream_t s) {
-  if (nslots <= 0) return hipSuccess;
-  if (w1 < 1 || w2 < 1 || w1 > kIcMaxNuv || w2 > kIcMaxNuv || nslots > INT_MAX ||
-      (cull && (!lo1 || !hi1 || !lo2 || !hi2)))
-    return hipErrorInvalidValue;
+                                   const vec4_t<T>* lo2, const vec4_t<T>* hi2, int cull, long long n,
+                                   const int* list, unsigned* npair, unsigned* nrow, unsigned* ncol, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  if (!slot_args_ok(nslots, w1, w2, list ? 0 : cull, lo1, hi1, lo2, hi2) || n > nslots) return hipErrorInvalidValue;
   const SlotScreen<T> sc = slot_screen<T>(nslots, w1, w2, su1, su2, C1, R1, C2, R2, lo1, hi1, lo2, hi2, cull);
-  hipLaunchKernelGGL(k_isect_slot_count<T>, dim3(slot_blocks(nslots, sc.cells1, sc.cells2)), dim3(256), 0, s, sc,
+  hipLaunchKernelGGL(k_isect_slot_count<T>, dim3(slot_blocks(n, sc.cells1, sc.cells2)), dim3(256), 0, s, sc, n, list,
                      npair, nrow, ncol);
   return hipGetLastError();
 }
@@ -636,14 +685,13 @@ template <class T>
 hipError_t launch_isect_slot_emit(long long nslots, int w1, int w2, const int* su1, const int* su2,
                                   const vec4_t<T>* C1, const vec4_t<T>* R1, const vec4_t<T>* C2, const vec4_t<T>* R2,
                                   const vec4_t<T>* lo1, const vec4_t<T>* hi1, const vec4_t<T>* lo2,
-                                  const vec4_t<T>* hi2, int cull, const unsigned long long* offp,
-                                  const unsigned long long* offr, const unsigned long long* offc, IPair* out,
-                                  const vec4_t<T>* reg1, const vec4_t<T>* reg2, int pnuv1, int pnuv2, int* nsu1,
-                                  int* nsu2, vec4_t<T>* nreg1, vec4_t<T>* nreg2, hipStream_t s) {
-  if (nslots <= 0) return hipSuccess;
-  if (w1 < 1 || w2 < 1 || w1 > kIcMaxNuv || w2 > kIcMaxNuv || nslots > INT_MAX ||
-      (cull && (!lo1 || !hi1 || !lo2 || !hi2)))
-    return hipErrorInvalidValue;
+                                  const vec4_t<T>* hi2, int cull, long long n, const int* list,
+                                  const unsigned long long* offp, const unsigned long long* offr,
+                                  const unsigned long long* offc, IPair* out, const vec4_t<T>* reg1,
+                                  const vec4_t<T>* reg2, int pnuv1, int pnuv2, int* nsu1, int* nsu2,
+                                  vec4_t<T>* nreg1, vec4_t<T>* nreg2, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  if (!slot_args_ok(nslots, w1, w2, list ? 0 : cull, lo1, hi1, lo2, hi2) || n > nslots) return hipErrorInvalidValue;
   const SlotScreen<T> sc = slot_screen<T>(nslots, w1, w2, su1, su2, C1, R1, C2, R2, lo1, hi1, lo2, hi2, cull);
   SlotUnits<T> su;
   su.reg1 = reg1;
@@ -657,7 +705,7 @@ hipError_t launch_isect_slot_emit(long long nslots, int w1, int w2, const int* s
   const int units = nsu1 != nullptr;
   if (units && (!nsu2 || !nreg1 || !nreg2 || !reg1 || !reg2)) return hipErrorInvalidValue;
   const size_t lds = units ? sizeof(int) * 4 * slot_groups(sc.cells1, sc.cells2) * sc.cells2 : 0;
-  hipLaunchKernelGGL(k_isect_slot_emit<T>, dim3(slot_blocks(nslots, sc.cells1, sc.cells2)), dim3(256), lds, s, sc,
+  hipLaunchKernelGGL(k_isect_slot_emit<T>, dim3(slot_blocks(n, sc.cells1, sc.cells2)), dim3(256), lds, s, sc, n, list,
                      offp, offr, offc, out, units, su);
   return hipGetLastError();
 }
@@ -669,16 +717,19 @@ hipError_t launch_gather_totals(const unsigned long long* a, const unsigned long
 }
 
 #define JSI_IS(T)                                                                                                  \
+  template hipError_t launch_isect_slot_cull<T>(long long, int, int, const int*, const int*, const vec4_t<T>*,     \
+                                                const vec4_t<T>*, const vec4_t<T>*, const vec4_t<T>*, unsigned*,    \
+                                                const unsigned long long*, int*, hipStream_t);                      \
   template hipError_t launch_isect_slot_count<T>(long long, int, int, const int*, const int*, const vec4_t<T>*,    \
                                                  const vec4_t<T>*, const vec4_t<T>*, const vec4_t<T>*,             \
                                                  const vec4_t<T>*, const vec4_t<T>*, const vec4_t<T>*,             \
-                                                 const vec4_t<T>*, int, unsigned*, unsigned*, unsigned*,            \
-                                                 hipStream_t);                                                      \
+                                                 const vec4_t<T>*, int, long long, const int*, unsigned*, unsigned*, \
+                                                 unsigned*, hipStream_t);                                           \
   template hipError_t launch_isect_slot_emit<T>(                                                                   \
       long long, int, int, const int*, const int*, const vec4_t<T>*, const vec4_t<T>*, const vec4_t<T>*,           \
-      const vec4_t<T>*, const vec4_t<T>*, const vec4_t<T>*, const vec4_t<T>*, const vec4_t<T>*, int,               \
-      const unsigned long long*, const unsigned long long*, const unsigned long long*, IPair*, const vec4_t<T>*,   \
-      const vec4_t<T>*, int, int, int*, int*, vec4_t<T>*, vec4_t<T>*, hipStream_t);
+      const vec4_t<T>*, const vec4_t<T>*, const vec4_t<T>*, const vec4_t<T>*, const vec4_t<T>*, int, long long,    \
+      const int*, const unsigned long long*, const unsigned long long*, const unsigned long long*, IPair*,         \
+      const vec4_t<T>*, const vec4_t<T>*, int, int, int*, int*, vec4_t<T>*, vec4_t<T>*, hipStream_t);
 JSI_IS(float)
 JSI_IS(double)
 #undef JSI_IS

@@ -2878,19 +2878,10 @@ __global__ __launch_bounds__(256) void k_points_tile(SurfDev<T> sd, const vec4_t
       const T* knots = KL ? (dir ? sKv : sKu) : (dir ? sd.V : sd.U);
       const int span = find_span(dir ? sd.nkv : sd.nku, deg, tt, knots);
       T(*const nb)[NK][NB] = dir ? sNv : sNu;
-      if (BICUBIC) {  // degree 3 both ways: the register-resident recurrence (no scratch)
-        T d[3][4];
-        basis_ders_p<T, 3>(span, tt, knots, der, d);
-#pragma unroll
-        for (int k = 0; k < NK; ++k)
-#pragma unroll
-          for (int j = 0; j <= 3; ++j) nb[e][k][j] = k <= der ? d[k][j] : T(0);
-      } else {
-        T d[3][MAXD + 1];
-        basis_ders<T, MAXD>(span, deg, tt, knots, der, d);
-        for (int k = 0; k < NK; ++k)
-          for (int j = 0; j <= deg; ++j) nb[e][k][j] = k <= der ? d[k][j] : T(0);
-      }
+      if (BICUBIC)  // degree 3 both ways: one recurrence, no degree switch
+        basis_row_fixed<T, 3, NK, NB>(span, tt, knots, der, nb[e]);
+      else  // fixed-degree recurrences up to degree 7 (no scratch)
+        basis_row_any<T, MAXD, NK, NB>(span, deg, tt, knots, der, nb[e]);
       (dir ? sSv : sSu)[e] = span;
     }
   }

@@ -141,4 +141,40 @@ __device__ void basis_ders(int span, int p, T u, const T* U, int der, T (&ders)[
   }
 }
 
+// A2.3 rows of a runtime degree p <= MAXP written to row[k][j] (k < NK; orders
+// above der are zero): degrees up to min(MAXP, 7) take the fixed-degree
+// recurrence, whose arrays stay in registers (the general one indexes its
+// arrays at run time and lives in scratch memory); the results are the same.
+template <class T, int P, int NK, int NB>
+__device__ __forceinline__ void basis_row_fixed(int span, T u, const T* U, int der, T (*row)[NB]) {
+  T d[3][P + 1];
+  basis_ders_p<T, P>(span, u, U, der, d);
+#pragma unroll
+  for (int k = 0; k < NK; ++k)
+#pragma unroll
+    for (int j = 0; j <= P; ++j) row[k][j] = k <= der ? d[k][j] : T(0);
+}
+
+template <class T, int MAXP, int NK, int NB>
+__device__ __forceinline__ void basis_row_any(int span, int p, T u, const T* U, int der, T (*row)[NB]) {
+  static_assert(NB >= MAXP + 1, "row too short");
+  switch (p) {
+    case 0: basis_row_fixed<T, 0, NK, NB>(span, u, U, der, row); return;
+    case 1: if constexpr (MAXP >= 1) { basis_row_fixed<T, 1, NK, NB>(span, u, U, der, row); return; } break;
+    case 2: if constexpr (MAXP >= 2) { basis_row_fixed<T, 2, NK, NB>(span, u, U, der, row); return; } break;
+    case 3: if constexpr (MAXP >= 3) { basis_row_fixed<T, 3, NK, NB>(span, u, U, der, row); return; } break;
+    case 4: if constexpr (MAXP >= 4) { basis_row_fixed<T, 4, NK, NB>(span, u, U, der, row); return; } break;
+    case 5: if constexpr (MAXP >= 5) { basis_row_fixed<T, 5, NK, NB>(span, u, U, der, row); return; } break;
+    case 6: if constexpr (MAXP >= 6) { basis_row_fixed<T, 6, NK, NB>(span, u, U, der, row); return; } break;
+    case 7: if constexpr (MAXP >= 7) { basis_row_fixed<T, 7, NK, NB>(span, u, U, der, row); return; } break;
+    default: break;
+  }
+  if constexpr (MAXP > 7) {
+    T d[3][MAXP + 1];
+    basis_ders<T, MAXP>(span, p, u, U, der, d);
+    for (int k = 0; k < NK; ++k)
+      for (int j = 0; j <= p; ++j) row[k][j] = k <= der ? d[k][j] : T(0);
+  }
+}
+
 }  // namespace jsi

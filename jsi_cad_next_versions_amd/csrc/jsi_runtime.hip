@@ -60,6 +60,7 @@ unsigned long long next_pow2(unsigned long long x) {
 }
 
 constexpr unsigned kSmallCap = 16384;               // records sorted after selection
+constexpr long long kSlotListMin = 1 << 18;  // intersection rounds with this many slots cull them first
 
 
 }  // namespace
@@ -1027,6 +1028,10 @@ int jsi_ctx::intersect(int h1, const jsi_region& r1, int h2, const jsi_region& r
   // (JSI_ISECT_CELLS=0 keeps the grid-point kernel, the AABB and tile-box passes)
   const char* icenv = std::getenv("JSI_ISECT_CELLS");
   const bool fuse_cells = !(icenv && std::strcmp(icenv, "0") == 0);
+  // rounds of at least this many slots list the slots that pass the unit-box
+  // test before the screen (JSI_SLOT_LIST_MIN overrides the default)
+  const char* slenv = std::getenv("JSI_SLOT_LIST_MIN");
+  const long long slot_list_min = slenv ? std::atoll(slenv) : kSlotListMin;
   V *tlo1 = nullptr, *thi1 = nullptr, *tlo2 = nullptr, *thi2 = nullptr;  // tile boxes of the current units
   bool units_ready = false;  // the screen also wrote the next round's units
   unsigned long long ready_nu1 = 0, ready_nu2 = 0;
@@ -1062,23 +1067,43 @@ int jsi_ctx::intersect(int h1, const jsi_region& r1, int h2, const jsi_region& r
         // units of at most 64 cells: per-slot counts of pairs and new units,
         // their scans, then the ordered pairs and (when a round follows) the
         // next round's units in one pass
-        const size_t ns = (size_t)nslots;
+        // many slots: cull them one per lane first and list those that pass
+        size_t ns = (size_t)nslots;
+        int* slist = nullptr;
+        if (P.cull && nslots >= slot_list_min) {
+          const size_t nw = (ns + 63) / 64;
+          unsigned* wcount;
+          unsigned long long* woff;
+          void* wtmp;
+          RCK(buf("i.wcount", nw, &wcount));
+          RCK(buf("i.woff", nw + 1, &woff));
+          RCK(buf("i.slist", ns, &slist));
+          const size_t wtb = scan_tmp_bytes(nw);
+          RCK(buf_raw("i.scantmp", wtb, &wtmp, 0));
+          HCK(launch_isect_slot_cull<T>(nslots, w1, w2, su1, su2, lo1, hi1, lo2, hi2, wcount, nullptr, nullptr,
+                                        stream));
+          HCK(scan_u32_to_u64(wcount, woff, nw, wtmp, wtb, stream));
+          HCK(launch_isect_slot_cull<T>(nslots, w1, w2, su1, su2, lo1, hi1, lo2, hi2, nullptr, woff, slist, stream));
+          unsigned long long nl = 0;
+          RCK(d2h(&nl, woff + nw, sizeof(nl)));
+          ns = (size_t)nl;
+        }
         unsigned* cnt;
         unsigned long long *off, *tot;
         void* scantmp;
-        RCK(buf("i.scount", 3 * ns, &cnt));
+        RCK(buf("i.scount", 3 * std::max(ns, (size_t)1), &cnt));
         RCK(buf("i.soff", 3 * (ns + 1), &off));
         RCK(buf("i.stot", 3, &tot));
         const size_t stb = scan_tmp_bytes(ns);
         RCK(buf_raw("i.scantmp", stb, &scantmp, 0));
         unsigned *cp = cnt, *cr = cnt + ns, *cc = cnt + 2 * ns;
         unsigned long long *op = off, *orr = off + ns + 1, *oc = off + 2 * (ns + 1);
-        HCK(launch_isect_slot_count<T>(nslots, w1, w2, su1, su2, C1, R1, C2, R2, lo1, hi1, lo2, hi2, P.cull, cp, cr,
-                                       cc, stream));
+        HCK(launch_isect_slot_count<T>(nslots, w1, w2, su1, su2, C1, R1, C2, R2, lo1, hi1, lo2, hi2, P.cull,
+                                       (long long)ns, slist, cp, cr, cc, stream));
         HCK(scan_u32_to_u64(cp, op, ns, scantmp, stb, stream));
         HCK(scan_u32_to_u64(cr, orr, ns, scantmp, stb, stream));
         HCK(scan_u32_to_u64(cc, oc, ns, scantmp, stb, stream));
-        HCK(launch_gather_totals(op, orr, oc, nslots, tot, stream));
+        HCK(launch_gather_totals(op, orr, oc, (long long)ns, tot, stream));
         unsigned long long t3[3];
         RCK(d2h(t3, tot, sizeof(t3)));
         npairs = t3[0];
@@ -1099,8 +1124,9 @@ int jsi_ctx::intersect(int h1, const jsi_region& r1, int h2, const jsi_region& r
           ready_nu2 = t3[2];
         }
         if (npairs > 0)
-          HCK(launch_isect_slot_emit<T>(nslots, w1, w2, su1, su2, C1, R1, C2, R2, lo1, hi1, lo2, hi2, P.cull, op, orr,
-                                        oc, pairs, reg1, reg2, nuv1, nuv2, nsu1, nsu2, nreg1, nreg2, stream));
+          HCK(launch_isect_slot_emit<T>(nslots, w1, w2, su1, su2, C1, R1, C2, R2, lo1, hi1, lo2, hi2, P.cull,
+                                        (long long)ns, slist, op, orr, oc, pairs, reg1, reg2, nuv1, nuv2, nsu1, nsu2,
+                                        nreg1, nreg2, stream));
       } else {
         uint64_t* list;
         unsigned long long* small;

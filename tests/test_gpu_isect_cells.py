@@ -46,15 +46,20 @@ def wavy(n, deg, seed, lift=0.0, rational=True, periods=3.0, tilt=0.0):
 
 
 def run_three(ctx, orc, a, b, nuv, monkeypatch, **kw):
-    """The fused path, the grid-point path and the oracle on one query."""
+    """The fused path (slots screened directly, and listed after the unit-box
+    test as rounds with many slots are), the grid-point path and the oracle
+    on one query."""
     r1, r2 = S.full_region(a, nuv, 1), S.full_region(b, nuv, 2)
     p = default_params(trace=1, **kw)
     res, pairs = ctx.intersection(a, r1, b, r2, p, cap=1 << 20)
+    monkeypatch.setenv("JSI_SLOT_LIST_MIN", "1")
+    lres, lpairs = ctx.intersection(a, r1, b, r2, p, cap=1 << 20)
+    monkeypatch.delenv("JSI_SLOT_LIST_MIN")
     monkeypatch.setenv("JSI_ISECT_CELLS", "0")
     pres, ppairs = ctx.intersection(a, r1, b, r2, p, cap=1 << 20)
     monkeypatch.delenv("JSI_ISECT_CELLS")
     ores, opairs = orc.intersect(a, r1, b, r2, p, cap=1 << 20)
-    for other, opl in ((pres, ppairs), (ores, opairs)):
+    for other, opl in ((lres, lpairs), (pres, ppairs), (ores, opairs)):
         assert res.rounds_list() == other.rounds_list()
         assert res.isect_refine_nuv == other.isect_refine_nuv
         np.testing.assert_array_equal(pairs, opl)
@@ -150,11 +155,14 @@ def test_unequal_first_grids(ctx, orc, monkeypatch, nuv1, nuv2, prec):
     r1, r2 = S.full_region(a, nuv1, 1), S.full_region(b, nuv2, 2)
     p = default_params(trace=1, precision=prec)
     res, pairs = ctx.intersection(a, r1, b, r2, p, cap=1 << 20)
+    monkeypatch.setenv("JSI_SLOT_LIST_MIN", "1")
+    lres, lpairs = ctx.intersection(a, r1, b, r2, p, cap=1 << 20)
+    monkeypatch.delenv("JSI_SLOT_LIST_MIN")
     monkeypatch.setenv("JSI_ISECT_CELLS", "0")
     pres, ppairs = ctx.intersection(a, r1, b, r2, p, cap=1 << 20)
     monkeypatch.delenv("JSI_ISECT_CELLS")
     ores, opairs = orc.intersect(a, r1, b, r2, p, cap=1 << 20)
-    for other, opl in ((pres, ppairs), (ores, opairs)):
+    for other, opl in ((lres, lpairs), (pres, ppairs), (ores, opairs)):
         assert res.rounds_list() == other.rounds_list()
         np.testing.assert_array_equal(pairs, opl)
     assert res.rounds == 6 and res.round[1].units1 != res.round[1].units2
@@ -169,11 +177,15 @@ def test_dense_first_round_ten_million_slots(ctx, monkeypatch):
     r1, r2 = S.full_region(a, nuv, 1), S.full_region(b, nuv, 2)
     p = default_params(trace=1, precision="fp64")
     res, pairs = ctx.intersection(a, r1, b, r2, p, cap=1 << 16)
+    monkeypatch.setenv("JSI_SLOT_LIST_MIN", str(1 << 40))  # every round screened directly
+    dres, dpairs = ctx.intersection(a, r1, b, r2, p, cap=1 << 16)
+    monkeypatch.delenv("JSI_SLOT_LIST_MIN")
     monkeypatch.setenv("JSI_ISECT_CELLS", "0")
     pres, ppairs = ctx.intersection(a, r1, b, r2, p, cap=1 << 16)
     monkeypatch.delenv("JSI_ISECT_CELLS")
-    assert res.rounds_list() == pres.rounds_list()
-    np.testing.assert_array_equal(pairs, ppairs)
+    for other, opl in ((dres, dpairs), (pres, ppairs)):
+        assert res.rounds_list() == other.rounds_list()
+        np.testing.assert_array_equal(pairs, opl)
     assert [r["npairs"] for r in res.rounds_list()] == [10389328, 15836, 14360, 16715, 20375, 25402]
 
 
