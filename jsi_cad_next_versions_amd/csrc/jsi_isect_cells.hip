@@ -182,23 +182,46 @@ __global__ __launch_bounds__(64) void k_isect_cells(IcJob<T> j0, IcJob<T> j1, in
     __syncthreads();
     // phase 3: each point adds this chunk's rows, in s order, for every (k, l)
     if (active && !sd.zero) {
+      if (nrows <= kRows) {  // the first chunk holds every row of the unit:
+                             // the point's rows are sv - vlo + s2, all present
+                             // (later chunks, run for other units, add none)
+        const int rb = sv - vlo;
+        if (r0 == 0) {
 #pragma unroll
-      for (int l = 0; l < NK; ++l)
+          for (int l = 0; l < NK; ++l)
 #pragma unroll
-        for (int k = 0; k < NK - l; ++k) {
-          if (l > lv || k > ku) continue;
-          const int m = out_row(k, l);
-          constexpr int UR = MAXD <= 7 ? MAXD + 1 : 1;
+            for (int k = 0; k < NK - l; ++k) {
+              if (l > lv || k > ku) continue;
+              const int m = out_row(k, l);
+              constexpr int UR = MAXD <= 7 ? MAXD + 1 : 1;
 #pragma unroll UR
-          for (int s2 = 0; s2 <= MAXD; ++s2) {
-            if (s2 > q) break;
-            const int rr = sv - q + s2 - v0 - r0;
-            if (rr < 0 || rr >= nr) continue;
-            const T nv = sN[rpt][l][s2];
+              for (int s2 = 0; s2 <= MAXD; ++s2) {
+                if (s2 > q) break;
+                const T nv = sN[rpt][l][s2];
 #pragma unroll
-            for (int c = 0; c < NC; ++c) h[m][c] += nv * sR[cpt][rr][k][c];
-          }
+                for (int c = 0; c < NC; ++c) h[m][c] += nv * sR[cpt][rb + s2][k][c];
+              }
+            }
         }
+      } else {
+#pragma unroll
+        for (int l = 0; l < NK; ++l)
+#pragma unroll
+          for (int k = 0; k < NK - l; ++k) {
+            if (l > lv || k > ku) continue;
+            const int m = out_row(k, l);
+            constexpr int UR = MAXD <= 7 ? MAXD + 1 : 1;
+#pragma unroll UR
+            for (int s2 = 0; s2 <= MAXD; ++s2) {
+              if (s2 > q) break;
+              const int rr = sv - q + s2 - v0 - r0;
+              if (rr < 0 || rr >= nr) continue;
+              const T nv = sN[rpt][l][s2];
+#pragma unroll
+              for (int c = 0; c < NC; ++c) h[m][c] += nv * sR[cpt][rr][k][c];
+            }
+          }
+      }
     }
     __syncthreads();  // sR is rewritten by the next chunk (or becomes the points)
   }

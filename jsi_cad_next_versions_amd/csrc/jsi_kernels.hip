@@ -2837,11 +2837,16 @@ __global__ __launch_bounds__(256) void k_points_tile(SurfDev<T> sd, const vec4_t
   constexpr int NK = MODE == kEvalPoint ? 1 : 3;  // derivative orders kept per direction
   constexpr int kRows = PtRows<MAXD>::value;
   constexpr int kKn = KL ? kEcKnots : 1;
-  __shared__ T sKu[kKn], sKv[kKn];
+  // one buffer, three lives: knots (phases 0-1), u-contractions (phases 2-3)
+  // and, for the 18-value output, the tile's results; barriers separate them
+  constexpr int kKsz = 2 * kKn, kRsz = TP * kRows * NK * NC, kOsz = MODE == kEvalFull18 ? TP * TP * 18 : 1;
+  constexpr int kA = kKsz > kRsz ? (kKsz > kOsz ? kKsz : kOsz) : (kRsz > kOsz ? kRsz : kOsz);
+  __shared__ T sA[kA];
+  T* const sKu = sA;
+  T* const sKv = sA + kKn;
+  T(*const sR)[kRows][NK][NC] = reinterpret_cast<T(*)[kRows][NK][NC]>(sA);
   __shared__ T sNu[TP][NK][NB], sNv[TP][NK][NB];
   __shared__ int sSu[TP], sSv[TP];
-  __shared__ T sR[TP][kRows][NK][NC];
-  __shared__ int sV0, sNr;
   __shared__ unsigned long long sk[4][3];
   const int tid = threadIdx.x;
   const int tw = (nuv + TP - 1) / TP, tpu = tw * tw;
@@ -2886,19 +2891,15 @@ __global__ __launch_bounds__(256) void k_points_tile(SurfDev<T> sd, const vec4_t
     }
   }
   __syncthreads();
-  // control rows the tile reads: [vlo - q, vhi]
-  if (tid == 0) {
-    int vlo = INT_MAX, vhi = INT_MIN;
-    if (!sd.zero)
-      for (int e = 0; e < ht; ++e) {
-        vlo = min(vlo, sSv[e]);
-        vhi = max(vhi, sSv[e]);
-      }
-    sV0 = vlo - q;
-    sNr = sd.zero ? 0 : vhi - vlo + q + 1;
-  }
-  __syncthreads();
-  const int v0 = sV0, nrows = sNr;
+  // control rows the tile reads: [vlo - q, vhi] (every thread reads the
+  // tile's row spans)
+  int vlo = INT_MAX, vhi = INT_MIN;
+  if (!sd.zero)
+    for (int e = 0; e < ht; ++e) {
+      vlo = min(vlo, sSv[e]);
+      vhi = max(vhi, sSv[e]);
+    }
+  const int v0 = vlo - q, nrows = sd.zero ? 0 : vhi - vlo + q + 1;
   T h[6][4];
 #pragma unroll
   for (int m = 0; m < 6; ++m)
@@ -2950,23 +2951,43 @@ __global__ __launch_bounds__(256) void k_points_tile(SurfDev<T> sd, const vec4_t
     __syncthreads();
     // phase 3: each point adds this chunk's rows, in s order, for every (k, l)
     if (active && !sd.zero) {
+      if (nrows <= kRows) {  // one chunk (r0 = 0) holds every row the tile
+                             // reads: the point's rows are sv - vlo + s2
+        const int rb = sv - vlo;
 #pragma unroll
-      for (int l = 0; l < NK; ++l)
+        for (int l = 0; l < NK; ++l)
 #pragma unroll
-        for (int k = 0; k < NK - l; ++k) {  // unrolled: h stays in registers
-          if (l > lv || k > ku) continue;
-          const int m = out_row(k, l);
-          constexpr int UR = MAXD <= 7 ? MAXD + 1 : 1;
+          for (int k = 0; k < NK - l; ++k) {
+            if (l > lv || k > ku) continue;
+            const int m = out_row(k, l);
+            constexpr int UR = MAXD <= 7 ? MAXD + 1 : 1;
 #pragma unroll UR
-          for (int s2 = 0; s2 <= MAXD; ++s2) {
-            if (s2 > q) break;
-            const int rr = sv - q + s2 - v0 - r0;
-            if (rr < 0 || rr >= nr) continue;
-            const T nv = sNv[ey][l][s2];
+            for (int s2 = 0; s2 <= MAXD; ++s2) {
+              if (s2 > q) break;
+              const T nv = sNv[ey][l][s2];
 #pragma unroll
-            for (int c = 0; c < NC; ++c) h[m][c] += nv * sR[ex][rr][k][c];
+              for (int c = 0; c < NC; ++c) h[m][c] += nv * sR[ex][rb + s2][k][c];
+            }
           }
-        }
+      } else {
+#pragma unroll
+        for (int l = 0; l < NK; ++l)
+#pragma unroll
+          for (int k = 0; k < NK - l; ++k) {  // unrolled: h stays in registers
+            if (l > lv || k > ku) continue;
+            const int m = out_row(k, l);
+            constexpr int UR = MAXD <= 7 ? MAXD + 1 : 1;
+#pragma unroll UR
+            for (int s2 = 0; s2 <= MAXD; ++s2) {
+              if (s2 > q) break;
+              const int rr = sv - q + s2 - v0 - r0;
+              if (rr < 0 || rr >= nr) continue;
+              const T nv = sNv[ey][l][s2];
+#pragma unroll
+              for (int c = 0; c < NC; ++c) h[m][c] += nv * sR[ex][rr][k][c];
+            }
+          }
+      }
     }
     __syncthreads();  // sR is rewritten by the next chunk
   }
@@ -2999,23 +3020,35 @@ __global__ __launch_bounds__(256) void k_points_tile(SurfDev<T> sd, const vec4_t
   }
   if (MODE == kEvalFull18) {
     // 18 values per point, point-major: a tile row is one contiguous run of
-    // 18 wd values, so half a tile at a time goes through LDS and is written
-    // with consecutive lanes on consecutive addresses
-    __shared__ T sOut[TP / 2][TP * 18];
+    // 18 wd values, so the tile goes through LDS (the u-contractions' storage,
+    // free after the last barrier of the chunk loop) and is written with
+    // consecutive lanes on consecutive addresses; no barrier follows the
+    // stores, so the workgroup ends with them in flight
+    T(*const sOut)[TP * 18] = reinterpret_cast<T(*)[TP * 18]>(sA);
+    if (active)
 #pragma unroll
-    for (int half = 0; half < 2; ++half) {
-      if (active && (ey >> 3) == half)
+      for (int m = 0; m < 6; ++m)
 #pragma unroll
-        for (int m = 0; m < 6; ++m)
+        for (int c = 0; c < 3; ++c) sOut[ey][ex * 18 + 3 * m + c] = h[m][c];
+    __syncthreads();
+    // pairs of values: every point starts at an even offset (18 values), so
+    // the pairs are aligned; a full-width row is TP * 9 pairs, a constant
+    using T2 = typename std::conditional<std::is_same<T, float>::value, float2, double2>::type;
+    T2* const o2 = reinterpret_cast<T2*>(out + ((size_t)unit * npts + (size_t)row0 * nuv + col0) * 18);
+    const T2(*const s2)[TP * 9] = reinterpret_cast<const T2(*)[TP * 9]>(sA);
+    const int stride = nuv * 9;  // pairs per grid row
+    if (wd == TP) {
 #pragma unroll
-          for (int c = 0; c < 3; ++c) sOut[ey & 7][ex * 18 + 3 * m + c] = h[m][c];
-      __syncthreads();
-      const int rows = min(TP / 2, ht - (TP / 2) * half), run = wd * 18;
-      for (int e = tid; e < rows * run; e += 256) {
-        const int ry = e / run, off = e - ry * run;
-        out[((size_t)unit * npts + (size_t)(row0 + (TP / 2) * half + ry) * nuv + col0) * 18 + off] = sOut[ry][off];
+      for (int it = 0; it < (TP * TP * 9 + 255) / 256; ++it) {
+        const int e = tid + 256 * it, ry = e / (TP * 9), off = e - ry * (TP * 9);
+        if (ry < ht) o2[ry * stride + off] = s2[ry][off];
       }
-      __syncthreads();
+    } else {
+      const int run = wd * 9;
+      for (int e = tid; e < ht * run; e += 256) {
+        const int ry = e / run, off = e - ry * run;
+        o2[ry * stride + off] = s2[ry][off];
+      }
     }
     return;
   }
