@@ -243,7 +243,8 @@ def main():
     ap.add_argument("--rates", default="1000,2000,3000,4000",
                     help="Poisson arrival rates (req/s) of the open-loop runs of the benchmark query")
     ap.add_argument("--rate-seconds", type=float, default=1.0, help="arrival window of each open-loop run")
-    ap.add_argument("--serve-rates", default="500,800", help="Poisson arrival rates of the uneven-mix runs (req/s)")
+    ap.add_argument("--serve-rates", default="500,1000,2000,3000,4000",
+                    help="Poisson arrival rates of the uneven-mix runs (req/s)")
     ap.add_argument("--serve-seconds", type=float, default=2.0, help="arrival window of the mix runs (0 = skip)")
     ap.add_argument("--concurrency", type=int, default=4,
                     help="server workers of the served form (each takes batches of up to 64 queued queries)")
