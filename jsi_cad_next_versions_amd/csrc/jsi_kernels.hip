@@ -193,7 +193,10 @@ __device__ __forceinline__ void ec_tile(const SurfDev<T>& sd, const vec4_t<T>* r
       sR[ex][4 * rr + 3] = row[3];
     }
     __syncthreads();
-    // phase 3: each corner adds the rows of this chunk, in s2 order
+    // phase 3: each corner adds the rows of this chunk, in s2 order; when one
+    // chunk holds every row the tile reads (nrows <= kRows, r0 = 0), a
+    // corner's rows are sv - vlo + s2, all present, and need no check
+    const bool one = nrows <= kRows;
 #pragma unroll
     for (int c = 0; c < kCorners; ++c) {
       const int x = lane + 64 * c;
@@ -201,16 +204,29 @@ __device__ __forceinline__ void ec_tile(const SurfDev<T>& sd, const vec4_t<T>* r
       if (x >= E * E || ey > ht || ex > wd) continue;
       const int sv = sSv[ey];
       constexpr int UR = MAXD <= 7 ? MAXD + 1 : 1;
+      if (one) {
+        const int rb = sv - vlo;
 #pragma unroll UR
-      for (int s2 = 0; s2 <= MAXD; ++s2) {
-        if (s2 > q) break;
-        const int rr = sv - q + s2 - v0 - r0;
-        if (rr < 0 || rr >= nr) continue;
-        const T nvs = sNv[ey][s2];
-        acc[c][0] += nvs * sR[ex][4 * rr + 0];
-        acc[c][1] += nvs * sR[ex][4 * rr + 1];
-        acc[c][2] += nvs * sR[ex][4 * rr + 2];
-        if (RAT) acc[c][3] += nvs * sR[ex][4 * rr + 3];
+        for (int s2 = 0; s2 <= MAXD; ++s2) {
+          if (s2 > q) break;
+          const T nvs = sNv[ey][s2];
+          acc[c][0] += nvs * sR[ex][4 * (rb + s2) + 0];
+          acc[c][1] += nvs * sR[ex][4 * (rb + s2) + 1];
+          acc[c][2] += nvs * sR[ex][4 * (rb + s2) + 2];
+          if (RAT) acc[c][3] += nvs * sR[ex][4 * (rb + s2) + 3];
+        }
+      } else {
+#pragma unroll UR
+        for (int s2 = 0; s2 <= MAXD; ++s2) {
+          if (s2 > q) break;
+          const int rr = sv - q + s2 - v0 - r0;
+          if (rr < 0 || rr >= nr) continue;
+          const T nvs = sNv[ey][s2];
+          acc[c][0] += nvs * sR[ex][4 * rr + 0];
+          acc[c][1] += nvs * sR[ex][4 * rr + 1];
+          acc[c][2] += nvs * sR[ex][4 * rr + 2];
+          if (RAT) acc[c][3] += nvs * sR[ex][4 * rr + 3];
+        }
       }
     }
     __syncthreads();  // sR is rewritten by the next chunk (or becomes sP)
