@@ -2374,6 +2374,70 @@ __device__ __forceinline__ void k_rank_sort_body(const Cand* in, const unsigned 
     if (last_out && r == min(keep, m) - 1) *last_out = mine;  // the last kept record
   }
 }
+// The same ordering for the batched form, where 64 queries' rank sorts would
+// each do their m^2 compares: runs of kRun records are sorted in LDS (a
+// bitonic network, one workgroup per run), then each record's position is
+// its index in its own run plus, for every other run, the number of that
+// run's records below it (a binary search; the keys are unique, so the
+// positions are a permutation and equal the rank sort's).  O(m log m) work.
+// `runs` holds the sorted runs (m_cap rounded up to kRun records); positions
+// past m in the last run hold sentinels above every real key.
+constexpr int kRun = 1024;
+__device__ __forceinline__ void k_run_sort_body(const Cand* in, const unsigned long long* m_dev, unsigned m_cap,
+                                                Cand* runs) {
+  __shared__ Cand s[kRun];
+  const unsigned m = (unsigned)min((unsigned long long)m_cap, *m_dev);
+  const unsigned base = blockIdx.x * kRun;
+  if (base >= m) return;  // whole workgroup
+  const int t = threadIdx.x;  // kRun / 2 threads, two records each
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    Cand c;
+    c.hi = c.lo = ~0ull;
+    if (base + t + h * (kRun / 2) < m) c = in[base + t + h * (kRun / 2)];
+    s[t + h * (kRun / 2)] = c;
+  }
+  __syncthreads();
+  for (int k = 2; k <= kRun; k <<= 1)
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      const int i = 2 * t - (t & (j - 1)), l = i + j;
+      const bool asc = (i & k) == 0;
+      const Cand x = s[i], y = s[l];
+      if (asc ? cand_less(y, x) : cand_less(x, y)) {
+        s[i] = y;
+        s[l] = x;
+      }
+      __syncthreads();
+    }
+  runs[base + t] = s[t];
+  runs[base + t + kRun / 2] = s[t + kRun / 2];
+}
+
+__device__ __forceinline__ void k_run_rank_body(const Cand* runs, const unsigned long long* m_dev, unsigned m_cap,
+                                                unsigned keep, Cand* out, Cand* last_out) {
+  const unsigned m = (unsigned)min((unsigned long long)m_cap, *m_dev);
+  const unsigned x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= m) return;
+  const Cand c = runs[x];
+  const unsigned r = x / kRun, nruns = (m + kRun - 1) / kRun;
+  unsigned rank = x - r * kRun;
+  for (unsigned q = 0; q < nruns; ++q) {
+    if (q == r) continue;
+    const Cand* R = runs + (size_t)q * kRun;
+    unsigned lo = 0, hi = min((unsigned)kRun, m - q * kRun);
+    while (lo < hi) {
+      const unsigned mid = (lo + hi) >> 1;
+      if (cand_less(R[mid], c))
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    rank += lo;
+  }
+  if (rank < keep) out[rank] = c;
+  if (last_out && rank == min(keep, m) - 1) *last_out = c;  // the last kept record
+}
+
 __global__ __launch_bounds__(1024) void k_rank_sort(const Cand* in, const unsigned long long* m_dev, unsigned m_cap,
                                                     unsigned keep, Cand* out, Cand* last_out) {
   k_rank_sort_body(in, m_dev, m_cap, keep, out, last_out);
@@ -4131,13 +4195,30 @@ static void topk_brec(BatchProgram& p, const DistBatchShape& sh, const std::vect
                            (const unsigned long long*)small_count(q), small_cap, (unsigned long long)k, q.fail));
     brec<1024, FT::value>(p, 1, 64, a);
   }
-  {
+  if (tq.size() <= 4) {  // few queries: the rank sort spreads over the GPU
     using FT = std::integral_constant<decltype(&k_rank_sort_body), &k_rank_sort_body>;
     std::vector<typename BodyArgs<FT::value>::type> a;
     for (const auto& q : tq)
       a.push_back(bargs<FT::value>(q.fail, (const Cand*)q.topk, (const unsigned long long*)small_count(q), small_cap,
-                           (unsigned)k, q.sorted, q.last));
+                                   (unsigned)k, q.sorted, q.last));
     brec<1024, FT::value>(p, blocks_for(small_cap, 64), 64 * kRankWaves, a);
+    return;
+  }
+  {  // sorted runs in the candidate buffer (free after the compaction)
+    using FT = std::integral_constant<decltype(&k_run_sort_body), &k_run_sort_body>;
+    std::vector<typename BodyArgs<FT::value>::type> a;
+    for (const auto& q : tq)
+      a.push_back(bargs<FT::value>(q.fail, (const Cand*)q.topk, (const unsigned long long*)small_count(q), small_cap,
+                                   q.cand));
+    brec<kRun / 2, FT::value>(p, blocks_for(small_cap, kRun), kRun / 2, a);
+  }
+  {
+    using FT = std::integral_constant<decltype(&k_run_rank_body), &k_run_rank_body>;
+    std::vector<typename BodyArgs<FT::value>::type> a;
+    for (const auto& q : tq)
+      a.push_back(bargs<FT::value>(q.fail, (const Cand*)q.cand, (const unsigned long long*)small_count(q), small_cap,
+                                   (unsigned)k, q.sorted, q.last));
+    brec<256, FT::value>(p, blocks_for(small_cap, 256), 256, a);
   }
 }
 
