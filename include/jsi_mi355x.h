@@ -210,7 +210,8 @@ int jsi_min_distance_noalloc(jsi_ctx* ctx, int h1, const jsi_region* r1, int h2,
  * "seed_pairs" (refinement seed cell pairs per unit of k, 1..256, default
  * 32), "filter_walk" (slots a refinement-filter workgroup walks, 1..16,
  * default 8), "r1_scale" (round-1 screen workgroups, percent of the default
- * grid, 10..1000).  None of them changes a result.  The table is shared by a
+ * grid, 10..1000), "r1_seeds" (round-1 seed tile pairs whose exact k-th d2
+ * bounds the threshold, 1..256, default 256).  None of them changes a result.  The table is shared by a
  * context and its server workers.  No counterpart in the reference (which
  * has fixed launch shapes); autotune.py writes the table. */
 int jsi_tune_set(jsi_ctx* ctx, int degree, int nuv, const char* knob, int value);

@@ -291,9 +291,9 @@ std::string DistBatcher::key(const jsi_query& q) {
   }
   const Tuning tn = tuning_of(q);
   char buf[256];
-  snprintf(buf, sizeof(buf), "p%d n%d,%d r%d R%d k%d c%d m%d v%d,%d,%d t%d,%d,%d", P.precision, q.r1.nuv, q.r2.nuv,
-           P.dist_refine_nuv, query_rounds(P, q.r1, q.r2), P.k, P.cull ? 1 : 0, std::min(P.k, P.polish), v1, v2, pv,
-           tn.seed_pairs, tn.filter_walk, tn.r1_scale);
+  snprintf(buf, sizeof(buf), "p%d n%d,%d r%d R%d k%d c%d m%d v%d,%d,%d t%d,%d,%d,%d", P.precision, q.r1.nuv,
+           q.r2.nuv, P.dist_refine_nuv, query_rounds(P, q.r1, q.r2), P.k, P.cull ? 1 : 0, std::min(P.k, P.polish), v1,
+           v2, pv, tn.seed_pairs, tn.filter_walk, tn.r1_scale, tn.r1_seeds);
   return buf;
 }
 

@@ -424,6 +424,7 @@ struct Tuning {
   int seed_pairs = 32;  // refinement rounds: seed cell pairs per unit of k (from the first slots)
   int filter_walk = 8;  // refinement filter: slots a workgroup walks (1..16)
   int r1_scale = 100;   // round-1 screen: workgroups, percent of the default grid
+  int r1_seeds = 256;   // round 1: seed tile pairs (the best side-1 tiles' best partners)
 };
 
 struct DistBatchShape {

@@ -3947,7 +3947,7 @@ static void topk_brec(BatchProgram& p, const DistBatchShape& sh, const std::vect
   const int pairs_tp = std::min(w1 * w1, kTileCells) * std::min(w2 * w2, kTileCells);
   const unsigned long long want =
       ((unsigned long long)sh.tune.seed_pairs * (unsigned long long)k + pairs_tp - 1) / (unsigned long long)pairs_tp;
-  const int nseed = nslots == 1 ? std::min(256, tpu1) : (int)std::max(4ull, std::min(64ull, want));
+  const int nseed = nslots == 1 ? std::min(sh.tune.r1_seeds, tpu1) : (int)std::max(4ull, std::min(64ull, want));
   const unsigned long long cap = sh.cand_cap;
   const unsigned small_cap = sh.small_cap;
   auto thr = [](const TopkQ<S>& q) { return reinterpret_cast<S*>(q.small + 6); };

@@ -328,7 +328,7 @@ int jsi_ctx::dist_topk(int nslots, const int* su1, const int* su2, int w1, int n
   // parent-rank order): about 32 k seed pairs from the first slots
   const unsigned long long want =
       ((unsigned long long)tune_.seed_pairs * (unsigned long long)k + pairs_tp - 1) / (unsigned long long)pairs_tp;
-  const int nseed = nslots == 1 ? std::min(256, tpu1) : (int)std::max(4ull, std::min(64ull, want));
+  const int nseed = nslots == 1 ? std::min(tune_.r1_seeds, tpu1) : (int)std::max(4ull, std::min(64ull, want));
   if (npos <= 8192) {
     unsigned long long *bestv = nullptr, *range, *work;
     RCK(buf("d.range", 2, &range));
@@ -1836,6 +1836,8 @@ extern "C" int jsi_tune_set(jsi_ctx* ctx, int degree, int nuv, const char* knob,
     t.filter_walk = value;
   } else if (k == "r1_scale" && value >= 10 && value <= 1000) {
     t.r1_scale = value;
+  } else if (k == "r1_seeds" && value >= 1 && value <= 256) {
+    t.r1_seeds = value;
   } else {
     return JSI_ERR_INVALID;
   }

@@ -23,7 +23,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libjsi_mi355x.so")
 # autotuned launch choices per (degree, nuv) (autotune.py writes it; jsi_tune_set)
 TUNING_PATH = os.path.join(HERE, "tuning.json")
-TUNING_KNOBS = ("seed_pairs", "filter_walk", "r1_scale")
+TUNING_KNOBS = ("seed_pairs", "filter_walk", "r1_scale", "r1_seeds")
 
 _f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
