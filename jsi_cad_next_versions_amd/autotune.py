@@ -10,7 +10,8 @@ from the built-in defaults, the value of each knob of jsi_tune_set:
   tighter threshold costs a larger seed);
 * ``filter_walk``: slots a refinement-filter workgroup walks (fewer, longer
   lived workgroups against more, shorter ones);
-* ``r1_scale``: round-1 screen workgroups, percent of the default grid.
+* ``r1_scale``: round-1 screen workgroups, percent of the default grid;
+* ``r1_seeds``: round-1 seed tile pairs (fewer loosen the screen's threshold).
 
 A change is kept when it is faster by more than ``--min-gain`` (run-to-run
 noise).  The knobs change launch shapes and thresholds only, never a result.
@@ -34,8 +35,9 @@ CANDIDATES = {
     "seed_pairs": [8, 16, 32, 64, 128],
     "filter_walk": [2, 4, 8, 16],
     "r1_scale": [50, 100, 200, 400],
+    "r1_seeds": [64, 128, 256],
 }
-DEFAULTS = {"seed_pairs": 32, "filter_walk": 8, "r1_scale": 100}
+DEFAULTS = {"seed_pairs": 32, "filter_walk": 8, "r1_scale": 100, "r1_seeds": 256}
 
 
 WORKERS = 4

@@ -151,16 +151,17 @@ def test_graph_replays_with_new_inputs(ctx):
 
 def test_tuning_changes_no_result(ctx):
     """Extreme autotuner choices (seed size, refinement-filter walk, round-1
-    grid) change launch shapes and thresholds only: the single-query and the
-    batched results equal the built-in choices' results."""
+    grid, round-1 seed tile pairs) change launch shapes and thresholds only:
+    the single-query and the batched results equal the built-in choices'
+    results."""
     a, b = S.hard_wave_pair()
     p = default_params(k=1024, precision="fp32screen", polish=16, trace=1)
     r1, r2 = S.full_region(a, 128, 1), S.full_region(b, 128, 2)
     ctx.tune_clear()
     ref = ctx.minimum_distance(a, r1, b, r2, p)[0]
     try:
-        for seed, walk, scale in ((8, 2, 50), (128, 16, 400)):
-            for knob, v in (("seed_pairs", seed), ("filter_walk", walk), ("r1_scale", scale)):
+        for seed, walk, scale, r1s in ((8, 2, 50, 1), (128, 16, 400, 256)):
+            for knob, v in (("seed_pairs", seed), ("filter_walk", walk), ("r1_scale", scale), ("r1_seeds", r1s)):
                 ctx.tune_set(3, 128, knob, v)
             same(ctx.minimum_distance(a, r1, b, r2, p)[0], ref)
             for r in run_batched(ctx, [(a, b, r1, r2, p)] * 5):
