@@ -389,6 +389,7 @@ struct SlotScreen {
   const vec4_t<T>*C1, *R1, *C2, *R2;
   const vec4_t<T>*lo1, *hi1, *lo2, *hi2;  // unit boxes (cull)
   int cull;
+  SlotF32 f;  // FP32SCREEN (f.X1 non-null): float records and unit boxes
 };
 template <class T>
 struct SlotUnits {
@@ -410,27 +411,48 @@ __host__ __device__ inline int slot_groups(int cells1, int cells2) {
   return a < b ? a : (b > 0 ? b : 1);
 }
 
-// The unit-box test of one slot (the tile-pair cull of k_isect_list).
+// The unit-box test of one slot (the tile-pair cull of k_isect_list; the
+// float boxes of the records under FP32SCREEN).
+template <class B>
+__device__ __forceinline__ bool boxes_meet(const B& l1, const B& h1, const B& l2, const B& h2) {
+  return l1.x <= h2.x && l2.x <= h1.x && l1.y <= h2.y && l2.y <= h1.y && l1.z <= h2.z && l2.z <= h1.z;
+}
 template <class T>
 __device__ __forceinline__ bool slot_boxes_meet(const SlotScreen<T>& s, long long slot) {
   if (!s.cull) return true;
   const int u1 = s.su1[slot], u2 = s.su2[slot];
-  const vec4_t<T> l1 = s.lo1[u1], h1 = s.hi1[u1], l2 = s.lo2[u2], h2 = s.hi2[u2];
-  return l1.x <= h2.x && l2.x <= h1.x && l1.y <= h2.y && l2.y <= h1.y && l1.z <= h2.z && l2.z <= h1.z;
+  if (s.f.X1) return boxes_meet(s.f.lo1[u1], s.f.hi1[u1], s.f.lo2[u2], s.f.hi2[u2]);
+  return boxes_meet(s.lo1[u1], s.hi1[u1], s.lo2[u2], s.hi2[u2]);
 }
 
+// The strict 3-axis overlap test of overlap_aabb_host (intersection.cu:46-63)
+// in the records' own arithmetic (k_isect_pass's expression).
+template <class V>
+__device__ __forceinline__ bool cells_overlap(const V& c1, const V& r1, const V& c2, const V& r2) {
+  return fabs(c1.x - c2.x) < (r1.x + r2.x) && fabs(c1.y - c2.y) < (r1.y + r2.y) && fabs(c1.z - c2.z) < (r1.z + r2.z);
+}
+
+// The hits of row (slot, i): bit j = cell j of the side-2 unit.  Under
+// FP32SCREEN the float records' test picks the candidates first and the FP64
+// test decides them (k_isect_retest's decision), so the hits are the FP64
+// screen's.
 template <class T>
 __device__ __forceinline__ unsigned long long slot_row_mask(const SlotScreen<T>& s, long long slot, int i) {
   const int u1 = s.su1[slot], u2 = s.su2[slot];
-  const vec4_t<T> c1 = s.C1[(size_t)u1 * s.cells1 + i], r1 = s.R1[(size_t)u1 * s.cells1 + i];
-  const vec4_t<T>* const c2p = s.C2 + (size_t)u2 * s.cells2;
-  const vec4_t<T>* const r2p = s.R2 + (size_t)u2 * s.cells2;
+  const size_t a = (size_t)u1 * s.cells1 + i, b0 = (size_t)u2 * s.cells2;
+  unsigned long long cand = ~0ull >> (64 - s.cells2);
+  if (s.f.X1) {
+    const float4 x1 = s.f.X1[a], y1 = s.f.Y1[a];
+    cand = 0;
+    for (int j = 0; j < s.cells2; ++j)
+      cand |= (unsigned long long)cells_overlap(x1, y1, s.f.X2[b0 + j], s.f.Y2[b0 + j]) << j;
+    if (!cand) return 0;
+  }
+  const vec4_t<T> c1 = s.C1[a], r1 = s.R1[a];
   unsigned long long m = 0;
-  for (int j = 0; j < s.cells2; ++j) {
-    const vec4_t<T> c2 = c2p[j], r2 = r2p[j];
-    const bool hit = fabs(c1.x - c2.x) < (r1.x + r2.x) && fabs(c1.y - c2.y) < (r1.y + r2.y) &&
-                     fabs(c1.z - c2.z) < (r1.z + r2.z);
-    m |= (unsigned long long)hit << j;
+  for (unsigned long long r = cand; r; r &= r - 1) {
+    const int j = __ffsll((long long)r) - 1;
+    m |= (unsigned long long)cells_overlap(c1, r1, s.C2[b0 + j], s.R2[b0 + j]) << j;
   }
   return m;
 }
@@ -642,8 +664,10 @@ hipError_t launch_isect_cells(const IcJob<T>& a, const IcJob<T>* b, int use_k, h
 template <class T>
 static SlotScreen<T> slot_screen(long long nslots, int w1, int w2, const int* su1, const int* su2, const vec4_t<T>* C1,
                                  const vec4_t<T>* R1, const vec4_t<T>* C2, const vec4_t<T>* R2, const vec4_t<T>* lo1,
-                                 const vec4_t<T>* hi1, const vec4_t<T>* lo2, const vec4_t<T>* hi2, int cull) {
+                                 const vec4_t<T>* hi1, const vec4_t<T>* lo2, const vec4_t<T>* hi2, int cull,
+                                 const SlotF32* f) {
   SlotScreen<T> s;
+  s.f = f ? *f : SlotF32();
   s.nslots = nslots;
   s.cells1 = w1 * w1;
   s.cells2 = w2 * w2;
@@ -668,7 +692,14 @@ static unsigned slot_blocks(long long n, int cells1, int cells2) {
 }
 
 static bool slot_args_ok(long long nslots, int w1, int w2, int cull, const void* lo1, const void* hi1,
-                         const void* lo2, const void* hi2) {
+                         const void* lo2, const void* hi2, const SlotF32* f) {
+  if (f && (!f->X1 || !f->Y1 || !f->X2 || !f->Y2)) return false;
+  if (f) {
+    lo1 = f->lo1;
+    hi1 = f->hi1;
+    lo2 = f->lo2;
+    hi2 = f->hi2;
+  }
   return w1 >= 1 && w2 >= 1 && w1 <= kIcMaxNuv && w2 <= kIcMaxNuv && nslots <= INT_MAX &&
          !(cull && (!lo1 || !hi1 || !lo2 || !hi2));
 }
@@ -677,11 +708,11 @@ template <class T>
 hipError_t launch_isect_slot_cull(long long nslots, int w1, int w2, const int* su1, const int* su2,
                                   const vec4_t<T>* lo1, const vec4_t<T>* hi1, const vec4_t<T>* lo2,
                                   const vec4_t<T>* hi2, unsigned* wcount, const unsigned long long* woff, int* list,
-                                  hipStream_t s) {
+                                  const SlotF32* f, hipStream_t s) {
   if (nslots <= 0) return hipSuccess;
-  if (!slot_args_ok(nslots, w1, w2, 1, lo1, hi1, lo2, hi2) || (!wcount == !list)) return hipErrorInvalidValue;
+  if (!slot_args_ok(nslots, w1, w2, 1, lo1, hi1, lo2, hi2, f) || (!wcount == !list)) return hipErrorInvalidValue;
   const SlotScreen<T> sc = slot_screen<T>(nslots, w1, w2, su1, su2, nullptr, nullptr, nullptr, nullptr, lo1, hi1,
-                                          lo2, hi2, 1);
+                                          lo2, hi2, 1, f);
   const dim3 g((unsigned)((nslots + 255) / 256)), b(256);
   if (wcount)
     hipLaunchKernelGGL(k_isect_slot_cull<T>, g, b, 0, s, sc, wcount);
@@ -695,10 +726,12 @@ hipError_t launch_isect_slot_count(long long nslots, int w1, int w2, const int* 
                                    const vec4_t<T>* C1, const vec4_t<T>* R1, const vec4_t<T>* C2,
                                    const vec4_t<T>* R2, const vec4_t<T>* lo1, const vec4_t<T>* hi1,
                                    const vec4_t<T>* lo2, const vec4_t<T>* hi2, int cull, long long n,
-                                   const int* list, unsigned* npair, unsigned* nrow, unsigned* ncol, hipStream_t s) {
+                                   const int* list, unsigned* npair, unsigned* nrow, unsigned* ncol, const SlotF32* f,
+                                   hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  if (!slot_args_ok(nslots, w1, w2, list ? 0 : cull, lo1, hi1, lo2, hi2) || n > nslots) return hipErrorInvalidValue;
-  const SlotScreen<T> sc = slot_screen<T>(nslots, w1, w2, su1, su2, C1, R1, C2, R2, lo1, hi1, lo2, hi2, cull);
+  if (!slot_args_ok(nslots, w1, w2, list ? 0 : cull, lo1, hi1, lo2, hi2, f) || n > nslots)
+    return hipErrorInvalidValue;
+  const SlotScreen<T> sc = slot_screen<T>(nslots, w1, w2, su1, su2, C1, R1, C2, R2, lo1, hi1, lo2, hi2, cull, f);
   hipLaunchKernelGGL(k_isect_slot_count<T>, dim3(slot_blocks(n, sc.cells1, sc.cells2)), dim3(256), 0, s, sc, n, list,
                      npair, nrow, ncol);
   return hipGetLastError();
@@ -712,10 +745,11 @@ hipError_t launch_isect_slot_emit(long long nslots, int w1, int w2, const int* s
                                   const unsigned long long* offp, const unsigned long long* offr,
                                   const unsigned long long* offc, IPair* out, const vec4_t<T>* reg1,
                                   const vec4_t<T>* reg2, int pnuv1, int pnuv2, int* nsu1, int* nsu2,
-                                  vec4_t<T>* nreg1, vec4_t<T>* nreg2, hipStream_t s) {
+                                  vec4_t<T>* nreg1, vec4_t<T>* nreg2, const SlotF32* f, hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  if (!slot_args_ok(nslots, w1, w2, list ? 0 : cull, lo1, hi1, lo2, hi2) || n > nslots) return hipErrorInvalidValue;
-  const SlotScreen<T> sc = slot_screen<T>(nslots, w1, w2, su1, su2, C1, R1, C2, R2, lo1, hi1, lo2, hi2, cull);
+  if (!slot_args_ok(nslots, w1, w2, list ? 0 : cull, lo1, hi1, lo2, hi2, f) || n > nslots)
+    return hipErrorInvalidValue;
+  const SlotScreen<T> sc = slot_screen<T>(nslots, w1, w2, su1, su2, C1, R1, C2, R2, lo1, hi1, lo2, hi2, cull, f);
   SlotUnits<T> su;
   su.reg1 = reg1;
   su.reg2 = reg2;
@@ -742,17 +776,17 @@ hipError_t launch_gather_totals(const unsigned long long* a, const unsigned long
 #define JSI_IS(T)                                                                                                  \
   template hipError_t launch_isect_slot_cull<T>(long long, int, int, const int*, const int*, const vec4_t<T>*,     \
                                                 const vec4_t<T>*, const vec4_t<T>*, const vec4_t<T>*, unsigned*,    \
-                                                const unsigned long long*, int*, hipStream_t);                      \
+                                                const unsigned long long*, int*, const SlotF32*, hipStream_t);      \
   template hipError_t launch_isect_slot_count<T>(long long, int, int, const int*, const int*, const vec4_t<T>*,    \
                                                  const vec4_t<T>*, const vec4_t<T>*, const vec4_t<T>*,             \
                                                  const vec4_t<T>*, const vec4_t<T>*, const vec4_t<T>*,             \
                                                  const vec4_t<T>*, int, long long, const int*, unsigned*, unsigned*, \
-                                                 unsigned*, hipStream_t);                                           \
+                                                 unsigned*, const SlotF32*, hipStream_t);                           \
   template hipError_t launch_isect_slot_emit<T>(                                                                   \
       long long, int, int, const int*, const int*, const vec4_t<T>*, const vec4_t<T>*, const vec4_t<T>*,           \
       const vec4_t<T>*, const vec4_t<T>*, const vec4_t<T>*, const vec4_t<T>*, const vec4_t<T>*, int, long long,    \
       const int*, const unsigned long long*, const unsigned long long*, const unsigned long long*, IPair*,         \
-      const vec4_t<T>*, const vec4_t<T>*, int, int, int*, int*, vec4_t<T>*, vec4_t<T>*, hipStream_t);
+      const vec4_t<T>*, const vec4_t<T>*, int, int, int*, int*, vec4_t<T>*, vec4_t<T>*, const SlotF32*, hipStream_t);
 JSI_IS(float)
 JSI_IS(double)
 #undef JSI_IS

@@ -1040,7 +1040,11 @@ int jsi_ctx::intersect(int h1, const jsi_region& r1, int h2, const jsi_region& r
     const int w1 = nuv1 - 1, w2 = nuv2 - 1;
     const int tpu1 = ((w1 + kTile - 1) / kTile) * ((w1 + kTile - 1) / kTile);
     const int tpu2 = ((w2 + kTile - 1) / kTile) * ((w2 + kTile - 1) / kTile);
-    if (s32) {
+    // units of at most 64 cells are screened per slot (FP32SCREEN too, with
+    // its float records; the screen_out hook keeps the candidate list form)
+    const bool small = fuse_cells && w1 <= kIcMaxNuv && w2 <= kIcMaxNuv;
+    SlotF32 f32;
+    if (s32 && !(small && !P.screen_out)) {
       if constexpr (std::is_same<T, double>::value) {
         RCK(isect_screen32(round, (int)nslots, su1, su2, nunits1, w1, nunits2, w2, C1, R1, C2, R2,
                            round == 1 ? V1c : nullptr, round == 1 ? V1r : nullptr, round == 1 ? V2c : nullptr,
@@ -1049,8 +1053,36 @@ int jsi_ctx::intersect(int h1, const jsi_region& r1, int h2, const jsi_region& r
         if (round == 1 && use_k) RCK(initial_k<T>(nuv1, nuv2, r1, r2, iu, iv, res));
       }
     } else {
+      if constexpr (std::is_same<T, double>::value) {
+        if (s32) {  // float records of both sides (round 1: covering VERSION5's float boxes too) and their unit boxes
+          const size_t n1 = (size_t)nunits1 * w1 * w1, n2 = (size_t)nunits2 * w2 * w2;
+          float4 *X1, *Y1, *X2, *Y2, *fl1, *fh1, *fl2, *fh2;
+          RCK(buf("i32.X1", n1, &X1));
+          RCK(buf("i32.Y1", n1, &Y1));
+          RCK(buf("i32.X2", n2, &X2));
+          RCK(buf("i32.Y2", n2, &Y2));
+          HCK(launch_isect_boxes32(C1, R1, round == 1 ? V1c : nullptr, round == 1 ? V1r : nullptr, n1, X1, Y1,
+                                   stream));
+          HCK(launch_isect_boxes32(C2, R2, round == 1 ? V2c : nullptr, round == 1 ? V2r : nullptr, n2, X2, Y2,
+                                   stream));
+          RCK(buf("tb32.lo1", (size_t)nunits1, &fl1));
+          RCK(buf("tb32.hi1", (size_t)nunits1, &fh1));
+          RCK(buf("tb32.lo2", (size_t)nunits2, &fl2));
+          RCK(buf("tb32.hi2", (size_t)nunits2, &fh2));
+          HCK(launch_tilebox<float>(X1, Y1, nunits1, w1, 1, fl1, fh1, stream));
+          HCK(launch_tilebox<float>(X2, Y2, nunits2, w2, 1, fl2, fh2, stream));
+          f32.X1 = X1;
+          f32.Y1 = Y1;
+          f32.X2 = X2;
+          f32.Y2 = Y2;
+          f32.lo1 = fl1;
+          f32.hi1 = fh1;
+          f32.lo2 = fl2;
+          f32.hi2 = fh2;
+        }
+      }
       V *lo1 = tlo1, *hi1 = thi1, *lo2 = tlo2, *hi2 = thi2;
-      if (!lo1) {  // tile boxes not written by the evaluation
+      if (!lo1 && !s32) {  // tile boxes not written by the evaluation
         RCK(buf("tb.lo1", (size_t)nunits1 * tpu1, &lo1));
         RCK(buf("tb.hi1", (size_t)nunits1 * tpu1, &hi1));
         RCK(buf("tb.lo2", (size_t)nunits2 * tpu2, &lo2));
@@ -1063,10 +1095,11 @@ int jsi_ctx::intersect(int h1, const jsi_region& r1, int h2, const jsi_region& r
         set_err("intersection round %d: %lld slots exceed the supported range", round, nslots);
         return JSI_ERR_CAPACITY;
       }
-      if (fuse_cells && w1 <= kIcMaxNuv && w2 <= kIcMaxNuv) {
+      if (small) {
         // units of at most 64 cells: per-slot counts of pairs and new units,
         // their scans, then the ordered pairs and (when a round follows) the
         // next round's units in one pass
+        const SlotF32* fp = s32 ? &f32 : nullptr;
         // many slots: cull them one per lane first and list those that pass
         size_t ns = (size_t)nslots;
         int* slist = nullptr;
@@ -1080,10 +1113,11 @@ int jsi_ctx::intersect(int h1, const jsi_region& r1, int h2, const jsi_region& r
           RCK(buf("i.slist", ns, &slist));
           const size_t wtb = scan_tmp_bytes(nw);
           RCK(buf_raw("i.scantmp", wtb, &wtmp, 0));
-          HCK(launch_isect_slot_cull<T>(nslots, w1, w2, su1, su2, lo1, hi1, lo2, hi2, wcount, nullptr, nullptr,
+          HCK(launch_isect_slot_cull<T>(nslots, w1, w2, su1, su2, lo1, hi1, lo2, hi2, wcount, nullptr, nullptr, fp,
                                         stream));
           HCK(scan_u32_to_u64(wcount, woff, nw, wtmp, wtb, stream));
-          HCK(launch_isect_slot_cull<T>(nslots, w1, w2, su1, su2, lo1, hi1, lo2, hi2, nullptr, woff, slist, stream));
+          HCK(launch_isect_slot_cull<T>(nslots, w1, w2, su1, su2, lo1, hi1, lo2, hi2, nullptr, woff, slist, fp,
+                                        stream));
           unsigned long long nl = 0;
           RCK(d2h(&nl, woff + nw, sizeof(nl)));
           ns = (size_t)nl;
@@ -1099,7 +1133,7 @@ int jsi_ctx::intersect(int h1, const jsi_region& r1, int h2, const jsi_region& r
         unsigned *cp = cnt, *cr = cnt + ns, *cc = cnt + 2 * ns;
         unsigned long long *op = off, *orr = off + ns + 1, *oc = off + 2 * (ns + 1);
         HCK(launch_isect_slot_count<T>(nslots, w1, w2, su1, su2, C1, R1, C2, R2, lo1, hi1, lo2, hi2, P.cull,
-                                       (long long)ns, slist, cp, cr, cc, stream));
+                                       (long long)ns, slist, cp, cr, cc, fp, stream));
         HCK(scan_u32_to_u64(cp, op, ns, scantmp, stb, stream));
         HCK(scan_u32_to_u64(cr, orr, ns, scantmp, stb, stream));
         HCK(scan_u32_to_u64(cc, oc, ns, scantmp, stb, stream));
@@ -1126,7 +1160,7 @@ int jsi_ctx::intersect(int h1, const jsi_region& r1, int h2, const jsi_region& r
         if (npairs > 0)
           HCK(launch_isect_slot_emit<T>(nslots, w1, w2, su1, su2, C1, R1, C2, R2, lo1, hi1, lo2, hi2, P.cull,
                                         (long long)ns, slist, op, orr, oc, pairs, reg1, reg2, nuv1, nuv2, nsu1, nsu2,
-                                        nreg1, nreg2, stream));
+                                        nreg1, nreg2, fp, stream));
       } else {
         uint64_t* list;
         unsigned long long* small;
