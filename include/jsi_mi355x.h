@@ -123,8 +123,8 @@ typedef struct jsi_result {
   int rounds;            /* rounds screened */
   int isect_refine_nuv;  /* intersection: refinement nuv after the query (halving state) */
   double ms_total;       /* host wall clock for the whole query */
-  double ms_eval;        /* evaluation + AABB construction */
-  double ms_screen;      /* screening + top-k selection */
+  double ms_eval;        /* evaluation + AABB construction (measured in profiling mode */
+  double ms_screen;      /* screening + top-k selection     or with CAD_TIMING_BREAKDOWN set) */
   jsi_round_info round[JSI_MAX_ROUNDS];
   jsi_closest closest;   /* distance with jsi_params.polish > 0 */
   /* intersection with use_compute_k: compute_K of the two initial regions

@@ -989,7 +989,10 @@ int jsi_ctx::intersect(int h1, const jsi_region& r1, int h2, const jsi_region& r
   RCK(buf(nm("su", 2, cur).c_str(), 1, &su2));
   HCK(launch_fill_i32(su1, 0, 1, stream));
   HCK(launch_fill_i32(su2, 0, 1, stream));
-  if (prof) {
+  // the phase timings cost a synchronisation each: taken when profiling or
+  // when the legacy API prints them (CAD_TIMING_BREAKDOWN)
+  const bool timing = prof || std::getenv("CAD_TIMING_BREAKDOWN") != nullptr;
+  if (timing) {
     HCK(hipStreamSynchronize(stream));
     res->ms_eval += now_ms() - te;
   }
@@ -1199,8 +1202,10 @@ int jsi_ctx::intersect(int h1, const jsi_region& r1, int h2, const jsi_region& r
         }
       }
     }
-    HCK(hipStreamSynchronize(stream));
-    res->ms_screen += now_ms() - ts;
+    if (timing) {
+      HCK(hipStreamSynchronize(stream));
+      res->ms_screen += now_ms() - ts;
+    }
     jsi_round_info& ri = res->round[round - 1];
     ri.npairs = (int64_t)npairs;
     ri.slots = nslots;
@@ -1314,8 +1319,10 @@ int jsi_ctx::intersect(int h1, const jsi_region& r1, int h2, const jsi_region& r
       RCK(isect_units<T>(h1, nreg1, (int)nu1, g_nuv, use_k, iu[1], iv[1], nC1, nR1, "i1"));
       RCK(isect_units<T>(h2, nreg2, (int)nu2, g_nuv, use_k, iu[2], iv[2], nC2, nR2, "i2"));
     }
-    HCK(hipStreamSynchronize(stream));
-    res->ms_eval += now_ms() - te;
+    if (timing) {
+      HCK(hipStreamSynchronize(stream));
+      res->ms_eval += now_ms() - te;
+    }
     cur = nxt;
     su1 = nsu1;
     su2 = nsu2;
@@ -1373,6 +1380,7 @@ int jsi_ctx::intersect(int h1, const jsi_region& r1, int h2, const jsi_region& r
     RCK(d2h(hp.data(), pairs, sizeof(IPair) * m));
     for (unsigned long long e = 0; e < m; ++e) out[e] = jsi_pair{0.0, hp[e].slot, hp[e].i, hp[e].j, 0};
   }
+  HCK(hipStreamSynchronize(stream));  // the query's last kernels have finished when it returns
   res->ms_total = now_ms() - t0;
   return JSI_OK;
 }
