@@ -594,14 +594,14 @@ int ic_blocks(const IcJob<T>& j) {
   return (j.nunits + upw - 1) / upw;
 }
 
-// variant of a job: 0-2 = degree class (3, 7, kMaxDeg - 1), +3 rational,
-// +6 bicubic (degree class 0 only)
+// variant of a job: 0-3 = degree class (3, 5, 7, kMaxDeg - 1), +4 rational,
+// +8 bicubic (degree class 0 only)
 template <class T>
 int ic_variant(const IcJob<T>& j) {
   const int d = std::max(j.sd.degu, j.sd.degv);
-  const int dc = d <= 3 ? 0 : d <= 7 ? 1 : 2;
+  const int dc = d <= 3 ? 0 : d <= 5 ? 1 : d <= 7 ? 2 : 3;
   const bool bic = j.sd.degu == 3 && j.sd.degv == 3 && !j.sd.zero;
-  return dc + (j.sd.rational ? 3 : 0) + (bic ? 6 : 0);
+  return dc + (j.sd.rational ? 4 : 0) + (bic ? 8 : 0);
 }
 
 template <class T, int MAXD, bool RAT, bool BIC>
@@ -615,7 +615,7 @@ void ic_launch_k(int use_k, const IcJob<T>& j0, const IcJob<T>& j1, int nb0, int
 
 template <class T, int MAXD>
 void ic_launch_rat(int var, int use_k, const IcJob<T>& j0, const IcJob<T>& j1, int nb0, int nb, hipStream_t s) {
-  const bool rat = (var / 3) % 2 == 1, bic = var >= 6;
+  const bool rat = (var / 4) % 2 == 1, bic = var >= 8;
   if constexpr (MAXD == 3) {
     if (bic) {
       if (rat)
@@ -634,9 +634,10 @@ void ic_launch_rat(int var, int use_k, const IcJob<T>& j0, const IcJob<T>& j1, i
 template <class T>
 void ic_launch(int var, int use_k, const IcJob<T>& j0, const IcJob<T>& j1, int nb0, int nb, hipStream_t s) {
   if (nb <= 0) return;
-  switch (var % 3) {
+  switch (var % 4) {
     case 0: ic_launch_rat<T, 3>(var, use_k, j0, j1, nb0, nb, s); break;
-    case 1: ic_launch_rat<T, 7>(var, use_k, j0, j1, nb0, nb, s); break;
+    case 1: ic_launch_rat<T, 5>(var, use_k, j0, j1, nb0, nb, s); break;
+    case 2: ic_launch_rat<T, 7>(var, use_k, j0, j1, nb0, nb, s); break;
     default: ic_launch_rat<T, kMaxDeg - 1>(var, use_k, j0, j1, nb0, nb, s); break;
   }
 }

@@ -2742,19 +2742,56 @@ __global__ void k_isect_scatter(const IPair* cand, const unsigned* flag, const u
   if (e < n && flag[e]) out[idx[e]] = cand[e];
 }
 
-// Insertion sort of each row's pairs by j (rows are short).
+// Sort of each row's pairs by j.  A row's pairs share (slot, i), so only
+// their j values move: rows of up to 64 pairs are sorted in registers by a
+// bitonic network of 16, 32 or 64 (indices fixed at compile time, padding
+// above every j), longer rows by insertion in place.
+template <int N>
+__device__ __forceinline__ void row_sort_net(IPair* row, int len) {
+  int v[N];
+#pragma unroll
+  for (int x = 0; x < N; ++x) v[x] = x < len ? row[x].j : INT_MAX;
+#pragma unroll
+  for (int k = 2; k <= N; k <<= 1)
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1)
+#pragma unroll
+      for (int x = 0; x < N; ++x) {
+        const int l = x ^ j;
+        if (l > x) {
+          const int a = v[x], b = v[l];
+          const bool asc = (x & k) == 0;
+          v[x] = asc ? min(a, b) : max(a, b);
+          v[l] = asc ? max(a, b) : min(a, b);
+        }
+      }
+#pragma unroll
+  for (int x = 0; x < N; ++x)
+    if (x < len) row[x].j = v[x];
+}
+
 __global__ void k_row_sort(const unsigned long long* rowoff, unsigned long long nrows, IPair* out) {
   const unsigned long long r = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nrows) return;
   const unsigned long long b = rowoff[r], e = rowoff[r + 1];
-  for (unsigned long long x = b + 1; x < e; ++x) {
-    const IPair v = out[x];
-    unsigned long long y = x;
-    while (y > b && out[y - 1].j > v.j) {
-      out[y] = out[y - 1];
-      --y;
+  const unsigned long long len = e - b;
+  if (len <= 1) return;
+  if (len <= 16) {
+    row_sort_net<16>(out + b, (int)len);
+  } else if (len <= 32) {
+    row_sort_net<32>(out + b, (int)len);
+  } else if (len <= 64) {
+    row_sort_net<64>(out + b, (int)len);
+  } else {
+    for (unsigned long long x = b + 1; x < e; ++x) {
+      const IPair v = out[x];
+      unsigned long long y = x;
+      while (y > b && out[y - 1].j > v.j) {
+        out[y] = out[y - 1];
+        --y;
+      }
+      out[y] = v;
     }
-    out[y] = v;
   }
 }
 
