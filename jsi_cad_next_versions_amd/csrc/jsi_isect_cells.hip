@@ -65,14 +65,21 @@ __global__ __launch_bounds__(64) void k_isect_cells(IcJob<T> j0, IcJob<T> j1, in
   constexpr int NC = RAT ? 4 : 3;
   constexpr int NK = PADK ? 3 : 1;  // derivative orders kept per direction
   constexpr int kRows = kIcRows;
-  constexpr int kRsz = kIcMaxCols * kRows * NK * NC;
+  // u-contractions of a column: kRows x NK entries of NC values, read as
+  // 16-byte words; an odd number of such words per column puts the columns
+  // that neighbouring lanes read on distinct banks
+  constexpr int kRw = kRows * NK * NC;
+  static_assert(kRw * sizeof(T) % 16 == 0, "column of u-contractions: whole 16-byte words");
+  constexpr int kRs = (kRw * (int)sizeof(T) / 16) % 2 == 0 ? kRw + 16 / (int)sizeof(T) : kRw;
+  constexpr int kRsz = kIcMaxCols * kRs;
   constexpr int kA = kRsz > 3 * 64 ? kRsz : 3 * 64;
   __shared__ T sN[2 * kIcMaxCols][NK][NB];  // A2.3 rows: columns [0, 16), rows [16, 32)
   __shared__ int sS[2 * kIcMaxCols];        // their spans
   // two lives: u-contractions [column][row][k][c] (phases 2-3), then the
   // points' coordinates [c][lane] (phase 5); a barrier separates them
   __shared__ T sA[kA];
-  T(*const sR)[kRows][NK][NC] = reinterpret_cast<T(*)[kRows][NK][NC]>(sA);
+  // entry (column, row, u order) of the u-contractions
+  const auto sR = [&](int cx, int rr, int k) -> T* { return sA + cx * kRs + (rr * NK + k) * NC; };
   const bool second = (int)blockIdx.x >= nb0;  // uniform: one body, its arguments selected
   const SurfDev<T> sd = second ? j1.sd : j0.sd;
   const vec4_t<T>* const regions = second ? j1.reg : j0.reg;
@@ -177,7 +184,7 @@ __global__ __launch_bounds__(64) void k_isect_cells(IcJob<T> j0, IcJob<T> j1, in
       for (int k = 0; k < NK; ++k)
         if (k <= ku)
 #pragma unroll
-          for (int c = 0; c < NC; ++c) sR[cs][rr][k][c] = acc[k][c];
+          for (int c = 0; c < NC; ++c) sR(cs, rr, k)[c] = acc[k][c];
     }
     __syncthreads();
     // phase 3: each point adds this chunk's rows, in s order, for every (k, l)
@@ -199,7 +206,7 @@ __global__ __launch_bounds__(64) void k_isect_cells(IcJob<T> j0, IcJob<T> j1, in
                 if (s2 > q) break;
                 const T nv = sN[rpt][l][s2];
 #pragma unroll
-                for (int c = 0; c < NC; ++c) h[m][c] += nv * sR[cpt][rb + s2][k][c];
+                for (int c = 0; c < NC; ++c) h[m][c] += nv * sR(cpt, rb + s2, k)[c];
               }
             }
         }
@@ -218,7 +225,7 @@ __global__ __launch_bounds__(64) void k_isect_cells(IcJob<T> j0, IcJob<T> j1, in
               if (rr < 0 || rr >= nr) continue;
               const T nv = sN[rpt][l][s2];
 #pragma unroll
-              for (int c = 0; c < NC; ++c) h[m][c] += nv * sR[cpt][rr][k][c];
+              for (int c = 0; c < NC; ++c) h[m][c] += nv * sR(cpt, rr, k)[c];
             }
           }
       }

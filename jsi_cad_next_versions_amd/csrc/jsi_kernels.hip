@@ -2956,12 +2956,20 @@ __global__ __launch_bounds__(256) void k_points_tile(SurfDev<T> sd, const vec4_t
   constexpr int kKn = KL ? kEcKnots : 1;
   // one buffer, three lives: knots (phases 0-1), u-contractions (phases 2-3)
   // and, for the 18-value output, the tile's results; barriers separate them
-  constexpr int kKsz = 2 * kKn, kRsz = TP * kRows * NK * NC, kOsz = MODE == kEvalFull18 ? TP * TP * 18 : 1;
+  // u-contractions of a column: kRows x NK entries of NC values, read as
+  // 16-byte words; the column stride is an odd number of such words, so the
+  // 16 columns that the lanes of a wavefront read fall on distinct banks (an
+  // even stride put them on 1-2 bank groups: 8- and 16-way conflicts)
+  constexpr int kRw = kRows * NK * NC;
+  static_assert(kRw * sizeof(T) % 16 == 0, "column of u-contractions: whole 16-byte words");
+  constexpr int kRs = (kRw * (int)sizeof(T) / 16) % 2 == 0 ? kRw + 16 / (int)sizeof(T) : kRw;
+  constexpr int kKsz = 2 * kKn, kRsz = TP * kRs, kOsz = MODE == kEvalFull18 ? TP * TP * 18 : 1;
   constexpr int kA = kKsz > kRsz ? (kKsz > kOsz ? kKsz : kOsz) : (kRsz > kOsz ? kRsz : kOsz);
   __shared__ T sA[kA];
   T* const sKu = sA;
   T* const sKv = sA + kKn;
-  T(*const sR)[kRows][NK][NC] = reinterpret_cast<T(*)[kRows][NK][NC]>(sA);
+  // entry (column, row, u order) of the u-contractions
+  const auto sR = [&](int cx, int rr, int k) -> T* { return sA + cx * kRs + (rr * NK + k) * NC; };
   __shared__ T sNu[TP][NK][NB], sNv[TP][NK][NB];
   __shared__ int sSu[TP], sSv[TP];
   __shared__ unsigned long long sk[4][3];
@@ -3063,7 +3071,7 @@ __global__ __launch_bounds__(256) void k_points_tile(SurfDev<T> sd, const vec4_t
       for (int k = 0; k < NK; ++k)
         if (k <= ku)
 #pragma unroll
-          for (int c = 0; c < NC; ++c) sR[cx][rr][k][c] = acc[k][c];
+          for (int c = 0; c < NC; ++c) sR(cx, rr, k)[c] = acc[k][c];
     }
     __syncthreads();
     // phase 3: each point adds this chunk's rows, in s order, for every (k, l)
@@ -3083,7 +3091,7 @@ __global__ __launch_bounds__(256) void k_points_tile(SurfDev<T> sd, const vec4_t
               if (s2 > q) break;
               const T nv = sNv[ey][l][s2];
 #pragma unroll
-              for (int c = 0; c < NC; ++c) h[m][c] += nv * sR[ex][rb + s2][k][c];
+              for (int c = 0; c < NC; ++c) h[m][c] += nv * sR(ex, rb + s2, k)[c];
             }
           }
       } else {
@@ -3101,7 +3109,7 @@ __global__ __launch_bounds__(256) void k_points_tile(SurfDev<T> sd, const vec4_t
               if (rr < 0 || rr >= nr) continue;
               const T nv = sNv[ey][l][s2];
 #pragma unroll
-              for (int c = 0; c < NC; ++c) h[m][c] += nv * sR[ex][rr][k][c];
+              for (int c = 0; c < NC; ++c) h[m][c] += nv * sR(ex, rr, k)[c];
             }
           }
       }

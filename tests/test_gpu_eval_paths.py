@@ -43,7 +43,7 @@ def orc():
 
 
 @pytest.mark.parametrize("n,deg", CASES)
-@pytest.mark.parametrize("nuv", [13, 40])
+@pytest.mark.parametrize("nuv", [13, 40, 77])
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 def test_eval_grid_matches_oracle(ctx, orc, n, deg, nuv, dtype):
     s = wavy(n, deg, seed=n + deg)
