@@ -78,6 +78,30 @@ __device__ __forceinline__ vec4_t<T> child_region(const vec4_t<T>& pr, int pnuv,
   return c;
 }
 
+// One axis of a cell's float overlap record (FP32SCREEN intersection): the
+// hull [lo, hi] of the cell's FP64 acceptance interval [c - r (1 + 4 u64),
+// c + r (1 + 4 u64)], widened by 4 u64 (|c| + r) for the FP64 arithmetic, and
+// with has32 also of the float interval [c32 - r32 (1 + 3u), c32 + r32 (1 +
+// 3u)] of VERSION5's own float boxes; stored as xo = fl32(mid) and yo =
+// ru32((half + |xo - mid|)(1 + 3u) + 4 u64 (|mid| + half) + 1e-30), so two
+// records whose hulls meet pass the float test.
+__device__ __forceinline__ void record32_axis(double c64, double r64, bool has32, float c32, float r32, float* xo,
+                                              float* yo) {
+  constexpr double u = 5.9604644775390625e-08, u64 = 1.1102230246251565e-16;
+  const double w = r64 * (1.0 + 4.0 * u64) + 4.0 * u64 * (fabs(c64) + r64);
+  double lo = c64 - w, hi = c64 + w;
+  if (has32) {
+    const double w32 = (double)r32 * (1.0 + 3.0 * u) + 4.0 * u64 * (fabs((double)c32) + (double)r32);
+    lo = fmin(lo, (double)c32 - w32);
+    hi = fmax(hi, (double)c32 + w32);
+  }
+  const double mid = 0.5 * (lo + hi), half = 0.5 * (hi - lo);
+  const float cf = __double2float_rn(mid);
+  *xo = cf;
+  *yo = __double2float_ru((half + fabs((double)cf - mid)) * (1.0 + 3.0 * u) + 4.0 * u64 * (fabs(mid) + half) +
+                          1e-30);
+}
+
 constexpr int kEcKnots = 128;  // knots per direction staged in LDS (when both vectors fit)
 
 }  // namespace jsi

@@ -273,6 +273,13 @@ struct IcJob {
   vec4_t<T>* R = nullptr;
   vec4_t<T>* lo = nullptr;
   vec4_t<T>* hi = nullptr;
+  // FP32SCREEN (T = double): each cell's float record (record32_axis) and
+  // each unit's float box of them, the values k_isect_boxes32 and
+  // launch_tilebox<float> would write
+  float4* X = nullptr;
+  float4* Y = nullptr;
+  float4* flo = nullptr;
+  float4* fhi = nullptr;
 };
 // Overlap screen of a round whose units have at most kIcMaxNuv^2 cells per
 // side, per slot (jsi_isect_cells.hip).  Cull (rounds with many slots): with

@@ -302,6 +302,12 @@ __global__ __launch_bounds__(64) void k_isect_cells(IcJob<T> j0, IcJob<T> j1, in
   const T inf = infinity_v<T>();
   T mn[3] = {inf, inf, inf}, mx[3] = {-inf, -inf, -inf};
   T amax = T(0);
+  // FP32SCREEN: the cells' float records and the unit's float box of them
+  // (k_isect_boxes32 and k_tilebox<float>, in their operations)
+  float4* const X32 = second ? j1.X : j0.X;
+  const float finf = __uint_as_float(0x7f800000u);
+  float fmn[3] = {finf, finf, finf}, fmx[3] = {-finf, -finf, -finf};
+  float famax = 0.f;
   if (active && ix < w && iy < w) {
     const int p0 = lane, p1 = p0 + 1, p2 = p0 + nuv, p3 = p2 + 1;
     T lo[3], hi[3];
@@ -332,6 +338,40 @@ __global__ __launch_bounds__(64) void k_isect_cells(IcJob<T> j0, IcJob<T> j1, in
       mn[a] = fmin(mn[a], v[a] - rr[a]);
       mx[a] = fmax(mx[a], v[a] + rr[a]);
       amax = fmax(amax, fabs(v[a]) + fabs(rr[a]));
+    }
+    if constexpr (std::is_same<T, double>::value) {
+      if (X32) {  // uniform: a kernel argument
+        float4 x, y;
+        record32_axis(ctr.x, rad.x, false, 0.f, 0.f, &x.x, &y.x);
+        record32_axis(ctr.y, rad.y, false, 0.f, 0.f, &x.y, &y.y);
+        record32_axis(ctr.z, rad.z, false, 0.f, 0.f, &x.z, &y.z);
+        x.w = y.w = 0.f;
+        X32[o] = x;
+        (second ? j1.Y : j0.Y)[o] = y;
+        const float xv[3] = {x.x, x.y, x.z}, yv[3] = {y.x, y.y, y.z};
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          fmn[a] = fminf(fmn[a], xv[a] - yv[a]);
+          fmx[a] = fmaxf(fmx[a], xv[a] + yv[a]);
+          famax = fmaxf(famax, fabsf(xv[a]) + fabsf(yv[a]));
+        }
+      }
+    }
+  }
+  if (X32) {  // uniform: a kernel argument
+    for (int off = L >> 1; off > 0; off >>= 1) {
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        fmn[a] = fminf(fmn[a], __shfl_xor(fmn[a], off));
+        fmx[a] = fmaxf(fmx[a], __shfl_xor(fmx[a], off));
+      }
+      famax = fmaxf(famax, __shfl_xor(famax, off));
+    }
+    if (live && e == 0) {
+      // the outward margin of k_tilebox<float>
+      const float delta = famax * 9.5367431640625e-07f + 1e-30f;
+      (second ? j1.flo : j0.flo)[unit] = make_float4(fmn[0] - delta, fmn[1] - delta, fmn[2] - delta, 0.f);
+      (second ? j1.fhi : j0.fhi)[unit] = make_float4(fmx[0] + delta, fmx[1] + delta, fmx[2] + delta, 0.f);
     }
   }
   vec4_t<T>* const lo = second ? j1.lo : j0.lo;

@@ -2691,31 +2691,17 @@ __global__ void k_isect_boxes32(const double4* C, const double4* R, const float4
                                 unsigned long long n, float4* X, float4* Y) {
   const unsigned long long e = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
-  constexpr double u = 5.9604644775390625e-08, u64 = 1.1102230246251565e-16;
   const double4 c = C[e], r = R[e];
   float4 cv = make_float4(0.f, 0.f, 0.f, 0.f), rv = cv;
   if (Cv) {
     cv = Cv[e];
     rv = Rv[e];
   }
-  auto axis = [&](double c64, double r64, float c32, float r32, float* xo, float* yo) {
-    const double w = r64 * (1.0 + 4.0 * u64) + 4.0 * u64 * (fabs(c64) + r64);
-    double lo = c64 - w, hi = c64 + w;
-    if (Cv) {
-      const double w32 = (double)r32 * (1.0 + 3.0 * u) + 4.0 * u64 * (fabs((double)c32) + (double)r32);
-      lo = fmin(lo, (double)c32 - w32);
-      hi = fmax(hi, (double)c32 + w32);
-    }
-    const double mid = 0.5 * (lo + hi), half = 0.5 * (hi - lo);
-    const float cf = __double2float_rn(mid);
-    *xo = cf;
-    *yo = __double2float_ru((half + fabs((double)cf - mid)) * (1.0 + 3.0 * u) + 4.0 * u64 * (fabs(mid) + half) +
-                            1e-30);
-  };
+  const bool has32 = Cv != nullptr;
   float4 x, y;
-  axis(c.x, r.x, cv.x, rv.x, &x.x, &y.x);
-  axis(c.y, r.y, cv.y, rv.y, &x.y, &y.y);
-  axis(c.z, r.z, cv.z, rv.z, &x.z, &y.z);
+  record32_axis(c.x, r.x, has32, cv.x, rv.x, &x.x, &y.x);
+  record32_axis(c.y, r.y, has32, cv.y, rv.y, &x.y, &y.y);
+  record32_axis(c.z, r.z, has32, cv.z, rv.z, &x.z, &y.z);
   x.w = y.w = 0.f;
   X[e] = x;
   Y[e] = y;

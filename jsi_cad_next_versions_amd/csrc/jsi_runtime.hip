@@ -1038,6 +1038,7 @@ int jsi_ctx::intersect(int h1, const jsi_region& r1, int h2, const jsi_region& r
   V *tlo1 = nullptr, *thi1 = nullptr, *tlo2 = nullptr, *thi2 = nullptr;  // tile boxes of the current units
   bool units_ready = false;  // the screen also wrote the next round's units
   unsigned long long ready_nu1 = 0, ready_nu2 = 0;
+  bool f32_ready = false;  // FP32SCREEN: the evaluation wrote the units' float records and boxes
   for (int round = 1;; ++round) {
     const double ts = now_ms();
     const int w1 = nuv1 - 1, w2 = nuv2 - 1;
@@ -1064,16 +1065,18 @@ int jsi_ctx::intersect(int h1, const jsi_region& r1, int h2, const jsi_region& r
           RCK(buf("i32.Y1", n1, &Y1));
           RCK(buf("i32.X2", n2, &X2));
           RCK(buf("i32.Y2", n2, &Y2));
-          HCK(launch_isect_boxes32(C1, R1, round == 1 ? V1c : nullptr, round == 1 ? V1r : nullptr, n1, X1, Y1,
-                                   stream));
-          HCK(launch_isect_boxes32(C2, R2, round == 1 ? V2c : nullptr, round == 1 ? V2r : nullptr, n2, X2, Y2,
-                                   stream));
           RCK(buf("tb32.lo1", (size_t)nunits1, &fl1));
           RCK(buf("tb32.hi1", (size_t)nunits1, &fh1));
           RCK(buf("tb32.lo2", (size_t)nunits2, &fl2));
           RCK(buf("tb32.hi2", (size_t)nunits2, &fh2));
-          HCK(launch_tilebox<float>(X1, Y1, nunits1, w1, 1, fl1, fh1, stream));
-          HCK(launch_tilebox<float>(X2, Y2, nunits2, w2, 1, fl2, fh2, stream));
+          if (!f32_ready) {  // not written by the small-unit evaluation
+            HCK(launch_isect_boxes32(C1, R1, round == 1 ? V1c : nullptr, round == 1 ? V1r : nullptr, n1, X1, Y1,
+                                     stream));
+            HCK(launch_isect_boxes32(C2, R2, round == 1 ? V2c : nullptr, round == 1 ? V2r : nullptr, n2, X2, Y2,
+                                     stream));
+            HCK(launch_tilebox<float>(X1, Y1, nunits1, w1, 1, fl1, fh1, stream));
+            HCK(launch_tilebox<float>(X2, Y2, nunits2, w2, 1, fl2, fh2, stream));
+          }
           f32.X1 = X1;
           f32.Y1 = Y1;
           f32.X2 = X2;
@@ -1276,6 +1279,7 @@ int jsi_ctx::intersect(int h1, const jsi_region& r1, int h2, const jsi_region& r
     RCK(buf(nm("C", 2, nxt).c_str(), (size_t)nu2 * wr * wr, &nC2));
     RCK(buf(nm("R", 2, nxt).c_str(), (size_t)nu2 * wr * wr, &nR2));
     tlo1 = thi1 = tlo2 = thi2 = nullptr;
+    f32_ready = false;
     if (fuse_cells && g_nuv >= 2 && g_nuv <= kIcMaxNuv) {
       IcJob<T> j1, j2;
       RCK(surf_dev<T>(h1, &j1.sd));
@@ -1302,6 +1306,20 @@ int jsi_ctx::intersect(int h1, const jsi_region& r1, int h2, const jsi_region& r
         j1.hi = thi1;
         j2.lo = tlo2;
         j2.hi = thi2;
+      }
+      if constexpr (std::is_same<T, double>::value) {
+        if (s32 && !P.screen_out) {  // the next screen's float records and unit boxes, from the same pass
+          const size_t n1 = (size_t)nu1 * wr * wr, n2 = (size_t)nu2 * wr * wr;
+          RCK(buf("i32.X1", n1, &j1.X));
+          RCK(buf("i32.Y1", n1, &j1.Y));
+          RCK(buf("i32.X2", n2, &j2.X));
+          RCK(buf("i32.Y2", n2, &j2.Y));
+          RCK(buf("tb32.lo1", (size_t)nu1, &j1.flo));
+          RCK(buf("tb32.hi1", (size_t)nu1, &j1.fhi));
+          RCK(buf("tb32.lo2", (size_t)nu2, &j2.flo));
+          RCK(buf("tb32.hi2", (size_t)nu2, &j2.fhi));
+          f32_ready = true;
+        }
       }
       if (prof) HCK(hipEventRecord(ev[0], stream));
       HCK(launch_isect_cells<T>(j1, &j2, use_k, stream));
