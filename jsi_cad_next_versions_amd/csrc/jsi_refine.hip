@@ -344,6 +344,23 @@ __device__ inline void cell_centre(const vec4_t<TR>& R, int nuv, int cell, doubl
 
 constexpr int kPolishKnots = 256;  // knots per vector staged in LDS
 
+// The four knot vectors in LDS when each fits (the span searches of every
+// evaluation are chains of dependent loads), s1 / s2 then pointing there.
+// Every thread of the workgroup calls it.
+__device__ __forceinline__ void stage_knots(double (*sk)[kPolishKnots], SurfDev<double>& s1, SurfDev<double>& s2) {
+  if (s1.nku <= kPolishKnots && s1.nkv <= kPolishKnots && s2.nku <= kPolishKnots && s2.nkv <= kPolishKnots) {
+    for (int i = threadIdx.x; i < s1.nku; i += blockDim.x) sk[0][i] = s1.U[i];
+    for (int i = threadIdx.x; i < s1.nkv; i += blockDim.x) sk[1][i] = s1.V[i];
+    for (int i = threadIdx.x; i < s2.nku; i += blockDim.x) sk[2][i] = s2.U[i];
+    for (int i = threadIdx.x; i < s2.nkv; i += blockDim.x) sk[3][i] = s2.V[i];
+    __syncthreads();
+    s1.U = sk[0];
+    s1.V = sk[1];
+    s2.U = sk[2];
+    s2.V = sk[3];
+  }
+}
+
 template <class TR, int MAXP, bool FAST>
 __device__ __forceinline__ void k_polish_body(SurfDev<double> s1g, SurfDev<double> s2g, const Cand* rec, int m,
                                                const int* su1, const int* su2, const vec4_t<TR>* reg1,
@@ -353,17 +370,7 @@ __device__ __forceinline__ void k_polish_body(SurfDev<double> s1g, SurfDev<doubl
   // evaluation are chains of dependent loads
   __shared__ double sk[4][kPolishKnots];
   SurfDev<double> s1 = s1g, s2 = s2g;
-  if (s1.nku <= kPolishKnots && s1.nkv <= kPolishKnots && s2.nku <= kPolishKnots && s2.nkv <= kPolishKnots) {
-    for (int i = threadIdx.x; i < s1.nku; i += blockDim.x) sk[0][i] = s1g.U[i];
-    for (int i = threadIdx.x; i < s1.nkv; i += blockDim.x) sk[1][i] = s1g.V[i];
-    for (int i = threadIdx.x; i < s2.nku; i += blockDim.x) sk[2][i] = s2g.U[i];
-    for (int i = threadIdx.x; i < s2.nkv; i += blockDim.x) sk[3][i] = s2g.V[i];
-    __syncthreads();
-    s1.U = sk[0];
-    s1.V = sk[1];
-    s2.U = sk[2];
-    s2.V = sk[3];
-  }
+  stage_knots(sk, s1, s2);
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= m) return;
   const Cand c = rec[e];
@@ -515,6 +522,8 @@ __global__ __launch_bounds__(64) void k_curve_proj(SurfDev<double> s1, SurfDev<d
                                                    long long n, const int* su1, const int* su2,
                                                    const vec4_t<TR>* reg1, const vec4_t<TR>* reg2, int nuv1,
                                                    int nuv2, double4 box1, double4 box2, CurveRec* out) {
+  __shared__ double sk[4][kPolishKnots];
+  stage_knots(sk, s1, s2);
   const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
   const IPair pr = pairs[e];
@@ -802,7 +811,9 @@ __global__ __launch_bounds__(64) void k_march(SurfDev<double> s1, SurfDev<double
                                               MarchOut* out) {
   const int k = blockIdx.x;
   const int lane = threadIdx.x;
-  if (k >= nstart) return;
+  if (k >= nstart) return;  // whole workgroup
+  __shared__ double sk[4][kPolishKnots];
+  stage_knots(sk, s1, s2);
   const int s = starts[k];
   const double lo[4] = {box1.x, box1.y, box2.x, box2.y};
   const double hi[4] = {box1.z, box1.w, box2.z, box2.w};
@@ -883,20 +894,25 @@ __global__ __launch_bounds__(64) void k_march(SurfDev<double> s1, SurfDev<double
       for (int q = 0; q < 3; ++q) T[q] = tn[q];
     }
     for (int q = 0; q < 3; ++q) p[q] = E1[q];
+    // four runs of 64 seeds per iteration, so their loads are in flight
+    // together; the lowest index with a hit wins
     int hit = -1;
-    for (int q0 = 0; q0 < nseeds; q0 += 64) {
-      const int q = q0 + lane;
-      bool near = false;
-      // the start seed counts only once the march has left it
-      if (q < nseeds && !(q == s && step == 0)) {
-        const double dq[3] = {p[0] - seeds[q].p[0], p[1] - seeds[q].p[1], p[2] - seeds[q].p[2]};
-        near = dot3(dq, dq) < rr2;
+    for (int q0 = 0; q0 < nseeds && hit < 0; q0 += 256) {
+      unsigned long long m[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int q = q0 + 64 * c + lane;
+        bool near = false;
+        // the start seed counts only once the march has left it
+        if (q < nseeds && !(q == s && step == 0)) {
+          const double dq[3] = {p[0] - seeds[q].p[0], p[1] - seeds[q].p[1], p[2] - seeds[q].p[2]};
+          near = dot3(dq, dq) < rr2;
+        }
+        m[c] = __ballot(near);
       }
-      const unsigned long long m = __ballot(near);
-      if (m != 0ull) {
-        hit = q0 + __ffsll((long long)m) - 1;
-        break;
-      }
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+        if (hit < 0 && m[c] != 0ull) hit = q0 + 64 * c + __ffsll((long long)m[c]) - 1;
     }
     if (hit >= 0) {
       succ = hit;
