@@ -118,6 +118,10 @@ __device__ __forceinline__ void ec_tile(const SurfDev<T>& sd, const vec4_t<T>* r
   if (unit >= nunits || (nunits_dev && unit >= *nunits_dev)) return;  // whole wavefront
   const int row0 = (t / tw) * kTile, col0 = (t % tw) * kTile;
   const int ht = min(kTile, w - row0), wd = min(kTile, w - col0);  // cells
+  // the tile's (ht + 1) x (wd + 1) corners, numbered row by row: x = ey cw +
+  // ex (a 15 x 15-cell unit has 256, four per lane)
+  const int cw = wd + 1, ncorn = (ht + 1) * cw;
+  const float rcw = 1.f / (float)cw;  // ey = floor((x + 0.5) / cw), exact for x < 2^12
   const int p = sd.degu, q = sd.degv;
   const vec4_t<T> r = regions[unit];
   // phase 0: knot vectors in LDS
@@ -199,9 +203,10 @@ __device__ __forceinline__ void ec_tile(const SurfDev<T>& sd, const vec4_t<T>* r
     const bool one = nrows <= kRows;
 #pragma unroll
     for (int c = 0; c < kCorners; ++c) {
+      if (64 * c >= ncorn) break;  // uniform
       const int x = lane + 64 * c;
-      const int ey = x / E, ex = x % E;
-      if (x >= E * E || ey > ht || ex > wd) continue;
+      const int ey = (int)(((float)x + 0.5f) * rcw), ex = x - ey * cw;
+      if (x >= ncorn) continue;
       const int sv = sSv[ey];
       constexpr int UR = MAXD <= 7 ? MAXD + 1 : 1;
       if (one) {
@@ -234,11 +239,11 @@ __device__ __forceinline__ void ec_tile(const SurfDev<T>& sd, const vec4_t<T>* r
   // phase 4: corner points into sP (reuses the knot storage)
 #pragma unroll
   for (int c = 0; c < kCorners; ++c) {
+    if (64 * c >= ncorn) break;  // uniform
     const int x = lane + 64 * c;
-    if (x >= E * E) continue;
-    const int ey = x / E, ex = x % E;
+    if (x >= ncorn) continue;
     T S[3] = {T(0), T(0), T(0)};
-    if (ey <= ht && ex <= wd && !sd.zero) {
+    if (!sd.zero) {
       if (RAT) {
         const T F = T(1) / acc[c][3];
         S[0] = acc[c][0] * F;
@@ -251,8 +256,8 @@ __device__ __forceinline__ void ec_tile(const SurfDev<T>& sd, const vec4_t<T>* r
       }
     }
     sP[x] = S[0];
-    sP[E * E + x] = S[1];
-    sP[2 * E * E + x] = S[2];
+    sP[kPts / 3 + x] = S[1];
+    sP[2 * (kPts / 3) + x] = S[2];
   }
   __syncthreads();
   // phase 5: cell centres (and the tile's box of them)
@@ -264,13 +269,13 @@ __device__ __forceinline__ void ec_tile(const SurfDev<T>& sd, const vec4_t<T>* r
   for (int cidx = lane; cidx < kTileCells; cidx += 64) {
     const int ly = cidx / kTile, lx = cidx % kTile;
     if (ly >= ht || lx >= wd) continue;
-    const int p0 = ly * E + lx, p1 = p0 + 1, p2 = p0 + E, p3 = p2 + 1;
+    const int p0 = ly * cw + lx, p1 = p0 + 1, p2 = p0 + cw, p3 = p2 + 1;
     vec4_t<T> ctr;
     T* c = &ctr.x;
     const T k = T(0);
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-      const T* pa = sP + a * E * E;
+      const T* pa = sP + a * (kPts / 3);
       const T a0 = pa[p0], a1 = pa[p1], a2 = pa[p2], a3 = pa[p3];
       const T lo = smin(a0, smin(a1, smin(a2, a3)));
       const T hi = smax(a0, smax(a1, smax(a2, a3)));
