@@ -291,7 +291,9 @@ struct IcJob {
 // Emit, given the exclusive scans of the three counts: the pairs in (slot, i,
 // j) order and, with nsu1 non-null, the next round's units (per pair its
 // units, per unit its region: cells of the parents reg1 / reg2 on pnuv1 /
-// pnuv2 grids), numbered by first appearance.
+// pnuv2 grids), numbered by first appearance.  masks (optional, n x w1^2
+// words): the count pass writes each row's hit mask there and the emit pass
+// reads it instead of repeating the tests.
 // FP32SCREEN inputs of the slot screen (T = double): the float records of
 // both sides (k_isect_boxes32) and their unit boxes; the float test picks the
 // candidates, the FP64 test decides them.
@@ -310,7 +312,7 @@ hipError_t launch_isect_slot_count(long long nslots, int w1, int w2, const int* 
                                    const vec4_t<T>* R2, const vec4_t<T>* lo1, const vec4_t<T>* hi1,
                                    const vec4_t<T>* lo2, const vec4_t<T>* hi2, int cull, long long n,
                                    const int* list, unsigned* npair, unsigned* nrow, unsigned* ncol, const SlotF32* f,
-                                   hipStream_t s);
+                                   hipStream_t s, unsigned long long* masks = nullptr);
 template <class T>
 hipError_t launch_isect_slot_emit(long long nslots, int w1, int w2, const int* su1, const int* su2,
                                   const vec4_t<T>* C1, const vec4_t<T>* R1, const vec4_t<T>* C2, const vec4_t<T>* R2,
@@ -319,7 +321,8 @@ hipError_t launch_isect_slot_emit(long long nslots, int w1, int w2, const int* s
                                   const unsigned long long* offp, const unsigned long long* offr,
                                   const unsigned long long* offc, IPair* out, const vec4_t<T>* reg1,
                                   const vec4_t<T>* reg2, int pnuv1, int pnuv2, int* nsu1, int* nsu2,
-                                  vec4_t<T>* nreg1, vec4_t<T>* nreg2, const SlotF32* f, hipStream_t s);
+                                  vec4_t<T>* nreg1, vec4_t<T>* nreg2, const SlotF32* f, hipStream_t s,
+                                  const unsigned long long* masks = nullptr);
 // out[0..2] = a[n], b[n], c[n] (the totals of three scans, one copy back)
 hipError_t launch_gather_totals(const unsigned long long* a, const unsigned long long* b,
                                 const unsigned long long* c, long long n, unsigned long long* out, hipStream_t s);

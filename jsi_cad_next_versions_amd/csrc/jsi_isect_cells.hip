@@ -531,7 +531,8 @@ __global__ __launch_bounds__(256) void k_isect_slot_list(SlotScreen<T> s, const 
 // n entries: slots 0 .. n-1, or list[0 .. n-1]
 template <class T>
 __global__ __launch_bounds__(256) void k_isect_slot_count(SlotScreen<T> s, long long n, const int* list,
-                                                          unsigned* npair, unsigned* nrow, unsigned* ncol) {
+                                                          unsigned* npair, unsigned* nrow, unsigned* ncol,
+                                                          unsigned long long* masks) {
   const int G = slot_lanes(s.cells1), spw = slot_groups(s.cells1, s.cells2);
   const long long nwaves = (n + spw - 1) / spw;
   const int lane = threadIdx.x & 63, grp = lane / G, i = lane % G;
@@ -541,6 +542,7 @@ __global__ __launch_bounds__(256) void k_isect_slot_count(SlotScreen<T> s, long 
     const long long slot = in ? (list ? (long long)list[e] : e) : 0;
     const bool valid = in && i < s.cells1 && (list || slot_boxes_meet(s, slot));
     const unsigned long long m = valid ? slot_row_mask(s, slot, i) : 0ull;
+    if (masks && in && i < s.cells1) masks[e * s.cells1 + i] = m;  // for the emit pass
     unsigned pc = (unsigned)__popcll(m), nz = m != 0;
     unsigned long long om = m;
     for (int off = G >> 1; off > 0; off >>= 1) {
@@ -561,7 +563,7 @@ __global__ __launch_bounds__(256) void k_isect_slot_emit(SlotScreen<T> s, long l
                                                          const unsigned long long* offp,
                                                          const unsigned long long* offr,
                                                          const unsigned long long* offc, IPair* out, int units,
-                                                         SlotUnits<T> u) {
+                                                         SlotUnits<T> u, const unsigned long long* masks) {
   extern __shared__ int sTab[];  // per wavefront: spw groups x cells2 side-2 unit ids
   const int G = slot_lanes(s.cells1), spw = slot_groups(s.cells1, s.cells2);
   const long long nwaves = (n + spw - 1) / spw;
@@ -571,8 +573,11 @@ __global__ __launch_bounds__(256) void k_isect_slot_emit(SlotScreen<T> s, long l
     const long long e = (base + (threadIdx.x >> 6)) * spw + grp;
     const bool in = grp < spw && e < n;
     const long long slot = in ? (list ? (long long)list[e] : e) : 0;
-    const bool valid = in && i < s.cells1 && (list || slot_boxes_meet(s, slot));
-    const unsigned long long m = valid ? slot_row_mask(s, slot, i) : 0ull;
+    // the rows' hit masks: written by the count pass, or recomputed
+    const bool row = in && i < s.cells1;
+    const bool valid = row && (masks || list || slot_boxes_meet(s, slot));
+    const unsigned long long m = masks ? (row ? masks[e * s.cells1 + i] : 0ull)
+                                       : (valid ? slot_row_mask(s, slot, i) : 0ull);
     // inclusive scans over the group's rows: pairs, non-empty rows, j seen
     unsigned pc = (unsigned)__popcll(m), nz = m != 0;
     unsigned long long om = m;
@@ -775,13 +780,13 @@ hipError_t launch_isect_slot_count(long long nslots, int w1, int w2, const int* 
                                    const vec4_t<T>* R2, const vec4_t<T>* lo1, const vec4_t<T>* hi1,
                                    const vec4_t<T>* lo2, const vec4_t<T>* hi2, int cull, long long n,
                                    const int* list, unsigned* npair, unsigned* nrow, unsigned* ncol, const SlotF32* f,
-                                   hipStream_t s) {
+                                   hipStream_t s, unsigned long long* masks) {
   if (n <= 0) return hipSuccess;
   if (!slot_args_ok(nslots, w1, w2, list ? 0 : cull, lo1, hi1, lo2, hi2, f) || n > nslots)
     return hipErrorInvalidValue;
   const SlotScreen<T> sc = slot_screen<T>(nslots, w1, w2, su1, su2, C1, R1, C2, R2, lo1, hi1, lo2, hi2, cull, f);
   hipLaunchKernelGGL(k_isect_slot_count<T>, dim3(slot_blocks(n, sc.cells1, sc.cells2)), dim3(256), 0, s, sc, n, list,
-                     npair, nrow, ncol);
+                     npair, nrow, ncol, masks);
   return hipGetLastError();
 }
 
@@ -793,7 +798,8 @@ hipError_t launch_isect_slot_emit(long long nslots, int w1, int w2, const int* s
                                   const unsigned long long* offp, const unsigned long long* offr,
                                   const unsigned long long* offc, IPair* out, const vec4_t<T>* reg1,
                                   const vec4_t<T>* reg2, int pnuv1, int pnuv2, int* nsu1, int* nsu2,
-                                  vec4_t<T>* nreg1, vec4_t<T>* nreg2, const SlotF32* f, hipStream_t s) {
+                                  vec4_t<T>* nreg1, vec4_t<T>* nreg2, const SlotF32* f, hipStream_t s,
+                                  const unsigned long long* masks) {
   if (n <= 0) return hipSuccess;
   if (!slot_args_ok(nslots, w1, w2, list ? 0 : cull, lo1, hi1, lo2, hi2, f) || n > nslots)
     return hipErrorInvalidValue;
@@ -811,7 +817,7 @@ hipError_t launch_isect_slot_emit(long long nslots, int w1, int w2, const int* s
   if (units && (!nsu2 || !nreg1 || !nreg2 || !reg1 || !reg2)) return hipErrorInvalidValue;
   const size_t lds = units ? sizeof(int) * 4 * slot_groups(sc.cells1, sc.cells2) * sc.cells2 : 0;
   hipLaunchKernelGGL(k_isect_slot_emit<T>, dim3(slot_blocks(n, sc.cells1, sc.cells2)), dim3(256), lds, s, sc, n, list,
-                     offp, offr, offc, out, units, su);
+                     offp, offr, offc, out, units, su, masks);
   return hipGetLastError();
 }
 
@@ -829,12 +835,13 @@ hipError_t launch_gather_totals(const unsigned long long* a, const unsigned long
                                                  const vec4_t<T>*, const vec4_t<T>*, const vec4_t<T>*,             \
                                                  const vec4_t<T>*, const vec4_t<T>*, const vec4_t<T>*,             \
                                                  const vec4_t<T>*, int, long long, const int*, unsigned*, unsigned*, \
-                                                 unsigned*, const SlotF32*, hipStream_t);                           \
+                                                 unsigned*, const SlotF32*, hipStream_t, unsigned long long*);      \
   template hipError_t launch_isect_slot_emit<T>(                                                                   \
       long long, int, int, const int*, const int*, const vec4_t<T>*, const vec4_t<T>*, const vec4_t<T>*,           \
       const vec4_t<T>*, const vec4_t<T>*, const vec4_t<T>*, const vec4_t<T>*, const vec4_t<T>*, int, long long,    \
       const int*, const unsigned long long*, const unsigned long long*, const unsigned long long*, IPair*,         \
-      const vec4_t<T>*, const vec4_t<T>*, int, int, int*, int*, vec4_t<T>*, vec4_t<T>*, const SlotF32*, hipStream_t);
+      const vec4_t<T>*, const vec4_t<T>*, int, int, int*, int*, vec4_t<T>*, vec4_t<T>*, const SlotF32*, hipStream_t, \
+      const unsigned long long*);
 JSI_IS(float)
 JSI_IS(double)
 #undef JSI_IS

@@ -61,6 +61,7 @@ unsigned long long next_pow2(unsigned long long x) {
 
 constexpr unsigned kSmallCap = 16384;               // records sorted after selection
 constexpr long long kSlotListMin = 1 << 18;  // intersection rounds with this many slots cull them first
+constexpr size_t kSlotMaskBytes = size_t(256) << 20;  // row hit masks kept between the slot passes, at most
 
 
 }  // namespace
@@ -1033,6 +1034,8 @@ int jsi_ctx::intersect(int h1, const jsi_region& r1, int h2, const jsi_region& r
   const bool fuse_cells = !(icenv && std::strcmp(icenv, "0") == 0);
   // rounds of at least this many slots list the slots that pass the unit-box
   // test before the screen (JSI_SLOT_LIST_MIN overrides the default)
+  const char* smenv = std::getenv("JSI_SLOT_MASKS");  // 0: the emit pass repeats the tests
+  const bool slot_masks = !(smenv && std::atoi(smenv) == 0);
   const char* slenv = std::getenv("JSI_SLOT_LIST_MIN");
   const long long slot_list_min = slenv ? std::atoll(slenv) : kSlotListMin;
   V *tlo1 = nullptr, *thi1 = nullptr, *tlo2 = nullptr, *thi2 = nullptr;  // tile boxes of the current units
@@ -1138,8 +1141,14 @@ int jsi_ctx::intersect(int h1, const jsi_region& r1, int h2, const jsi_region& r
         RCK(buf_raw("i.scantmp", stb, &scantmp, 0));
         unsigned *cp = cnt, *cr = cnt + ns, *cc = cnt + 2 * ns;
         unsigned long long *op = off, *orr = off + ns + 1, *oc = off + 2 * (ns + 1);
+        // the rows' hit masks, kept from the count pass for the emit pass
+        // when they fit kSlotMaskBytes (a dense round's are recomputed)
+        unsigned long long* masks = nullptr;
+        const size_t mrows = ns * (size_t)w1 * (size_t)w1;
+        if (slot_masks && mrows * sizeof(unsigned long long) <= kSlotMaskBytes)
+          RCK(buf("i.masks", std::max(mrows, (size_t)1), &masks));
         HCK(launch_isect_slot_count<T>(nslots, w1, w2, su1, su2, C1, R1, C2, R2, lo1, hi1, lo2, hi2, P.cull,
-                                       (long long)ns, slist, cp, cr, cc, fp, stream));
+                                       (long long)ns, slist, cp, cr, cc, fp, stream, masks));
         HCK(scan_u32_to_u64(cp, op, ns, scantmp, stb, stream));
         HCK(scan_u32_to_u64(cr, orr, ns, scantmp, stb, stream));
         HCK(scan_u32_to_u64(cc, oc, ns, scantmp, stb, stream));
@@ -1166,7 +1175,7 @@ int jsi_ctx::intersect(int h1, const jsi_region& r1, int h2, const jsi_region& r
         if (npairs > 0)
           HCK(launch_isect_slot_emit<T>(nslots, w1, w2, su1, su2, C1, R1, C2, R2, lo1, hi1, lo2, hi2, P.cull,
                                         (long long)ns, slist, op, orr, oc, pairs, reg1, reg2, nuv1, nuv2, nsu1, nsu2,
-                                        nreg1, nreg2, fp, stream));
+                                        nreg1, nreg2, fp, stream, masks));
       } else {
         uint64_t* list;
         unsigned long long* small;

@@ -41,7 +41,7 @@ size_t estimate_bytes(const jsi_query& q, bool batched) {
   const size_t n1 = (size_t)q.r1.nuv * q.r1.nuv, n2 = (size_t)q.r2.nuv * q.r2.nuv;
   const size_t scal = q.params.precision == JSI_PREC_EXACT32 ? 4 : 8;
   size_t b = (n1 + n2) * scal * 3 * 2 + (n1 + n2) * scal * 8 + ((size_t)64 << 20);
-  if (q.kind == JSI_QUERY_INTERSECT) b += (size_t)256 << 20;
+  if (q.kind == JSI_QUERY_INTERSECT) b += (size_t)512 << 20;  // pair lists, row hit masks
   return b;
 }
 

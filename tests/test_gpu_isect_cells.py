@@ -47,19 +47,23 @@ def wavy(n, deg, seed, lift=0.0, rational=True, periods=3.0, tilt=0.0):
 
 def run_three(ctx, orc, a, b, nuv, monkeypatch, **kw):
     """The fused path (slots screened directly, and listed after the unit-box
-    test as rounds with many slots are), the grid-point path and the oracle
-    on one query."""
+    test as rounds with many slots are; the emit pass reading the count
+    pass's row masks, and repeating the tests), the grid-point path and the
+    oracle on one query."""
     r1, r2 = S.full_region(a, nuv, 1), S.full_region(b, nuv, 2)
     p = default_params(trace=1, **kw)
     res, pairs = ctx.intersection(a, r1, b, r2, p, cap=1 << 20)
     monkeypatch.setenv("JSI_SLOT_LIST_MIN", "1")
     lres, lpairs = ctx.intersection(a, r1, b, r2, p, cap=1 << 20)
     monkeypatch.delenv("JSI_SLOT_LIST_MIN")
+    monkeypatch.setenv("JSI_SLOT_MASKS", "0")
+    mres, mpairs = ctx.intersection(a, r1, b, r2, p, cap=1 << 20)
+    monkeypatch.delenv("JSI_SLOT_MASKS")
     monkeypatch.setenv("JSI_ISECT_CELLS", "0")
     pres, ppairs = ctx.intersection(a, r1, b, r2, p, cap=1 << 20)
     monkeypatch.delenv("JSI_ISECT_CELLS")
     ores, opairs = orc.intersect(a, r1, b, r2, p, cap=1 << 20)
-    for other, opl in ((lres, lpairs), (pres, ppairs), (ores, opairs)):
+    for other, opl in ((lres, lpairs), (mres, mpairs), (pres, ppairs), (ores, opairs)):
         assert res.rounds_list() == other.rounds_list()
         assert res.isect_refine_nuv == other.isect_refine_nuv
         np.testing.assert_array_equal(pairs, opl)
