@@ -248,7 +248,12 @@ hipError_t segmented_sort_pairs(IPair* pairs, unsigned long long n, const unsign
                                 unsigned long long nrows, void* tmp, size_t tmp_bytes, hipStream_t s);
 hipError_t scan_u32_to_u64(const unsigned* in, unsigned long long* out, unsigned long long n, void* tmp,
                            size_t tmp_bytes, hipStream_t s);
-size_t scan_tmp_bytes(unsigned long long n);
+// three scans of n entries in the same launches: in + y in_stride -> out + y
+// out_stride (in_stride >= n, out_stride >= n + 1), y = 0..2
+hipError_t scan3_u32_to_u64(const unsigned* in, size_t in_stride, unsigned long long* out, size_t out_stride,
+                            unsigned long long n, void* tmp, size_t tmp_bytes, hipStream_t s);
+// scratch of a scan of n entries over `arrays` arrays at once
+size_t scan_tmp_bytes(unsigned long long n, int arrays = 1);
 hipError_t launch_isect_dedup_flags(const IPair* pairs, unsigned long long n, int cells2, int* firstpos2,
                                     unsigned* flag1, unsigned* flag2, hipStream_t s);
 template <class T>

@@ -476,7 +476,9 @@ __device__ __forceinline__ bool slot_boxes_meet(const SlotScreen<T>& s, long lon
 // in the records' own arithmetic (k_isect_pass's expression).
 template <class V>
 __device__ __forceinline__ bool cells_overlap(const V& c1, const V& r1, const V& c2, const V& r2) {
-  return fabs(c1.x - c2.x) < (r1.x + r2.x) && fabs(c1.y - c2.y) < (r1.y + r2.y) && fabs(c1.z - c2.z) < (r1.z + r2.z);
+  // the three axis tests of overlap_aabb_host, evaluated without branches
+  return (fabs(c1.x - c2.x) < (r1.x + r2.x)) & (fabs(c1.y - c2.y) < (r1.y + r2.y)) &
+         (fabs(c1.z - c2.z) < (r1.z + r2.z));
 }
 
 // The hits of row (slot, i): bit j = cell j of the side-2 unit.  Under
@@ -497,6 +499,12 @@ __device__ __forceinline__ unsigned long long slot_row_mask(const SlotScreen<T>&
   }
   const vec4_t<T> c1 = s.C1[a], r1 = s.R1[a];
   unsigned long long m = 0;
+  if (!s.f.X1) {  // every cell a candidate: a counted loop, whose loads run ahead
+#pragma unroll 4
+    for (int j = 0; j < s.cells2; ++j)
+      m |= (unsigned long long)cells_overlap(c1, r1, s.C2[b0 + j], s.R2[b0 + j]) << j;
+    return m;
+  }
   for (unsigned long long r = cand; r; r &= r - 1) {
     const int j = __ffsll((long long)r) - 1;
     m |= (unsigned long long)cells_overlap(c1, r1, s.C2[b0 + j], s.R2[b0 + j]) << j;

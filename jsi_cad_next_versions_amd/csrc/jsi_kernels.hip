@@ -2786,13 +2786,18 @@ __global__ void k_row_sort(const unsigned long long* rowoff, unsigned long long 
   }
 }
 
-// Exclusive scan (u32 or u64 input -> u64 output with the total at out[n]).
+// Exclusive scan (u32 or u64 input -> u64 output with the total at out[n]);
+// blockIdx.y selects one of several arrays of the same length, `*_stride`
+// elements apart, scanned in the same launches.
 constexpr int kScanBlock = 4096;  // 1024 threads x 4
 
 template <class IN>
 __global__ __launch_bounds__(1024) void k_scan_reduce(const IN* in, unsigned long long n,
-                                                      unsigned long long* sums) {
+                                                      unsigned long long* sums, size_t in_stride,
+                                                      size_t sums_stride) {
   __shared__ unsigned long long s[1024];
+  in += blockIdx.y * in_stride;
+  sums += blockIdx.y * sums_stride;
   const unsigned long long base = (unsigned long long)blockIdx.x * kScanBlock;
   unsigned long long v = 0;
   for (int e = 0; e < 4; ++e) {
@@ -2810,8 +2815,12 @@ __global__ __launch_bounds__(1024) void k_scan_reduce(const IN* in, unsigned lon
 
 template <class IN>
 __global__ __launch_bounds__(1024) void k_scan_apply(const IN* in, unsigned long long n,
-                                                     const unsigned long long* block_off, unsigned long long* out) {
+                                                     const unsigned long long* block_off, unsigned long long* out,
+                                                     size_t in_stride, size_t off_stride, size_t out_stride) {
   __shared__ unsigned long long s[1024];
+  in += blockIdx.y * in_stride;
+  out += blockIdx.y * out_stride;
+  if (block_off) block_off += blockIdx.y * off_stride;
   const unsigned long long base = (unsigned long long)blockIdx.x * kScanBlock;
   unsigned long long v[4];
   unsigned long long t = 0;
@@ -3699,32 +3708,37 @@ hipError_t launch_isect_scatter(const IPair* cand, const unsigned* flag, const u
   return hipGetLastError();
 }
 
-size_t scan_tmp_bytes(unsigned long long n) {
+size_t scan_tmp_bytes(unsigned long long n, int arrays) {
   size_t total = 0;
   unsigned long long m = n;
   while (true) {
     const unsigned long long nb = (m + kScanBlock - 1) / kScanBlock;
-    total += (nb + 1) * 2 * sizeof(unsigned long long);
+    total += (nb + 1) * 2 * sizeof(unsigned long long) * (size_t)arrays;
     if (nb <= 1) break;
     m = nb;
   }
   return total + 256;
 }
 
+// K arrays: in + y in_stride -> out + y out_stride, y < K
 template <class IN>
-static hipError_t scan_rec(const IN* in, unsigned long long* out, unsigned long long n, unsigned long long* tmp,
-                           hipStream_t s) {
+static hipError_t scan_rec(const IN* in, size_t in_stride, unsigned long long* out, size_t out_stride,
+                           unsigned long long n, int K, unsigned long long* tmp, hipStream_t s) {
   const unsigned long long nb = (n + kScanBlock - 1) / kScanBlock;
-  if (nb == 1) {  // one block: the scan and the total in one launch
-    hipLaunchKernelGGL(k_scan_apply<IN>, dim3(1), dim3(1024), 0, s, in, n, nullptr, out);
+  if (nb == 1) {  // one block per array: the scan and the total in one launch
+    hipLaunchKernelGGL(k_scan_apply<IN>, dim3(1, (unsigned)K), dim3(1024), 0, s, in, n, nullptr, out, in_stride,
+                       (size_t)0, out_stride);
     return hipGetLastError();
   }
+  const size_t st = nb + 1;  // per-array stride of the block sums and offsets
   unsigned long long* sums = tmp;
-  unsigned long long* offs = tmp + nb + 1;
-  hipLaunchKernelGGL(k_scan_reduce<IN>, dim3((unsigned)nb), dim3(1024), 0, s, in, n, sums);
-  hipError_t e = scan_rec<unsigned long long>(sums, offs, nb, tmp + 2 * (nb + 1), s);
+  unsigned long long* offs = tmp + (size_t)K * st;
+  hipLaunchKernelGGL(k_scan_reduce<IN>, dim3((unsigned)nb, (unsigned)K), dim3(1024), 0, s, in, n, sums, in_stride,
+                     st);
+  hipError_t e = scan_rec<unsigned long long>(sums, st, offs, st, nb, K, tmp + 2 * (size_t)K * st, s);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_scan_apply<IN>, dim3((unsigned)nb), dim3(1024), 0, s, in, n, offs, out);
+  hipLaunchKernelGGL(k_scan_apply<IN>, dim3((unsigned)nb, (unsigned)K), dim3(1024), 0, s, in, n, offs, out,
+                     in_stride, st, out_stride);
   return hipGetLastError();
 }
 
@@ -3732,7 +3746,20 @@ hipError_t scan_u32_to_u64(const unsigned* in, unsigned long long* out, unsigned
                            size_t tmp_bytes, hipStream_t s) {
   if (n == 0) return launch_fill_u32(reinterpret_cast<unsigned*>(out), 0u, 2, s);
   if (tmp_bytes < scan_tmp_bytes(n)) return hipErrorInvalidValue;
-  return scan_rec<unsigned>(in, out, n, static_cast<unsigned long long*>(tmp), s);
+  return scan_rec<unsigned>(in, 0, out, 0, n, 1, static_cast<unsigned long long*>(tmp), s);
+}
+
+hipError_t scan3_u32_to_u64(const unsigned* in, size_t in_stride, unsigned long long* out, size_t out_stride,
+                            unsigned long long n, void* tmp, size_t tmp_bytes, hipStream_t s) {
+  if (n == 0) {
+    for (int y = 0; y < 3; ++y) {
+      const hipError_t e = launch_fill_u32(reinterpret_cast<unsigned*>(out + y * out_stride), 0u, 2, s);
+      if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+  }
+  if (tmp_bytes < scan_tmp_bytes(n, 3) || in_stride < n || out_stride < n + 1) return hipErrorInvalidValue;
+  return scan_rec<unsigned>(in, in_stride, out, out_stride, n, 3, static_cast<unsigned long long*>(tmp), s);
 }
 
 hipError_t launch_isect_dedup_flags(const IPair* pairs, unsigned long long n, int cells2, int* firstpos2,

@@ -1137,7 +1137,7 @@ int jsi_ctx::intersect(int h1, const jsi_region& r1, int h2, const jsi_region& r
         RCK(buf("i.scount", 3 * std::max(ns, (size_t)1), &cnt));
         RCK(buf("i.soff", 3 * (ns + 1), &off));
         RCK(buf("i.stot", 3, &tot));
-        const size_t stb = scan_tmp_bytes(ns);
+        const size_t stb = scan_tmp_bytes(ns, 3);
         RCK(buf_raw("i.scantmp", stb, &scantmp, 0));
         unsigned *cp = cnt, *cr = cnt + ns, *cc = cnt + 2 * ns;
         unsigned long long *op = off, *orr = off + ns + 1, *oc = off + 2 * (ns + 1);
@@ -1149,9 +1149,7 @@ int jsi_ctx::intersect(int h1, const jsi_region& r1, int h2, const jsi_region& r
           RCK(buf("i.masks", std::max(mrows, (size_t)1), &masks));
         HCK(launch_isect_slot_count<T>(nslots, w1, w2, su1, su2, C1, R1, C2, R2, lo1, hi1, lo2, hi2, P.cull,
                                        (long long)ns, slist, cp, cr, cc, fp, stream, masks));
-        HCK(scan_u32_to_u64(cp, op, ns, scantmp, stb, stream));
-        HCK(scan_u32_to_u64(cr, orr, ns, scantmp, stb, stream));
-        HCK(scan_u32_to_u64(cc, oc, ns, scantmp, stb, stream));
+        HCK(scan3_u32_to_u64(cp, ns, op, ns + 1, ns, scantmp, stb, stream));  // pairs, rows, columns
         HCK(launch_gather_totals(op, orr, oc, (long long)ns, tot, stream));
         unsigned long long t3[3];
         RCK(d2h(t3, tot, sizeof(t3)));
