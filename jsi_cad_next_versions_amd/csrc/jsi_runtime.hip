@@ -148,6 +148,12 @@ int jsi_ctx::pinned(size_t bytes, void** out) {
 
 int jsi_ctx::d2h(void* dst, const void* src, size_t bytes) {
   if (bytes == 0) return JSI_OK;
+  if (pin_small_ && bytes <= kPinSmall && dst != pin_small_) {
+    HCK(hipMemcpyAsync(pin_small_, src, bytes, hipMemcpyDeviceToHost, stream));
+    HCK(hipStreamSynchronize(stream));
+    std::memcpy(dst, pin_small_, bytes);
+    return JSI_OK;
+  }
   HCK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream));
   HCK(hipStreamSynchronize(stream));
   return JSI_OK;
@@ -155,6 +161,12 @@ int jsi_ctx::d2h(void* dst, const void* src, size_t bytes) {
 
 int jsi_ctx::h2d(void* dst, const void* src, size_t bytes) {
   if (bytes == 0) return JSI_OK;
+  if (pin_small_ && bytes <= kPinSmall && src != pin_small_) {
+    std::memcpy(pin_small_, src, bytes);
+    HCK(hipMemcpyAsync(dst, pin_small_, bytes, hipMemcpyHostToDevice, stream));
+    HCK(hipStreamSynchronize(stream));
+    return JSI_OK;
+  }
   HCK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream));
   HCK(hipStreamSynchronize(stream));
   return JSI_OK;
@@ -1614,6 +1626,7 @@ int jsi_ctx_create(int device, size_t hbm_cap_bytes, jsi_ctx** out) {
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->ncu = prop.multiProcessorCount;
   if (c->ncu <= 0) c->ncu = 256;
   c->grid = c->ncu * 8;
+  if (hipHostMalloc(&c->pin_small_, jsi_ctx::kPinSmall, hipHostMallocDefault) != hipSuccess) c->pin_small_ = nullptr;
   *out = c;
   return JSI_OK;
 }
@@ -1637,6 +1650,7 @@ int jsi_ctx_create_worker(jsi_ctx* parent, jsi_ctx** out) {
     set_err("hipStreamCreate: %s", hipGetErrorString(e));
     return JSI_ERR_DEVICE;
   }
+  if (hipHostMalloc(&c->pin_small_, jsi_ctx::kPinSmall, hipHostMallocDefault) != hipSuccess) c->pin_small_ = nullptr;
   *out = c;
   return JSI_OK;
 }
@@ -1662,6 +1676,7 @@ int jsi_ctx_destroy(jsi_ctx* c) {
   for (hipEvent_t e : c->ev)
     if (e) (void)hipEventDestroy(e);
   if (c->pinned_) (void)hipHostFree(c->pinned_);
+  if (c->pin_small_) (void)hipHostFree(c->pin_small_);
   (void)hipStreamDestroy(c->stream);
   delete c;
   return JSI_OK;
