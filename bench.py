@@ -246,7 +246,7 @@ def main():
     ap.add_argument("--serve-rates", default="500,1000,2000,3000,4000",
                     help="Poisson arrival rates of the uneven-mix runs (req/s)")
     ap.add_argument("--serve-seconds", type=float, default=2.0, help="arrival window of the mix runs (0 = skip)")
-    ap.add_argument("--concurrency", type=int, default=4,
+    ap.add_argument("--concurrency", type=int, default=6,
                     help="server workers of the served form (each takes batches of up to 64 queued queries)")
     ap.add_argument("--mix-slots", type=int, default=8,
                     help="server workers of the uneven batch and the mixed serving runs")

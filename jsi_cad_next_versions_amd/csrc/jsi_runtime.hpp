@@ -163,10 +163,11 @@ struct jsi_ctx {
   // page-locked host staging (grown on demand, freed with the context)
   void* pinned_ = nullptr;
   size_t pinned_bytes_ = 0;
-  // page-locked staging of the small copies a query makes (round totals,
-  // regions), allocated with the context: a copy to or from pageable memory
-  // goes through the runtime's own staging and waits longer
-  static constexpr size_t kPinSmall = 4096;
+  // page-locked staging of the copies of up to 1 MB a query makes (round
+  // totals, regions, result lists), allocated with the context: a copy to or
+  // from pageable memory goes through the runtime's own staging and waits
+  // longer
+  static constexpr size_t kPinSmall = size_t(1) << 20;
   void* pin_small_ = nullptr;
   int pinned(size_t bytes, void** out);
   int d2h(void* dst, const void* src, size_t bytes);
