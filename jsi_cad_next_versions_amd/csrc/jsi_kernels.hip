@@ -1007,9 +1007,24 @@ __device__ __forceinline__ void k_dist_filter_body(const uint64_t* list, const u
     const int nA = g1.ht * g1.wd, nB = g2.ht * g2.wd;
     const int nBp = (nB + U - 1) / U * U;
     __syncthreads();
-    if ((int)threadIdx.x < nB) {
+    // side-2 cell t of the tile pair: staged in LDS (EXP: as -2b, |b|^2) and
+    // kept in registers for the chunk boxes
+    const bool bin = (int)threadIdx.x < nB;
+    vec4_t<T> bme;
+    bme.x = bme.y = bme.z = bme.w = T(0);
+    if (bin) {
       const int j = (g2.row0 + threadIdx.x / g2.wd) * w2 + g2.col0 + threadIdx.x % g2.wd;
-      sB[threadIdx.x] = C2[(size_t)u2 * w2 * w2 + j];
+      bme = C2[(size_t)u2 * w2 * w2 + j];
+      if (EXP) {
+        vec4_t<T> x;
+        x.x = T(-2) * bme.x;
+        x.y = T(-2) * bme.y;
+        x.z = T(-2) * bme.z;
+        x.w = bme.x * bme.x + bme.y * bme.y + bme.z * bme.z;
+        sB[threadIdx.x] = x;
+      } else {
+        sB[threadIdx.x] = bme;
+      }
       sJ[threadIdx.x] = j;
     } else if ((int)threadIdx.x < nBp) {
       vec4_t<T> b;
@@ -1025,43 +1040,37 @@ __device__ __forceinline__ void k_dist_filter_body(const uint64_t* list, const u
       i = (g1.row0 + threadIdx.x / g1.wd) * w1 + g1.col0 + threadIdx.x % g1.wd;
       a = C1[(size_t)u1 * w1 * w1 + i];
     }
-    __syncthreads();
-    // bounding box of every 32-cell chunk of the side-2 tile
+    // bounding box of every 32-cell chunk of the side-2 tile: a reduction
+    // over the 32 lanes that hold the chunk's cells (min and max do not
+    // depend on the order)
     constexpr int kChunk = 32;
     const int nchunk = (nBp + kChunk - 1) / kChunk;
-    if ((int)threadIdx.x < nchunk) {
+    {
       const T inf = infinity_v<T>();
-      vec4_t<T> lo, hi;
-      lo.x = lo.y = lo.z = inf;
-      hi.x = hi.y = hi.z = -inf;
-      lo.w = hi.w = T(0);
-      const int c0 = threadIdx.x * kChunk, c1 = min(nB, c0 + kChunk);
-      for (int c = c0; c < c1; ++c) {
-        const vec4_t<T> b = sB[c];
-        lo.x = fmin(lo.x, b.x);
-        lo.y = fmin(lo.y, b.y);
-        lo.z = fmin(lo.z, b.z);
-        hi.x = fmax(hi.x, b.x);
-        hi.y = fmax(hi.y, b.y);
-        hi.z = fmax(hi.z, b.z);
+      T bl[3] = {bin ? bme.x : inf, bin ? bme.y : inf, bin ? bme.z : inf};
+      T bh[3] = {bin ? bme.x : -inf, bin ? bme.y : -inf, bin ? bme.z : -inf};
+#pragma unroll
+      for (int off = kChunk / 2; off > 0; off >>= 1)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          bl[c] = fmin(bl[c], __shfl_xor(bl[c], off));
+          bh[c] = fmax(bh[c], __shfl_xor(bh[c], off));
+        }
+      if ((threadIdx.x & (kChunk - 1)) == 0 && (int)(threadIdx.x / kChunk) < nchunk) {
+        vec4_t<T> lo, hi;
+        lo.x = bl[0];
+        lo.y = bl[1];
+        lo.z = bl[2];
+        hi.x = bh[0];
+        hi.y = bh[1];
+        hi.z = bh[2];
+        lo.w = hi.w = T(0);
+        sLo[threadIdx.x / kChunk] = lo;
+        sHi[threadIdx.x / kChunk] = hi;
       }
-      sLo[threadIdx.x] = lo;
-      sHi[threadIdx.x] = hi;
     }
     T thr = tv;
-    if (EXP) {
-      __syncthreads();  // chunk boxes read the plain centres
-      if ((int)threadIdx.x < nB) {
-        const vec4_t<T> b = sB[threadIdx.x];
-        vec4_t<T> x;
-        x.x = T(-2) * b.x;
-        x.y = T(-2) * b.y;
-        x.z = T(-2) * b.z;
-        x.w = b.x * b.x + b.y * b.y + b.z * b.z;
-        sB[threadIdx.x] = x;
-      }
-      thr = (tv - (a.x * a.x + a.y * a.y + a.z * a.z)) + delta;  // NaN for inactive lanes
-    }
+    if (EXP) thr = (tv - (a.x * a.x + a.y * a.y + a.z * a.z)) + delta;  // NaN for inactive lanes
     __syncthreads();
     const uint64_t key_si = pack_idx((uint32_t)slot, (uint32_t)i, 0u);
     const int nvalid = __popcll(__ballot(va));
