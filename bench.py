@@ -236,7 +236,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--polish", type=int, default=64, help="top-k records polished to FP64 closest points")
     ap.add_argument("--isect-steps", type=int, default=5, help="intersection-curve queries (saddle / wave plane)")
-    ap.add_argument("--eval-steps", type=int, default=5, help="grid evaluations (evalWithCuda, nuv=1024) per dtype")
+    ap.add_argument("--eval-steps", type=int, default=10, help="grid evaluations (evalWithCuda, nuv=1024) per dtype")
     ap.add_argument("--batch", type=int, default=4096, help="queries of the uneven batch (0 = skip)")
     ap.add_argument("--latency-rate", type=float, default=3000.0,
                     help="offered rate (req/s) of the open-loop run whose p50/p99 head the JSON line")
@@ -396,7 +396,10 @@ def main():
         evaluation = {"workload": "evalWithCuda grid of the hard-wave surface, nuv=1024 (1,048,576 points, 18 values each)",
                       "peak_GBps": 8000.0}
         for dt, name in ((np.float32, "f32"), (np.float64, "f64")):
-            ctx.eval_grid(a, 1.0 / 1024, 1.0 / 1024, 0.0, 0.0, 1024, dtype=dt)  # warm-up
+            # warm-up: after the host-side legs before it the GPU has idled and
+            # lowered its clocks, which a single launch does not bring back
+            for _ in range(10):
+                ctx.eval_grid(a, 1.0 / 1024, 1.0 / 1024, 0.0, 0.0, 1024, dtype=dt)
             ctx.profile(True)
             for _ in range(args.eval_steps):
                 ctx.eval_grid(a, 1.0 / 1024, 1.0 / 1024, 0.0, 0.0, 1024, dtype=dt)
