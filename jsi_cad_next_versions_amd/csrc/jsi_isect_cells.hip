@@ -540,16 +540,77 @@ __global__ __launch_bounds__(256) void k_isect_slot_list(SlotScreen<T> s, const 
 template <class T>
 __global__ __launch_bounds__(256) void k_isect_slot_count(SlotScreen<T> s, long long n, const int* list,
                                                           unsigned* npair, unsigned* nrow, unsigned* ncol,
-                                                          unsigned long long* masks) {
+                                                          unsigned long long* masks, int stage) {
+  // stage (FP64 screen, launched with the LDS for it): each wavefront copies
+  // the side-2 cells of its slots into LDS once, and the rows test them from
+  // there instead of every lane loading each cell from device memory
+  extern __shared__ unsigned char sdyn[];
   const int G = slot_lanes(s.cells1), spw = slot_groups(s.cells1, s.cells2);
   const long long nwaves = (n + spw - 1) / spw;
   const int lane = threadIdx.x & 63, grp = lane / G, i = lane % G;
+  const int nst = spw * s.cells2;  // staged cells per wavefront
+  vec4_t<T>* const stC = reinterpret_cast<vec4_t<T>*>(sdyn) + (size_t)(threadIdx.x >> 6) * 2 * nst;
+  vec4_t<T>* const stR = stC + nst;
   for (long long base = (long long)blockIdx.x * 4; base < nwaves; base += (long long)gridDim.x * 4) {
-    const long long e = (base + (threadIdx.x >> 6)) * spw + grp;
+    const long long wbase = (base + (threadIdx.x >> 6)) * spw;  // the wavefront's first entry
+    const long long e = wbase + grp;
     const bool in = grp < spw && e < n;
     const long long slot = in ? (list ? (long long)list[e] : e) : 0;
     const bool valid = in && i < s.cells1 && (list || slot_boxes_meet(s, slot));
-    const unsigned long long m = valid ? slot_row_mask(s, slot, i) : 0ull;
+    unsigned long long m = 0;
+    if (stage) {  // uniform: a kernel argument
+      // FP32SCREEN stages the float records (the FP64 test of the candidates
+      // reads its cells from device memory), the FP64 screen the cells
+      float4* const sfX = reinterpret_cast<float4*>(stC);
+      float4* const sfY = sfX + nst;
+      for (int x = lane; x < nst; x += 64) {
+        const int g = x / s.cells2, j = x - g * s.cells2;
+        if (wbase + g < n) {
+          const long long sg = list ? (long long)list[wbase + g] : wbase + g;
+          const size_t b = (size_t)s.su2[sg] * s.cells2 + j;
+          if (s.f.X1) {
+            sfX[x] = s.f.X2[b];
+            sfY[x] = s.f.Y2[b];
+          } else {
+            stC[x] = s.C2[b];
+            stR[x] = s.R2[b];
+          }
+        }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      if (valid) {
+        const size_t a = (size_t)s.su1[slot] * s.cells1 + i;
+        if (s.f.X1) {  // slot_row_mask's two tests, the float records from LDS
+          const float4 x1 = s.f.X1[a], y1 = s.f.Y1[a];
+          const float4* const gx = sfX + grp * s.cells2;
+          const float4* const gy = sfY + grp * s.cells2;
+          unsigned long long cand = 0;
+#pragma unroll 4
+          for (int j = 0; j < s.cells2; ++j) cand |= (unsigned long long)cells_overlap(x1, y1, gx[j], gy[j]) << j;
+          if (cand) {
+            const vec4_t<T> c1 = s.C1[a], r1 = s.R1[a];
+            const size_t b0 = (size_t)s.su2[slot] * s.cells2;
+            for (unsigned long long r = cand; r; r &= r - 1) {
+              const int j = __ffsll((long long)r) - 1;
+              m |= (unsigned long long)cells_overlap(c1, r1, s.C2[b0 + j], s.R2[b0 + j]) << j;
+            }
+          }
+        } else {
+          const vec4_t<T> c1 = s.C1[a], r1 = s.R1[a];
+          const vec4_t<T>* const gc = stC + grp * s.cells2;
+          const vec4_t<T>* const gr = stR + grp * s.cells2;
+#pragma unroll 4
+          for (int j = 0; j < s.cells2; ++j) m |= (unsigned long long)cells_overlap(c1, r1, gc[j], gr[j]) << j;
+        }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();  // the next entries' cells overwrite the staging
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+      m = valid ? slot_row_mask(s, slot, i) : 0ull;
+    }
     if (masks && in && i < s.cells1) masks[e * s.cells1 + i] = m;  // for the emit pass
     unsigned pc = (unsigned)__popcll(m), nz = m != 0;
     unsigned long long om = m;
@@ -793,8 +854,13 @@ hipError_t launch_isect_slot_count(long long nslots, int w1, int w2, const int* 
   if (!slot_args_ok(nslots, w1, w2, list ? 0 : cull, lo1, hi1, lo2, hi2, f) || n > nslots)
     return hipErrorInvalidValue;
   const SlotScreen<T> sc = slot_screen<T>(nslots, w1, w2, su1, su2, C1, R1, C2, R2, lo1, hi1, lo2, hi2, cull, f);
-  hipLaunchKernelGGL(k_isect_slot_count<T>, dim3(slot_blocks(n, sc.cells1, sc.cells2)), dim3(256), 0, s, sc, n, list,
-                     npair, nrow, ncol, masks);
+  // each wavefront stages its slots' side-2 cells (FP32SCREEN: their float
+  // records) in LDS, up to 32 KB per workgroup
+  const size_t st_bytes =
+      (size_t)4 * slot_groups(sc.cells1, sc.cells2) * sc.cells2 * 2 * sizeof(vec4_t<T>);
+  const int stage = st_bytes <= 32768 ? 1 : 0;
+  hipLaunchKernelGGL(k_isect_slot_count<T>, dim3(slot_blocks(n, sc.cells1, sc.cells2)), dim3(256),
+                     stage ? st_bytes : 0, s, sc, n, list, npair, nrow, ncol, masks, stage);
   return hipGetLastError();
 }
 
