@@ -249,9 +249,11 @@ hipError_t segmented_sort_pairs(IPair* pairs, unsigned long long n, const unsign
 hipError_t scan_u32_to_u64(const unsigned* in, unsigned long long* out, unsigned long long n, void* tmp,
                            size_t tmp_bytes, hipStream_t s);
 // three scans of n entries in the same launches: in + y in_stride -> out + y
-// out_stride (in_stride >= n, out_stride >= n + 1), y = 0..2
+// out_stride (in_stride >= n, out_stride >= n + 1), y = 0..2; totals
+// (optional): the three totals side by side
 hipError_t scan3_u32_to_u64(const unsigned* in, size_t in_stride, unsigned long long* out, size_t out_stride,
-                            unsigned long long n, void* tmp, size_t tmp_bytes, hipStream_t s);
+                            unsigned long long n, void* tmp, size_t tmp_bytes, hipStream_t s,
+                            unsigned long long* totals = nullptr);
 // scratch of a scan of n entries over `arrays` arrays at once
 size_t scan_tmp_bytes(unsigned long long n, int arrays = 1);
 hipError_t launch_isect_dedup_flags(const IPair* pairs, unsigned long long n, int cells2, int* firstpos2,
@@ -328,9 +330,6 @@ hipError_t launch_isect_slot_emit(long long nslots, int w1, int w2, const int* s
                                   const vec4_t<T>* reg2, int pnuv1, int pnuv2, int* nsu1, int* nsu2,
                                   vec4_t<T>* nreg1, vec4_t<T>* nreg2, const SlotF32* f, hipStream_t s,
                                   const unsigned long long* masks = nullptr);
-// out[0..2] = a[n], b[n], c[n] (the totals of three scans, one copy back)
-hipError_t launch_gather_totals(const unsigned long long* a, const unsigned long long* b,
-                                const unsigned long long* c, long long n, unsigned long long* out, hipStream_t s);
 // b may be null; two jobs of the same kernel variant share one launch.
 template <class T>
 hipError_t launch_isect_cells(const IcJob<T>& a, const IcJob<T>* b, int use_k, hipStream_t s);

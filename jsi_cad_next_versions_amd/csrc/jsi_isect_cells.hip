@@ -701,13 +701,6 @@ __global__ __launch_bounds__(256) void k_isect_slot_emit(SlotScreen<T> s, long l
   }
 }
 
-__global__ void k_gather_totals(const unsigned long long* a, const unsigned long long* b,
-                                const unsigned long long* c, long long n, unsigned long long* out) {
-  out[0] = a[n];
-  out[1] = b[n];
-  out[2] = c[n];
-}
-
 template <class T>
 int ic_blocks(const IcJob<T>& j) {
   if (j.nunits <= 0 || j.nuv < 2 || j.nuv > kIcMaxNuv) return 0;
@@ -895,11 +888,6 @@ hipError_t launch_isect_slot_emit(long long nslots, int w1, int w2, const int* s
   return hipGetLastError();
 }
 
-hipError_t launch_gather_totals(const unsigned long long* a, const unsigned long long* b,
-                                const unsigned long long* c, long long n, unsigned long long* out, hipStream_t s) {
-  hipLaunchKernelGGL(k_gather_totals, dim3(1), dim3(1), 0, s, a, b, c, n, out);
-  return hipGetLastError();
-}
 
 #define JSI_IS(T)                                                                                                  \
   template hipError_t launch_isect_slot_cull<T>(long long, int, int, const int*, const int*, const vec4_t<T>*,     \

@@ -2823,13 +2823,32 @@ __global__ __launch_bounds__(1024) void k_scan_reduce(const IN* in, unsigned lon
 }
 
 template <class IN>
+// block_off: each block's offset; or, with block_off null and sums given,
+// the block adds up sums[0 .. blockIdx.x) itself (at most 1024 blocks), so a
+// scan of up to 4M entries takes two launches.  total (optional): total[y]
+// receives the array's total as well as out[n].
 __global__ __launch_bounds__(1024) void k_scan_apply(const IN* in, unsigned long long n,
                                                      const unsigned long long* block_off, unsigned long long* out,
-                                                     size_t in_stride, size_t off_stride, size_t out_stride) {
+                                                     size_t in_stride, size_t off_stride, size_t out_stride,
+                                                     const unsigned long long* sums, unsigned long long* total) {
   __shared__ unsigned long long s[1024];
   in += blockIdx.y * in_stride;
   out += blockIdx.y * out_stride;
   if (block_off) block_off += blockIdx.y * off_stride;
+  unsigned long long boff = 0;
+  if (!block_off && sums) {
+    sums += blockIdx.y * off_stride;
+    s[threadIdx.x] = threadIdx.x < blockIdx.x ? sums[threadIdx.x] : 0ull;
+    __syncthreads();
+    for (int off = 512; off > 0; off >>= 1) {
+      if ((int)threadIdx.x < off) s[threadIdx.x] += s[threadIdx.x + off];
+      __syncthreads();
+    }
+    boff = s[0];
+    __syncthreads();
+  } else if (block_off) {
+    boff = block_off[blockIdx.x];
+  }
   const unsigned long long base = (unsigned long long)blockIdx.x * kScanBlock;
   unsigned long long v[4];
   unsigned long long t = 0;
@@ -2846,12 +2865,15 @@ __global__ __launch_bounds__(1024) void k_scan_apply(const IN* in, unsigned long
     s[threadIdx.x] += x;
     __syncthreads();
   }
-  unsigned long long run = (block_off ? block_off[blockIdx.x] : 0ull) + s[threadIdx.x] - t;
+  unsigned long long run = boff + s[threadIdx.x] - t;
   for (int e = 0; e < 4; ++e) {
     const unsigned long long i = base + threadIdx.x * 4 + e;
     if (i < n) out[i] = run;
     run += v[e];
-    if (i == n - 1) out[n] = run;
+    if (i == n - 1) {
+      out[n] = run;
+      if (total) total[blockIdx.y] = run;
+    }
   }
 }
 
@@ -3729,14 +3751,17 @@ size_t scan_tmp_bytes(unsigned long long n, int arrays) {
   return total + 256;
 }
 
-// K arrays: in + y in_stride -> out + y out_stride, y < K
+// K arrays: in + y in_stride -> out + y out_stride, y < K (total: see
+// k_scan_apply)
 template <class IN>
 static hipError_t scan_rec(const IN* in, size_t in_stride, unsigned long long* out, size_t out_stride,
-                           unsigned long long n, int K, unsigned long long* tmp, hipStream_t s) {
+                           unsigned long long n, int K, unsigned long long* tmp, hipStream_t s,
+                           unsigned long long* total = nullptr) {
   const unsigned long long nb = (n + kScanBlock - 1) / kScanBlock;
+  const unsigned long long* none = nullptr;
   if (nb == 1) {  // one block per array: the scan and the total in one launch
-    hipLaunchKernelGGL(k_scan_apply<IN>, dim3(1, (unsigned)K), dim3(1024), 0, s, in, n, nullptr, out, in_stride,
-                       (size_t)0, out_stride);
+    hipLaunchKernelGGL(k_scan_apply<IN>, dim3(1, (unsigned)K), dim3(1024), 0, s, in, n, none, out, in_stride,
+                       (size_t)0, out_stride, none, total);
     return hipGetLastError();
   }
   const size_t st = nb + 1;  // per-array stride of the block sums and offsets
@@ -3744,10 +3769,15 @@ static hipError_t scan_rec(const IN* in, size_t in_stride, unsigned long long* o
   unsigned long long* offs = tmp + (size_t)K * st;
   hipLaunchKernelGGL(k_scan_reduce<IN>, dim3((unsigned)nb, (unsigned)K), dim3(1024), 0, s, in, n, sums, in_stride,
                      st);
+  if (nb <= 1024) {  // each block adds up the sums before it
+    hipLaunchKernelGGL(k_scan_apply<IN>, dim3((unsigned)nb, (unsigned)K), dim3(1024), 0, s, in, n, none, out,
+                       in_stride, st, out_stride, (const unsigned long long*)sums, total);
+    return hipGetLastError();
+  }
   hipError_t e = scan_rec<unsigned long long>(sums, st, offs, st, nb, K, tmp + 2 * (size_t)K * st, s);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_scan_apply<IN>, dim3((unsigned)nb, (unsigned)K), dim3(1024), 0, s, in, n, offs, out,
-                     in_stride, st, out_stride);
+  hipLaunchKernelGGL(k_scan_apply<IN>, dim3((unsigned)nb, (unsigned)K), dim3(1024), 0, s, in, n,
+                     (const unsigned long long*)offs, out, in_stride, st, out_stride, none, total);
   return hipGetLastError();
 }
 
@@ -3759,16 +3789,17 @@ hipError_t scan_u32_to_u64(const unsigned* in, unsigned long long* out, unsigned
 }
 
 hipError_t scan3_u32_to_u64(const unsigned* in, size_t in_stride, unsigned long long* out, size_t out_stride,
-                            unsigned long long n, void* tmp, size_t tmp_bytes, hipStream_t s) {
+                            unsigned long long n, void* tmp, size_t tmp_bytes, hipStream_t s,
+                            unsigned long long* totals) {
   if (n == 0) {
     for (int y = 0; y < 3; ++y) {
       const hipError_t e = launch_fill_u32(reinterpret_cast<unsigned*>(out + y * out_stride), 0u, 2, s);
       if (e != hipSuccess) return e;
     }
-    return hipSuccess;
+    return totals ? launch_fill_u32(reinterpret_cast<unsigned*>(totals), 0u, 6, s) : hipSuccess;
   }
   if (tmp_bytes < scan_tmp_bytes(n, 3) || in_stride < n || out_stride < n + 1) return hipErrorInvalidValue;
-  return scan_rec<unsigned>(in, in_stride, out, out_stride, n, 3, static_cast<unsigned long long*>(tmp), s);
+  return scan_rec<unsigned>(in, in_stride, out, out_stride, n, 3, static_cast<unsigned long long*>(tmp), s, totals);
 }
 
 hipError_t launch_isect_dedup_flags(const IPair* pairs, unsigned long long n, int cells2, int* firstpos2,

@@ -1161,8 +1161,7 @@ int jsi_ctx::intersect(int h1, const jsi_region& r1, int h2, const jsi_region& r
           RCK(buf("i.masks", std::max(mrows, (size_t)1), &masks));
         HCK(launch_isect_slot_count<T>(nslots, w1, w2, su1, su2, C1, R1, C2, R2, lo1, hi1, lo2, hi2, P.cull,
                                        (long long)ns, slist, cp, cr, cc, fp, stream, masks));
-        HCK(scan3_u32_to_u64(cp, ns, op, ns + 1, ns, scantmp, stb, stream));  // pairs, rows, columns
-        HCK(launch_gather_totals(op, orr, oc, (long long)ns, tot, stream));
+        HCK(scan3_u32_to_u64(cp, ns, op, ns + 1, ns, scantmp, stb, stream, tot));  // pairs, rows, columns
         unsigned long long t3[3];
         RCK(d2h(t3, tot, sizeof(t3)));
         npairs = t3[0];
