@@ -218,7 +218,9 @@ hipError_t launch_dist_dedup(const Cand* rec, int n, int cells1, int cells2, con
                              unsigned long long* zero = nullptr, int nzero = 0);
 size_t dedup_work_bytes();  // size of `work` for launch_dist_dedup
 
-// Intersection screen.
+// Intersection screen.  launch_isect_pass masks (optional, mask_tp x 256 x 4
+// words): the count pass (write = 0) keeps the hits of the first mask_tp
+// listed tile pairs there and the write pass reads them.
 template <class T>
 hipError_t launch_isect_list(int nslots, const int* su1, const int* su2, int w1, int w2, const vec4_t<T>* lo1,
                              const vec4_t<T>* hi1, const vec4_t<T>* lo2, const vec4_t<T>* hi2, int cull,
@@ -229,7 +231,7 @@ hipError_t launch_isect_pass(int write, const uint64_t* list, const unsigned lon
                              const int* su2, int w1, int w2, const vec4_t<T>* C1, const vec4_t<T>* R1,
                              const vec4_t<T>* C2, const vec4_t<T>* R2, unsigned* rowcount,
                              const unsigned long long* rowoff, unsigned* rowcursor, IPair* out, int grid,
-                             hipStream_t s);
+                             hipStream_t s, unsigned long long* masks = nullptr, unsigned long long mask_tp = 0);
 hipError_t launch_row_sort(const unsigned long long* rowoff, unsigned long long nrows, IPair* out, hipStream_t s);
 // FP32SCREEN intersection: float records covering the FP64 (and, given Cv/Rv,
 // the VERSION5 float) acceptance intervals of each cell; FP64 re-test of the

@@ -2634,7 +2634,11 @@ __global__ __launch_bounds__(256) void k_isect_pass(const uint64_t* list, const 
                                                     const vec4_t<T>* C1, const vec4_t<T>* R1,
                                                     const vec4_t<T>* C2, const vec4_t<T>* R2, unsigned* rowcount,
                                                     const unsigned long long* rowoff, unsigned* rowcursor,
-                                                    IPair* out) {
+                                                    IPair* out, unsigned long long* masks,
+                                                    unsigned long long mask_tp) {
+  // masks (optional): the count pass keeps each lane's hits of the first
+  // mask_tp listed tile pairs (four words: bit jj of word jj / 64), and the
+  // write pass reads them instead of repeating the tests
   __shared__ vec4_t<T> sC[kTileCells];
   __shared__ vec4_t<T> sR[kTileCells];
   __shared__ int sJ[kTileCells];
@@ -2666,24 +2670,44 @@ __global__ __launch_bounds__(256) void k_isect_pass(const uint64_t* list, const 
     __syncthreads();
     if (!va) continue;
     const size_t row = (size_t)slot * cells1 + i;
-    unsigned cnt = 0;
-    for (int jj = 0; jj < nB; ++jj) {
-      const vec4_t<T> c2 = sC[jj], r2 = sR[jj];
-      const bool hit = fabs(c1.x - c2.x) < (r1.x + r2.x) && fabs(c1.y - c2.y) < (r1.y + r2.y) &&
-                       fabs(c1.z - c2.z) < (r1.z + r2.z);
-      if (hit) {
-        if (WRITE) {
+    unsigned long long* const mw = masks && it < mask_tp ? masks + ((size_t)it * kTileCells + threadIdx.x) * 4 : nullptr;
+    if (WRITE && mw) {
+      for (int w = 0; w < 4; ++w)
+        for (unsigned long long r = mw[w]; r; r &= r - 1) {
+          const int jj = 64 * w + __ffsll((long long)r) - 1;
           const unsigned k = atomicAdd(&rowcursor[row], 1u);
           IPair pr;
           pr.slot = slot;
           pr.i = i;
           pr.j = sJ[jj];
           out[rowoff[row] + k] = pr;
-        } else {
-          ++cnt;
         }
+      continue;
+    }
+    unsigned cnt = 0;
+    unsigned long long word = 0;
+    for (int jj = 0; jj < nB; ++jj) {
+      const vec4_t<T> c2 = sC[jj], r2 = sR[jj];
+      const bool hit = (fabs(c1.x - c2.x) < (r1.x + r2.x)) & (fabs(c1.y - c2.y) < (r1.y + r2.y)) &
+                       (fabs(c1.z - c2.z) < (r1.z + r2.z));
+      if (!WRITE) {
+        word |= (unsigned long long)hit << (jj & 63);
+        if ((jj & 63) == 63 || jj == nB - 1) {
+          if (mw) mw[jj >> 6] = word;
+          cnt += (unsigned)__popcll(word);
+          word = 0;
+        }
+      } else if (hit) {
+        const unsigned k = atomicAdd(&rowcursor[row], 1u);
+        IPair pr;
+        pr.slot = slot;
+        pr.i = i;
+        pr.j = sJ[jj];
+        out[rowoff[row] + k] = pr;
       }
     }
+    if (!WRITE && mw)
+      for (int w = (nB + 63) >> 6; w < 4; ++w) mw[w] = 0;  // words past the tile's cells
     if (!WRITE && cnt) atomicAdd(&rowcount[row], cnt);
   }
 }
@@ -3702,13 +3726,14 @@ hipError_t launch_isect_pass(int write, const uint64_t* list, const unsigned lon
                              const int* su2, int w1, int w2, const vec4_t<T>* C1, const vec4_t<T>* R1,
                              const vec4_t<T>* C2, const vec4_t<T>* R2, unsigned* rowcount,
                              const unsigned long long* rowoff, unsigned* rowcursor, IPair* out, int grid,
-                             hipStream_t s) {
+                             hipStream_t s, unsigned long long* masks, unsigned long long mask_tp) {
+  if (!masks) mask_tp = 0;
   if (write)
     hipLaunchKernelGGL((k_isect_pass<T, true>), dim3(grid), dim3(256), 0, s, list, list_count, su1, su2, w1, w2, C1,
-                       R1, C2, R2, rowcount, rowoff, rowcursor, out);
+                       R1, C2, R2, rowcount, rowoff, rowcursor, out, masks, mask_tp);
   else
     hipLaunchKernelGGL((k_isect_pass<T, false>), dim3(grid), dim3(256), 0, s, list, list_count, su1, su2, w1, w2,
-                       C1, R1, C2, R2, rowcount, rowoff, rowcursor, out);
+                       C1, R1, C2, R2, rowcount, rowoff, rowcursor, out, masks, mask_tp);
   return hipGetLastError();
 }
 
@@ -3874,7 +3899,7 @@ hipError_t launch_fill_u32(unsigned* p, unsigned value, unsigned long long n, hi
   template hipError_t launch_isect_pass<T>(int, const uint64_t*, const unsigned long long*, const int*, const int*, \
                                            int, int, const vec4_t<T>*, const vec4_t<T>*, const vec4_t<T>*,         \
                                            const vec4_t<T>*, unsigned*, const unsigned long long*, unsigned*,      \
-                                           IPair*, int, hipStream_t);                                              \
+                                           IPair*, int, hipStream_t, unsigned long long*, unsigned long long);                                              \
   template hipError_t launch_isect_units<T>(const IPair*, unsigned long long, int, const int*, const int*,         \
                                             const vec4_t<T>*, const vec4_t<T>*, int, int, const unsigned*,         \
                                             const unsigned*, const unsigned long long*, const unsigned long long*, \

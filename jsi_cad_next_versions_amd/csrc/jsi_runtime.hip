@@ -1202,8 +1202,16 @@ int jsi_ctx::intersect(int h1, const jsi_region& r1, int h2, const jsi_region& r
         const size_t stb = scan_tmp_bytes(nrows);
         RCK(buf_raw("i.scantmp", stb, &scantmp, 0));
         HCK(launch_fill_u32(rowcount, 0u, nrows, stream));
+        // the hit masks of the first listed tile pairs (four words per
+        // side-1 cell, 8 KB per tile pair, up to kSlotMaskBytes), kept for
+        // the write pass
+        unsigned long long* tmasks = nullptr;
+        const unsigned long long mask_tp =
+            slot_masks ? std::min<unsigned long long>(ntp, kSlotMaskBytes / (kTileCells * 4 * sizeof(unsigned long long)))
+                       : 0ull;
+        if (mask_tp) RCK(buf("i.tmasks", (size_t)mask_tp * kTileCells * 4, &tmasks));
         HCK(launch_isect_pass<T>(0, list, small, su1, su2, w1, w2, C1, R1, C2, R2, rowcount, rowoff, rowcursor, nullptr,
-                                 grid, stream));
+                                 grid, stream, tmasks, mask_tp));
         HCK(scan_u32_to_u64(rowcount, rowoff, nrows, scantmp, stb, stream));
         RCK(d2h(&npairs, rowoff + nrows, sizeof(npairs)));
         if (round == 1 && use_k) RCK(initial_k<T>(nuv1, nuv2, r1, r2, iu, iv, res));
@@ -1211,7 +1219,7 @@ int jsi_ctx::intersect(int h1, const jsi_region& r1, int h2, const jsi_region& r
         if (npairs > 0) {
           HCK(launch_fill_u32(rowcursor, 0u, nrows, stream));
           HCK(launch_isect_pass<T>(1, list, small, su1, su2, w1, w2, C1, R1, C2, R2, rowcount, rowoff, rowcursor, pairs,
-                                   grid, stream));
+                                   grid, stream, tmasks, mask_tp));
           if (npairs > 16ull * nrows) {  // long rows: segmented radix sort
             const size_t tb = segmented_sort_tmp_bytes(npairs, nrows);
             void* st;
