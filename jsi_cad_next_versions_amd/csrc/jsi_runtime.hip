@@ -159,6 +159,27 @@ int jsi_ctx::d2h(void* dst, const void* src, size_t bytes) {
   return JSI_OK;
 }
 
+int jsi_ctx::d2h_many(std::initializer_list<Copy> copies) {
+  size_t off = 0;
+  for (const Copy& c : copies) off += (c.bytes + 15) & ~(size_t)15;
+  if (!pin_small_ || off > kPinSmall) {  // one by one
+    for (const Copy& c : copies) RCK(d2h(c.dst, c.src, c.bytes));
+    return JSI_OK;
+  }
+  off = 0;
+  for (const Copy& c : copies) {
+    if (c.bytes) HCK(hipMemcpyAsync(static_cast<char*>(pin_small_) + off, c.src, c.bytes, hipMemcpyDeviceToHost, stream));
+    off += (c.bytes + 15) & ~(size_t)15;
+  }
+  HCK(hipStreamSynchronize(stream));
+  off = 0;
+  for (const Copy& c : copies) {
+    if (c.bytes) std::memcpy(c.dst, static_cast<const char*>(pin_small_) + off, c.bytes);
+    off += (c.bytes + 15) & ~(size_t)15;
+  }
+  return JSI_OK;
+}
+
 int jsi_ctx::h2d(void* dst, const void* src, size_t bytes) {
   if (bytes == 0) return JSI_OK;
   if (pin_small_ && bytes <= kPinSmall && src != pin_small_) {
@@ -838,12 +859,16 @@ int jsi_ctx::isect_units(int h, const vec4_t<T>* reg, int nunits, int nuv, int u
 // k_aabb, so the float values are the ones the padding used).
 template <class T>
 int jsi_ctx::initial_k(int nuv1, int nuv2, const jsi_region& r1, const jsi_region& r2, const T* iu, const T* iv,
-                       jsi_result* res) {
+                       jsi_result* res, Copy extra) {
+  // both sides' maxima (and the caller's extra read-back) with one wait
+  unsigned long long* kmax1;
+  unsigned long long* kmax2;
+  RCK(buf("kmax.i1", 3, &kmax1));
+  RCK(buf("kmax.i2", 3, &kmax2));
+  unsigned long long mm[2][3];
+  RCK(d2h_many({Copy{mm[0], kmax1, sizeof(mm[0])}, Copy{mm[1], kmax2, sizeof(mm[1])}, extra}));
   for (int side = 1; side <= 2; ++side) {
-    unsigned long long* kmax;
-    RCK(buf(side == 1 ? "kmax.i1" : "kmax.i2", 3, &kmax));
-    unsigned long long m[3];
-    RCK(d2h(m, kmax, sizeof(m)));
+    const unsigned long long* m = mm[side - 1];
     T v[3];
     for (int c = 0; c < 3; ++c) {
       if (std::is_same<T, float>::value) {
@@ -1213,8 +1238,10 @@ int jsi_ctx::intersect(int h1, const jsi_region& r1, int h2, const jsi_region& r
         HCK(launch_isect_pass<T>(0, list, small, su1, su2, w1, w2, C1, R1, C2, R2, rowcount, rowoff, rowcursor, nullptr,
                                  grid, stream, tmasks, mask_tp));
         HCK(scan_u32_to_u64(rowcount, rowoff, nrows, scantmp, stb, stream));
-        RCK(d2h(&npairs, rowoff + nrows, sizeof(npairs)));
-        if (round == 1 && use_k) RCK(initial_k<T>(nuv1, nuv2, r1, r2, iu, iv, res));
+        if (round == 1 && use_k)  // the pair count comes back with the first round's maxima
+          RCK(initial_k<T>(nuv1, nuv2, r1, r2, iu, iv, res, Copy{&npairs, rowoff + nrows, sizeof(npairs)}));
+        else
+          RCK(d2h(&npairs, rowoff + nrows, sizeof(npairs)));
         RCK(buf(nm("pairs", 1, cur).c_str(), std::max(npairs, 1ull), &pairs));
         if (npairs > 0) {
           HCK(launch_fill_u32(rowcursor, 0u, nrows, stream));

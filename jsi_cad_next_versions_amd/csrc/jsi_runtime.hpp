@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <atomic>
+#include <initializer_list>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -171,6 +172,13 @@ struct jsi_ctx {
   void* pin_small_ = nullptr;
   int pinned(size_t bytes, void** out);
   int d2h(void* dst, const void* src, size_t bytes);
+  // several small device -> host copies with one wait (together at most 1 MB)
+  struct Copy {
+    void* dst;
+    const void* src;
+    size_t bytes;
+  };
+  int d2h_many(std::initializer_list<Copy> copies);
   int h2d(void* dst, const void* src, size_t bytes);
 
   template <class T>
@@ -205,7 +213,7 @@ struct jsi_ctx {
              int nuv1, int nuv2, jsi_result* res);
   template <class T>
   int initial_k(int nuv1, int nuv2, const jsi_region& r1, const jsi_region& r2, const T* iu, const T* iv,
-                jsi_result* res);
+                jsi_result* res, Copy extra = Copy{nullptr, nullptr, 0});
   template <class T>
   int intersect(int h1, const jsi_region& r1, int h2, const jsi_region& r2, const jsi_params& P, jsi_result* res,
                 jsi_pair* out, int cap_pairs, CurveReq* creq = nullptr);
