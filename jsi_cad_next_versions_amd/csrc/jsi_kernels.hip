@@ -854,7 +854,12 @@ __device__ __forceinline__ void k_dist_list_rows_body(int nslots, const int* su1
                                                         const vec4_t<T>* lo2, const vec4_t<T>* hi2,
                                                         const T* thresh, int cull, uint64_t* list,
                                                         unsigned* list_count) {
-  __shared__ unsigned wsum[4], sbase;
+  // side-2 tiles in groups of four runs of 256 (a 511 x 511-cell unit has
+  // 1024 tiles: one group): the boxes of a group are loaded together and its
+  // kept tile pairs reserved with one atomic, so a workgroup waits for one
+  // load and one reservation per group, not per run
+  constexpr int kRuns = 4;
+  __shared__ unsigned wsum[kRuns][4], sbase;
   const int tpu1 = tiles_per_unit(w1), tpu2 = tiles_per_unit(w2);
   const int pos = blockIdx.x;  // slot * tpu1 + t1
   if (pos >= nslots * tpu1) return;  // whole workgroup
@@ -863,29 +868,45 @@ __device__ __forceinline__ void k_dist_list_rows_body(int nslots, const int* su1
   const vec4_t<T> l1 = lo1[a], h1 = hi1[a];
   const T th = *thresh;
   const int lane = lane_id(), wv = threadIdx.x >> 6;
-  for (int c0 = 0; c0 < tpu2; c0 += 256) {
-    const int t2 = c0 + (int)threadIdx.x;
-    bool keep = t2 < tpu2;
-    if (keep && cull) {
-      const vec4_t<T> l2 = lo2[ub + t2], h2 = hi2[ub + t2];
-      const T gx = smax(T(0), smax(l1.x - h2.x, l2.x - h1.x));
-      const T gy = smax(T(0), smax(l1.y - h2.y, l2.y - h1.y));
-      const T gz = smax(T(0), smax(l1.z - h2.z, l2.z - h1.z));
-      const T lb2 = FMA ? fma(gx, gx, fma(gy, gy, gz * gz)) : gx * gx + gy * gy + gz * gz;
-      keep = !(lb2 > th);
+  for (int g0 = 0; g0 < tpu2; g0 += kRuns * 256) {
+    bool keep[kRuns];
+    unsigned long long m[kRuns];
+#pragma unroll
+    for (int r = 0; r < kRuns; ++r) {
+      const int t2 = g0 + 256 * r + (int)threadIdx.x;
+      keep[r] = t2 < tpu2;
+      if (keep[r] && cull) {
+        const vec4_t<T> l2 = lo2[ub + t2], h2 = hi2[ub + t2];
+        const T gx = smax(T(0), smax(l1.x - h2.x, l2.x - h1.x));
+        const T gy = smax(T(0), smax(l1.y - h2.y, l2.y - h1.y));
+        const T gz = smax(T(0), smax(l1.z - h2.z, l2.z - h1.z));
+        const T lb2 = FMA ? fma(gx, gx, fma(gy, gy, gz * gz)) : gx * gx + gy * gy + gz * gz;
+        keep[r] = !(lb2 > th);
+      }
     }
-    const unsigned long long m = __ballot(keep);
-    if (lane == 0) wsum[wv] = (unsigned)__popcll(m);
+#pragma unroll
+    for (int r = 0; r < kRuns; ++r) {
+      m[r] = __ballot(keep[r]);
+      if (lane == 0) wsum[r][wv] = (unsigned)__popcll(m[r]);
+    }
     __syncthreads();
-    const unsigned total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    unsigned total = 0;
+#pragma unroll
+    for (int r = 0; r < kRuns; ++r) total += wsum[r][0] + wsum[r][1] + wsum[r][2] + wsum[r][3];
     if (total) {  // uniform
       if (threadIdx.x == 0) sbase = atomicAdd(list_count, total);
       __syncthreads();
-      unsigned off = sbase;
-      for (int v = 0; v < wv; ++v) off += wsum[v];
-      if (keep) list[off + __popcll(m & ((1ull << lane) - 1))] = tp_encode(slot, t1, t2);
+      unsigned off = sbase;  // runs in order, waves in order within a run
+#pragma unroll
+      for (int r = 0; r < kRuns; ++r) {
+        unsigned o = off;
+        for (int v = 0; v < wv; ++v) o += wsum[r][v];
+        if (keep[r])
+          list[o + __popcll(m[r] & ((1ull << lane) - 1))] = tp_encode(slot, t1, g0 + 256 * r + (int)threadIdx.x);
+        off += wsum[r][0] + wsum[r][1] + wsum[r][2] + wsum[r][3];
+      }
     }
-    __syncthreads();  // wsum / sbase are rewritten by the next chunk
+    __syncthreads();  // wsum / sbase are rewritten by the next group
   }
 }
 template <class T, bool FMA>
