@@ -639,15 +639,18 @@ __global__ __launch_bounds__(1024) void k_seed_list(const unsigned long long* be
 
 constexpr int kSeedSplitMax = 16;    // workgroups per seed tile pair at most
 constexpr int kSeedHistBins = 1024;  // bins of the seed histogram
+constexpr int kSeedSample = 4;       // the range pass reads every 4th side-2 cell
 
 // Two passes over the d2 of the seed tile pairs (no candidate records are
 // written).  Each workgroup handles one seed tile pair against a 1/split
 // slice of its side-2 cells; seed tile pair `it` is list[it], or with
-// ordered (one tile per unit, slots in parent rank order) slot `it`.  MINMAX: per-workgroup smallest and largest key
-// -> mm[2*block].  Otherwise: a histogram in LDS, bins linear in the key bits
-// over [min, max] (its resolution follows the spread of the values: refined
-// cells have d2 within 1e-5 of each other), whose non-zero bins are added to
-// the global histogram gh.
+// ordered (one tile per unit, slots in parent rank order) slot `it`.  MINMAX:
+// per-workgroup smallest and largest key of a sample (every kSeedSample-th
+// side-2 cell) -> mm[2*block].  Otherwise: a histogram in LDS, bins linear in
+// the key bits over the sampled [min, max] (its resolution follows the spread
+// of the values: refined cells have d2 within 1e-5 of each other), keys
+// outside it counted in the end bins, whose non-zero bins are added to the
+// global histogram gh.
 template <class T, bool FMA, bool MINMAX>
 __device__ __forceinline__ void k_seed_hist_body(const uint64_t* list, int n, int ordered, const int* su1,
                                                    const int* su2, int w1, int w2, const vec4_t<T>* C1,
@@ -682,7 +685,7 @@ __device__ __forceinline__ void k_seed_hist_body(const uint64_t* list, int n, in
     if ((int)threadIdx.x < nA) {
       const int i = (g1.row0 + threadIdx.x / g1.wd) * w1 + g1.col0 + threadIdx.x % g1.wd;
       const vec4_t<T> a = C1[(size_t)u1 * w1 * w1 + i];
-      for (int jj = 0; jj < j1 - j0; ++jj) {
+      for (int jj = 0; jj < j1 - j0; jj += MINMAX ? kSeedSample : 1) {
         const vec4_t<T> b = sB[jj];
         const unsigned long long key = ord_bits(pair_d2<T, FMA>(a.x, a.y, a.z, b.x, b.y, b.z));
         if (MINMAX) {
@@ -785,15 +788,17 @@ __device__ __forceinline__ void k_seed_thresh_body(const unsigned* gh, const uns
   if (before < (unsigned long long)k && cum[b] >= (unsigned long long)k) found = b;
   __syncthreads();
   if (b == 0) {
-    if (found < 0) {
-      *thresh = infinity_v<T>();
-    } else {
+    // Bin `found` < last holds keys up to its upper edge only, so the k
+    // smallest keys are at most that edge.  The last bin also counts every
+    // key above the sampled range: then there is no threshold.
+    const unsigned long long inf_bits = ord_bits(infinity_v<T>());
+    unsigned long long edge = inf_bits;
+    if (found >= 0 && found < kSeedHistBins - 1) {
       const unsigned long long lo = range[0], hi = range[1];
       const unsigned long long width = (hi > lo ? hi - lo : 0ull) / kSeedHistBins + 1;
-      unsigned long long edge = lo + (unsigned long long)(found + 1) * width - 1;
-      if (edge > hi) edge = hi;  // hi is the largest seed key, so rank k lies at or below it
-      *thresh = from_bits<T>(edge);
+      edge = lo + (unsigned long long)(found + 1) * width - 1;
     }
+    *thresh = edge >= inf_bits ? infinity_v<T>() : from_bits<T>(edge);
   }
 }
 template <class T>
