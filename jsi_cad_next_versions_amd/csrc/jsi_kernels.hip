@@ -641,6 +641,15 @@ constexpr int kSeedSplitMax = 16;    // workgroups per seed tile pair at most
 constexpr int kSeedHistBins = 1024;  // bins of the seed histogram
 constexpr int kSeedSample = 4;       // the range pass reads every 4th side-2 cell
 
+// Bin width of the seed histogram over the key range [lo, hi]: 2^shift, the
+// least power of two that puts hi in bin kSeedHistBins - 1 or below (a shift
+// instead of a 64-bit division per key; 512-1024 bins span the range).
+__device__ __forceinline__ int seed_bin_shift(unsigned long long lo, unsigned long long hi) {
+  const unsigned long long r = (hi > lo ? hi - lo : 0ull) >> 10;
+  static_assert(kSeedHistBins == 1024, "the shift assumes 1024 bins");
+  return r ? 64 - __clzll(r) : 0;
+}
+
 // Two passes over the d2 of the seed tile pairs (no candidate records are
 // written).  Each workgroup handles one seed tile pair against a 1/split
 // slice of its side-2 cells; seed tile pair `it` is list[it], or with
@@ -659,11 +668,11 @@ __device__ __forceinline__ void k_seed_hist_body(const uint64_t* list, int n, in
   __shared__ vec4_t<T> sB[kTileCells];
   __shared__ unsigned h[MINMAX ? 1 : kSeedHistBins];
   __shared__ unsigned long long rmin[4], rmax[4];
-  unsigned long long lo = 0, width = 1;
+  unsigned long long lo = 0;
+  int shift = 0;
   if (!MINMAX) {
     lo = range[0];
-    const unsigned long long hi = range[1];
-    width = (hi > lo ? hi - lo : 0ull) / kSeedHistBins + 1;
+    shift = seed_bin_shift(lo, range[1]);
     for (int b = threadIdx.x; b < kSeedHistBins; b += blockDim.x) h[b] = 0;
   }
   unsigned long long kmin = ~0ull, kmax = 0ull;
@@ -692,7 +701,7 @@ __device__ __forceinline__ void k_seed_hist_body(const uint64_t* list, int n, in
           kmin = key < kmin ? key : kmin;
           kmax = key > kmax ? key : kmax;
         } else {
-          unsigned long long bin = key <= lo ? 0ull : (key - lo) / width;
+          unsigned long long bin = key <= lo ? 0ull : (key - lo) >> shift;
           if (bin >= (unsigned long long)kSeedHistBins) bin = kSeedHistBins - 1;
           atomicAdd(&h[bin], 1u);
         }
@@ -794,9 +803,9 @@ __device__ __forceinline__ void k_seed_thresh_body(const unsigned* gh, const uns
     const unsigned long long inf_bits = ord_bits(infinity_v<T>());
     unsigned long long edge = inf_bits;
     if (found >= 0 && found < kSeedHistBins - 1) {
-      const unsigned long long lo = range[0], hi = range[1];
-      const unsigned long long width = (hi > lo ? hi - lo : 0ull) / kSeedHistBins + 1;
-      edge = lo + (unsigned long long)(found + 1) * width - 1;
+      const unsigned long long lo = range[0];
+      const int shift = seed_bin_shift(lo, range[1]);
+      edge = lo + ((unsigned long long)(found + 1) << shift) - 1;
     }
     *thresh = edge >= inf_bits ? infinity_v<T>() : from_bits<T>(edge);
   }
