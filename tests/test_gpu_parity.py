@@ -87,6 +87,22 @@ def test_min_distance_hardwave_matches_oracle(ctx, orc, prec, k):
     np.testing.assert_array_equal(pairs, opairs)
 
 
+@pytest.mark.parametrize("prec", ["exact32", "fp64", "fp32screen"])
+@pytest.mark.parametrize("k", [4032, 4095, 4096])
+def test_min_distance_k_near_all_pairs(ctx, orc, prec, k):
+    """nuv=9: 64 x 64 = 4096 cell pairs in round 1, so rank k lies at or near
+    the largest seed key: the seed histogram's last bin (keys above the
+    sampled range) must give no threshold rather than a cut below rank k."""
+    a, b = S.hard_wave_pair()
+    r1, r2 = regions(a, b, 9)
+    p = default_params(k=k, precision=prec, trace=1)
+    res, pairs = ctx.minimum_distance(a, r1, b, r2, p)
+    ores, opairs = orc.min_distance(a, r1, b, r2, p)
+    same_rounds(res, ores)
+    assert res.distance == ores.distance
+    np.testing.assert_array_equal(pairs, opairs)
+
+
 @pytest.mark.parametrize("prec", ["fp64", "fp32screen"])
 @pytest.mark.parametrize("nref", [3, 4, 5, 9, 13, 16, 17])
 def test_min_distance_refine_grids_match_oracle(ctx, orc, prec, nref):
