@@ -4130,8 +4130,12 @@ static void topk_brec(BatchProgram& p, const DistBatchShape& sh, const std::vect
       brec<1024, GT::value>(p, 1, 1024, b);
     }
     const int m = nseed < npos ? nseed : (int)npos;
-    int split = m > 0 ? 1024 / m : 1;
-    split = split < 1 ? 1 : (split > kSeedSplitMax ? kSeedSplitMax : split);
+    // the queries of a batch fill the chip together: about 4096 workgroups
+    // over the batch, so fewer partial histograms are cleared and added to
+    // the global ones than with the single-query split
+    const long long mq = (long long)m * (long long)tq.size();
+    int split = mq > 0 ? (int)std::min<long long>(kSeedSplitMax, 4096 / mq) : 1;
+    split = split < 1 ? 1 : split;
     const int nb = m > 0 ? m * split : 1;
     auto gh = [](const TopkQ<S>& q) {
       return reinterpret_cast<unsigned*>(q.work + 2 * (size_t)kSeedMaxTiles * kSeedSplitMax);
